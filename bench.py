@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""PairHMM benchmark on one MI355X.
+
+Workload (default): 4096 synthetic active regions with uneven shapes --
+haplotypes U[2,64], reads log-uniform[10,500], haplotype length U[100,600],
+read length one of {100,150,250} per region -- packed once into HBM and run
+in one launch sequence per step (FP32 pass over length-binned tiles, then the
+FP64 rescue). A step is one pass over all regions; inputs are resident in
+HBM before the timed region starts.
+
+Prints one JSON line: GCUPS (cell updates per second / 1e9), the FP32-pass
+roofline against MI355X's 157.3 TFLOP/s FP32 peak at 12 flops per cell, the
+plain one-region-per-launch GPU form on a sample, accuracy against the FP64
+oracle and the reference routing on a sample, and the CPU baseline (Rovaca's
+region scheme on the host cores) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOPS_PER_CELL = 12          # M: 4 mul + 2 add, I: 2 mul + 1 add, D: 2 mul + 1 add
+FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 vector peak
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(name, n_regions):
+    from rovaca_amd import workload
+    if name == "batched":
+        return workload.batched_regions(n_regions, seed=1), f"batched_uneven_{n_regions}_regions"
+    if name == "stress":
+        return workload.stress_regions(n_regions, seed=2), f"precision_stress_{n_regions}_regions"
+    if name == "single":
+        return [workload.single_region(0)], "single_region_8x32"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def accuracy(regions, batch, out, sample):
+    """max/mean |delta log10 L| vs the FP64 oracle and the reference routing, on a sample."""
+    import oracle
+    d64, dref = [], []
+    for x in sample:
+        reg = regions[x]
+        got = batch.region_result(out, x)
+        want, route, ok = oracle.region_reference(reg)
+        m = ~np.isnan(want)
+        dref.append(np.abs(got[m] - want[m]))
+        f64 = np.full_like(want, np.nan)
+        H = reg["n_haps"]
+        for r in range(reg["n_reads"]):
+            o0, o1 = reg["read_off"][r], reg["read_off"][r + 1]
+            if o1 == o0:
+                continue
+            args = [reg[k][o0:o1] for k in "qidc"]
+            for h in range(H):
+                hb = reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]]
+                if len(hb):
+                    f64[r, h] = oracle.pair_log10_f64(hb, reg["read_bases"][o0:o1], *args)
+        m = ~np.isnan(f64)
+        d64.append(np.abs(got[m] - f64[m]))
+    d64 = np.concatenate(d64)
+    dref = np.concatenate(dref)
+    return {"sample_regions": len(sample), "sample_pairs": int(len(d64)),
+            "max_abs_vs_fp64_oracle": float(d64.max()), "mean_abs_vs_fp64_oracle": float(d64.mean()),
+            "max_abs_vs_reference_routing": float(dref.max()),
+            "mean_abs_vs_reference_routing": float(dref.mean())}
+
+
+def cpu_baseline(regions, target_seconds):
+    """Rovaca's region scheme (three passes, 16-lane inter kernel, Rovaca's own
+    intra kernel for pass 3) on the host cores, over a bounded sample."""
+    import oracle
+    from rovaca_amd import workload
+    L = oracle.baseline()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    # calibrate on one core with a few regions, then size the sample for ~target_seconds
+    cal = regions[:4]
+    secs1, _ = oracle.baseline_run(cal, 1)
+    rate1 = workload.cells(cal) / max(secs1, 1e-6)
+    want_cells = rate1 * threads * target_seconds
+    sample, tot = [], 0
+    for r in regions[4:]:
+        sample.append(r)
+        tot += workload.cells([r])
+        if tot >= want_cells:
+            break
+    secs, _ = oracle.baseline_run(sample, threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": tot / secs / 1e9, "unit": "GCUPS", "cores": threads,
+            "kind": "port",
+            "sample": f"{len(sample)} regions of the same workload ({tot / 1e9:.1f} Gcells), {secs:.1f} s wall",
+            "isa": "avx512" if L.orc_baseline_isa() == 512 else "avx2",
+            "pass3_intra": "rovaca_compiled" if L.uses_reference_intra else "restatement",
+            "single_core_gcups": rate1 / 1e9, "cpu_model": model}
+
+
+def plain_gpu(regions, n):
+    """One region per launch (rvc_phmm_region_sync), sequentially, on a sample."""
+    import rovaca_amd as rv
+    from rovaca_amd import workload
+    sample = regions[:n]
+    rv.region_sync(sample[0])
+    t0 = time.perf_counter()
+    for r in sample:
+        rv.region_sync(r)
+    secs = time.perf_counter() - t0
+    return {"value": workload.cells(sample) / secs / 1e9, "unit": "GCUPS", "regions": len(sample),
+            "mean_region_ms": secs / len(sample) * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="batched", choices=["batched", "stress", "single"])
+    ap.add_argument("--regions", type=int, default=4096)
+    ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip plain-GPU and accuracy samples")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
+              "(regions are independent and would be scattered to per-GPU queues, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import rovaca_amd as rv
+    from rovaca_amd import workload
+
+    t = time.time()
+    regions, wname = make_workload(args.workload, args.regions)
+    cells = workload.cells(regions)
+    log(f"workload {wname}: {len(regions)} regions, {cells / 1e9:.1f} Gcells, generated in {time.time() - t:.1f} s")
+    rv.init(0)
+    t = time.time()
+    batch = rv.Batch(regions, use_graph=bool(args.graph))
+    info0 = batch.info()
+    log(f"packed + uploaded in {time.time() - t:.1f} s, {info0['device_bytes'] / 2**30:.2f} GiB on device, "
+        f"{info0['n_pairs']} pairs")
+    for w in range(args.warmup):
+        batch.run()
+        log(f"warmup {w} done")
+    # timed region: each run() returns after the device finished (stream synchronised)
+    dev_total, dev_fp32 = [], []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        batch.run()
+        inf = batch.info()
+        dev_total.append(inf["total_ms"])
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    ms_per_step = wall / args.steps * 1e3
+    value = cells * args.steps / wall / 1e9
+    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {value:.1f} GCUPS")
+    # FP32 / FP64 split from an eager (non-graph) run with HIP events on the batch stream
+    ebatch = batch
+    if args.graph:
+        batch.close()
+        ebatch = rv.Batch(regions, use_graph=False)
+    ebatch.run()
+    split = []
+    for s in range(max(1, min(3, args.steps))):
+        ebatch.run()
+        split.append(ebatch.info())
+    fp32_ms = float(np.mean([x["fp32_ms"] for x in split]))
+    fp64_ms = float(np.mean([x["fp64_ms"] for x in split]))
+    n_rescued = int(split[-1]["n_rescued"])
+    out, flags = ebatch.results()
+    rescued_cells = 0
+    # cells of the pairs that went to FP64
+    for x, reg in enumerate(regions):
+        fl = ebatch.region_result(flags, x)
+        if (fl & 2).any():
+            hl = np.diff(reg["hap_off"])
+            rl = np.diff(reg["read_off"])
+            rr, hh = np.nonzero(fl & 2)
+            rescued_cells += int((rl[rr].astype(np.int64) * hl[hh]).sum())
+    fp32_tflops = cells * FLOPS_PER_CELL / (fp32_ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": round(fp32_tflops, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(fp32_tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "kernel": "k_fp32 (FP32 forward pass, all length bins)", "kernel_ms": round(fp32_ms, 3),
+            "flops_per_cell": FLOPS_PER_CELL, "cells_per_launch_sequence": cells}
+    result = {
+        "metric": "PairHMM GCUPS (batched uneven regions, 1 MI355X)",
+        "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 (fp64 rescue)", "data": "synthetic",
+        "config": {"workload": wname, "regions": len(regions), "pairs": int(info0["n_pairs"]),
+                   "cells": int(cells), "graph": bool(args.graph), "seed": 1},
+        "roofline": roof,
+        "device_ms": {"fp32_pass": round(fp32_ms, 3), "fp64_rescue": round(fp64_ms, 3),
+                      "launch_sequence_mean": round(float(np.mean(dev_total)), 3)},
+        "rescue": {"pairs": n_rescued, "fraction": n_rescued / max(1, int(info0["n_pairs"])),
+                   "cells": rescued_cells},
+    }
+    if not args.no_extras:
+        log("plain one-region-per-launch GPU form")
+        result["plain_gpu"] = plain_gpu(regions, min(256, len(regions)))
+        result["plain_gpu"]["batched_over_plain"] = round(value / result["plain_gpu"]["value"], 2)
+        log("accuracy sample")
+        small = [x for x, r in enumerate(regions) if r["n_haps"] * r["n_reads"] <= 1200][:12]
+        if small:
+            result["accuracy"] = accuracy(regions, ebatch, out, small)
+    rv.shutdown()
+    if not args.no_cpu:
+        log("CPU baseline")
+        try:
+            result["cpu_baseline"] = cpu_baseline(regions, args.cpu_seconds)
+            result["cpu_baseline"]["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
+        except Exception as e:  # noqa: BLE001
+            result["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,236 @@
+"""PairHMM oracle (test infrastructure only).
+
+Loads ``liboracle.so`` (our CPU restatement of Rovaca's PairHMM, built from
+``phmm_oracle.cpp`` by this folder's Makefile) and, when it has been built,
+``_ref/libref_intra_*.so`` (Rovaca's own intra-pair kernel compiled from the
+reference sources). Only tests/, the smoke check and the benchmark's CPU
+baseline import this package, always as the checker; the product never does.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
+        pair_args = [_u8p, ctypes.c_int, _u8p, _u8p, _u8p, _u8p, _u8p, ctypes.c_int, ctypes.c_int]
+        for name, res in [("orc_pair_raw_f32", ctypes.c_float), ("orc_pair_raw_f64", ctypes.c_double),
+                          ("orc_intra_log10", ctypes.c_double), ("orc_pair_log10_f64", ctypes.c_double),
+                          ("orc_pair_log10_ld", ctypes.c_double), ("orc_bruteforce_log10", ctypes.c_double)]:
+            f = getattr(L, name)
+            f.argtypes = pair_args
+            f.restype = res
+        L.orc_tables_f32.argtypes = [_f32p, _f32p]
+        L.orc_tables_f64.argtypes = [_f64p, _f64p]
+        L.orc_mm_f32.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_mm_f32.restype = ctypes.c_float
+        L.orc_mm_f64.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_mm_f64.restype = ctypes.c_double
+        L.orc_region_reference.argtypes = [ctypes.c_uint32, _i32p, _u8p, ctypes.c_uint32, _i32p, _u8p, _u8p, _u8p,
+                                           _u8p, _u8p, ctypes.c_uint32, ctypes.c_uint32, _f64p, _u8p]
+        L.orc_region_reference.restype = ctypes.c_int
+        L.orc_preprocess_read.argtypes = [_u8p, _u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          _u8p, _u8p, _u8p, _u8p]
+        L.orc_postprocess.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, _i32p, _f64p, _i32p]
+        L.orc_postprocess.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def ref():
+    """Rovaca's own intra kernel (AVX-512 if this CPU has it, else AVX2), or None."""
+    global _ref
+    if _ref is None:
+        flags = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                flags = f.read()
+        except OSError:
+            pass
+        want = ["avx512f", "avx512dq", "avx512bw", "avx512vl"]
+        name = "libref_intra_avx512.so" if all(w in flags for w in want) else "libref_intra_avx2.so"
+        path = os.path.join(_HERE, "_ref", name)
+        if not os.path.exists(path):
+            return None
+        L = ctypes.CDLL(path)
+        L.ref_intra_log10.argtypes = [_u8p, ctypes.c_int, _u8p, _u8p, _u8p, _u8p, _u8p, ctypes.c_int, ctypes.c_int]
+        L.ref_intra_log10.restype = ctypes.c_double
+        L.ref_intra_isa.restype = ctypes.c_int
+        _ref = L
+    return _ref
+
+
+def _u8(a):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    return a, a.ctypes.data_as(_u8p)
+
+
+def _pair_call(fn, hap, read, q, i, d, c, extra):
+    hap, hp = _u8(np.frombuffer(hap, np.uint8) if isinstance(hap, (bytes, bytearray)) else hap)
+    read, rp = _u8(np.frombuffer(read, np.uint8) if isinstance(read, (bytes, bytearray)) else read)
+    arrs = [_u8(x) for x in (q, i, d, c)]
+    return fn(hp, len(hap), rp, arrs[0][1], arrs[1][1], arrs[2][1], arrs[3][1], len(read), extra)
+
+
+def pair_raw_f32(hap, read, q, i, d, c, mode=1):
+    return _pair_call(lib().orc_pair_raw_f32, hap, read, q, i, d, c, mode)
+
+
+def pair_raw_f64(hap, read, q, i, d, c, mode=1):
+    return _pair_call(lib().orc_pair_raw_f64, hap, read, q, i, d, c, mode)
+
+
+def intra_log10(hap, read, q, i, d, c, use_double=False):
+    return _pair_call(lib().orc_intra_log10, hap, read, q, i, d, c, int(use_double))
+
+
+def pair_log10_f64(hap, read, q, i, d, c, wildcard_n=True):
+    return _pair_call(lib().orc_pair_log10_f64, hap, read, q, i, d, c, int(wildcard_n))
+
+
+def pair_log10_ld(hap, read, q, i, d, c, wildcard_n=True):
+    return _pair_call(lib().orc_pair_log10_ld, hap, read, q, i, d, c, int(wildcard_n))
+
+
+def bruteforce_log10(hap, read, q, i, d, c, wildcard_n=True):
+    return _pair_call(lib().orc_bruteforce_log10, hap, read, q, i, d, c, int(wildcard_n))
+
+
+def ref_intra_log10(hap, read, q, i, d, c, use_double=False):
+    R = ref()
+    if R is None:
+        raise RuntimeError("oracle/_ref is not built")
+    return _pair_call(R.ref_intra_log10, hap, read, q, i, d, c, int(use_double))
+
+
+def tables_f32():
+    p = np.zeros(128, np.float32)
+    m = np.zeros(128 * 129 // 2, np.float32)
+    lib().orc_tables_f32(p.ctypes.data_as(_f32p), m.ctypes.data_as(_f32p))
+    return p, m
+
+
+def tables_f64():
+    p = np.zeros(128, np.float64)
+    m = np.zeros(128 * 129 // 2, np.float64)
+    lib().orc_tables_f64(p.ctypes.data_as(_f64p), m.ctypes.data_as(_f64p))
+    return p, m
+
+
+def region_reference(region, float_width=16, double_width=8):
+    """Reference routing for one region (dict with flat arrays, see rovaca_amd.regions).
+
+    Returns (read-major [n_reads, n_haps] float64, route codes uint8, ok flag)."""
+    L = lib()
+    nh, nr = region["n_haps"], region["n_reads"]
+    out = np.empty(nh * nr, np.float64)
+    route = np.zeros(nh * nr, np.uint8)
+    keep = [np.ascontiguousarray(region[k]) for k in
+            ("hap_off", "hap_bases", "read_off", "read_bases", "q", "i", "d", "c")]
+    ok = L.orc_region_reference(nh, keep[0].ctypes.data_as(_i32p), keep[1].ctypes.data_as(_u8p), nr,
+                                keep[2].ctypes.data_as(_i32p), keep[3].ctypes.data_as(_u8p),
+                                keep[4].ctypes.data_as(_u8p), keep[5].ctypes.data_as(_u8p),
+                                keep[6].ctypes.data_as(_u8p), keep[7].ctypes.data_as(_u8p),
+                                float_width, double_width, out.ctypes.data_as(_f64p), route.ctypes.data_as(_u8p))
+    return out.reshape(nr, nh), route.reshape(nr, nh), bool(ok)
+
+
+def preprocess_read(bases, qual, mq, min_q, pcr):
+    n = len(bases)
+    b = np.ascontiguousarray(np.frombuffer(bytes(bases), np.uint8) if isinstance(bases, (bytes, bytearray)) else bases,
+                             dtype=np.uint8)
+    ql = np.ascontiguousarray(qual, dtype=np.uint8)
+    outs = [np.zeros(n, np.uint8) for _ in range(4)]
+    lib().orc_preprocess_read(b.ctypes.data_as(_u8p), ql.ctypes.data_as(_u8p), n, mq, min_q, pcr,
+                              *[o.ctypes.data_as(_u8p) for o in outs])
+    return outs
+
+
+def postprocess(read_major, read_len):
+    rm = np.ascontiguousarray(read_major, dtype=np.float64).copy()
+    n, h = rm.shape
+    rl = np.ascontiguousarray(read_len, dtype=np.int32)
+    hap_major = np.empty((h, n), np.float64)
+    kept = np.zeros(max(n, 1), np.int32)
+    nk = lib().orc_postprocess(n, h, rm.ctypes.data_as(_f64p), rl.ctypes.data_as(_i32p),
+                               hap_major.ctypes.data_as(_f64p), kept.ctypes.data_as(_i32p))
+    return hap_major, kept[:nk].copy()
+
+
+class Region(ctypes.Structure):
+    """Same layout as rvc_region (include/rovaca_phmm.h)."""
+    _fields_ = [("n_haps", ctypes.c_uint32), ("n_reads", ctypes.c_uint32), ("hap_off", _i32p),
+                ("hap_bases", _u8p), ("read_off", _i32p), ("read_bases", _u8p), ("q", _u8p), ("i", _u8p),
+                ("d", _u8p), ("c", _u8p)]
+
+
+_base = None
+
+
+def _cpu_flags():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("flags"):
+                    return set(ln.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+def baseline():
+    """The CPU baseline library for this host's ISA (AVX-512 if present, else AVX2)."""
+    global _base
+    if _base is None:
+        avx512 = {"avx512f", "avx512dq", "avx512bw", "avx512vl"} <= _cpu_flags()
+        path = os.path.join(_HERE, "libcpu_baseline_avx512.so" if avx512 else "libcpu_baseline_avx2.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
+        L.orc_baseline_run.argtypes = [ctypes.POINTER(Region), ctypes.c_uint32, ctypes.c_int, _f64p]
+        L.orc_baseline_run.restype = ctypes.c_double
+        L.orc_baseline_set_intra.argtypes = [ctypes.c_char_p]
+        L.orc_baseline_isa.restype = ctypes.c_int
+        ref_name = "libref_intra_avx512.so" if avx512 else "libref_intra_avx2.so"
+        ref_path = os.path.join(_HERE, "_ref", ref_name)
+        L.uses_reference_intra = os.path.exists(ref_path) and L.orc_baseline_set_intra(ref_path.encode()) == 0
+        _base = L
+    return _base
+
+
+def baseline_run(regions, n_threads, want_out=False):
+    """Times Rovaca's region scheme on n_threads host cores; returns (seconds, outputs or None)."""
+    L = baseline()
+    arr = (Region * len(regions))()
+    keep = []
+    for x, r in enumerate(regions):
+        cr = arr[x]
+        cr.n_haps, cr.n_reads = int(r["n_haps"]), int(r["n_reads"])
+        for k, t in (("hap_off", _i32p), ("read_off", _i32p)):
+            a = np.ascontiguousarray(r[k], np.int32)
+            keep.append(a)
+            setattr(cr, k, a.ctypes.data_as(t))
+        for k in ("hap_bases", "read_bases", "q", "i", "d", "c"):
+            a = np.ascontiguousarray(r[k], np.uint8)
+            keep.append(a)
+            setattr(cr, k, a.ctypes.data_as(_u8p))
+    out = None
+    if want_out:
+        out = np.empty(sum(int(r["n_haps"]) * int(r["n_reads"]) for r in regions), np.float64)
+    secs = L.orc_baseline_run(arr, len(regions), n_threads, out.ctypes.data_as(_f64p) if out is not None else None)
+    return secs, out

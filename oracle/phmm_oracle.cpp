@@ -1,0 +1,601 @@
+// PairHMM oracle: a CPU restatement of Rovaca's PairHMM numerical scheme and
+// routing. TEST INFRASTRUCTURE ONLY: only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg load this library, and only as the checker.
+// The product (rovaca_amd/) never links, loads or calls anything here.
+//
+// What each exported function restates (paths relative to the Rovaca tree,
+// ph/ = src/haplotypecaller/pairhmm/):
+//   orc_tables_*            ph/roDV_common/context.h:44-60 (jacobian and
+//                           match-to-match tables), :98 / :138 (ph2pr),
+//                           :115-126 / :155-166 (set_mm_prob)
+//   orc_pair_raw_f32/_f64   the forward recurrence of
+//                           ph/roDV_inter/pairhmm_inter.cpp:141-246 (inter,
+//                           prior*1/3, exact-byte compare) and
+//                           ph/roDV_intra/pairhmm_impl.cpp:235-427 (intra,
+//                           prior/3, N wildcard); boundaries from
+//                           pairhmm_inter.cpp:256-271 and
+//                           pairhmm_impl.cpp:50-70,122-174
+//   orc_intra_log10         ph/roDV_intra/pairhmm_api.cpp:14-23 (FP32 first,
+//                           FP64 below 1e-28, float log10 on the FP32 path)
+//   orc_pair_log10_ld       the same model in long double without the scaling
+//                           constant (independent check of scaling/underflow)
+//   orc_bruteforce_log10    explicit enumeration of every HMM state path
+//                           (independent of the dynamic programme)
+//   orc_region_reference    ph/roDV_schedule.cpp:364-834 (pair list, sort,
+//                           greedy 16/8-lane grouping, three passes)
+//   orc_preprocess_read     ph/roDV_schedule.cpp:494-649 plus
+//                           hc/genotype/read_record.cpp:184-210 (MQ cap,
+//                           min-quality floor, GOP 45 / GCP 10 defaults) and
+//                           the PCR indel model ph/pairhmm_internal.cpp:260-314,
+//                           :391-429, tables ph/adapter.h:13-15
+//   orc_postprocess         ph/pairhmm_internal.cpp:22-44 (poorly modelled
+//                           reads), :248-258 (normalize), :489-508 (transpose),
+//                           :576-623 (wrapper order), constants ph/adapter.h:20-23
+//
+// Compiled with -ffp-contract=off so every product and sum rounds exactly as
+// written; the reference's GCC build may fuse some of them (FMA), which moves
+// results by a few float ulps only.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Tables (ph/roDV_common/context.h)
+// ---------------------------------------------------------------------------
+constexpr int kMaxQual = 254;
+constexpr double kJacTol = 8.0;
+constexpr double kJacStep = 0.0001;
+constexpr int kJacSize = (int)(kJacTol / kJacStep) + 1;  // 80001
+constexpr int kMMSize = ((kMaxQual + 1) * (kMaxQual + 2)) >> 1;
+
+template <class T>
+struct Tables {
+  T jac[kJacSize];
+  T mm[kMMSize];
+  T ph2pr[128];
+  T initial;
+  bool ready = false;
+
+  static int fast_round(T d) {
+    return (d > (T)0.0) ? (int)(d + (T)0.5) : (int)(d - (T)0.5);
+  }
+  // approximateLog10SumLog10 evaluated in precision T (context.h:64-82)
+  T approx_log10_sum(T small, T big) const {
+    if (small > big) std::swap(small, big);
+    if (std::isinf(small) || std::isinf(big)) return big;
+    T diff = big - small;
+    if (diff >= (T)kJacTol) return big;
+    int ind = fast_round((T)(diff * (T)(1.0 / kJacStep)));
+    return big + jac[ind];
+  }
+  T mm_prob(int ins, int del) const {
+    int mn = del, mx = ins;
+    if (ins <= del) { mn = ins; mx = del; }
+    return mm[((mx * (mx + 1)) >> 1) + mn];
+  }
+};
+
+Tables<float> g_tf;
+Tables<double> g_td;
+
+void ensure_tables() {
+  if (!g_tf.ready) {
+    // float context: context.h:132-146
+    for (int k = 0; k < kJacSize; k++)
+      g_tf.jac[k] = (float)(std::log10(1.0 + std::pow(10.0, -((double)k) * kJacStep)));
+    const double inv_ln10 = 1.0 / std::log(10.0);
+    for (int i = 0, off = 0; i <= kMaxQual; off += ++i)
+      for (int j = 0; j <= i; j++) {
+        double s = (double)g_tf.approx_log10_sum((float)(-0.1 * i), (float)(-0.1 * j));
+        double l = std::log1p(-std::min(1.0, std::pow(10.0, s))) * inv_ln10;
+        g_tf.mm[off + j] = (float)std::pow(10.0, l);
+      }
+    for (int x = 0; x < 128; x++) g_tf.ph2pr[x] = powf(10.f, -((float)x) / 10.f);
+    g_tf.initial = ldexpf(1.f, 120);
+    g_tf.ready = true;
+  }
+  if (!g_td.ready) {
+    // double context: context.h:92-106
+    for (int k = 0; k < kJacSize; k++)
+      g_td.jac[k] = std::log10(1.0 + std::pow(10.0, -((double)k) * kJacStep));
+    const double inv_ln10 = 1.0 / std::log(10.0);
+    for (int i = 0, off = 0; i <= kMaxQual; off += ++i)
+      for (int j = 0; j <= i; j++) {
+        double s = g_td.approx_log10_sum(-0.1 * i, -0.1 * j);
+        double l = std::log1p(-std::min(1.0, std::pow(10.0, s))) * inv_ln10;
+        g_td.mm[off + j] = std::pow(10.0, l);
+      }
+    for (int x = 0; x < 128; x++) g_td.ph2pr[x] = std::pow(10.0, -((double)x) / 10.0);
+    g_td.initial = ldexp(1.0, 1020);
+    g_td.ready = true;
+  }
+}
+
+template <class T>
+const Tables<T>& tables();
+template <>
+const Tables<float>& tables<float>() { ensure_tables(); return g_tf; }
+template <>
+const Tables<double>& tables<double>() { ensure_tables(); return g_td; }
+
+// ConvertChar (ph/roDV_common/common.h:42-56): A=0 C=1 T=2 G=3 N=4; the
+// other letters in B..T read the zero-initialised table (class 0).
+inline int intra_class(uint8_t ch) {
+  switch (ch) {
+    case 'A': return 0;
+    case 'C': return 1;
+    case 'T': return 2;
+    case 'G': return 3;
+    case 'N': return 4;
+    default: return 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward recurrence, row-major, one pair (precision T).
+// mode 0 = inter: prior mismatch = ph2pr[q] * (1/3), match iff bytes equal.
+// mode 1 = intra: prior mismatch = ph2pr[q] / 3, N (either side) matches all.
+// Returns S = sum_j M[R-1][j] + sum_j I[R-1][j] with row -1 D = C/H.
+// ---------------------------------------------------------------------------
+template <class T>
+T pair_raw(const uint8_t* hap, int H, const uint8_t* rd, const uint8_t* q, const uint8_t* ins,
+           const uint8_t* del, const uint8_t* gcp, int R, int mode) {
+  const Tables<T>& tb = tables<T>();
+  std::vector<T> pm(H + 1), pi(H + 1), pd(H + 1);  // previous row, column j+1 at index j+1
+  const T init_d = tb.initial / (T)H;
+  // row -1: M = I = 0, D = C/H for every column including the corner (index 0)
+  for (int j = 0; j <= H; j++) { pm[j] = 0; pi[j] = 0; pd[j] = init_d; }
+  std::vector<T> cm(H + 1), ci(H + 1), cd(H + 1);
+  const T one_third = (T)1.0 / (T)3.0;
+  for (int k = 0; k < R; k++) {
+    const int iq = q[k] & 127, ii = ins[k] & 127, id = del[k] & 127, ic = gcp[k] & 127;
+    const T dist = tb.ph2pr[iq];
+    const T distm = mode == 0 ? dist * one_third : dist / (T)3.0;
+    const T one_m = (T)1.0 - dist;
+    const T gapm = (T)1.0 - tb.ph2pr[ic];
+    const T tmm = tb.mm_prob(ii, id);
+    const T tmi = tb.ph2pr[ii], tii = tb.ph2pr[ic];
+    const T tmd = tb.ph2pr[id], tdd = tb.ph2pr[ic];
+    const int rc = intra_class(rd[k]);
+    cm[0] = 0; ci[0] = 0; cd[0] = 0;  // column -1 is zero for rows >= 0
+    for (int j = 0; j < H; j++) {
+      bool match;
+      if (mode == 0) {
+        match = rd[k] == hap[j];
+      } else {
+        const int hc = intra_class(hap[j]);
+        match = (rc == 4) || (hc == 4) || (rc == hc);
+      }
+      const T prior = match ? one_m : distm;
+      const T t1 = pm[j] * tmm;
+      const T t2 = pi[j] * gapm;
+      const T t3 = pd[j] * gapm;
+      cm[j + 1] = ((t1 + t2) + t3) * prior;
+      ci[j + 1] = pm[j + 1] * tmi + pi[j + 1] * tii;
+      cd[j + 1] = cm[j] * tmd + cd[j] * tdd;
+    }
+    std::swap(pm, cm);
+    std::swap(pi, ci);
+    std::swap(pd, cd);
+  }
+  T sm = 0, si = 0;
+  for (int j = 1; j <= H; j++) { sm += pm[j]; si += pi[j]; }
+  return sm + si;
+}
+
+// Same model in long double, unscaled (row -1 D = 1/H); returns log10 L.
+long double pair_ld(const uint8_t* hap, int H, const uint8_t* rd, const uint8_t* q, const uint8_t* ins,
+                    const uint8_t* del, const uint8_t* gcp, int R, int wildcard_n) {
+  const Tables<double>& tb = tables<double>();
+  std::vector<long double> pm(H + 1), pi(H + 1), pd(H + 1), cm(H + 1), ci(H + 1), cd(H + 1);
+  for (int j = 0; j <= H; j++) { pm[j] = 0; pi[j] = 0; pd[j] = 1.0L / H; }
+  for (int k = 0; k < R; k++) {
+    const int iq = q[k] & 127, ii = ins[k] & 127, id = del[k] & 127, ic = gcp[k] & 127;
+    const long double dist = powl(10.0L, -(long double)iq / 10.0L);
+    const long double one_m = 1.0L - dist, distm = dist / 3.0L;
+    const long double g = powl(10.0L, -(long double)ic / 10.0L);
+    const long double gapm = 1.0L - g;
+    const long double tmm = (long double)tb.mm_prob(ii, id);
+    const long double tmi = powl(10.0L, -(long double)ii / 10.0L);
+    const long double tmd = powl(10.0L, -(long double)id / 10.0L);
+    cm[0] = ci[0] = cd[0] = 0;
+    for (int j = 0; j < H; j++) {
+      bool match = rd[k] == hap[j];
+      if (wildcard_n) {
+        const int rc = intra_class(rd[k]), hc = intra_class(hap[j]);
+        match = (rc == 4) || (hc == 4) || (rc == hc);
+      }
+      const long double prior = match ? one_m : distm;
+      cm[j + 1] = prior * (tmm * pm[j] + gapm * (pi[j] + pd[j]));
+      ci[j + 1] = tmi * pm[j + 1] + g * pi[j + 1];
+      cd[j + 1] = tmd * cm[j] + g * cd[j];
+    }
+    std::swap(pm, cm); std::swap(pi, ci); std::swap(pd, cd);
+  }
+  long double s = 0;
+  for (int j = 1; j <= H; j++) s += pm[j] + pi[j];
+  return log10l(s);
+}
+
+// ---------------------------------------------------------------------------
+// Brute force: enumerate every state path of the pair HMM explicitly.
+// A path starts in the deletion state of row -1 at a column j0 in [-1, H-1]
+// with weight 1/H, must leave it by a match move, and ends in M or I on the
+// last read row. Moves: match (k+1, j+1), insertion (k+1, j), deletion (k, j+1).
+// The transition into a cell uses the parameters of the cell's read row.
+// ---------------------------------------------------------------------------
+struct BF {
+  const uint8_t *hap, *rd;
+  int H, R, wildcard;
+  std::vector<long double> tmm, tgm, tmi, tii, tmd, tdd, pm_, px_;
+  long double total = 0;
+  long double prior(int k, int j) const {
+    bool match = rd[k] == hap[j];
+    if (wildcard) {
+      const int rc = intra_class(rd[k]), hc = intra_class(hap[j]);
+      match = (rc == 4) || (hc == 4) || (rc == hc);
+    }
+    return match ? pm_[k] : px_[k];
+  }
+  // state 0 = M, 1 = I, 2 = D
+  void walk(int st, int k, int j, long double w) {
+    if (k == R - 1 && st != 2) total += w;
+    if (k + 1 <= R - 1 && j + 1 <= H - 1)
+      walk(0, k + 1, j + 1, w * (st == 0 ? tmm[k + 1] : tgm[k + 1]) * prior(k + 1, j + 1));
+    if (k + 1 <= R - 1 && st != 2) walk(1, k + 1, j, w * (st == 0 ? tmi[k + 1] : tii[k + 1]));
+    if (j + 1 <= H - 1 && st != 1) walk(2, k, j + 1, w * (st == 0 ? tmd[k] : tdd[k]));
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+// ---- tables --------------------------------------------------------------
+void orc_tables_f32(float* ph2pr128, float* mm_tri /* 128*129/2 */) {
+  ensure_tables();
+  for (int x = 0; x < 128; x++) ph2pr128[x] = g_tf.ph2pr[x];
+  for (int i = 0, off = 0; i < 128; off += ++i)
+    for (int j = 0; j <= i; j++) mm_tri[off + j] = g_tf.mm[off + j];
+}
+void orc_tables_f64(double* ph2pr128, double* mm_tri) {
+  ensure_tables();
+  for (int x = 0; x < 128; x++) ph2pr128[x] = g_td.ph2pr[x];
+  for (int i = 0, off = 0; i < 128; off += ++i)
+    for (int j = 0; j <= i; j++) mm_tri[off + j] = g_td.mm[off + j];
+}
+float orc_mm_f32(int ins, int del) { return tables<float>().mm_prob(ins & 127, del & 127); }
+double orc_mm_f64(int ins, int del) { return tables<double>().mm_prob(ins & 127, del & 127); }
+
+// ---- per pair ------------------------------------------------------------
+float orc_pair_raw_f32(const uint8_t* hap, int H, const uint8_t* rd, const uint8_t* q, const uint8_t* i,
+                       const uint8_t* d, const uint8_t* c, int R, int mode) {
+  return pair_raw<float>(hap, H, rd, q, i, d, c, R, mode);
+}
+double orc_pair_raw_f64(const uint8_t* hap, int H, const uint8_t* rd, const uint8_t* q, const uint8_t* i,
+                        const uint8_t* d, const uint8_t* c, int R, int mode) {
+  return pair_raw<double>(hap, H, rd, q, i, d, c, R, mode);
+}
+// log10 constants, ph/adapter.h:25-26
+static const float kLog10CF = 36.1236000061f;
+static const double kLog10CD = 307.050595577260822;
+
+// ph/roDV_intra/pairhmm_api.cpp:14-23
+double orc_intra_log10(const uint8_t* hap, int H, const uint8_t* rd, const uint8_t* q, const uint8_t* i,
+                       const uint8_t* d, const uint8_t* c, int R, int use_double) {
+  float sf = use_double ? 0.f : pair_raw<float>(hap, H, rd, q, i, d, c, R, 1);
+  if (sf < 1e-28f) {
+    double sd = pair_raw<double>(hap, H, rd, q, i, d, c, R, 1);
+    return std::log10(sd) - kLog10CD;
+  }
+  return (double)(log10f(sf) - kLog10CF);
+}
+// FP64 oracle value of one pair: the model in double with C = 2^1020.
+double orc_pair_log10_f64(const uint8_t* hap, int H, const uint8_t* rd, const uint8_t* q, const uint8_t* i,
+                          const uint8_t* d, const uint8_t* c, int R, int wildcard_n) {
+  return std::log10(pair_raw<double>(hap, H, rd, q, i, d, c, R, wildcard_n ? 1 : 0)) - kLog10CD;
+}
+double orc_pair_log10_ld(const uint8_t* hap, int H, const uint8_t* rd, const uint8_t* q, const uint8_t* i,
+                         const uint8_t* d, const uint8_t* c, int R, int wildcard_n) {
+  return (double)pair_ld(hap, H, rd, q, i, d, c, R, wildcard_n);
+}
+double orc_bruteforce_log10(const uint8_t* hap, int H, const uint8_t* rd, const uint8_t* q, const uint8_t* i,
+                            const uint8_t* d, const uint8_t* c, int R, int wildcard_n) {
+  if (H <= 0 || R <= 0 || H > 7 || R > 7) return std::numeric_limits<double>::quiet_NaN();
+  const Tables<double>& tb = tables<double>();
+  BF bf;
+  bf.hap = hap; bf.rd = rd; bf.H = H; bf.R = R; bf.wildcard = wildcard_n;
+  for (int k = 0; k < R; k++) {
+    const int iq = q[k] & 127, ii = i[k] & 127, id = d[k] & 127, ic = c[k] & 127;
+    const long double dist = powl(10.0L, -(long double)iq / 10.0L);
+    const long double g = powl(10.0L, -(long double)ic / 10.0L);
+    bf.pm_.push_back(1.0L - dist);
+    bf.px_.push_back(dist / 3.0L);
+    bf.tmm.push_back((long double)tb.mm_prob(ii, id));
+    bf.tgm.push_back(1.0L - g);
+    bf.tmi.push_back(powl(10.0L, -(long double)ii / 10.0L));
+    bf.tii.push_back(g);
+    bf.tmd.push_back(powl(10.0L, -(long double)id / 10.0L));
+    bf.tdd.push_back(g);
+  }
+  for (int j0 = -1; j0 <= H - 2; j0++)
+    bf.walk(0, 0, j0 + 1, (1.0L / H) * bf.tgm[0] * bf.prior(0, j0 + 1));
+  return (double)log10l(bf.total);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Region routing (ph/roDV_schedule.cpp:364-834). Inputs are flat arrays:
+// hap_off[n_haps+1] / read_off[n_reads+1] index hap_bases / read_bases and the
+// per-base q, i, d, c arrays. Output is read-major [n_reads][n_haps]; pairs
+// with an empty side keep NaN. route[] receives, per pair:
+//   0 not computed, 1 inter FP32, 2 inter FP64 (rescued), 3 intra FP32,
+//   4 intra FP64 after an FP32 attempt inside intra, 5 intra FP64 after a
+//   failed inter FP32 attempt.
+// Returns 1 on success, 0 where the reference returns false.
+// ---------------------------------------------------------------------------
+namespace {
+struct HapReadPair {  // same fields and ordering key as ph/roDV_schedule.cpp:37-68
+  uint32_t read_idx;
+  uint16_t hap_idx;
+  uint16_t hap_len;
+  uint16_t read_len;
+  uint8_t flags;  // 1 = contains N, 2 = used
+  bool contains_n() const { return flags & 1; }
+  bool used() const { return flags & 2; }
+  void set_used(bool v) { flags = v ? (flags | 2) : (flags & ~2); }
+};
+
+// greedy_grouping, ph/roDV_schedule.cpp:86-142
+std::vector<std::vector<size_t>> greedy_grouping(std::vector<HapReadPair>& pairs, double max_idle, uint32_t width) {
+  std::vector<std::vector<size_t>> groups;
+  const size_t total = pairs.size();
+  for (size_t i = 0; i < total; ++i) {
+    if (pairs[i].used() || pairs[i].contains_n()) continue;
+    std::vector<size_t> g;
+    size_t idx = i;
+    while (idx < total && g.size() < width) {
+      if (!pairs[idx].used() && !pairs[idx].contains_n()) g.push_back(idx);
+      ++idx;
+    }
+    if (g.size() < width) break;
+    uint32_t mh = 0, mr = 0;
+    uint64_t sum = 0;
+    for (size_t p : g) {
+      mh = std::max<uint32_t>(mh, pairs[p].hap_len);
+      mr = std::max<uint32_t>(mr, pairs[p].read_len);
+      sum += (uint64_t)pairs[p].hap_len * pairs[p].read_len;
+    }
+    uint64_t cap = (uint64_t)mh * mr * width;
+    uint64_t idle = cap > sum ? cap - sum : 0;
+    double ratio = cap == 0 ? 0.0 : (double)idle / (double)cap;
+    if (ratio <= max_idle) {
+      for (size_t p : g) pairs[p].set_used(true);
+      groups.push_back(std::move(g));
+    }
+  }
+  return groups;
+}
+}  // namespace
+
+extern "C" int orc_region_reference(uint32_t n_haps, const int32_t* hap_off, const uint8_t* hap_bases, uint32_t n_reads,
+                                    const int32_t* read_off, const uint8_t* read_bases, const uint8_t* q,
+                                    const uint8_t* qi, const uint8_t* qd, const uint8_t* qc, uint32_t float_width,
+                                    uint32_t double_width, double* out, uint8_t* route) {
+  const size_t M = n_haps, N = n_reads;
+  for (size_t x = 0; x < M * N; x++) { out[x] = std::numeric_limits<double>::quiet_NaN(); route[x] = 0; }
+  if (M > 4096 || M == 0 || N == 0) return 0;
+  std::vector<char> hap_n(M, 0), read_n(N, 0);
+  for (size_t h = 0; h < M; h++)
+    for (int32_t x = hap_off[h]; x < hap_off[h + 1]; x++)
+      if (hap_bases[x] == 'N') { hap_n[h] = 1; break; }
+  for (size_t r = 0; r < N; r++)
+    for (int32_t x = read_off[r]; x < read_off[r + 1]; x++)
+      if (read_bases[x] == 'N') { read_n[r] = 1; break; }
+  std::vector<HapReadPair> pairs;
+  pairs.reserve(M * N);
+  for (size_t h = 0; h < M; h++) {
+    const uint32_t hl = hap_off[h + 1] - hap_off[h];
+    if (hl > 1536) return 0;
+    for (size_t r = 0; r < N; r++) {
+      const uint32_t rl = read_off[r + 1] - read_off[r];
+      if (rl > 768) return 0;
+      if (hl > 0 && rl > 0)
+        pairs.push_back(HapReadPair{(uint32_t)r, (uint16_t)h, (uint16_t)hl, (uint16_t)rl,
+                                    (uint8_t)((hap_n[h] || read_n[r]) ? 1 : 0)});
+    }
+  }
+  if (pairs.empty()) return 1;
+  std::sort(pairs.begin(), pairs.end(), [](const HapReadPair& a, const HapReadPair& b) {
+    if (a.hap_len == b.hap_len) return a.read_len > b.read_len;
+    return a.hap_len > b.hap_len;
+  });
+  auto args = [&](const HapReadPair& p, const uint8_t*& hb, int& hl, const uint8_t*& rb, int& rl, const uint8_t*& a,
+                  const uint8_t*& b, const uint8_t*& c, const uint8_t*& d) {
+    hb = hap_bases + hap_off[p.hap_idx];
+    hl = hap_off[p.hap_idx + 1] - hap_off[p.hap_idx];
+    const int32_t o = read_off[p.read_idx];
+    rb = read_bases + o; rl = read_off[p.read_idx + 1] - o;
+    a = q + o; b = qi + o; c = qd + o; d = qc + o;
+  };
+  std::vector<char> processed_float(M * N, 0), processed_double(M * N, 0);
+  // pass 1: FP32 inter groups
+  if (float_width > 0) {
+    auto groups = greedy_grouping(pairs, 0.02, float_width);
+    for (const auto& g : groups)
+      for (size_t pi : g) {
+        HapReadPair& p = pairs[pi];
+        const uint8_t *hb, *rb, *a, *b, *c, *d; int hl, rl;
+        args(p, hb, hl, rb, rl, a, b, c, d);
+        float s = pair_raw<float>(hb, hl, rb, a, b, c, d, rl, 0);
+        const size_t key = (size_t)p.hap_idx * N + p.read_idx;
+        if ((double)s < (double)1e-28f) {
+          p.set_used(false);
+        } else {
+          p.set_used(true);
+          out[(size_t)p.read_idx * M + p.hap_idx] = std::log10((double)s) - (double)kLog10CF;
+          route[(size_t)p.read_idx * M + p.hap_idx] = 1;
+          processed_double[key] = 1;
+        }
+        processed_float[key] = 1;
+      }
+  }
+  // pass 2: FP64 inter groups over the FP32 failures
+  if (double_width > 0) {
+    for (auto& p : pairs)
+      if (!processed_float[(size_t)p.hap_idx * N + p.read_idx]) p.set_used(true);
+    auto groups = greedy_grouping(pairs, 0.02, double_width);
+    for (const auto& g : groups)
+      for (size_t pi : g) {
+        const HapReadPair& p = pairs[pi];
+        const uint8_t *hb, *rb, *a, *b, *c, *d; int hl, rl;
+        args(p, hb, hl, rb, rl, a, b, c, d);
+        double s = pair_raw<double>(hb, hl, rb, a, b, c, d, rl, 0);
+        out[(size_t)p.read_idx * M + p.hap_idx] = std::log10(s) - kLog10CD;
+        route[(size_t)p.read_idx * M + p.hap_idx] = 2;
+        processed_double[(size_t)p.hap_idx * N + p.read_idx] = 1;
+      }
+  }
+  // pass 3: intra for everything left
+  for (size_t h = 0; h < M; h++)
+    for (size_t r = 0; r < N; r++) {
+      if (processed_double[h * N + r]) continue;
+      const int hl = hap_off[h + 1] - hap_off[h];
+      const int32_t o = read_off[r];
+      const int rl = read_off[r + 1] - o;
+      if (hl <= 0 || rl <= 0) continue;
+      const bool use_double = processed_float[h * N + r];
+      const uint8_t* hb = hap_bases + hap_off[h];
+      float sf = use_double ? 0.f : pair_raw<float>(hb, hl, read_bases + o, q + o, qi + o, qd + o, qc + o, rl, 1);
+      double v;
+      uint8_t rt;
+      if (sf < 1e-28f) {
+        double sd = pair_raw<double>(hb, hl, read_bases + o, q + o, qi + o, qd + o, qc + o, rl, 1);
+        v = std::log10(sd) - kLog10CD;
+        rt = use_double ? 5 : 4;
+      } else {
+        v = (double)(log10f(sf) - kLog10CF);
+        rt = 3;
+      }
+      out[r * M + h] = v;
+      route[r * M + h] = rt;
+    }
+  return 1;
+}
+
+// ---------------------------------------------------------------------------
+// Read preprocessing (ph/roDV_schedule.cpp:522-649) and PCR indel model.
+// ---------------------------------------------------------------------------
+namespace {
+// ph/adapter.h:13-15
+const int kPcrHostile[21] = {40, 40, 39, 38, 37, 36, 34, 32, 28, 23, 17, 10, 10, 10, 10, 10, 10, 10, 10, 10, 10};
+const int kPcrAggressive[21] = {40, 40, 40, 39, 39, 39, 38, 38, 37, 37, 36, 35, 34, 33, 32, 30, 28, 26, 23, 20, 17};
+const int kPcrConservative[21] = {40, 40, 40, 40, 39, 39, 39, 39, 39, 38, 38, 38, 37, 37, 37, 36, 36, 35, 34, 33, 33};
+
+// ph/pairhmm_internal.cpp:544-574
+int n_repetitions(const uint8_t* unit_full, int off_unit, int unit_len, const uint8_t* test_full, int off_test,
+                  int test_len, bool leading) {
+  if (test_len == 0) return 0;
+  int n = 0;
+  const int diff = test_len - unit_len;
+  if (leading) {
+    for (int start = 0; start <= diff; start += unit_len) {
+      if (std::memcmp(test_full + start + off_test, unit_full + off_unit, unit_len) == 0) ++n;
+      else return n;
+    }
+  } else {
+    for (int start = diff; start >= 0; start -= unit_len) {
+      if (std::memcmp(test_full + start + off_test, unit_full + off_unit, unit_len) == 0) ++n;
+      else return n;
+    }
+  }
+  return n;
+}
+
+// ph/pairhmm_internal.cpp:260-314 (including its stored backward offset)
+int tandem_repeat_units(const uint8_t* b, int num, int offset) {
+  int max_bw = 0;
+  std::pair<int, int> bw(offset, 1);
+  for (int str = 1; str <= 8; str++) {
+    if (offset + 1 - str < 0) break;
+    max_bw = n_repetitions(b, offset - str + 1, str, b, 0, offset + 1, false);
+    if (max_bw > 1) { bw = {offset - str - 1, str}; break; }
+  }
+  int max_rl = max_bw;
+  if (offset < num - 1) {
+    std::pair<int, int> fw(offset + 1, 1);
+    int max_fw = 0;
+    for (int str = 1; str <= 8; str++) {
+      if (offset + str + 1 > num) break;
+      max_fw = n_repetitions(b, offset + 1, str, b, offset + 1, num - offset - 1, true);
+      if (max_fw > 1) { fw = {offset + 1, str}; break; }
+    }
+    if (fw == bw) {
+      max_rl = max_bw + max_fw;
+    } else {
+      max_bw = n_repetitions(b, fw.first, fw.second, b, 0, offset + 1, false);
+      max_rl = max_bw + max_fw;
+    }
+  }
+  return max_rl > 20 ? 20 : max_rl;
+}
+}  // namespace
+
+// bases: decoded ASCII read bases; qual: raw Phred; pcr: 0 NONE, 1 HOSTILE,
+// 2 AGGRESSIVE, 3 CONSERVATIVE (hc/common/enum.h:8).
+extern "C" void orc_preprocess_read(const uint8_t* bases, const uint8_t* qual, int len, int mq, int min_q, int pcr,
+                                    uint8_t* q, uint8_t* qi, uint8_t* qd, uint8_t* qc) {
+  for (int x = 0; x < len; x++) {
+    uint8_t v = std::min<uint8_t>(qual[x], (uint8_t)mq);
+    q[x] = v < min_q ? (uint8_t)6 : v;
+    qi[x] = '-';
+    qc[x] = (uint8_t)('+' - 33);
+  }
+  if (pcr != 0) {
+    const int* tab = pcr == 1 ? kPcrHostile : pcr == 2 ? kPcrAggressive : kPcrConservative;
+    for (int x = 1; x < len; x++) {
+      const int rl = tandem_repeat_units(bases, len, x - 1);
+      qi[x - 1] = qi[x - 1] < tab[rl] ? qi[x - 1] : (uint8_t)tab[rl];
+    }
+  }
+  for (int x = 0; x < len; x++) qd[x] = qi[x];
+}
+
+// Post-processing of a read-major [n][h] matrix (ph/pairhmm_internal.cpp:576-623):
+// normalize, drop poorly modelled reads, transpose into hap-major [h][n] whose
+// trailing (n - kept) columns stay NaN. kept[] receives the surviving read
+// indices; returns their count.
+extern "C" int orc_postprocess(int n, int h, double* read_major, const int32_t* read_len, double* hap_major,
+                               int32_t* kept) {
+  for (int r = 0; r < n; r++) {
+    double* row = read_major + (size_t)r * h;
+    if (h == 0) continue;
+    double best = row[0];  // std::max_element: a leading NaN wins
+    for (int x = 1; x < h; x++)
+      if (best < row[x]) best = row[x];
+    const double cmp = best + (-4.5);
+    for (int x = 0; x < h; x++) row[x] = row[x] < cmp ? cmp : row[x];
+  }
+  int nk = 0;
+  std::vector<int> keep;
+  for (int r = 0; r < n; r++) {
+    double best = -std::numeric_limits<double>::infinity();
+    for (int x = 0; x < h; x++)
+      if (!std::isnan(read_major[(size_t)r * h + x])) best = std::max(best, read_major[(size_t)r * h + x]);
+    const double thr = std::min(2.0, std::ceil(read_len[r] * 0.02)) * (-4.0);
+    if (!(best < thr)) keep.push_back(r);
+  }
+  for (size_t x = 0; x < (size_t)h * n; x++) hap_major[x] = std::numeric_limits<double>::quiet_NaN();
+  for (int r : keep) {
+    for (int x = 0; x < h; x++) hap_major[(size_t)x * n + nk] = read_major[(size_t)r * h + x];
+    kept[nk++] = r;
+  }
+  return nk;
+}

@@ -1,0 +1,292 @@
+"""MI355X PairHMM engine for Rovaca's HaplotypeCaller (Python bindings).
+
+The engine itself is ``librovaca_phmm.so`` (HIP kernels for gfx950 plus a C++
+host side) with the C ABI declared in ``include/rovaca_phmm.h``. This module
+binds that ABI with ctypes for tests, the smoke check and the benchmark. It
+never computes likelihoods itself: if the library is missing, every call
+raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librovaca_phmm.so")
+
+RVC_OK = 0
+RVC_E_INVALID = 1
+RVC_E_LIMIT = 2
+RVC_E_DEVICE = 3
+RVC_E_STATE = 4
+RVC_E_BUDGET = 5
+RVC_E_TIMEOUT = 6
+RVC_E_SHUTDOWN = 7
+PAIR_COMPUTED = 1
+PAIR_FP64 = 2
+PCR_NONE, PCR_HOSTILE, PCR_AGGRESSIVE, PCR_CONSERVATIVE = 0, 1, 2, 3
+
+# Every symbol include/rovaca_phmm.h declares.
+EXPORTS = [
+    "rvc_phmm_init", "rvc_phmm_shutdown", "rvc_phmm_last_error", "rvc_phmm_region_sync",
+    "rvc_phmm_call_pairhmm", "rvc_phmm_batch_create", "rvc_phmm_batch_run", "rvc_phmm_batch_results",
+    "rvc_phmm_batch_info", "rvc_phmm_batch_destroy", "rvc_phmm_submit", "rvc_phmm_wait", "rvc_phmm_poll",
+    "rvc_phmm_server_stats", "rvc_phmm_preprocess_read", "rvc_phmm_postprocess", "rvc_phmm_tables",
+]
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class RvcRegion(ctypes.Structure):
+    _fields_ = [("n_haps", ctypes.c_uint32), ("n_reads", ctypes.c_uint32), ("hap_off", _i32p),
+                ("hap_bases", _u8p), ("read_off", _i32p), ("read_bases", _u8p), ("q", _u8p), ("i", _u8p),
+                ("d", _u8p), ("c", _u8p)]
+
+
+class RvcRawRegion(ctypes.Structure):
+    _fields_ = [("n_haps", ctypes.c_uint32), ("n_reads", ctypes.c_uint32), ("hap_off", _i32p),
+                ("hap_bases", _u8p), ("read_off", _i32p), ("read_bases", _u8p), ("read_qual", _u8p),
+                ("read_mq", _u8p)]
+
+
+class RvcConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("server_threads", ctypes.c_uint32),
+                ("hbm_budget_bytes", ctypes.c_uint64), ("max_batch_regions", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 8)]
+
+
+class RvcBatchOpts(ctypes.Structure):
+    _fields_ = [("persistent_waves_per_cu", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 6)]
+
+
+class RvcBatchInfo(ctypes.Structure):
+    _fields_ = [("n_pairs", ctypes.c_uint64), ("n_cells", ctypes.c_uint64), ("n_rescued", ctypes.c_uint64),
+                ("rescued_cells", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
+                ("fp32_ms", ctypes.c_double), ("fp64_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("n_fp32_launches", ctypes.c_uint32), ("n_fp64_launches", ctypes.c_uint32)]
+
+
+class RvcServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("launches", ctypes.c_uint64),
+                ("cells", ctypes.c_uint64), ("rescued_pairs", ctypes.c_uint64),
+                ("resident_bytes", ctypes.c_uint64), ("peak_resident_bytes", ctypes.c_uint64),
+                ("busy_ms", ctypes.c_double)]
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rvc error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """The loaded engine library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rovaca_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.rvc_phmm_init.argtypes = [ctypes.POINTER(RvcConfig)]
+        L.rvc_phmm_last_error.restype = ctypes.c_char_p
+        L.rvc_phmm_region_sync.argtypes = [ctypes.POINTER(RvcRegion), _f64p, _u8p]
+        L.rvc_phmm_call_pairhmm.argtypes = [ctypes.POINTER(RvcRawRegion), ctypes.c_int32, ctypes.c_int32, _f64p,
+                                            _u32p, _u32p]
+        L.rvc_phmm_batch_create.argtypes = [ctypes.POINTER(RvcRegion), ctypes.c_uint32,
+                                            ctypes.POINTER(RvcBatchOpts), ctypes.POINTER(ctypes.c_void_p)]
+        L.rvc_phmm_batch_run.argtypes = [ctypes.c_void_p]
+        L.rvc_phmm_batch_results.argtypes = [ctypes.c_void_p, _f64p, _u8p]
+        L.rvc_phmm_batch_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(RvcBatchInfo)]
+        L.rvc_phmm_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.rvc_phmm_submit.argtypes = [ctypes.POINTER(RvcRegion), ctypes.POINTER(ctypes.c_uint64)]
+        L.rvc_phmm_wait.argtypes = [ctypes.c_uint64, _f64p, _u8p, ctypes.c_int64]
+        L.rvc_phmm_poll.argtypes = [ctypes.c_uint64]
+        L.rvc_phmm_server_stats.argtypes = [ctypes.POINTER(RvcServerStats)]
+        L.rvc_phmm_preprocess_read.argtypes = [_u8p, _u8p, ctypes.c_int32, ctypes.c_uint8, ctypes.c_int32,
+                                               ctypes.c_int32, _u8p, _u8p, _u8p]
+        L.rvc_phmm_postprocess.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _f64p, _i32p, _f64p, _u32p]
+        L.rvc_phmm_tables.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _f64p,
+                                      _f64p]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != RVC_OK:
+        raise EngineError(rc, lib().rvc_phmm_last_error().decode(errors="replace"))
+
+
+def _ptr(a, t=_u8p):
+    return a.ctypes.data_as(t)
+
+
+def make_c_region(region):
+    """ctypes rvc_region viewing the arrays of a region dict (keep the dict alive)."""
+    cr = RvcRegion()
+    cr.n_haps = int(region["n_haps"])
+    cr.n_reads = int(region["n_reads"])
+    cr.hap_off = _ptr(region["hap_off"], _i32p)
+    cr.hap_bases = _ptr(region["hap_bases"])
+    cr.read_off = _ptr(region["read_off"], _i32p)
+    cr.read_bases = _ptr(region["read_bases"])
+    for k in "qidc":
+        setattr(cr, k, _ptr(region[k]))
+    return cr
+
+
+def init(device=0, server=False, hbm_budget_bytes=0, max_batch_regions=0):
+    cfg = RvcConfig()
+    cfg.device = device
+    cfg.server_threads = 1 if server else 0
+    cfg.hbm_budget_bytes = hbm_budget_bytes
+    cfg.max_batch_regions = max_batch_regions
+    _check(lib().rvc_phmm_init(ctypes.byref(cfg)))
+
+
+def shutdown():
+    _check(lib().rvc_phmm_shutdown())
+
+
+def region_sync(region):
+    """Read-major [n_reads, n_haps] log10 likelihoods and per-pair flags of one region."""
+    nr, nh = int(region["n_reads"]), int(region["n_haps"])
+    out = np.empty(nr * nh, np.float64)
+    flags = np.zeros(nr * nh, np.uint8)
+    cr = make_c_region(region)
+    _check(lib().rvc_phmm_region_sync(ctypes.byref(cr), _ptr(out, _f64p), _ptr(flags)))
+    return out.reshape(nr, nh), flags.reshape(nr, nh)
+
+
+def call_pairhmm(haps, reads, quals, mqs, min_quality_threshold=0, pcr=PCR_NONE):
+    """The region-level call of the reference (raw reads in, [H][N] matrix and kept reads out)."""
+    hap_off = np.zeros(len(haps) + 1, np.int32)
+    hap_off[1:] = np.cumsum([len(h) for h in haps])
+    read_off = np.zeros(len(reads) + 1, np.int32)
+    read_off[1:] = np.cumsum([len(r) for r in reads])
+    hb = np.frombuffer(b"".join(bytes(h) for h in haps) or b"\0", np.uint8).copy()
+    rb = np.frombuffer(b"".join(bytes(r) for r in reads) or b"\0", np.uint8).copy()
+    rq = np.concatenate([np.asarray(q, np.uint8) for q in quals] + [np.zeros(1, np.uint8)])
+    mq = np.asarray(mqs, np.uint8)
+    raw = RvcRawRegion()
+    raw.n_haps, raw.n_reads = len(haps), len(reads)
+    raw.hap_off, raw.hap_bases = _ptr(hap_off, _i32p), _ptr(hb)
+    raw.read_off, raw.read_bases = _ptr(read_off, _i32p), _ptr(rb)
+    raw.read_qual, raw.read_mq = _ptr(rq), _ptr(mq)
+    H, N = len(haps), len(reads)
+    out = np.empty(max(H * N, 1), np.float64)
+    kept = np.zeros(max(N, 1), np.uint32)
+    nk = ctypes.c_uint32(0)
+    rc = lib().rvc_phmm_call_pairhmm(ctypes.byref(raw), min_quality_threshold, pcr, _ptr(out, _f64p),
+                                     _ptr(kept, _u32p), ctypes.byref(nk))
+    return rc, out[:H * N].reshape(H, N), kept[:nk.value].copy()
+
+
+class Batch:
+    """Many regions packed, uploaded and launched together."""
+
+    def __init__(self, regions, use_graph=False):
+        self._regions = regions
+        arr = (RvcRegion * len(regions))()
+        for x, r in enumerate(regions):
+            arr[x] = make_c_region(r)
+        self._arr = arr
+        opts = RvcBatchOpts()
+        opts.use_graph = 1 if use_graph else 0
+        h = ctypes.c_void_p()
+        _check(lib().rvc_phmm_batch_create(arr, len(regions), ctypes.byref(opts), ctypes.byref(h)))
+        self.h = h
+        self.offsets = np.zeros(len(regions) + 1, np.int64)
+        self.offsets[1:] = np.cumsum([int(r["n_reads"]) * int(r["n_haps"]) for r in regions])
+
+    def run(self):
+        _check(lib().rvc_phmm_batch_run(self.h))
+
+    def info(self):
+        inf = RvcBatchInfo()
+        _check(lib().rvc_phmm_batch_info(self.h, ctypes.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in RvcBatchInfo._fields_}
+
+    def results(self):
+        n = int(self.offsets[-1])
+        out = np.empty(n, np.float64)
+        flags = np.zeros(n, np.uint8)
+        _check(lib().rvc_phmm_batch_results(self.h, _ptr(out, _f64p), _ptr(flags)))
+        return out, flags
+
+    def region_result(self, out, r):
+        reg = self._regions[r]
+        return out[self.offsets[r]:self.offsets[r + 1]].reshape(int(reg["n_reads"]), int(reg["n_haps"]))
+
+    def close(self):
+        if self.h:
+            lib().rvc_phmm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def submit(region):
+    cr = make_c_region(region)
+    t = ctypes.c_uint64()
+    _check(lib().rvc_phmm_submit(ctypes.byref(cr), ctypes.byref(t)))
+    return t.value
+
+
+def wait(ticket, region, timeout_ms=-1):
+    nr, nh = int(region["n_reads"]), int(region["n_haps"])
+    out = np.empty(nr * nh, np.float64)
+    flags = np.zeros(nr * nh, np.uint8)
+    _check(lib().rvc_phmm_wait(ticket, _ptr(out, _f64p), _ptr(flags), timeout_ms))
+    return out.reshape(nr, nh), flags.reshape(nr, nh)
+
+
+def preprocess_read(bases, qual, mq, min_quality_threshold=0, pcr=PCR_NONE):
+    """Host-side read preprocessing: returns (q, gop, gcp) uint8 arrays."""
+    b = np.ascontiguousarray(np.frombuffer(bytes(bases), np.uint8) if isinstance(bases, (bytes, bytearray))
+                             else bases, dtype=np.uint8)
+    ql = np.ascontiguousarray(qual, dtype=np.uint8)
+    n = len(b)
+    q, g, c = (np.zeros(max(n, 1), np.uint8) for _ in range(3))
+    _check(lib().rvc_phmm_preprocess_read(_ptr(b), _ptr(ql), n, mq, min_quality_threshold, pcr, _ptr(q), _ptr(g),
+                                          _ptr(c)))
+    return q[:n], g[:n], c[:n]
+
+
+def postprocess(read_major, read_len):
+    """Host-side normalize / filter / transpose: returns ([H][N] matrix, kept read indices)."""
+    rm = np.ascontiguousarray(read_major, dtype=np.float64).copy()
+    n, h = rm.shape
+    rl = np.ascontiguousarray(read_len, dtype=np.int32)
+    out = np.empty((h, n), np.float64)
+    kept = np.zeros(n, np.uint32)
+    nk = lib().rvc_phmm_postprocess(n, h, _ptr(rm, _f64p), _ptr(rl, _i32p), _ptr(out, _f64p), _ptr(kept, _u32p))
+    if nk < 0:
+        _check(-nk)
+    return out, kept[:nk].copy()
+
+
+def tables():
+    """(ph2pr_f32[128], mm_f32[8256], ph2pr_f64[128], mm_f64[8256]) as the engine builds them."""
+    pf = np.zeros(128, np.float32)
+    mf = np.zeros(128 * 129 // 2, np.float32)
+    pd = np.zeros(128, np.float64)
+    md = np.zeros(128 * 129 // 2, np.float64)
+    fp = ctypes.POINTER(ctypes.c_float)
+    _check(lib().rvc_phmm_tables(_ptr(pf, fp), _ptr(mf, fp), _ptr(pd, _f64p), _ptr(md, _f64p)))
+    return pf, mf, pd, md
+
+
+def server_stats():
+    st = RvcServerStats()
+    _check(lib().rvc_phmm_server_stats(ctypes.byref(st)))
+    return {k: getattr(st, k) for k, _ in RvcServerStats._fields_}

@@ -1,0 +1,269 @@
+// The C ABI of include/rovaca_phmm.h: lifecycle, one-region calls, batches.
+// The serving entry points live in phmm_server.cpp.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/rovaca_phmm.h"
+#include "phmm_adapter.h"
+#include "phmm_engine.h"
+#include "phmm_server.h"
+
+using namespace rvc;
+
+struct rvc_batch {
+  HostPack pack;
+  DeviceBatch dev;
+  bool use_graph = false;
+  bool ran = false;
+};
+
+namespace {
+
+// Per-thread device workspace of the one-region path: a stream and a grow-only
+// device allocation reused from call to call. Workspaces are owned by a
+// registry and released by rvc_phmm_shutdown, never by thread exit.
+struct SyncWorkspace {
+  hipStream_t stream = nullptr;
+  void* mem = nullptr;
+  size_t bytes = 0;
+  HostPack pack;
+  DeviceBatch batch;
+};
+std::mutex g_ws_mu;
+std::vector<SyncWorkspace*> g_ws_all;
+thread_local SyncWorkspace* t_ws = nullptr;
+uint64_t g_ws_generation = 1;
+thread_local uint64_t t_ws_generation = 0;
+
+int get_workspace(SyncWorkspace** out) {
+  if (!t_ws || t_ws_generation != g_ws_generation) {
+    auto* w = new (std::nothrow) SyncWorkspace;
+    if (!w) return set_error(RVC_E_INVALID, "out of host memory");
+    hipError_t e = hipSetDevice(g_engine.device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete w;
+      return set_error(RVC_E_DEVICE, "workspace stream: %s", hipGetErrorString(e));
+    }
+    w->batch.stream = w->stream;
+    w->batch.own_stream = false;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    g_ws_all.push_back(w);
+    t_ws = w;
+    t_ws_generation = g_ws_generation;
+  }
+  *out = t_ws;
+  return RVC_OK;
+}
+
+void release_workspaces() {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  for (SyncWorkspace* w : g_ws_all) {
+    w->batch.release();
+    if (w->mem) hipFree(w->mem);
+    if (w->stream) hipStreamDestroy(w->stream);
+    delete w;
+  }
+  g_ws_all.clear();
+  ++g_ws_generation;
+}
+
+FP32Policy& policy() {
+  static FP32Policy p;
+  return p;
+}
+
+}  // namespace
+
+namespace rvc {
+// One region through the device on the calling thread's workspace.
+int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags) {
+  if (!g_engine.ready) return set_error(RVC_E_STATE, "rvc_phmm_init has not run");
+  int rc = validate_region(region);
+  if (rc) return rc;
+  if (!out) return set_error(RVC_E_INVALID, "null output");
+  SyncWorkspace* w = nullptr;
+  if ((rc = get_workspace(&w))) return rc;
+  if ((rc = build_host_pack(region, 1, policy(), &w->pack))) return rc;
+  w->batch.graph_exec = nullptr;
+  w->batch.graph = nullptr;
+  if ((rc = w->batch.upload(w->pack, w->stream, w->mem, w->bytes))) return rc;
+  if (w->batch.own_mem) {  // the workspace was too small: keep the new allocation
+    if (w->mem) hipFree(w->mem);
+    w->mem = w->batch.mem;
+    w->bytes = w->batch.bytes;
+    w->batch.own_mem = false;
+  }
+  if ((rc = w->batch.run(false))) return rc;
+  return w->batch.download(out, flags, w->stream);
+}
+}  // namespace rvc
+
+extern "C" {
+
+int rvc_phmm_init(const rvc_config* cfg) {
+  int rc = engine_init(cfg);
+  if (rc) return rc;
+  if (cfg && cfg->server_threads) {
+    rc = server_start();
+    if (rc) {
+      engine_shutdown();
+      return rc;
+    }
+  }
+  return RVC_OK;
+}
+
+int rvc_phmm_shutdown(void) {
+  server_stop();
+  release_workspaces();
+  return engine_shutdown();
+}
+
+const char* rvc_phmm_last_error(void) { return last_error(); }
+
+int rvc_phmm_region_sync(const rvc_region* region, double* out_read_major, uint8_t* out_flags) {
+  return region_sync_impl(region, out_read_major, out_flags);
+}
+
+int rvc_phmm_call_pairhmm(const rvc_raw_region* rg, int32_t min_quality_threshold, int32_t pcr_option,
+                          double* out_hap_major, uint32_t* kept_reads, uint32_t* n_kept) {
+  if (!rg || !out_hap_major || !kept_reads || !n_kept) return set_error(RVC_E_INVALID, "null argument");
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  const uint32_t H = rg->n_haps, N = rg->n_reads;
+  auto fail = [&](int code) {
+    for (size_t x = 0; x < (size_t)H * N; ++x) out_hap_major[x] = nan;
+    for (uint32_t r = 0; r < N; ++r) kept_reads[r] = r;
+    *n_kept = N;
+    return code;
+  };
+  if (H == 0 || N == 0 || !rg->hap_off || !rg->read_off || !rg->read_qual || !rg->read_mq)
+    return fail(set_error(RVC_E_INVALID, "empty or incomplete region"));
+  if (pcr_option < RVC_PCR_NONE || pcr_option > RVC_PCR_CONSERVATIVE)
+    return fail(set_error(RVC_E_INVALID, "unknown PCR indel model %d", pcr_option));
+  const int32_t total = rg->read_off[N];
+  std::vector<uint8_t> q(total), gop(total), gcp(total);
+  std::vector<int32_t> rlen(N);
+  for (uint32_t r = 0; r < N; ++r) {
+    const int32_t o = rg->read_off[r], len = rg->read_off[r + 1] - o;
+    if (len < 0) return fail(set_error(RVC_E_INVALID, "read %u has negative length", r));
+    rlen[r] = len;
+    preprocess_read(rg->read_bases + o, rg->read_qual + o, len, rg->read_mq[r], min_quality_threshold, pcr_option,
+                    q.data() + o, gop.data() + o, gcp.data() + o);
+  }
+  rvc_region reg;
+  reg.n_haps = H;
+  reg.n_reads = N;
+  reg.hap_off = rg->hap_off;
+  reg.hap_bases = rg->hap_bases;
+  reg.read_off = rg->read_off;
+  reg.read_bases = rg->read_bases;
+  reg.q = q.data();
+  reg.i = gop.data();
+  reg.d = gop.data();
+  reg.c = gcp.data();
+  std::vector<double> read_major((size_t)H * N);
+  int rc = region_sync_impl(&reg, read_major.data(), nullptr);
+  if (rc) return fail(rc);
+  *n_kept = (uint32_t)postprocess(N, H, read_major.data(), rlen.data(), out_hap_major, kept_reads);
+  return RVC_OK;
+}
+
+int rvc_phmm_batch_create(const rvc_region* regions, uint32_t n_regions, const rvc_batch_opts* opts,
+                          rvc_batch** out) {
+  if (!out) return set_error(RVC_E_INVALID, "null output handle");
+  *out = nullptr;
+  if (!g_engine.ready) return set_error(RVC_E_STATE, "rvc_phmm_init has not run");
+  if (!regions || n_regions == 0) return set_error(RVC_E_INVALID, "no regions");
+  for (uint32_t r = 0; r < n_regions; ++r) {
+    int rc = validate_region(&regions[r]);
+    if (rc) return rc;
+  }
+  auto* b = new (std::nothrow) rvc_batch;
+  if (!b) return set_error(RVC_E_INVALID, "out of host memory");
+  std::unique_ptr<rvc_batch> guard(b);
+  int rc = build_host_pack(regions, n_regions, policy(), &b->pack);
+  if (rc) return rc;
+  if (hipSetDevice(g_engine.device) != hipSuccess || hipStreamCreateWithFlags(&b->dev.stream, hipStreamNonBlocking) != hipSuccess)
+    return set_error(RVC_E_DEVICE, "batch stream");
+  b->dev.own_stream = true;
+  if ((rc = b->dev.upload(b->pack, b->dev.stream, nullptr, 0))) return rc;
+  if (hipStreamSynchronize(b->dev.stream) != hipSuccess) return set_error(RVC_E_DEVICE, "batch upload");
+  b->use_graph = opts && opts->use_graph;
+  *out = guard.release();
+  return RVC_OK;
+}
+
+int rvc_phmm_batch_run(rvc_batch* b) {
+  if (!b) return set_error(RVC_E_INVALID, "null batch");
+  int rc = b->dev.run(b->use_graph);
+  if (rc == RVC_OK) b->ran = true;
+  return rc;
+}
+
+int rvc_phmm_batch_results(rvc_batch* b, double* out, uint8_t* flags) {
+  if (!b || !out) return set_error(RVC_E_INVALID, "null argument");
+  if (!b->ran) return set_error(RVC_E_STATE, "batch has not run");
+  return b->dev.download(out, flags, b->dev.stream);
+}
+
+int rvc_phmm_batch_info(const rvc_batch* b, rvc_batch_info* info) {
+  if (!b || !info) return set_error(RVC_E_INVALID, "null argument");
+  std::memset(info, 0, sizeof *info);
+  info->n_pairs = b->pack.n_pairs;
+  info->n_cells = b->pack.n_cells;
+  info->n_rescued = b->dev.n_rescued;
+  info->device_bytes = b->dev.bytes;
+  info->fp32_ms = b->dev.fp32_ms;
+  info->fp64_ms = b->dev.fp64_ms;
+  info->total_ms = b->dev.total_ms;
+  info->n_fp32_launches = b->dev.n_fp32_launches;
+  info->n_fp64_launches = b->dev.n_fp64_launches;
+  return RVC_OK;
+}
+
+int rvc_phmm_batch_destroy(rvc_batch* b) {
+  if (!b) return set_error(RVC_E_INVALID, "null batch");
+  delete b;
+  return RVC_OK;
+}
+
+int rvc_phmm_preprocess_read(const uint8_t* bases, const uint8_t* qual, int32_t len, uint8_t mq,
+                             int32_t min_quality_threshold, int32_t pcr_option, uint8_t* q, uint8_t* gop,
+                             uint8_t* gcp) {
+  if (len < 0 || (len > 0 && (!bases || !qual || !q || !gop || !gcp)))
+    return set_error(RVC_E_INVALID, "bad read arguments");
+  if (pcr_option < RVC_PCR_NONE || pcr_option > RVC_PCR_CONSERVATIVE)
+    return set_error(RVC_E_INVALID, "unknown PCR indel model %d", pcr_option);
+  preprocess_read(bases, qual, len, mq, min_quality_threshold, pcr_option, q, gop, gcp);
+  return RVC_OK;
+}
+
+int rvc_phmm_postprocess(uint32_t n_reads, uint32_t n_haps, double* read_major, const int32_t* read_len,
+                         double* out_hap_major, uint32_t* kept_reads) {
+  if (n_reads == 0 || n_haps == 0 || !read_major || !read_len || !out_hap_major || !kept_reads)
+    return -set_error(RVC_E_INVALID, "bad post-processing arguments");
+  return postprocess(n_reads, n_haps, read_major, read_len, out_hap_major, kept_reads);
+}
+
+int rvc_phmm_tables(float* ph2pr_f32, float* mm_f32, double* ph2pr_f64, double* mm_f64) {
+  static HostTables* t = [] {
+    auto* p = new HostTables;
+    build_tables(p);
+    return p;
+  }();
+  if (ph2pr_f32) std::memcpy(ph2pr_f32, t->ph2pr_f, sizeof t->ph2pr_f);
+  if (mm_f32) std::memcpy(mm_f32, t->mm_f, sizeof t->mm_f);
+  if (ph2pr_f64) std::memcpy(ph2pr_f64, t->ph2pr_d, sizeof t->ph2pr_d);
+  if (mm_f64) std::memcpy(mm_f64, t->mm_d, sizeof t->mm_d);
+  return RVC_OK;
+}
+
+}  // extern "C"

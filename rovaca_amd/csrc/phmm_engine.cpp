@@ -1,0 +1,505 @@
+// Host side of the MI355X PairHMM engine: device set-up, packing of regions
+// into the device format, length-binned tile queues, launches and the C ABI
+// of include/rovaca_phmm.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/rovaca_phmm.h"
+#include "phmm_engine.h"
+#include "phmm_internal.h"
+#include "phmm_tables.h"
+
+namespace rvc {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+namespace {
+thread_local std::string t_last_error;
+}
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  t_last_error = buf;
+  return code;
+}
+const char* last_error() { return t_last_error.c_str(); }
+
+#define RVC_HIP(call)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess) return set_error(RVC_E_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// engine-wide state
+// ---------------------------------------------------------------------------
+Engine g_engine;
+
+int engine_init(const rvc_config* cfg) {
+  std::lock_guard<std::mutex> lk(g_engine.mu);
+  if (g_engine.ready) return set_error(RVC_E_STATE, "rvc_phmm_init called twice");
+  const int dev = cfg ? cfg->device : 0;
+  int n = 0;
+  RVC_HIP(hipGetDeviceCount(&n));
+  if (dev < 0 || dev >= n) return set_error(RVC_E_INVALID, "device %d out of range (%d devices)", dev, n);
+  RVC_HIP(hipSetDevice(dev));
+  hipDeviceProp_t prop;
+  RVC_HIP(hipGetDeviceProperties(&prop, dev));
+  g_engine.device = dev;
+  g_engine.n_cu = prop.multiProcessorCount;
+  auto* ht = new (std::nothrow) HostTables;
+  if (!ht) return set_error(RVC_E_INVALID, "out of host memory");
+  build_tables(ht);
+  g_engine.host_tables.reset(ht);
+  void* p = nullptr;
+  const size_t bytes = sizeof(HostTables);
+  RVC_HIP(hipMalloc(&p, bytes));
+  RVC_HIP(hipMemcpy(p, ht, bytes, hipMemcpyHostToDevice));
+  g_engine.dev_tables_mem = p;
+  auto* base = static_cast<HostTables*>(p);
+  g_engine.tables.ph2pr_f = base->ph2pr_f;
+  g_engine.tables.mm_f = base->mm_f;
+  g_engine.tables.ph2pr_d = base->ph2pr_d;
+  g_engine.tables.mm_d = base->mm_d;
+  g_engine.hbm_budget = (cfg && cfg->hbm_budget_bytes) ? cfg->hbm_budget_bytes : (64ull << 30);
+  g_engine.max_batch_regions = (cfg && cfg->max_batch_regions) ? cfg->max_batch_regions : 4096u;
+  g_engine.ready = true;
+  return RVC_OK;
+}
+
+int engine_shutdown() {
+  std::lock_guard<std::mutex> lk(g_engine.mu);
+  if (!g_engine.ready) return set_error(RVC_E_STATE, "engine not initialised");
+  if (g_engine.dev_tables_mem) hipFree(g_engine.dev_tables_mem);
+  g_engine.dev_tables_mem = nullptr;
+  g_engine.ready = false;
+  return RVC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// validation and packing
+// ---------------------------------------------------------------------------
+int validate_region(const rvc_region* r) {
+  if (!r) return set_error(RVC_E_INVALID, "null region");
+  if (r->n_haps == 0 || r->n_reads == 0) return set_error(RVC_E_INVALID, "region without haplotypes or reads");
+  if (!r->hap_off || !r->read_off || !r->q || !r->i || !r->d || !r->c)
+    return set_error(RVC_E_INVALID, "null array in region");
+  if (r->n_haps > (uint32_t)kMaxHaps)
+    return set_error(RVC_E_LIMIT, "%u haplotypes exceed the limit of %d", r->n_haps, kMaxHaps);
+  for (uint32_t h = 0; h < r->n_haps; ++h) {
+    const int32_t l = r->hap_off[h + 1] - r->hap_off[h];
+    if (l < 0) return set_error(RVC_E_INVALID, "haplotype %u has negative length", h);
+    if (l > kMaxHapLen) return set_error(RVC_E_LIMIT, "haplotype %u has %d bases (limit %d)", h, l, kMaxHapLen);
+    if (l > 0 && !r->hap_bases) return set_error(RVC_E_INVALID, "null hap_bases");
+  }
+  for (uint32_t x = 0; x < r->n_reads; ++x) {
+    const int32_t l = r->read_off[x + 1] - r->read_off[x];
+    if (l < 0) return set_error(RVC_E_INVALID, "read %u has negative length", x);
+    if (l > kMaxReadLen) return set_error(RVC_E_LIMIT, "read %u has %d bases (limit %d)", x, l, kMaxReadLen);
+    if (l > 0 && !r->read_bases) return set_error(RVC_E_INVALID, "null read_bases");
+  }
+  return RVC_OK;
+}
+
+namespace {
+inline uint32_t base_code(uint8_t ch, bool* is_n) {
+  switch (ch) {
+    case 'A': *is_n = false; return 0;
+    case 'C': *is_n = false; return 1;
+    case 'G': *is_n = false; return 2;
+    case 'T': *is_n = false; return 3;
+    default: *is_n = true; return 0;  // 'N' and any other byte match every base
+  }
+}
+inline uint32_t align_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
+}  // namespace
+
+void FP32Policy::pick(uint32_t R, int* W, int* RPL) const {
+  if (R <= 256) { *W = 16; *RPL = (int)std::max<uint32_t>(1, (R + 15) / 16); }
+  else if (R <= 512) { *W = 32; *RPL = (int)((R + 31) / 32); }
+  else { *W = 64; *RPL = (int)((R + 63) / 64); }
+  if (R <= 256 && rpl16_override[*RPL] != 0) {
+    *W = rpl16_override[*RPL] >> 8;
+    *RPL = rpl16_override[*RPL] & 0xff;
+  }
+}
+
+// Appends the packed bases of seq[0..n) at base offset off (a multiple of
+// kSeqAlign) of the pack / N-mask word arrays.
+static void pack_seq(const uint8_t* seq, uint32_t n, uint32_t off, std::vector<uint32_t>& pack,
+                     std::vector<uint32_t>& nmask) {
+  for (uint32_t x = 0; x < n; ++x) {
+    bool is_n;
+    const uint32_t code = base_code(seq[x], &is_n);
+    const uint32_t g = off + x;
+    pack[g >> 4] |= code << ((g & 15u) * 2u);
+    if (is_n) nmask[g >> 5] |= 1u << (g & 31u);
+  }
+}
+
+int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Policy& pol, HostPack* hp) {
+  HostPack& P = *hp;
+  P = HostPack();
+  P.n_regions = n_regions;
+  P.region.resize(n_regions);
+  uint64_t n_hap_total = 0, n_read_total = 0, hap_bases = 0, read_bases = 0, pairs = 0;
+  for (uint32_t r = 0; r < n_regions; ++r) {
+    const rvc_region& g = regions[r];
+    RegionLayout& L = P.region[r];
+    L.n_haps = g.n_haps;
+    L.n_reads = g.n_reads;
+    L.hap0 = (uint32_t)n_hap_total;
+    L.read0 = (uint32_t)n_read_total;
+    L.pair0 = pairs;
+    n_hap_total += g.n_haps;
+    n_read_total += g.n_reads;
+    pairs += (uint64_t)g.n_haps * g.n_reads;
+    for (uint32_t h = 0; h < g.n_haps; ++h) hap_bases += align_up((uint32_t)(g.hap_off[h + 1] - g.hap_off[h]), kSeqAlign);
+    for (uint32_t x = 0; x < g.n_reads; ++x)
+      read_bases += align_up((uint32_t)(g.read_off[x + 1] - g.read_off[x]), kSeqAlign);
+  }
+  if (hap_bases > 0xffffffffull - kSeqAlign || read_bases > 0xffffffffull - kSeqAlign || pairs > 0xffffffffull)
+    return set_error(RVC_E_LIMIT, "batch too large for 32-bit offsets");
+  P.n_pairs_slots = pairs;
+  P.hap_off.resize(n_hap_total);
+  P.hap_len.resize(n_hap_total);
+  P.read_off.resize(n_read_total);
+  P.read_len.resize(n_read_total);
+  P.read_perm.resize(n_read_total);
+  const uint32_t hw = (uint32_t)(hap_bases / 16 + 2), hn = (uint32_t)(hap_bases / 32 + 2);
+  const uint32_t rw = (uint32_t)(read_bases / 16 + 2), rn = (uint32_t)(read_bases / 32 + 2);
+  P.hap_pack.assign(hw, 0);
+  P.hap_nmask.assign(hn, 0);
+  P.read_pack.assign(rw, 0);
+  P.read_nmask.assign(rn, 0);
+  P.rq.assign(read_bases + 64, 0);
+  P.ri.assign(read_bases + 64, 0);
+  P.rd.assign(read_bases + 64, 0);
+  P.rc.assign(read_bases + 64, 0);
+  uint32_t hcur = 0, rcur = 0;
+  P.fp64_cap.assign(kNumFp64Bins, 0);
+  std::vector<std::vector<Tile>> bins(kFp32Variants);
+  for (uint32_t r = 0; r < n_regions; ++r) {
+    const rvc_region& g = regions[r];
+    RegionLayout& L = P.region[r];
+    for (uint32_t h = 0; h < g.n_haps; ++h) {
+      const uint32_t len = (uint32_t)(g.hap_off[h + 1] - g.hap_off[h]);
+      P.hap_off[L.hap0 + h] = hcur;
+      P.hap_len[L.hap0 + h] = len;
+      pack_seq(g.hap_bases + g.hap_off[h], len, hcur, P.hap_pack, P.hap_nmask);
+      hcur += align_up(len, kSeqAlign);
+    }
+    // reads of a region in descending length, so that tiles group equal lengths
+    std::vector<uint32_t> order(g.n_reads);
+    for (uint32_t x = 0; x < g.n_reads; ++x) order[x] = x;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+      return (g.read_off[a + 1] - g.read_off[a]) > (g.read_off[b + 1] - g.read_off[b]);
+    });
+    for (uint32_t sx = 0; sx < g.n_reads; ++sx) {
+      const uint32_t x = order[sx];
+      const uint32_t len = (uint32_t)(g.read_off[x + 1] - g.read_off[x]);
+      const uint32_t gi = L.read0 + sx;
+      P.read_perm[gi] = x;
+      P.read_off[gi] = rcur;
+      P.read_len[gi] = len;
+      const int32_t o = g.read_off[x];
+      pack_seq(g.read_bases + o, len, rcur, P.read_pack, P.read_nmask);
+      std::memcpy(&P.rq[rcur], g.q + o, len);
+      std::memcpy(&P.ri[rcur], g.i + o, len);
+      std::memcpy(&P.rd[rcur], g.d + o, len);
+      std::memcpy(&P.rc[rcur], g.c + o, len);
+      rcur += align_up(len, kSeqAlign);
+    }
+    // tiles: one haplotype x up to 64/W reads of similar length
+    uint32_t sx = 0;
+    while (sx < g.n_reads) {
+      const uint32_t R0 = P.read_len[L.read0 + sx];
+      if (R0 == 0) break;  // the rest are empty reads
+      int W, RPL;
+      pol.pick(R0, &W, &RPL);
+      const uint32_t nseg_max = 64u / (uint32_t)W;
+      uint32_t nseg = 0;
+      while (nseg < nseg_max && sx + nseg < g.n_reads && P.read_len[L.read0 + sx + nseg] > 0) ++nseg;
+      const int v = fp32_variant_index(W, RPL);
+      for (uint32_t h = 0; h < g.n_haps; ++h) {
+        const uint32_t hl = P.hap_len[L.hap0 + h];
+        if (hl == 0) continue;
+        Tile t;
+        t.hap = L.hap0 + h;
+        t.read0 = L.read0 + sx;
+        t.out0 = (uint32_t)(L.pair0 + (uint64_t)sx * g.n_haps + h);
+        t.meta = nseg | (g.n_haps << 4);
+        bins[v].push_back(t);
+        for (uint32_t s = 0; s < nseg; ++s) {
+          const uint32_t rl = P.read_len[L.read0 + sx + s];
+          P.fp64_cap[fp64_bin(rl)] += 1;
+          P.n_pairs += 1;
+          P.n_cells += (uint64_t)rl * hl;
+        }
+      }
+      sx += nseg;
+    }
+  }
+  // Longest haplotypes first inside each bin (largest work first).
+  P.tiles.clear();
+  P.bin_start.assign(kFp32Variants + 1, 0);
+  for (int v = 0; v < kFp32Variants; ++v) {
+    auto& b = bins[v];
+    std::stable_sort(b.begin(), b.end(),
+                     [&](const Tile& x, const Tile& y) { return P.hap_len[x.hap] > P.hap_len[y.hap]; });
+    P.bin_start[v] = (uint32_t)P.tiles.size();
+    P.tiles.insert(P.tiles.end(), b.begin(), b.end());
+  }
+  P.bin_start[kFp32Variants] = (uint32_t)P.tiles.size();
+  P.fp64_base.assign(kNumFp64Bins + 1, 0);
+  for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
+  return RVC_OK;
+}
+
+int fp32_variant_index(int W, int RPL) {
+  const int w = W == 16 ? 0 : W == 32 ? 1 : 2;
+  return w * 16 + (RPL - 1);
+}
+void fp32_variant_shape(int v, int* W, int* RPL) {
+  *W = v / 16 == 0 ? 16 : v / 16 == 1 ? 32 : 64;
+  *RPL = v % 16 + 1;
+}
+
+// ---------------------------------------------------------------------------
+// device batch
+// ---------------------------------------------------------------------------
+namespace {
+template <typename T>
+size_t bytes_of(const std::vector<T>& v) { return v.size() * sizeof(T); }
+inline size_t round256(size_t x) { return (x + 255) & ~size_t(255); }
+}  // namespace
+
+DeviceBatch::~DeviceBatch() { release(); }
+
+void DeviceBatch::release() {
+  if (graph_exec) hipGraphExecDestroy(graph_exec);
+  if (graph) hipGraphDestroy(graph);
+  graph_exec = nullptr;
+  graph = nullptr;
+  for (auto& e : ev) if (e) hipEventDestroy(e);
+  for (auto& e : ev) e = nullptr;
+  if (own_stream && stream) hipStreamDestroy(stream);
+  stream = nullptr;
+  if (mem && own_mem) hipFree(mem);
+  mem = nullptr;
+}
+
+int DeviceBatch::upload(const HostPack& P, hipStream_t s, void* reuse_mem, size_t reuse_bytes) {
+  pack = &P;
+  // layout of one device allocation
+  size_t off = 0;
+  auto take = [&](size_t b) { size_t o = off; off += round256(b); return o; };
+  const size_t o_tables = 0;
+  (void)o_tables;
+  const size_t o_hpack = take(bytes_of(P.hap_pack)), o_hn = take(bytes_of(P.hap_nmask));
+  const size_t o_hoff = take(bytes_of(P.hap_off)), o_hlen = take(bytes_of(P.hap_len));
+  const size_t o_rpack = take(bytes_of(P.read_pack)), o_rn = take(bytes_of(P.read_nmask));
+  const size_t o_rq = take(P.rq.size()), o_ri = take(P.ri.size()), o_rd = take(P.rd.size()), o_rc = take(P.rc.size());
+  const size_t o_roff = take(bytes_of(P.read_off)), o_rlen = take(bytes_of(P.read_len));
+  const size_t o_tiles = take(bytes_of(P.tiles) + sizeof(Tile));
+  const size_t o_out = take(P.n_pairs_slots * sizeof(double) + 8);
+  const size_t o_flags = take(P.n_pairs_slots + 8);
+  const size_t o_resc = take((size_t)P.fp64_base[kNumFp64Bins] * sizeof(RescueEntry) + sizeof(RescueEntry));
+  const size_t o_rbase = take(kNumFp64Bins * sizeof(uint32_t));
+  const size_t n_counters = kFp32Variants + kNumFp64Bins + kNumFp64Bins;
+  const size_t o_cnt = take(n_counters * sizeof(uint32_t));
+  bytes = off;
+  if (reuse_mem && reuse_bytes >= bytes) {
+    mem = reuse_mem;
+    own_mem = false;
+  } else {
+    RVC_HIP(hipMalloc(&mem, bytes));
+    own_mem = true;
+  }
+  char* base = static_cast<char*>(mem);
+  auto cp = [&](size_t o, const void* src, size_t b) -> hipError_t {
+    if (b == 0) return hipSuccess;
+    return hipMemcpyAsync(base + o, src, b, hipMemcpyHostToDevice, s);
+  };
+  RVC_HIP(cp(o_hpack, P.hap_pack.data(), bytes_of(P.hap_pack)));
+  RVC_HIP(cp(o_hn, P.hap_nmask.data(), bytes_of(P.hap_nmask)));
+  RVC_HIP(cp(o_hoff, P.hap_off.data(), bytes_of(P.hap_off)));
+  RVC_HIP(cp(o_hlen, P.hap_len.data(), bytes_of(P.hap_len)));
+  RVC_HIP(cp(o_rpack, P.read_pack.data(), bytes_of(P.read_pack)));
+  RVC_HIP(cp(o_rn, P.read_nmask.data(), bytes_of(P.read_nmask)));
+  RVC_HIP(cp(o_rq, P.rq.data(), P.rq.size()));
+  RVC_HIP(cp(o_ri, P.ri.data(), P.ri.size()));
+  RVC_HIP(cp(o_rd, P.rd.data(), P.rd.size()));
+  RVC_HIP(cp(o_rc, P.rc.data(), P.rc.size()));
+  RVC_HIP(cp(o_roff, P.read_off.data(), bytes_of(P.read_off)));
+  RVC_HIP(cp(o_rlen, P.read_len.data(), bytes_of(P.read_len)));
+  RVC_HIP(cp(o_tiles, P.tiles.data(), bytes_of(P.tiles)));
+  RVC_HIP(cp(o_rbase, P.fp64_base.data(), kNumFp64Bins * sizeof(uint32_t)));
+  // results start as NaN with no flags; pairs with an empty side keep that
+  std::vector<double>& nan_fill = const_cast<HostPack&>(P).nan_scratch;
+  nan_fill.assign(P.n_pairs_slots, std::numeric_limits<double>::quiet_NaN());
+  RVC_HIP(cp(o_out, nan_fill.data(), P.n_pairs_slots * sizeof(double)));
+  RVC_HIP(hipMemsetAsync(base + o_flags, 0, P.n_pairs_slots, s));
+  seqs.hap_pack = reinterpret_cast<const uint32_t*>(base + o_hpack);
+  seqs.hap_nmask = reinterpret_cast<const uint32_t*>(base + o_hn);
+  seqs.hap_off = reinterpret_cast<const uint32_t*>(base + o_hoff);
+  seqs.hap_len = reinterpret_cast<const uint32_t*>(base + o_hlen);
+  seqs.read_pack = reinterpret_cast<const uint32_t*>(base + o_rpack);
+  seqs.read_nmask = reinterpret_cast<const uint32_t*>(base + o_rn);
+  seqs.rq = reinterpret_cast<const uint8_t*>(base + o_rq);
+  seqs.ri = reinterpret_cast<const uint8_t*>(base + o_ri);
+  seqs.rd = reinterpret_cast<const uint8_t*>(base + o_rd);
+  seqs.rc = reinterpret_cast<const uint8_t*>(base + o_rc);
+  seqs.read_off = reinterpret_cast<const uint32_t*>(base + o_roff);
+  seqs.read_len = reinterpret_cast<const uint32_t*>(base + o_rlen);
+  tiles = reinterpret_cast<const Tile*>(base + o_tiles);
+  out = reinterpret_cast<double*>(base + o_out);
+  flags = reinterpret_cast<uint8_t*>(base + o_flags);
+  rescue = reinterpret_cast<RescueEntry*>(base + o_resc);
+  rescue_base = reinterpret_cast<const uint32_t*>(base + o_rbase);
+  counters = reinterpret_cast<uint32_t*>(base + o_cnt);
+  n_counters_ = (uint32_t)n_counters;
+  return RVC_OK;
+}
+
+int DeviceBatch::enqueue(hipStream_t s, bool timed) {
+  const HostPack& P = *pack;
+  uint32_t* heads = counters;
+  uint32_t* rescue_count = counters + kFp32Variants + kNumFp64Bins;
+  RVC_HIP(launch_zero_u32(counters, n_counters_, s));
+  if (timed) RVC_HIP(hipEventRecord(ev[0], s));
+  n_fp32_launches = 0;
+  n_fp64_launches = 0;
+  for (int v = 0; v < kFp32Variants; ++v) {
+    const uint32_t n = P.bin_start[v + 1] - P.bin_start[v];
+    if (n == 0) continue;
+    int W, RPL;
+    fp32_variant_shape(v, &W, &RPL);
+    Fp32Args a;
+    a.s = seqs;
+    a.t = g_engine.tables;
+    a.tiles = tiles + P.bin_start[v];
+    a.n_tiles = n;
+    a.head = heads + v;
+    a.out = out;
+    a.flags = flags;
+    a.rescue = rescue;
+    a.rescue_count = rescue_count;
+    a.rescue_base = rescue_base;
+    const int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL));
+    const int grid = (int)std::min<uint64_t>(n, (uint64_t)per_cu * g_engine.n_cu);
+    hipError_t e = launch_fp32(W, RPL, grid, s, a);
+    if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP32 launch W=%d RPL=%d: %s", W, RPL, hipGetErrorString(e));
+    ++n_fp32_launches;
+  }
+  if (timed) RVC_HIP(hipEventRecord(ev[1], s));
+  for (int b = 0; b < kNumFp64Bins; ++b) {
+    if (P.fp64_cap[b] == 0) continue;
+    Fp64Args a;
+    a.s = seqs;
+    a.t = g_engine.tables;
+    a.list = rescue + P.fp64_base[b];
+    a.count = rescue_count + b;
+    a.head = heads + kFp32Variants + b;
+    a.out = out;
+    a.flags = flags;
+    const int per_cu = std::max(1, fp64_blocks_per_cu(b));
+    int W, RPL;
+    fp64_bin_shape(b, &W, &RPL);
+    const uint64_t max_waves = (P.fp64_cap[b] + (64 / W) - 1) / (64 / W);
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(max_waves, (uint64_t)per_cu * g_engine.n_cu));
+    hipError_t e = launch_fp64(b, grid, s, a);
+    if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP64 launch bin %d: %s", b, hipGetErrorString(e));
+    ++n_fp64_launches;
+  }
+  if (timed) RVC_HIP(hipEventRecord(ev[2], s));
+  return RVC_OK;
+}
+
+int DeviceBatch::ensure_events() {
+  for (auto& e : ev)
+    if (!e) RVC_HIP(hipEventCreate(&e));
+  return RVC_OK;
+}
+
+int DeviceBatch::run(bool use_graph_) {
+  int rc = ensure_events();
+  if (rc) return rc;
+  if (use_graph_) {
+    if (!graph_exec) {
+      RVC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue(stream, false);
+      hipGraph_t gr = nullptr;
+      hipError_t e = hipStreamEndCapture(stream, &gr);
+      if (rc) return rc;
+      if (e != hipSuccess) return set_error(RVC_E_DEVICE, "graph capture: %s", hipGetErrorString(e));
+      graph = gr;
+      RVC_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    }
+    RVC_HIP(hipEventRecord(ev[0], stream));
+    RVC_HIP(hipGraphLaunch(graph_exec, stream));
+    RVC_HIP(hipEventRecord(ev[2], stream));
+    RVC_HIP(hipEventSynchronize(ev[2]));
+    float ms = 0;
+    RVC_HIP(hipEventElapsedTime(&ms, ev[0], ev[2]));
+    total_ms = ms;
+    fp32_ms = fp64_ms = -1;
+  } else {
+    rc = enqueue(stream, true);
+    if (rc) return rc;
+    RVC_HIP(hipEventSynchronize(ev[2]));
+    float a = 0, b = 0;
+    RVC_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+    RVC_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+    fp32_ms = a;
+    fp64_ms = b;
+    total_ms = a + b;
+  }
+  // rescued pair count of this run
+  uint32_t cnt[kNumFp64Bins];
+  RVC_HIP(hipMemcpy(cnt, counters + kFp32Variants + kNumFp64Bins, sizeof cnt, hipMemcpyDeviceToHost));
+  n_rescued = 0;
+  for (int b = 0; b < kNumFp64Bins; ++b) n_rescued += cnt[b];
+  return RVC_OK;
+}
+
+int DeviceBatch::download(double* host_out, uint8_t* host_flags, hipStream_t s) {
+  const HostPack& P = *pack;
+  std::vector<double>& tmp = const_cast<HostPack&>(P).nan_scratch;
+  std::vector<uint8_t>& tf = const_cast<HostPack&>(P).flag_scratch;
+  tmp.resize(P.n_pairs_slots);
+  tf.resize(P.n_pairs_slots);
+  RVC_HIP(hipMemcpyAsync(tmp.data(), out, P.n_pairs_slots * sizeof(double), hipMemcpyDeviceToHost, s));
+  RVC_HIP(hipMemcpyAsync(tf.data(), flags, P.n_pairs_slots, hipMemcpyDeviceToHost, s));
+  RVC_HIP(hipStreamSynchronize(s));
+  // device rows are in descending read length; put them back in input order
+  for (uint32_t r = 0; r < P.n_regions; ++r) {
+    const RegionLayout& L = P.region[r];
+    for (uint32_t sx = 0; sx < L.n_reads; ++sx) {
+      const uint32_t x = P.read_perm[L.read0 + sx];
+      const size_t src = L.pair0 + (size_t)sx * L.n_haps, dst = L.pair0 + (size_t)x * L.n_haps;
+      std::memcpy(host_out + dst, tmp.data() + src, L.n_haps * sizeof(double));
+      if (host_flags) std::memcpy(host_flags + dst, tf.data() + src, L.n_haps);
+    }
+  }
+  return RVC_OK;
+}
+
+}  // namespace rvc

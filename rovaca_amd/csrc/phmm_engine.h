@@ -1,0 +1,117 @@
+// Host engine internals shared by the C ABI, the region adapter and the
+// serving loop.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../../include/rovaca_phmm.h"
+#include "phmm_internal.h"
+#include "phmm_tables.h"
+
+namespace rvc {
+
+int set_error(int code, const char* fmt, ...);
+const char* last_error();
+
+struct Engine {
+  std::mutex mu;
+  bool ready = false;
+  int device = 0;
+  int n_cu = 0;
+  std::unique_ptr<HostTables> host_tables;
+  void* dev_tables_mem = nullptr;
+  DevTables tables{};
+  uint64_t hbm_budget = 0;
+  uint32_t max_batch_regions = 0;
+};
+extern Engine g_engine;
+
+int engine_init(const rvc_config* cfg);
+int engine_shutdown();
+int validate_region(const rvc_region* r);
+
+// FP32 kernel variants: W in {16, 32, 64} x RPL in 1..16.
+constexpr int kFp32Variants = 48;
+int fp32_variant_index(int W, int RPL);
+void fp32_variant_shape(int v, int* W, int* RPL);
+int fp32_blocks_per_cu(int W, int RPL);
+int fp64_blocks_per_cu(int bin);
+
+// Chooses the (W, RPL) kernel for a tile whose longest read has R bases.
+// rpl16_override[rpl] (for reads up to 256 bases, indexed by the minimal
+// 16-lane row count) may name another shape as (W << 8) | RPL; it is filled
+// by the tile tuner.
+struct FP32Policy {
+  uint16_t rpl16_override[17] = {};
+  void pick(uint32_t R, int* W, int* RPL) const;
+};
+
+struct RegionLayout {
+  uint32_t n_haps = 0, n_reads = 0;
+  uint32_t hap0 = 0, read0 = 0;  // first global haplotype / read index
+  uint64_t pair0 = 0;            // first result slot
+};
+
+// A batch packed on the host in device format (see DevSeqs).
+struct HostPack {
+  uint32_t n_regions = 0;
+  std::vector<RegionLayout> region;
+  std::vector<uint32_t> hap_pack, hap_nmask, hap_off, hap_len;
+  std::vector<uint32_t> read_pack, read_nmask, read_off, read_len;
+  std::vector<uint8_t> rq, ri, rd, rc;
+  std::vector<uint32_t> read_perm;  // device read slot -> input read index within its region
+  std::vector<Tile> tiles;          // grouped by FP32 variant
+  std::vector<uint32_t> bin_start;  // kFp32Variants + 1
+  std::vector<uint32_t> fp64_cap;   // pairs per FP64 bin (capacity of its rescue list)
+  std::vector<uint32_t> fp64_base;  // kNumFp64Bins + 1
+  uint64_t n_pairs_slots = 0;       // sum of n_haps * n_reads
+  uint64_t n_pairs = 0;             // pairs with two non-empty sides
+  uint64_t n_cells = 0;
+  std::vector<double> nan_scratch;
+  std::vector<uint8_t> flag_scratch;
+};
+
+int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Policy& pol, HostPack* hp);
+
+struct DeviceBatch {
+  const HostPack* pack = nullptr;
+  void* mem = nullptr;
+  bool own_mem = false;
+  size_t bytes = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  DevSeqs seqs{};
+  const Tile* tiles = nullptr;
+  double* out = nullptr;
+  uint8_t* flags = nullptr;
+  RescueEntry* rescue = nullptr;
+  const uint32_t* rescue_base = nullptr;
+  uint32_t* counters = nullptr;
+  uint32_t n_counters_ = 0;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  double fp32_ms = 0, fp64_ms = 0, total_ms = 0;
+  uint64_t n_rescued = 0;
+  uint32_t n_fp32_launches = 0, n_fp64_launches = 0;
+
+  ~DeviceBatch();
+  void release();
+  // Lays the batch out in one device allocation (reuse_mem if it is big
+  // enough) and queues the host-to-device copies on s.
+  int upload(const HostPack& P, hipStream_t s, void* reuse_mem, size_t reuse_bytes);
+  // Queues counter reset, FP32 launches and FP64 rescue launches on s.
+  int enqueue(hipStream_t s, bool timed);
+  int ensure_events();
+  // Runs the launch sequence on 'stream' (optionally as a hipGraph) and waits.
+  int run(bool use_graph);
+  // Copies results back in input read order.
+  int download(double* host_out, uint8_t* host_flags, hipStream_t s);
+};
+
+}  // namespace rvc

@@ -1,0 +1,107 @@
+// Internal types shared by the device kernels and the host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rvc {
+
+// Limits of the reference scheduler (ph/roDV_schedule.cpp:70-73).
+constexpr int kMaxHaps = 4096;
+constexpr int kMaxHapLen = 1536;
+constexpr int kMaxReadLen = 768;
+
+// Every packed sequence starts on a 32-base boundary, so its 2-bit words
+// (16 bases per uint32) and N-mask words (32 bases per uint32) are aligned.
+constexpr uint32_t kSeqAlign = 32;
+// LDS bytes for one staged haplotype: W leading pad bytes (negative columns
+// of the anti-diagonal sweep), up to 1536 bases, W trailing pad bytes.
+constexpr int kHapBuf = kMaxHapLen + 2 * 64;
+
+// Phred -> probability tables on the device (built on the host exactly as
+// ph/roDV_common/context.h:44-60,92-146 builds them).
+struct DevTables {
+  const float* ph2pr_f;   // 128
+  const float* mm_f;      // 128*129/2, index max*(max+1)/2 + min
+  const double* ph2pr_d;
+  const double* mm_d;
+};
+
+// Packed sequences of one batch, all in device memory.
+struct DevSeqs {
+  const uint32_t* hap_pack;   // 2 bits per base, 16 bases per word, A=0 C=1 G=2 T=3
+  const uint32_t* hap_nmask;  // 1 bit per base, set for N (wildcard)
+  const uint32_t* hap_off;    // first base of each haplotype (multiple of kSeqAlign)
+  const uint32_t* hap_len;
+  const uint32_t* read_pack;
+  const uint32_t* read_nmask;
+  const uint8_t* rq;          // per read base: base, insertion, deletion, gap-continuation quality
+  const uint8_t* ri;
+  const uint8_t* rd;
+  const uint8_t* rc;
+  const uint32_t* read_off;
+  const uint32_t* read_len;
+};
+
+// FP32 work unit: one haplotype against up to 64/W consecutive reads.
+// The result of segment s goes to out[out0 + s * stride].
+struct Tile {
+  uint32_t hap;
+  uint32_t read0;
+  uint32_t out0;
+  uint32_t meta;  // bits 0-3: number of segments in use; bits 4-31: stride
+};
+
+// One pair queued for the FP64 recomputation.
+struct RescueEntry {
+  uint32_t hap;
+  uint32_t read;
+  uint32_t out;
+  uint32_t pad;
+};
+
+// FP64 bins: (segment width, rows per lane) chosen from the read length so
+// that no lane holds more than 12 double rows.
+constexpr int kNumFp64Bins = 20;
+__host__ __device__ inline int fp64_bin(uint32_t R) {
+  if (R <= 128) return (int)((R + 15) / 16) - 1;          // W=16, RPL 1..8   -> bins 0..7
+  if (R <= 256) return 8 + (int)((R + 31) / 32) - 5;      // W=32, RPL 5..8   -> bins 8..11
+  return 12 + (int)((R + 63) / 64) - 5;                   // W=64, RPL 5..12  -> bins 12..19
+}
+inline void fp64_bin_shape(int bin, int* W, int* RPL) {
+  if (bin < 8) { *W = 16; *RPL = bin + 1; }
+  else if (bin < 12) { *W = 32; *RPL = bin - 8 + 5; }
+  else { *W = 64; *RPL = bin - 12 + 5; }
+}
+
+struct Fp32Args {
+  DevSeqs s;
+  DevTables t;
+  const Tile* tiles;
+  uint32_t n_tiles;
+  uint32_t* head;            // work-queue counter of this launch
+  double* out;
+  uint8_t* flags;
+  RescueEntry* rescue;       // kNumFp64Bins lists, list b starts at rescue_base[b]
+  uint32_t* rescue_count;    // kNumFp64Bins counters
+  const uint32_t* rescue_base;
+};
+
+struct Fp64Args {
+  DevSeqs s;
+  DevTables t;
+  const RescueEntry* list;   // this bin's list
+  const uint32_t* count;     // this bin's counter
+  uint32_t* head;            // this launch's work-queue counter
+  double* out;
+  uint8_t* flags;
+};
+
+// Kernel launchers (phmm_kernels.hip).
+bool fp32_supported(int W, int RPL);
+hipError_t launch_fp32(int W, int RPL, int grid, hipStream_t stream, const Fp32Args& a);
+hipError_t launch_fp64(int bin, int grid, hipStream_t stream, const Fp64Args& a);
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
+
+}  // namespace rvc

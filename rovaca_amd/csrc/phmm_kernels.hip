@@ -1,0 +1,362 @@
+// PairHMM forward kernels for gfx950 (CDNA4).
+//
+// Layout of the work inside one wavefront (64 lanes): the wave is cut into
+// 64/W segments of W lanes; each segment sweeps one read x haplotype matrix.
+// Lane k of a segment owns read rows k*RPL .. k*RPL+RPL-1 (registers), and at
+// step s it computes column j = s - k of all its rows, so the segment works
+// on a skewed band of W anti-diagonals at once. The three values of a lane's
+// last row (M, I, D) move one lane down per step through DPP (row_shr:1 for
+// W = 16, wave_shr:1 for W = 32/64), which feeds the first row of the next
+// lane one step later; lane 0 of a segment takes the row -1 boundary
+// (M = I = 0, D = C/H) from the DPP 'old' operand. The haplotype is staged in
+// LDS as one code byte per base; lane k reads byte (s - k) each step.
+//
+// Rows past the end of a read are padding rows that carry the sum
+// M[R-1][j] + I[R-1][j] down unchanged (prior 0, I <- M + I), so the result
+// always comes out of the segment's last lane and last row register.
+//
+// Recurrence and boundaries follow Rovaca's kernels:
+//   ph/roDV_inter/pairhmm_inter.cpp:16-102 (per-row parameters), :170-177
+//   (M/I/D update), :230-244 (final-row sum), :256-271 (row -1 = C/H);
+//   ph/roDV_intra/pairhmm_impl.cpp:13-44 (N matches every base).
+// FP32 uses C = 2^120 and hands pairs whose sum is below 1e-28 to the FP64
+// kernel (C = 2^1020): ph/roDV_schedule.cpp:35,702-713; ph/roDV_intra/pairhmm_api.cpp:14-23.
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <utility>
+
+#include "phmm_internal.h"
+
+namespace rvc {
+namespace {
+
+template <typename T>
+struct Num;
+template <>
+struct Num<float> {
+  __device__ static float C() { return 0x1p120f; }
+};
+template <>
+struct Num<double> {
+  __device__ static double C() { return 0x1p1020; }
+};
+
+// Move v one lane down inside each W-lane segment; lane 0 of a segment
+// receives 'old'.
+template <int W>
+__device__ __forceinline__ float lane_down(float v, float old, int k) {
+  if constexpr (W == 16) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x111, 0xf, 0xf, false));
+  } else if constexpr (W == 64) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138, 0xf, 0xf, false));
+  } else {
+    const float t =
+        __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138, 0xf, 0xf, false));
+    return k == 0 ? old : t;
+  }
+}
+
+template <int W>
+__device__ __forceinline__ double lane_down(double v, double old, int k) {
+  const long long vb = __double_as_longlong(v), ob = __double_as_longlong(old);
+  const int vlo = (int)(vb & 0xffffffffLL), vhi = (int)(vb >> 32);
+  const int olo = (int)(ob & 0xffffffffLL), ohi = (int)(ob >> 32);
+  int lo, hi;
+  if constexpr (W == 16) {
+    lo = __builtin_amdgcn_update_dpp(olo, vlo, 0x111, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(ohi, vhi, 0x111, 0xf, 0xf, false);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(olo, vlo, 0x138, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(ohi, vhi, 0x138, 0xf, 0xf, false);
+  }
+  double r = __longlong_as_double(((long long)(unsigned)lo) | ((long long)hi << 32));
+  if constexpr (W == 32) r = (k == 0) ? old : r;
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ const T* ph2pr_tab(const DevTables& t);
+template <>
+__device__ __forceinline__ const float* ph2pr_tab<float>(const DevTables& t) { return t.ph2pr_f; }
+template <>
+__device__ __forceinline__ const double* ph2pr_tab<double>(const DevTables& t) { return t.ph2pr_d; }
+template <typename T>
+__device__ __forceinline__ const T* mm_tab(const DevTables& t);
+template <>
+__device__ __forceinline__ const float* mm_tab<float>(const DevTables& t) { return t.mm_f; }
+template <>
+__device__ __forceinline__ const double* mm_tab<double>(const DevTables& t) { return t.mm_d; }
+
+// Decode haplotype bases [0, H) into LDS codes: bits 0-1 base, bit 2 N.
+__device__ __forceinline__ void stage_hap(const DevSeqs& s, uint32_t hap, uint32_t H, uint8_t* dst, int first,
+                                          int step) {
+  const uint32_t off = s.hap_off[hap];
+  for (uint32_t x = first; x < H; x += step) {
+    const uint32_t g = off + x;
+    const uint32_t b = (s.hap_pack[g >> 4] >> ((g & 15u) * 2u)) & 3u;
+    const uint32_t n = (s.hap_nmask[g >> 5] >> (g & 31u)) & 1u;
+    dst[x] = (uint8_t)(b | (n << 2));
+  }
+}
+
+// One anti-diagonal step of a lane: new column j of all RPL rows from
+// column j-1 (M, I, D) and the row above (d* at j-1, u* at j).
+template <typename T, int RPL>
+__device__ __forceinline__ void sweep_step(const T (&M)[RPL], const T (&I)[RPL], const T (&D)[RPL], T (&nM)[RPL],
+                                           T (&nI)[RPL], T (&nD)[RPL], T dM, T dI, T dD, T uM, T uI, int hc,
+                                           const int (&rcode)[RPL], const T (&pm)[RPL], const T (&px)[RPL],
+                                           const T (&tmm)[RPL], const T (&tgm)[RPL], const T (&tmi)[RPL],
+                                           const T (&tgg)[RPL], const T (&tmd)[RPL]) {
+  const bool hn = (hc & 4) != 0;
+  {
+    const bool mt = (hc == rcode[0]) || hn;
+    const T pr = mt ? pm[0] : px[0];
+    nM[0] = pr * fma(tmm[0], dM, tgm[0] * (dI + dD));
+    nI[0] = fma(tmi[0], uM, tgg[0] * uI);
+    nD[0] = fma(tmd[0], M[0], tgg[0] * D[0]);
+  }
+#pragma unroll
+  for (int r = 1; r < RPL; ++r) {
+    const bool mt = (hc == rcode[r]) || hn;
+    const T pr = mt ? pm[r] : px[r];
+    nM[r] = pr * fma(tmm[r], M[r - 1], tgm[r] * (I[r - 1] + D[r - 1]));
+    nI[r] = fma(tmi[r], nM[r - 1], tgg[r] * nI[r - 1]);
+    nD[r] = fma(tmd[r], M[r], tgg[r] * D[r]);
+  }
+}
+
+// Forward sweep of one pair per segment. hb points at the segment's staged
+// haplotype codes (hb[j] = code of base j). Returns the raw sum S in the
+// segment's last lane; H is this segment's haplotype length, Hmax the
+// largest among the wave's segments.
+template <typename T, int W, int RPL>
+__device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, const uint8_t* hb, int k, uint32_t H,
+                                     uint32_t Hmax, uint32_t read) {
+  const T* ph2pr = ph2pr_tab<T>(tb);
+  const T* mmt = mm_tab<T>(tb);
+  const uint32_t R = s.read_len[read];
+  const uint32_t roff = s.read_off[read];
+
+  // Per-row parameters (ph/roDV_inter/pairhmm_inter.cpp:60-79).
+  T pm[RPL], px[RPL], tmm[RPL], tgm[RPL], tmi[RPL], tgg[RPL], tmd[RPL];
+  int rcode[RPL];
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const uint32_t row = (uint32_t)k * RPL + r;
+    if (row < R) {
+      const uint32_t g = roff + row;
+      const int iq = s.rq[g] & 127, ii = s.ri[g] & 127, id = s.rd[g] & 127, ic = s.rc[g] & 127;
+      const uint32_t base = (s.read_pack[g >> 4] >> ((g & 15u) * 2u)) & 3u;
+      const uint32_t isn = (s.read_nmask[g >> 5] >> (g & 31u)) & 1u;
+      const T dist = ph2pr[iq];
+      pm[r] = (T)1 - dist;
+      px[r] = isn ? pm[r] : dist * ((T)1 / (T)3);
+      const T gc = ph2pr[ic];
+      tgm[r] = (T)1 - gc;
+      tgg[r] = gc;
+      tmi[r] = ph2pr[ii];
+      tmd[r] = ph2pr[id];
+      const int mx = ii > id ? ii : id, mn = ii > id ? id : ii;
+      tmm[r] = mmt[((mx * (mx + 1)) >> 1) + mn];
+      rcode[r] = (int)base;
+    } else {
+      // padding row: M = 0, I carries M + I of the row above, D = 0
+      pm[r] = 0; px[r] = 0; tmm[r] = 0; tgm[r] = 0;
+      tmi[r] = 1; tgg[r] = 1; tmd[r] = 0;
+      rcode[r] = 0;
+    }
+  }
+
+  const T initD = Num<T>::C() / (T)H;
+  const T old_d = initD;  // row -1 deletion value fed to lane 0
+  T Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) { Ma[r] = 0; Ia[r] = 0; Da[r] = 0; }
+  // Row above this lane's first row: at column j (u*) and column j-1 (d*).
+  T uM = 0, uI = 0, uD = (k == 0) ? initD : (T)0;
+  T dM = 0, dI = 0, dD = uD;
+  T S = 0;
+  // Two steps per iteration with ping-pong registers (a -> b -> a), so the
+  // loop carries no register copies. A step past the last column only
+  // touches columns >= H, which nothing reads, so the count is rounded up.
+  const int nsteps = (int)Hmax + W - 1;
+  const int mysteps = (int)H + W - 1;
+  const uint8_t* hp = hb - k;  // hp[s] = code of column s - k
+  for (int st = 0; st < nsteps; st += 2) {
+    const int hc0 = hp[st];
+    const int hc1 = hp[st + 1];
+    sweep_step<T, RPL>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, hc0, rcode, pm, px, tmm, tgm, tmi, tgg, tmd);
+    if (st < mysteps) S += Mb[RPL - 1] + Ib[RPL - 1];
+    const T sM = lane_down<W>(Mb[RPL - 1], (T)0, k);
+    const T sI = lane_down<W>(Ib[RPL - 1], (T)0, k);
+    const T sD = lane_down<W>(Db[RPL - 1], old_d, k);
+    sweep_step<T, RPL>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, hc1, rcode, pm, px, tmm, tgm, tmi, tgg, tmd);
+    if (st + 1 < mysteps) S += Ma[RPL - 1] + Ia[RPL - 1];
+    dM = sM; dI = sI; dD = sD;
+    uM = lane_down<W>(Ma[RPL - 1], (T)0, k);
+    uI = lane_down<W>(Ia[RPL - 1], (T)0, k);
+    uD = lane_down<W>(Da[RPL - 1], old_d, k);
+  }
+  return S;
+}
+
+// FP32 pass: persistent waves pull tiles (one haplotype x up to 64/W reads).
+template <int W, int RPL>
+__global__ __launch_bounds__(64) void k_fp32(Fp32Args a) {
+  __shared__ uint8_t hbuf[kHapBuf];
+  constexpr int NSEG = 64 / W;
+  const int lane = threadIdx.x;
+  const int seg = lane / W, k = lane % W;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(a.head, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= a.n_tiles) break;
+    const Tile tl = a.tiles[t];
+    const int nseg = (int)(tl.meta & 0xfu);
+    const uint32_t stride = tl.meta >> 4;
+    const bool active = seg < nseg;
+    const uint32_t read = tl.read0 + (active ? (uint32_t)seg : 0u);
+    const uint32_t H = a.s.hap_len[tl.hap];
+    __syncthreads();
+    stage_hap(a.s, tl.hap, H, hbuf + 64, lane, 64);
+    __syncthreads();
+    const float S = forward<float, W, RPL>(a.s, a.t, hbuf + 64, k, H, H, read);
+    if (active && k == W - 1) {
+      const uint32_t o = tl.out0 + (uint32_t)seg * stride;
+      if (S >= 1e-28f) {
+        a.out[o] = log10((double)S) - (double)36.1236000061f;
+        a.flags[o] = 1;
+      } else {
+        const int b = fp64_bin(a.s.read_len[read]);
+        const uint32_t idx = atomicAdd(a.rescue_count + b, 1u);
+        RescueEntry e;
+        e.hap = tl.hap; e.read = read; e.out = o; e.pad = 0;
+        a.rescue[a.rescue_base[b] + idx] = e;
+      }
+    }
+    (void)NSEG;
+  }
+}
+
+// FP64 rescue: persistent waves pull 64/W queued pairs at a time.
+template <int W, int RPL>
+__global__ __launch_bounds__(64) void k_fp64(Fp64Args a) {
+  constexpr int NSEG = 64 / W;
+  __shared__ uint8_t hbuf[NSEG][kHapBuf];
+  const int lane = threadIdx.x;
+  const int seg = lane / W, k = lane % W;
+  const uint32_t count = *a.count;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(a.head, (uint32_t)NSEG);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= count) break;
+    const uint32_t e = t + seg;
+    const bool active = e < count;
+    const RescueEntry en = a.list[active ? e : t];
+    const uint32_t H = a.s.hap_len[en.hap];
+    uint32_t Hmax = H;
+#pragma unroll
+    for (int g = 0; g < NSEG; ++g) {
+      const uint32_t h2 = (uint32_t)__shfl((int)H, g * W);
+      Hmax = h2 > Hmax ? h2 : Hmax;
+    }
+    __syncthreads();
+    stage_hap(a.s, en.hap, H, &hbuf[seg][64], k, W);
+    __syncthreads();
+    const double S = forward<double, W, RPL>(a.s, a.t, &hbuf[seg][64], k, H, Hmax, en.read);
+    if (active && k == W - 1) {
+      a.out[en.out] = log10(S) - 307.050595577260822;
+      a.flags[en.out] = 3;
+    }
+  }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+using Fp32Fn = void (*)(Fp32Args);
+using Fp64Fn = void (*)(Fp64Args);
+
+template <int W, int... R>
+constexpr auto fp32_row(std::integer_sequence<int, R...>) {
+  return std::array<Fp32Fn, sizeof...(R)>{{&k_fp32<W, R + 1>...}};
+}
+
+// FP32 variants: W = 16, 32, 64 with 1..16 rows per lane (W = 64 up to 12).
+const std::array<Fp32Fn, 16> kFp32W16 = fp32_row<16>(std::make_integer_sequence<int, 16>{});
+const std::array<Fp32Fn, 16> kFp32W32 = fp32_row<32>(std::make_integer_sequence<int, 16>{});
+const std::array<Fp32Fn, 12> kFp32W64 = fp32_row<64>(std::make_integer_sequence<int, 12>{});
+
+const std::array<Fp64Fn, kNumFp64Bins> kFp64 = {{
+    &k_fp64<16, 1>, &k_fp64<16, 2>, &k_fp64<16, 3>, &k_fp64<16, 4>, &k_fp64<16, 5>, &k_fp64<16, 6>,
+    &k_fp64<16, 7>, &k_fp64<16, 8>, &k_fp64<32, 5>, &k_fp64<32, 6>, &k_fp64<32, 7>, &k_fp64<32, 8>,
+    &k_fp64<64, 5>, &k_fp64<64, 6>, &k_fp64<64, 7>, &k_fp64<64, 8>, &k_fp64<64, 9>, &k_fp64<64, 10>,
+    &k_fp64<64, 11>, &k_fp64<64, 12>}};
+
+Fp32Fn fp32_fn(int W, int RPL) {
+  if (RPL < 1) return nullptr;
+  if (W == 16 && RPL <= 16) return kFp32W16[RPL - 1];
+  if (W == 32 && RPL <= 16) return kFp32W32[RPL - 1];
+  if (W == 64 && RPL <= 12) return kFp32W64[RPL - 1];
+  return nullptr;
+}
+}  // namespace
+
+bool fp32_supported(int W, int RPL) { return fp32_fn(W, RPL) != nullptr; }
+
+hipError_t launch_fp32(int W, int RPL, int grid, hipStream_t stream, const Fp32Args& a) {
+  Fp32Fn f = fp32_fn(W, RPL);
+  if (!f) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(f, dim3(grid), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fp64(int bin, int grid, hipStream_t stream, const Fp64Args& a) {
+  if (bin < 0 || bin >= kNumFp64Bins) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(kFp64[bin], dim3(grid), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace rvc
+
+namespace rvc {
+
+// Resident 64-lane blocks per CU for a kernel (register and LDS limited).
+int fp32_blocks_per_cu(int W, int RPL) {
+  static int cache[3][17] = {};
+  const int w = W == 16 ? 0 : W == 32 ? 1 : 2;
+  if (RPL < 1 || RPL > 16) return 1;
+  if (cache[w][RPL] == 0) {
+    int n = 0;
+    Fp32Fn f = fp32_fn(W, RPL);
+    if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64, 0) != hipSuccess || n < 1) n = 1;
+    cache[w][RPL] = n;
+  }
+  return cache[w][RPL];
+}
+
+int fp64_blocks_per_cu(int bin) {
+  static int cache[kNumFp64Bins] = {};
+  if (bin < 0 || bin >= kNumFp64Bins) return 1;
+  if (cache[bin] == 0) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kFp64[bin], 64, 0) != hipSuccess || n < 1) n = 1;
+    cache[bin] = n;
+  }
+  return cache[bin];
+}
+
+}  // namespace rvc

@@ -1,0 +1,110 @@
+"""The C ABI library: loads, exports every declared symbol, and its host-only
+steps (tables, read preprocessing, post-processing) match the oracle. No
+device work here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+import rovaca_amd as rv
+from conftest import ROOT
+
+
+def declared_functions():
+    names = []
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            names += re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(rvc_\w+)\s*\(", text, re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = rv.lib()
+    names = declared_functions()
+    assert len(names) >= 17
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing
+    assert sorted(rv.EXPORTS) == names
+
+
+def test_tables_bit_exact():
+    pf, mf, pd, md = rv.tables()
+    of, om = oracle.tables_f32()
+    od, odm = oracle.tables_f64()
+    assert np.array_equal(pf.view(np.uint32), of.view(np.uint32))
+    assert np.array_equal(mf.view(np.uint32), om.view(np.uint32))
+    assert np.array_equal(pd.view(np.uint64), od.view(np.uint64))
+    assert np.array_equal(md.view(np.uint64), odm.view(np.uint64))
+
+
+def test_tables_closed_form():
+    pf, mf, pd, md = rv.tables()
+    for q in (0, 10, 30, 45, 127):
+        assert abs(pd[q] - 10 ** (-q / 10)) <= 1e-15 * max(1.0, 10 ** (-q / 10)) + 1e-300
+    # match-to-match for GOP 45/45 is 1 - 2*10^-4.5 up to the 1e-4-step log table
+    assert abs(md[45 * 46 // 2 + 45] - (1 - 2 * 10 ** -4.5)) < 1e-8
+
+
+@pytest.mark.parametrize("pcr", [0, 1, 2, 3])
+def test_preprocess_matches_oracle(pcr):
+    rng = np.random.default_rng(100 + pcr)
+    for trial in range(40):
+        n = int(rng.integers(1, 160))
+        # repeat-rich reads exercise the PCR model
+        unit = np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, int(rng.integers(1, 5)))]
+        bases = np.resize(unit, n).copy()
+        flip = rng.random(n) < 0.2
+        bases[flip] = np.frombuffer(b"ACGTN", np.uint8)[rng.integers(0, 5, int(flip.sum()))]
+        qual = rng.integers(0, 60, n).astype(np.uint8)
+        mq = int(rng.integers(0, 70))
+        min_q = int(rng.integers(0, 30))
+        q, g, c = rv.preprocess_read(bases, qual, mq, min_q, pcr)
+        oq, oi, od, oc = oracle.preprocess_read(bases, qual, mq, min_q, pcr)
+        assert np.array_equal(q, oq) and np.array_equal(g, oi) and np.array_equal(g, od) and np.array_equal(c, oc)
+
+
+def test_preprocess_known_values():
+    # no PCR model: GOP 45, GCP 10; quals capped at MQ then floored to 6 below the threshold
+    q, g, c = rv.preprocess_read(b"ACGTACGT", np.array([10, 20, 30, 40, 50, 60, 5, 17], np.uint8), 40, 18, 0)
+    assert list(q) == [6, 20, 30, 40, 40, 40, 6, 6]
+    assert list(g) == [45] * 8 and list(c) == [10] * 8
+    # a 12-base poly-A run under CONSERVATIVE: repeat length 12 caps GOP at 38, last base keeps 45
+    q, g, c = rv.preprocess_read(b"A" * 12, np.full(12, 30, np.uint8), 60, 0, 3)
+    assert g[-1] == 45 and all(x <= 40 for x in g[:-1])
+
+
+def test_postprocess_matches_oracle():
+    rng = np.random.default_rng(9)
+    for trial in range(30):
+        n, h = int(rng.integers(1, 30)), int(rng.integers(1, 10))
+        rm = rng.uniform(-20, -0.1, (n, h))
+        rm[rng.random((n, h)) < 0.05] = np.nan
+        rl = rng.integers(0, 300, n).astype(np.int32)
+        a, ka = rv.postprocess(rm, rl)
+        b, kb = oracle.postprocess(rm, rl)
+        assert np.array_equal(ka, kb)
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        assert np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
+
+
+def test_postprocess_closed_form():
+    # read 0 is kept (best -1); read 1 (best -9 < -8 for 100 bp) is dropped; floor at best - 4.5
+    rm = np.array([[-1.0, -10.0], [-9.0, -12.0]])
+    out, kept = rv.postprocess(rm, np.array([100, 100], np.int32))
+    assert list(kept) == [0]
+    assert out[0, 0] == -1.0 and out[1, 0] == -5.5
+    assert np.isnan(out[:, 1]).all()
+
+
+def test_calls_fail_cleanly_without_init():
+    reg = {"n_haps": 1, "n_reads": 1, "hap_off": np.array([0, 4], np.int32),
+           "hap_bases": np.frombuffer(b"ACGT", np.uint8).copy(), "read_off": np.array([0, 4], np.int32),
+           "read_bases": np.frombuffer(b"ACGT", np.uint8).copy()}
+    for k in "qidc":
+        reg[k] = np.full(4, 30, np.uint8)
+    with pytest.raises(rv.EngineError) as e:
+        rv.region_sync(reg)
+    assert e.value.code == rv.RVC_E_STATE

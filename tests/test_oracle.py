@@ -1,0 +1,123 @@
+"""The oracle checked against what pins it: Rovaca's known-answer tests and
+closed forms, brute-force path enumeration, an unscaled long-double
+evaluation, and (where it was built) Rovaca's own intra kernel."""
+import numpy as np
+import pytest
+
+import oracle
+from rovaca_amd import workload
+
+
+def test_kats_fp32_route(kats):
+    # ph/test/pairhmm_unittest.cpp:389-402: relative error < 1e-5
+    for hap, rd, q, i, d, c, exp in kats:
+        v = oracle.intra_log10(hap, rd, q, i, d, c)
+        assert abs(v - exp) / abs(exp) < 1e-5
+
+
+def test_kats_fp64(kats):
+    for hap, rd, q, i, d, c, exp in kats:
+        v = oracle.intra_log10(hap, rd, q, i, d, c, use_double=True)
+        assert abs(v - exp) / abs(exp) < 1e-5
+        assert abs(oracle.pair_log10_f64(hap, rd, q, i, d, c) - v) < 1e-12
+
+
+def test_closed_forms():
+    # ph/test/pairhmm_unittest.cpp:108-141
+    q4 = np.full(4, 43, np.uint8)
+    assert abs(oracle.intra_log10(b"ACGT", b"ACGT", q4, q4, q4, q4) - (-0.6022797)) < 1e-5 * 0.6022797
+    q5 = np.full(5, 43, np.uint8)
+    v = oracle.intra_log10(b"ACGTA", b"ACGTA", q5, q5, q5, q5)
+    assert abs(v - (-0.69925308227539062)) < 1e-6
+
+
+def _rand_pair(rng, H, R, alphabet=b"ACGT", nprob=0.0):
+    al = np.frombuffer(alphabet, np.uint8)
+    hap = al[rng.integers(0, len(al), H)].copy()
+    rd = al[rng.integers(0, len(al), R)].copy()
+    rd[rng.random(R) < nprob] = ord("N")
+    hap[rng.random(H) < nprob] = ord("N")
+    q = rng.integers(2, 45, R).astype(np.uint8)
+    i = rng.integers(10, 50, R).astype(np.uint8)
+    d = rng.integers(10, 50, R).astype(np.uint8)
+    c = rng.integers(5, 20, R).astype(np.uint8)
+    return hap, rd, q, i, d, c
+
+
+def test_bruteforce_matches_dp():
+    """Explicit enumeration of every state path (independent of the DP) equals
+    the long-double DP and the FP64 oracle on tiny pairs, with and without N."""
+    rng = np.random.default_rng(7)
+    for trial in range(60):
+        H, R = int(rng.integers(1, 6)), int(rng.integers(1, 6))
+        args = _rand_pair(rng, H, R, nprob=0.2 if trial % 3 == 0 else 0.0)
+        bf = oracle.bruteforce_log10(*args)
+        ld = oracle.pair_log10_ld(*args)
+        f64 = oracle.pair_log10_f64(*args)
+        assert abs(bf - ld) < 1e-12, (H, R)
+        assert abs(f64 - ld) < 1e-10, (H, R)
+
+
+def test_planted_mistake_is_caught():
+    """A transposed operand (insertion and deletion qualities swapped) moves the
+    oracle away from brute force on asymmetric inputs."""
+    rng = np.random.default_rng(3)
+    hap, rd, q, i, d, c = _rand_pair(rng, 4, 4)
+    i[:] = 20
+    d[:] = 40
+    bf = oracle.bruteforce_log10(hap, rd, q, i, d, c)
+    wrong = oracle.pair_log10_ld(hap, rd, q, d, i, c)
+    assert abs(bf - wrong) > 1e-6
+
+
+def test_scaling_constant_independence():
+    """FP64 with C = 2^1020 equals the unscaled long-double evaluation, so the
+    constant cancels exactly as the log10 offset says."""
+    rng = np.random.default_rng(11)
+    for _ in range(10):
+        args = _rand_pair(rng, int(rng.integers(20, 120)), int(rng.integers(10, 90)), nprob=0.01)
+        assert abs(oracle.pair_log10_f64(*args) - oracle.pair_log10_ld(*args)) < 1e-9
+
+
+def test_n_is_wildcard_symmetric():
+    rng = np.random.default_rng(5)
+    hap, rd, q, i, d, c = _rand_pair(rng, 30, 20)
+    rd2 = rd.copy()
+    rd2[5] = ord("N")
+    hap2 = hap.copy()
+    # N in the read matches whatever the haplotype holds: equal to a read whose base 5 equals the aligned hap base
+    v_n = oracle.pair_log10_f64(hap, rd2, q, i, d, c)
+    v_a = [oracle.pair_log10_f64(hap2, np.where(np.arange(20) == 5, b, rd), q, i, d, c) for b in b"ACGT"]
+    assert v_n >= max(v_a) - 1e-12
+
+
+@pytest.mark.skipif(oracle.ref() is None, reason="oracle/_ref (Rovaca's intra kernel) not built")
+def test_against_reference_intra_kernel():
+    """The restatement equals Rovaca's compiled intra kernel: bit for bit on
+    nearly every pair; the FP32 route may differ by a float ulp of log10f where
+    the reference's compiler fused a multiply-add."""
+    rng = np.random.default_rng(13)
+    exact = 0
+    for trial in range(200):
+        H, R = int(rng.integers(1, 300)), int(rng.integers(1, 200))
+        args = _rand_pair(rng, H, R, nprob=0.01 if trial % 2 else 0.0)
+        a, b = oracle.intra_log10(*args), oracle.ref_intra_log10(*args)
+        assert abs(a - b) <= 8e-6 * max(1.0, abs(b) / 64)
+        exact += a == b
+        assert abs(oracle.intra_log10(*args, use_double=True) - oracle.ref_intra_log10(*args, use_double=True)) < 1e-9
+    assert exact >= 190
+
+
+def test_region_routing_and_golden():
+    """Region routing restatement on the single-region workload: every pair is
+    within 1e-5 of the FP64 oracle and of the committed reference fixture."""
+    import os
+    from conftest import GOLDEN
+    g = np.load(os.path.join(GOLDEN, "single_region_seed0.npz"))
+    reg = {k: g[k] for k in ("hap_off", "hap_bases", "read_off", "read_bases", "q", "i", "d", "c")}
+    reg["n_haps"], reg["n_reads"] = len(reg["hap_off"]) - 1, len(reg["read_off"]) - 1
+    out, route, ok = oracle.region_reference(reg)
+    assert ok
+    assert np.abs(out - g["ref_intra"]).max() < 1e-5
+    assert np.abs(out - g["fp64"]).max() < 1e-5
+    assert set(np.unique(route)) <= {1, 2, 3, 4, 5}
