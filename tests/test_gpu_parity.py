@@ -226,7 +226,9 @@ def test_call_pairhmm_region_adapter():
     H, N = reg["n_haps"], reg["n_reads"]
     haps = [reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]] for h in range(H)]
     reads = [reg["read_bases"][reg["read_off"][r]:reg["read_off"][r + 1]] for r in range(N)]
-    quals = [rng.integers(0, 50, len(r)).astype(np.uint8) for r in reads]
+    # the region's own base qualities with some noise; MQ 20-60 caps some of them
+    quals = [np.clip(reg["q"][reg["read_off"][r]:reg["read_off"][r + 1]].astype(int)
+                     + rng.integers(-3, 4, reads[r].size), 0, 60).astype(np.uint8) for r in range(N)]
     mqs = rng.integers(20, 61, N).astype(np.uint8)
     for pcr in (rv.PCR_NONE, rv.PCR_CONSERVATIVE):
         rc, mat, kept = rv.call_pairhmm(haps, reads, quals, mqs, min_quality_threshold=18, pcr=pcr)
