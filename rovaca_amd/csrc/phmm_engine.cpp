@@ -132,7 +132,10 @@ inline uint32_t align_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
 }  // namespace
 
 void FP32Policy::pick(uint32_t R, int* W, int* RPL) const {
-  if (R <= 256) { *W = 16; *RPL = (int)std::max<uint32_t>(1, (R + 15) / 16); }
+  // 16-lane shapes come in steps of 3 rows per lane, so that reads of nearby
+  // lengths share one launch (each launch has a tail of its own)
+  static const int kRpl16[17] = {0, 1, 2, 4, 4, 7, 7, 7, 10, 10, 10, 13, 13, 13, 16, 16, 16};
+  if (R <= 256) { *W = 16; *RPL = kRpl16[std::max<uint32_t>(1, (R + 15) / 16)]; }
   else if (R <= 512) { *W = 32; *RPL = (int)((R + 31) / 32); }
   else { *W = 64; *RPL = (int)((R + 63) / 64); }
   if (R <= 256 && rpl16_override[*RPL] != 0) {
@@ -225,6 +228,8 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
       std::memcpy(&P.ri[rcur], g.i + o, len);
       std::memcpy(&P.rd[rcur], g.d + o, len);
       std::memcpy(&P.rc[rcur], g.c + o, len);
+      for (uint32_t y = 0; y < len; ++y)
+        if ((g.c[o + y] & 127) == 0) P.fold_gap = false;
       rcur += align_up(len, kSeqAlign);
     }
     // tiles: one haplotype x up to 64/W reads of similar length
@@ -403,9 +408,9 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     a.rescue = rescue;
     a.rescue_count = rescue_count;
     a.rescue_base = rescue_base;
-    const int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL));
+    const int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL, P.fold_gap));
     const int grid = (int)std::min<uint64_t>(n, (uint64_t)per_cu * g_engine.n_cu);
-    hipError_t e = launch_fp32(W, RPL, grid, s, a);
+    hipError_t e = launch_fp32(W, RPL, P.fold_gap, grid, s, a);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP32 launch W=%d RPL=%d: %s", W, RPL, hipGetErrorString(e));
     ++n_fp32_launches;
   }

@@ -39,7 +39,7 @@ int validate_region(const rvc_region* r);
 constexpr int kFp32Variants = 48;
 int fp32_variant_index(int W, int RPL);
 void fp32_variant_shape(int v, int* W, int* RPL);
-int fp32_blocks_per_cu(int W, int RPL);
+int fp32_blocks_per_cu(int W, int RPL, bool xf);
 int fp64_blocks_per_cu(int bin);
 
 // Chooses the (W, RPL) kernel for a tile whose longest read has R bases.
@@ -72,6 +72,7 @@ struct HostPack {
   uint64_t n_pairs_slots = 0;       // sum of n_haps * n_reads
   uint64_t n_pairs = 0;             // pairs with two non-empty sides
   uint64_t n_cells = 0;
+  bool fold_gap = true;             // every gap-continuation quality is > 0
   std::vector<double> nan_scratch;
   std::vector<uint8_t> flag_scratch;
 };

@@ -15,9 +15,10 @@ constexpr int kMaxReadLen = 768;
 // Every packed sequence starts on a 32-base boundary, so its 2-bit words
 // (16 bases per uint32) and N-mask words (32 bases per uint32) are aligned.
 constexpr uint32_t kSeqAlign = 32;
-// LDS bytes for one staged haplotype: W leading pad bytes (negative columns
-// of the anti-diagonal sweep), up to 1536 bases, W trailing pad bytes.
-constexpr int kHapBuf = kMaxHapLen + 2 * 64;
+// LDS bytes for one staged haplotype: 64 leading pad bytes (negative columns
+// of the anti-diagonal sweep), up to 1536 bases, 64 + 16 trailing pad bytes
+// (columns past the end, read by the last steps and one step of prefetch).
+constexpr int kHapBuf = kMaxHapLen + 2 * 64 + 16;
 
 // Phred -> probability tables on the device (built on the host exactly as
 // ph/roDV_common/context.h:44-60,92-146 builds them).
@@ -100,7 +101,7 @@ struct Fp64Args {
 
 // Kernel launchers (phmm_kernels.hip).
 bool fp32_supported(int W, int RPL);
-hipError_t launch_fp32(int W, int RPL, int grid, hipStream_t stream, const Fp32Args& a);
+hipError_t launch_fp32(int W, int RPL, bool xf, int grid, hipStream_t stream, const Fp32Args& a);
 hipError_t launch_fp64(int bin, int grid, hipStream_t stream, const Fp64Args& a);
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
 

@@ -104,27 +104,33 @@ __device__ __forceinline__ void stage_hap(const DevSeqs& s, uint32_t hap, uint32
 
 // One anti-diagonal step of a lane: new column j of all RPL rows from
 // column j-1 (M, I, D) and the row above (d* at j-1, u* at j).
-template <typename T, int RPL>
+// XF = false: I and D are the model's values and
+//   M = prior * (tMM*M' + tGM*(I' + D')).
+// XF = true: I and D are stored pre-multiplied by the next row's tGM (see
+// forward()), so M = prior * (tMM*M' + I' + D') and one multiply per cell goes.
+template <typename T, int RPL, bool XF>
 __device__ __forceinline__ void sweep_step(const T (&M)[RPL], const T (&I)[RPL], const T (&D)[RPL], T (&nM)[RPL],
                                            T (&nI)[RPL], T (&nD)[RPL], T dM, T dI, T dD, T uM, T uI, int hc,
                                            const int (&rcode)[RPL], const T (&pm)[RPL], const T (&px)[RPL],
                                            const T (&tmm)[RPL], const T (&tgm)[RPL], const T (&tmi)[RPL],
-                                           const T (&tgg)[RPL], const T (&tmd)[RPL]) {
+                                           const T (&tii)[RPL], const T (&tmd)[RPL], const T (&tdd)[RPL]) {
   const bool hn = (hc & 4) != 0;
   {
     const bool mt = (hc == rcode[0]) || hn;
     const T pr = mt ? pm[0] : px[0];
-    nM[0] = pr * fma(tmm[0], dM, tgm[0] * (dI + dD));
-    nI[0] = fma(tmi[0], uM, tgg[0] * uI);
-    nD[0] = fma(tmd[0], M[0], tgg[0] * D[0]);
+    if constexpr (XF) nM[0] = pr * fma(tmm[0], dM, dI + dD);
+    else nM[0] = pr * fma(tmm[0], dM, tgm[0] * (dI + dD));
+    nI[0] = fma(tmi[0], uM, tii[0] * uI);
+    nD[0] = fma(tmd[0], M[0], tdd[0] * D[0]);
   }
 #pragma unroll
   for (int r = 1; r < RPL; ++r) {
     const bool mt = (hc == rcode[r]) || hn;
     const T pr = mt ? pm[r] : px[r];
-    nM[r] = pr * fma(tmm[r], M[r - 1], tgm[r] * (I[r - 1] + D[r - 1]));
-    nI[r] = fma(tmi[r], nM[r - 1], tgg[r] * nI[r - 1]);
-    nD[r] = fma(tmd[r], M[r], tgg[r] * D[r]);
+    if constexpr (XF) nM[r] = pr * fma(tmm[r], M[r - 1], I[r - 1] + D[r - 1]);
+    else nM[r] = pr * fma(tmm[r], M[r - 1], tgm[r] * (I[r - 1] + D[r - 1]));
+    nI[r] = fma(tmi[r], nM[r - 1], tii[r] * nI[r - 1]);
+    nD[r] = fma(tmd[r], M[r], tdd[r] * D[r]);
   }
 }
 
@@ -132,7 +138,15 @@ __device__ __forceinline__ void sweep_step(const T (&M)[RPL], const T (&I)[RPL],
 // haplotype codes (hb[j] = code of base j). Returns the raw sum S in the
 // segment's last lane; H is this segment's haplotype length, Hmax the
 // largest among the wave's segments.
-template <typename T, int W, int RPL>
+//
+// XF = true stores I'[k] = g[k+1]*I[k] and D'[k] = g[k+1]*D[k] with g the
+// gap-to-match probability tGM of a row (1 past the last row). Then
+//   M[k+1] = prior * (tMM*M[k] + I'[k] + D'[k])
+//   I'[k]  = (g[k+1]*tMI)*M[k-1] + (g[k+1]*tII/g[k])*I'[k-1]
+//   D'[k]  = (g[k+1]*tMD)*M[k][j-1] + tDD*D'[k][j-1]
+// which is the same recurrence with folded coefficients. It needs g > 0 for
+// every row (gap-continuation quality not 0); the host checks that per batch.
+template <typename T, int W, int RPL, bool XF>
 __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, const uint8_t* hb, int k, uint32_t H,
                                      uint32_t Hmax, uint32_t read) {
   const T* ph2pr = ph2pr_tab<T>(tb);
@@ -141,7 +155,7 @@ __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, cons
   const uint32_t roff = s.read_off[read];
 
   // Per-row parameters (ph/roDV_inter/pairhmm_inter.cpp:60-79).
-  T pm[RPL], px[RPL], tmm[RPL], tgm[RPL], tmi[RPL], tgg[RPL], tmd[RPL];
+  T pm[RPL], px[RPL], tmm[RPL], tgm[RPL], tmi[RPL], tii[RPL], tmd[RPL], tdd[RPL];
   int rcode[RPL];
 #pragma unroll
   for (int r = 0; r < RPL; ++r) {
@@ -155,23 +169,35 @@ __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, cons
       pm[r] = (T)1 - dist;
       px[r] = isn ? pm[r] : dist * ((T)1 / (T)3);
       const T gc = ph2pr[ic];
-      tgm[r] = (T)1 - gc;
-      tgg[r] = gc;
-      tmi[r] = ph2pr[ii];
-      tmd[r] = ph2pr[id];
       const int mx = ii > id ? ii : id, mn = ii > id ? id : ii;
       tmm[r] = mmt[((mx * (mx + 1)) >> 1) + mn];
+      tgm[r] = (T)1 - gc;
+      if constexpr (XF) {
+        const T gnext = (row + 1 < R) ? (T)1 - ph2pr[s.rc[g + 1] & 127] : (T)1;
+        tmi[r] = gnext * ph2pr[ii];
+        tii[r] = gnext * gc / tgm[r];
+        tmd[r] = gnext * ph2pr[id];
+        tdd[r] = gc;
+      } else {
+        tmi[r] = ph2pr[ii];
+        tii[r] = gc;
+        tmd[r] = ph2pr[id];
+        tdd[r] = gc;
+      }
       rcode[r] = (int)base;
     } else {
       // padding row: M = 0, I carries M + I of the row above, D = 0
       pm[r] = 0; px[r] = 0; tmm[r] = 0; tgm[r] = 0;
-      tmi[r] = 1; tgg[r] = 1; tmd[r] = 0;
+      tmi[r] = 1; tii[r] = 1; tmd[r] = 0; tdd[r] = 1;
       rcode[r] = 0;
     }
   }
 
-  const T initD = Num<T>::C() / (T)H;
-  const T old_d = initD;  // row -1 deletion value fed to lane 0
+  // Row -1 deletion value C/H, fed to lane 0 (pre-multiplied by row 0's tGM
+  // in the folded form).
+  T initD = Num<T>::C() / (T)H;
+  if constexpr (XF) initD = ((T)1 - ph2pr[s.rc[roff] & 127]) * initD;
+  const T old_d = initD;
   T Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
 #pragma unroll
   for (int r = 0; r < RPL; ++r) { Ma[r] = 0; Ia[r] = 0; Da[r] = 0; }
@@ -182,18 +208,23 @@ __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, cons
   // Two steps per iteration with ping-pong registers (a -> b -> a), so the
   // loop carries no register copies. A step past the last column only
   // touches columns >= H, which nothing reads, so the count is rounded up.
+  // The haplotype codes of the next iteration are read one iteration ahead.
   const int nsteps = (int)Hmax + W - 1;
   const int mysteps = (int)H + W - 1;
   const uint8_t* hp = hb - k;  // hp[s] = code of column s - k
+  int hn0 = hp[0], hn1 = hp[1];
   for (int st = 0; st < nsteps; st += 2) {
-    const int hc0 = hp[st];
-    const int hc1 = hp[st + 1];
-    sweep_step<T, RPL>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, hc0, rcode, pm, px, tmm, tgm, tmi, tgg, tmd);
+    const int hc0 = hn0, hc1 = hn1;
+    hn0 = hp[st + 2];
+    hn1 = hp[st + 3];
+    sweep_step<T, RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, hc0, rcode, pm, px, tmm, tgm, tmi, tii, tmd,
+                           tdd);
     if (st < mysteps) S += Mb[RPL - 1] + Ib[RPL - 1];
     const T sM = lane_down<W>(Mb[RPL - 1], (T)0, k);
     const T sI = lane_down<W>(Ib[RPL - 1], (T)0, k);
     const T sD = lane_down<W>(Db[RPL - 1], old_d, k);
-    sweep_step<T, RPL>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, hc1, rcode, pm, px, tmm, tgm, tmi, tgg, tmd);
+    sweep_step<T, RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, hc1, rcode, pm, px, tmm, tgm, tmi, tii, tmd,
+                           tdd);
     if (st + 1 < mysteps) S += Ma[RPL - 1] + Ia[RPL - 1];
     dM = sM; dI = sI; dD = sD;
     uM = lane_down<W>(Ma[RPL - 1], (T)0, k);
@@ -204,8 +235,9 @@ __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, cons
 }
 
 // FP32 pass: persistent waves pull tiles (one haplotype x up to 64/W reads).
-template <int W, int RPL>
-__global__ __launch_bounds__(64) void k_fp32(Fp32Args a) {
+// Two waves per SIMD at least: the register budget is then 256 per lane.
+template <int W, int RPL, bool XF>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_fp32(Fp32Args a) {
   __shared__ uint8_t hbuf[kHapBuf];
   constexpr int NSEG = 64 / W;
   const int lane = threadIdx.x;
@@ -224,7 +256,7 @@ __global__ __launch_bounds__(64) void k_fp32(Fp32Args a) {
     __syncthreads();
     stage_hap(a.s, tl.hap, H, hbuf + 64, lane, 64);
     __syncthreads();
-    const float S = forward<float, W, RPL>(a.s, a.t, hbuf + 64, k, H, H, read);
+    const float S = forward<float, W, RPL, XF>(a.s, a.t, hbuf + 64, k, H, H, read);
     if (active && k == W - 1) {
       const uint32_t o = tl.out0 + (uint32_t)seg * stride;
       if (S >= 1e-28f) {
@@ -268,7 +300,7 @@ __global__ __launch_bounds__(64) void k_fp64(Fp64Args a) {
     __syncthreads();
     stage_hap(a.s, en.hap, H, &hbuf[seg][64], k, W);
     __syncthreads();
-    const double S = forward<double, W, RPL>(a.s, a.t, &hbuf[seg][64], k, H, Hmax, en.read);
+    const double S = forward<double, W, RPL, false>(a.s, a.t, &hbuf[seg][64], k, H, Hmax, en.read);
     if (active && k == W - 1) {
       a.out[en.out] = log10(S) - 307.050595577260822;
       a.flags[en.out] = 3;
@@ -284,15 +316,19 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 using Fp32Fn = void (*)(Fp32Args);
 using Fp64Fn = void (*)(Fp64Args);
 
-template <int W, int... R>
+template <int W, bool XF, int... R>
 constexpr auto fp32_row(std::integer_sequence<int, R...>) {
-  return std::array<Fp32Fn, sizeof...(R)>{{&k_fp32<W, R + 1>...}};
+  return std::array<Fp32Fn, sizeof...(R)>{{&k_fp32<W, R + 1, XF>...}};
 }
 
-// FP32 variants: W = 16, 32, 64 with 1..16 rows per lane (W = 64 up to 12).
-const std::array<Fp32Fn, 16> kFp32W16 = fp32_row<16>(std::make_integer_sequence<int, 16>{});
-const std::array<Fp32Fn, 16> kFp32W32 = fp32_row<32>(std::make_integer_sequence<int, 16>{});
-const std::array<Fp32Fn, 12> kFp32W64 = fp32_row<64>(std::make_integer_sequence<int, 12>{});
+// FP32 variants: W = 16, 32, 64 with 1..16 rows per lane (W = 64 up to 12),
+// with and without the folded gap-to-match factor.
+const std::array<Fp32Fn, 16> kFp32W16[2] = {fp32_row<16, false>(std::make_integer_sequence<int, 16>{}),
+                                            fp32_row<16, true>(std::make_integer_sequence<int, 16>{})};
+const std::array<Fp32Fn, 16> kFp32W32[2] = {fp32_row<32, false>(std::make_integer_sequence<int, 16>{}),
+                                            fp32_row<32, true>(std::make_integer_sequence<int, 16>{})};
+const std::array<Fp32Fn, 12> kFp32W64[2] = {fp32_row<64, false>(std::make_integer_sequence<int, 12>{}),
+                                            fp32_row<64, true>(std::make_integer_sequence<int, 12>{})};
 
 const std::array<Fp64Fn, kNumFp64Bins> kFp64 = {{
     &k_fp64<16, 1>, &k_fp64<16, 2>, &k_fp64<16, 3>, &k_fp64<16, 4>, &k_fp64<16, 5>, &k_fp64<16, 6>,
@@ -300,19 +336,19 @@ const std::array<Fp64Fn, kNumFp64Bins> kFp64 = {{
     &k_fp64<64, 5>, &k_fp64<64, 6>, &k_fp64<64, 7>, &k_fp64<64, 8>, &k_fp64<64, 9>, &k_fp64<64, 10>,
     &k_fp64<64, 11>, &k_fp64<64, 12>}};
 
-Fp32Fn fp32_fn(int W, int RPL) {
+Fp32Fn fp32_fn(int W, int RPL, bool xf) {
   if (RPL < 1) return nullptr;
-  if (W == 16 && RPL <= 16) return kFp32W16[RPL - 1];
-  if (W == 32 && RPL <= 16) return kFp32W32[RPL - 1];
-  if (W == 64 && RPL <= 12) return kFp32W64[RPL - 1];
+  if (W == 16 && RPL <= 16) return kFp32W16[xf][RPL - 1];
+  if (W == 32 && RPL <= 16) return kFp32W32[xf][RPL - 1];
+  if (W == 64 && RPL <= 12) return kFp32W64[xf][RPL - 1];
   return nullptr;
 }
 }  // namespace
 
-bool fp32_supported(int W, int RPL) { return fp32_fn(W, RPL) != nullptr; }
+bool fp32_supported(int W, int RPL) { return fp32_fn(W, RPL, false) != nullptr; }
 
-hipError_t launch_fp32(int W, int RPL, int grid, hipStream_t stream, const Fp32Args& a) {
-  Fp32Fn f = fp32_fn(W, RPL);
+hipError_t launch_fp32(int W, int RPL, bool xf, int grid, hipStream_t stream, const Fp32Args& a) {
+  Fp32Fn f = fp32_fn(W, RPL, xf);
   if (!f) return hipErrorInvalidValue;
   hipLaunchKernelGGL(f, dim3(grid), dim3(64), 0, stream, a);
   return hipGetLastError();
@@ -335,17 +371,17 @@ hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream) {
 namespace rvc {
 
 // Resident 64-lane blocks per CU for a kernel (register and LDS limited).
-int fp32_blocks_per_cu(int W, int RPL) {
-  static int cache[3][17] = {};
+int fp32_blocks_per_cu(int W, int RPL, bool xf) {
+  static int cache[2][3][17] = {};
   const int w = W == 16 ? 0 : W == 32 ? 1 : 2;
   if (RPL < 1 || RPL > 16) return 1;
-  if (cache[w][RPL] == 0) {
+  if (cache[xf][w][RPL] == 0) {
     int n = 0;
-    Fp32Fn f = fp32_fn(W, RPL);
+    Fp32Fn f = fp32_fn(W, RPL, xf);
     if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64, 0) != hipSuccess || n < 1) n = 1;
-    cache[w][RPL] = n;
+    cache[xf][w][RPL] = n;
   }
-  return cache[w][RPL];
+  return cache[xf][w][RPL];
 }
 
 int fp64_blocks_per_cu(int bin) {
