@@ -122,12 +122,99 @@ def plain_gpu(regions, n):
             "mean_region_ms": secs / len(sample) * 1e3}
 
 
+def run_single(args):
+    """One 8 x 32 region through the plain form, repeated: latency and GCUPS."""
+    import rovaca_amd as rv
+    from rovaca_amd import workload
+    reg = workload.single_region(0)
+    cells = workload.cells([reg])
+    rv.init(0)
+    for _ in range(max(1, args.warmup)):
+        rv.region_sync(reg)
+    lat = []
+    for _ in range(max(20, args.steps)):
+        t0 = time.perf_counter()
+        rv.region_sync(reg)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    rv.shutdown()
+    lat = np.array(lat)
+    res = {"metric": "PairHMM single-region latency (8 haplotypes x 32 reads, 1 MI355X)",
+           "value": round(float(np.median(lat)), 4), "unit": "ms", "higher_is_better": False, "n_gpus": 1,
+           "steps": len(lat), "warmup": args.warmup, "dtype": "fp32 (fp64 rescue)", "data": "synthetic",
+           "config": {"workload": "single_region_8x32", "cells": cells},
+           "gcups_at_median": round(cells / (np.median(lat) * 1e-3) / 1e9, 2),
+           "p99_ms": round(float(np.percentile(lat, 99)), 4)}
+    if not args.no_cpu:
+        import oracle
+        oracle.baseline_run([reg], 1)
+        secs = min(oracle.baseline_run([reg], 1)[0] for _ in range(5))
+        res["cpu_baseline"] = {"value": round(secs * 1e3, 4), "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": "the same region, one worker thread, best of 5"}
+    return res
+
+
+def run_served(args):
+    """Served stream: Poisson arrivals from 32 producer threads at each rate;
+    'deep' mixes 1 % regions of 2000 reads x 128 haplotypes (1000 bp)."""
+    import rovaca_amd as rv
+    from rovaca_amd import workload
+    t = time.time()
+    pool = workload.batched_regions(min(args.regions, 2048), seed=3)
+    deep = []
+    if args.workload == "deep":
+        deep = [workload.deep_region(seed=40 + x) for x in range(4)]
+    regions = pool + deep
+    log(f"pool of {len(pool)} regions (+{len(deep)} deep) generated in {time.time() - t:.1f} s")
+    rv.init(0, server=True, hbm_budget_bytes=64 << 30, max_batch_regions=4096)
+    rng = np.random.default_rng(9)
+    rates = [float(x) for x in args.rates.split(",")] if args.workload == "served" else [2000.0]
+    # warm-up: a short stream at the lowest rate
+    arr = np.cumsum(rng.exponential(1.0 / rates[0], 200))
+    rv.loadgen_run(regions, arr, np.arange(200) % len(pool), args.producers)
+    out = []
+    for rate in rates:
+        n = max(64, int(rate * args.duration))
+        arr = np.cumsum(rng.exponential(1.0 / rate, n))
+        order = rng.integers(0, len(pool), n)
+        if deep:
+            is_deep = rng.random(n) < 0.01
+            order[is_deep] = len(pool) + rng.integers(0, len(deep), int(is_deep.sum()))
+        res, lat = rv.loadgen_run(regions, arr, order, args.producers)
+        rec = {"rate_per_s": rate, "regions": int(res["completed"]),
+               "gcups": round(res["cells"] / res["wall_s"] / 1e9, 1),
+               "offered_gcups": round(res["cells"] / arr[-1] / 1e9, 1),
+               "p50_ms": round(res["lat_p50_ms"], 3), "p99_ms": round(res["lat_p99_ms"], 3),
+               "max_ms": round(res["lat_max_ms"], 3), "queue_p50_ms": round(res["queue_p50_ms"], 3),
+               "queue_p99_ms": round(res["queue_p99_ms"], 3), "submit_late_p99_ms": round(res["late_p99_ms"], 3),
+               "wall_s": round(res["wall_s"], 3)}
+        if deep:
+            dl = lat[order >= len(pool)]
+            rec["deep_regions"] = int((order >= len(pool)).sum())
+            rec["deep_p50_ms"] = round(float(np.median(dl)), 3) if len(dl) else None
+        out.append(rec)
+        log(f"rate {rate:.0f}/s: {rec['gcups']} GCUPS, p50 {rec['p50_ms']} ms, p99 {rec['p99_ms']} ms")
+    st = rv.server_stats()
+    rv.shutdown()
+    top = out[-1]
+    name = "served" if args.workload == "served" else "served_with_1pct_deep_regions"
+    return {"metric": f"PairHMM {name} GCUPS + p99 region latency (1 MI355X)", "value": top["gcups"],
+            "unit": "GCUPS", "higher_is_better": True, "n_gpus": 1, "dtype": "fp32 (fp64 rescue)",
+            "data": "synthetic", "config": {"workload": name, "pool_regions": len(pool), "deep_regions": len(deep),
+                                            "producers": args.producers, "hbm_budget_gib": 64,
+                                            "arrivals": "poisson", "duration_s_per_rate": args.duration},
+            "rates": out, "server": {"launches": st["launches"], "peak_resident_gib": round(st["peak_resident_bytes"] / 2**30, 3),
+                                     "rescued_pairs": st["rescued_pairs"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batched", choices=["batched", "stress", "single"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "stress", "single", "served", "deep"])
+    ap.add_argument("--rates", default="2000,8000,32000", help="served: arrival rates, regions/s")
+    ap.add_argument("--duration", type=float, default=1.0, help="served: seconds of arrivals per rate")
+    ap.add_argument("--producers", type=int, default=32)
     ap.add_argument("--regions", type=int, default=4096)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -154,6 +241,13 @@ def main():
 
     import rovaca_amd as rv
     from rovaca_amd import workload
+
+    if args.workload in ("served", "deep"):
+        print(json.dumps(run_served(args)), flush=True)
+        return
+    if args.workload == "single":
+        print(json.dumps(run_single(args)), flush=True)
+        return
 
     t = time.time()
     regions, wname = make_workload(args.workload, args.regions)

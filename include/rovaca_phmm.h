@@ -128,9 +128,13 @@ int rvc_phmm_call_pairhmm(const rvc_raw_region* region, int32_t min_quality_thre
 typedef struct rvc_batch rvc_batch;
 
 typedef struct rvc_batch_opts {
-  uint32_t persistent_waves_per_cu; /* 0 = tuned default                              */
+  uint32_t persistent_waves_per_cu; /* 0 = as many as fit                             */
   uint32_t use_graph;               /* 1 = capture the launch sequence in a hipGraph  */
-  uint32_t reserved[6];
+  uint32_t fp32_fold;               /* 0 = tuned choice, 1 = fold the gap-to-match    */
+                                    /* factor into I/D where allowed, 2 = never       */
+  uint32_t rpl_round;               /* 0 = tuned choice, 1 = exact rows per lane,     */
+                                    /* 2 = rows per lane in steps of 3                */
+  uint32_t reserved[4];
 } rvc_batch_opts;
 
 typedef struct rvc_batch_info {
@@ -166,10 +170,20 @@ int rvc_phmm_batch_destroy(rvc_batch* b);
  * whatever else is waiting, and completes each ticket on its own. */
 typedef uint64_t rvc_ticket;
 
+/* Steady-clock nanoseconds of a request's life: submitted, launched in a
+ * batch, results ready on the host. */
+typedef struct rvc_ticket_timing {
+  uint64_t submit_ns;
+  uint64_t launch_ns;
+  uint64_t done_ns;
+} rvc_ticket_timing;
+
 int rvc_phmm_submit(const rvc_region* region, rvc_ticket* ticket);
 /* Blocks until the ticket completes (timeout_ms < 0: no limit). Copies the
- * read-major matrix and flags (optional) out and releases the ticket. */
-int rvc_phmm_wait(rvc_ticket ticket, double* out_read_major, uint8_t* out_flags, int64_t timeout_ms);
+ * read-major matrix and flags out (each optional), fills timing (optional)
+ * and releases the ticket. */
+int rvc_phmm_wait(rvc_ticket ticket, double* out_read_major, uint8_t* out_flags, int64_t timeout_ms,
+                  rvc_ticket_timing* timing);
 /* 1 if done, 0 if not yet, negative on a bad ticket. */
 int rvc_phmm_poll(rvc_ticket ticket);
 

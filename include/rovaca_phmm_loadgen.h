@@ -1,0 +1,42 @@
+/*
+ * rovaca_phmm_loadgen.h — load generator for the served form.
+ *
+ * No counterpart in Rovaca: it stands in for Rovaca's worker threads (the
+ * boost::asio pool that calls call_pairhmm once per active region,
+ * hc/haplotypecaller_engine.cpp:119,254) so the serving loop can be measured
+ * under an arrival process. Used by the benchmark and the tests.
+ */
+#ifndef ROVACA_PHMM_LOADGEN_H_
+#define ROVACA_PHMM_LOADGEN_H_
+
+#include <stdint.h>
+
+#include "rovaca_phmm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rvc_loadgen_result {
+  uint64_t submitted, completed, failed;
+  uint64_t cells;          /* sum of read_len * hap_len over completed regions            */
+  double wall_s;           /* first submission to last completion                          */
+  double lat_p50_ms, lat_p99_ms, lat_max_ms, lat_mean_ms; /* submit -> results ready      */
+  double queue_p50_ms, queue_p99_ms;                      /* submit -> launched           */
+  double late_p99_ms;      /* how late producers submitted against the schedule           */
+} rvc_loadgen_result;
+
+/* Submits region order[i] (or i, if order is NULL) at arrival_s[i] seconds
+ * after the start, for i in [0, n_arrivals), from n_producers threads
+ * (arrival i goes to producer i % n_producers); then waits for every ticket.
+ * The serving loop must be running (rvc_config.server_threads = 1).
+ * latency_ms (optional, n_arrivals entries) receives each request's
+ * submit -> results-ready time. */
+int rvc_loadgen_run(const rvc_region* regions, uint32_t n_regions, const uint32_t* order, const double* arrival_s,
+                    uint32_t n_arrivals, uint32_t n_producers, double* latency_ms, rvc_loadgen_result* result);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ROVACA_PHMM_LOADGEN_H_ */

@@ -32,6 +32,7 @@ EXPORTS = [
     "rvc_phmm_call_pairhmm", "rvc_phmm_batch_create", "rvc_phmm_batch_run", "rvc_phmm_batch_results",
     "rvc_phmm_batch_info", "rvc_phmm_batch_destroy", "rvc_phmm_submit", "rvc_phmm_wait", "rvc_phmm_poll",
     "rvc_phmm_server_stats", "rvc_phmm_preprocess_read", "rvc_phmm_postprocess", "rvc_phmm_tables",
+    "rvc_loadgen_run",
 ]
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -60,7 +61,7 @@ class RvcConfig(ctypes.Structure):
 
 class RvcBatchOpts(ctypes.Structure):
     _fields_ = [("persistent_waves_per_cu", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 6)]
+                ("fp32_fold", ctypes.c_uint32), ("rpl_round", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
 
 
 class RvcBatchInfo(ctypes.Structure):
@@ -75,6 +76,14 @@ class RvcServerStats(ctypes.Structure):
                 ("cells", ctypes.c_uint64), ("rescued_pairs", ctypes.c_uint64),
                 ("resident_bytes", ctypes.c_uint64), ("peak_resident_bytes", ctypes.c_uint64),
                 ("busy_ms", ctypes.c_double)]
+
+
+class RvcLoadgenResult(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("cells", ctypes.c_uint64), ("wall_s", ctypes.c_double), ("lat_p50_ms", ctypes.c_double),
+                ("lat_p99_ms", ctypes.c_double), ("lat_max_ms", ctypes.c_double), ("lat_mean_ms", ctypes.c_double),
+                ("queue_p50_ms", ctypes.c_double), ("queue_p99_ms", ctypes.c_double),
+                ("late_p99_ms", ctypes.c_double)]
 
 
 class EngineError(RuntimeError):
@@ -105,12 +114,14 @@ def lib():
         L.rvc_phmm_batch_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(RvcBatchInfo)]
         L.rvc_phmm_batch_destroy.argtypes = [ctypes.c_void_p]
         L.rvc_phmm_submit.argtypes = [ctypes.POINTER(RvcRegion), ctypes.POINTER(ctypes.c_uint64)]
-        L.rvc_phmm_wait.argtypes = [ctypes.c_uint64, _f64p, _u8p, ctypes.c_int64]
+        L.rvc_phmm_wait.argtypes = [ctypes.c_uint64, _f64p, _u8p, ctypes.c_int64, ctypes.c_void_p]
         L.rvc_phmm_poll.argtypes = [ctypes.c_uint64]
         L.rvc_phmm_server_stats.argtypes = [ctypes.POINTER(RvcServerStats)]
         L.rvc_phmm_preprocess_read.argtypes = [_u8p, _u8p, ctypes.c_int32, ctypes.c_uint8, ctypes.c_int32,
                                                ctypes.c_int32, _u8p, _u8p, _u8p]
         L.rvc_phmm_postprocess.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _f64p, _i32p, _f64p, _u32p]
+        L.rvc_loadgen_run.argtypes = [ctypes.POINTER(RvcRegion), ctypes.c_uint32, _u32p, _f64p, ctypes.c_uint32,
+                                      ctypes.c_uint32, _f64p, ctypes.POINTER(RvcLoadgenResult)]
         L.rvc_phmm_tables.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _f64p,
                                       _f64p]
         _lib = L
@@ -190,7 +201,7 @@ def call_pairhmm(haps, reads, quals, mqs, min_quality_threshold=0, pcr=PCR_NONE)
 class Batch:
     """Many regions packed, uploaded and launched together."""
 
-    def __init__(self, regions, use_graph=False):
+    def __init__(self, regions, use_graph=False, fp32_fold=0, rpl_round=0, waves_per_cu=0):
         self._regions = regions
         arr = (RvcRegion * len(regions))()
         for x, r in enumerate(regions):
@@ -198,6 +209,9 @@ class Batch:
         self._arr = arr
         opts = RvcBatchOpts()
         opts.use_graph = 1 if use_graph else 0
+        opts.fp32_fold = fp32_fold
+        opts.rpl_round = rpl_round
+        opts.persistent_waves_per_cu = waves_per_cu
         h = ctypes.c_void_p()
         _check(lib().rvc_phmm_batch_create(arr, len(regions), ctypes.byref(opts), ctypes.byref(h)))
         self.h = h
@@ -246,7 +260,7 @@ def wait(ticket, region, timeout_ms=-1):
     nr, nh = int(region["n_reads"]), int(region["n_haps"])
     out = np.empty(nr * nh, np.float64)
     flags = np.zeros(nr * nh, np.uint8)
-    _check(lib().rvc_phmm_wait(ticket, _ptr(out, _f64p), _ptr(flags), timeout_ms))
+    _check(lib().rvc_phmm_wait(ticket, _ptr(out, _f64p), _ptr(flags), timeout_ms, None))
     return out.reshape(nr, nh), flags.reshape(nr, nh)
 
 
@@ -284,6 +298,23 @@ def tables():
     fp = ctypes.POINTER(ctypes.c_float)
     _check(lib().rvc_phmm_tables(_ptr(pf, fp), _ptr(mf, fp), _ptr(pd, _f64p), _ptr(md, _f64p)))
     return pf, mf, pd, md
+
+
+def loadgen_run(regions, arrival_s, order=None, n_producers=32):
+    """Submits regions at the given arrival times from n_producers threads (C++)
+    and waits for all of them. Returns (summary dict, per-request latency ms)."""
+    arr = (RvcRegion * len(regions))()
+    for x, r in enumerate(regions):
+        arr[x] = make_c_region(r)
+    arrival = np.ascontiguousarray(arrival_s, np.float64)
+    n = len(arrival)
+    o = None if order is None else np.ascontiguousarray(order, np.uint32)
+    lat = np.zeros(n, np.float64)
+    res = RvcLoadgenResult()
+    rc = lib().rvc_loadgen_run(arr, len(regions), _ptr(o, _u32p) if o is not None else None, _ptr(arrival, _f64p),
+                               n, n_producers, _ptr(lat, _f64p), ctypes.byref(res))
+    _check(rc)
+    return {k: getattr(res, k) for k, _ in RvcLoadgenResult._fields_}, lat
 
 
 def server_stats():

@@ -189,7 +189,15 @@ int rvc_phmm_batch_create(const rvc_region* regions, uint32_t n_regions, const r
   auto* b = new (std::nothrow) rvc_batch;
   if (!b) return set_error(RVC_E_INVALID, "out of host memory");
   std::unique_ptr<rvc_batch> guard(b);
-  int rc = build_host_pack(regions, n_regions, policy(), &b->pack);
+  FP32Policy pol = policy();
+  if (opts) {
+    if (opts->fp32_fold == 1) pol.fold = true;
+    if (opts->fp32_fold == 2) pol.fold = false;
+    if (opts->rpl_round == 1) pol.rpl_steps_of_3 = false;
+    if (opts->rpl_round == 2) pol.rpl_steps_of_3 = true;
+    pol.waves_per_cu = opts->persistent_waves_per_cu;
+  }
+  int rc = build_host_pack(regions, n_regions, pol, &b->pack);
   if (rc) return rc;
   if (hipSetDevice(g_engine.device) != hipSuccess || hipStreamCreateWithFlags(&b->dev.stream, hipStreamNonBlocking) != hipSuccess)
     return set_error(RVC_E_DEVICE, "batch stream");

@@ -135,12 +135,13 @@ void FP32Policy::pick(uint32_t R, int* W, int* RPL) const {
   // 16-lane shapes come in steps of 3 rows per lane, so that reads of nearby
   // lengths share one launch (each launch has a tail of its own)
   static const int kRpl16[17] = {0, 1, 2, 4, 4, 7, 7, 7, 10, 10, 10, 13, 13, 13, 16, 16, 16};
-  if (R <= 256) { *W = 16; *RPL = kRpl16[std::max<uint32_t>(1, (R + 15) / 16)]; }
+  const int exact = (int)std::max<uint32_t>(1, (R + 15) / 16);
+  if (R <= 256) { *W = 16; *RPL = rpl_steps_of_3 ? kRpl16[exact] : exact; }
   else if (R <= 512) { *W = 32; *RPL = (int)((R + 31) / 32); }
   else { *W = 64; *RPL = (int)((R + 63) / 64); }
-  if (R <= 256 && rpl16_override[*RPL] != 0) {
-    *W = rpl16_override[*RPL] >> 8;
-    *RPL = rpl16_override[*RPL] & 0xff;
+  if (R <= 256 && rpl16_override[exact] != 0) {
+    *W = rpl16_override[exact] >> 8;
+    *RPL = rpl16_override[exact] & 0xff;
   }
 }
 
@@ -155,6 +156,24 @@ static void pack_seq(const uint8_t* seq, uint32_t n, uint32_t off, std::vector<u
     pack[g >> 4] |= code << ((g & 15u) * 2u);
     if (is_n) nmask[g >> 5] |= 1u << (g & 31u);
   }
+}
+
+// Appends tiles to dst ordered by haplotype length, longest first (a stable
+// counting sort: lengths are at most kMaxHapLen).
+static void append_longest_first(const std::vector<Tile>& tiles, const std::vector<uint32_t>& hap_len,
+                                 std::vector<Tile>& dst) {
+  if (tiles.empty()) return;
+  std::vector<uint32_t> count(kMaxHapLen + 2, 0);
+  for (const Tile& t : tiles) count[kMaxHapLen - hap_len[t.hap]]++;
+  uint32_t acc = 0;
+  for (auto& c : count) {
+    const uint32_t n = c;
+    c = acc;
+    acc += n;
+  }
+  const size_t base = dst.size();
+  dst.resize(base + tiles.size());
+  for (const Tile& t : tiles) dst[base + count[kMaxHapLen - hap_len[t.hap]]++] = t;
 }
 
 int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Policy& pol, HostPack* hp) {
@@ -198,6 +217,7 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
   P.rc.assign(read_bases + 64, 0);
   uint32_t hcur = 0, rcur = 0;
   P.fp64_cap.assign(kNumFp64Bins, 0);
+  P.waves_per_cu = pol.waves_per_cu;
   std::vector<std::vector<Tile>> bins(kFp32Variants);
   for (uint32_t r = 0; r < n_regions; ++r) {
     const rvc_region& g = regions[r];
@@ -262,15 +282,106 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
       sx += nseg;
     }
   }
-  // Longest haplotypes first inside each bin (largest work first).
   P.tiles.clear();
   P.bin_start.assign(kFp32Variants + 1, 0);
   for (int v = 0; v < kFp32Variants; ++v) {
-    auto& b = bins[v];
-    std::stable_sort(b.begin(), b.end(),
-                     [&](const Tile& x, const Tile& y) { return P.hap_len[x.hap] > P.hap_len[y.hap]; });
     P.bin_start[v] = (uint32_t)P.tiles.size();
-    P.tiles.insert(P.tiles.end(), b.begin(), b.end());
+    append_longest_first(bins[v], P.hap_len, P.tiles);
+  }
+  P.bin_start[kFp32Variants] = (uint32_t)P.tiles.size();
+  if (!pol.fold) P.fold_gap = false;
+  P.fp64_base.assign(kNumFp64Bins + 1, 0);
+  for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
+  return RVC_OK;
+}
+
+int merge_host_packs(const HostPack* const* packs, uint32_t n, HostPack* out) {
+  HostPack& P = *out;
+  P = HostPack();
+  uint64_t nh = 0, nr = 0, hw = 0, hn = 0, rw = 0, rn = 0, rq = 0, pairs = 0, tiles = 0;
+  for (uint32_t x = 0; x < n; ++x) {
+    const HostPack& A = *packs[x];
+    nh += A.hap_off.size();
+    nr += A.read_off.size();
+    hw += A.hap_pack.size();
+    hn += A.hap_nmask.size();
+    rw += A.read_pack.size();
+    rn += A.read_nmask.size();
+    rq += A.rq.size();
+    pairs += A.n_pairs_slots;
+    tiles += A.tiles.size();
+  }
+  if (hw * 16 > 0xffffffffull || rq > 0xffffffffull || pairs > 0xffffffffull)
+    return set_error(RVC_E_LIMIT, "merged batch too large for 32-bit offsets");
+  P.hap_off.reserve(nh); P.hap_len.reserve(nh);
+  P.read_off.reserve(nr); P.read_len.reserve(nr); P.read_perm.reserve(nr);
+  P.hap_pack.reserve(hw); P.hap_nmask.reserve(hn); P.read_pack.reserve(rw); P.read_nmask.reserve(rn);
+  P.rq.reserve(rq); P.ri.reserve(rq); P.rd.reserve(rq); P.rc.reserve(rq);
+  P.fp64_cap.assign(kNumFp64Bins, 0);
+  std::vector<std::vector<Tile>> bins(kFp32Variants);
+  for (uint32_t x = 0; x < n; ++x) {
+    const HostPack& A = *packs[x];
+    // every sequence block starts on a 32-base boundary, so words concatenate
+    const uint32_t hbase = (uint32_t)P.hap_pack.size() * 16, rbase = (uint32_t)P.rq.size();
+    const uint32_t hidx = (uint32_t)P.hap_off.size(), ridx = (uint32_t)P.read_off.size();
+    const uint64_t pbase = P.n_pairs_slots;
+    // trim the two spare words of each block so base offsets stay multiples of 32
+    const size_t hwords = (A.hap_pack.size() - 2) & ~size_t(1), rwords = (A.read_pack.size() - 2) & ~size_t(1);
+    P.hap_pack.insert(P.hap_pack.end(), A.hap_pack.begin(), A.hap_pack.begin() + hwords);
+    P.hap_nmask.insert(P.hap_nmask.end(), A.hap_nmask.begin(), A.hap_nmask.begin() + hwords / 2);
+    P.read_pack.insert(P.read_pack.end(), A.read_pack.begin(), A.read_pack.begin() + rwords);
+    P.read_nmask.insert(P.read_nmask.end(), A.read_nmask.begin(), A.read_nmask.begin() + rwords / 2);
+    const size_t rbytes = rwords * 16;
+    P.rq.insert(P.rq.end(), A.rq.begin(), A.rq.begin() + rbytes);
+    P.ri.insert(P.ri.end(), A.ri.begin(), A.ri.begin() + rbytes);
+    P.rd.insert(P.rd.end(), A.rd.begin(), A.rd.begin() + rbytes);
+    P.rc.insert(P.rc.end(), A.rc.begin(), A.rc.begin() + rbytes);
+    for (size_t h = 0; h < A.hap_off.size(); ++h) {
+      P.hap_off.push_back(A.hap_off[h] + hbase);
+      P.hap_len.push_back(A.hap_len[h]);
+    }
+    for (size_t r = 0; r < A.read_off.size(); ++r) {
+      P.read_off.push_back(A.read_off[r] + rbase);
+      P.read_len.push_back(A.read_len[r]);
+      P.read_perm.push_back(A.read_perm[r]);
+    }
+    for (const RegionLayout& L0 : A.region) {
+      RegionLayout L = L0;
+      L.hap0 += hidx;
+      L.read0 += ridx;
+      L.pair0 += pbase;
+      P.region.push_back(L);
+    }
+    for (int v = 0; v < kFp32Variants; ++v)
+      for (uint32_t t = A.bin_start[v]; t < A.bin_start[v + 1]; ++t) {
+        Tile tl = A.tiles[t];
+        tl.hap += hidx;
+        tl.read0 += ridx;
+        tl.out0 += (uint32_t)pbase;
+        bins[v].push_back(tl);
+      }
+    for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_cap[b] += A.fp64_cap[b];
+    P.n_pairs_slots += A.n_pairs_slots;
+    P.n_pairs += A.n_pairs;
+    P.n_cells += A.n_cells;
+    P.fold_gap = P.fold_gap && A.fold_gap;
+    P.waves_per_cu = A.waves_per_cu;
+  }
+  // spare words at the end, as build_host_pack leaves them
+  P.hap_pack.resize(P.hap_pack.size() + 2, 0);
+  P.hap_nmask.resize(P.hap_nmask.size() + 2, 0);
+  P.read_pack.resize(P.read_pack.size() + 2, 0);
+  P.read_nmask.resize(P.read_nmask.size() + 2, 0);
+  P.rq.resize(P.rq.size() + 64, 0);
+  P.ri.resize(P.ri.size() + 64, 0);
+  P.rd.resize(P.rd.size() + 64, 0);
+  P.rc.resize(P.rc.size() + 64, 0);
+  P.n_regions = (uint32_t)P.region.size();
+  P.tiles.reserve(tiles);
+  P.bin_start.assign(kFp32Variants + 1, 0);
+  for (int v = 0; v < kFp32Variants; ++v) {
+    P.bin_start[v] = (uint32_t)P.tiles.size();
+    append_longest_first(bins[v], P.hap_len, P.tiles);
   }
   P.bin_start[kFp32Variants] = (uint32_t)P.tiles.size();
   P.fp64_base.assign(kNumFp64Bins + 1, 0);
@@ -408,7 +519,8 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     a.rescue = rescue;
     a.rescue_count = rescue_count;
     a.rescue_base = rescue_base;
-    const int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL, P.fold_gap));
+    int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL, P.fold_gap));
+    if (P.waves_per_cu) per_cu = std::min<int>(per_cu, (int)P.waves_per_cu);
     const int grid = (int)std::min<uint64_t>(n, (uint64_t)per_cu * g_engine.n_cu);
     hipError_t e = launch_fp32(W, RPL, P.fold_gap, grid, s, a);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP32 launch W=%d RPL=%d: %s", W, RPL, hipGetErrorString(e));

@@ -48,6 +48,9 @@ int fp64_blocks_per_cu(int bin);
 // by the tile tuner.
 struct FP32Policy {
   uint16_t rpl16_override[17] = {};
+  bool rpl_steps_of_3 = true;  // 16-lane shapes only in RPL 1, 2, 4, 7, 10, 13, 16
+  bool fold = true;            // use the folded form when every gcp > 0
+  uint32_t waves_per_cu = 0;   // persistent grid: 0 = occupancy limit
   void pick(uint32_t R, int* W, int* RPL) const;
 };
 
@@ -73,11 +76,15 @@ struct HostPack {
   uint64_t n_pairs = 0;             // pairs with two non-empty sides
   uint64_t n_cells = 0;
   bool fold_gap = true;             // every gap-continuation quality is > 0
+  uint32_t waves_per_cu = 0;
   std::vector<double> nan_scratch;
   std::vector<uint8_t> flag_scratch;
 };
 
 int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Policy& pol, HostPack* hp);
+// Concatenates packs built by build_host_pack (any number of regions each)
+// into one batch, as if they had been packed together.
+int merge_host_packs(const HostPack* const* packs, uint32_t n, HostPack* out);
 
 struct DeviceBatch {
   const HostPack* pack = nullptr;

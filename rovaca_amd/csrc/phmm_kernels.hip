@@ -208,15 +208,12 @@ __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, cons
   // Two steps per iteration with ping-pong registers (a -> b -> a), so the
   // loop carries no register copies. A step past the last column only
   // touches columns >= H, which nothing reads, so the count is rounded up.
-  // The haplotype codes of the next iteration are read one iteration ahead.
   const int nsteps = (int)Hmax + W - 1;
   const int mysteps = (int)H + W - 1;
   const uint8_t* hp = hb - k;  // hp[s] = code of column s - k
-  int hn0 = hp[0], hn1 = hp[1];
   for (int st = 0; st < nsteps; st += 2) {
-    const int hc0 = hn0, hc1 = hn1;
-    hn0 = hp[st + 2];
-    hn1 = hp[st + 3];
+    const int hc0 = hp[st];
+    const int hc1 = hp[st + 1];
     sweep_step<T, RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, hc0, rcode, pm, px, tmm, tgm, tmi, tii, tmd,
                            tdd);
     if (st < mysteps) S += Mb[RPL - 1] + Ib[RPL - 1];

@@ -1,0 +1,103 @@
+// Load generator for the serving loop (see include/rovaca_phmm_loadgen.h).
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../include/rovaca_phmm.h"
+#include "../../include/rovaca_phmm_loadgen.h"
+#include "phmm_engine.h"
+
+namespace {
+using Clock = std::chrono::steady_clock;
+
+uint64_t ns_of(Clock::time_point t) {
+  return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t.time_since_epoch()).count();
+}
+
+double pct(std::vector<double> v, double p) {
+  if (v.empty()) return 0;
+  std::sort(v.begin(), v.end());
+  const size_t i = std::min(v.size() - 1, (size_t)(p * (double)(v.size() - 1) + 0.5));
+  return v[i];
+}
+}  // namespace
+
+extern "C" int rvc_loadgen_run(const rvc_region* regions, uint32_t n_regions, const uint32_t* order,
+                               const double* arrival_s, uint32_t n_arrivals, uint32_t n_producers,
+                               double* latency_ms, rvc_loadgen_result* result) {
+  using rvc::set_error;
+  if (!regions || !arrival_s || !result || n_producers == 0 || n_regions == 0)
+    return set_error(RVC_E_INVALID, "bad load generator arguments");
+  for (uint32_t i = 0; i < n_arrivals; ++i)
+    if (order && order[i] >= n_regions) return set_error(RVC_E_INVALID, "order[%u] out of range", i);
+  std::vector<rvc_ticket> tickets(n_arrivals, 0);
+  std::vector<int> sub_rc(n_arrivals, 0);
+  std::vector<double> late(n_arrivals, 0);
+  std::vector<rvc_ticket_timing> timing(n_arrivals);
+  std::vector<int> wait_rc(n_arrivals, 0);
+  const Clock::time_point start = Clock::now() + std::chrono::milliseconds(20);
+  auto producer = [&](uint32_t p) {
+    for (uint32_t i = p; i < n_arrivals; i += n_producers) {
+      const Clock::time_point due = start + std::chrono::nanoseconds((int64_t)(arrival_s[i] * 1e9));
+      std::this_thread::sleep_until(due);
+      late[i] = std::chrono::duration<double, std::milli>(Clock::now() - due).count();
+      const rvc_region* rg = &regions[order ? order[i] : i % n_regions];
+      sub_rc[i] = rvc_phmm_submit(rg, &tickets[i]);
+    }
+    // then collect this producer's tickets (results are not kept)
+    for (uint32_t i = p; i < n_arrivals; i += n_producers) {
+      if (sub_rc[i] != RVC_OK) continue;
+      wait_rc[i] = rvc_phmm_wait(tickets[i], nullptr, nullptr, -1, &timing[i]);
+    }
+  };
+  std::vector<std::thread> th;
+  for (uint32_t p = 0; p < n_producers; ++p) th.emplace_back(producer, p);
+  for (auto& t : th) t.join();
+  std::memset(result, 0, sizeof *result);
+  std::vector<double> lat, q, lt;
+  uint64_t first = UINT64_MAX, last = 0;
+  for (uint32_t i = 0; i < n_arrivals; ++i) {
+    if (sub_rc[i] != RVC_OK) {
+      result->failed++;
+      if (latency_ms) latency_ms[i] = -1;
+      continue;
+    }
+    result->submitted++;
+    if (wait_rc[i] != RVC_OK) {
+      result->failed++;
+      if (latency_ms) latency_ms[i] = -1;
+      continue;
+    }
+    result->completed++;
+    const rvc_region& rg = regions[order ? order[i] : i % n_regions];
+    uint64_t hs = 0, rs = 0;
+    for (uint32_t h = 0; h < rg.n_haps; ++h) hs += (uint64_t)(rg.hap_off[h + 1] - rg.hap_off[h]);
+    for (uint32_t r = 0; r < rg.n_reads; ++r) rs += (uint64_t)(rg.read_off[r + 1] - rg.read_off[r]);
+    result->cells += hs * rs;
+    const double l = (double)(timing[i].done_ns - timing[i].submit_ns) / 1e6;
+    lat.push_back(l);
+    q.push_back((double)(timing[i].launch_ns - timing[i].submit_ns) / 1e6);
+    lt.push_back(late[i]);
+    if (latency_ms) latency_ms[i] = l;
+    first = std::min(first, timing[i].submit_ns);
+    last = std::max(last, timing[i].done_ns);
+  }
+  (void)ns_of;
+  if (!lat.empty()) {
+    result->wall_s = (double)(last - first) / 1e9;
+    result->lat_p50_ms = pct(lat, 0.50);
+    result->lat_p99_ms = pct(lat, 0.99);
+    result->lat_max_ms = pct(lat, 1.0);
+    double s = 0;
+    for (double x : lat) s += x;
+    result->lat_mean_ms = s / (double)lat.size();
+    result->queue_p50_ms = pct(q, 0.50);
+    result->queue_p99_ms = pct(q, 0.99);
+    result->late_p99_ms = pct(lt, 0.99);
+  }
+  return result->failed ? set_error(RVC_E_DEVICE, "%llu requests failed", (unsigned long long)result->failed)
+                        : RVC_OK;
+}

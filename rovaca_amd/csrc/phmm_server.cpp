@@ -35,18 +35,29 @@ namespace {
 using Clock = std::chrono::steady_clock;
 constexpr int kSlots = 3;
 
-struct Request {
-  uint64_t id = 0;
-  std::vector<int32_t> hap_off, read_off;
-  std::vector<uint8_t> hap_bases, read_bases, q, i, d, c;
-  rvc_region view{};
-  uint64_t resident = 0;
-  uint64_t cells = 0;
+// Results of one finished launch, shared by its requests: each waiter copies
+// its own region out (rows back in input order) from here.
+struct BatchResult {
   std::vector<double> out;
   std::vector<uint8_t> flags;
+  std::vector<RegionLayout> region;
+  std::vector<uint32_t> read_perm;
+};
+
+struct Request {
+  uint64_t id = 0;
+  HostPack pack;  // the region, packed by the submitting thread
+  uint64_t resident = 0;
+  std::shared_ptr<const BatchResult> result;
+  uint32_t result_region = 0;
   int status = 0;
   bool done = false;
+  uint64_t t_submit = 0, t_launch = 0, t_done = 0;
 };
+
+uint64_t now_ns() {
+  return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now().time_since_epoch()).count();
+}
 
 struct Slot {
   hipStream_t stream = nullptr;
@@ -63,13 +74,13 @@ struct Slot {
   bool busy = false;
 };
 
-// Device bytes a region holds while in flight (packed sequences, tiles,
-// results, rescue list), an upper bound of what DeviceBatch::upload lays out.
-uint64_t footprint(const rvc_region& r) {
-  const uint64_t H = r.n_haps, N = r.n_reads;
-  const uint64_t hb = (uint64_t)r.hap_off[H] + 32 * H, rb = (uint64_t)r.read_off[N] + 32 * N;
-  const uint64_t pairs = H * N;
-  return hb / 2 + 8 * H + rb * 5 + 12 * N + 16 * (H * ((N + 3) / 4) + 1) + pairs * (8 + 1 + 16) + 4096;
+// Device bytes a packed region holds while in flight (sequences, tiles,
+// results, rescue list): what DeviceBatch::upload lays out for it.
+uint64_t footprint(const HostPack& p) {
+  const uint64_t words = p.hap_pack.size() + p.hap_nmask.size() + p.read_pack.size() + p.read_nmask.size();
+  const uint64_t index = p.hap_off.size() + p.hap_len.size() + p.read_off.size() + p.read_len.size();
+  return 4 * (words + index) + 4 * p.rq.size() + sizeof(Tile) * p.tiles.size() +
+         p.n_pairs_slots * (sizeof(double) + 1 + sizeof(RescueEntry)) + 4096;
 }
 
 struct Server {
@@ -90,20 +101,23 @@ struct Server {
 Server* g_srv = nullptr;
 
 void fail_requests(std::vector<Request*>& reqs, int code) {
+  const uint64_t t = now_ns();
   for (Request* r : reqs) {
     r->status = code;
+    r->t_done = t;
     r->done = true;
   }
 }
 
-// Packs and launches the slot's requests (server thread, no lock held).
+// Merges the slot's packed requests into one batch and launches it (server
+// thread, no lock held).
 int launch_slot(Slot& s) {
-  std::vector<rvc_region> regs;
-  regs.reserve(s.reqs.size());
-  for (Request* r : s.reqs) regs.push_back(r->view);
-  static FP32Policy pol;
-  int rc = build_host_pack(regs.data(), (uint32_t)regs.size(), pol, &s.pack);
+  std::vector<const HostPack*> packs;
+  packs.reserve(s.reqs.size());
+  for (Request* r : s.reqs) packs.push_back(&r->pack);
+  int rc = merge_host_packs(packs.data(), (uint32_t)packs.size(), &s.pack);
   if (rc) return rc;
+  for (Request* r : s.reqs) r->pack = HostPack();  // merged copy is all the launch needs
   rc = s.batch.upload(s.pack, s.stream, s.mem, s.bytes);
   if (rc) return rc;
   if (s.batch.own_mem) {
@@ -133,31 +147,31 @@ int launch_slot(Slot& s) {
   return RVC_OK;
 }
 
-// Scatters a finished slot's results into its requests (server thread).
+// Hands a finished slot's results to its requests (server thread): one bulk
+// copy out of the pinned buffers; each waiter then takes its own rows.
 void finish_slot(Slot& s) {
   const HostPack& P = s.pack;
   float ms = 0;
   if (hipEventElapsedTime(&ms, s.batch.ev[0], s.batch.ev[2]) != hipSuccess) ms = 0;
-  for (uint32_t r = 0; r < P.n_regions; ++r) {
-    const RegionLayout& L = P.region[r];
-    Request* rq = s.reqs[r];
-    rq->out.resize((size_t)L.n_haps * L.n_reads);
-    rq->flags.resize((size_t)L.n_haps * L.n_reads);
-    for (uint32_t sx = 0; sx < L.n_reads; ++sx) {
-      const uint32_t x = P.read_perm[L.read0 + sx];
-      std::memcpy(&rq->out[(size_t)x * L.n_haps], s.h_out + L.pair0 + (size_t)sx * L.n_haps,
-                  L.n_haps * sizeof(double));
-      std::memcpy(&rq->flags[(size_t)x * L.n_haps], s.h_flags + L.pair0 + (size_t)sx * L.n_haps, L.n_haps);
-    }
-  }
+  auto res = std::make_shared<BatchResult>();
+  const size_t n = P.n_pairs_slots;
+  res->out.assign(s.h_out, s.h_out + n);
+  res->flags.assign(s.h_flags, s.h_flags + n);
+  res->region = P.region;
+  res->read_perm = P.read_perm;
   uint64_t resc = 0;
-  for (size_t x = 0; x < P.n_pairs_slots; ++x) resc += (s.h_flags[x] & RVC_PAIR_FP64) ? 1 : 0;
+  for (size_t x = 0; x < n; ++x) resc += (s.h_flags[x] & RVC_PAIR_FP64) ? 1 : 0;
   std::lock_guard<std::mutex> lk(g_srv->mu);
   g_srv->busy_ms += ms;
   g_srv->rescued += resc;
   g_srv->cells += P.n_cells;
-  for (Request* rq : s.reqs) {
+  const uint64_t t = now_ns();
+  for (uint32_t r = 0; r < (uint32_t)s.reqs.size(); ++r) {
+    Request* rq = s.reqs[r];
+    rq->result = res;
+    rq->result_region = r;
     rq->status = RVC_OK;
+    rq->t_done = t;
     rq->done = true;
     ++g_srv->completed;
   }
@@ -188,6 +202,8 @@ void server_main() {
       if (s.reqs.empty()) break;  // the head does not fit yet: wait for a slot to finish
       s.busy = true;
       s.resident = add;
+      const uint64_t t = now_ns();
+      for (Request* r : s.reqs) r->t_launch = t;
       S.resident += add;
       if (S.resident > S.peak_resident) S.peak_resident = S.resident;
       ++S.launches;
@@ -293,32 +309,13 @@ int rvc_phmm_submit(const rvc_region* region, rvc_ticket* ticket) {
   if (rc) return rc;
   auto* r = new (std::nothrow) Request;
   if (!r) return set_error(RVC_E_INVALID, "out of host memory");
-  const uint32_t H = region->n_haps, N = region->n_reads;
-  r->hap_off.assign(region->hap_off, region->hap_off + H + 1);
-  r->read_off.assign(region->read_off, region->read_off + N + 1);
-  const int32_t hb = region->hap_off[H] - region->hap_off[0], rb = region->read_off[N] - region->read_off[0];
-  const int32_t h0 = region->hap_off[0], r0 = region->read_off[0];
-  for (auto& v : r->hap_off) v -= h0;
-  for (auto& v : r->read_off) v -= r0;
-  if (hb > 0) r->hap_bases.assign(region->hap_bases + h0, region->hap_bases + h0 + hb);
-  if (rb > 0) {
-    r->read_bases.assign(region->read_bases + r0, region->read_bases + r0 + rb);
-    r->q.assign(region->q + r0, region->q + r0 + rb);
-    r->i.assign(region->i + r0, region->i + r0 + rb);
-    r->d.assign(region->d + r0, region->d + r0 + rb);
-    r->c.assign(region->c + r0, region->c + r0 + rb);
+  static const FP32Policy pol;
+  rc = build_host_pack(region, 1, pol, &r->pack);
+  if (rc) {
+    delete r;
+    return rc;
   }
-  r->view.n_haps = H;
-  r->view.n_reads = N;
-  r->view.hap_off = r->hap_off.data();
-  r->view.hap_bases = r->hap_bases.data();
-  r->view.read_off = r->read_off.data();
-  r->view.read_bases = r->read_bases.data();
-  r->view.q = r->q.data();
-  r->view.i = r->i.data();
-  r->view.d = r->d.data();
-  r->view.c = r->c.data();
-  r->resident = footprint(r->view);
+  r->resident = footprint(r->pack);
   if (r->resident > g_engine.hbm_budget) {
     const uint64_t need = r->resident;
     delete r;
@@ -333,6 +330,7 @@ int rvc_phmm_submit(const rvc_region* region, rvc_ticket* ticket) {
       return set_error(RVC_E_SHUTDOWN, "server is stopping");
     }
     r->id = S.next_id++;
+    r->t_submit = now_ns();
     S.tickets[r->id] = r;
     S.pending.push_back(r);
     ++S.submitted;
@@ -342,7 +340,8 @@ int rvc_phmm_submit(const rvc_region* region, rvc_ticket* ticket) {
   return RVC_OK;
 }
 
-int rvc_phmm_wait(rvc_ticket ticket, double* out_read_major, uint8_t* out_flags, int64_t timeout_ms) {
+int rvc_phmm_wait(rvc_ticket ticket, double* out_read_major, uint8_t* out_flags, int64_t timeout_ms,
+                  rvc_ticket_timing* timing) {
   if (!g_srv) return set_error(RVC_E_STATE, "serving loop is not running");
   Server& S = *g_srv;
   std::unique_lock<std::mutex> lk(S.mu);
@@ -358,9 +357,20 @@ int rvc_phmm_wait(rvc_ticket ticket, double* out_read_major, uint8_t* out_flags,
   S.tickets.erase(it);
   lk.unlock();
   const int st = r->status;
-  if (st == RVC_OK) {
-    if (out_read_major) std::memcpy(out_read_major, r->out.data(), r->out.size() * sizeof(double));
-    if (out_flags) std::memcpy(out_flags, r->flags.data(), r->flags.size());
+  if (timing) {
+    timing->submit_ns = r->t_submit;
+    timing->launch_ns = r->t_launch;
+    timing->done_ns = r->t_done;
+  }
+  if (st == RVC_OK && (out_read_major || out_flags)) {
+    const BatchResult& B = *r->result;
+    const RegionLayout& L = B.region[r->result_region];
+    for (uint32_t sx = 0; sx < L.n_reads; ++sx) {
+      const uint32_t x = B.read_perm[L.read0 + sx];
+      const size_t src = L.pair0 + (size_t)sx * L.n_haps, dst = (size_t)x * L.n_haps;
+      if (out_read_major) std::memcpy(out_read_major + dst, B.out.data() + src, L.n_haps * sizeof(double));
+      if (out_flags) std::memcpy(out_flags + dst, B.flags.data() + src, L.n_haps);
+    }
   }
   delete r;
   return st == RVC_OK ? RVC_OK : set_error(st, "request failed with code %d", st);

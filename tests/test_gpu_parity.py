@@ -245,3 +245,37 @@ def test_call_pairhmm_region_adapter():
         m = ~np.isnan(want)
         assert np.array_equal(m, ~np.isnan(mat))
         assert np.abs(mat[m] - want[m]).max() < TOL
+
+
+def test_loadgen_serves_poisson_stream():
+    say("load generator: 300 regions at 3000/s from 8 producers")
+    regs = [r for r in workload.batched_regions(40, seed=71) if r["n_haps"] * r["n_reads"] < 6000]
+    rng = np.random.default_rng(1)
+    arr = np.cumsum(rng.exponential(1 / 3000.0, 300))
+    res, lat = rv.loadgen_run(regs, arr, np.arange(300) % len(regs), 8)
+    assert res["completed"] == 300 and res["failed"] == 0
+    assert (lat > 0).all() and res["lat_p99_ms"] >= res["lat_p50_ms"] > 0
+    assert res["cells"] == sum(workload.cells([regs[i % len(regs)]]) for i in range(300))
+
+
+def test_budget_admission():
+    say("admission by budget: a region larger than the whole budget is refused")
+    rv.shutdown()
+    rv.init(0, server=True, hbm_budget_bytes=1 << 20)
+    try:
+        big = workload.batched_regions(30, seed=81)
+        big = max(big, key=lambda r: r["n_haps"] * r["n_reads"])
+        with pytest.raises(rv.EngineError) as e:
+            rv.submit(big)
+        assert e.value.code == rv.RVC_E_BUDGET
+        small = [r for r in workload.batched_regions(30, seed=82) if r["n_haps"] * r["n_reads"] < 2000][:6]
+        # several small ones still go through one at a time under the tight budget
+        tickets = [rv.submit(r) for r in small]
+        for t, r in zip(tickets, small):
+            got, _ = rv.wait(t, r)
+            alone, _ = rv.region_sync(r)
+            assert np.array_equal(np.isnan(got), np.isnan(alone))
+            assert np.array_equal(got[~np.isnan(got)], alone[~np.isnan(alone)])
+    finally:
+        rv.shutdown()
+        rv.init(0, server=True)
