@@ -263,8 +263,7 @@ def test_budget_admission():
     rv.shutdown()
     rv.init(0, server=True, hbm_budget_bytes=1 << 20)
     try:
-        big = workload.batched_regions(30, seed=81)
-        big = max(big, key=lambda r: r["n_haps"] * r["n_reads"])
+        big = workload.deep_region(seed=81)  # 256k pairs: several MB on the device
         with pytest.raises(rv.EngineError) as e:
             rv.submit(big)
         assert e.value.code == rv.RVC_E_BUDGET
