@@ -92,7 +92,10 @@ typedef struct rvc_config {
   uint32_t server_threads;   /* 0 = no serving loop; 1 = start it at init          */
   uint64_t hbm_budget_bytes; /* serving: bound on resident request buffers (0 = 64 GiB) */
   uint32_t max_batch_regions;/* serving: regions per device launch (0 = 4096)       */
-  uint32_t reserved[8];
+  uint32_t reserved0;
+  const char* tune_table;    /* kernel-shape table written by the tuner; NULL = the */
+                             /* table installed beside the library, if any         */
+  uint32_t reserved[6];
 } rvc_config;
 
 /* ---- lifecycle -------------------------------------------------------------- */
@@ -134,7 +137,11 @@ typedef struct rvc_batch_opts {
                                     /* factor into I/D where allowed, 2 = never       */
   uint32_t rpl_round;               /* 0 = tuned choice, 1 = exact rows per lane,     */
                                     /* 2 = rows per lane in steps of 3                */
-  uint32_t reserved[4];
+  uint32_t occupancy;               /* 0 = tuned choice, else 2/3/4 waves per SIMD    */
+                                    /* for the 16-lane kernels                        */
+  uint32_t segment_width;           /* 0 = tuned choice, else 16/32/64 lanes per read */
+                                    /* for reads up to 256 bases                      */
+  uint32_t reserved[2];
 } rvc_batch_opts;
 
 typedef struct rvc_batch_info {

@@ -56,12 +56,13 @@ class RvcRawRegion(ctypes.Structure):
 class RvcConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("server_threads", ctypes.c_uint32),
                 ("hbm_budget_bytes", ctypes.c_uint64), ("max_batch_regions", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 8)]
+                ("reserved0", ctypes.c_uint32), ("tune_table", ctypes.c_char_p), ("reserved", ctypes.c_uint32 * 6)]
 
 
 class RvcBatchOpts(ctypes.Structure):
     _fields_ = [("persistent_waves_per_cu", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
-                ("fp32_fold", ctypes.c_uint32), ("rpl_round", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+                ("fp32_fold", ctypes.c_uint32), ("rpl_round", ctypes.c_uint32), ("occupancy", ctypes.c_uint32),
+                ("segment_width", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
 
 
 class RvcBatchInfo(ctypes.Structure):
@@ -151,8 +152,9 @@ def make_c_region(region):
     return cr
 
 
-def init(device=0, server=False, hbm_budget_bytes=0, max_batch_regions=0):
+def init(device=0, server=False, hbm_budget_bytes=0, max_batch_regions=0, tune_table=None):
     cfg = RvcConfig()
+    cfg.tune_table = tune_table.encode() if tune_table else None
     cfg.device = device
     cfg.server_threads = 1 if server else 0
     cfg.hbm_budget_bytes = hbm_budget_bytes
@@ -201,7 +203,8 @@ def call_pairhmm(haps, reads, quals, mqs, min_quality_threshold=0, pcr=PCR_NONE)
 class Batch:
     """Many regions packed, uploaded and launched together."""
 
-    def __init__(self, regions, use_graph=False, fp32_fold=0, rpl_round=0, waves_per_cu=0):
+    def __init__(self, regions, use_graph=False, fp32_fold=0, rpl_round=0, waves_per_cu=0, occupancy=0,
+                 segment_width=0):
         self._regions = regions
         arr = (RvcRegion * len(regions))()
         for x, r in enumerate(regions):
@@ -212,6 +215,8 @@ class Batch:
         opts.fp32_fold = fp32_fold
         opts.rpl_round = rpl_round
         opts.persistent_waves_per_cu = waves_per_cu
+        opts.occupancy = occupancy
+        opts.segment_width = segment_width
         h = ctypes.c_void_p()
         _check(lib().rvc_phmm_batch_create(arr, len(regions), ctypes.byref(opts), ctypes.byref(h)))
         self.h = h

@@ -2,6 +2,7 @@
 // The serving entry points live in phmm_server.cpp.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -75,10 +76,7 @@ void release_workspaces() {
   ++g_ws_generation;
 }
 
-FP32Policy& policy() {
-  static FP32Policy p;
-  return p;
-}
+FP32Policy& policy() { return default_policy(); }
 
 }  // namespace
 
@@ -196,6 +194,9 @@ int rvc_phmm_batch_create(const rvc_region* regions, uint32_t n_regions, const r
     if (opts->rpl_round == 1) pol.rpl_steps_of_3 = false;
     if (opts->rpl_round == 2) pol.rpl_steps_of_3 = true;
     pol.waves_per_cu = opts->persistent_waves_per_cu;
+    if (opts->occupancy) pol.force_occ = (uint8_t)std::min(4u, std::max(2u, opts->occupancy));
+    if (opts->segment_width == 16 || opts->segment_width == 32 || opts->segment_width == 64)
+      pol.force_w = (uint8_t)opts->segment_width;
   }
   int rc = build_host_pack(regions, n_regions, pol, &b->pack);
   if (rc) return rc;

@@ -1,6 +1,7 @@
 // Host side of the MI355X PairHMM engine: device set-up, packing of regions
 // into the device format, length-binned tile queues, launches and the C ABI
 // of include/rovaca_phmm.h.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -80,6 +81,27 @@ int engine_init(const rvc_config* cfg) {
   g_engine.tables.mm_d = base->mm_d;
   g_engine.hbm_budget = (cfg && cfg->hbm_budget_bytes) ? cfg->hbm_budget_bytes : (64ull << 30);
   g_engine.max_batch_regions = (cfg && cfg->max_batch_regions) ? cfg->max_batch_regions : 4096u;
+  // kernel-shape table: the one named in the config, else tuned/<device>.txt beside the library
+  std::string table;
+  if (cfg && cfg->tune_table) {
+    table = cfg->tune_table;
+  } else {
+    Dl_info info;
+    if (dladdr((void*)&engine_init, &info) && info.dli_fname) {
+      std::string lib = info.dli_fname;
+      const size_t slash = lib.rfind('/');
+      table = (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/tuned/mi355x.txt";
+      FILE* f = fopen(table.c_str(), "r");
+      if (f) fclose(f); else table.clear();
+    }
+  }
+  default_policy() = FP32Policy();
+  g_engine.tune_source.clear();
+  if (!table.empty()) {
+    const int rc = load_tune_table(table.c_str(), &default_policy());
+    if (rc) return rc;
+    g_engine.tune_source = table;
+  }
   g_engine.ready = true;
   return RVC_OK;
 }
@@ -131,6 +153,49 @@ inline uint32_t base_code(uint8_t ch, bool* is_n) {
 inline uint32_t align_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
 }  // namespace
 
+FP32Policy& default_policy() {
+  static FP32Policy p;
+  return p;
+}
+
+// Table format (text): lines "occ16 <17 ints>", "shape16 <17 ints>" (0 or
+// W*256+RPL), "steps3 0|1", "fold 0|1"; '#' starts a comment.
+int load_tune_table(const char* path, FP32Policy* pol) {
+  FILE* f = fopen(path, "r");
+  if (!f) return set_error(RVC_E_INVALID, "cannot open tune table %s", path);
+  FP32Policy p;
+  char line[1024];
+  int rc = RVC_OK;
+  while (fgets(line, sizeof line, f)) {
+    char key[32];
+    int n = 0;
+    if (line[0] == '#' || sscanf(line, "%31s%n", key, &n) != 1) continue;
+    const char* rest = line + n;
+    int v[17], cnt = 0, used = 0;
+    while (cnt < 17 && sscanf(rest, "%d%n", &v[cnt], &used) == 1) { rest += used; ++cnt; }
+    if (!strcmp(key, "occ16") && cnt == 17) {
+      for (int x = 0; x < 17; ++x) p.occ16[x] = (uint8_t)std::min(4, std::max(2, v[x]));
+    } else if (!strcmp(key, "shape16") && cnt == 17) {
+      for (int x = 0; x < 17; ++x) {
+        const int w = v[x] >> 8, r = v[x] & 0xff;
+        const bool ok = v[x] == 0 || ((w == 16 || w == 32 || w == 64) && r >= 1 && r * w >= x * 16 && fp32_supported(w, r));
+        if (!ok) { rc = set_error(RVC_E_INVALID, "tune table %s: bad shape16 entry %d", path, v[x]); break; }
+        p.rpl16_override[x] = (uint16_t)v[x];
+      }
+    } else if (!strcmp(key, "steps3") && cnt == 1) {
+      p.rpl_steps_of_3 = v[0] != 0;
+    } else if (!strcmp(key, "fold") && cnt == 1) {
+      p.fold = v[0] != 0;
+    } else {
+      rc = set_error(RVC_E_INVALID, "tune table %s: cannot read line '%s'", path, key);
+    }
+    if (rc) break;
+  }
+  fclose(f);
+  if (rc == RVC_OK) *pol = p;
+  return rc;
+}
+
 void FP32Policy::pick(uint32_t R, int* W, int* RPL) const {
   // 16-lane shapes come in steps of 3 rows per lane, so that reads of nearby
   // lengths share one launch (each launch has a tail of its own)
@@ -139,7 +204,10 @@ void FP32Policy::pick(uint32_t R, int* W, int* RPL) const {
   if (R <= 256) { *W = 16; *RPL = rpl_steps_of_3 ? kRpl16[exact] : exact; }
   else if (R <= 512) { *W = 32; *RPL = (int)((R + 31) / 32); }
   else { *W = 64; *RPL = (int)((R + 63) / 64); }
-  if (R <= 256 && rpl16_override[exact] != 0) {
+  if (R <= 256 && force_w) {
+    *W = force_w;
+    *RPL = (int)std::max<uint32_t>(1, (R + force_w - 1) / force_w);
+  } else if (R <= 256 && rpl16_override[exact] != 0) {
     *W = rpl16_override[exact] >> 8;
     *RPL = rpl16_override[exact] & 0xff;
   }
@@ -218,6 +286,11 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
   uint32_t hcur = 0, rcur = 0;
   P.fp64_cap.assign(kNumFp64Bins, 0);
   P.waves_per_cu = pol.waves_per_cu;
+  for (int v = 0; v < kFp32Variants; ++v) {
+    int W, RPL;
+    fp32_variant_shape(v, &W, &RPL);
+    P.occ[v] = (uint8_t)pol.occupancy(W, RPL);
+  }
   std::vector<std::vector<Tile>> bins(kFp32Variants);
   for (uint32_t r = 0; r < n_regions; ++r) {
     const rvc_region& g = regions[r];
@@ -366,6 +439,7 @@ int merge_host_packs(const HostPack* const* packs, uint32_t n, HostPack* out) {
     P.n_cells += A.n_cells;
     P.fold_gap = P.fold_gap && A.fold_gap;
     P.waves_per_cu = A.waves_per_cu;
+    std::memcpy(P.occ, A.occ, sizeof P.occ);
   }
   // spare words at the end, as build_host_pack leaves them
   P.hap_pack.resize(P.hap_pack.size() + 2, 0);
@@ -519,10 +593,10 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     a.rescue = rescue;
     a.rescue_count = rescue_count;
     a.rescue_base = rescue_base;
-    int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL, P.fold_gap));
+    int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL, P.fold_gap, P.occ[v]));
     if (P.waves_per_cu) per_cu = std::min<int>(per_cu, (int)P.waves_per_cu);
     const int grid = (int)std::min<uint64_t>(n, (uint64_t)per_cu * g_engine.n_cu);
-    hipError_t e = launch_fp32(W, RPL, P.fold_gap, grid, s, a);
+    hipError_t e = launch_fp32(W, RPL, P.fold_gap, P.occ[v], grid, s, a);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP32 launch W=%d RPL=%d: %s", W, RPL, hipGetErrorString(e));
     ++n_fp32_launches;
   }

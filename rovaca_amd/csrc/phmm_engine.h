@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/rovaca_phmm.h"
@@ -28,6 +29,7 @@ struct Engine {
   DevTables tables{};
   uint64_t hbm_budget = 0;
   uint32_t max_batch_regions = 0;
+  std::string tune_source;  // where the kernel-shape table came from ("" = built-in defaults)
 };
 extern Engine g_engine;
 
@@ -39,7 +41,7 @@ int validate_region(const rvc_region* r);
 constexpr int kFp32Variants = 48;
 int fp32_variant_index(int W, int RPL);
 void fp32_variant_shape(int v, int* W, int* RPL);
-int fp32_blocks_per_cu(int W, int RPL, bool xf);
+int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ);
 int fp64_blocks_per_cu(int bin);
 
 // Chooses the (W, RPL) kernel for a tile whose longest read has R bases.
@@ -48,11 +50,21 @@ int fp64_blocks_per_cu(int bin);
 // by the tile tuner.
 struct FP32Policy {
   uint16_t rpl16_override[17] = {};
+  // waves per SIMD the 16-lane kernel of each RPL is compiled for (2, 3, 4)
+  uint8_t occ16[17] = {4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 2, 2, 2, 2, 2, 2};
   bool rpl_steps_of_3 = true;  // 16-lane shapes only in RPL 1, 2, 4, 7, 10, 13, 16
   bool fold = true;            // use the folded form when every gcp > 0
   uint32_t waves_per_cu = 0;   // persistent grid: 0 = occupancy limit
+  uint8_t force_occ = 0;       // batch option: 2/3/4 for every 16-lane kernel
+  uint8_t force_w = 0;         // batch option: 16/32/64 lanes per read for reads <= 256
   void pick(uint32_t R, int* W, int* RPL) const;
+  int occupancy(int W, int RPL) const { return W == 16 ? (force_occ ? force_occ : occ16[RPL]) : 2; }
 };
+
+// The process-wide policy (built-in defaults, replaced by a tuned table at init).
+FP32Policy& default_policy();
+// Reads a table written by rovaca_amd/tune.py; returns RVC_OK or an error code.
+int load_tune_table(const char* path, FP32Policy* pol);
 
 struct RegionLayout {
   uint32_t n_haps = 0, n_reads = 0;
@@ -77,6 +89,7 @@ struct HostPack {
   uint64_t n_cells = 0;
   bool fold_gap = true;             // every gap-continuation quality is > 0
   uint32_t waves_per_cu = 0;
+  uint8_t occ[kFp32Variants] = {};  // occupancy target per FP32 variant
   std::vector<double> nan_scratch;
   std::vector<uint8_t> flag_scratch;
 };

@@ -63,17 +63,20 @@ struct RescueEntry {
 };
 
 // FP64 bins: (segment width, rows per lane) chosen from the read length so
-// that no lane holds more than 12 double rows.
-constexpr int kNumFp64Bins = 20;
+// that no lane holds more than 12 double rows. Few bins on purpose: the FP64
+// pass is one launch per bin that can occur, whether or not any pair of the
+// batch needs it.
+constexpr int kNumFp64Bins = 4;
 __host__ __device__ inline int fp64_bin(uint32_t R) {
-  if (R <= 128) return (int)((R + 15) / 16) - 1;          // W=16, RPL 1..8   -> bins 0..7
-  if (R <= 256) return 8 + (int)((R + 31) / 32) - 5;      // W=32, RPL 5..8   -> bins 8..11
-  return 12 + (int)((R + 63) / 64) - 5;                   // W=64, RPL 5..12  -> bins 12..19
+  if (R <= 64) return 0;    // W=16, RPL 4
+  if (R <= 128) return 1;   // W=16, RPL 8
+  if (R <= 256) return 2;   // W=32, RPL 8
+  return 3;                 // W=64, RPL 12
 }
 inline void fp64_bin_shape(int bin, int* W, int* RPL) {
-  if (bin < 8) { *W = 16; *RPL = bin + 1; }
-  else if (bin < 12) { *W = 32; *RPL = bin - 8 + 5; }
-  else { *W = 64; *RPL = bin - 12 + 5; }
+  static const int kW[kNumFp64Bins] = {16, 16, 32, 64}, kR[kNumFp64Bins] = {4, 8, 8, 12};
+  *W = kW[bin];
+  *RPL = kR[bin];
 }
 
 struct Fp32Args {
@@ -101,7 +104,7 @@ struct Fp64Args {
 
 // Kernel launchers (phmm_kernels.hip).
 bool fp32_supported(int W, int RPL);
-hipError_t launch_fp32(int W, int RPL, bool xf, int grid, hipStream_t stream, const Fp32Args& a);
+hipError_t launch_fp32(int W, int RPL, bool xf, int occ, int grid, hipStream_t stream, const Fp32Args& a);
 hipError_t launch_fp64(int bin, int grid, hipStream_t stream, const Fp64Args& a);
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
 

@@ -24,6 +24,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
 #include <utility>
@@ -232,9 +233,10 @@ __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, cons
 }
 
 // FP32 pass: persistent waves pull tiles (one haplotype x up to 64/W reads).
-// Two waves per SIMD at least: the register budget is then 256 per lane.
-template <int W, int RPL, bool XF>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_fp32(Fp32Args a) {
+// OCC = waves per SIMD the register allocation must allow (512 / OCC VGPRs
+// per lane): the sweep is issue-latency bound, so occupancy is a tuning knob.
+template <int W, int RPL, bool XF, int OCC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void k_fp32(Fp32Args a) {
   __shared__ uint8_t hbuf[kHapBuf];
   constexpr int NSEG = 64 / W;
   const int lane = threadIdx.x;
@@ -313,39 +315,42 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 using Fp32Fn = void (*)(Fp32Args);
 using Fp64Fn = void (*)(Fp64Args);
 
-template <int W, bool XF, int... R>
+template <int W, bool XF, int OCC, int... R>
 constexpr auto fp32_row(std::integer_sequence<int, R...>) {
-  return std::array<Fp32Fn, sizeof...(R)>{{&k_fp32<W, R + 1, XF>...}};
+  return std::array<Fp32Fn, sizeof...(R)>{{&k_fp32<W, R + 1, XF, OCC>...}};
 }
 
-// FP32 variants: W = 16, 32, 64 with 1..16 rows per lane (W = 64 up to 12),
+// FP32 variants: W = 16 with 1..16 rows per lane at occupancy targets 2, 3
+// and 4 waves per SIMD; W = 32 (1..16) and W = 64 (1..12) at 2 waves; each
 // with and without the folded gap-to-match factor.
-const std::array<Fp32Fn, 16> kFp32W16[2] = {fp32_row<16, false>(std::make_integer_sequence<int, 16>{}),
-                                            fp32_row<16, true>(std::make_integer_sequence<int, 16>{})};
-const std::array<Fp32Fn, 16> kFp32W32[2] = {fp32_row<32, false>(std::make_integer_sequence<int, 16>{}),
-                                            fp32_row<32, true>(std::make_integer_sequence<int, 16>{})};
-const std::array<Fp32Fn, 12> kFp32W64[2] = {fp32_row<64, false>(std::make_integer_sequence<int, 12>{}),
-                                            fp32_row<64, true>(std::make_integer_sequence<int, 12>{})};
+using Row16 = std::array<Fp32Fn, 16>;
+const Row16 kFp32W16[2][3] = {
+    {fp32_row<16, false, 2>(std::make_integer_sequence<int, 16>{}),
+     fp32_row<16, false, 3>(std::make_integer_sequence<int, 16>{}),
+     fp32_row<16, false, 4>(std::make_integer_sequence<int, 16>{})},
+    {fp32_row<16, true, 2>(std::make_integer_sequence<int, 16>{}),
+     fp32_row<16, true, 3>(std::make_integer_sequence<int, 16>{}),
+     fp32_row<16, true, 4>(std::make_integer_sequence<int, 16>{})}};
+const Row16 kFp32W32[2] = {fp32_row<32, false, 2>(std::make_integer_sequence<int, 16>{}),
+                           fp32_row<32, true, 2>(std::make_integer_sequence<int, 16>{})};
+const std::array<Fp32Fn, 12> kFp32W64[2] = {fp32_row<64, false, 2>(std::make_integer_sequence<int, 12>{}),
+                                            fp32_row<64, true, 2>(std::make_integer_sequence<int, 12>{})};
 
-const std::array<Fp64Fn, kNumFp64Bins> kFp64 = {{
-    &k_fp64<16, 1>, &k_fp64<16, 2>, &k_fp64<16, 3>, &k_fp64<16, 4>, &k_fp64<16, 5>, &k_fp64<16, 6>,
-    &k_fp64<16, 7>, &k_fp64<16, 8>, &k_fp64<32, 5>, &k_fp64<32, 6>, &k_fp64<32, 7>, &k_fp64<32, 8>,
-    &k_fp64<64, 5>, &k_fp64<64, 6>, &k_fp64<64, 7>, &k_fp64<64, 8>, &k_fp64<64, 9>, &k_fp64<64, 10>,
-    &k_fp64<64, 11>, &k_fp64<64, 12>}};
+const std::array<Fp64Fn, kNumFp64Bins> kFp64 = {{&k_fp64<16, 4>, &k_fp64<16, 8>, &k_fp64<32, 8>, &k_fp64<64, 12>}};
 
-Fp32Fn fp32_fn(int W, int RPL, bool xf) {
+Fp32Fn fp32_fn(int W, int RPL, bool xf, int occ) {
   if (RPL < 1) return nullptr;
-  if (W == 16 && RPL <= 16) return kFp32W16[xf][RPL - 1];
+  if (W == 16 && RPL <= 16) return kFp32W16[xf][std::min(4, std::max(2, occ)) - 2][RPL - 1];
   if (W == 32 && RPL <= 16) return kFp32W32[xf][RPL - 1];
   if (W == 64 && RPL <= 12) return kFp32W64[xf][RPL - 1];
   return nullptr;
 }
 }  // namespace
 
-bool fp32_supported(int W, int RPL) { return fp32_fn(W, RPL, false) != nullptr; }
+bool fp32_supported(int W, int RPL) { return fp32_fn(W, RPL, false, 2) != nullptr; }
 
-hipError_t launch_fp32(int W, int RPL, bool xf, int grid, hipStream_t stream, const Fp32Args& a) {
-  Fp32Fn f = fp32_fn(W, RPL, xf);
+hipError_t launch_fp32(int W, int RPL, bool xf, int occ, int grid, hipStream_t stream, const Fp32Args& a) {
+  Fp32Fn f = fp32_fn(W, RPL, xf, occ);
   if (!f) return hipErrorInvalidValue;
   hipLaunchKernelGGL(f, dim3(grid), dim3(64), 0, stream, a);
   return hipGetLastError();
@@ -368,17 +373,18 @@ hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream) {
 namespace rvc {
 
 // Resident 64-lane blocks per CU for a kernel (register and LDS limited).
-int fp32_blocks_per_cu(int W, int RPL, bool xf) {
-  static int cache[2][3][17] = {};
+int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ) {
+  static int cache[2][3][3][17] = {};
   const int w = W == 16 ? 0 : W == 32 ? 1 : 2;
+  const int o = std::min(4, std::max(2, occ)) - 2;
   if (RPL < 1 || RPL > 16) return 1;
-  if (cache[xf][w][RPL] == 0) {
+  if (cache[xf][o][w][RPL] == 0) {
     int n = 0;
-    Fp32Fn f = fp32_fn(W, RPL, xf);
+    Fp32Fn f = fp32_fn(W, RPL, xf, occ);
     if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64, 0) != hipSuccess || n < 1) n = 1;
-    cache[xf][w][RPL] = n;
+    cache[xf][o][w][RPL] = n;
   }
-  return cache[xf][w][RPL];
+  return cache[xf][o][w][RPL];
 }
 
 int fp64_blocks_per_cu(int bin) {

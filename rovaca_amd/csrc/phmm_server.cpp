@@ -309,8 +309,7 @@ int rvc_phmm_submit(const rvc_region* region, rvc_ticket* ticket) {
   if (rc) return rc;
   auto* r = new (std::nothrow) Request;
   if (!r) return set_error(RVC_E_INVALID, "out of host memory");
-  static const FP32Policy pol;
-  rc = build_host_pack(region, 1, pol, &r->pack);
+  rc = build_host_pack(region, 1, default_policy(), &r->pack);
   if (rc) {
     delete r;
     return rc;
