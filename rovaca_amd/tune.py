@@ -65,6 +65,7 @@ def main():
             if -(-R // w) <= (16 if w == 32 else 12):
                 cands.append((w, 2))
         best = None
+        best16 = None
         for w, occ in cands:
             b = rv.Batch(regs, rpl_round=1, occupancy=occ, segment_width=w)
             b.run()
@@ -75,10 +76,13 @@ def main():
             print(f"[tune] R={R:3d} W={w:2d} occ={occ}: {ms:8.3f} ms {gcups:8.1f} GCUPS", flush=True)
             if best is None or ms < best[2]:
                 best = (w, occ, ms)
+            if w == 16 and (best16 is None or ms < best16[1]):
+                best16 = (occ, ms)
+        # the 16-lane occupancy is kept even where a wider shape wins, because
+        # reads of other lengths can round up to this row count
+        occ16[rpl] = best16[0]
         w, occ, _ = best
-        if w == 16:
-            occ16[rpl] = occ
-        else:
+        if w != 16:
             shape16[rpl] = w * 256 + (-(-R // w))
     occ16[0] = occ16[1]
     rv.shutdown()
