@@ -157,6 +157,18 @@ typedef struct rvc_batch_info {
   uint32_t n_fp64_launches;
 } rvc_batch_info;
 
+/* One kernel launch of the last timed (non-graph) run of a batch. */
+typedef struct rvc_kernel_time {
+  char name[48];          /* e.g. "k_fp32<16,10,fold,occ4>"                          */
+  uint32_t precision;     /* 32 or 64                                                */
+  uint32_t width;         /* lanes per read (W)                                      */
+  uint32_t rows_per_lane; /* RPL                                                     */
+  uint32_t occupancy;     /* waves per SIMD the kernel was compiled for              */
+  uint64_t tiles;         /* work units queued (FP64: rescue capacity)               */
+  uint64_t cells;         /* sum of read_len * hap_len of the pairs it computes      */
+  double ms;              /* device time between the events around the launch        */
+} rvc_kernel_time;
+
 /* Packs regions (2-bit bases + N mask, int8 quals), uploads them and builds
  * the length-binned tile queues. Nothing runs yet. */
 int rvc_phmm_batch_create(const rvc_region* regions, uint32_t n_regions, const rvc_batch_opts* opts,
@@ -167,6 +179,9 @@ int rvc_phmm_batch_run(rvc_batch* b);
  * offsets are the running sum of n_reads * n_haps. flags may be NULL. */
 int rvc_phmm_batch_results(rvc_batch* b, double* out, uint8_t* flags);
 int rvc_phmm_batch_info(const rvc_batch* b, rvc_batch_info* info);
+/* Per-launch times of the last non-graph run: fills up to max entries and
+ * sets *n to the number of launches. */
+int rvc_phmm_batch_kernels(const rvc_batch* b, rvc_kernel_time* out, uint32_t max, uint32_t* n);
 int rvc_phmm_batch_destroy(rvc_batch* b);
 
 /* ---- served: continuous batching ------------------------------------------- */

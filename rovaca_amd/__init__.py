@@ -32,7 +32,7 @@ EXPORTS = [
     "rvc_phmm_call_pairhmm", "rvc_phmm_batch_create", "rvc_phmm_batch_run", "rvc_phmm_batch_results",
     "rvc_phmm_batch_info", "rvc_phmm_batch_destroy", "rvc_phmm_submit", "rvc_phmm_wait", "rvc_phmm_poll",
     "rvc_phmm_server_stats", "rvc_phmm_preprocess_read", "rvc_phmm_postprocess", "rvc_phmm_tables",
-    "rvc_loadgen_run",
+    "rvc_loadgen_run", "rvc_phmm_batch_kernels",
 ]
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -70,6 +70,12 @@ class RvcBatchInfo(ctypes.Structure):
                 ("rescued_cells", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
                 ("fp32_ms", ctypes.c_double), ("fp64_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("n_fp32_launches", ctypes.c_uint32), ("n_fp64_launches", ctypes.c_uint32)]
+
+
+class RvcKernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("precision", ctypes.c_uint32), ("width", ctypes.c_uint32),
+                ("rows_per_lane", ctypes.c_uint32), ("occupancy", ctypes.c_uint32), ("tiles", ctypes.c_uint64),
+                ("cells", ctypes.c_uint64), ("ms", ctypes.c_double)]
 
 
 class RvcServerStats(ctypes.Structure):
@@ -114,6 +120,7 @@ def lib():
         L.rvc_phmm_batch_results.argtypes = [ctypes.c_void_p, _f64p, _u8p]
         L.rvc_phmm_batch_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(RvcBatchInfo)]
         L.rvc_phmm_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.rvc_phmm_batch_kernels.argtypes = [ctypes.c_void_p, ctypes.POINTER(RvcKernelTime), ctypes.c_uint32, _u32p]
         L.rvc_phmm_submit.argtypes = [ctypes.POINTER(RvcRegion), ctypes.POINTER(ctypes.c_uint64)]
         L.rvc_phmm_wait.argtypes = [ctypes.c_uint64, _f64p, _u8p, ctypes.c_int64, ctypes.c_void_p]
         L.rvc_phmm_poll.argtypes = [ctypes.c_uint64]
@@ -230,6 +237,19 @@ class Batch:
         inf = RvcBatchInfo()
         _check(lib().rvc_phmm_batch_info(self.h, ctypes.byref(inf)))
         return {k: getattr(inf, k) for k, _ in RvcBatchInfo._fields_}
+
+    def kernels(self):
+        """Per-launch device times of the last non-graph run."""
+        n = ctypes.c_uint32(0)
+        arr = (RvcKernelTime * 64)()
+        _check(lib().rvc_phmm_batch_kernels(self.h, arr, 64, ctypes.byref(n)))
+        out = []
+        for x in range(min(n.value, 64)):
+            k = arr[x]
+            out.append({"name": k.name.decode(), "precision": k.precision, "width": k.width,
+                        "rows_per_lane": k.rows_per_lane, "occupancy": k.occupancy, "tiles": k.tiles,
+                        "cells": k.cells, "ms": k.ms})
+        return out
 
     def results(self):
         n = int(self.offsets[-1])

@@ -238,6 +238,14 @@ int rvc_phmm_batch_info(const rvc_batch* b, rvc_batch_info* info) {
   return RVC_OK;
 }
 
+int rvc_phmm_batch_kernels(const rvc_batch* b, rvc_kernel_time* out, uint32_t max, uint32_t* n) {
+  if (!b || !n || (max && !out)) return set_error(RVC_E_INVALID, "null argument");
+  const auto& L = b->dev.launches;
+  *n = (uint32_t)L.size();
+  for (uint32_t x = 0; x < max && x < L.size(); ++x) out[x] = L[x];
+  return RVC_OK;
+}
+
 int rvc_phmm_batch_destroy(rvc_batch* b) {
   if (!b) return set_error(RVC_E_INVALID, "null batch");
   delete b;

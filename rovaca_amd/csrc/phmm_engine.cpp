@@ -350,6 +350,7 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
           P.fp64_cap[fp64_bin(rl)] += 1;
           P.n_pairs += 1;
           P.n_cells += (uint64_t)rl * hl;
+          P.variant_cells[v] += (uint64_t)rl * hl;
         }
       }
       sx += nseg;
@@ -434,6 +435,7 @@ int merge_host_packs(const HostPack* const* packs, uint32_t n, HostPack* out) {
         bins[v].push_back(tl);
       }
     for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_cap[b] += A.fp64_cap[b];
+    for (int v = 0; v < kFp32Variants; ++v) P.variant_cells[v] += A.variant_cells[v];
     P.n_pairs_slots += A.n_pairs_slots;
     P.n_pairs += A.n_pairs;
     P.n_cells += A.n_cells;
@@ -490,6 +492,8 @@ void DeviceBatch::release() {
   graph = nullptr;
   for (auto& e : ev) if (e) hipEventDestroy(e);
   for (auto& e : ev) e = nullptr;
+  for (auto& e : lev) if (e) hipEventDestroy(e);
+  lev.clear();
   if (own_stream && stream) hipStreamDestroy(stream);
   stream = nullptr;
   if (mem && own_mem) hipFree(mem);
@@ -577,6 +581,19 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
   if (timed) RVC_HIP(hipEventRecord(ev[0], s));
   n_fp32_launches = 0;
   n_fp64_launches = 0;
+  launches.clear();
+  auto mark = [&](int idx) -> int {
+    if (!timed) return RVC_OK;
+    while ((int)lev.size() <= idx) {
+      hipEvent_t e;
+      RVC_HIP(hipEventCreate(&e));
+      lev.push_back(e);
+    }
+    RVC_HIP(hipEventRecord(lev[idx], s));
+    return RVC_OK;
+  };
+  int rc = mark(0);
+  if (rc) return rc;
   for (int v = 0; v < kFp32Variants; ++v) {
     const uint32_t n = P.bin_start[v + 1] - P.bin_start[v];
     if (n == 0) continue;
@@ -599,6 +616,12 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     hipError_t e = launch_fp32(W, RPL, P.fold_gap, P.occ[v], grid, s, a);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP32 launch W=%d RPL=%d: %s", W, RPL, hipGetErrorString(e));
     ++n_fp32_launches;
+    rvc_kernel_time kt{};
+    snprintf(kt.name, sizeof kt.name, "k_fp32<%d,%d,%s,occ%d>", W, RPL, P.fold_gap ? "fold" : "plain", P.occ[v]);
+    kt.precision = 32; kt.width = W; kt.rows_per_lane = RPL; kt.occupancy = P.occ[v];
+    kt.tiles = n; kt.cells = P.variant_cells[v];
+    launches.push_back(kt);
+    if ((rc = mark((int)launches.size()))) return rc;
   }
   if (timed) RVC_HIP(hipEventRecord(ev[1], s));
   for (int b = 0; b < kNumFp64Bins; ++b) {
@@ -619,6 +642,12 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     hipError_t e = launch_fp64(b, grid, s, a);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP64 launch bin %d: %s", b, hipGetErrorString(e));
     ++n_fp64_launches;
+    rvc_kernel_time kt{};
+    snprintf(kt.name, sizeof kt.name, "k_fp64<%d,%d>", W, RPL);
+    kt.precision = 64; kt.width = W; kt.rows_per_lane = RPL; kt.occupancy = 0;
+    kt.tiles = P.fp64_cap[b];
+    launches.push_back(kt);
+    if ((rc = mark((int)launches.size()))) return rc;
   }
   if (timed) RVC_HIP(hipEventRecord(ev[2], s));
   return RVC_OK;
@@ -637,6 +666,7 @@ int DeviceBatch::run(bool use_graph_) {
     if (!graph_exec) {
       RVC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       rc = enqueue(stream, false);
+      launches.clear();
       hipGraph_t gr = nullptr;
       hipError_t e = hipStreamEndCapture(stream, &gr);
       if (rc) return rc;
@@ -662,6 +692,11 @@ int DeviceBatch::run(bool use_graph_) {
     fp32_ms = a;
     fp64_ms = b;
     total_ms = a + b;
+    for (size_t x = 0; x < launches.size(); ++x) {
+      float m = 0;
+      RVC_HIP(hipEventElapsedTime(&m, lev[x], lev[x + 1]));
+      launches[x].ms = m;
+    }
   }
   // rescued pair count of this run
   uint32_t cnt[kNumFp64Bins];

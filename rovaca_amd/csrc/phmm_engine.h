@@ -90,6 +90,7 @@ struct HostPack {
   bool fold_gap = true;             // every gap-continuation quality is > 0
   uint32_t waves_per_cu = 0;
   uint8_t occ[kFp32Variants] = {};  // occupancy target per FP32 variant
+  uint64_t variant_cells[kFp32Variants] = {};  // cells computed by each FP32 variant
   std::vector<double> nan_scratch;
   std::vector<uint8_t> flag_scratch;
 };
@@ -115,6 +116,9 @@ struct DeviceBatch {
   uint32_t* counters = nullptr;
   uint32_t n_counters_ = 0;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  // per-launch events of timed runs: launch x is bracketed by lev[x], lev[x+1]
+  std::vector<hipEvent_t> lev;
+  std::vector<rvc_kernel_time> launches;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   double fp32_ms = 0, fp64_ms = 0, total_ms = 0;
