@@ -141,7 +141,9 @@ typedef struct rvc_batch_opts {
                                     /* for the 16-lane kernels                        */
   uint32_t segment_width;           /* 0 = tuned choice, else 16/32/64 lanes per read */
                                     /* for reads up to 256 bases                      */
-  uint32_t reserved[2];
+  uint32_t prior_table;             /* 0 = tuned choice, 1 = prior-table kernel for   */
+                                    /* haplotypes without N, 2 = compare-and-select   */
+  uint32_t reserved[1];
 } rvc_batch_opts;
 
 typedef struct rvc_batch_info {

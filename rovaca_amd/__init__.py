@@ -62,7 +62,8 @@ class RvcConfig(ctypes.Structure):
 class RvcBatchOpts(ctypes.Structure):
     _fields_ = [("persistent_waves_per_cu", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
                 ("fp32_fold", ctypes.c_uint32), ("rpl_round", ctypes.c_uint32), ("occupancy", ctypes.c_uint32),
-                ("segment_width", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+                ("segment_width", ctypes.c_uint32), ("prior_table", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 1)]
 
 
 class RvcBatchInfo(ctypes.Structure):
@@ -211,7 +212,7 @@ class Batch:
     """Many regions packed, uploaded and launched together."""
 
     def __init__(self, regions, use_graph=False, fp32_fold=0, rpl_round=0, waves_per_cu=0, occupancy=0,
-                 segment_width=0):
+                 segment_width=0, prior_table=0):
         self._regions = regions
         arr = (RvcRegion * len(regions))()
         for x, r in enumerate(regions):
@@ -224,6 +225,7 @@ class Batch:
         opts.persistent_waves_per_cu = waves_per_cu
         opts.occupancy = occupancy
         opts.segment_width = segment_width
+        opts.prior_table = prior_table
         h = ctypes.c_void_p()
         _check(lib().rvc_phmm_batch_create(arr, len(regions), ctypes.byref(opts), ctypes.byref(h)))
         self.h = h

@@ -197,6 +197,8 @@ int rvc_phmm_batch_create(const rvc_region* regions, uint32_t n_regions, const r
     if (opts->occupancy) pol.force_occ = (uint8_t)std::min(4u, std::max(2u, opts->occupancy));
     if (opts->segment_width == 16 || opts->segment_width == 32 || opts->segment_width == 64)
       pol.force_w = (uint8_t)opts->segment_width;
+    if (opts->prior_table == 1) pol.prior_table = true;
+    if (opts->prior_table == 2) pol.prior_table = false;
   }
   int rc = build_host_pack(regions, n_regions, pol, &b->pack);
   if (rc) return rc;

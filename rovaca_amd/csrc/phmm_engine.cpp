@@ -175,6 +175,12 @@ int load_tune_table(const char* path, FP32Policy* pol) {
     while (cnt < 17 && sscanf(rest, "%d%n", &v[cnt], &used) == 1) { rest += used; ++cnt; }
     if (!strcmp(key, "occ16") && cnt == 17) {
       for (int x = 0; x < 17; ++x) p.occ16[x] = (uint8_t)std::min(4, std::max(2, v[x]));
+    } else if (!strcmp(key, "occpt16") && cnt == 17) {
+      for (int x = 0; x < 17; ++x) p.occpt16[x] = (uint8_t)std::min(4, std::max(2, v[x]));
+    } else if (!strcmp(key, "occpt32") && cnt == 17) {
+      for (int x = 0; x < 17; ++x) p.occpt32[x] = (uint8_t)std::min(4, std::max(2, v[x]));
+    } else if (!strcmp(key, "table") && cnt == 1) {
+      p.prior_table = v[0] != 0;
     } else if (!strcmp(key, "shape16") && cnt == 17) {
       for (int x = 0; x < 17; ++x) {
         const int w = v[x] >> 8, r = v[x] & 0xff;
@@ -286,20 +292,25 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
   uint32_t hcur = 0, rcur = 0;
   P.fp64_cap.assign(kNumFp64Bins, 0);
   P.waves_per_cu = pol.waves_per_cu;
-  for (int v = 0; v < kFp32Variants; ++v) {
+  P.prior_table = pol.prior_table;
+  for (int b = 0; b < kFp32Bins; ++b) {
     int W, RPL;
-    fp32_variant_shape(v, &W, &RPL);
-    P.occ[v] = (uint8_t)pol.occupancy(W, RPL);
+    fp32_variant_shape(b % kFp32Variants, &W, &RPL);
+    P.occ[b] = (uint8_t)pol.occupancy(W, RPL, b >= kFp32Variants);
   }
-  std::vector<std::vector<Tile>> bins(kFp32Variants);
+  std::vector<std::vector<Tile>> bins(kFp32Bins);
+  std::vector<uint8_t> hap_has_n;
   for (uint32_t r = 0; r < n_regions; ++r) {
     const rvc_region& g = regions[r];
     RegionLayout& L = P.region[r];
+    hap_has_n.assign(g.n_haps, 0);
     for (uint32_t h = 0; h < g.n_haps; ++h) {
       const uint32_t len = (uint32_t)(g.hap_off[h + 1] - g.hap_off[h]);
       P.hap_off[L.hap0 + h] = hcur;
       P.hap_len[L.hap0 + h] = len;
       pack_seq(g.hap_bases + g.hap_off[h], len, hcur, P.hap_pack, P.hap_nmask);
+      for (uint32_t w = hcur >> 5; w < (hcur + len + 31) >> 5; ++w)
+        if (P.hap_nmask[w]) hap_has_n[h] = 1;
       hcur += align_up(len, kSeqAlign);
     }
     // reads of a region in descending length, so that tiles group equal lengths
@@ -335,10 +346,11 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
       const uint32_t nseg_max = 64u / (uint32_t)W;
       uint32_t nseg = 0;
       while (nseg < nseg_max && sx + nseg < g.n_reads && P.read_len[L.read0 + sx + nseg] > 0) ++nseg;
-      const int v = fp32_variant_index(W, RPL);
+      const int v0 = fp32_variant_index(W, RPL);
       for (uint32_t h = 0; h < g.n_haps; ++h) {
         const uint32_t hl = P.hap_len[L.hap0 + h];
         if (hl == 0) continue;
+        const int v = v0 + (hap_has_n[h] ? 0 : kFp32Variants);
         Tile t;
         t.hap = L.hap0 + h;
         t.read0 = L.read0 + sx;
@@ -357,12 +369,12 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
     }
   }
   P.tiles.clear();
-  P.bin_start.assign(kFp32Variants + 1, 0);
-  for (int v = 0; v < kFp32Variants; ++v) {
+  P.bin_start.assign(kFp32Bins + 1, 0);
+  for (int v = 0; v < kFp32Bins; ++v) {
     P.bin_start[v] = (uint32_t)P.tiles.size();
     append_longest_first(bins[v], P.hap_len, P.tiles);
   }
-  P.bin_start[kFp32Variants] = (uint32_t)P.tiles.size();
+  P.bin_start[kFp32Bins] = (uint32_t)P.tiles.size();
   if (!pol.fold) P.fold_gap = false;
   P.fp64_base.assign(kNumFp64Bins + 1, 0);
   for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
@@ -392,7 +404,7 @@ int merge_host_packs(const HostPack* const* packs, uint32_t n, HostPack* out) {
   P.hap_pack.reserve(hw); P.hap_nmask.reserve(hn); P.read_pack.reserve(rw); P.read_nmask.reserve(rn);
   P.rq.reserve(rq); P.ri.reserve(rq); P.rd.reserve(rq); P.rc.reserve(rq);
   P.fp64_cap.assign(kNumFp64Bins, 0);
-  std::vector<std::vector<Tile>> bins(kFp32Variants);
+  std::vector<std::vector<Tile>> bins(kFp32Bins);
   for (uint32_t x = 0; x < n; ++x) {
     const HostPack& A = *packs[x];
     // every sequence block starts on a 32-base boundary, so words concatenate
@@ -426,7 +438,7 @@ int merge_host_packs(const HostPack* const* packs, uint32_t n, HostPack* out) {
       L.pair0 += pbase;
       P.region.push_back(L);
     }
-    for (int v = 0; v < kFp32Variants; ++v)
+    for (int v = 0; v < kFp32Bins; ++v)
       for (uint32_t t = A.bin_start[v]; t < A.bin_start[v + 1]; ++t) {
         Tile tl = A.tiles[t];
         tl.hap += hidx;
@@ -435,12 +447,13 @@ int merge_host_packs(const HostPack* const* packs, uint32_t n, HostPack* out) {
         bins[v].push_back(tl);
       }
     for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_cap[b] += A.fp64_cap[b];
-    for (int v = 0; v < kFp32Variants; ++v) P.variant_cells[v] += A.variant_cells[v];
+    for (int v = 0; v < kFp32Bins; ++v) P.variant_cells[v] += A.variant_cells[v];
     P.n_pairs_slots += A.n_pairs_slots;
     P.n_pairs += A.n_pairs;
     P.n_cells += A.n_cells;
     P.fold_gap = P.fold_gap && A.fold_gap;
     P.waves_per_cu = A.waves_per_cu;
+    P.prior_table = A.prior_table;
     std::memcpy(P.occ, A.occ, sizeof P.occ);
   }
   // spare words at the end, as build_host_pack leaves them
@@ -454,12 +467,12 @@ int merge_host_packs(const HostPack* const* packs, uint32_t n, HostPack* out) {
   P.rc.resize(P.rc.size() + 64, 0);
   P.n_regions = (uint32_t)P.region.size();
   P.tiles.reserve(tiles);
-  P.bin_start.assign(kFp32Variants + 1, 0);
-  for (int v = 0; v < kFp32Variants; ++v) {
+  P.bin_start.assign(kFp32Bins + 1, 0);
+  for (int v = 0; v < kFp32Bins; ++v) {
     P.bin_start[v] = (uint32_t)P.tiles.size();
     append_longest_first(bins[v], P.hap_len, P.tiles);
   }
-  P.bin_start[kFp32Variants] = (uint32_t)P.tiles.size();
+  P.bin_start[kFp32Bins] = (uint32_t)P.tiles.size();
   P.fp64_base.assign(kNumFp64Bins + 1, 0);
   for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
   return RVC_OK;
@@ -517,7 +530,7 @@ int DeviceBatch::upload(const HostPack& P, hipStream_t s, void* reuse_mem, size_
   const size_t o_flags = take(P.n_pairs_slots + 8);
   const size_t o_resc = take((size_t)P.fp64_base[kNumFp64Bins] * sizeof(RescueEntry) + sizeof(RescueEntry));
   const size_t o_rbase = take(kNumFp64Bins * sizeof(uint32_t));
-  const size_t n_counters = kFp32Variants + kNumFp64Bins + kNumFp64Bins;
+  const size_t n_counters = kFp32Bins + kNumFp64Bins + kNumFp64Bins;
   const size_t o_cnt = take(n_counters * sizeof(uint32_t));
   bytes = off;
   if (reuse_mem && reuse_bytes >= bytes) {
@@ -576,7 +589,7 @@ int DeviceBatch::upload(const HostPack& P, hipStream_t s, void* reuse_mem, size_
 int DeviceBatch::enqueue(hipStream_t s, bool timed) {
   const HostPack& P = *pack;
   uint32_t* heads = counters;
-  uint32_t* rescue_count = counters + kFp32Variants + kNumFp64Bins;
+  uint32_t* rescue_count = counters + kFp32Bins + kNumFp64Bins;
   RVC_HIP(launch_zero_u32(counters, n_counters_, s));
   if (timed) RVC_HIP(hipEventRecord(ev[0], s));
   n_fp32_launches = 0;
@@ -594,11 +607,12 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
   };
   int rc = mark(0);
   if (rc) return rc;
-  for (int v = 0; v < kFp32Variants; ++v) {
+  for (int v = 0; v < kFp32Bins; ++v) {
     const uint32_t n = P.bin_start[v + 1] - P.bin_start[v];
     if (n == 0) continue;
     int W, RPL;
-    fp32_variant_shape(v, &W, &RPL);
+    fp32_variant_shape(v % kFp32Variants, &W, &RPL);
+    const bool pt = v >= kFp32Variants && P.prior_table && P.fold_gap;
     Fp32Args a;
     a.s = seqs;
     a.t = g_engine.tables;
@@ -610,14 +624,16 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     a.rescue = rescue;
     a.rescue_count = rescue_count;
     a.rescue_base = rescue_base;
-    int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL, P.fold_gap, P.occ[v]));
+    const int occ = pt ? P.occ[v] : P.occ[v % kFp32Variants];
+    int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL, P.fold_gap, occ, pt));
     if (P.waves_per_cu) per_cu = std::min<int>(per_cu, (int)P.waves_per_cu);
     const int grid = (int)std::min<uint64_t>(n, (uint64_t)per_cu * g_engine.n_cu);
-    hipError_t e = launch_fp32(W, RPL, P.fold_gap, P.occ[v], grid, s, a);
+    hipError_t e = launch_fp32(W, RPL, P.fold_gap, occ, pt, grid, s, a);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP32 launch W=%d RPL=%d: %s", W, RPL, hipGetErrorString(e));
     ++n_fp32_launches;
     rvc_kernel_time kt{};
-    snprintf(kt.name, sizeof kt.name, "k_fp32<%d,%d,%s,occ%d>", W, RPL, P.fold_gap ? "fold" : "plain", P.occ[v]);
+    snprintf(kt.name, sizeof kt.name, "k_fp32%s<%d,%d,%s,occ%d>", pt ? "pt" : "", W, RPL,
+             P.fold_gap ? "fold" : "plain", occ);
     kt.precision = 32; kt.width = W; kt.rows_per_lane = RPL; kt.occupancy = P.occ[v];
     kt.tiles = n; kt.cells = P.variant_cells[v];
     launches.push_back(kt);
@@ -631,7 +647,7 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     a.t = g_engine.tables;
     a.list = rescue + P.fp64_base[b];
     a.count = rescue_count + b;
-    a.head = heads + kFp32Variants + b;
+    a.head = heads + kFp32Bins + b;
     a.out = out;
     a.flags = flags;
     const int per_cu = std::max(1, fp64_blocks_per_cu(b));
@@ -700,7 +716,7 @@ int DeviceBatch::run(bool use_graph_) {
   }
   // rescued pair count of this run
   uint32_t cnt[kNumFp64Bins];
-  RVC_HIP(hipMemcpy(cnt, counters + kFp32Variants + kNumFp64Bins, sizeof cnt, hipMemcpyDeviceToHost));
+  RVC_HIP(hipMemcpy(cnt, counters + kFp32Bins + kNumFp64Bins, sizeof cnt, hipMemcpyDeviceToHost));
   n_rescued = 0;
   for (int b = 0; b < kNumFp64Bins; ++b) n_rescued += cnt[b];
   return RVC_OK;

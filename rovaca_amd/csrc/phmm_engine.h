@@ -37,11 +37,14 @@ int engine_init(const rvc_config* cfg);
 int engine_shutdown();
 int validate_region(const rvc_region* r);
 
-// FP32 kernel variants: W in {16, 32, 64} x RPL in 1..16.
+// FP32 kernel shapes: W in {16, 32, 64} x RPL in 1..16. Tile bins are
+// [table][shape]: table = 1 for haplotypes without N (prior-table kernel
+// allowed), 0 otherwise.
 constexpr int kFp32Variants = 48;
+constexpr int kFp32Bins = 2 * kFp32Variants;
 int fp32_variant_index(int W, int RPL);
 void fp32_variant_shape(int v, int* W, int* RPL);
-int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ);
+int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ, bool pt);
 int fp64_blocks_per_cu(int bin);
 
 // Chooses the (W, RPL) kernel for a tile whose longest read has R bases.
@@ -55,10 +58,18 @@ struct FP32Policy {
   bool rpl_steps_of_3 = true;  // 16-lane shapes only in RPL 1, 2, 4, 7, 10, 13, 16
   bool fold = true;            // use the folded form when every gcp > 0
   uint32_t waves_per_cu = 0;   // persistent grid: 0 = occupancy limit
+  // the same for the prior-table kernels (W = 16 and W = 32)
+  uint8_t occpt16[17] = {4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 2, 2, 2, 2, 2, 2};
+  uint8_t occpt32[17] = {4, 4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 3, 2, 2, 2, 2};
+  bool prior_table = true;     // use the prior-table kernel where the haplotype has no N
   uint8_t force_occ = 0;       // batch option: 2/3/4 for every 16-lane kernel
   uint8_t force_w = 0;         // batch option: 16/32/64 lanes per read for reads <= 256
   void pick(uint32_t R, int* W, int* RPL) const;
-  int occupancy(int W, int RPL) const { return W == 16 ? (force_occ ? force_occ : occ16[RPL]) : 2; }
+  int occupancy(int W, int RPL, bool pt) const {
+    if (force_occ && (W == 16 || (pt && W == 32))) return force_occ;
+    if (pt) return W == 16 ? occpt16[RPL] : W == 32 ? occpt32[RPL] : 2;
+    return W == 16 ? occ16[RPL] : 2;
+  }
 };
 
 // The process-wide policy (built-in defaults, replaced by a tuned table at init).
@@ -80,8 +91,8 @@ struct HostPack {
   std::vector<uint32_t> read_pack, read_nmask, read_off, read_len;
   std::vector<uint8_t> rq, ri, rd, rc;
   std::vector<uint32_t> read_perm;  // device read slot -> input read index within its region
-  std::vector<Tile> tiles;          // grouped by FP32 variant
-  std::vector<uint32_t> bin_start;  // kFp32Variants + 1
+  std::vector<Tile> tiles;          // grouped by FP32 bin
+  std::vector<uint32_t> bin_start;  // kFp32Bins + 1
   std::vector<uint32_t> fp64_cap;   // pairs per FP64 bin (capacity of its rescue list)
   std::vector<uint32_t> fp64_base;  // kNumFp64Bins + 1
   uint64_t n_pairs_slots = 0;       // sum of n_haps * n_reads
@@ -89,8 +100,9 @@ struct HostPack {
   uint64_t n_cells = 0;
   bool fold_gap = true;             // every gap-continuation quality is > 0
   uint32_t waves_per_cu = 0;
-  uint8_t occ[kFp32Variants] = {};  // occupancy target per FP32 variant
-  uint64_t variant_cells[kFp32Variants] = {};  // cells computed by each FP32 variant
+  uint8_t occ[kFp32Bins] = {};            // occupancy target per FP32 bin
+  uint64_t variant_cells[kFp32Bins] = {};  // cells computed by each FP32 bin
+  bool prior_table = true;                 // launch table bins with the prior-table kernel
   std::vector<double> nan_scratch;
   std::vector<uint8_t> flag_scratch;
 };

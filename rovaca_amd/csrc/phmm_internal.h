@@ -104,7 +104,8 @@ struct Fp64Args {
 
 // Kernel launchers (phmm_kernels.hip).
 bool fp32_supported(int W, int RPL);
-hipError_t launch_fp32(int W, int RPL, bool xf, int occ, int grid, hipStream_t stream, const Fp32Args& a);
+hipError_t launch_fp32(int W, int RPL, bool xf, int occ, bool pt, int grid, hipStream_t stream,
+                       const Fp32Args& a);
 hipError_t launch_fp64(int bin, int grid, hipStream_t stream, const Fp64Args& a);
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
 

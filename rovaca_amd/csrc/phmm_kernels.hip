@@ -232,6 +232,21 @@ __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, cons
   return S;
 }
 
+// Result of one FP32 segment: log10 likelihood, or a place on the FP64 list.
+__device__ __forceinline__ void finish_fp32(const Fp32Args& a, const Tile& tl, int seg, uint32_t read, float S) {
+  const uint32_t o = tl.out0 + (uint32_t)seg * (tl.meta >> 4);
+  if (S >= 1e-28f) {
+    a.out[o] = log10((double)S) - (double)36.1236000061f;
+    a.flags[o] = 1;
+  } else {
+    const int b = fp64_bin(a.s.read_len[read]);
+    const uint32_t idx = atomicAdd(a.rescue_count + b, 1u);
+    RescueEntry e;
+    e.hap = tl.hap; e.read = read; e.out = o; e.pad = 0;
+    a.rescue[a.rescue_base[b] + idx] = e;
+  }
+}
+
 // FP32 pass: persistent waves pull tiles (one haplotype x up to 64/W reads).
 // OCC = waves per SIMD the register allocation must allow (512 / OCC VGPRs
 // per lane): the sweep is issue-latency bound, so occupancy is a tuning knob.
@@ -256,20 +271,194 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     stage_hap(a.s, tl.hap, H, hbuf + 64, lane, 64);
     __syncthreads();
     const float S = forward<float, W, RPL, XF>(a.s, a.t, hbuf + 64, k, H, H, read);
-    if (active && k == W - 1) {
-      const uint32_t o = tl.out0 + (uint32_t)seg * stride;
-      if (S >= 1e-28f) {
-        a.out[o] = log10((double)S) - (double)36.1236000061f;
-        a.flags[o] = 1;
-      } else {
-        const int b = fp64_bin(a.s.read_len[read]);
-        const uint32_t idx = atomicAdd(a.rescue_count + b, 1u);
-        RescueEntry e;
-        e.hap = tl.hap; e.read = read; e.out = o; e.pad = 0;
-        a.rescue[a.rescue_base[b] + idx] = e;
+    if (active && k == W - 1) finish_fp32(a, tl, seg, read, S);
+    (void)NSEG;
+    (void)stride;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Prior-table variant of the FP32 pass, for haplotypes without N.
+//
+// The prior of a cell depends only on its read row and on the haplotype base
+// (4 values), so each tile first writes, per lane and row, the prior for each
+// of the 4 bases into LDS as ptab[base][row/4][lane] (float4 = 4 rows). A step
+// then reads the 4-row groups of its column's base with one ds_read_b128 each,
+// one step ahead, instead of a compare and a select per cell. The haplotype
+// itself sits in LDS as its 2-bit stream (16 bases per word, 64 zero columns
+// in front); a funnel shift yields the codes of 4 consecutive columns.
+// ---------------------------------------------------------------------------
+constexpr int kHapWords = 112;  // 64 pad columns + 1536 bases + tail, 16 per word
+
+template <int RPL, bool XF>
+__device__ __forceinline__ void sweep_step_pt(const float (&M)[RPL], const float (&I)[RPL], const float (&D)[RPL],
+                                              float (&nM)[RPL], float (&nI)[RPL], float (&nD)[RPL], float dM, float dI,
+                                              float dD, float uM, float uI, const float* pr, const float (&tmm)[RPL],
+                                              const float (&tgm)[RPL], const float (&tmi)[RPL],
+                                              const float (&tii)[RPL], const float (&tmd)[RPL],
+                                              const float (&tdd)[RPL]) {
+  if constexpr (XF) nM[0] = pr[0] * fmaf(tmm[0], dM, dI + dD);
+  else nM[0] = pr[0] * fmaf(tmm[0], dM, tgm[0] * (dI + dD));
+  nI[0] = fmaf(tmi[0], uM, tii[0] * uI);
+  nD[0] = fmaf(tmd[0], M[0], tdd[0] * D[0]);
+#pragma unroll
+  for (int r = 1; r < RPL; ++r) {
+    if constexpr (XF) nM[r] = pr[r] * fmaf(tmm[r], M[r - 1], I[r - 1] + D[r - 1]);
+    else nM[r] = pr[r] * fmaf(tmm[r], M[r - 1], tgm[r] * (I[r - 1] + D[r - 1]));
+    nI[r] = fmaf(tmi[r], nM[r - 1], tii[r] * nI[r - 1]);
+    nD[r] = fmaf(tmd[r], M[r], tdd[r] * D[r]);
+  }
+}
+
+template <int W, int RPL, bool XF, int OCC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void k_fp32pt(Fp32Args a) {
+  constexpr int NQ = (RPL + 3) / 4;
+  __shared__ float4 ptab[4 * NQ * 64];
+  __shared__ uint32_t hpk[kHapWords];
+  const int lane = threadIdx.x;
+  const int seg = lane / W, k = lane % W;
+  const float* ph2pr = a.t.ph2pr_f;
+  const float* mmt = a.t.mm_f;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(a.head, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= a.n_tiles) break;
+    const Tile tl = a.tiles[t];
+    const int nseg = (int)(tl.meta & 0xfu);
+    const bool active = seg < nseg;
+    const uint32_t read = tl.read0 + (active ? (uint32_t)seg : 0u);
+    const uint32_t H = a.s.hap_len[tl.hap];
+    __syncthreads();
+    {
+      const uint32_t w0 = a.s.hap_off[tl.hap] >> 4, nw = (H + 15) >> 4;
+      for (int x = lane; x < kHapWords; x += 64) {
+        const int hx = x - 4;
+        hpk[x] = (hx >= 0 && hx < (int)nw) ? a.s.hap_pack[w0 + hx] : 0u;
       }
     }
-    (void)NSEG;
+    // per-row transition parameters (registers) and priors (LDS table)
+    const uint32_t R = a.s.read_len[read];
+    const uint32_t roff = a.s.read_off[read];
+    float tmm[RPL], tgm[RPL], tmi[RPL], tii[RPL], tmd[RPL], tdd[RPL];
+    float pm[NQ * 4], px[NQ * 4];
+    int rc[NQ * 4];
+#pragma unroll
+    for (int r = 0; r < NQ * 4; ++r) {
+      const uint32_t row = (uint32_t)k * RPL + r;
+      pm[r] = 0.f; px[r] = 0.f; rc[r] = 0;
+      if (r >= RPL) continue;
+      if (row < R) {
+        const uint32_t g = roff + row;
+        const int iq = a.s.rq[g] & 127, ii = a.s.ri[g] & 127, id = a.s.rd[g] & 127, ic = a.s.rc[g] & 127;
+        const uint32_t base = (a.s.read_pack[g >> 4] >> ((g & 15u) * 2u)) & 3u;
+        const uint32_t isn = (a.s.read_nmask[g >> 5] >> (g & 31u)) & 1u;
+        const float dist = ph2pr[iq];
+        pm[r] = 1.f - dist;
+        px[r] = isn ? pm[r] : dist * (1.f / 3.f);
+        rc[r] = isn ? -1 : (int)base;
+        const float gc = ph2pr[ic];
+        const int mx = ii > id ? ii : id, mn = ii > id ? id : ii;
+        tmm[r] = mmt[((mx * (mx + 1)) >> 1) + mn];
+        tgm[r] = 1.f - gc;
+        if constexpr (XF) {
+          const float gnext = (row + 1 < R) ? 1.f - ph2pr[a.s.rc[g + 1] & 127] : 1.f;
+          tmi[r] = gnext * ph2pr[ii];
+          tii[r] = gnext * gc / tgm[r];
+          tmd[r] = gnext * ph2pr[id];
+          tdd[r] = gc;
+        } else {
+          tmi[r] = ph2pr[ii];
+          tii[r] = gc;
+          tmd[r] = ph2pr[id];
+          tdd[r] = gc;
+        }
+      } else {
+        // padding row: M = 0, I carries M + I of the row above, D = 0
+        tmm[r] = 0.f; tgm[r] = 0.f; tmi[r] = 1.f; tii[r] = 1.f; tmd[r] = 0.f; tdd[r] = 1.f;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        float4 v;
+        v.x = (rc[4 * q + 0] < 0 || rc[4 * q + 0] == c) ? pm[4 * q + 0] : px[4 * q + 0];
+        v.y = (rc[4 * q + 1] < 0 || rc[4 * q + 1] == c) ? pm[4 * q + 1] : px[4 * q + 1];
+        v.z = (rc[4 * q + 2] < 0 || rc[4 * q + 2] == c) ? pm[4 * q + 2] : px[4 * q + 2];
+        v.w = (rc[4 * q + 3] < 0 || rc[4 * q + 3] == c) ? pm[4 * q + 3] : px[4 * q + 3];
+        ptab[(c * NQ + q) * 64 + lane] = v;
+      }
+    __syncthreads();
+
+    float initD = 0x1p120f / (float)H;
+    if constexpr (XF) initD = (1.f - ph2pr[a.s.rc[roff] & 127]) * initD;
+    const float old_d = initD;
+    float Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) { Ma[r] = 0.f; Ia[r] = 0.f; Da[r] = 0.f; }
+    float uM = 0.f, uI = 0.f, uD = (k == 0) ? initD : 0.f;
+    float dM = 0.f, dI = 0.f, dD = uD;
+    float S = 0.f;
+    const int nsteps = (int)H + W - 1;
+    const char* tab = reinterpret_cast<const char*>(ptab) + lane * 16;
+    // codes of columns st-k .. st-k+3 (stream position p = column + 64 >= 1)
+    auto window = [&](int st) -> uint32_t {
+      const int p = st - k + 64;
+      return __builtin_amdgcn_alignbit(hpk[(p >> 4) + 1], hpk[p >> 4], (uint32_t)(p & 15) * 2u);
+    };
+    auto load = [&](uint32_t codes, int i, float (&pr)[NQ * 4]) {
+      const uint32_t c = (codes >> (2 * i)) & 3u;
+      const float4* b = reinterpret_cast<const float4*>(tab + c * (uint32_t)(NQ * 1024));
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const float4 v = b[q * 64];
+        pr[4 * q + 0] = v.x; pr[4 * q + 1] = v.y; pr[4 * q + 2] = v.z; pr[4 * q + 3] = v.w;
+      }
+    };
+    float pA[NQ * 4], pB[NQ * 4];
+    uint32_t cw = window(0);
+    load(cw, 0, pA);
+    // sched_barrier(0) keeps each step's table reads ahead of the previous
+    // step's arithmetic, so their latency overlaps a whole step.
+    for (int st = 0; st < nsteps; st += 4) {
+      const uint32_t cw_next = window(st + 4);
+      // step st: a -> b
+      load(cw, 1, pB);
+      __builtin_amdgcn_sched_barrier(0);
+      sweep_step_pt<RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
+      if (st < nsteps) S += Mb[RPL - 1] + Ib[RPL - 1];
+      const float s1M = lane_down<W>(Mb[RPL - 1], 0.f, k);
+      const float s1I = lane_down<W>(Ib[RPL - 1], 0.f, k);
+      const float s1D = lane_down<W>(Db[RPL - 1], old_d, k);
+      // step st+1: b -> a
+      load(cw, 2, pA);
+      __builtin_amdgcn_sched_barrier(0);
+      sweep_step_pt<RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, pB, tmm, tgm, tmi, tii, tmd, tdd);
+      if (st + 1 < nsteps) S += Ma[RPL - 1] + Ia[RPL - 1];
+      const float s2M = lane_down<W>(Ma[RPL - 1], 0.f, k);
+      const float s2I = lane_down<W>(Ia[RPL - 1], 0.f, k);
+      const float s2D = lane_down<W>(Da[RPL - 1], old_d, k);
+      // step st+2: a -> b
+      load(cw, 3, pB);
+      __builtin_amdgcn_sched_barrier(0);
+      sweep_step_pt<RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, pA, tmm, tgm, tmi, tii, tmd, tdd);
+      if (st + 2 < nsteps) S += Mb[RPL - 1] + Ib[RPL - 1];
+      const float s3M = lane_down<W>(Mb[RPL - 1], 0.f, k);
+      const float s3I = lane_down<W>(Ib[RPL - 1], 0.f, k);
+      const float s3D = lane_down<W>(Db[RPL - 1], old_d, k);
+      // step st+3: b -> a
+      cw = cw_next;
+      load(cw, 0, pA);
+      __builtin_amdgcn_sched_barrier(0);
+      sweep_step_pt<RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, pB, tmm, tgm, tmi, tii, tmd, tdd);
+      if (st + 3 < nsteps) S += Ma[RPL - 1] + Ia[RPL - 1];
+      dM = s3M; dI = s3I; dD = s3D;
+      uM = lane_down<W>(Ma[RPL - 1], 0.f, k);
+      uI = lane_down<W>(Ia[RPL - 1], 0.f, k);
+      uD = lane_down<W>(Da[RPL - 1], old_d, k);
+    }
+    if (active && k == W - 1) finish_fp32(a, tl, seg, read, S);
   }
 }
 
@@ -319,6 +508,10 @@ template <int W, bool XF, int OCC, int... R>
 constexpr auto fp32_row(std::integer_sequence<int, R...>) {
   return std::array<Fp32Fn, sizeof...(R)>{{&k_fp32<W, R + 1, XF, OCC>...}};
 }
+template <int W, int OCC, int... R>
+constexpr auto fp32pt_row(std::integer_sequence<int, R...>) {
+  return std::array<Fp32Fn, sizeof...(R)>{{&k_fp32pt<W, R + 1, true, OCC>...}};
+}
 
 // FP32 variants: W = 16 with 1..16 rows per lane at occupancy targets 2, 3
 // and 4 waves per SIMD; W = 32 (1..16) and W = 64 (1..12) at 2 waves; each
@@ -338,7 +531,27 @@ const std::array<Fp32Fn, 12> kFp32W64[2] = {fp32_row<64, false, 2>(std::make_int
 
 const std::array<Fp64Fn, kNumFp64Bins> kFp64 = {{&k_fp64<16, 4>, &k_fp64<16, 8>, &k_fp64<32, 8>, &k_fp64<64, 12>}};
 
-Fp32Fn fp32_fn(int W, int RPL, bool xf, int occ) {
+// Prior-table variants (folded form only): W = 16 and 32 at 2, 3, 4 waves per
+// SIMD, W = 64 at 2.
+const Row16 kFp32PtW16[3] = {fp32pt_row<16, 2>(std::make_integer_sequence<int, 16>{}),
+                             fp32pt_row<16, 3>(std::make_integer_sequence<int, 16>{}),
+                             fp32pt_row<16, 4>(std::make_integer_sequence<int, 16>{})};
+const Row16 kFp32PtW32[3] = {fp32pt_row<32, 2>(std::make_integer_sequence<int, 16>{}),
+                             fp32pt_row<32, 3>(std::make_integer_sequence<int, 16>{}),
+                             fp32pt_row<32, 4>(std::make_integer_sequence<int, 16>{})};
+const std::array<Fp32Fn, 12> kFp32PtW64 = fp32pt_row<64, 2>(std::make_integer_sequence<int, 12>{});
+
+Fp32Fn fp32pt_fn(int W, int RPL, int occ) {
+  if (RPL < 1) return nullptr;
+  const int o = std::min(4, std::max(2, occ)) - 2;
+  if (W == 16 && RPL <= 16) return kFp32PtW16[o][RPL - 1];
+  if (W == 32 && RPL <= 16) return kFp32PtW32[o][RPL - 1];
+  if (W == 64 && RPL <= 12) return kFp32PtW64[RPL - 1];
+  return nullptr;
+}
+
+Fp32Fn fp32_fn(int W, int RPL, bool xf, int occ, bool pt) {
+  if (pt) return xf ? fp32pt_fn(W, RPL, occ) : nullptr;
   if (RPL < 1) return nullptr;
   if (W == 16 && RPL <= 16) return kFp32W16[xf][std::min(4, std::max(2, occ)) - 2][RPL - 1];
   if (W == 32 && RPL <= 16) return kFp32W32[xf][RPL - 1];
@@ -347,10 +560,10 @@ Fp32Fn fp32_fn(int W, int RPL, bool xf, int occ) {
 }
 }  // namespace
 
-bool fp32_supported(int W, int RPL) { return fp32_fn(W, RPL, false, 2) != nullptr; }
+bool fp32_supported(int W, int RPL) { return fp32_fn(W, RPL, false, 2, false) != nullptr; }
 
-hipError_t launch_fp32(int W, int RPL, bool xf, int occ, int grid, hipStream_t stream, const Fp32Args& a) {
-  Fp32Fn f = fp32_fn(W, RPL, xf, occ);
+hipError_t launch_fp32(int W, int RPL, bool xf, int occ, bool pt, int grid, hipStream_t stream, const Fp32Args& a) {
+  Fp32Fn f = fp32_fn(W, RPL, xf, occ, pt);
   if (!f) return hipErrorInvalidValue;
   hipLaunchKernelGGL(f, dim3(grid), dim3(64), 0, stream, a);
   return hipGetLastError();
@@ -373,18 +586,19 @@ hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream) {
 namespace rvc {
 
 // Resident 64-lane blocks per CU for a kernel (register and LDS limited).
-int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ) {
-  static int cache[2][3][3][17] = {};
+int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ, bool pt) {
+  static int cache[2][2][3][3][17] = {};
   const int w = W == 16 ? 0 : W == 32 ? 1 : 2;
   const int o = std::min(4, std::max(2, occ)) - 2;
   if (RPL < 1 || RPL > 16) return 1;
-  if (cache[xf][o][w][RPL] == 0) {
+  int& slot = cache[pt][xf][o][w][RPL];
+  if (slot == 0) {
     int n = 0;
-    Fp32Fn f = fp32_fn(W, RPL, xf, occ);
+    Fp32Fn f = fp32_fn(W, RPL, xf, occ, pt);
     if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64, 0) != hipSuccess || n < 1) n = 1;
-    cache[xf][o][w][RPL] = n;
+    slot = n;
   }
-  return cache[xf][o][w][RPL];
+  return slot;
 }
 
 int fp64_blocks_per_cu(int bin) {

@@ -1,12 +1,15 @@
 """Tile-shape tuner for the FP32 kernel.
 
 For every read length class of the 16-lane kernel (RPL = 1..16 rows per
-lane, reads of 16*RPL bases) it times each candidate shape on the device --
-16 lanes per read at 2, 3 and 4 waves per SIMD, and 32 or 64 lanes per read --
-on a fixed synthetic batch, and writes the fastest as a table that the engine
-loads at init (tuned/mi355x.txt beside the library):
+lane, reads of 16*RPL bases) it times, on a fixed synthetic batch, the
+compare-and-select kernel at 2, 3 and 4 waves per SIMD and the prior-table
+kernel with 16, 32 and 64 lanes per read at each occupancy, and writes the
+fastest as a table that the engine loads at init (tuned/mi355x.txt beside the
+library):
 
-    occ16   <waves per SIMD of the 16-lane kernel, index = RPL 0..16>
+    occ16   <waves per SIMD, compare-and-select 16-lane kernel, index = RPL 0..16>
+    occpt16 <the same for the prior-table 16-lane kernel>
+    occpt32 <the same for the prior-table 32-lane kernel, index = its own RPL>
     shape16 <0 = 16 lanes, else W*256 + RPL, index = RPL 0..16>
 
 Usage: python -m rovaca_amd.tune [--out PATH] [--regions N]
@@ -53,48 +56,65 @@ def main():
     pool = base_pool(args.regions)
     print(f"[tune] pool of {len(pool)} regions in {time.time() - t0:.1f} s", flush=True)
     rv.init(0, tune_table=os.devnull)  # start from the built-in defaults
-    occ16 = [4] * 17
+    occ16, occpt16, occpt32 = [4] * 17, [4] * 17, [4] * 17
     shape16 = [0] * 17
     report = []
+
+    def time_batch(regs, cells, **kw):
+        b = rv.Batch(regs, rpl_round=1, **kw)
+        b.run()
+        ms = min((b.run(), b.info()["fp32_ms"])[1] for _ in range(args.repeats))
+        b.close()
+        return ms, cells / (ms * 1e-3) / 1e9
+
     for rpl in range(1, 17):
         R = 16 * rpl
         regs = truncated(pool, R)
         cells = workload.cells(regs)
-        cands = [(16, o) for o in (2, 3, 4)]
-        for w in (32, 64):
-            if -(-R // w) <= (16 if w == 32 else 12):
-                cands.append((w, 2))
-        best = None
+        # compare-and-select kernels (used for haplotypes that contain N): occupancy of the 16-lane shape
         best16 = None
+        for occ in (2, 3, 4):
+            ms, g = time_batch(regs, cells, occupancy=occ, segment_width=16, prior_table=2)
+            report.append((rpl, "cmp", 16, occ, ms, g))
+            print(f"[tune] R={R:3d} cmp W=16 occ={occ}: {ms:8.3f} ms {g:8.1f} GCUPS", flush=True)
+            if best16 is None or ms < best16[1]:
+                best16 = (occ, ms)
+        occ16[rpl] = best16[0]
+        # prior-table kernels (haplotypes without N): shape and occupancy
+        best, bpt = None, {16: None, 32: None}
+        cands = [(16, o) for o in (2, 3, 4)]
+        if -(-R // 32) <= 16:
+            cands += [(32, o) for o in (2, 3, 4)]
+        if -(-R // 64) <= 12:
+            cands.append((64, 2))
         for w, occ in cands:
-            b = rv.Batch(regs, rpl_round=1, occupancy=occ, segment_width=w)
-            b.run()
-            ms = min((b.run(), b.info()["fp32_ms"])[1] for _ in range(args.repeats))
-            b.close()
-            gcups = cells / (ms * 1e-3) / 1e9
-            report.append((rpl, w, occ, ms, gcups))
-            print(f"[tune] R={R:3d} W={w:2d} occ={occ}: {ms:8.3f} ms {gcups:8.1f} GCUPS", flush=True)
+            ms, g = time_batch(regs, cells, occupancy=occ, segment_width=w, prior_table=1)
+            report.append((rpl, "table", w, occ, ms, g))
+            print(f"[tune] R={R:3d} table W={w:2d} occ={occ}: {ms:8.3f} ms {g:8.1f} GCUPS", flush=True)
             if best is None or ms < best[2]:
                 best = (w, occ, ms)
-            if w == 16 and (best16 is None or ms < best16[1]):
-                best16 = (occ, ms)
-        # the 16-lane occupancy is kept even where a wider shape wins, because
-        # reads of other lengths can round up to this row count
-        occ16[rpl] = best16[0]
+            if w in bpt and (bpt[w] is None or ms < bpt[w][1]):
+                bpt[w] = (occ, ms)
+        occpt16[rpl] = bpt[16][0]
+        # the 32-lane table kernel of this read length has ceil(R/32) rows per lane
+        if bpt[32] is not None:
+            occpt32[-(-R // 32)] = bpt[32][0]
         w, occ, _ = best
         if w != 16:
             shape16[rpl] = w * 256 + (-(-R // w))
-    occ16[0] = occ16[1]
+    occ16[0], occpt16[0], occpt32[0] = occ16[1], occpt16[1], occpt32[1]
     rv.shutdown()
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     with open(args.out, "w") as f:
         f.write(f"# FP32 kernel shapes tuned by rovaca_amd/tune.py on {datetime.date.today()}\n")
         f.write(f"# batch: {args.regions} regions x 64 haplotypes (~350 bp) x 256 reads of 16*RPL bases\n")
-        for rpl, w, occ, ms, g in report:
-            f.write(f"# RPL {rpl:2d} W {w:2d} occ {occ}: {ms:.3f} ms, {g:.1f} GCUPS\n")
+        for rpl, kind, w, occ, ms, g in report:
+            f.write(f"# RPL {rpl:2d} {kind:5s} W {w:2d} occ {occ}: {ms:.3f} ms, {g:.1f} GCUPS\n")
         f.write("occ16 " + " ".join(str(x) for x in occ16) + "\n")
+        f.write("occpt16 " + " ".join(str(x) for x in occpt16) + "\n")
+        f.write("occpt32 " + " ".join(str(x) for x in occpt32) + "\n")
         f.write("shape16 " + " ".join(str(x) for x in shape16) + "\n")
-        f.write("steps3 1\nfold 1\n")
+        f.write("steps3 1\nfold 1\ntable 1\n")
     print(f"[tune] wrote {args.out}", flush=True)
 
 
