@@ -35,6 +35,7 @@ struct SyncWorkspace {
   void* mem = nullptr;
   size_t bytes = 0;
   HostPack pack;
+  PinnedBuf staging, results;
   DeviceBatch batch;
 };
 std::mutex g_ws_mu;
@@ -68,6 +69,8 @@ void release_workspaces() {
   std::lock_guard<std::mutex> lk(g_ws_mu);
   for (SyncWorkspace* w : g_ws_all) {
     w->batch.release();
+    w->staging.release();
+    w->results.release();
     if (w->mem) hipFree(w->mem);
     if (w->stream) hipStreamDestroy(w->stream);
     delete w;
@@ -92,15 +95,28 @@ int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags) {
   if ((rc = build_host_pack(region, 1, policy(), &w->pack))) return rc;
   w->batch.graph_exec = nullptr;
   w->batch.graph = nullptr;
-  if ((rc = w->batch.upload(w->pack, w->stream, w->mem, w->bytes))) return rc;
+  const BatchLayout L = batch_layout(pack_sizes(w->pack));
+  if ((rc = w->staging.reserve(L.in_bytes))) return rc;
+  stage_pack(w->pack, L, w->staging.ptr);
+  if ((rc = w->batch.upload_staged(w->pack, L, w->staging.ptr, w->stream, w->mem, w->bytes))) return rc;
   if (w->batch.own_mem) {  // the workspace was too small: keep the new allocation
     if (w->mem) hipFree(w->mem);
     w->mem = w->batch.mem;
     w->bytes = w->batch.bytes;
     w->batch.own_mem = false;
   }
-  if ((rc = w->batch.run(false))) return rc;
-  return w->batch.download(out, flags, w->stream);
+  // kernels, then one copy of results and flags into pinned memory, one wait
+  if ((rc = w->batch.enqueue(w->stream, false))) return rc;
+  const size_t n = w->pack.n_pairs_slots;
+  if ((rc = w->results.reserve(n * (sizeof(double) + 1) + 64))) return rc;
+  double* h_out = reinterpret_cast<double*>(w->results.ptr);
+  uint8_t* h_flags = reinterpret_cast<uint8_t*>(w->results.ptr + n * sizeof(double));
+  hipError_t e = hipMemcpyAsync(h_out, w->batch.out, n * sizeof(double), hipMemcpyDeviceToHost, w->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(h_flags, w->batch.flags, n, hipMemcpyDeviceToHost, w->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+  if (e != hipSuccess) return set_error(RVC_E_DEVICE, "region results: %s", hipGetErrorString(e));
+  unpermute_results(w->pack, h_out, h_flags, out, flags);
+  return RVC_OK;
 }
 }  // namespace rvc
 
@@ -207,6 +223,7 @@ int rvc_phmm_batch_create(const rvc_region* regions, uint32_t n_regions, const r
   b->dev.own_stream = true;
   if ((rc = b->dev.upload(b->pack, b->dev.stream, nullptr, 0))) return rc;
   if (hipStreamSynchronize(b->dev.stream) != hipSuccess) return set_error(RVC_E_DEVICE, "batch upload");
+  std::vector<char>().swap(b->pack.staging);
   b->use_graph = opts && opts->use_graph;
   *out = guard.release();
   return RVC_OK;

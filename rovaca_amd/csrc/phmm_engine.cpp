@@ -381,103 +381,6 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
   return RVC_OK;
 }
 
-int merge_host_packs(const HostPack* const* packs, uint32_t n, HostPack* out) {
-  HostPack& P = *out;
-  P = HostPack();
-  uint64_t nh = 0, nr = 0, hw = 0, hn = 0, rw = 0, rn = 0, rq = 0, pairs = 0, tiles = 0;
-  for (uint32_t x = 0; x < n; ++x) {
-    const HostPack& A = *packs[x];
-    nh += A.hap_off.size();
-    nr += A.read_off.size();
-    hw += A.hap_pack.size();
-    hn += A.hap_nmask.size();
-    rw += A.read_pack.size();
-    rn += A.read_nmask.size();
-    rq += A.rq.size();
-    pairs += A.n_pairs_slots;
-    tiles += A.tiles.size();
-  }
-  if (hw * 16 > 0xffffffffull || rq > 0xffffffffull || pairs > 0xffffffffull)
-    return set_error(RVC_E_LIMIT, "merged batch too large for 32-bit offsets");
-  P.hap_off.reserve(nh); P.hap_len.reserve(nh);
-  P.read_off.reserve(nr); P.read_len.reserve(nr); P.read_perm.reserve(nr);
-  P.hap_pack.reserve(hw); P.hap_nmask.reserve(hn); P.read_pack.reserve(rw); P.read_nmask.reserve(rn);
-  P.rq.reserve(rq); P.ri.reserve(rq); P.rd.reserve(rq); P.rc.reserve(rq);
-  P.fp64_cap.assign(kNumFp64Bins, 0);
-  std::vector<std::vector<Tile>> bins(kFp32Bins);
-  for (uint32_t x = 0; x < n; ++x) {
-    const HostPack& A = *packs[x];
-    // every sequence block starts on a 32-base boundary, so words concatenate
-    const uint32_t hbase = (uint32_t)P.hap_pack.size() * 16, rbase = (uint32_t)P.rq.size();
-    const uint32_t hidx = (uint32_t)P.hap_off.size(), ridx = (uint32_t)P.read_off.size();
-    const uint64_t pbase = P.n_pairs_slots;
-    // trim the two spare words of each block so base offsets stay multiples of 32
-    const size_t hwords = (A.hap_pack.size() - 2) & ~size_t(1), rwords = (A.read_pack.size() - 2) & ~size_t(1);
-    P.hap_pack.insert(P.hap_pack.end(), A.hap_pack.begin(), A.hap_pack.begin() + hwords);
-    P.hap_nmask.insert(P.hap_nmask.end(), A.hap_nmask.begin(), A.hap_nmask.begin() + hwords / 2);
-    P.read_pack.insert(P.read_pack.end(), A.read_pack.begin(), A.read_pack.begin() + rwords);
-    P.read_nmask.insert(P.read_nmask.end(), A.read_nmask.begin(), A.read_nmask.begin() + rwords / 2);
-    const size_t rbytes = rwords * 16;
-    P.rq.insert(P.rq.end(), A.rq.begin(), A.rq.begin() + rbytes);
-    P.ri.insert(P.ri.end(), A.ri.begin(), A.ri.begin() + rbytes);
-    P.rd.insert(P.rd.end(), A.rd.begin(), A.rd.begin() + rbytes);
-    P.rc.insert(P.rc.end(), A.rc.begin(), A.rc.begin() + rbytes);
-    for (size_t h = 0; h < A.hap_off.size(); ++h) {
-      P.hap_off.push_back(A.hap_off[h] + hbase);
-      P.hap_len.push_back(A.hap_len[h]);
-    }
-    for (size_t r = 0; r < A.read_off.size(); ++r) {
-      P.read_off.push_back(A.read_off[r] + rbase);
-      P.read_len.push_back(A.read_len[r]);
-      P.read_perm.push_back(A.read_perm[r]);
-    }
-    for (const RegionLayout& L0 : A.region) {
-      RegionLayout L = L0;
-      L.hap0 += hidx;
-      L.read0 += ridx;
-      L.pair0 += pbase;
-      P.region.push_back(L);
-    }
-    for (int v = 0; v < kFp32Bins; ++v)
-      for (uint32_t t = A.bin_start[v]; t < A.bin_start[v + 1]; ++t) {
-        Tile tl = A.tiles[t];
-        tl.hap += hidx;
-        tl.read0 += ridx;
-        tl.out0 += (uint32_t)pbase;
-        bins[v].push_back(tl);
-      }
-    for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_cap[b] += A.fp64_cap[b];
-    for (int v = 0; v < kFp32Bins; ++v) P.variant_cells[v] += A.variant_cells[v];
-    P.n_pairs_slots += A.n_pairs_slots;
-    P.n_pairs += A.n_pairs;
-    P.n_cells += A.n_cells;
-    P.fold_gap = P.fold_gap && A.fold_gap;
-    P.waves_per_cu = A.waves_per_cu;
-    P.prior_table = A.prior_table;
-    std::memcpy(P.occ, A.occ, sizeof P.occ);
-  }
-  // spare words at the end, as build_host_pack leaves them
-  P.hap_pack.resize(P.hap_pack.size() + 2, 0);
-  P.hap_nmask.resize(P.hap_nmask.size() + 2, 0);
-  P.read_pack.resize(P.read_pack.size() + 2, 0);
-  P.read_nmask.resize(P.read_nmask.size() + 2, 0);
-  P.rq.resize(P.rq.size() + 64, 0);
-  P.ri.resize(P.ri.size() + 64, 0);
-  P.rd.resize(P.rd.size() + 64, 0);
-  P.rc.resize(P.rc.size() + 64, 0);
-  P.n_regions = (uint32_t)P.region.size();
-  P.tiles.reserve(tiles);
-  P.bin_start.assign(kFp32Bins + 1, 0);
-  for (int v = 0; v < kFp32Bins; ++v) {
-    P.bin_start[v] = (uint32_t)P.tiles.size();
-    append_longest_first(bins[v], P.hap_len, P.tiles);
-  }
-  P.bin_start[kFp32Bins] = (uint32_t)P.tiles.size();
-  P.fp64_base.assign(kNumFp64Bins + 1, 0);
-  for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
-  return RVC_OK;
-}
-
 int fp32_variant_index(int W, int RPL) {
   const int w = W == 16 ? 0 : W == 32 ? 1 : 2;
   return w * 16 + (RPL - 1);
@@ -513,26 +416,214 @@ void DeviceBatch::release() {
   mem = nullptr;
 }
 
-int DeviceBatch::upload(const HostPack& P, hipStream_t s, void* reuse_mem, size_t reuse_bytes) {
-  pack = &P;
-  // layout of one device allocation
+PackSizes pack_sizes(const HostPack& P) {
+  PackSizes z;
+  z.hap_words = P.hap_pack.size();
+  z.hap_nwords = P.hap_nmask.size();
+  z.n_haps = P.hap_off.size();
+  z.read_words = P.read_pack.size();
+  z.read_nwords = P.read_nmask.size();
+  z.qual_bytes = P.rq.size();
+  z.n_reads = P.read_off.size();
+  z.n_tiles = P.tiles.size();
+  z.n_pairs_slots = P.n_pairs_slots;
+  z.n_rescue = P.fp64_base.empty() ? 0 : P.fp64_base[kNumFp64Bins];
+  return z;
+}
+
+BatchLayout batch_layout(const PackSizes& z) {
+  BatchLayout L;
   size_t off = 0;
   auto take = [&](size_t b) { size_t o = off; off += round256(b); return o; };
-  const size_t o_tables = 0;
-  (void)o_tables;
-  const size_t o_hpack = take(bytes_of(P.hap_pack)), o_hn = take(bytes_of(P.hap_nmask));
-  const size_t o_hoff = take(bytes_of(P.hap_off)), o_hlen = take(bytes_of(P.hap_len));
-  const size_t o_rpack = take(bytes_of(P.read_pack)), o_rn = take(bytes_of(P.read_nmask));
-  const size_t o_rq = take(P.rq.size()), o_ri = take(P.ri.size()), o_rd = take(P.rd.size()), o_rc = take(P.rc.size());
-  const size_t o_roff = take(bytes_of(P.read_off)), o_rlen = take(bytes_of(P.read_len));
-  const size_t o_tiles = take(bytes_of(P.tiles) + sizeof(Tile));
-  const size_t o_out = take(P.n_pairs_slots * sizeof(double) + 8);
-  const size_t o_flags = take(P.n_pairs_slots + 8);
-  const size_t o_resc = take((size_t)P.fp64_base[kNumFp64Bins] * sizeof(RescueEntry) + sizeof(RescueEntry));
-  const size_t o_rbase = take(kNumFp64Bins * sizeof(uint32_t));
-  const size_t n_counters = kFp32Bins + kNumFp64Bins + kNumFp64Bins;
-  const size_t o_cnt = take(n_counters * sizeof(uint32_t));
-  bytes = off;
+  L.hpack = take(4 * z.hap_words);
+  L.hn = take(4 * z.hap_nwords);
+  L.hoff = take(4 * z.n_haps);
+  L.hlen = take(4 * z.n_haps);
+  L.rpack = take(4 * z.read_words);
+  L.rn = take(4 * z.read_nwords);
+  L.rq = take(z.qual_bytes);
+  L.ri = take(z.qual_bytes);
+  L.rd = take(z.qual_bytes);
+  L.rc = take(z.qual_bytes);
+  L.roff = take(4 * z.n_reads);
+  L.rlen = take(4 * z.n_reads);
+  L.tiles = take(sizeof(Tile) * (z.n_tiles + 1));
+  L.rbase = take(4 * kNumFp64Bins);
+  L.in_bytes = off;
+  L.out = take(sizeof(double) * (z.n_pairs_slots + 1));
+  L.flags = take(z.n_pairs_slots + 8);
+  L.resc = take(sizeof(RescueEntry) * (z.n_rescue + 1));
+  L.cnt = take(4 * kNumCounters);
+  L.bytes = off;
+  return L;
+}
+
+void stage_pack(const HostPack& P, const BatchLayout& L, char* dst) {
+  auto put = [&](size_t o, const void* src, size_t b) { if (b) std::memcpy(dst + o, src, b); };
+  put(L.hpack, P.hap_pack.data(), bytes_of(P.hap_pack));
+  put(L.hn, P.hap_nmask.data(), bytes_of(P.hap_nmask));
+  put(L.hoff, P.hap_off.data(), bytes_of(P.hap_off));
+  put(L.hlen, P.hap_len.data(), bytes_of(P.hap_len));
+  put(L.rpack, P.read_pack.data(), bytes_of(P.read_pack));
+  put(L.rn, P.read_nmask.data(), bytes_of(P.read_nmask));
+  put(L.rq, P.rq.data(), P.rq.size());
+  put(L.ri, P.ri.data(), P.ri.size());
+  put(L.rd, P.rd.data(), P.rd.size());
+  put(L.rc, P.rc.data(), P.rc.size());
+  put(L.roff, P.read_off.data(), bytes_of(P.read_off));
+  put(L.rlen, P.read_len.data(), bytes_of(P.read_len));
+  put(L.tiles, P.tiles.data(), bytes_of(P.tiles));
+  put(L.rbase, P.fp64_base.data(), kNumFp64Bins * sizeof(uint32_t));
+}
+
+PinnedBuf::~PinnedBuf() { release(); }
+
+void PinnedBuf::release() {
+  if (ptr) hipHostFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+}
+
+int PinnedBuf::reserve(size_t bytes) {
+  if (bytes <= cap) return RVC_OK;
+  release();
+  const size_t want = bytes + bytes / 4 + (1u << 20);
+  if (hipHostMalloc((void**)&ptr, want, hipHostMallocDefault) != hipSuccess) {
+    ptr = nullptr;
+    return set_error(RVC_E_DEVICE, "pinned host buffer of %zu bytes", want);
+  }
+  cap = want;
+  return RVC_OK;
+}
+
+int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, BatchLayout* Lout, PinnedBuf* buf) {
+  HostPack& P = *meta;
+  P = HostPack();
+  // every sequence block starts on a 32-base boundary: a pack's words are
+  // taken without its two spare words, rounded down to an even count, so the
+  // next block still starts on a 32-base boundary
+  auto hwords = [](const HostPack& A) { return (A.hap_pack.size() - 2) & ~size_t(1); };
+  auto rwords = [](const HostPack& A) { return (A.read_pack.size() - 2) & ~size_t(1); };
+  PackSizes z;
+  std::vector<uint64_t> bin_count(kFp32Bins, 0);
+  P.fp64_cap.assign(kNumFp64Bins, 0);
+  for (uint32_t x = 0; x < n; ++x) {
+    const HostPack& A = *packs[x];
+    z.hap_words += hwords(A);
+    z.n_haps += A.hap_off.size();
+    z.read_words += rwords(A);
+    z.n_reads += A.read_off.size();
+    z.n_tiles += A.tiles.size();
+    z.n_pairs_slots += A.n_pairs_slots;
+    for (int v = 0; v < kFp32Bins; ++v) bin_count[v] += A.bin_start[v + 1] - A.bin_start[v];
+    for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_cap[b] += A.fp64_cap[b];
+  }
+  if (z.hap_words * 16 > 0xffffffffull || z.read_words * 16 > 0xffffffffull || z.n_pairs_slots > 0xffffffffull)
+    return set_error(RVC_E_LIMIT, "merged batch too large for 32-bit offsets");
+  z.hap_nwords = z.hap_words / 2 + 2;
+  z.hap_words += 2;
+  z.qual_bytes = z.read_words * 16 + 64;
+  z.read_nwords = z.read_words / 2 + 2;
+  z.read_words += 2;
+  P.fp64_base.assign(kNumFp64Bins + 1, 0);
+  for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
+  z.n_rescue = P.fp64_base[kNumFp64Bins];
+  const BatchLayout L = batch_layout(z);
+  int rc = buf->reserve(L.in_bytes);
+  if (rc) return rc;
+  char* dst = buf->ptr;
+  uint32_t* hpack = reinterpret_cast<uint32_t*>(dst + L.hpack);
+  uint32_t* hn = reinterpret_cast<uint32_t*>(dst + L.hn);
+  uint32_t* hoff = reinterpret_cast<uint32_t*>(dst + L.hoff);
+  uint32_t* hlen = reinterpret_cast<uint32_t*>(dst + L.hlen);
+  uint32_t* rpack = reinterpret_cast<uint32_t*>(dst + L.rpack);
+  uint32_t* rn = reinterpret_cast<uint32_t*>(dst + L.rn);
+  uint32_t* roff = reinterpret_cast<uint32_t*>(dst + L.roff);
+  uint32_t* rlen = reinterpret_cast<uint32_t*>(dst + L.rlen);
+  Tile* tiles = reinterpret_cast<Tile*>(dst + L.tiles);
+  char* quals[4] = {dst + L.rq, dst + L.ri, dst + L.rd, dst + L.rc};
+  P.bin_start.assign(kFp32Bins + 1, 0);
+  for (int v = 0; v < kFp32Bins; ++v) P.bin_start[v + 1] = P.bin_start[v] + (uint32_t)bin_count[v];
+  std::vector<uint32_t> bin_cur(P.bin_start.begin(), P.bin_start.end() - 1);
+  P.region.reserve(n);
+  P.read_perm.resize(z.n_reads);
+  uint64_t hw = 0, rw = 0, hidx = 0, ridx = 0, pbase = 0;
+  for (uint32_t x = 0; x < n; ++x) {
+    const HostPack& A = *packs[x];
+    const size_t aw = hwords(A), bw = rwords(A);
+    std::memcpy(hpack + hw, A.hap_pack.data(), aw * 4);
+    std::memcpy(hn + hw / 2, A.hap_nmask.data(), aw / 2 * 4);
+    std::memcpy(rpack + rw, A.read_pack.data(), bw * 4);
+    std::memcpy(rn + rw / 2, A.read_nmask.data(), bw / 2 * 4);
+    const std::vector<uint8_t>* aq[4] = {&A.rq, &A.ri, &A.rd, &A.rc};
+    for (int k = 0; k < 4; ++k) std::memcpy(quals[k] + rw * 16, aq[k]->data(), bw * 16);
+    const uint32_t hbase = (uint32_t)(hw * 16), rbase = (uint32_t)(rw * 16);
+    for (size_t h = 0; h < A.hap_off.size(); ++h) {
+      hoff[hidx + h] = A.hap_off[h] + hbase;
+      hlen[hidx + h] = A.hap_len[h];
+    }
+    for (size_t r = 0; r < A.read_off.size(); ++r) {
+      roff[ridx + r] = A.read_off[r] + rbase;
+      rlen[ridx + r] = A.read_len[r];
+    }
+    std::memcpy(P.read_perm.data() + ridx, A.read_perm.data(), A.read_perm.size() * 4);
+    for (const RegionLayout& L0 : A.region) {
+      RegionLayout R = L0;
+      R.hap0 += (uint32_t)hidx;
+      R.read0 += (uint32_t)ridx;
+      R.pair0 += pbase;
+      P.region.push_back(R);
+    }
+    for (int v = 0; v < kFp32Bins; ++v) {
+      Tile* t = tiles + bin_cur[v];
+      for (uint32_t k = A.bin_start[v]; k < A.bin_start[v + 1]; ++k, ++t) {
+        *t = A.tiles[k];
+        t->hap += (uint32_t)hidx;
+        t->read0 += (uint32_t)ridx;
+        t->out0 += (uint32_t)pbase;
+      }
+      bin_cur[v] += A.bin_start[v + 1] - A.bin_start[v];
+      P.variant_cells[v] += A.variant_cells[v];
+    }
+    hw += aw;
+    rw += bw;
+    hidx += A.hap_off.size();
+    ridx += A.read_off.size();
+    pbase += A.n_pairs_slots;
+    P.n_pairs += A.n_pairs;
+    P.n_cells += A.n_cells;
+    P.fold_gap = P.fold_gap && A.fold_gap;
+    P.waves_per_cu = A.waves_per_cu;
+    P.prior_table = A.prior_table;
+    std::memcpy(P.occ, A.occ, sizeof P.occ);
+  }
+  // spare words and bytes at the ends, read by the last steps of a sweep
+  std::memset(hpack + hw, 0, 8);
+  std::memset(hn + hw / 2, 0, 8);
+  std::memset(rpack + rw, 0, 8);
+  std::memset(rn + rw / 2, 0, 8);
+  for (int k = 0; k < 4; ++k) std::memset(quals[k] + rw * 16, 0, 64);
+  std::memset(static_cast<void*>(tiles + z.n_tiles), 0, sizeof(Tile));
+  std::memcpy(dst + L.rbase, P.fp64_base.data(), kNumFp64Bins * sizeof(uint32_t));
+  P.n_pairs_slots = z.n_pairs_slots;
+  P.n_regions = (uint32_t)P.region.size();
+  *Lout = L;
+  return RVC_OK;
+}
+
+int DeviceBatch::upload(const HostPack& P, hipStream_t s, void* reuse_mem, size_t reuse_bytes) {
+  const BatchLayout L = batch_layout(pack_sizes(P));
+  std::vector<char>& img = const_cast<HostPack&>(P).staging;
+  img.resize(L.in_bytes);
+  stage_pack(P, L, img.data());
+  return upload_staged(P, L, img.data(), s, reuse_mem, reuse_bytes);
+}
+
+int DeviceBatch::upload_staged(const HostPack& P, const BatchLayout& L, const void* host, hipStream_t s,
+                               void* reuse_mem, size_t reuse_bytes) {
+  pack = &P;
+  layout = L;
+  bytes = L.bytes;
   if (reuse_mem && reuse_bytes >= bytes) {
     mem = reuse_mem;
     own_mem = false;
@@ -541,48 +632,30 @@ int DeviceBatch::upload(const HostPack& P, hipStream_t s, void* reuse_mem, size_
     own_mem = true;
   }
   char* base = static_cast<char*>(mem);
-  auto cp = [&](size_t o, const void* src, size_t b) -> hipError_t {
-    if (b == 0) return hipSuccess;
-    return hipMemcpyAsync(base + o, src, b, hipMemcpyHostToDevice, s);
-  };
-  RVC_HIP(cp(o_hpack, P.hap_pack.data(), bytes_of(P.hap_pack)));
-  RVC_HIP(cp(o_hn, P.hap_nmask.data(), bytes_of(P.hap_nmask)));
-  RVC_HIP(cp(o_hoff, P.hap_off.data(), bytes_of(P.hap_off)));
-  RVC_HIP(cp(o_hlen, P.hap_len.data(), bytes_of(P.hap_len)));
-  RVC_HIP(cp(o_rpack, P.read_pack.data(), bytes_of(P.read_pack)));
-  RVC_HIP(cp(o_rn, P.read_nmask.data(), bytes_of(P.read_nmask)));
-  RVC_HIP(cp(o_rq, P.rq.data(), P.rq.size()));
-  RVC_HIP(cp(o_ri, P.ri.data(), P.ri.size()));
-  RVC_HIP(cp(o_rd, P.rd.data(), P.rd.size()));
-  RVC_HIP(cp(o_rc, P.rc.data(), P.rc.size()));
-  RVC_HIP(cp(o_roff, P.read_off.data(), bytes_of(P.read_off)));
-  RVC_HIP(cp(o_rlen, P.read_len.data(), bytes_of(P.read_len)));
-  RVC_HIP(cp(o_tiles, P.tiles.data(), bytes_of(P.tiles)));
-  RVC_HIP(cp(o_rbase, P.fp64_base.data(), kNumFp64Bins * sizeof(uint32_t)));
+  RVC_HIP(hipMemcpyAsync(base, host, L.in_bytes, hipMemcpyHostToDevice, s));
   // results start as NaN with no flags; pairs with an empty side keep that
-  std::vector<double>& nan_fill = const_cast<HostPack&>(P).nan_scratch;
-  nan_fill.assign(P.n_pairs_slots, std::numeric_limits<double>::quiet_NaN());
-  RVC_HIP(cp(o_out, nan_fill.data(), P.n_pairs_slots * sizeof(double)));
-  RVC_HIP(hipMemsetAsync(base + o_flags, 0, P.n_pairs_slots, s));
-  seqs.hap_pack = reinterpret_cast<const uint32_t*>(base + o_hpack);
-  seqs.hap_nmask = reinterpret_cast<const uint32_t*>(base + o_hn);
-  seqs.hap_off = reinterpret_cast<const uint32_t*>(base + o_hoff);
-  seqs.hap_len = reinterpret_cast<const uint32_t*>(base + o_hlen);
-  seqs.read_pack = reinterpret_cast<const uint32_t*>(base + o_rpack);
-  seqs.read_nmask = reinterpret_cast<const uint32_t*>(base + o_rn);
-  seqs.rq = reinterpret_cast<const uint8_t*>(base + o_rq);
-  seqs.ri = reinterpret_cast<const uint8_t*>(base + o_ri);
-  seqs.rd = reinterpret_cast<const uint8_t*>(base + o_rd);
-  seqs.rc = reinterpret_cast<const uint8_t*>(base + o_rc);
-  seqs.read_off = reinterpret_cast<const uint32_t*>(base + o_roff);
-  seqs.read_len = reinterpret_cast<const uint32_t*>(base + o_rlen);
-  tiles = reinterpret_cast<const Tile*>(base + o_tiles);
-  out = reinterpret_cast<double*>(base + o_out);
-  flags = reinterpret_cast<uint8_t*>(base + o_flags);
-  rescue = reinterpret_cast<RescueEntry*>(base + o_resc);
-  rescue_base = reinterpret_cast<const uint32_t*>(base + o_rbase);
-  counters = reinterpret_cast<uint32_t*>(base + o_cnt);
-  n_counters_ = (uint32_t)n_counters;
+  const uint64_t nan_bits = 0x7ff8000000000000ull;
+  RVC_HIP(launch_fill_u64(reinterpret_cast<uint64_t*>(base + L.out), P.n_pairs_slots, nan_bits, s));
+  RVC_HIP(launch_zero_u32(reinterpret_cast<uint32_t*>(base + L.flags), (uint32_t)((P.n_pairs_slots + 3) / 4), s));
+  seqs.hap_pack = reinterpret_cast<const uint32_t*>(base + L.hpack);
+  seqs.hap_nmask = reinterpret_cast<const uint32_t*>(base + L.hn);
+  seqs.hap_off = reinterpret_cast<const uint32_t*>(base + L.hoff);
+  seqs.hap_len = reinterpret_cast<const uint32_t*>(base + L.hlen);
+  seqs.read_pack = reinterpret_cast<const uint32_t*>(base + L.rpack);
+  seqs.read_nmask = reinterpret_cast<const uint32_t*>(base + L.rn);
+  seqs.rq = reinterpret_cast<const uint8_t*>(base + L.rq);
+  seqs.ri = reinterpret_cast<const uint8_t*>(base + L.ri);
+  seqs.rd = reinterpret_cast<const uint8_t*>(base + L.rd);
+  seqs.rc = reinterpret_cast<const uint8_t*>(base + L.rc);
+  seqs.read_off = reinterpret_cast<const uint32_t*>(base + L.roff);
+  seqs.read_len = reinterpret_cast<const uint32_t*>(base + L.rlen);
+  tiles = reinterpret_cast<const Tile*>(base + L.tiles);
+  out = reinterpret_cast<double*>(base + L.out);
+  flags = reinterpret_cast<uint8_t*>(base + L.flags);
+  rescue = reinterpret_cast<RescueEntry*>(base + L.resc);
+  rescue_base = reinterpret_cast<const uint32_t*>(base + L.rbase);
+  counters = reinterpret_cast<uint32_t*>(base + L.cnt);
+  n_counters_ = kNumCounters;
   return RVC_OK;
 }
 
@@ -731,17 +804,22 @@ int DeviceBatch::download(double* host_out, uint8_t* host_flags, hipStream_t s) 
   RVC_HIP(hipMemcpyAsync(tmp.data(), out, P.n_pairs_slots * sizeof(double), hipMemcpyDeviceToHost, s));
   RVC_HIP(hipMemcpyAsync(tf.data(), flags, P.n_pairs_slots, hipMemcpyDeviceToHost, s));
   RVC_HIP(hipStreamSynchronize(s));
+  unpermute_results(P, tmp.data(), tf.data(), host_out, host_flags);
+  return RVC_OK;
+}
+
+void unpermute_results(const HostPack& P, const double* dev_out, const uint8_t* dev_flags, double* host_out,
+                       uint8_t* host_flags) {
   // device rows are in descending read length; put them back in input order
   for (uint32_t r = 0; r < P.n_regions; ++r) {
     const RegionLayout& L = P.region[r];
     for (uint32_t sx = 0; sx < L.n_reads; ++sx) {
       const uint32_t x = P.read_perm[L.read0 + sx];
       const size_t src = L.pair0 + (size_t)sx * L.n_haps, dst = L.pair0 + (size_t)x * L.n_haps;
-      std::memcpy(host_out + dst, tmp.data() + src, L.n_haps * sizeof(double));
-      if (host_flags) std::memcpy(host_flags + dst, tf.data() + src, L.n_haps);
+      if (host_out) std::memcpy(host_out + dst, dev_out + src, L.n_haps * sizeof(double));
+      if (host_flags) std::memcpy(host_flags + dst, dev_flags + src, L.n_haps);
     }
   }
-  return RVC_OK;
 }
 
 }  // namespace rvc

@@ -103,14 +103,53 @@ struct HostPack {
   uint8_t occ[kFp32Bins] = {};            // occupancy target per FP32 bin
   uint64_t variant_cells[kFp32Bins] = {};  // cells computed by each FP32 bin
   bool prior_table = true;                 // launch table bins with the prior-table kernel
+  std::vector<char> staging;               // host image of the device inputs (upload)
   std::vector<double> nan_scratch;
   std::vector<uint8_t> flag_scratch;
 };
 
+// Element counts of a batch's device arrays.
+struct PackSizes {
+  uint64_t hap_words = 0, hap_nwords = 0, n_haps = 0;
+  uint64_t read_words = 0, read_nwords = 0, qual_bytes = 0, n_reads = 0;
+  uint64_t n_tiles = 0, n_pairs_slots = 0, n_rescue = 0;
+};
+PackSizes pack_sizes(const HostPack& P);
+
+// Byte offsets of a batch in its device allocation. The inputs come first,
+// so one host-to-device copy of [0, in_bytes) uploads a staged batch.
+constexpr uint32_t kNumCounters = kFp32Bins + 2 * kNumFp64Bins;
+struct BatchLayout {
+  size_t hpack = 0, hn = 0, hoff = 0, hlen = 0, rpack = 0, rn = 0, rq = 0, ri = 0, rd = 0, rc = 0;
+  size_t roff = 0, rlen = 0, tiles = 0, rbase = 0;
+  size_t in_bytes = 0;
+  size_t out = 0, flags = 0, resc = 0, cnt = 0;
+  size_t bytes = 0;
+};
+BatchLayout batch_layout(const PackSizes& z);
+// Writes the input section of P at dst (in_bytes of L).
+void stage_pack(const HostPack& P, const BatchLayout& L, char* dst);
+
+// Grow-only pinned host buffer.
+struct PinnedBuf {
+  char* ptr = nullptr;
+  size_t cap = 0;
+  ~PinnedBuf();
+  int reserve(size_t bytes);  // keeps no contents
+  void release();
+};
+
+// Merges packs built by build_host_pack straight into a staged input image
+// (pinned, in device layout), as if the regions had been packed together.
+// 'meta' receives the merged bookkeeping (regions, read order, tile bins,
+// rescue bins, counts); its sequence arrays stay empty.
+int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, BatchLayout* L, PinnedBuf* buf);
+
 int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Policy& pol, HostPack* hp);
-// Concatenates packs built by build_host_pack (any number of regions each)
-// into one batch, as if they had been packed together.
-int merge_host_packs(const HostPack* const* packs, uint32_t n, HostPack* out);
+
+// Copies a batch's results (device order, as downloaded) to input read order.
+void unpermute_results(const HostPack& P, const double* dev_out, const uint8_t* dev_flags, double* host_out,
+                       uint8_t* host_flags);
 
 struct DeviceBatch {
   const HostPack* pack = nullptr;
@@ -139,9 +178,14 @@ struct DeviceBatch {
 
   ~DeviceBatch();
   void release();
+  BatchLayout layout;
   // Lays the batch out in one device allocation (reuse_mem if it is big
-  // enough) and queues the host-to-device copies on s.
+  // enough) and queues the host-to-device copy on s.
   int upload(const HostPack& P, hipStream_t s, void* reuse_mem, size_t reuse_bytes);
+  // The same for inputs already staged in device layout at 'host' (pinned
+  // memory makes the copy asynchronous); 'meta' describes the batch.
+  int upload_staged(const HostPack& meta, const BatchLayout& L, const void* host, hipStream_t s, void* reuse_mem,
+                    size_t reuse_bytes);
   // Queues counter reset, FP32 launches and FP64 rescue launches on s.
   int enqueue(hipStream_t s, bool timed);
   int ensure_events();

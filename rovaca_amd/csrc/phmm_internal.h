@@ -108,5 +108,6 @@ hipError_t launch_fp32(int W, int RPL, bool xf, int occ, bool pt, int grid, hipS
                        const Fp32Args& a);
 hipError_t launch_fp64(int bin, int grid, hipStream_t stream, const Fp64Args& a);
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
+hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t stream);
 
 }  // namespace rvc

@@ -501,6 +501,12 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   if (i < n) p[i] = 0;
 }
 
+// Grid-stride fill of n 64-bit words.
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t value) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = value;
+}
+
 using Fp32Fn = void (*)(Fp32Args);
 using Fp64Fn = void (*)(Fp64Args);
 
@@ -578,6 +584,13 @@ hipError_t launch_fp64(int bin, int grid, hipStream_t stream, const Fp64Args& a)
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, value);
   return hipGetLastError();
 }
 

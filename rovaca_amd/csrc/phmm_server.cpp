@@ -2,17 +2,20 @@
 // threads (Rovaca's worker pool calls call_pairhmm once per region from up to
 // --nthreads workers, hc/haplotypecaller_engine.cpp:119,254).
 //
-// submit() copies a region and returns a ticket at once. One server thread
-// owns kSlots launch slots, each with its own stream and device arena. Whenever
-// a slot is free it admits every waiting request, in arrival order, whose
-// device footprint still fits the HBM budget next to the requests already in
-// flight, packs them into one batch and queues the FP32 pass, the FP64 rescue
-// and the copy of the results on the slot's stream. Slots run concurrently,
-// so requests that arrive while earlier launches are on the device start
-// while those finish. Completed tickets are released to their waiters one by
-// one.
+// submit() packs a region on the caller's thread and returns a ticket at once.
+// kSlots launch slots, each with its own thread, stream, pinned staging
+// buffer and device arena, take work from one queue: a free slot admits every
+// waiting request, in arrival order, whose device footprint still fits the
+// HBM budget next to the requests already in flight, merges them straight into
+// its pinned staging buffer in device layout, and queues one host-to-device
+// copy, the FP32 pass, the FP64 rescue and the copy of the results on its
+// stream. Slots work concurrently, so requests that arrive while earlier
+// launches are on the device are staged and started while those finish.
+// Results land in pinned buffers that the finished requests share; each waiter
+// copies its own region out.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -35,11 +38,68 @@ namespace {
 using Clock = std::chrono::steady_clock;
 constexpr int kSlots = 3;
 
-// Results of one finished launch, shared by its requests: each waiter copies
-// its own region out (rows back in input order) from here.
+// Pinned host memory for one launch's results: log10 L and flags per pair
+// slot, plus the FP64 rescue counters.
+struct ResultBuf {
+  char* mem = nullptr;
+  size_t cap = 0;  // pair slots
+  double* out() const { return reinterpret_cast<double*>(mem); }
+  uint8_t* flags() const { return reinterpret_cast<uint8_t*>(mem + cap * sizeof(double)); }
+  uint32_t* rescued() const { return reinterpret_cast<uint32_t*>(mem + cap * (sizeof(double) + 1) + 64); }
+  ~ResultBuf() {
+    if (mem) hipHostFree(mem);
+  }
+  int reserve(size_t n) {
+    if (n <= cap && mem) return RVC_OK;
+    if (mem) hipHostFree(mem);
+    mem = nullptr;
+    cap = 0;
+    const size_t c = n + n / 4 + 4096;
+    if (hipHostMalloc((void**)&mem, c * (sizeof(double) + 1) + 64 + 64, hipHostMallocDefault) != hipSuccess) {
+      mem = nullptr;
+      return set_error(RVC_E_DEVICE, "pinned result buffer for %zu pairs", n);
+    }
+    cap = c;
+    return RVC_OK;
+  }
+};
+
+// Result buffers are recycled: a finished launch hands its buffer to the
+// requests it served, and the buffer comes back when the last of them has
+// been waited for (or dropped at shutdown).
+struct ResultPool : std::enable_shared_from_this<ResultPool> {
+  std::mutex mu;
+  std::vector<std::unique_ptr<ResultBuf>> free;
+  std::shared_ptr<ResultBuf> acquire(size_t n, int* rc) {
+    std::unique_ptr<ResultBuf> b;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      size_t best = free.size();
+      for (size_t x = 0; x < free.size(); ++x)
+        if (free[x]->cap >= n && (best == free.size() || free[x]->cap < free[best]->cap)) best = x;
+      if (best == free.size() && !free.empty()) best = free.size() - 1;
+      if (best < free.size()) {
+        b = std::move(free[best]);
+        free.erase(free.begin() + (long)best);
+      }
+    }
+    if (!b) b.reset(new (std::nothrow) ResultBuf);
+    if (!b) {
+      *rc = set_error(RVC_E_INVALID, "out of host memory");
+      return nullptr;
+    }
+    if ((*rc = b->reserve(n))) return nullptr;
+    std::shared_ptr<ResultPool> self = shared_from_this();
+    return std::shared_ptr<ResultBuf>(b.release(), [self](ResultBuf* p) {
+      std::lock_guard<std::mutex> lk(self->mu);
+      self->free.emplace_back(p);
+    });
+  }
+};
+
+// Results of one finished launch, shared by its requests.
 struct BatchResult {
-  std::vector<double> out;
-  std::vector<uint8_t> flags;
+  std::shared_ptr<ResultBuf> buf;
   std::vector<RegionLayout> region;
   std::vector<uint32_t> read_perm;
 };
@@ -48,6 +108,7 @@ struct Request {
   uint64_t id = 0;
   HostPack pack;  // the region, packed by the submitting thread
   uint64_t resident = 0;
+  uint32_t max_hap = 0;
   std::shared_ptr<const BatchResult> result;
   uint32_t result_region = 0;
   int status = 0;
@@ -60,28 +121,23 @@ uint64_t now_ns() {
 }
 
 struct Slot {
+  std::thread thr;
   hipStream_t stream = nullptr;
-  void* mem = nullptr;
-  size_t bytes = 0;
-  double* h_out = nullptr;   // pinned
-  uint8_t* h_flags = nullptr;
-  size_t h_cap = 0;
   hipEvent_t done_ev = nullptr;
-  HostPack pack;
+  void* mem = nullptr;  // device arena (grow-only)
+  size_t bytes = 0;
+  PinnedBuf staging;
+  HostPack meta;
   DeviceBatch batch;
+  std::shared_ptr<ResultBuf> res;
   std::vector<Request*> reqs;
   uint64_t resident = 0;
-  bool busy = false;
 };
 
-// Device bytes a packed region holds while in flight (sequences, tiles,
-// results, rescue list): what DeviceBatch::upload lays out for it.
-uint64_t footprint(const HostPack& p) {
-  const uint64_t words = p.hap_pack.size() + p.hap_nmask.size() + p.read_pack.size() + p.read_nmask.size();
-  const uint64_t index = p.hap_off.size() + p.hap_len.size() + p.read_off.size() + p.read_len.size();
-  return 4 * (words + index) + 4 * p.rq.size() + sizeof(Tile) * p.tiles.size() +
-         p.n_pairs_slots * (sizeof(double) + 1 + sizeof(RescueEntry)) + 4096;
-}
+// Device bytes a packed region holds while in flight: its share of the
+// layout DeviceBatch::upload_staged builds (sequences, tiles, results, rescue
+// list, counters).
+uint64_t footprint(const HostPack& p) { return batch_layout(pack_sizes(p)).bytes; }
 
 struct Server {
   std::mutex mu;
@@ -90,9 +146,9 @@ struct Server {
   std::unordered_map<uint64_t, Request*> tickets;
   uint64_t next_id = 1;
   uint64_t resident = 0, peak_resident = 0;
-  bool running = false, stopping = false;
-  std::thread thr;
+  bool stopping = false;
   Slot slots[kSlots];
+  std::shared_ptr<ResultPool> pool = std::make_shared<ResultPool>();
   // statistics
   uint64_t submitted = 0, completed = 0, launches = 0, cells = 0, rescued = 0;
   double busy_ms = 0;
@@ -100,146 +156,121 @@ struct Server {
 
 Server* g_srv = nullptr;
 
-void fail_requests(std::vector<Request*>& reqs, int code) {
-  const uint64_t t = now_ns();
-  for (Request* r : reqs) {
-    r->status = code;
-    r->t_done = t;
-    r->done = true;
-  }
-}
-
-// Merges the slot's packed requests into one batch and launches it (server
-// thread, no lock held).
+// Merges the slot's requests into its staging buffer and queues the launch
+// (slot thread, no lock held).
 int launch_slot(Slot& s) {
+  // longest haplotypes first, so that the persistent waves meet the longest
+  // tiles of every shape bin first
+  std::stable_sort(s.reqs.begin(), s.reqs.end(),
+                   [](const Request* a, const Request* b) { return a->max_hap > b->max_hap; });
   std::vector<const HostPack*> packs;
   packs.reserve(s.reqs.size());
   for (Request* r : s.reqs) packs.push_back(&r->pack);
-  int rc = merge_host_packs(packs.data(), (uint32_t)packs.size(), &s.pack);
+  BatchLayout L;
+  int rc = stage_merged(packs.data(), (uint32_t)packs.size(), &s.meta, &L, &s.staging);
   if (rc) return rc;
-  for (Request* r : s.reqs) r->pack = HostPack();  // merged copy is all the launch needs
-  rc = s.batch.upload(s.pack, s.stream, s.mem, s.bytes);
-  if (rc) return rc;
-  if (s.batch.own_mem) {
+  for (Request* r : s.reqs) r->pack = HostPack();  // the staged image is all the launch needs
+  if (!s.mem || s.bytes < L.bytes) {
     if (s.mem) hipFree(s.mem);
-    s.mem = s.batch.mem;
-    s.bytes = s.batch.bytes;
-    s.batch.own_mem = false;
+    s.mem = nullptr;
+    s.bytes = 0;
+    const size_t want = L.bytes + L.bytes / 4 + (16u << 20);
+    if (hipMalloc(&s.mem, want) != hipSuccess) {
+      s.mem = nullptr;
+      return set_error(RVC_E_DEVICE, "device arena of %zu bytes", want);
+    }
+    s.bytes = want;
   }
-  const size_t n = s.pack.n_pairs_slots;
-  if (n > s.h_cap) {
-    if (s.h_out) hipHostFree(s.h_out);
-    if (s.h_flags) hipHostFree(s.h_flags);
-    s.h_out = nullptr;
-    s.h_flags = nullptr;
-    size_t cap = n + n / 2 + 1024;
-    if (hipHostMalloc((void**)&s.h_out, cap * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&s.h_flags, cap, hipHostMallocDefault) != hipSuccess)
-      return set_error(RVC_E_DEVICE, "pinned result buffers");
-    s.h_cap = cap;
-  }
+  if ((rc = s.batch.upload_staged(s.meta, L, s.staging.ptr, s.stream, s.mem, s.bytes))) return rc;
+  const size_t n = s.meta.n_pairs_slots;
+  s.res = g_srv->pool->acquire(n, &rc);
+  if (!s.res) return rc;
   if ((rc = s.batch.ensure_events())) return rc;
   if ((rc = s.batch.enqueue(s.stream, true))) return rc;
-  if (hipMemcpyAsync(s.h_out, s.batch.out, n * sizeof(double), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
-      hipMemcpyAsync(s.h_flags, s.batch.flags, n, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+  if (hipMemcpyAsync(s.res->out(), s.batch.out, n * sizeof(double), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+      hipMemcpyAsync(s.res->flags(), s.batch.flags, n, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+      hipMemcpyAsync(s.res->rescued(), s.batch.counters + kFp32Bins + kNumFp64Bins, kNumFp64Bins * sizeof(uint32_t),
+                     hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
       hipEventRecord(s.done_ev, s.stream) != hipSuccess)
     return set_error(RVC_E_DEVICE, "result copy");
   return RVC_OK;
 }
 
-// Hands a finished slot's results to its requests (server thread): one bulk
-// copy out of the pinned buffers; each waiter then takes its own rows.
-void finish_slot(Slot& s) {
-  const HostPack& P = s.pack;
-  float ms = 0;
-  if (hipEventElapsedTime(&ms, s.batch.ev[0], s.batch.ev[2]) != hipSuccess) ms = 0;
-  auto res = std::make_shared<BatchResult>();
-  const size_t n = P.n_pairs_slots;
-  res->out.assign(s.h_out, s.h_out + n);
-  res->flags.assign(s.h_flags, s.h_flags + n);
-  res->region = P.region;
-  res->read_perm = P.read_perm;
-  uint64_t resc = 0;
-  for (size_t x = 0; x < n; ++x) resc += (s.h_flags[x] & RVC_PAIR_FP64) ? 1 : 0;
-  std::lock_guard<std::mutex> lk(g_srv->mu);
-  g_srv->busy_ms += ms;
-  g_srv->rescued += resc;
-  g_srv->cells += P.n_cells;
-  const uint64_t t = now_ns();
-  for (uint32_t r = 0; r < (uint32_t)s.reqs.size(); ++r) {
-    Request* rq = s.reqs[r];
-    rq->result = res;
-    rq->result_region = r;
-    rq->status = RVC_OK;
-    rq->t_done = t;
-    rq->done = true;
-    ++g_srv->completed;
+// Waits for the slot's launch without holding a core: the device time of a
+// launch is far above the polling interval.
+int wait_slot(Slot& s) {
+  for (;;) {
+    const hipError_t e = hipEventQuery(s.done_ev);
+    if (e == hipSuccess) return RVC_OK;
+    if (e != hipErrorNotReady) return set_error(RVC_E_DEVICE, "launch failed: %s", hipGetErrorString(e));
+    std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
-  g_srv->resident -= s.resident;
-  s.resident = 0;
-  s.reqs.clear();
-  s.busy = false;
-  g_srv->cv_done.notify_all();
 }
 
-void server_main() {
+void slot_main(Slot* sp) {
   Server& S = *g_srv;
+  Slot& s = *sp;
   hipSetDevice(g_engine.device);
   std::unique_lock<std::mutex> lk(S.mu);
   for (;;) {
-    bool progress = false;
-    // 1. launch: fill every free slot with what fits the budget
-    for (Slot& s : S.slots) {
-      if (s.busy || S.pending.empty()) continue;
-      uint64_t add = 0;
-      while (!S.pending.empty() && s.reqs.size() < g_engine.max_batch_regions) {
-        Request* r = S.pending.front();
-        if (S.resident + add + r->resident > g_engine.hbm_budget) break;
-        add += r->resident;
-        s.reqs.push_back(r);
-        S.pending.pop_front();
-      }
-      if (s.reqs.empty()) break;  // the head does not fit yet: wait for a slot to finish
-      s.busy = true;
-      s.resident = add;
-      const uint64_t t = now_ns();
-      for (Request* r : s.reqs) r->t_launch = t;
-      S.resident += add;
-      if (S.resident > S.peak_resident) S.peak_resident = S.resident;
-      ++S.launches;
-      lk.unlock();
-      const int rc = launch_slot(s);
-      lk.lock();
-      if (rc) {
-        hipStreamSynchronize(s.stream);
-        fail_requests(s.reqs, rc);
-        S.completed += s.reqs.size();
-        S.resident -= s.resident;
-        s.resident = 0;
-        s.reqs.clear();
-        s.busy = false;
-        S.cv_done.notify_all();
-      }
-      progress = true;
+    S.cv_work.wait(lk, [&] {
+      return (S.stopping && S.pending.empty()) ||
+             (!S.pending.empty() && S.resident + S.pending.front()->resident <= g_engine.hbm_budget);
+    });
+    if (S.pending.empty()) break;  // stopping and drained
+    uint64_t add = 0;
+    while (!S.pending.empty() && s.reqs.size() < g_engine.max_batch_regions) {
+      Request* r = S.pending.front();
+      if (S.resident + add + r->resident > g_engine.hbm_budget) break;
+      add += r->resident;
+      s.reqs.push_back(r);
+      S.pending.pop_front();
     }
-    // 2. complete finished slots
-    for (Slot& s : S.slots) {
-      if (!s.busy) continue;
-      if (hipEventQuery(s.done_ev) == hipSuccess) {
-        lk.unlock();
-        finish_slot(s);
-        lk.lock();
-        progress = true;
-      }
+    s.resident = add;
+    S.resident += add;
+    if (S.resident > S.peak_resident) S.peak_resident = S.resident;
+    ++S.launches;
+    const uint64_t t = now_ns();
+    for (Request* r : s.reqs) r->t_launch = t;
+    lk.unlock();
+    int rc = launch_slot(s);
+    if (rc == RVC_OK) rc = wait_slot(s);
+    std::shared_ptr<BatchResult> res;
+    float ms = 0;
+    uint64_t resc = 0;
+    if (rc == RVC_OK) {
+      res = std::make_shared<BatchResult>();
+      res->buf = std::move(s.res);
+      res->region = std::move(s.meta.region);
+      res->read_perm = std::move(s.meta.read_perm);
+      if (hipEventElapsedTime(&ms, s.batch.ev[0], s.batch.ev[2]) != hipSuccess) ms = 0;
+      for (int b = 0; b < kNumFp64Bins; ++b) resc += res->buf->rescued()[b];
+    } else {
+      hipStreamSynchronize(s.stream);
+      s.res.reset();
     }
-    if (progress) continue;
-    bool any_busy = false;
-    for (Slot& s : S.slots) any_busy |= s.busy;
-    if (S.stopping && !any_busy && S.pending.empty()) break;
-    if (any_busy)
-      S.cv_work.wait_for(lk, std::chrono::microseconds(20));
-    else
-      S.cv_work.wait(lk);
+    const uint64_t cells = s.meta.n_cells;
+    lk.lock();
+    const uint64_t t_done = now_ns();
+    for (uint32_t r = 0; r < (uint32_t)s.reqs.size(); ++r) {
+      Request* rq = s.reqs[r];
+      rq->result = res;
+      rq->result_region = r;
+      rq->status = rc;
+      rq->t_done = t_done;
+      rq->done = true;
+    }
+    S.completed += s.reqs.size();
+    if (rc == RVC_OK) {
+      S.busy_ms += ms;
+      S.rescued += resc;
+      S.cells += cells;
+    }
+    S.resident -= s.resident;
+    s.resident = 0;
+    s.reqs.clear();
+    S.cv_done.notify_all();
+    S.cv_work.notify_all();  // memory came free: another slot may admit the head now
   }
 }
 
@@ -252,15 +283,18 @@ int server_start() {
   for (Slot& s : S->slots) {
     if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&s.done_ev, hipEventDisableTiming) != hipSuccess) {
+      for (Slot& t : S->slots) {
+        if (t.done_ev) hipEventDestroy(t.done_ev);
+        if (t.stream) hipStreamDestroy(t.stream);
+      }
       delete S;
       return set_error(RVC_E_DEVICE, "server stream");
     }
     s.batch.stream = s.stream;
     s.batch.own_stream = false;
   }
-  S->running = true;
   g_srv = S;
-  S->thr = std::thread(server_main);
+  for (Slot& s : S->slots) s.thr = std::thread(slot_main, &s);
   return RVC_OK;
 }
 
@@ -272,23 +306,19 @@ void server_stop() {
     S->stopping = true;
   }
   S->cv_work.notify_all();
-  if (S->thr.joinable()) S->thr.join();
+  for (Slot& s : S->slots)
+    if (s.thr.joinable()) s.thr.join();
   {
     std::lock_guard<std::mutex> lk(S->mu);
-    for (Request* r : S->pending) {
-      r->status = RVC_E_SHUTDOWN;
-      r->done = true;
-    }
-    S->pending.clear();
     for (auto& kv : S->tickets) delete kv.second;
     S->tickets.clear();
   }
   S->cv_done.notify_all();
   for (Slot& s : S->slots) {
     s.batch.release();
+    s.res.reset();
+    s.staging.release();
     if (s.mem) hipFree(s.mem);
-    if (s.h_out) hipHostFree(s.h_out);
-    if (s.h_flags) hipHostFree(s.h_flags);
     if (s.done_ev) hipEventDestroy(s.done_ev);
     if (s.stream) hipStreamDestroy(s.stream);
   }
@@ -315,6 +345,7 @@ int rvc_phmm_submit(const rvc_region* region, rvc_ticket* ticket) {
     return rc;
   }
   r->resident = footprint(r->pack);
+  for (uint32_t h : r->pack.hap_len) r->max_hap = std::max(r->max_hap, h);
   if (r->resident > g_engine.hbm_budget) {
     const uint64_t need = r->resident;
     delete r;
@@ -364,11 +395,13 @@ int rvc_phmm_wait(rvc_ticket ticket, double* out_read_major, uint8_t* out_flags,
   if (st == RVC_OK && (out_read_major || out_flags)) {
     const BatchResult& B = *r->result;
     const RegionLayout& L = B.region[r->result_region];
+    const double* bo = B.buf->out();
+    const uint8_t* bf = B.buf->flags();
     for (uint32_t sx = 0; sx < L.n_reads; ++sx) {
       const uint32_t x = B.read_perm[L.read0 + sx];
       const size_t src = L.pair0 + (size_t)sx * L.n_haps, dst = (size_t)x * L.n_haps;
-      if (out_read_major) std::memcpy(out_read_major + dst, B.out.data() + src, L.n_haps * sizeof(double));
-      if (out_flags) std::memcpy(out_flags + dst, B.flags.data() + src, L.n_haps);
+      if (out_read_major) std::memcpy(out_read_major + dst, bo + src, L.n_haps * sizeof(double));
+      if (out_flags) std::memcpy(out_flags + dst, bf + src, L.n_haps);
     }
   }
   delete r;
