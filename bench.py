@@ -8,11 +8,17 @@ in one launch sequence per step (FP32 pass over length-binned tiles, then the
 FP64 rescue). A step is one pass over all regions; inputs are resident in
 HBM before the timed region starts.
 
-Prints one JSON line: GCUPS (cell updates per second / 1e9), the FP32-pass
-roofline against MI355X's 157.3 TFLOP/s FP32 peak at 12 flops per cell, the
-plain one-region-per-launch GPU form on a sample, accuracy against the FP64
-oracle and the reference routing on a sample, and the CPU baseline (Rovaca's
+Prints one JSON line: GCUPS (cell updates per second / 1e9), the roofline of
+the dominant FP32 kernel against MI355X's 157.3 TFLOP/s FP32 peak at 12 flops
+per cell, the plain one-region-per-launch GPU form on a sample, accuracy
+against the FP64 oracle and the reference routing on a sample, the same
+regions served one by one (Poisson arrivals from 32 producer threads, GCUPS
+with p50/p99 region latency per arrival rate), and the CPU baseline (Rovaca's
 region scheme on the host cores) on a bounded sample.
+
+Other modes: --workload stress (precision stress, 2048 regions), single (one
+8 x 32 region, latency), served (the serving loop alone), deep (served with
+1 % deep regions of 2000 reads x 128 haplotypes of 1000 bp).
 """
 import argparse
 import json
@@ -27,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 FLOPS_PER_CELL = 12          # M: 4 mul + 2 add, I: 2 mul + 1 add, D: 2 mul + 1 add
 FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 vector peak
+METRIC = "PairHMM GCUPS + p99 region latency, 1 MI355X vs host AVX-512 & unbatched GPU"
 
 
 def log(msg):
@@ -140,7 +147,7 @@ def run_single(args):
     lat = np.array(lat)
     res = {"metric": "PairHMM single-region latency (8 haplotypes x 32 reads, 1 MI355X)",
            "value": round(float(np.median(lat)), 4), "unit": "ms", "higher_is_better": False, "n_gpus": 1,
-           "steps": len(lat), "warmup": args.warmup, "dtype": "fp32 (fp64 rescue)", "data": "synthetic",
+           "steps": len(lat), "warmup": args.warmup, "dtype": "f32", "rescue_dtype": "f64", "data": "synthetic",
            "config": {"workload": "single_region_8x32", "cells": cells},
            "gcups_at_median": round(cells / (np.median(lat) * 1e-3) / 1e9, 2),
            "p99_ms": round(float(np.percentile(lat, 99)), 4)}
@@ -153,52 +160,66 @@ def run_single(args):
     return res
 
 
-def run_served(args):
-    """Served stream: Poisson arrivals from 32 producer threads at each rate;
-    'deep' mixes 1 % regions of 2000 reads x 128 haplotypes (1000 bp)."""
+def served_stream(pool, deep, rates, duration, producers):
+    """Poisson arrivals of pool regions (and 1 % deep ones if given) from
+    'producers' host threads at each rate, through the serving loop."""
     import rovaca_amd as rv
-    from rovaca_amd import workload
-    t = time.time()
-    pool = workload.batched_regions(min(args.regions, 2048), seed=3)
-    deep = []
-    if args.workload == "deep":
-        deep = [workload.deep_region(seed=40 + x) for x in range(4)]
     regions = pool + deep
-    log(f"pool of {len(pool)} regions (+{len(deep)} deep) generated in {time.time() - t:.1f} s")
     rv.init(0, server=True, hbm_budget_bytes=64 << 30, max_batch_regions=4096)
     rng = np.random.default_rng(9)
-    rates = [float(x) for x in args.rates.split(",")] if args.workload == "served" else [2000.0]
     # warm-up: a short stream at the lowest rate
     arr = np.cumsum(rng.exponential(1.0 / rates[0], 200))
-    rv.loadgen_run(regions, arr, np.arange(200) % len(pool), args.producers)
+    rv.loadgen_run(regions, arr, np.arange(200) % len(pool), producers)
     out = []
     for rate in rates:
-        n = max(64, int(rate * args.duration))
+        n = max(64, int(rate * duration))
         arr = np.cumsum(rng.exponential(1.0 / rate, n))
         order = rng.integers(0, len(pool), n)
         if deep:
             is_deep = rng.random(n) < 0.01
             order[is_deep] = len(pool) + rng.integers(0, len(deep), int(is_deep.sum()))
-        res, lat = rv.loadgen_run(regions, arr, order, args.producers)
+        st0 = rv.server_stats()
+        res, lat = rv.loadgen_run(regions, arr, order, producers)
+        st1 = rv.server_stats()
+        launches = st1["launches"] - st0["launches"]
         rec = {"rate_per_s": rate, "regions": int(res["completed"]),
                "gcups": round(res["cells"] / res["wall_s"] / 1e9, 1),
                "offered_gcups": round(res["cells"] / arr[-1] / 1e9, 1),
                "p50_ms": round(res["lat_p50_ms"], 3), "p99_ms": round(res["lat_p99_ms"], 3),
                "max_ms": round(res["lat_max_ms"], 3), "queue_p50_ms": round(res["queue_p50_ms"], 3),
                "queue_p99_ms": round(res["queue_p99_ms"], 3), "submit_late_p99_ms": round(res["late_p99_ms"], 3),
-               "wall_s": round(res["wall_s"], 3)}
+               "wall_s": round(res["wall_s"], 3), "launches": int(launches),
+               "mean_regions_per_launch": round(int(res["completed"]) / max(1, launches), 1),
+               "host_stage_ms": round(st1["stage_ms"] - st0["stage_ms"], 1),
+               "buffer_grows": int(st1["buffer_grows"] - st0["buffer_grows"])}
         if deep:
             dl = lat[order >= len(pool)]
             rec["deep_regions"] = int((order >= len(pool)).sum())
             rec["deep_p50_ms"] = round(float(np.median(dl)), 3) if len(dl) else None
         out.append(rec)
-        log(f"rate {rate:.0f}/s: {rec['gcups']} GCUPS, p50 {rec['p50_ms']} ms, p99 {rec['p99_ms']} ms")
+        log(f"rate {rate:.0f}/s: {rec['gcups']} GCUPS, p50 {rec['p50_ms']} ms, p99 {rec['p99_ms']} ms, "
+            f"{rec['launches']} launches")
     st = rv.server_stats()
     rv.shutdown()
+    return out, st
+
+
+def run_served(args):
+    """Served stream: Poisson arrivals from 32 producer threads at each rate;
+    'deep' mixes 1 % regions of 2000 reads x 128 haplotypes (1000 bp)."""
+    from rovaca_amd import workload
+    t = time.time()
+    pool = workload.batched_regions(min(args.regions, 2048), seed=3)
+    deep = []
+    if args.workload == "deep":
+        deep = [workload.deep_region(seed=40 + x) for x in range(4)]
+    log(f"pool of {len(pool)} regions (+{len(deep)} deep) generated in {time.time() - t:.1f} s")
+    rates = [float(x) for x in args.rates.split(",")] if args.workload == "served" else [2000.0]
+    out, st = served_stream(pool, deep, rates, args.duration, args.producers)
     top = out[-1]
     name = "served" if args.workload == "served" else "served_with_1pct_deep_regions"
     return {"metric": f"PairHMM {name} GCUPS + p99 region latency (1 MI355X)", "value": top["gcups"],
-            "unit": "GCUPS", "higher_is_better": True, "n_gpus": 1, "dtype": "fp32 (fp64 rescue)",
+            "unit": "GCUPS", "higher_is_better": True, "n_gpus": 1, "dtype": "f32", "rescue_dtype": "f64",
             "data": "synthetic", "config": {"workload": name, "pool_regions": len(pool), "deep_regions": len(deep),
                                             "producers": args.producers, "hbm_budget_gib": 64,
                                             "arrivals": "poisson", "duration_s_per_rate": args.duration},
@@ -220,6 +241,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip plain-GPU and accuracy samples")
+    ap.add_argument("--no-served", action="store_true", help="skip the served stream of the batched workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (known bytes) to scale HBM byte counters")
     args = ap.parse_args()
@@ -316,10 +338,10 @@ def main():
                           "ms": round(fp32_ms, 3), "cells": cells},
             "traffic_source": os.path.relpath(tpath, ROOT) if traffic else None}
     result = {
-        "metric": "PairHMM GCUPS (batched uneven regions, 1 MI355X)",
+        "metric": METRIC,
         "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32 (fp64 rescue)", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f32", "rescue_dtype": "f64", "data": "synthetic",
         "config": {"workload": wname, "regions": len(regions), "pairs": int(info0["n_pairs"]),
                    "cells": int(cells), "graph": bool(args.graph), "seed": 1},
         "roofline": roof,
@@ -340,11 +362,23 @@ def main():
         if small:
             result["accuracy"] = accuracy(regions, ebatch, out, small)
     rv.shutdown()
+    if not args.no_served:
+        # the same regions arriving one by one through the serving loop
+        log("served stream")
+        rates = [float(x) for x in args.rates.split(",")]
+        srv, _ = served_stream(regions, [], rates, args.duration, args.producers)
+        result["served"] = {"pool": wname, "producers": args.producers, "arrivals": "poisson",
+                            "duration_s_per_rate": args.duration, "hbm_budget_gib": 64, "rates": srv}
+        if "plain_gpu" in result:
+            result["served"]["top_rate_over_plain"] = round(srv[-1]["gcups"] / result["plain_gpu"]["value"], 2)
     if not args.no_cpu:
         log("CPU baseline")
         try:
             result["cpu_baseline"] = cpu_baseline(regions, args.cpu_seconds)
             result["cpu_baseline"]["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
+            if "served" in result:
+                result["served"]["top_rate_over_cpu"] = round(
+                    result["served"]["rates"][-1]["gcups"] / result["cpu_baseline"]["value"], 1)
         except Exception as e:  # noqa: BLE001
             result["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(result), flush=True)

@@ -143,7 +143,10 @@ typedef struct rvc_batch_opts {
                                     /* for reads up to 256 bases                      */
   uint32_t prior_table;             /* 0 = tuned choice, 1 = prior-table kernel for   */
                                     /* haplotypes without N, 2 = compare-and-select   */
-  uint32_t reserved[1];
+  uint32_t hap_run;                 /* haplotypes per tile: a tile sets up its reads  */
+                                    /* once and sweeps a run of haplotypes. 0 = up to */
+                                    /* the tuned maximum, fewer for small batches;    */
+                                    /* 1..64 = exactly that many                      */
 } rvc_batch_opts;
 
 typedef struct rvc_batch_info {
@@ -216,6 +219,8 @@ typedef struct rvc_server_stats {
   uint64_t cells, rescued_pairs;
   uint64_t resident_bytes, peak_resident_bytes;
   double busy_ms;          /* device time of all serving launches */
+  double stage_ms;         /* host time spent merging requests into launches */
+  uint64_t buffer_grows;   /* device arena and pinned buffer reallocations */
 } rvc_server_stats;
 int rvc_phmm_server_stats(rvc_server_stats* s);
 

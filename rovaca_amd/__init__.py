@@ -63,7 +63,7 @@ class RvcBatchOpts(ctypes.Structure):
     _fields_ = [("persistent_waves_per_cu", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
                 ("fp32_fold", ctypes.c_uint32), ("rpl_round", ctypes.c_uint32), ("occupancy", ctypes.c_uint32),
                 ("segment_width", ctypes.c_uint32), ("prior_table", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 1)]
+                ("hap_run", ctypes.c_uint32)]
 
 
 class RvcBatchInfo(ctypes.Structure):
@@ -83,7 +83,7 @@ class RvcServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("launches", ctypes.c_uint64),
                 ("cells", ctypes.c_uint64), ("rescued_pairs", ctypes.c_uint64),
                 ("resident_bytes", ctypes.c_uint64), ("peak_resident_bytes", ctypes.c_uint64),
-                ("busy_ms", ctypes.c_double)]
+                ("busy_ms", ctypes.c_double), ("stage_ms", ctypes.c_double), ("buffer_grows", ctypes.c_uint64)]
 
 
 class RvcLoadgenResult(ctypes.Structure):
@@ -212,7 +212,7 @@ class Batch:
     """Many regions packed, uploaded and launched together."""
 
     def __init__(self, regions, use_graph=False, fp32_fold=0, rpl_round=0, waves_per_cu=0, occupancy=0,
-                 segment_width=0, prior_table=0):
+                 segment_width=0, prior_table=0, hap_run=0):
         self._regions = regions
         arr = (RvcRegion * len(regions))()
         for x, r in enumerate(regions):
@@ -226,6 +226,7 @@ class Batch:
         opts.occupancy = occupancy
         opts.segment_width = segment_width
         opts.prior_table = prior_table
+        opts.hap_run = hap_run
         h = ctypes.c_void_p()
         _check(lib().rvc_phmm_batch_create(arr, len(regions), ctypes.byref(opts), ctypes.byref(h)))
         self.h = h

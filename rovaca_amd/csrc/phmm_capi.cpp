@@ -93,6 +93,7 @@ int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags) {
   SyncWorkspace* w = nullptr;
   if ((rc = get_workspace(&w))) return rc;
   if ((rc = build_host_pack(region, 1, policy(), &w->pack))) return rc;
+  split_runs(w->pack);
   w->batch.graph_exec = nullptr;
   w->batch.graph = nullptr;
   const BatchLayout L = batch_layout(pack_sizes(w->pack));
@@ -215,9 +216,14 @@ int rvc_phmm_batch_create(const rvc_region* regions, uint32_t n_regions, const r
       pol.force_w = (uint8_t)opts->segment_width;
     if (opts->prior_table == 1) pol.prior_table = true;
     if (opts->prior_table == 2) pol.prior_table = false;
+    if (opts->hap_run) {
+      pol.hap_run = std::min<uint32_t>(kMaxHapRun, opts->hap_run);
+      pol.hap_run_fixed = true;
+    }
   }
   int rc = build_host_pack(regions, n_regions, pol, &b->pack);
   if (rc) return rc;
+  split_runs(b->pack);
   if (hipSetDevice(g_engine.device) != hipSuccess || hipStreamCreateWithFlags(&b->dev.stream, hipStreamNonBlocking) != hipSuccess)
     return set_error(RVC_E_DEVICE, "batch stream");
   b->dev.own_stream = true;

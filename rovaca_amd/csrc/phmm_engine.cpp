@@ -158,8 +158,10 @@ FP32Policy& default_policy() {
   return p;
 }
 
-// Table format (text): lines "occ16 <17 ints>", "shape16 <17 ints>" (0 or
-// W*256+RPL), "steps3 0|1", "fold 0|1"; '#' starts a comment.
+// Table format (text): lines "occ16 <17 ints>", "occpt16 <17 ints>",
+// "occpt32 <17 ints>" (waves per SIMD), "shape16 <17 ints>" (0 or W*256+RPL),
+// "steps3 0|1", "fold 0|1", "table 0|1", "hap_run <1..64>"; '#' starts a
+// comment.
 int load_tune_table(const char* path, FP32Policy* pol) {
   FILE* f = fopen(path, "r");
   if (!f) return set_error(RVC_E_INVALID, "cannot open tune table %s", path);
@@ -192,6 +194,8 @@ int load_tune_table(const char* path, FP32Policy* pol) {
       p.rpl_steps_of_3 = v[0] != 0;
     } else if (!strcmp(key, "fold") && cnt == 1) {
       p.fold = v[0] != 0;
+    } else if (!strcmp(key, "hap_run") && cnt == 1) {
+      p.hap_run = (uint32_t)std::min<int>((int)kMaxHapRun, std::max(1, v[0]));
     } else {
       rc = set_error(RVC_E_INVALID, "tune table %s: cannot read line '%s'", path, key);
     }
@@ -232,22 +236,32 @@ static void pack_seq(const uint8_t* seq, uint32_t n, uint32_t off, std::vector<u
   }
 }
 
-// Appends tiles to dst ordered by haplotype length, longest first (a stable
-// counting sort: lengths are at most kMaxHapLen).
-static void append_longest_first(const std::vector<Tile>& tiles, const std::vector<uint32_t>& hap_len,
+// Appends tiles to dst ordered by cost (anti-diagonal steps of the tile),
+// largest first, so that the persistent waves end on small tiles. Costs are
+// bucketed by 16 steps; the order inside a bucket is kept.
+static void append_largest_first(const std::vector<Tile>& tiles, const std::vector<uint32_t>& cost,
                                  std::vector<Tile>& dst) {
   if (tiles.empty()) return;
-  std::vector<uint32_t> count(kMaxHapLen + 2, 0);
-  for (const Tile& t : tiles) count[kMaxHapLen - hap_len[t.hap]]++;
+  const size_t base = dst.size();
+  dst.resize(base + tiles.size());
+  if (tiles.size() < 512) {
+    std::vector<uint32_t> idx(tiles.size());
+    for (uint32_t x = 0; x < idx.size(); ++x) idx[x] = x;
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t p, uint32_t q) { return (cost[p] >> 4) > (cost[q] >> 4); });
+    for (size_t x = 0; x < idx.size(); ++x) dst[base + x] = tiles[idx[x]];
+    return;
+  }
+  constexpr uint32_t kBuckets = 8192;
+  std::vector<uint32_t> count(kBuckets + 1, 0);
+  auto bucket = [&](size_t x) { return kBuckets - std::min<uint32_t>(cost[x] >> 4, kBuckets); };
+  for (size_t x = 0; x < tiles.size(); ++x) count[bucket(x)]++;
   uint32_t acc = 0;
   for (auto& c : count) {
     const uint32_t n = c;
     c = acc;
     acc += n;
   }
-  const size_t base = dst.size();
-  dst.resize(base + tiles.size());
-  for (const Tile& t : tiles) dst[base + count[kMaxHapLen - hap_len[t.hap]]++] = t;
+  for (size_t x = 0; x < tiles.size(); ++x) dst[base + count[bucket(x)]++] = tiles[x];
 }
 
 int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Policy& pol, HostPack* hp) {
@@ -293,12 +307,16 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
   P.fp64_cap.assign(kNumFp64Bins, 0);
   P.waves_per_cu = pol.waves_per_cu;
   P.prior_table = pol.prior_table;
+  P.hap_run = std::max<uint32_t>(1, std::min<uint32_t>(kMaxHapRun, pol.hap_run));
+  P.hap_run_fixed = pol.hap_run_fixed;
   for (int b = 0; b < kFp32Bins; ++b) {
     int W, RPL;
     fp32_variant_shape(b % kFp32Variants, &W, &RPL);
     P.occ[b] = (uint8_t)pol.occupancy(W, RPL, b >= kFp32Variants);
   }
   std::vector<std::vector<Tile>> bins(kFp32Bins);
+  std::vector<std::vector<uint32_t>> bin_cost(kFp32Bins);
+  const uint32_t run_max = P.hap_run;
   std::vector<uint8_t> hap_has_n;
   for (uint32_t r = 0; r < n_regions; ++r) {
     const rvc_region& g = regions[r];
@@ -347,23 +365,40 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
       uint32_t nseg = 0;
       while (nseg < nseg_max && sx + nseg < g.n_reads && P.read_len[L.read0 + sx + nseg] > 0) ++nseg;
       const int v0 = fp32_variant_index(W, RPL);
-      for (uint32_t h = 0; h < g.n_haps; ++h) {
-        const uint32_t hl = P.hap_len[L.hap0 + h];
-        if (hl == 0) continue;
-        const int v = v0 + (hap_has_n[h] ? 0 : kFp32Variants);
+      // runs of consecutive non-empty haplotypes of the same kind (with or
+      // without N), at most run_max per tile
+      uint32_t h = 0;
+      while (h < g.n_haps) {
+        if (P.hap_len[L.hap0 + h] == 0) {
+          ++h;
+          continue;
+        }
+        const uint8_t kind = hap_has_n[h];
+        uint32_t nh = 0, cost = 0;
+        while (nh < run_max && h + nh < g.n_haps && P.hap_len[L.hap0 + h + nh] > 0 && hap_has_n[h + nh] == kind) {
+          cost += P.hap_len[L.hap0 + h + nh] + (uint32_t)W - 1;
+          ++nh;
+        }
+        const int v = v0 + (kind ? 0 : kFp32Variants);
         Tile t;
         t.hap = L.hap0 + h;
         t.read0 = L.read0 + sx;
         t.out0 = (uint32_t)(L.pair0 + (uint64_t)sx * g.n_haps + h);
-        t.meta = nseg | (g.n_haps << 4);
+        t.meta = tile_meta(nseg, g.n_haps, nh);
         bins[v].push_back(t);
-        for (uint32_t s = 0; s < nseg; ++s) {
-          const uint32_t rl = P.read_len[L.read0 + sx + s];
-          P.fp64_cap[fp64_bin(rl)] += 1;
-          P.n_pairs += 1;
-          P.n_cells += (uint64_t)rl * hl;
-          P.variant_cells[v] += (uint64_t)rl * hl;
+        bin_cost[v].push_back(cost);
+        P.bin_units[v] += nh;
+        for (uint32_t hh = h; hh < h + nh; ++hh) {
+          const uint32_t hl = P.hap_len[L.hap0 + hh];
+          for (uint32_t s = 0; s < nseg; ++s) {
+            const uint32_t rl = P.read_len[L.read0 + sx + s];
+            P.fp64_cap[fp64_bin(rl)] += 1;
+            P.n_pairs += 1;
+            P.n_cells += (uint64_t)rl * hl;
+            P.variant_cells[v] += (uint64_t)rl * hl;
+          }
         }
+        h += nh;
       }
       sx += nseg;
     }
@@ -372,13 +407,72 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
   P.bin_start.assign(kFp32Bins + 1, 0);
   for (int v = 0; v < kFp32Bins; ++v) {
     P.bin_start[v] = (uint32_t)P.tiles.size();
-    append_longest_first(bins[v], P.hap_len, P.tiles);
+    append_largest_first(bins[v], bin_cost[v], P.tiles);
   }
   P.bin_start[kFp32Bins] = (uint32_t)P.tiles.size();
   if (!pol.fold) P.fold_gap = false;
   P.fp64_base.assign(kNumFp64Bins + 1, 0);
   for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
   return RVC_OK;
+}
+
+uint32_t fp32_grid(const HostPack& P, int v) {
+  int W, RPL;
+  fp32_variant_shape(v % kFp32Variants, &W, &RPL);
+  const bool pt = v >= kFp32Variants && P.prior_table && P.fold_gap;
+  const int occ = pt ? P.occ[v] : P.occ[v % kFp32Variants];
+  int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL, P.fold_gap, occ, pt));
+  if (P.waves_per_cu) per_cu = std::min<int>(per_cu, (int)P.waves_per_cu);
+  return (uint32_t)per_cu * (uint32_t)g_engine.n_cu;
+}
+
+uint32_t run_for_bin(const HostPack& P, int v, uint64_t units) {
+  constexpr uint64_t kTilesPerWave = 32;
+  if (P.hap_run <= 1 || units == 0) return 1;
+  if (P.hap_run_fixed) return P.hap_run;
+  const uint64_t run = units / (kTilesPerWave * fp32_grid(P, v));
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(P.hap_run, run));
+}
+
+namespace {
+// Appends the pieces of tile t with at most 'run' haplotypes each.
+template <typename F>
+void split_tile(const Tile& t, uint32_t run, F&& emit) {
+  const uint32_t nh = tile_nh(t.meta), nseg = tile_nseg(t.meta), stride = tile_stride(t.meta);
+  for (uint32_t h = 0; h < nh; h += run) {
+    Tile p = t;
+    p.hap = t.hap + h;
+    p.out0 = t.out0 + h;
+    p.meta = tile_meta(nseg, stride, std::min(run, nh - h));
+    emit(p);
+  }
+}
+}  // namespace
+
+void split_runs(HostPack& P) {
+  std::vector<Tile> out;
+  std::vector<uint32_t> start(kFp32Bins + 1, 0);
+  std::vector<Tile> bin;
+  std::vector<uint32_t> cost;
+  for (int v = 0; v < kFp32Bins; ++v) {
+    start[v] = (uint32_t)out.size();
+    const uint32_t run = run_for_bin(P, v, P.bin_units[v]);
+    int W, RPL;
+    fp32_variant_shape(v % kFp32Variants, &W, &RPL);
+    bin.clear();
+    cost.clear();
+    for (uint32_t x = P.bin_start[v]; x < P.bin_start[v + 1]; ++x)
+      split_tile(P.tiles[x], run, [&](const Tile& p) {
+        uint32_t c = 0;
+        for (uint32_t h = 0; h < tile_nh(p.meta); ++h) c += P.hap_len[p.hap + h] + (uint32_t)W - 1;
+        bin.push_back(p);
+        cost.push_back(c);
+      });
+    append_largest_first(bin, cost, out);
+  }
+  start[kFp32Bins] = (uint32_t)out.size();
+  P.tiles.swap(out);
+  P.bin_start.swap(start);
 }
 
 int fp32_variant_index(int W, int RPL) {
@@ -486,8 +580,8 @@ void PinnedBuf::release() {
 
 int PinnedBuf::reserve(size_t bytes) {
   if (bytes <= cap) return RVC_OK;
+  const size_t want = std::max(bytes + bytes / 4 + (1u << 20), 2 * cap);
   release();
-  const size_t want = bytes + bytes / 4 + (1u << 20);
   if (hipHostMalloc((void**)&ptr, want, hipHostMallocDefault) != hipSuccess) {
     ptr = nullptr;
     return set_error(RVC_E_DEVICE, "pinned host buffer of %zu bytes", want);
@@ -505,7 +599,6 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
   auto hwords = [](const HostPack& A) { return (A.hap_pack.size() - 2) & ~size_t(1); };
   auto rwords = [](const HostPack& A) { return (A.read_pack.size() - 2) & ~size_t(1); };
   PackSizes z;
-  std::vector<uint64_t> bin_count(kFp32Bins, 0);
   P.fp64_cap.assign(kNumFp64Bins, 0);
   for (uint32_t x = 0; x < n; ++x) {
     const HostPack& A = *packs[x];
@@ -513,11 +606,32 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
     z.n_haps += A.hap_off.size();
     z.read_words += rwords(A);
     z.n_reads += A.read_off.size();
-    z.n_tiles += A.tiles.size();
     z.n_pairs_slots += A.n_pairs_slots;
-    for (int v = 0; v < kFp32Bins; ++v) bin_count[v] += A.bin_start[v + 1] - A.bin_start[v];
     for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_cap[b] += A.fp64_cap[b];
+    for (int v = 0; v < kFp32Bins; ++v) {
+      P.bin_units[v] += A.bin_units[v];
+      P.variant_cells[v] += A.variant_cells[v];
+    }
+    P.n_pairs += A.n_pairs;
+    P.n_cells += A.n_cells;
+    P.fold_gap = P.fold_gap && A.fold_gap;
+    P.waves_per_cu = A.waves_per_cu;
+    P.prior_table = A.prior_table;
+    P.hap_run = A.hap_run;
+    P.hap_run_fixed = A.hap_run_fixed;
+    std::memcpy(P.occ, A.occ, sizeof P.occ);
   }
+  // tiles of the merged batch: each pack's runs split for the batch's size
+  uint32_t run[kFp32Bins];
+  std::vector<uint64_t> bin_count(kFp32Bins, 0);
+  for (int v = 0; v < kFp32Bins; ++v) run[v] = run_for_bin(P, v, P.bin_units[v]);
+  for (uint32_t x = 0; x < n; ++x) {
+    const HostPack& A = *packs[x];
+    for (int v = 0; v < kFp32Bins; ++v)
+      for (uint32_t k = A.bin_start[v]; k < A.bin_start[v + 1]; ++k)
+        bin_count[v] += (tile_nh(A.tiles[k].meta) + run[v] - 1) / run[v];
+  }
+  for (int v = 0; v < kFp32Bins; ++v) z.n_tiles += bin_count[v];
   if (z.hap_words * 16 > 0xffffffffull || z.read_words * 16 > 0xffffffffull || z.n_pairs_slots > 0xffffffffull)
     return set_error(RVC_E_LIMIT, "merged batch too large for 32-bit offsets");
   z.hap_nwords = z.hap_words / 2 + 2;
@@ -576,26 +690,20 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
     }
     for (int v = 0; v < kFp32Bins; ++v) {
       Tile* t = tiles + bin_cur[v];
-      for (uint32_t k = A.bin_start[v]; k < A.bin_start[v + 1]; ++k, ++t) {
-        *t = A.tiles[k];
-        t->hap += (uint32_t)hidx;
-        t->read0 += (uint32_t)ridx;
-        t->out0 += (uint32_t)pbase;
+      for (uint32_t k = A.bin_start[v]; k < A.bin_start[v + 1]; ++k) {
+        Tile b = A.tiles[k];
+        b.hap += (uint32_t)hidx;
+        b.read0 += (uint32_t)ridx;
+        b.out0 += (uint32_t)pbase;
+        split_tile(b, run[v], [&](const Tile& q) { *t++ = q; });
       }
-      bin_cur[v] += A.bin_start[v + 1] - A.bin_start[v];
-      P.variant_cells[v] += A.variant_cells[v];
+      bin_cur[v] = (uint32_t)(t - tiles);
     }
     hw += aw;
     rw += bw;
     hidx += A.hap_off.size();
     ridx += A.read_off.size();
     pbase += A.n_pairs_slots;
-    P.n_pairs += A.n_pairs;
-    P.n_cells += A.n_cells;
-    P.fold_gap = P.fold_gap && A.fold_gap;
-    P.waves_per_cu = A.waves_per_cu;
-    P.prior_table = A.prior_table;
-    std::memcpy(P.occ, A.occ, sizeof P.occ);
   }
   // spare words and bytes at the ends, read by the last steps of a sweep
   std::memset(hpack + hw, 0, 8);
@@ -698,9 +806,7 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     a.rescue_count = rescue_count;
     a.rescue_base = rescue_base;
     const int occ = pt ? P.occ[v] : P.occ[v % kFp32Variants];
-    int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL, P.fold_gap, occ, pt));
-    if (P.waves_per_cu) per_cu = std::min<int>(per_cu, (int)P.waves_per_cu);
-    const int grid = (int)std::min<uint64_t>(n, (uint64_t)per_cu * g_engine.n_cu);
+    const int grid = (int)std::min<uint64_t>(n, fp32_grid(P, v));
     hipError_t e = launch_fp32(W, RPL, P.fold_gap, occ, pt, grid, s, a);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP32 launch W=%d RPL=%d: %s", W, RPL, hipGetErrorString(e));
     ++n_fp32_launches;

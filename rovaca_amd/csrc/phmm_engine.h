@@ -64,6 +64,8 @@ struct FP32Policy {
   bool prior_table = true;     // use the prior-table kernel where the haplotype has no N
   uint8_t force_occ = 0;       // batch option: 2/3/4 for every 16-lane kernel
   uint8_t force_w = 0;         // batch option: 16/32/64 lanes per read for reads <= 256
+  uint32_t hap_run = 16;       // most haplotypes per tile (consecutive, of one region)
+  bool hap_run_fixed = false;  // batch option: exactly hap_run per tile, whatever the batch size
   void pick(uint32_t R, int* W, int* RPL) const;
   int occupancy(int W, int RPL, bool pt) const {
     if (force_occ && (W == 16 || (pt && W == 32))) return force_occ;
@@ -103,6 +105,9 @@ struct HostPack {
   uint8_t occ[kFp32Bins] = {};            // occupancy target per FP32 bin
   uint64_t variant_cells[kFp32Bins] = {};  // cells computed by each FP32 bin
   bool prior_table = true;                 // launch table bins with the prior-table kernel
+  uint32_t hap_run = 1;                    // longest haplotype run a tile may keep
+  bool hap_run_fixed = false;              // keep runs of hap_run whatever the batch size
+  uint64_t bin_units[kFp32Bins] = {};      // (read group, haplotype) pairs per FP32 bin
   std::vector<char> staging;               // host image of the device inputs (upload)
   std::vector<double> nan_scratch;
   std::vector<uint8_t> flag_scratch;
@@ -145,7 +150,17 @@ struct PinnedBuf {
 // rescue bins, counts); its sequence arrays stay empty.
 int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, BatchLayout* L, PinnedBuf* buf);
 
+// Packs regions; tiles hold runs of up to pol.hap_run haplotypes, in
+// largest-first order per bin. Call split_runs before uploading such a pack
+// on its own (stage_merged splits on the fly).
 int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Policy& pol, HostPack* hp);
+// Persistent grid of FP32 bin v's kernel for this batch.
+uint32_t fp32_grid(const HostPack& P, int v);
+// Haplotypes per tile for FP32 bin v: long runs amortise a tile's set-up,
+// but every persistent wave needs a few dozen tiles so that all end together.
+uint32_t run_for_bin(const HostPack& P, int v, uint64_t units);
+// Splits the pack's tiles into runs of run_for_bin haplotypes, largest first.
+void split_runs(HostPack& P);
 
 // Copies a batch's results (device order, as downloaded) to input read order.
 void unpermute_results(const HostPack& P, const double* dev_out, const uint8_t* dev_flags, double* host_out,

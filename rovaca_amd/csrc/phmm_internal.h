@@ -45,14 +45,23 @@ struct DevSeqs {
   const uint32_t* read_len;
 };
 
-// FP32 work unit: one haplotype against up to 64/W consecutive reads.
-// The result of segment s goes to out[out0 + s * stride].
+// FP32 work unit: a run of consecutive haplotypes hap .. hap+nh-1 of one
+// region against up to 64/W consecutive reads; the reads' row parameters are
+// set up once per tile and reused for every haplotype of the run. The result
+// of segment s and haplotype hap+h goes to out[out0 + s * stride + h].
 struct Tile {
   uint32_t hap;
   uint32_t read0;
   uint32_t out0;
-  uint32_t meta;  // bits 0-3: number of segments in use; bits 4-31: stride
+  uint32_t meta;  // bits 0-3: segments in use; bits 4-16: stride; bits 17-23: nh
 };
+constexpr uint32_t kMaxHapRun = 64;
+__host__ __device__ inline uint32_t tile_meta(uint32_t nseg, uint32_t stride, uint32_t nh) {
+  return nseg | (stride << 4) | (nh << 17);
+}
+__host__ __device__ inline uint32_t tile_nseg(uint32_t meta) { return meta & 0xfu; }
+__host__ __device__ inline uint32_t tile_stride(uint32_t meta) { return (meta >> 4) & 0x1fffu; }
+__host__ __device__ inline uint32_t tile_nh(uint32_t meta) { return meta >> 17; }
 
 // One pair queued for the FP64 recomputation.
 struct RescueEntry {

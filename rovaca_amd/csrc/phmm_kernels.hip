@@ -46,7 +46,20 @@ struct Num<double> {
 };
 
 // Move v one lane down inside each W-lane segment; lane 0 of a segment
-// receives 'old'.
+// receives 0 (DPP bound control: no register has to be cleared first).
+template <int W>
+__device__ __forceinline__ float lane_down0(float v, int k) {
+  if constexpr (W == 16) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, true));
+  } else if constexpr (W == 64) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+  } else {
+    const float t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+    return k == 0 ? 0.f : t;
+  }
+}
+
+// The same, with lane 0 of a segment receiving 'old'.
 template <int W>
 __device__ __forceinline__ float lane_down(float v, float old, int k) {
   if constexpr (W == 16) {
@@ -121,7 +134,7 @@ __device__ __forceinline__ void sweep_step(const T (&M)[RPL], const T (&I)[RPL],
     const T pr = mt ? pm[0] : px[0];
     if constexpr (XF) nM[0] = pr * fma(tmm[0], dM, dI + dD);
     else nM[0] = pr * fma(tmm[0], dM, tgm[0] * (dI + dD));
-    nI[0] = fma(tmi[0], uM, tii[0] * uI);
+    nI[0] = fma(tii[0], uI, tmi[0] * uM);
     nD[0] = fma(tmd[0], M[0], tdd[0] * D[0]);
   }
 #pragma unroll
@@ -130,7 +143,7 @@ __device__ __forceinline__ void sweep_step(const T (&M)[RPL], const T (&I)[RPL],
     const T pr = mt ? pm[r] : px[r];
     if constexpr (XF) nM[r] = pr * fma(tmm[r], M[r - 1], I[r - 1] + D[r - 1]);
     else nM[r] = pr * fma(tmm[r], M[r - 1], tgm[r] * (I[r - 1] + D[r - 1]));
-    nI[r] = fma(tmi[r], nM[r - 1], tii[r] * nI[r - 1]);
+    nI[r] = fma(tii[r], nI[r - 1], tmi[r] * nM[r - 1]);
     nD[r] = fma(tmd[r], M[r], tdd[r] * D[r]);
   }
 }
@@ -232,9 +245,9 @@ __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, cons
   return S;
 }
 
-// Result of one FP32 segment: log10 likelihood, or a place on the FP64 list.
-__device__ __forceinline__ void finish_fp32(const Fp32Args& a, const Tile& tl, int seg, uint32_t read, float S) {
-  const uint32_t o = tl.out0 + (uint32_t)seg * (tl.meta >> 4);
+// Result of one FP32 pair (output slot o): log10 likelihood, or a place on
+// the FP64 list.
+__device__ __forceinline__ void finish_fp32(const Fp32Args& a, uint32_t o, uint32_t hap, uint32_t read, float S) {
   if (S >= 1e-28f) {
     a.out[o] = log10((double)S) - (double)36.1236000061f;
     a.flags[o] = 1;
@@ -242,18 +255,18 @@ __device__ __forceinline__ void finish_fp32(const Fp32Args& a, const Tile& tl, i
     const int b = fp64_bin(a.s.read_len[read]);
     const uint32_t idx = atomicAdd(a.rescue_count + b, 1u);
     RescueEntry e;
-    e.hap = tl.hap; e.read = read; e.out = o; e.pad = 0;
+    e.hap = hap; e.read = read; e.out = o; e.pad = 0;
     a.rescue[a.rescue_base[b] + idx] = e;
   }
 }
 
-// FP32 pass: persistent waves pull tiles (one haplotype x up to 64/W reads).
-// OCC = waves per SIMD the register allocation must allow (512 / OCC VGPRs
-// per lane): the sweep is issue-latency bound, so occupancy is a tuning knob.
+// FP32 pass: persistent waves pull tiles (a run of haplotypes x up to 64/W
+// reads). OCC = waves per SIMD the register allocation must allow (512 / OCC
+// VGPRs per lane): the sweep is issue-latency bound, so occupancy is a tuning
+// knob.
 template <int W, int RPL, bool XF, int OCC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void k_fp32(Fp32Args a) {
   __shared__ uint8_t hbuf[kHapBuf];
-  constexpr int NSEG = 64 / W;
   const int lane = threadIdx.x;
   const int seg = lane / W, k = lane % W;
   for (;;) {
@@ -262,18 +275,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     t = __builtin_amdgcn_readfirstlane(t);
     if (t >= a.n_tiles) break;
     const Tile tl = a.tiles[t];
-    const int nseg = (int)(tl.meta & 0xfu);
-    const uint32_t stride = tl.meta >> 4;
-    const bool active = seg < nseg;
+    const bool active = seg < (int)tile_nseg(tl.meta);
+    const uint32_t stride = tile_stride(tl.meta), nh = tile_nh(tl.meta);
     const uint32_t read = tl.read0 + (active ? (uint32_t)seg : 0u);
-    const uint32_t H = a.s.hap_len[tl.hap];
-    __syncthreads();
-    stage_hap(a.s, tl.hap, H, hbuf + 64, lane, 64);
-    __syncthreads();
-    const float S = forward<float, W, RPL, XF>(a.s, a.t, hbuf + 64, k, H, H, read);
-    if (active && k == W - 1) finish_fp32(a, tl, seg, read, S);
-    (void)NSEG;
-    (void)stride;
+    for (uint32_t h = 0; h < nh; ++h) {
+      const uint32_t hap = tl.hap + h;
+      const uint32_t H = a.s.hap_len[hap];
+      __syncthreads();
+      stage_hap(a.s, hap, H, hbuf + 64, lane, 64);
+      __syncthreads();
+      const float S = forward<float, W, RPL, XF>(a.s, a.t, hbuf + 64, k, H, H, read);
+      if (active && k == W - 1) finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, S);
+    }
   }
 }
 
@@ -299,13 +312,13 @@ __device__ __forceinline__ void sweep_step_pt(const float (&M)[RPL], const float
                                               const float (&tdd)[RPL]) {
   if constexpr (XF) nM[0] = pr[0] * fmaf(tmm[0], dM, dI + dD);
   else nM[0] = pr[0] * fmaf(tmm[0], dM, tgm[0] * (dI + dD));
-  nI[0] = fmaf(tmi[0], uM, tii[0] * uI);
+  nI[0] = fmaf(tii[0], uI, tmi[0] * uM);
   nD[0] = fmaf(tmd[0], M[0], tdd[0] * D[0]);
 #pragma unroll
   for (int r = 1; r < RPL; ++r) {
     if constexpr (XF) nM[r] = pr[r] * fmaf(tmm[r], M[r - 1], I[r - 1] + D[r - 1]);
     else nM[r] = pr[r] * fmaf(tmm[r], M[r - 1], tgm[r] * (I[r - 1] + D[r - 1]));
-    nI[r] = fmaf(tmi[r], nM[r - 1], tii[r] * nI[r - 1]);
+    nI[r] = fmaf(tii[r], nI[r - 1], tmi[r] * nM[r - 1]);
     nD[r] = fmaf(tmd[r], M[r], tdd[r] * D[r]);
   }
 }
@@ -325,19 +338,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     t = __builtin_amdgcn_readfirstlane(t);
     if (t >= a.n_tiles) break;
     const Tile tl = a.tiles[t];
-    const int nseg = (int)(tl.meta & 0xfu);
-    const bool active = seg < nseg;
+    const bool active = seg < (int)tile_nseg(tl.meta);
+    const uint32_t stride = tile_stride(tl.meta), nh = tile_nh(tl.meta);
     const uint32_t read = tl.read0 + (active ? (uint32_t)seg : 0u);
-    const uint32_t H = a.s.hap_len[tl.hap];
     __syncthreads();
-    {
-      const uint32_t w0 = a.s.hap_off[tl.hap] >> 4, nw = (H + 15) >> 4;
-      for (int x = lane; x < kHapWords; x += 64) {
-        const int hx = x - 4;
-        hpk[x] = (hx >= 0 && hx < (int)nw) ? a.s.hap_pack[w0 + hx] : 0u;
-      }
-    }
-    // per-row transition parameters (registers) and priors (LDS table)
+    // per-row transition parameters (registers) and priors (LDS table), once
+    // for all haplotypes of the tile
     const uint32_t R = a.s.read_len[read];
     const uint32_t roff = a.s.read_off[read];
     float tmm[RPL], tgm[RPL], tmi[RPL], tii[RPL], tmd[RPL], tdd[RPL];
@@ -389,10 +395,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
         v.w = (rc[4 * q + 3] < 0 || rc[4 * q + 3] == c) ? pm[4 * q + 3] : px[4 * q + 3];
         ptab[(c * NQ + q) * 64 + lane] = v;
       }
+    const float g0 = XF ? 1.f - ph2pr[a.s.rc[roff] & 127] : 1.f;
+    for (uint32_t h = 0; h < nh; ++h) {
+    const uint32_t hap = tl.hap + h;
+    const uint32_t H = a.s.hap_len[hap];
+    __syncthreads();
+    {
+      const uint32_t w0 = a.s.hap_off[hap] >> 4, nw = (H + 15) >> 4;
+      for (int x = lane; x < kHapWords; x += 64) {
+        const int hx = x - 4;
+        hpk[x] = (hx >= 0 && hx < (int)nw) ? a.s.hap_pack[w0 + hx] : 0u;
+      }
+    }
     __syncthreads();
 
     float initD = 0x1p120f / (float)H;
-    if constexpr (XF) initD = (1.f - ph2pr[a.s.rc[roff] & 127]) * initD;
+    if constexpr (XF) initD = g0 * initD;
     const float old_d = initD;
     float Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
 #pragma unroll
@@ -428,24 +446,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
       __builtin_amdgcn_sched_barrier(0);
       sweep_step_pt<RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
       if (st < nsteps) S += Mb[RPL - 1] + Ib[RPL - 1];
-      const float s1M = lane_down<W>(Mb[RPL - 1], 0.f, k);
-      const float s1I = lane_down<W>(Ib[RPL - 1], 0.f, k);
+      const float s1M = lane_down0<W>(Mb[RPL - 1], k);
+      const float s1I = lane_down0<W>(Ib[RPL - 1], k);
       const float s1D = lane_down<W>(Db[RPL - 1], old_d, k);
       // step st+1: b -> a
       load(cw, 2, pA);
       __builtin_amdgcn_sched_barrier(0);
       sweep_step_pt<RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, pB, tmm, tgm, tmi, tii, tmd, tdd);
       if (st + 1 < nsteps) S += Ma[RPL - 1] + Ia[RPL - 1];
-      const float s2M = lane_down<W>(Ma[RPL - 1], 0.f, k);
-      const float s2I = lane_down<W>(Ia[RPL - 1], 0.f, k);
+      const float s2M = lane_down0<W>(Ma[RPL - 1], k);
+      const float s2I = lane_down0<W>(Ia[RPL - 1], k);
       const float s2D = lane_down<W>(Da[RPL - 1], old_d, k);
       // step st+2: a -> b
       load(cw, 3, pB);
       __builtin_amdgcn_sched_barrier(0);
       sweep_step_pt<RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, pA, tmm, tgm, tmi, tii, tmd, tdd);
       if (st + 2 < nsteps) S += Mb[RPL - 1] + Ib[RPL - 1];
-      const float s3M = lane_down<W>(Mb[RPL - 1], 0.f, k);
-      const float s3I = lane_down<W>(Ib[RPL - 1], 0.f, k);
+      const float s3M = lane_down0<W>(Mb[RPL - 1], k);
+      const float s3I = lane_down0<W>(Ib[RPL - 1], k);
       const float s3D = lane_down<W>(Db[RPL - 1], old_d, k);
       // step st+3: b -> a
       cw = cw_next;
@@ -454,11 +472,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
       sweep_step_pt<RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, pB, tmm, tgm, tmi, tii, tmd, tdd);
       if (st + 3 < nsteps) S += Ma[RPL - 1] + Ia[RPL - 1];
       dM = s3M; dI = s3I; dD = s3D;
-      uM = lane_down<W>(Ma[RPL - 1], 0.f, k);
-      uI = lane_down<W>(Ia[RPL - 1], 0.f, k);
+      uM = lane_down0<W>(Ma[RPL - 1], k);
+      uI = lane_down0<W>(Ia[RPL - 1], k);
       uD = lane_down<W>(Da[RPL - 1], old_d, k);
     }
-    if (active && k == W - 1) finish_fp32(a, tl, seg, read, S);
+    if (active && k == W - 1) finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, S);
+    }
   }
 }
 

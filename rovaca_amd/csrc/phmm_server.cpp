@@ -49,12 +49,13 @@ struct ResultBuf {
   ~ResultBuf() {
     if (mem) hipHostFree(mem);
   }
-  int reserve(size_t n) {
+  int reserve(size_t n, bool* grew) {
     if (n <= cap && mem) return RVC_OK;
+    *grew = true;
     if (mem) hipHostFree(mem);
     mem = nullptr;
     cap = 0;
-    const size_t c = n + n / 4 + 4096;
+    const size_t c = std::max(n + n / 4 + 4096, 2 * cap);
     if (hipHostMalloc((void**)&mem, c * (sizeof(double) + 1) + 64 + 64, hipHostMallocDefault) != hipSuccess) {
       mem = nullptr;
       return set_error(RVC_E_DEVICE, "pinned result buffer for %zu pairs", n);
@@ -70,7 +71,7 @@ struct ResultBuf {
 struct ResultPool : std::enable_shared_from_this<ResultPool> {
   std::mutex mu;
   std::vector<std::unique_ptr<ResultBuf>> free;
-  std::shared_ptr<ResultBuf> acquire(size_t n, int* rc) {
+  std::shared_ptr<ResultBuf> acquire(size_t n, int* rc, bool* grew) {
     std::unique_ptr<ResultBuf> b;
     {
       std::lock_guard<std::mutex> lk(mu);
@@ -88,7 +89,7 @@ struct ResultPool : std::enable_shared_from_this<ResultPool> {
       *rc = set_error(RVC_E_INVALID, "out of host memory");
       return nullptr;
     }
-    if ((*rc = b->reserve(n))) return nullptr;
+    if ((*rc = b->reserve(n, grew))) return nullptr;
     std::shared_ptr<ResultPool> self = shared_from_this();
     return std::shared_ptr<ResultBuf>(b.release(), [self](ResultBuf* p) {
       std::lock_guard<std::mutex> lk(self->mu);
@@ -136,8 +137,13 @@ struct Slot {
 
 // Device bytes a packed region holds while in flight: its share of the
 // layout DeviceBatch::upload_staged builds (sequences, tiles, results, rescue
-// list, counters).
-uint64_t footprint(const HostPack& p) { return batch_layout(pack_sizes(p)).bytes; }
+// list, counters), with its tiles counted as if split into single haplotypes.
+uint64_t footprint(const HostPack& p) {
+  PackSizes z = pack_sizes(p);
+  z.n_tiles = 0;
+  for (int v = 0; v < kFp32Bins; ++v) z.n_tiles += p.bin_units[v];
+  return batch_layout(z).bytes;
+}
 
 struct Server {
   std::mutex mu;
@@ -152,6 +158,8 @@ struct Server {
   // statistics
   uint64_t submitted = 0, completed = 0, launches = 0, cells = 0, rescued = 0;
   double busy_ms = 0;
+  std::atomic<uint64_t> grows{0};
+  std::atomic<uint64_t> stage_ns{0};
 };
 
 Server* g_srv = nullptr;
@@ -167,24 +175,32 @@ int launch_slot(Slot& s) {
   packs.reserve(s.reqs.size());
   for (Request* r : s.reqs) packs.push_back(&r->pack);
   BatchLayout L;
+  const uint64_t t0 = now_ns();
+  const size_t staging_cap = s.staging.cap;
   int rc = stage_merged(packs.data(), (uint32_t)packs.size(), &s.meta, &L, &s.staging);
   if (rc) return rc;
+  g_srv->stage_ns += now_ns() - t0;
+  if (s.staging.cap != staging_cap) ++g_srv->grows;
   for (Request* r : s.reqs) r->pack = HostPack();  // the staged image is all the launch needs
   if (!s.mem || s.bytes < L.bytes) {
+    // grow by doubling: freeing device memory waits for the device
+    const size_t want = std::max(L.bytes + L.bytes / 4 + (16u << 20), 2 * s.bytes);
     if (s.mem) hipFree(s.mem);
     s.mem = nullptr;
     s.bytes = 0;
-    const size_t want = L.bytes + L.bytes / 4 + (16u << 20);
     if (hipMalloc(&s.mem, want) != hipSuccess) {
       s.mem = nullptr;
       return set_error(RVC_E_DEVICE, "device arena of %zu bytes", want);
     }
     s.bytes = want;
+    ++g_srv->grows;
   }
   if ((rc = s.batch.upload_staged(s.meta, L, s.staging.ptr, s.stream, s.mem, s.bytes))) return rc;
   const size_t n = s.meta.n_pairs_slots;
-  s.res = g_srv->pool->acquire(n, &rc);
+  bool grew = false;
+  s.res = g_srv->pool->acquire(n, &rc, &grew);
   if (!s.res) return rc;
+  if (grew) ++g_srv->grows;
   if ((rc = s.batch.ensure_events())) return rc;
   if ((rc = s.batch.enqueue(s.stream, true))) return rc;
   if (hipMemcpyAsync(s.res->out(), s.batch.out, n * sizeof(double), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
@@ -292,6 +308,12 @@ int server_start() {
     }
     s.batch.stream = s.stream;
     s.batch.own_stream = false;
+    // starting sizes that hold a few hundred typical regions, so that the
+    // first launches do not reallocate (each reallocation waits for the device)
+    const size_t arena = std::min<uint64_t>(g_engine.hbm_budget / kSlots + (16u << 20), 512ull << 20);
+    if (hipMalloc(&s.mem, arena) == hipSuccess) s.bytes = arena;
+    else s.mem = nullptr;
+    s.staging.reserve(128u << 20);
   }
   g_srv = S;
   for (Slot& s : S->slots) s.thr = std::thread(slot_main, &s);
@@ -428,6 +450,8 @@ int rvc_phmm_server_stats(rvc_server_stats* st) {
   st->resident_bytes = g_srv->resident;
   st->peak_resident_bytes = g_srv->peak_resident;
   st->busy_ms = g_srv->busy_ms;
+  st->stage_ms = (double)g_srv->stage_ns.load() * 1e-6;
+  st->buffer_grows = g_srv->grows.load();
   return RVC_OK;
 }
 

@@ -173,6 +173,32 @@ def test_batch_equals_alone():
     b.close()
 
 
+@pytest.mark.parametrize("run", [3, 64])
+def test_hap_runs_equal_alone(run):
+    say(f"tiles sweeping runs of {run} haplotypes equal each region alone (bit for bit)")
+    regs = [r for r in workload.batched_regions(16, seed=61) if r["n_haps"] * r["n_reads"] < 6000][:8]
+    # N in some haplotypes of the first region breaks its runs: those take the
+    # compare-and-select kernel, the others the prior-table kernel
+    r0 = dict(regs[0])
+    hb = r0["hap_bases"].copy()
+    for h in range(1, r0["n_haps"], 3):
+        hb[r0["hap_off"][h] + 5] = ord("N")
+    r0["hap_bases"] = hb
+    regs[0] = r0
+    b = rv.Batch(regs, hap_run=run)
+    b.run()
+    out, flags = b.results()
+    for x, reg in enumerate(regs):
+        alone, af = rv.region_sync(reg)
+        mixed = b.region_result(out, x)
+        assert np.array_equal(np.isnan(alone), np.isnan(mixed))
+        assert np.array_equal(alone[~np.isnan(alone)], mixed[~np.isnan(mixed)])
+        assert np.array_equal(af, b.region_result(flags, x))
+    names = {k["name"] for k in b.kernels() if k["precision"] == 32}
+    assert any(n.startswith("k_fp32pt") for n in names) and any(n.startswith("k_fp32<") for n in names)
+    b.close()
+
+
 def test_graph_replays_match_eager():
     say("hipGraph replays equal the eager run")
     regs = [r for r in workload.stress_regions(8, seed=41) if r["n_haps"] * r["n_reads"] < 5000]
