@@ -28,7 +28,9 @@ typedef struct rvc_loadgen_result {
 
 /* Submits region order[i] (or i, if order is NULL) at arrival_s[i] seconds
  * after the start, for i in [0, n_arrivals), from n_producers threads
- * (arrival i goes to producer i % n_producers); then waits for every ticket.
+ * (arrival i goes to producer i % n_producers). Submissions never wait for
+ * results; a collector thread per producer waits for that producer's tickets
+ * in order as they are submitted.
  * The serving loop must be running (rvc_config.server_threads = 1).
  * latency_ms (optional, n_arrivals entries) receives each request's
  * submit -> results-ready time. */

@@ -2,7 +2,9 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -39,19 +41,37 @@ extern "C" int rvc_loadgen_run(const rvc_region* regions, uint32_t n_regions, co
   std::vector<rvc_ticket_timing> timing(n_arrivals);
   std::vector<int> wait_rc(n_arrivals, 0);
   const Clock::time_point start = Clock::now() + std::chrono::milliseconds(20);
+  // Each producer submits at its arrival times (open loop: a submission never
+  // waits for an earlier result) while a collector thread of its own waits for
+  // its tickets in order, as the worker that would consume each result.
   auto producer = [&](uint32_t p) {
+    std::mutex mu;
+    std::condition_variable cv;
+    uint32_t n_sub = 0;  // this producer's arrivals submitted so far
+    std::thread collector([&] {
+      uint32_t k = 0;
+      for (uint32_t i = p; i < n_arrivals; i += n_producers, ++k) {
+        {
+          std::unique_lock<std::mutex> lk(mu);
+          cv.wait(lk, [&] { return n_sub > k; });
+        }
+        if (sub_rc[i] != RVC_OK) continue;
+        wait_rc[i] = rvc_phmm_wait(tickets[i], nullptr, nullptr, -1, &timing[i]);
+      }
+    });
     for (uint32_t i = p; i < n_arrivals; i += n_producers) {
       const Clock::time_point due = start + std::chrono::nanoseconds((int64_t)(arrival_s[i] * 1e9));
       std::this_thread::sleep_until(due);
       late[i] = std::chrono::duration<double, std::milli>(Clock::now() - due).count();
       const rvc_region* rg = &regions[order ? order[i] : i % n_regions];
       sub_rc[i] = rvc_phmm_submit(rg, &tickets[i]);
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        ++n_sub;
+      }
+      cv.notify_one();
     }
-    // then collect this producer's tickets (results are not kept)
-    for (uint32_t i = p; i < n_arrivals; i += n_producers) {
-      if (sub_rc[i] != RVC_OK) continue;
-      wait_rc[i] = rvc_phmm_wait(tickets[i], nullptr, nullptr, -1, &timing[i]);
-    }
+    collector.join();
   };
   std::vector<std::thread> th;
   for (uint32_t p = 0; p < n_producers; ++p) th.emplace_back(producer, p);
