@@ -116,6 +116,21 @@ __device__ __forceinline__ void stage_hap(const DevSeqs& s, uint32_t hap, uint32
   }
 }
 
+// I = tMI*M_up + tII*I_up. In float the product off the row-to-row chain
+// (tMI*M_up) is rounded and tII*I_up is fused, so the chain down a lane's rows
+// is one FMA per row; in double the other way round keeps the FP64 kernels
+// within 256 registers (two waves per SIMD).
+template <typename T>
+__device__ __forceinline__ T i_update(T tmi, T um, T tii, T ui);
+template <>
+__device__ __forceinline__ float i_update<float>(float tmi, float um, float tii, float ui) {
+  return fmaf(tii, ui, tmi * um);
+}
+template <>
+__device__ __forceinline__ double i_update<double>(double tmi, double um, double tii, double ui) {
+  return fma(tmi, um, tii * ui);
+}
+
 // One anti-diagonal step of a lane: new column j of all RPL rows from
 // column j-1 (M, I, D) and the row above (d* at j-1, u* at j).
 // XF = false: I and D are the model's values and
@@ -134,7 +149,7 @@ __device__ __forceinline__ void sweep_step(const T (&M)[RPL], const T (&I)[RPL],
     const T pr = mt ? pm[0] : px[0];
     if constexpr (XF) nM[0] = pr * fma(tmm[0], dM, dI + dD);
     else nM[0] = pr * fma(tmm[0], dM, tgm[0] * (dI + dD));
-    nI[0] = fma(tii[0], uI, tmi[0] * uM);
+    nI[0] = i_update(tmi[0], uM, tii[0], uI);
     nD[0] = fma(tmd[0], M[0], tdd[0] * D[0]);
   }
 #pragma unroll
@@ -143,7 +158,7 @@ __device__ __forceinline__ void sweep_step(const T (&M)[RPL], const T (&I)[RPL],
     const T pr = mt ? pm[r] : px[r];
     if constexpr (XF) nM[r] = pr * fma(tmm[r], M[r - 1], I[r - 1] + D[r - 1]);
     else nM[r] = pr * fma(tmm[r], M[r - 1], tgm[r] * (I[r - 1] + D[r - 1]));
-    nI[r] = fma(tii[r], nI[r - 1], tmi[r] * nM[r - 1]);
+    nI[r] = i_update(tmi[r], nM[r - 1], tii[r], nI[r - 1]);
     nD[r] = fma(tmd[r], M[r], tdd[r] * D[r]);
   }
 }
