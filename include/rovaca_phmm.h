@@ -123,7 +123,9 @@ int rvc_phmm_region_sync(const rvc_region* region, double* out_read_major, uint8
  * first *n_kept columns belong to kept_reads[0 .. *n_kept), the rest stay
  * NaN. On failure every entry is NaN, *n_kept = n_reads, kept_reads lists all
  * reads and the return code says why (the reference logs and returns the
- * all-NaN matrix, :605-609). */
+ * all-NaN matrix, :605-609). Thread-safe: with the serving loop running
+ * (rvc_config.server_threads = 1) the regions of concurrent callers are
+ * batched together on the device; otherwise each call runs its own region. */
 int rvc_phmm_call_pairhmm(const rvc_raw_region* region, int32_t min_quality_threshold, int32_t pcr_option,
                           double* out_hap_major, uint32_t* kept_reads, uint32_t* n_kept);
 

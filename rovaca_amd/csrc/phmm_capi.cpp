@@ -184,8 +184,11 @@ int rvc_phmm_call_pairhmm(const rvc_raw_region* rg, int32_t min_quality_threshol
   reg.i = gop.data();
   reg.d = gop.data();
   reg.c = gcp.data();
+  // with the serving loop running, concurrent callers (Rovaca's worker
+  // threads) are batched together; otherwise the region runs on its own
   std::vector<double> read_major((size_t)H * N);
-  int rc = region_sync_impl(&reg, read_major.data(), nullptr);
+  int rc = server_running() ? served_region(&reg, read_major.data(), nullptr)
+                            : region_sync_impl(&reg, read_major.data(), nullptr);
   if (rc) return fail(rc);
   *n_kept = (uint32_t)postprocess(N, H, read_major.data(), rlen.data(), out_hap_major, kept_reads);
   return RVC_OK;

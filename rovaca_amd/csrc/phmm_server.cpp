@@ -348,6 +348,8 @@ void server_stop() {
   delete S;
 }
 
+bool server_running() { return g_srv != nullptr; }
+
 }  // namespace rvc
 
 using namespace rvc;
@@ -456,3 +458,12 @@ int rvc_phmm_server_stats(rvc_server_stats* st) {
 }
 
 }  // extern "C"
+
+namespace rvc {
+int served_region(const rvc_region* region, double* out, uint8_t* flags) {
+  rvc_ticket t = 0;
+  const int rc = rvc_phmm_submit(region, &t);
+  if (rc) return rc;
+  return rvc_phmm_wait(t, out, flags, -1, nullptr);
+}
+}  // namespace rvc

@@ -8,6 +8,10 @@ namespace rvc {
 
 int server_start();
 void server_stop();
+bool server_running();
+// One region through the serving loop (submit + wait), as a worker thread
+// would call it.
+int served_region(const rvc_region* region, double* out, uint8_t* flags);
 int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags);
 
 }  // namespace rvc

@@ -273,6 +273,36 @@ def test_call_pairhmm_region_adapter():
         assert np.abs(mat[m] - want[m]).max() < TOL
 
 
+def test_call_pairhmm_concurrent_equals_alone():
+    say("call_pairhmm from 12 threads at once (served, batched together) equals each call alone")
+    rng = np.random.default_rng(71)
+    regs = [r for r in workload.batched_regions(24, seed=72) if r["n_haps"] * r["n_reads"] < 6000][:12]
+    calls = []
+    for reg in regs:
+        H, N = reg["n_haps"], reg["n_reads"]
+        haps = [reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]] for h in range(H)]
+        reads = [reg["read_bases"][reg["read_off"][r]:reg["read_off"][r + 1]] for r in range(N)]
+        quals = [reg["q"][reg["read_off"][r]:reg["read_off"][r + 1]] for r in range(N)]
+        mqs = rng.integers(20, 61, N).astype(np.uint8)
+        calls.append((haps, reads, quals, mqs))
+    alone = [rv.call_pairhmm(*c, min_quality_threshold=10, pcr=rv.PCR_CONSERVATIVE) for c in calls]
+    mixed = [None] * len(calls)
+
+    def worker(x):
+        mixed[x] = rv.call_pairhmm(*calls[x], min_quality_threshold=10, pcr=rv.PCR_CONSERVATIVE)
+
+    th = [threading.Thread(target=worker, args=(x,)) for x in range(len(calls))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for (rc0, m0, k0), (rc1, m1, k1) in zip(alone, mixed):
+        assert rc0 == 0 and rc1 == 0
+        assert np.array_equal(k0, k1)
+        assert np.array_equal(np.isnan(m0), np.isnan(m1))
+        assert np.array_equal(m0[~np.isnan(m0)], m1[~np.isnan(m1)])
+
+
 def test_loadgen_serves_poisson_stream():
     say("load generator: 300 regions at 3000/s from 8 producers")
     regs = [r for r in workload.batched_regions(40, seed=71) if r["n_haps"] * r["n_reads"] < 6000]
