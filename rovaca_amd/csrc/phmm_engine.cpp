@@ -207,17 +207,20 @@ int load_tune_table(const char* path, FP32Policy* pol) {
 }
 
 void FP32Policy::pick(uint32_t R, int* W, int* RPL) const {
+  // rows for R + 1: the segment always ends in a padding row, whose I holds
+  // M + I of the read's last row (the prior-table kernel sums it directly)
+  const uint32_t Rp = R + 1;
   // 16-lane shapes come in steps of 3 rows per lane, so that reads of nearby
   // lengths share one launch (each launch has a tail of its own)
   static const int kRpl16[17] = {0, 1, 2, 4, 4, 7, 7, 7, 10, 10, 10, 13, 13, 13, 16, 16, 16};
-  const int exact = (int)std::max<uint32_t>(1, (R + 15) / 16);
-  if (R <= 256) { *W = 16; *RPL = rpl_steps_of_3 ? kRpl16[exact] : exact; }
-  else if (R <= 512) { *W = 32; *RPL = (int)((R + 31) / 32); }
-  else { *W = 64; *RPL = (int)((R + 63) / 64); }
-  if (R <= 256 && force_w) {
+  const int exact = (int)std::max<uint32_t>(1, (Rp + 15) / 16);
+  if (Rp <= 256) { *W = 16; *RPL = rpl_steps_of_3 ? kRpl16[exact] : exact; }
+  else if (Rp <= 512) { *W = 32; *RPL = (int)((Rp + 31) / 32); }
+  else { *W = 64; *RPL = (int)((Rp + 63) / 64); }
+  if (Rp <= 256 && force_w) {
     *W = force_w;
-    *RPL = (int)std::max<uint32_t>(1, (R + force_w - 1) / force_w);
-  } else if (R <= 256 && rpl16_override[exact] != 0) {
+    *RPL = (int)std::max<uint32_t>(1, (Rp + force_w - 1) / force_w);
+  } else if (Rp <= 256 && rpl16_override[exact] != 0) {
     *W = rpl16_override[exact] >> 8;
     *RPL = rpl16_override[exact] & 0xff;
   }

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <array>
+#include <type_traits>
 #include <cstdint>
 #include <utility>
 
@@ -70,6 +71,20 @@ __device__ __forceinline__ float lane_down(float v, float old, int k) {
     const float t =
         __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138, 0xf, 0xf, false));
     return k == 0 ? old : t;
+  }
+}
+
+// The same, where lane 0 of a segment keeps the value 'keep' already holds:
+// with a loop-carried 'keep' that starts as the boundary value, the DPP move
+// writes only lanes 1.. and no register copy is needed per step. W = 32 must
+// still select lane 32's boundary explicitly.
+template <int W>
+__device__ __forceinline__ float lane_down_keep(float v, float keep, float old, int k) {
+  if constexpr (W == 32) {
+    return lane_down<W>(v, old, k);
+  } else {
+    constexpr int ctrl = W == 16 ? 0x111 : 0x138;
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(keep), __float_as_int(v), ctrl, 0xf, 0xf, false));
   }
 }
 
@@ -432,6 +447,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     for (int r = 0; r < RPL; ++r) { Ma[r] = 0.f; Ia[r] = 0.f; Da[r] = 0.f; }
     float uM = 0.f, uI = 0.f, uD = (k == 0) ? initD : 0.f;
     float dM = 0.f, dI = 0.f, dD = uD;
+    // D moved down a lane: lane 0 keeps the row -1 value that these
+    // loop-carried registers already hold (no register copy per step)
+    float s1D = old_d, s2D = old_d, s3D = old_d;
     float S = 0.f;
     const int nsteps = (int)H + W - 1;
     const char* tab = reinterpret_cast<const char*>(ptab) + lane * 16;
@@ -452,44 +470,43 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     float pA[NQ * 4], pB[NQ * 4];
     uint32_t cw = window(0);
     load(cw, 0, pA);
+    // Four steps a -> b -> a -> b -> a per iteration. The read's last row is
+    // always a padding row (the host picks RPL with R < W * RPL), which holds
+    // M[R-1][j] + I[R-1][j] in I, so the sum takes one add per step.
     // sched_barrier(0) keeps each step's table reads ahead of the previous
     // step's arithmetic, so their latency overlaps a whole step.
     for (int st = 0; st < nsteps; st += 4) {
       const uint32_t cw_next = window(st + 4);
-      // step st: a -> b
       load(cw, 1, pB);
       __builtin_amdgcn_sched_barrier(0);
       sweep_step_pt<RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
-      if (st < nsteps) S += Mb[RPL - 1] + Ib[RPL - 1];
+      S += Ib[RPL - 1];
       const float s1M = lane_down0<W>(Mb[RPL - 1], k);
       const float s1I = lane_down0<W>(Ib[RPL - 1], k);
-      const float s1D = lane_down<W>(Db[RPL - 1], old_d, k);
-      // step st+1: b -> a
+      s1D = lane_down_keep<W>(Db[RPL - 1], s1D, old_d, k);
       load(cw, 2, pA);
       __builtin_amdgcn_sched_barrier(0);
       sweep_step_pt<RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, pB, tmm, tgm, tmi, tii, tmd, tdd);
-      if (st + 1 < nsteps) S += Ma[RPL - 1] + Ia[RPL - 1];
+      if (st + 1 < nsteps) S += Ia[RPL - 1];
       const float s2M = lane_down0<W>(Ma[RPL - 1], k);
       const float s2I = lane_down0<W>(Ia[RPL - 1], k);
-      const float s2D = lane_down<W>(Da[RPL - 1], old_d, k);
-      // step st+2: a -> b
+      s2D = lane_down_keep<W>(Da[RPL - 1], s2D, old_d, k);
       load(cw, 3, pB);
       __builtin_amdgcn_sched_barrier(0);
       sweep_step_pt<RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, pA, tmm, tgm, tmi, tii, tmd, tdd);
-      if (st + 2 < nsteps) S += Mb[RPL - 1] + Ib[RPL - 1];
+      if (st + 2 < nsteps) S += Ib[RPL - 1];
       const float s3M = lane_down0<W>(Mb[RPL - 1], k);
       const float s3I = lane_down0<W>(Ib[RPL - 1], k);
-      const float s3D = lane_down<W>(Db[RPL - 1], old_d, k);
-      // step st+3: b -> a
+      s3D = lane_down_keep<W>(Db[RPL - 1], s3D, old_d, k);
       cw = cw_next;
       load(cw, 0, pA);
       __builtin_amdgcn_sched_barrier(0);
       sweep_step_pt<RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, pB, tmm, tgm, tmi, tii, tmd, tdd);
-      if (st + 3 < nsteps) S += Ma[RPL - 1] + Ia[RPL - 1];
+      if (st + 3 < nsteps) S += Ia[RPL - 1];
       dM = s3M; dI = s3I; dD = s3D;
       uM = lane_down0<W>(Ma[RPL - 1], k);
       uI = lane_down0<W>(Ia[RPL - 1], k);
-      uD = lane_down<W>(Da[RPL - 1], old_d, k);
+      uD = lane_down_keep<W>(Da[RPL - 1], uD, old_d, k);
     }
     if (active && k == W - 1) finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, S);
     }
@@ -566,8 +583,8 @@ const Row16 kFp32W16[2][3] = {
      fp32_row<16, true, 4>(std::make_integer_sequence<int, 16>{})}};
 const Row16 kFp32W32[2] = {fp32_row<32, false, 2>(std::make_integer_sequence<int, 16>{}),
                            fp32_row<32, true, 2>(std::make_integer_sequence<int, 16>{})};
-const std::array<Fp32Fn, 12> kFp32W64[2] = {fp32_row<64, false, 2>(std::make_integer_sequence<int, 12>{}),
-                                            fp32_row<64, true, 2>(std::make_integer_sequence<int, 12>{})};
+const std::array<Fp32Fn, 13> kFp32W64[2] = {fp32_row<64, false, 2>(std::make_integer_sequence<int, 13>{}),
+                                            fp32_row<64, true, 2>(std::make_integer_sequence<int, 13>{})};
 
 const std::array<Fp64Fn, kNumFp64Bins> kFp64 = {{&k_fp64<16, 4>, &k_fp64<16, 8>, &k_fp64<32, 8>, &k_fp64<64, 12>}};
 
@@ -579,14 +596,14 @@ const Row16 kFp32PtW16[3] = {fp32pt_row<16, 2>(std::make_integer_sequence<int, 1
 const Row16 kFp32PtW32[3] = {fp32pt_row<32, 2>(std::make_integer_sequence<int, 16>{}),
                              fp32pt_row<32, 3>(std::make_integer_sequence<int, 16>{}),
                              fp32pt_row<32, 4>(std::make_integer_sequence<int, 16>{})};
-const std::array<Fp32Fn, 12> kFp32PtW64 = fp32pt_row<64, 2>(std::make_integer_sequence<int, 12>{});
+const std::array<Fp32Fn, 13> kFp32PtW64 = fp32pt_row<64, 2>(std::make_integer_sequence<int, 13>{});
 
 Fp32Fn fp32pt_fn(int W, int RPL, int occ) {
   if (RPL < 1) return nullptr;
   const int o = std::min(4, std::max(2, occ)) - 2;
   if (W == 16 && RPL <= 16) return kFp32PtW16[o][RPL - 1];
   if (W == 32 && RPL <= 16) return kFp32PtW32[o][RPL - 1];
-  if (W == 64 && RPL <= 12) return kFp32PtW64[RPL - 1];
+  if (W == 64 && RPL <= 13) return kFp32PtW64[RPL - 1];
   return nullptr;
 }
 
@@ -595,7 +612,7 @@ Fp32Fn fp32_fn(int W, int RPL, bool xf, int occ, bool pt) {
   if (RPL < 1) return nullptr;
   if (W == 16 && RPL <= 16) return kFp32W16[xf][std::min(4, std::max(2, occ)) - 2][RPL - 1];
   if (W == 32 && RPL <= 16) return kFp32W32[xf][RPL - 1];
-  if (W == 64 && RPL <= 12) return kFp32W64[xf][RPL - 1];
+  if (W == 64 && RPL <= 13) return kFp32W64[xf][RPL - 1];
   return nullptr;
 }
 }  // namespace

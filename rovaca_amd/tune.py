@@ -1,7 +1,7 @@
 """Tile-shape tuner for the FP32 kernel.
 
 For every read length class of the 16-lane kernel (RPL = 1..16 rows per
-lane, reads of 16*RPL bases) it times, on a fixed synthetic batch, the
+lane, reads of 16*RPL - 1 bases: a segment always keeps one padding row) it times, on a fixed synthetic batch, the
 compare-and-select kernel at 2, 3 and 4 waves per SIMD and the prior-table
 kernel with 16, 32 and 64 lanes per read at each occupancy, and writes the
 fastest as a table that the engine loads at init (tuned/mi355x.txt beside the
@@ -68,7 +68,7 @@ def main():
         return ms, cells / (ms * 1e-3) / 1e9
 
     for rpl in range(1, 17):
-        R = 16 * rpl
+        R = 16 * rpl - 1  # the longest read of this row count (one padding row stays)
         regs = truncated(pool, R)
         cells = workload.cells(regs)
         # compare-and-select kernels (used for haplotypes that contain N): occupancy of the 16-lane shape
@@ -83,9 +83,9 @@ def main():
         # prior-table kernels (haplotypes without N): shape and occupancy
         best, bpt = None, {16: None, 32: None}
         cands = [(16, o) for o in (2, 3, 4)]
-        if -(-R // 32) <= 16:
+        if -(-(R + 1) // 32) <= 16:
             cands += [(32, o) for o in (2, 3, 4)]
-        if -(-R // 64) <= 12:
+        if -(-(R + 1) // 64) <= 13:
             cands.append((64, 2))
         for w, occ in cands:
             ms, g = time_batch(regs, cells, occupancy=occ, segment_width=w, prior_table=1)
@@ -96,12 +96,12 @@ def main():
             if w in bpt and (bpt[w] is None or ms < bpt[w][1]):
                 bpt[w] = (occ, ms)
         occpt16[rpl] = bpt[16][0]
-        # the 32-lane table kernel of this read length has ceil(R/32) rows per lane
+        # the 32-lane table kernel of this read length has ceil((R+1)/32) rows per lane
         if bpt[32] is not None:
-            occpt32[-(-R // 32)] = bpt[32][0]
+            occpt32[-(-(R + 1) // 32)] = bpt[32][0]
         w, occ, _ = best
         if w != 16:
-            shape16[rpl] = w * 256 + (-(-R // w))
+            shape16[rpl] = w * 256 + (-(-(R + 1) // w))
     occ16[0], occpt16[0], occpt32[0] = occ16[1], occpt16[1], occpt32[1]
     rv.shutdown()
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
