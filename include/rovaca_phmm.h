@@ -141,7 +141,7 @@ typedef struct rvc_batch_opts {
                                     /* 2 = rows per lane in steps of 3                */
   uint32_t occupancy;               /* 0 = tuned choice, else 2/3/4 waves per SIMD    */
                                     /* for the 16-lane kernels                        */
-  uint32_t segment_width;           /* 0 = tuned choice, else 16/32/64 lanes per read */
+  uint32_t segment_width;           /* 0 = tuned choice, else 8/16/32/64 lanes per read */
                                     /* for reads up to 256 bases                      */
   uint32_t prior_table;             /* 0 = tuned choice, 1 = prior-table kernel for   */
                                     /* haplotypes without N, 2 = compare-and-select   */

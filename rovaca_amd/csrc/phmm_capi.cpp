@@ -215,7 +215,8 @@ int rvc_phmm_batch_create(const rvc_region* regions, uint32_t n_regions, const r
     if (opts->rpl_round == 2) pol.rpl_steps_of_3 = true;
     pol.waves_per_cu = opts->persistent_waves_per_cu;
     if (opts->occupancy) pol.force_occ = (uint8_t)std::min(4u, std::max(2u, opts->occupancy));
-    if (opts->segment_width == 16 || opts->segment_width == 32 || opts->segment_width == 64)
+    if (opts->segment_width == 8 || opts->segment_width == 16 || opts->segment_width == 32 ||
+        opts->segment_width == 64)
       pol.force_w = (uint8_t)opts->segment_width;
     if (opts->prior_table == 1) pol.prior_table = true;
     if (opts->prior_table == 2) pol.prior_table = false;

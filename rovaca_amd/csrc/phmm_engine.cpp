@@ -159,7 +159,8 @@ FP32Policy& default_policy() {
 }
 
 // Table format (text): lines "occ16 <17 ints>", "occpt16 <17 ints>",
-// "occpt32 <17 ints>" (waves per SIMD), "shape16 <17 ints>" (0 or W*256+RPL),
+// "occpt32 <17 ints>", "occpt8 <17 ints>" (waves per SIMD), "shape16 <17
+// ints>" (0 or W*256+RPL, RPL 0 = fewest rows of width W),
 // "steps3 0|1", "fold 0|1", "table 0|1", "hap_run <1..64>"; '#' starts a
 // comment.
 int load_tune_table(const char* path, FP32Policy* pol) {
@@ -183,10 +184,15 @@ int load_tune_table(const char* path, FP32Policy* pol) {
       for (int x = 0; x < 17; ++x) p.occpt32[x] = (uint8_t)std::min(4, std::max(2, v[x]));
     } else if (!strcmp(key, "table") && cnt == 1) {
       p.prior_table = v[0] != 0;
+    } else if (!strcmp(key, "occpt8") && cnt == 17) {
+      for (int x = 0; x < 17; ++x) p.occpt8[x] = (uint8_t)std::min(4, std::max(2, v[x]));
     } else if (!strcmp(key, "shape16") && cnt == 17) {
       for (int x = 0; x < 17; ++x) {
         const int w = v[x] >> 8, r = v[x] & 0xff;
-        const bool ok = v[x] == 0 || ((w == 16 || w == 32 || w == 64) && r >= 1 && r * w >= x * 16 && fp32_supported(w, r));
+        // RPL 0: the fewest rows of width w for the longest read of class x
+        const int rr = r ? r : (x * 16 + w - 1) / std::max(1, w);
+        const bool ok = v[x] == 0 || ((w == 8 || w == 16 || w == 32 || w == 64) && rr >= 1 &&
+                                      rr * w >= x * 16 && fp32_supported(w, rr));
         if (!ok) { rc = set_error(RVC_E_INVALID, "tune table %s: bad shape16 entry %d", path, v[x]); break; }
         p.rpl16_override[x] = (uint16_t)v[x];
       }
@@ -223,6 +229,10 @@ void FP32Policy::pick(uint32_t R, int* W, int* RPL) const {
   } else if (Rp <= 256 && rpl16_override[exact] != 0) {
     *W = rpl16_override[exact] >> 8;
     *RPL = rpl16_override[exact] & 0xff;
+    if (*RPL == 0) {  // fewest rows of width W, in steps of 3 like the 16-lane shapes
+      *RPL = (int)((Rp + *W - 1) / *W);
+      if (rpl_steps_of_3 && *RPL <= 16) *RPL = kRpl16[*RPL];
+    }
   }
 }
 
@@ -479,11 +489,12 @@ void split_runs(HostPack& P) {
 }
 
 int fp32_variant_index(int W, int RPL) {
-  const int w = W == 16 ? 0 : W == 32 ? 1 : 2;
+  const int w = W == 16 ? 0 : W == 32 ? 1 : W == 64 ? 2 : 3;
   return w * 16 + (RPL - 1);
 }
 void fp32_variant_shape(int v, int* W, int* RPL) {
-  *W = v / 16 == 0 ? 16 : v / 16 == 1 ? 32 : 64;
+  static const int kW[4] = {16, 32, 64, 8};
+  *W = kW[v / 16];
   *RPL = v % 16 + 1;
 }
 

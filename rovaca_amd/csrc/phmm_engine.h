@@ -37,10 +37,10 @@ int engine_init(const rvc_config* cfg);
 int engine_shutdown();
 int validate_region(const rvc_region* r);
 
-// FP32 kernel shapes: W in {16, 32, 64} x RPL in 1..16. Tile bins are
+// FP32 kernel shapes: W in {16, 32, 64, 8} x RPL in 1..16. Tile bins are
 // [table][shape]: table = 1 for haplotypes without N (prior-table kernel
 // allowed), 0 otherwise.
-constexpr int kFp32Variants = 48;
+constexpr int kFp32Variants = 64;
 constexpr int kFp32Bins = 2 * kFp32Variants;
 int fp32_variant_index(int W, int RPL);
 void fp32_variant_shape(int v, int* W, int* RPL);
@@ -48,9 +48,10 @@ int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ, bool pt);
 int fp64_blocks_per_cu(int bin);
 
 // Chooses the (W, RPL) kernel for a tile whose longest read has R bases.
-// rpl16_override[rpl] (for reads up to 256 bases, indexed by the minimal
-// 16-lane row count) may name another shape as (W << 8) | RPL; it is filled
-// by the tile tuner.
+// rpl16_override[x] (for reads up to 255 bases, indexed by the minimal
+// 16-lane row count x of R + 1 rows) may name another shape as
+// (W << 8) | RPL, where RPL = 0 means the fewest rows of that width; it is
+// filled by the tile tuner.
 struct FP32Policy {
   uint16_t rpl16_override[17] = {};
   // waves per SIMD the 16-lane kernel of each RPL is compiled for (2, 3, 4)
@@ -61,6 +62,7 @@ struct FP32Policy {
   // the same for the prior-table kernels (W = 16 and W = 32)
   uint8_t occpt16[17] = {4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 2, 2, 2, 2, 2, 2};
   uint8_t occpt32[17] = {4, 4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 3, 2, 2, 2, 2};
+  uint8_t occpt8[17] = {4, 4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 3, 4, 4, 4, 4};
   bool prior_table = true;     // use the prior-table kernel where the haplotype has no N
   uint8_t force_occ = 0;       // batch option: 2/3/4 for every 16-lane kernel
   uint8_t force_w = 0;         // batch option: 16/32/64 lanes per read for reads <= 256
@@ -68,8 +70,8 @@ struct FP32Policy {
   bool hap_run_fixed = false;  // batch option: exactly hap_run per tile, whatever the batch size
   void pick(uint32_t R, int* W, int* RPL) const;
   int occupancy(int W, int RPL, bool pt) const {
-    if (force_occ && (W == 16 || (pt && W == 32))) return force_occ;
-    if (pt) return W == 16 ? occpt16[RPL] : W == 32 ? occpt32[RPL] : 2;
+    if (force_occ && (W == 16 || (pt && (W == 32 || W == 8)))) return force_occ;
+    if (pt) return W == 16 ? occpt16[RPL] : W == 32 ? occpt32[RPL] : W == 8 ? occpt8[RPL] : 2;
     return W == 16 ? occ16[RPL] : 2;
   }
 };

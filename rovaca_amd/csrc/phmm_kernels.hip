@@ -46,11 +46,33 @@ struct Num<double> {
   __device__ static double C() { return 0x1p1020; }
 };
 
+// Segment and position of a lane. W = 8 interleaves two segments in each
+// 16-lane DPP row (even and odd lanes), so that row_shr:2 moves a value one
+// position down inside its segment and the row's lanes 0 and 1 are the two
+// segments' first positions; the other widths use consecutive lanes.
+template <int W>
+__device__ __forceinline__ int seg_of(int lane) {
+  if constexpr (W == 8) return ((lane >> 4) << 1) | (lane & 1);
+  else return lane / W;
+}
+template <int W>
+__device__ __forceinline__ int pos_of(int lane) {
+  if constexpr (W == 8) return (lane & 15) >> 1;
+  else return lane % W;
+}
+
+// DPP control of the one-position move: row_shr:2 (W = 8), row_shr:1 (16),
+// wave_shr:1 (32, 64).
+template <int W>
+constexpr int kDownCtrl = W == 8 ? 0x112 : W == 16 ? 0x111 : 0x138;
+
 // Move v one lane down inside each W-lane segment; lane 0 of a segment
 // receives 0 (DPP bound control: no register has to be cleared first).
 template <int W>
 __device__ __forceinline__ float lane_down0(float v, int k) {
-  if constexpr (W == 16) {
+  if constexpr (W == 8) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xf, 0xf, true));
+  } else if constexpr (W == 16) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, true));
   } else if constexpr (W == 64) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
@@ -63,7 +85,9 @@ __device__ __forceinline__ float lane_down0(float v, int k) {
 // The same, with lane 0 of a segment receiving 'old'.
 template <int W>
 __device__ __forceinline__ float lane_down(float v, float old, int k) {
-  if constexpr (W == 16) {
+  if constexpr (W == 8) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x112, 0xf, 0xf, false));
+  } else if constexpr (W == 16) {
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x111, 0xf, 0xf, false));
   } else if constexpr (W == 64) {
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138, 0xf, 0xf, false));
@@ -83,8 +107,8 @@ __device__ __forceinline__ float lane_down_keep(float v, float keep, float old, 
   if constexpr (W == 32) {
     return lane_down<W>(v, old, k);
   } else {
-    constexpr int ctrl = W == 16 ? 0x111 : 0x138;
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(keep), __float_as_int(v), ctrl, 0xf, 0xf, false));
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(keep), __float_as_int(v), kDownCtrl<W>, 0xf, 0xf, false));
   }
 }
 
@@ -298,7 +322,7 @@ template <int W, int RPL, bool XF, int OCC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void k_fp32(Fp32Args a) {
   __shared__ uint8_t hbuf[kHapBuf];
   const int lane = threadIdx.x;
-  const int seg = lane / W, k = lane % W;
+  const int seg = seg_of<W>(lane), k = pos_of<W>(lane);
   for (;;) {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(a.head, 1u);
@@ -359,7 +383,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
   __shared__ float4 ptab[4 * NQ * 64];
   __shared__ uint32_t hpk[kHapWords];
   const int lane = threadIdx.x;
-  const int seg = lane / W, k = lane % W;
+  const int seg = seg_of<W>(lane), k = pos_of<W>(lane);
   const float* ph2pr = a.t.ph2pr_f;
   const float* mmt = a.t.mm_f;
   for (;;) {
@@ -585,11 +609,13 @@ const Row16 kFp32W32[2] = {fp32_row<32, false, 2>(std::make_integer_sequence<int
                            fp32_row<32, true, 2>(std::make_integer_sequence<int, 16>{})};
 const std::array<Fp32Fn, 13> kFp32W64[2] = {fp32_row<64, false, 2>(std::make_integer_sequence<int, 13>{}),
                                             fp32_row<64, true, 2>(std::make_integer_sequence<int, 13>{})};
+const Row16 kFp32W8[2] = {fp32_row<8, false, 2>(std::make_integer_sequence<int, 16>{}),
+                          fp32_row<8, true, 2>(std::make_integer_sequence<int, 16>{})};
 
 const std::array<Fp64Fn, kNumFp64Bins> kFp64 = {{&k_fp64<16, 4>, &k_fp64<16, 8>, &k_fp64<32, 8>, &k_fp64<64, 12>}};
 
-// Prior-table variants (folded form only): W = 16 and 32 at 2, 3, 4 waves per
-// SIMD, W = 64 at 2.
+// Prior-table variants (folded form only): W = 8, 16 and 32 at 2, 3, 4 waves
+// per SIMD, W = 64 at 2.
 const Row16 kFp32PtW16[3] = {fp32pt_row<16, 2>(std::make_integer_sequence<int, 16>{}),
                              fp32pt_row<16, 3>(std::make_integer_sequence<int, 16>{}),
                              fp32pt_row<16, 4>(std::make_integer_sequence<int, 16>{})};
@@ -597,6 +623,9 @@ const Row16 kFp32PtW32[3] = {fp32pt_row<32, 2>(std::make_integer_sequence<int, 1
                              fp32pt_row<32, 3>(std::make_integer_sequence<int, 16>{}),
                              fp32pt_row<32, 4>(std::make_integer_sequence<int, 16>{})};
 const std::array<Fp32Fn, 13> kFp32PtW64 = fp32pt_row<64, 2>(std::make_integer_sequence<int, 13>{});
+const Row16 kFp32PtW8[3] = {fp32pt_row<8, 2>(std::make_integer_sequence<int, 16>{}),
+                            fp32pt_row<8, 3>(std::make_integer_sequence<int, 16>{}),
+                            fp32pt_row<8, 4>(std::make_integer_sequence<int, 16>{})};
 
 Fp32Fn fp32pt_fn(int W, int RPL, int occ) {
   if (RPL < 1) return nullptr;
@@ -604,6 +633,7 @@ Fp32Fn fp32pt_fn(int W, int RPL, int occ) {
   if (W == 16 && RPL <= 16) return kFp32PtW16[o][RPL - 1];
   if (W == 32 && RPL <= 16) return kFp32PtW32[o][RPL - 1];
   if (W == 64 && RPL <= 13) return kFp32PtW64[RPL - 1];
+  if (W == 8 && RPL <= 16) return kFp32PtW8[o][RPL - 1];
   return nullptr;
 }
 
@@ -613,6 +643,7 @@ Fp32Fn fp32_fn(int W, int RPL, bool xf, int occ, bool pt) {
   if (W == 16 && RPL <= 16) return kFp32W16[xf][std::min(4, std::max(2, occ)) - 2][RPL - 1];
   if (W == 32 && RPL <= 16) return kFp32W32[xf][RPL - 1];
   if (W == 64 && RPL <= 13) return kFp32W64[xf][RPL - 1];
+  if (W == 8 && RPL <= 16) return kFp32W8[xf][RPL - 1];
   return nullptr;
 }
 }  // namespace
@@ -651,8 +682,8 @@ namespace rvc {
 
 // Resident 64-lane blocks per CU for a kernel (register and LDS limited).
 int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ, bool pt) {
-  static int cache[2][2][3][3][17] = {};
-  const int w = W == 16 ? 0 : W == 32 ? 1 : 2;
+  static int cache[2][2][3][4][17] = {};
+  const int w = W == 16 ? 0 : W == 32 ? 1 : W == 64 ? 2 : 3;
   const int o = std::min(4, std::max(2, occ)) - 2;
   if (RPL < 1 || RPL > 16) return 1;
   int& slot = cache[pt][xf][o][w][RPL];

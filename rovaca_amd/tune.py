@@ -3,14 +3,15 @@
 For every read length class of the 16-lane kernel (RPL = 1..16 rows per
 lane, reads of 16*RPL - 1 bases: a segment always keeps one padding row) it times, on a fixed synthetic batch, the
 compare-and-select kernel at 2, 3 and 4 waves per SIMD and the prior-table
-kernel with 16, 32 and 64 lanes per read at each occupancy, and writes the
+kernel with 8, 16, 32 and 64 lanes per read at each occupancy, and writes the
 fastest as a table that the engine loads at init (tuned/mi355x.txt beside the
 library):
 
     occ16   <waves per SIMD, compare-and-select 16-lane kernel, index = RPL 0..16>
     occpt16 <the same for the prior-table 16-lane kernel>
     occpt32 <the same for the prior-table 32-lane kernel, index = its own RPL>
-    shape16 <0 = 16 lanes, else W*256 + RPL, index = RPL 0..16>
+    occpt8  <the same for the prior-table 8-lane kernel, index = its own RPL>
+    shape16 <0 = 16 lanes, else W*256 + RPL (RPL 0: fewest rows of W lanes), index = RPL 0..16>
 
 Usage: python -m rovaca_amd.tune [--out PATH] [--regions N]
 """
@@ -56,7 +57,7 @@ def main():
     pool = base_pool(args.regions)
     print(f"[tune] pool of {len(pool)} regions in {time.time() - t0:.1f} s", flush=True)
     rv.init(0, tune_table=os.devnull)  # start from the built-in defaults
-    occ16, occpt16, occpt32 = [4] * 17, [4] * 17, [4] * 17
+    occ16, occpt16, occpt32, occpt8 = [4] * 17, [4] * 17, [4] * 17, [4] * 17
     shape16 = [0] * 17
     report = []
 
@@ -81,8 +82,10 @@ def main():
                 best16 = (occ, ms)
         occ16[rpl] = best16[0]
         # prior-table kernels (haplotypes without N): shape and occupancy
-        best, bpt = None, {16: None, 32: None}
+        best, bpt = None, {16: None, 32: None, 8: None}
         cands = [(16, o) for o in (2, 3, 4)]
+        if -(-(R + 1) // 8) <= 16:
+            cands += [(8, o) for o in (2, 3, 4)]
         if -(-(R + 1) // 32) <= 16:
             cands += [(32, o) for o in (2, 3, 4)]
         if -(-(R + 1) // 64) <= 13:
@@ -99,10 +102,18 @@ def main():
         # the 32-lane table kernel of this read length has ceil((R+1)/32) rows per lane
         if bpt[32] is not None:
             occpt32[-(-(R + 1) // 32)] = bpt[32][0]
+        if bpt[8] is not None:
+            occpt8[-(-(R + 1) // 8)] = bpt[8][0]
         w, occ, _ = best
-        if w != 16:
+        if w == 8:
+            shape16[rpl] = 8 * 256  # rows per lane follow the read length
+        elif w != 16:
             shape16[rpl] = w * 256 + (-(-(R + 1) // w))
     occ16[0], occpt16[0], occpt32[0] = occ16[1], occpt16[1], occpt32[1]
+    # 8-lane shapes were timed at even row counts; odd ones take the next one's
+    for r in range(15, 0, -2):
+        occpt8[r] = occpt8[r + 1]
+    occpt8[0] = occpt8[1]
     rv.shutdown()
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     with open(args.out, "w") as f:
@@ -113,6 +124,7 @@ def main():
         f.write("occ16 " + " ".join(str(x) for x in occ16) + "\n")
         f.write("occpt16 " + " ".join(str(x) for x in occpt16) + "\n")
         f.write("occpt32 " + " ".join(str(x) for x in occpt32) + "\n")
+        f.write("occpt8 " + " ".join(str(x) for x in occpt8) + "\n")
         f.write("shape16 " + " ".join(str(x) for x in shape16) + "\n")
         f.write("steps3 1\nfold 1\ntable 1\n")
     print(f"[tune] wrote {args.out}", flush=True)

@@ -199,6 +199,40 @@ def test_hap_runs_equal_alone(run):
     b.close()
 
 
+@pytest.mark.parametrize("width", [8, 32])
+def test_segment_width_equals_default(width):
+    say(f"{width}-lane segments give the same bits as the default shapes")
+    rng = np.random.default_rng(81)
+    regs = []
+    for x in range(6):
+        reg = workload.batched_regions(1, seed=82 + x)[0]
+        # reads of 20..120 bases (the 8-lane shapes hold up to 127)
+        H, N = reg["n_haps"], min(reg["n_reads"], 40)
+        haps = [reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]] for h in range(H)]
+        reads, quals = [], []
+        for r in range(N):
+            o = reg["read_off"][r]
+            n = int(rng.integers(20, 121))
+            reads.append(reg["read_bases"][o:o + n])
+            quals.append(reg["q"][o:o + n])
+        if x == 0:  # N in a haplotype: compare-and-select kernel
+            haps[1] = haps[1].copy()
+            haps[1][3] = ord("N")
+        regs.append(workload.make_region(haps, reads, quals))
+    b = rv.Batch(regs, segment_width=width)
+    b.run()
+    out, flags = b.results()
+    names = {k["name"] for k in b.kernels() if k["precision"] == 32}
+    assert all(f"<{width}," in n for n in names)
+    for x, reg in enumerate(regs):
+        alone, af = rv.region_sync(reg)
+        mixed = b.region_result(out, x)
+        assert np.array_equal(np.isnan(alone), np.isnan(mixed))
+        assert np.array_equal(alone[~np.isnan(alone)], mixed[~np.isnan(mixed)])
+        assert np.array_equal(af, b.region_result(flags, x))
+    b.close()
+
+
 def test_graph_replays_match_eager():
     say("hipGraph replays equal the eager run")
     regs = [r for r in workload.stress_regions(8, seed=41) if r["n_haps"] * r["n_reads"] < 5000]
