@@ -317,7 +317,7 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
   P.rd.assign(read_bases + 64, 0);
   P.rc.assign(read_bases + 64, 0);
   uint32_t hcur = 0, rcur = 0;
-  P.fp64_cap.assign(kNumFp64Bins, 0);
+  P.fp64_cap.assign(kNumFp64Lists, 0);
   P.waves_per_cu = pol.waves_per_cu;
   P.prior_table = pol.prior_table;
   P.hap_run = std::max<uint32_t>(1, std::min<uint32_t>(kMaxHapRun, pol.hap_run));
@@ -405,7 +405,7 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
           const uint32_t hl = P.hap_len[L.hap0 + hh];
           for (uint32_t s = 0; s < nseg; ++s) {
             const uint32_t rl = P.read_len[L.read0 + sx + s];
-            P.fp64_cap[fp64_bin(rl)] += 1;
+            P.fp64_cap[(kind ? 0 : kNumFp64Bins) + fp64_bin(rl)] += 1;
             P.n_pairs += 1;
             P.n_cells += (uint64_t)rl * hl;
             P.variant_cells[v] += (uint64_t)rl * hl;
@@ -424,8 +424,8 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
   }
   P.bin_start[kFp32Bins] = (uint32_t)P.tiles.size();
   if (!pol.fold) P.fold_gap = false;
-  P.fp64_base.assign(kNumFp64Bins + 1, 0);
-  for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
+  P.fp64_base.assign(kNumFp64Lists + 1, 0);
+  for (int b = 0; b < kNumFp64Lists; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
   return RVC_OK;
 }
 
@@ -535,7 +535,7 @@ PackSizes pack_sizes(const HostPack& P) {
   z.n_reads = P.read_off.size();
   z.n_tiles = P.tiles.size();
   z.n_pairs_slots = P.n_pairs_slots;
-  z.n_rescue = P.fp64_base.empty() ? 0 : P.fp64_base[kNumFp64Bins];
+  z.n_rescue = P.fp64_base.empty() ? 0 : P.fp64_base[kNumFp64Lists];
   return z;
 }
 
@@ -556,7 +556,7 @@ BatchLayout batch_layout(const PackSizes& z) {
   L.roff = take(4 * z.n_reads);
   L.rlen = take(4 * z.n_reads);
   L.tiles = take(sizeof(Tile) * (z.n_tiles + 1));
-  L.rbase = take(4 * kNumFp64Bins);
+  L.rbase = take(4 * kNumFp64Lists);
   L.in_bytes = off;
   L.out = take(sizeof(double) * (z.n_pairs_slots + 1));
   L.flags = take(z.n_pairs_slots + 8);
@@ -581,7 +581,7 @@ void stage_pack(const HostPack& P, const BatchLayout& L, char* dst) {
   put(L.roff, P.read_off.data(), bytes_of(P.read_off));
   put(L.rlen, P.read_len.data(), bytes_of(P.read_len));
   put(L.tiles, P.tiles.data(), bytes_of(P.tiles));
-  put(L.rbase, P.fp64_base.data(), kNumFp64Bins * sizeof(uint32_t));
+  put(L.rbase, P.fp64_base.data(), kNumFp64Lists * sizeof(uint32_t));
 }
 
 PinnedBuf::~PinnedBuf() { release(); }
@@ -613,7 +613,7 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
   auto hwords = [](const HostPack& A) { return (A.hap_pack.size() - 2) & ~size_t(1); };
   auto rwords = [](const HostPack& A) { return (A.read_pack.size() - 2) & ~size_t(1); };
   PackSizes z;
-  P.fp64_cap.assign(kNumFp64Bins, 0);
+  P.fp64_cap.assign(kNumFp64Lists, 0);
   for (uint32_t x = 0; x < n; ++x) {
     const HostPack& A = *packs[x];
     z.hap_words += hwords(A);
@@ -621,7 +621,7 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
     z.read_words += rwords(A);
     z.n_reads += A.read_off.size();
     z.n_pairs_slots += A.n_pairs_slots;
-    for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_cap[b] += A.fp64_cap[b];
+    for (int b = 0; b < kNumFp64Lists; ++b) P.fp64_cap[b] += A.fp64_cap[b];
     for (int v = 0; v < kFp32Bins; ++v) {
       P.bin_units[v] += A.bin_units[v];
       P.variant_cells[v] += A.variant_cells[v];
@@ -653,9 +653,9 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
   z.qual_bytes = z.read_words * 16 + 64;
   z.read_nwords = z.read_words / 2 + 2;
   z.read_words += 2;
-  P.fp64_base.assign(kNumFp64Bins + 1, 0);
-  for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
-  z.n_rescue = P.fp64_base[kNumFp64Bins];
+  P.fp64_base.assign(kNumFp64Lists + 1, 0);
+  for (int b = 0; b < kNumFp64Lists; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
+  z.n_rescue = P.fp64_base[kNumFp64Lists];
   const BatchLayout L = batch_layout(z);
   int rc = buf->reserve(L.in_bytes);
   if (rc) return rc;
@@ -726,7 +726,7 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
   std::memset(rn + rw / 2, 0, 8);
   for (int k = 0; k < 4; ++k) std::memset(quals[k] + rw * 16, 0, 64);
   std::memset(static_cast<void*>(tiles + z.n_tiles), 0, sizeof(Tile));
-  std::memcpy(dst + L.rbase, P.fp64_base.data(), kNumFp64Bins * sizeof(uint32_t));
+  std::memcpy(dst + L.rbase, P.fp64_base.data(), kNumFp64Lists * sizeof(uint32_t));
   P.n_pairs_slots = z.n_pairs_slots;
   P.n_regions = (uint32_t)P.region.size();
   *Lout = L;
@@ -784,7 +784,7 @@ int DeviceBatch::upload_staged(const HostPack& P, const BatchLayout& L, const vo
 int DeviceBatch::enqueue(hipStream_t s, bool timed) {
   const HostPack& P = *pack;
   uint32_t* heads = counters;
-  uint32_t* rescue_count = counters + kFp32Bins + kNumFp64Bins;
+  uint32_t* rescue_count = counters + kFp32Bins + kNumFp64Lists;
   RVC_HIP(launch_zero_u32(counters, n_counters_, s));
   if (timed) RVC_HIP(hipEventRecord(ev[0], s));
   n_fp32_launches = 0;
@@ -819,6 +819,7 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     a.rescue = rescue;
     a.rescue_count = rescue_count;
     a.rescue_base = rescue_base;
+    a.rescue_list0 = v >= kFp32Variants ? (uint32_t)kNumFp64Bins : 0u;
     const int occ = pt ? P.occ[v] : P.occ[v % kFp32Variants];
     const int grid = (int)std::min<uint64_t>(n, fp32_grid(P, v));
     hipError_t e = launch_fp32(W, RPL, P.fold_gap, occ, pt, grid, s, a);
@@ -833,28 +834,30 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     if ((rc = mark((int)launches.size()))) return rc;
   }
   if (timed) RVC_HIP(hipEventRecord(ev[1], s));
-  for (int b = 0; b < kNumFp64Bins; ++b) {
-    if (P.fp64_cap[b] == 0) continue;
+  for (int l = 0; l < kNumFp64Lists; ++l) {
+    if (P.fp64_cap[l] == 0) continue;
+    const int b = l % kNumFp64Bins;
+    const bool pt = l >= kNumFp64Bins && P.prior_table && P.fold_gap;
     Fp64Args a;
     a.s = seqs;
     a.t = g_engine.tables;
-    a.list = rescue + P.fp64_base[b];
-    a.count = rescue_count + b;
-    a.head = heads + kFp32Bins + b;
+    a.list = rescue + P.fp64_base[l];
+    a.count = rescue_count + l;
+    a.head = heads + kFp32Bins + l;
     a.out = out;
     a.flags = flags;
-    const int per_cu = std::max(1, fp64_blocks_per_cu(b));
+    const int per_cu = std::max(1, fp64_blocks_per_cu(b, pt));
     int W, RPL;
     fp64_bin_shape(b, &W, &RPL);
-    const uint64_t max_waves = (P.fp64_cap[b] + (64 / W) - 1) / (64 / W);
+    const uint64_t max_waves = (P.fp64_cap[l] + (64 / W) - 1) / (64 / W);
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(max_waves, (uint64_t)per_cu * g_engine.n_cu));
-    hipError_t e = launch_fp64(b, grid, s, a);
-    if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP64 launch bin %d: %s", b, hipGetErrorString(e));
+    hipError_t e = launch_fp64(b, pt, grid, s, a);
+    if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP64 launch list %d: %s", l, hipGetErrorString(e));
     ++n_fp64_launches;
     rvc_kernel_time kt{};
-    snprintf(kt.name, sizeof kt.name, "k_fp64<%d,%d>", W, RPL);
+    snprintf(kt.name, sizeof kt.name, "k_fp64%s<%d,%d>", pt ? "pt" : "", W, RPL);
     kt.precision = 64; kt.width = W; kt.rows_per_lane = RPL; kt.occupancy = 0;
-    kt.tiles = P.fp64_cap[b];
+    kt.tiles = P.fp64_cap[l];
     launches.push_back(kt);
     if ((rc = mark((int)launches.size()))) return rc;
   }
@@ -908,10 +911,10 @@ int DeviceBatch::run(bool use_graph_) {
     }
   }
   // rescued pair count of this run
-  uint32_t cnt[kNumFp64Bins];
-  RVC_HIP(hipMemcpy(cnt, counters + kFp32Bins + kNumFp64Bins, sizeof cnt, hipMemcpyDeviceToHost));
+  uint32_t cnt[kNumFp64Lists];
+  RVC_HIP(hipMemcpy(cnt, counters + kFp32Bins + kNumFp64Lists, sizeof cnt, hipMemcpyDeviceToHost));
   n_rescued = 0;
-  for (int b = 0; b < kNumFp64Bins; ++b) n_rescued += cnt[b];
+  for (int b = 0; b < kNumFp64Lists; ++b) n_rescued += cnt[b];
   return RVC_OK;
 }
 

@@ -45,7 +45,7 @@ constexpr int kFp32Bins = 2 * kFp32Variants;
 int fp32_variant_index(int W, int RPL);
 void fp32_variant_shape(int v, int* W, int* RPL);
 int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ, bool pt);
-int fp64_blocks_per_cu(int bin);
+int fp64_blocks_per_cu(int bin, bool pt);
 
 // Chooses the (W, RPL) kernel for a tile whose longest read has R bases.
 // rpl16_override[x] (for reads up to 255 bases, indexed by the minimal
@@ -97,8 +97,8 @@ struct HostPack {
   std::vector<uint32_t> read_perm;  // device read slot -> input read index within its region
   std::vector<Tile> tiles;          // grouped by FP32 bin
   std::vector<uint32_t> bin_start;  // kFp32Bins + 1
-  std::vector<uint32_t> fp64_cap;   // pairs per FP64 bin (capacity of its rescue list)
-  std::vector<uint32_t> fp64_base;  // kNumFp64Bins + 1
+  std::vector<uint32_t> fp64_cap;   // pairs per rescue list (its capacity)
+  std::vector<uint32_t> fp64_base;  // kNumFp64Lists + 1
   uint64_t n_pairs_slots = 0;       // sum of n_haps * n_reads
   uint64_t n_pairs = 0;             // pairs with two non-empty sides
   uint64_t n_cells = 0;
@@ -125,7 +125,7 @@ PackSizes pack_sizes(const HostPack& P);
 
 // Byte offsets of a batch in its device allocation. The inputs come first,
 // so one host-to-device copy of [0, in_bytes) uploads a staged batch.
-constexpr uint32_t kNumCounters = kFp32Bins + 2 * kNumFp64Bins;
+constexpr uint32_t kNumCounters = kFp32Bins + 2 * kNumFp64Lists;
 struct BatchLayout {
   size_t hpack = 0, hn = 0, hoff = 0, hlen = 0, rpack = 0, rn = 0, rq = 0, ri = 0, rd = 0, rc = 0;
   size_t roff = 0, rlen = 0, tiles = 0, rbase = 0;

@@ -82,6 +82,10 @@ __host__ __device__ inline int fp64_bin(uint32_t R) {
   if (R <= 256) return 2;   // W=32, RPL 8
   return 3;                 // W=64, RPL 12
 }
+// Rescue lists: one per FP64 bin for pairs whose haplotype has N (list b)
+// and one per bin for the others (list kNumFp64Bins + b), which the
+// prior-table FP64 kernel can take.
+constexpr int kNumFp64Lists = 2 * kNumFp64Bins;
 inline void fp64_bin_shape(int bin, int* W, int* RPL) {
   static const int kW[kNumFp64Bins] = {16, 16, 32, 64}, kR[kNumFp64Bins] = {4, 8, 8, 12};
   *W = kW[bin];
@@ -96,9 +100,10 @@ struct Fp32Args {
   uint32_t* head;            // work-queue counter of this launch
   double* out;
   uint8_t* flags;
-  RescueEntry* rescue;       // kNumFp64Bins lists, list b starts at rescue_base[b]
-  uint32_t* rescue_count;    // kNumFp64Bins counters
+  RescueEntry* rescue;       // kNumFp64Lists lists, list l starts at rescue_base[l]
+  uint32_t* rescue_count;    // kNumFp64Lists counters
   const uint32_t* rescue_base;
+  uint32_t rescue_list0;     // this launch's pairs go to list rescue_list0 + fp64_bin(R)
 };
 
 struct Fp64Args {
@@ -115,7 +120,9 @@ struct Fp64Args {
 bool fp32_supported(int W, int RPL);
 hipError_t launch_fp32(int W, int RPL, bool xf, int occ, bool pt, int grid, hipStream_t stream,
                        const Fp32Args& a);
-hipError_t launch_fp64(int bin, int grid, hipStream_t stream, const Fp64Args& a);
+// FP64 rescue of one list's bin; pt = the prior-table kernel (pairs without
+// N in the haplotype, folded form).
+hipError_t launch_fp64(int bin, bool pt, int grid, hipStream_t stream, const Fp64Args& a);
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t stream);
 

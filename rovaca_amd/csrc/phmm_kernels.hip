@@ -306,7 +306,7 @@ __device__ __forceinline__ void finish_fp32(const Fp32Args& a, uint32_t o, uint3
     a.out[o] = log10((double)S) - (double)36.1236000061f;
     a.flags[o] = 1;
   } else {
-    const int b = fp64_bin(a.s.read_len[read]);
+    const int b = (int)a.rescue_list0 + fp64_bin(a.s.read_len[read]);
     const uint32_t idx = atomicAdd(a.rescue_count + b, 1u);
     RescueEntry e;
     e.hap = hap; e.read = read; e.out = o; e.pad = 0;
@@ -357,23 +357,68 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
 // ---------------------------------------------------------------------------
 constexpr int kHapWords = 112;  // 64 pad columns + 1536 bases + tail, 16 per word
 
-template <int RPL, bool XF>
-__device__ __forceinline__ void sweep_step_pt(const float (&M)[RPL], const float (&I)[RPL], const float (&D)[RPL],
-                                              float (&nM)[RPL], float (&nI)[RPL], float (&nD)[RPL], float dM, float dI,
-                                              float dD, float uM, float uI, const float* pr, const float (&tmm)[RPL],
-                                              const float (&tgm)[RPL], const float (&tmi)[RPL],
-                                              const float (&tii)[RPL], const float (&tmd)[RPL],
-                                              const float (&tdd)[RPL]) {
-  if constexpr (XF) nM[0] = pr[0] * fmaf(tmm[0], dM, dI + dD);
-  else nM[0] = pr[0] * fmaf(tmm[0], dM, tgm[0] * (dI + dD));
-  nI[0] = fmaf(tii[0], uI, tmi[0] * uM);
-  nD[0] = fmaf(tmd[0], M[0], tdd[0] * D[0]);
+template <typename T, int RPL, bool XF>
+__device__ __forceinline__ void sweep_step_pt(const T (&M)[RPL], const T (&I)[RPL], const T (&D)[RPL], T (&nM)[RPL],
+                                              T (&nI)[RPL], T (&nD)[RPL], T dM, T dI, T dD, T uM, T uI, const T* pr,
+                                              const T (&tmm)[RPL], const T (&tgm)[RPL], const T (&tmi)[RPL],
+                                              const T (&tii)[RPL], const T (&tmd)[RPL], const T (&tdd)[RPL]) {
+  if constexpr (XF) nM[0] = pr[0] * fma(tmm[0], dM, dI + dD);
+  else nM[0] = pr[0] * fma(tmm[0], dM, tgm[0] * (dI + dD));
+  nI[0] = i_update<T>(tmi[0], uM, tii[0], uI);
+  nD[0] = fma(tmd[0], M[0], tdd[0] * D[0]);
 #pragma unroll
   for (int r = 1; r < RPL; ++r) {
-    if constexpr (XF) nM[r] = pr[r] * fmaf(tmm[r], M[r - 1], I[r - 1] + D[r - 1]);
-    else nM[r] = pr[r] * fmaf(tmm[r], M[r - 1], tgm[r] * (I[r - 1] + D[r - 1]));
-    nI[r] = fmaf(tii[r], nI[r - 1], tmi[r] * nM[r - 1]);
-    nD[r] = fmaf(tmd[r], M[r], tdd[r] * D[r]);
+    if constexpr (XF) nM[r] = pr[r] * fma(tmm[r], M[r - 1], I[r - 1] + D[r - 1]);
+    else nM[r] = pr[r] * fma(tmm[r], M[r - 1], tgm[r] * (I[r - 1] + D[r - 1]));
+    nI[r] = i_update<T>(tmi[r], nM[r - 1], tii[r], nI[r - 1]);
+    nD[r] = fma(tmd[r], M[r], tdd[r] * D[r]);
+  }
+}
+
+// Transition parameters of lane k's rows k*RPL + r (r < RPL) of a read, and
+// per row (r < NR) the match and mismatch priors and the base code (-1 for
+// N, which matches every haplotype base). Rows past the read are padding
+// rows: prior 0, I carries M + I of the row above, D stays 0.
+// (ph/roDV_inter/pairhmm_inter.cpp:60-79; folded form described above.)
+template <typename T, int RPL, int NR, bool XF>
+__device__ __forceinline__ void row_params(const DevSeqs& s, const T* ph2pr, const T* mmt, uint32_t read, int k,
+                                           T (&tmm)[RPL], T (&tgm)[RPL], T (&tmi)[RPL], T (&tii)[RPL],
+                                           T (&tmd)[RPL], T (&tdd)[RPL], T (&pm)[NR], T (&px)[NR], int (&rc)[NR]) {
+  const uint32_t R = s.read_len[read];
+  const uint32_t roff = s.read_off[read];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const uint32_t row = (uint32_t)k * RPL + r;
+    pm[r] = 0; px[r] = 0; rc[r] = 0;
+    if (r >= RPL) continue;
+    if (row < R) {
+      const uint32_t g = roff + row;
+      const int iq = s.rq[g] & 127, ii = s.ri[g] & 127, id = s.rd[g] & 127, ic = s.rc[g] & 127;
+      const uint32_t base = (s.read_pack[g >> 4] >> ((g & 15u) * 2u)) & 3u;
+      const uint32_t isn = (s.read_nmask[g >> 5] >> (g & 31u)) & 1u;
+      const T dist = ph2pr[iq];
+      pm[r] = (T)1 - dist;
+      px[r] = isn ? pm[r] : dist * ((T)1 / (T)3);
+      rc[r] = isn ? -1 : (int)base;
+      const T gc = ph2pr[ic];
+      const int mx = ii > id ? ii : id, mn = ii > id ? id : ii;
+      tmm[r] = mmt[((mx * (mx + 1)) >> 1) + mn];
+      tgm[r] = (T)1 - gc;
+      if constexpr (XF) {
+        const T gnext = (row + 1 < R) ? (T)1 - ph2pr[s.rc[g + 1] & 127] : (T)1;
+        tmi[r] = gnext * ph2pr[ii];
+        tii[r] = gnext * gc / tgm[r];
+        tmd[r] = gnext * ph2pr[id];
+        tdd[r] = gc;
+      } else {
+        tmi[r] = ph2pr[ii];
+        tii[r] = gc;
+        tmd[r] = ph2pr[id];
+        tdd[r] = gc;
+      }
+    } else {
+      tmm[r] = 0; tgm[r] = 0; tmi[r] = 1; tii[r] = 1; tmd[r] = 0; tdd[r] = 1;
+    }
   }
 }
 
@@ -398,46 +443,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     __syncthreads();
     // per-row transition parameters (registers) and priors (LDS table), once
     // for all haplotypes of the tile
-    const uint32_t R = a.s.read_len[read];
     const uint32_t roff = a.s.read_off[read];
     float tmm[RPL], tgm[RPL], tmi[RPL], tii[RPL], tmd[RPL], tdd[RPL];
     float pm[NQ * 4], px[NQ * 4];
     int rc[NQ * 4];
-#pragma unroll
-    for (int r = 0; r < NQ * 4; ++r) {
-      const uint32_t row = (uint32_t)k * RPL + r;
-      pm[r] = 0.f; px[r] = 0.f; rc[r] = 0;
-      if (r >= RPL) continue;
-      if (row < R) {
-        const uint32_t g = roff + row;
-        const int iq = a.s.rq[g] & 127, ii = a.s.ri[g] & 127, id = a.s.rd[g] & 127, ic = a.s.rc[g] & 127;
-        const uint32_t base = (a.s.read_pack[g >> 4] >> ((g & 15u) * 2u)) & 3u;
-        const uint32_t isn = (a.s.read_nmask[g >> 5] >> (g & 31u)) & 1u;
-        const float dist = ph2pr[iq];
-        pm[r] = 1.f - dist;
-        px[r] = isn ? pm[r] : dist * (1.f / 3.f);
-        rc[r] = isn ? -1 : (int)base;
-        const float gc = ph2pr[ic];
-        const int mx = ii > id ? ii : id, mn = ii > id ? id : ii;
-        tmm[r] = mmt[((mx * (mx + 1)) >> 1) + mn];
-        tgm[r] = 1.f - gc;
-        if constexpr (XF) {
-          const float gnext = (row + 1 < R) ? 1.f - ph2pr[a.s.rc[g + 1] & 127] : 1.f;
-          tmi[r] = gnext * ph2pr[ii];
-          tii[r] = gnext * gc / tgm[r];
-          tmd[r] = gnext * ph2pr[id];
-          tdd[r] = gc;
-        } else {
-          tmi[r] = ph2pr[ii];
-          tii[r] = gc;
-          tmd[r] = ph2pr[id];
-          tdd[r] = gc;
-        }
-      } else {
-        // padding row: M = 0, I carries M + I of the row above, D = 0
-        tmm[r] = 0.f; tgm[r] = 0.f; tmi[r] = 1.f; tii[r] = 1.f; tmd[r] = 0.f; tdd[r] = 1.f;
-      }
-    }
+    row_params<float, RPL, NQ * 4, XF>(a.s, ph2pr, mmt, read, k, tmm, tgm, tmi, tii, tmd, tdd, pm, px, rc);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
@@ -503,21 +513,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
       const uint32_t cw_next = window(st + 4);
       load(cw, 1, pB);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
+      sweep_step_pt<float, RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
       S += Ib[RPL - 1];
       const float s1M = lane_down0<W>(Mb[RPL - 1], k);
       const float s1I = lane_down0<W>(Ib[RPL - 1], k);
       s1D = lane_down_keep<W>(Db[RPL - 1], s1D, old_d, k);
       load(cw, 2, pA);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, pB, tmm, tgm, tmi, tii, tmd, tdd);
+      sweep_step_pt<float, RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, pB, tmm, tgm, tmi, tii, tmd, tdd);
       if (st + 1 < nsteps) S += Ia[RPL - 1];
       const float s2M = lane_down0<W>(Ma[RPL - 1], k);
       const float s2I = lane_down0<W>(Ia[RPL - 1], k);
       s2D = lane_down_keep<W>(Da[RPL - 1], s2D, old_d, k);
       load(cw, 3, pB);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, pA, tmm, tgm, tmi, tii, tmd, tdd);
+      sweep_step_pt<float, RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, pA, tmm, tgm, tmi, tii, tmd, tdd);
       if (st + 2 < nsteps) S += Ib[RPL - 1];
       const float s3M = lane_down0<W>(Mb[RPL - 1], k);
       const float s3I = lane_down0<W>(Ib[RPL - 1], k);
@@ -525,7 +535,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
       cw = cw_next;
       load(cw, 0, pA);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, pB, tmm, tgm, tmi, tii, tmd, tdd);
+      sweep_step_pt<float, RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, pB, tmm, tgm, tmi, tii, tmd, tdd);
       if (st + 3 < nsteps) S += Ia[RPL - 1];
       dM = s3M; dI = s3I; dD = s3D;
       uM = lane_down0<W>(Ma[RPL - 1], k);
@@ -533,6 +543,114 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
       uD = lane_down_keep<W>(Da[RPL - 1], uD, old_d, k);
     }
     if (active && k == W - 1) finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, S);
+    }
+  }
+}
+
+// FP64 rescue with the prior table, for pairs whose haplotype has no N, in
+// the folded form: the prior-table sweep in double (2 rows per ds_read_b128).
+// The segments of a wave hold different pairs, so each segment stages its own
+// haplotype stream and the wave sweeps to the longest of them; a segment adds
+// to its sum only while its own columns last.
+template <int W, int RPL>
+__global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
+  constexpr int NSEG = 64 / W;
+  constexpr int NP = (RPL + 1) / 2;
+  __shared__ double2 ptab[4 * NP * 64];
+  __shared__ uint32_t hpk[NSEG][kHapWords];
+  const int lane = threadIdx.x;
+  const int seg = lane / W, k = lane % W;
+  const double* ph2pr = a.t.ph2pr_d;
+  const double* mmt = a.t.mm_d;
+  const uint32_t count = *a.count;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(a.head, (uint32_t)NSEG);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= count) break;
+    const uint32_t e = t + seg;
+    const bool active = e < count;
+    const RescueEntry en = a.list[active ? e : t];
+    const uint32_t H = a.s.hap_len[en.hap];
+    uint32_t Hmax = H;
+#pragma unroll
+    for (int g = 0; g < NSEG; ++g) {
+      const uint32_t h2 = (uint32_t)__shfl((int)H, g * W);
+      Hmax = h2 > Hmax ? h2 : Hmax;
+    }
+    __syncthreads();
+    {
+      const uint32_t w0 = a.s.hap_off[en.hap] >> 4, nw = (H + 15) >> 4;
+      for (int x = k; x < kHapWords; x += W) {
+        const int hx = x - 4;
+        hpk[seg][x] = (hx >= 0 && hx < (int)nw) ? a.s.hap_pack[w0 + hx] : 0u;
+      }
+    }
+    double tmm[RPL], tgm[RPL], tmi[RPL], tii[RPL], tmd[RPL], tdd[RPL];
+    double pm[NP * 2], px[NP * 2];
+    int rc[NP * 2];
+    row_params<double, RPL, NP * 2, true>(a.s, ph2pr, mmt, en.read, k, tmm, tgm, tmi, tii, tmd, tdd, pm, px, rc);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        double2 v;
+        v.x = (rc[2 * q + 0] < 0 || rc[2 * q + 0] == c) ? pm[2 * q + 0] : px[2 * q + 0];
+        v.y = (rc[2 * q + 1] < 0 || rc[2 * q + 1] == c) ? pm[2 * q + 1] : px[2 * q + 1];
+        ptab[(c * NP + q) * 64 + lane] = v;
+      }
+    __syncthreads();
+
+    const uint32_t roff = a.s.read_off[en.read];
+    const double initD = (1.0 - ph2pr[a.s.rc[roff] & 127]) * (0x1p1020 / (double)H);
+    double Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) { Ma[r] = 0; Ia[r] = 0; Da[r] = 0; }
+    double uM = 0, uI = 0, uD = (k == 0) ? initD : 0.0;
+    double dM = 0, dI = 0, dD = uD;
+    double S = 0;
+    const int nsteps = (int)Hmax + W - 1;
+    const int mysteps = (int)H + W - 1;
+    const char* tab = reinterpret_cast<const char*>(ptab) + lane * 16;
+    const uint32_t* hs = hpk[seg];
+    auto codes_at = [&](int st) -> uint32_t {
+      const int p = st - k + 64;
+      return __builtin_amdgcn_alignbit(hs[(p >> 4) + 1], hs[p >> 4], (uint32_t)(p & 15) * 2u);
+    };
+    auto load = [&](uint32_t codes, int i, double (&pr)[NP * 2]) {
+      const uint32_t c = (codes >> (2 * i)) & 3u;
+      const double2* b = reinterpret_cast<const double2*>(tab + c * (uint32_t)(NP * 1024));
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const double2 v = b[q * 64];
+        pr[2 * q + 0] = v.x; pr[2 * q + 1] = v.y;
+      }
+    };
+    double pA[NP * 2], pB[NP * 2];
+    uint32_t cw = codes_at(0);
+    load(cw, 0, pA);
+    // two steps a -> b -> a per iteration, the table reads one step ahead
+    for (int st = 0; st < nsteps; st += 2) {
+      load(cw, (st + 1) & 3, pB);
+      __builtin_amdgcn_sched_barrier(0);
+      sweep_step_pt<double, RPL, true>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
+      if (st < mysteps) S += Mb[RPL - 1] + Ib[RPL - 1];
+      const double sM = lane_down<W>(Mb[RPL - 1], 0.0, k);
+      const double sI = lane_down<W>(Ib[RPL - 1], 0.0, k);
+      const double sD = lane_down<W>(Db[RPL - 1], initD, k);
+      if (((st + 2) & 3) == 0) cw = codes_at(st + 2);
+      load(cw, (st + 2) & 3, pA);
+      __builtin_amdgcn_sched_barrier(0);
+      sweep_step_pt<double, RPL, true>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, pB, tmm, tgm, tmi, tii, tmd, tdd);
+      if (st + 1 < mysteps) S += Ma[RPL - 1] + Ia[RPL - 1];
+      dM = sM; dI = sI; dD = sD;
+      uM = lane_down<W>(Ma[RPL - 1], 0.0, k);
+      uI = lane_down<W>(Ia[RPL - 1], 0.0, k);
+      uD = lane_down<W>(Da[RPL - 1], initD, k);
+    }
+    if (active && k == W - 1) {
+      a.out[en.out] = log10(S) - 307.050595577260822;
+      a.flags[en.out] = 3;
     }
   }
 }
@@ -613,6 +731,8 @@ const Row16 kFp32W8[2] = {fp32_row<8, false, 2>(std::make_integer_sequence<int, 
                           fp32_row<8, true, 2>(std::make_integer_sequence<int, 16>{})};
 
 const std::array<Fp64Fn, kNumFp64Bins> kFp64 = {{&k_fp64<16, 4>, &k_fp64<16, 8>, &k_fp64<32, 8>, &k_fp64<64, 12>}};
+const std::array<Fp64Fn, kNumFp64Bins> kFp64Pt = {
+    {&k_fp64pt<16, 4>, &k_fp64pt<16, 8>, &k_fp64pt<32, 8>, &k_fp64pt<64, 12>}};
 
 // Prior-table variants (folded form only): W = 8, 16 and 32 at 2, 3, 4 waves
 // per SIMD, W = 64 at 2.
@@ -657,9 +777,9 @@ hipError_t launch_fp32(int W, int RPL, bool xf, int occ, bool pt, int grid, hipS
   return hipGetLastError();
 }
 
-hipError_t launch_fp64(int bin, int grid, hipStream_t stream, const Fp64Args& a) {
+hipError_t launch_fp64(int bin, bool pt, int grid, hipStream_t stream, const Fp64Args& a) {
   if (bin < 0 || bin >= kNumFp64Bins) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(kFp64[bin], dim3(grid), dim3(64), 0, stream, a);
+  hipLaunchKernelGGL(pt ? kFp64Pt[bin] : kFp64[bin], dim3(grid), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
 
@@ -696,15 +816,18 @@ int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ, bool pt) {
   return slot;
 }
 
-int fp64_blocks_per_cu(int bin) {
-  static int cache[kNumFp64Bins] = {};
+int fp64_blocks_per_cu(int bin, bool pt) {
+  static int cache[2][kNumFp64Bins] = {};
   if (bin < 0 || bin >= kNumFp64Bins) return 1;
-  if (cache[bin] == 0) {
+  int& slot = cache[pt][bin];
+  if (slot == 0) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kFp64[bin], 64, 0) != hipSuccess || n < 1) n = 1;
-    cache[bin] = n;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pt ? kFp64Pt[bin] : kFp64[bin], 64, 0) != hipSuccess ||
+        n < 1)
+      n = 1;
+    slot = n;
   }
-  return cache[bin];
+  return slot;
 }
 
 }  // namespace rvc

@@ -205,8 +205,8 @@ int launch_slot(Slot& s) {
   if ((rc = s.batch.enqueue(s.stream, true))) return rc;
   if (hipMemcpyAsync(s.res->out(), s.batch.out, n * sizeof(double), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
       hipMemcpyAsync(s.res->flags(), s.batch.flags, n, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
-      hipMemcpyAsync(s.res->rescued(), s.batch.counters + kFp32Bins + kNumFp64Bins, kNumFp64Bins * sizeof(uint32_t),
-                     hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+      hipMemcpyAsync(s.res->rescued(), s.batch.counters + kFp32Bins + kNumFp64Lists,
+                     kNumFp64Lists * sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
       hipEventRecord(s.done_ev, s.stream) != hipSuccess)
     return set_error(RVC_E_DEVICE, "result copy");
   return RVC_OK;
@@ -260,7 +260,7 @@ void slot_main(Slot* sp) {
       res->region = std::move(s.meta.region);
       res->read_perm = std::move(s.meta.read_perm);
       if (hipEventElapsedTime(&ms, s.batch.ev[0], s.batch.ev[2]) != hipSuccess) ms = 0;
-      for (int b = 0; b < kNumFp64Bins; ++b) resc += res->buf->rescued()[b];
+      for (int b = 0; b < kNumFp64Lists; ++b) resc += res->buf->rescued()[b];
     } else {
       hipStreamSynchronize(s.stream);
       s.res.reset();

@@ -104,6 +104,34 @@ def test_rescue_path():
     assert n_resc > 0
 
 
+def test_rescue_both_fp64_kernels():
+    say("FP64 rescue of pairs with and without N in the haplotype (compare and prior-table kernels)")
+    regs = [r for r in workload.stress_regions(12, seed=6) if r["n_haps"] * r["n_reads"] < 4000][:3]
+    r0 = dict(regs[0])
+    hb = r0["hap_bases"].copy()
+    for h in range(0, r0["n_haps"], 2):
+        hb[r0["hap_off"][h] + 7] = ord("N")
+    r0["hap_bases"] = hb
+    regs[0] = r0
+    b = rv.Batch(regs)
+    b.run()
+    out, flags = b.results()
+    names = {k["name"] for k in b.kernels() if k["precision"] == 64}
+    assert any(n.startswith("k_fp64pt<") for n in names) and any(n.startswith("k_fp64<") for n in names)
+    for x, reg in enumerate(regs):
+        got = b.region_result(out, x)
+        fl = b.region_result(flags, x)
+        want, route, ok = oracle.region_reference(reg)
+        m = ~np.isnan(want)
+        assert np.array_equal(m, ~np.isnan(got))
+        assert np.abs(got[m] - want[m]).max() < TOL
+        if x == 0:
+            hn = np.zeros(reg["n_haps"], bool)
+            hn[::2] = True
+            assert ((fl & 2) != 0)[:, hn].any() and ((fl & 2) != 0)[:, ~hn].any()
+    b.close()
+
+
 @pytest.mark.parametrize("R,H", [(1, 1), (1, 40), (40, 1), (16, 16), (17, 300), (255, 200), (256, 256),
                                  (257, 100), (300, 700), (512, 512), (513, 90), (700, 1536), (768, 1536)])
 def test_shapes(R, H):
