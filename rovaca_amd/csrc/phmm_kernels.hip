@@ -130,6 +130,29 @@ __device__ __forceinline__ double lane_down(double v, double old, int k) {
   return r;
 }
 
+// Double versions: two 32-bit DPP moves per value.
+template <int W>
+__device__ __forceinline__ double lane_down0(double v, int k) {
+  const long long vb = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(vb & 0xffffffffLL), kDownCtrl<W>, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(vb >> 32), kDownCtrl<W>, 0xf, 0xf, true);
+  const double r = __longlong_as_double(((long long)(unsigned)lo) | ((long long)hi << 32));
+  if constexpr (W == 32) return k == 0 ? 0.0 : r;
+  return r;
+}
+template <int W>
+__device__ __forceinline__ double lane_down_keep(double v, double keep, double old, int k) {
+  if constexpr (W == 32) {
+    return lane_down<W>(v, old, k);
+  } else {
+    const long long vb = __double_as_longlong(v), kb = __double_as_longlong(keep);
+    const int lo = __builtin_amdgcn_update_dpp((int)(kb & 0xffffffffLL), (int)(vb & 0xffffffffLL), kDownCtrl<W>, 0xf,
+                                               0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(kb >> 32), (int)(vb >> 32), kDownCtrl<W>, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)(unsigned)lo) | ((long long)hi << 32));
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ const T* ph2pr_tab(const DevTables& t);
 template <>
@@ -285,15 +308,15 @@ __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, cons
     sweep_step<T, RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, hc0, rcode, pm, px, tmm, tgm, tmi, tii, tmd,
                            tdd);
     if (st < mysteps) S += Mb[RPL - 1] + Ib[RPL - 1];
-    const T sM = lane_down<W>(Mb[RPL - 1], (T)0, k);
-    const T sI = lane_down<W>(Ib[RPL - 1], (T)0, k);
+    const T sM = lane_down0<W>(Mb[RPL - 1], k);
+    const T sI = lane_down0<W>(Ib[RPL - 1], k);
     const T sD = lane_down<W>(Db[RPL - 1], old_d, k);
     sweep_step<T, RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, hc1, rcode, pm, px, tmm, tgm, tmi, tii, tmd,
                            tdd);
     if (st + 1 < mysteps) S += Ma[RPL - 1] + Ia[RPL - 1];
     dM = sM; dI = sI; dD = sD;
-    uM = lane_down<W>(Ma[RPL - 1], (T)0, k);
-    uI = lane_down<W>(Ia[RPL - 1], (T)0, k);
+    uM = lane_down0<W>(Ma[RPL - 1], k);
+    uI = lane_down0<W>(Ia[RPL - 1], k);
     uD = lane_down<W>(Da[RPL - 1], old_d, k);
   }
   return S;
@@ -608,6 +631,7 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
     for (int r = 0; r < RPL; ++r) { Ma[r] = 0; Ia[r] = 0; Da[r] = 0; }
     double uM = 0, uI = 0, uD = (k == 0) ? initD : 0.0;
     double dM = 0, dI = 0, dD = uD;
+    double sD = initD;  // loop-carried: lane 0 keeps the row -1 value
     double S = 0;
     const int nsteps = (int)Hmax + W - 1;
     const int mysteps = (int)H + W - 1;
@@ -635,18 +659,18 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
       __builtin_amdgcn_sched_barrier(0);
       sweep_step_pt<double, RPL, true>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
       if (st < mysteps) S += Mb[RPL - 1] + Ib[RPL - 1];
-      const double sM = lane_down<W>(Mb[RPL - 1], 0.0, k);
-      const double sI = lane_down<W>(Ib[RPL - 1], 0.0, k);
-      const double sD = lane_down<W>(Db[RPL - 1], initD, k);
+      const double sM = lane_down0<W>(Mb[RPL - 1], k);
+      const double sI = lane_down0<W>(Ib[RPL - 1], k);
+      sD = lane_down_keep<W>(Db[RPL - 1], sD, initD, k);
       if (((st + 2) & 3) == 0) cw = codes_at(st + 2);
       load(cw, (st + 2) & 3, pA);
       __builtin_amdgcn_sched_barrier(0);
       sweep_step_pt<double, RPL, true>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, pB, tmm, tgm, tmi, tii, tmd, tdd);
       if (st + 1 < mysteps) S += Ma[RPL - 1] + Ia[RPL - 1];
       dM = sM; dI = sI; dD = sD;
-      uM = lane_down<W>(Ma[RPL - 1], 0.0, k);
-      uI = lane_down<W>(Ia[RPL - 1], 0.0, k);
-      uD = lane_down<W>(Da[RPL - 1], initD, k);
+      uM = lane_down0<W>(Ma[RPL - 1], k);
+      uI = lane_down0<W>(Ia[RPL - 1], k);
+      uD = lane_down_keep<W>(Da[RPL - 1], uD, initD, k);
     }
     if (active && k == W - 1) {
       a.out[en.out] = log10(S) - 307.050595577260822;

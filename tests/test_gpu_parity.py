@@ -132,8 +132,9 @@ def test_rescue_both_fp64_kernels():
     b.close()
 
 
-@pytest.mark.parametrize("R,H", [(1, 1), (1, 40), (40, 1), (16, 16), (17, 300), (255, 200), (256, 256),
-                                 (257, 100), (300, 700), (512, 512), (513, 90), (700, 1536), (768, 1536)])
+@pytest.mark.parametrize("R,H", [(1, 1), (1, 40), (40, 1), (16, 16), (17, 300), (100, 350), (112, 300), (127, 200),
+                                 (128, 200), (255, 200), (256, 256), (257, 100), (300, 700), (512, 512), (513, 90),
+                                 (700, 1536), (767, 900), (768, 1536)])
 def test_shapes(R, H):
     say(f"shape R={R} H={H}")
     rng = np.random.default_rng(R * 7919 + H)
