@@ -483,18 +483,29 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
         ptab[(c * NQ + q) * 64 + lane] = v;
       }
     const float g0 = XF ? 1.f - ph2pr[a.s.rc[roff] & 127] : 1.f;
+    // this lane's words x = lane, lane + 64 of a haplotype's stream (64
+    // zero columns in front); the next haplotype's are loaded while the
+    // current one is swept
+    auto hap_words = [&](uint32_t hap, uint32_t (&w)[2]) {
+      const uint32_t Hh = a.s.hap_len[hap];
+      const uint32_t w0 = a.s.hap_off[hap] >> 4, nw = (Hh + 15) >> 4;
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int x = lane + 64 * i, hx = x - 4;
+        w[i] = (x < kHapWords && hx >= 0 && hx < (int)nw) ? a.s.hap_pack[w0 + hx] : 0u;
+      }
+    };
+    uint32_t words[2];
+    hap_words(tl.hap, words);
     for (uint32_t h = 0; h < nh; ++h) {
     const uint32_t hap = tl.hap + h;
     const uint32_t H = a.s.hap_len[hap];
     __syncthreads();
-    {
-      const uint32_t w0 = a.s.hap_off[hap] >> 4, nw = (H + 15) >> 4;
-      for (int x = lane; x < kHapWords; x += 64) {
-        const int hx = x - 4;
-        hpk[x] = (hx >= 0 && hx < (int)nw) ? a.s.hap_pack[w0 + hx] : 0u;
-      }
-    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      if (lane + 64 * i < kHapWords) hpk[lane + 64 * i] = words[i];
     __syncthreads();
+    if (h + 1 < nh) hap_words(hap + 1, words);
 
     float initD = 0x1p120f / (float)H;
     if constexpr (XF) initD = g0 * initD;
