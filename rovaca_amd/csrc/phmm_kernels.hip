@@ -498,85 +498,85 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     uint32_t words[2];
     hap_words(tl.hap, words);
     for (uint32_t h = 0; h < nh; ++h) {
-    const uint32_t hap = tl.hap + h;
-    const uint32_t H = a.s.hap_len[hap];
-    __syncthreads();
+      const uint32_t hap = tl.hap + h;
+      const uint32_t H = a.s.hap_len[hap];
+      __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-      if (lane + 64 * i < kHapWords) hpk[lane + 64 * i] = words[i];
-    __syncthreads();
-    if (h + 1 < nh) hap_words(hap + 1, words);
+      for (int i = 0; i < 2; ++i)
+        if (lane + 64 * i < kHapWords) hpk[lane + 64 * i] = words[i];
+      __syncthreads();
+      if (h + 1 < nh) hap_words(hap + 1, words);
 
-    float initD = 0x1p120f / (float)H;
-    if constexpr (XF) initD = g0 * initD;
-    const float old_d = initD;
-    float Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
+      float initD = 0x1p120f / (float)H;
+      if constexpr (XF) initD = g0 * initD;
+      const float old_d = initD;
+      float Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
 #pragma unroll
-    for (int r = 0; r < RPL; ++r) { Ma[r] = 0.f; Ia[r] = 0.f; Da[r] = 0.f; }
-    float uM = 0.f, uI = 0.f, uD = (k == 0) ? initD : 0.f;
-    float dM = 0.f, dI = 0.f, dD = uD;
-    // D moved down a lane: lane 0 keeps the row -1 value that these
-    // loop-carried registers already hold (no register copy per step)
-    float s1D = old_d, s2D = old_d, s3D = old_d;
-    float S = 0.f;
-    const int nsteps = (int)H + W - 1;
-    const char* tab = reinterpret_cast<const char*>(ptab) + lane * 16;
-    // codes of columns st-k .. st-k+3 (stream position p = column + 64 >= 1)
-    auto window = [&](int st) -> uint32_t {
-      const int p = st - k + 64;
-      return __builtin_amdgcn_alignbit(hpk[(p >> 4) + 1], hpk[p >> 4], (uint32_t)(p & 15) * 2u);
-    };
-    auto load = [&](uint32_t codes, int i, float (&pr)[NQ * 4]) {
-      const uint32_t c = (codes >> (2 * i)) & 3u;
-      const float4* b = reinterpret_cast<const float4*>(tab + c * (uint32_t)(NQ * 1024));
+      for (int r = 0; r < RPL; ++r) { Ma[r] = 0.f; Ia[r] = 0.f; Da[r] = 0.f; }
+      float uM = 0.f, uI = 0.f, uD = (k == 0) ? initD : 0.f;
+      float dM = 0.f, dI = 0.f, dD = uD;
+      // D moved down a lane: lane 0 keeps the row -1 value that these
+      // loop-carried registers already hold (no register copy per step)
+      float s1D = old_d, s2D = old_d, s3D = old_d;
+      float S = 0.f;
+      const int nsteps = (int)H + W - 1;
+      const char* tab = reinterpret_cast<const char*>(ptab) + lane * 16;
+      // codes of columns st-k .. st-k+3 (stream position p = column + 64 >= 1)
+      auto window = [&](int st) -> uint32_t {
+        const int p = st - k + 64;
+        return __builtin_amdgcn_alignbit(hpk[(p >> 4) + 1], hpk[p >> 4], (uint32_t)(p & 15) * 2u);
+      };
+      auto load = [&](uint32_t codes, int i, float (&pr)[NQ * 4]) {
+        const uint32_t c = (codes >> (2 * i)) & 3u;
+        const float4* b = reinterpret_cast<const float4*>(tab + c * (uint32_t)(NQ * 1024));
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const float4 v = b[q * 64];
-        pr[4 * q + 0] = v.x; pr[4 * q + 1] = v.y; pr[4 * q + 2] = v.z; pr[4 * q + 3] = v.w;
-      }
-    };
-    float pA[NQ * 4], pB[NQ * 4];
-    uint32_t cw = window(0);
-    load(cw, 0, pA);
-    // Four steps a -> b -> a -> b -> a per iteration. The read's last row is
-    // always a padding row (the host picks RPL with R < W * RPL), which holds
-    // M[R-1][j] + I[R-1][j] in I, so the sum takes one add per step.
-    // sched_barrier(0) keeps each step's table reads ahead of the previous
-    // step's arithmetic, so their latency overlaps a whole step.
-    for (int st = 0; st < nsteps; st += 4) {
-      const uint32_t cw_next = window(st + 4);
-      load(cw, 1, pB);
-      __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<float, RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
-      S += Ib[RPL - 1];
-      const float s1M = lane_down0<W>(Mb[RPL - 1], k);
-      const float s1I = lane_down0<W>(Ib[RPL - 1], k);
-      s1D = lane_down_keep<W>(Db[RPL - 1], s1D, old_d, k);
-      load(cw, 2, pA);
-      __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<float, RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, pB, tmm, tgm, tmi, tii, tmd, tdd);
-      if (st + 1 < nsteps) S += Ia[RPL - 1];
-      const float s2M = lane_down0<W>(Ma[RPL - 1], k);
-      const float s2I = lane_down0<W>(Ia[RPL - 1], k);
-      s2D = lane_down_keep<W>(Da[RPL - 1], s2D, old_d, k);
-      load(cw, 3, pB);
-      __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<float, RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, pA, tmm, tgm, tmi, tii, tmd, tdd);
-      if (st + 2 < nsteps) S += Ib[RPL - 1];
-      const float s3M = lane_down0<W>(Mb[RPL - 1], k);
-      const float s3I = lane_down0<W>(Ib[RPL - 1], k);
-      s3D = lane_down_keep<W>(Db[RPL - 1], s3D, old_d, k);
-      cw = cw_next;
+        for (int q = 0; q < NQ; ++q) {
+          const float4 v = b[q * 64];
+          pr[4 * q + 0] = v.x; pr[4 * q + 1] = v.y; pr[4 * q + 2] = v.z; pr[4 * q + 3] = v.w;
+        }
+      };
+      float pA[NQ * 4], pB[NQ * 4];
+      uint32_t cw = window(0);
       load(cw, 0, pA);
-      __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<float, RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, pB, tmm, tgm, tmi, tii, tmd, tdd);
-      if (st + 3 < nsteps) S += Ia[RPL - 1];
-      dM = s3M; dI = s3I; dD = s3D;
-      uM = lane_down0<W>(Ma[RPL - 1], k);
-      uI = lane_down0<W>(Ia[RPL - 1], k);
-      uD = lane_down_keep<W>(Da[RPL - 1], uD, old_d, k);
-    }
-    if (active && k == W - 1) finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, S);
+      // Four steps a -> b -> a -> b -> a per iteration. The read's last row is
+      // always a padding row (the host picks RPL with R < W * RPL), which holds
+      // M[R-1][j] + I[R-1][j] in I, so the sum takes one add per step.
+      // sched_barrier(0) keeps each step's table reads ahead of the previous
+      // step's arithmetic, so their latency overlaps a whole step.
+      for (int st = 0; st < nsteps; st += 4) {
+        const uint32_t cw_next = window(st + 4);
+        load(cw, 1, pB);
+        __builtin_amdgcn_sched_barrier(0);
+        sweep_step_pt<float, RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
+        S += Ib[RPL - 1];
+        const float s1M = lane_down0<W>(Mb[RPL - 1], k);
+        const float s1I = lane_down0<W>(Ib[RPL - 1], k);
+        s1D = lane_down_keep<W>(Db[RPL - 1], s1D, old_d, k);
+        load(cw, 2, pA);
+        __builtin_amdgcn_sched_barrier(0);
+        sweep_step_pt<float, RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, pB, tmm, tgm, tmi, tii, tmd, tdd);
+        if (st + 1 < nsteps) S += Ia[RPL - 1];
+        const float s2M = lane_down0<W>(Ma[RPL - 1], k);
+        const float s2I = lane_down0<W>(Ia[RPL - 1], k);
+        s2D = lane_down_keep<W>(Da[RPL - 1], s2D, old_d, k);
+        load(cw, 3, pB);
+        __builtin_amdgcn_sched_barrier(0);
+        sweep_step_pt<float, RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, pA, tmm, tgm, tmi, tii, tmd, tdd);
+        if (st + 2 < nsteps) S += Ib[RPL - 1];
+        const float s3M = lane_down0<W>(Mb[RPL - 1], k);
+        const float s3I = lane_down0<W>(Ib[RPL - 1], k);
+        s3D = lane_down_keep<W>(Db[RPL - 1], s3D, old_d, k);
+        cw = cw_next;
+        load(cw, 0, pA);
+        __builtin_amdgcn_sched_barrier(0);
+        sweep_step_pt<float, RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, pB, tmm, tgm, tmi, tii, tmd, tdd);
+        if (st + 3 < nsteps) S += Ia[RPL - 1];
+        dM = s3M; dI = s3I; dD = s3D;
+        uM = lane_down0<W>(Ma[RPL - 1], k);
+        uI = lane_down0<W>(Ia[RPL - 1], k);
+        uD = lane_down_keep<W>(Da[RPL - 1], uD, old_d, k);
+      }
+      if (active && k == W - 1) finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, S);
     }
   }
 }
