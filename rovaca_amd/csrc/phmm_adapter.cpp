@@ -7,6 +7,7 @@
 //   poorly modelled reads     ph/pairhmm_internal.cpp:22-44, :489-497 (ph/adapter.h:20-22)
 //   transpose + erase order   ph/pairhmm_internal.cpp:499-508, :576-623
 #include "phmm_adapter.h"
+#include "phmm_prep.h"
 
 #include <algorithm>
 #include <cmath>
@@ -22,67 +23,9 @@ constexpr uint8_t kGopCap[3][21] = {
     {40, 40, 39, 38, 37, 36, 34, 32, 28, 23, 17, 10, 10, 10, 10, 10, 10, 10, 10, 10, 10},   // HOSTILE
     {40, 40, 40, 39, 39, 39, 38, 38, 37, 37, 36, 35, 34, 33, 32, 30, 28, 26, 23, 20, 17},   // AGGRESSIVE
     {40, 40, 40, 40, 39, 39, 39, 39, 39, 38, 38, 38, 37, 37, 37, 36, 36, 35, 34, 33, 33}};  // CONSERVATIVE
-constexpr int kMaxUnit = 8;         // longest repeat unit examined
-constexpr int kMaxRepeatLen = 20;   // repeat counts are capped here
-
-// How many whole copies of unit = s[u .. u+ul) tile s[t .. t+tl): from the
-// front (leading) or from the back, stopping at the first mismatch.
-int count_copies(const uint8_t* s, int u, int ul, int t, int tl, bool leading) {
-  if (tl == 0) return 0;
-  int n = 0;
-  const int last = tl - ul;
-  if (leading) {
-    for (int p = 0; p <= last; p += ul) {
-      if (std::memcmp(s + t + p, s + u, ul) != 0) break;
-      ++n;
-    }
-  } else {
-    for (int p = last; p >= 0; p -= ul) {
-      if (std::memcmp(s + t + p, s + u, ul) != 0) break;
-      ++n;
-    }
-  }
-  return n;
-}
-
-// Tandem-repeat length around base 'pos' as Rovaca computes it
-// (ph/pairhmm_internal.cpp:260-314). The backward unit is recorded with the
-// start pos - len - 1, as there; that start never equals the forward unit's
-// start pos + 1, so the repeat length is always the forward unit's count on
-// both sides.
-int repeat_length(const uint8_t* s, int n, int pos) {
-  int bw_count = 0;
-  std::pair<int, int> bw_unit(pos, 1);
-  for (int ul = 1; ul <= kMaxUnit; ++ul) {
-    if (pos + 1 - ul < 0) break;
-    bw_count = count_copies(s, pos - ul + 1, ul, 0, pos + 1, false);
-    if (bw_count > 1) {
-      bw_unit = {pos - ul - 1, ul};
-      break;
-    }
-  }
-  int total = bw_count;
-  if (pos < n - 1) {
-    std::pair<int, int> fw_unit(pos + 1, 1);
-    int fw_count = 0;
-    for (int ul = 1; ul <= kMaxUnit; ++ul) {
-      if (pos + ul + 1 > n) break;
-      fw_count = count_copies(s, pos + 1, ul, pos + 1, n - pos - 1, true);
-      if (fw_count > 1) {
-        fw_unit = {pos + 1, ul};
-        break;
-      }
-    }
-    if (fw_unit == bw_unit) {
-      total = bw_count + fw_count;
-    } else {
-      total = count_copies(s, fw_unit.first, fw_unit.second, 0, pos + 1, false) + fw_count;
-    }
-  }
-  return std::min(total, kMaxRepeatLen);
-}
-
 }  // namespace
+
+const uint8_t* pcr_gop_caps() { return &kGopCap[0][0]; }
 
 void preprocess_read(const uint8_t* bases, const uint8_t* qual, int len, int mq, int min_q, int pcr, uint8_t* q,
                      uint8_t* gop, uint8_t* gcp) {

@@ -12,6 +12,10 @@ namespace rvc {
 void preprocess_read(const uint8_t* bases, const uint8_t* qual, int len, int mq, int min_q, int pcr, uint8_t* q,
                      uint8_t* gop, uint8_t* gcp);
 
+// The PCR indel model's GOP cap per tandem-repeat length 0..20, per model
+// (HOSTILE, AGGRESSIVE, CONSERVATIVE): 3 x 21 bytes.
+const uint8_t* pcr_gop_caps();
+
 // Normalize each read's row, drop poorly modelled reads and transpose into
 // hap-major [n_haps][n_reads] (trailing columns NaN). Returns the number of
 // kept reads, listed in kept[].

@@ -85,14 +85,14 @@ FP32Policy& policy() { return default_policy(); }
 
 namespace rvc {
 // One region through the device on the calling thread's workspace.
-int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags) {
+int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags, const DevicePrep* prep) {
   if (!g_engine.ready) return set_error(RVC_E_STATE, "rvc_phmm_init has not run");
   int rc = validate_region(region);
   if (rc) return rc;
   if (!out) return set_error(RVC_E_INVALID, "null output");
   SyncWorkspace* w = nullptr;
   if ((rc = get_workspace(&w))) return rc;
-  if ((rc = build_host_pack(region, 1, policy(), &w->pack))) return rc;
+  if ((rc = build_host_pack(region, 1, policy(), &w->pack, prep))) return rc;
   split_runs(w->pack);
   w->batch.graph_exec = nullptr;
   w->batch.graph = nullptr;
@@ -163,16 +163,14 @@ int rvc_phmm_call_pairhmm(const rvc_raw_region* rg, int32_t min_quality_threshol
     return fail(set_error(RVC_E_INVALID, "empty or incomplete region"));
   if (pcr_option < RVC_PCR_NONE || pcr_option > RVC_PCR_CONSERVATIVE)
     return fail(set_error(RVC_E_INVALID, "unknown PCR indel model %d", pcr_option));
-  const int32_t total = rg->read_off[N];
-  std::vector<uint8_t> q(total), gop(total), gcp(total);
   std::vector<int32_t> rlen(N);
   for (uint32_t r = 0; r < N; ++r) {
-    const int32_t o = rg->read_off[r], len = rg->read_off[r + 1] - o;
+    const int32_t len = rg->read_off[r + 1] - rg->read_off[r];
     if (len < 0) return fail(set_error(RVC_E_INVALID, "read %u has negative length", r));
     rlen[r] = len;
-    preprocess_read(rg->read_bases + o, rg->read_qual + o, len, rg->read_mq[r], min_quality_threshold, pcr_option,
-                    q.data() + o, gop.data() + o, gcp.data() + o);
   }
+  // the reads are preprocessed on the device (MQ cap, quality floor, GOP 45 /
+  // PCR model, GCP 10): the region carries the raw qualities
   rvc_region reg;
   reg.n_haps = H;
   reg.n_reads = N;
@@ -180,15 +178,19 @@ int rvc_phmm_call_pairhmm(const rvc_raw_region* rg, int32_t min_quality_threshol
   reg.hap_bases = rg->hap_bases;
   reg.read_off = rg->read_off;
   reg.read_bases = rg->read_bases;
-  reg.q = q.data();
-  reg.i = gop.data();
-  reg.d = gop.data();
-  reg.c = gcp.data();
+  reg.q = rg->read_qual;
+  reg.i = rg->read_qual;  // not read
+  reg.d = rg->read_qual;
+  reg.c = rg->read_qual;
+  DevicePrep prep;
+  prep.mq = rg->read_mq;
+  prep.min_q = min_quality_threshold;
+  prep.pcr = pcr_option;
   // with the serving loop running, concurrent callers (Rovaca's worker
   // threads) are batched together; otherwise the region runs on its own
   std::vector<double> read_major((size_t)H * N);
-  int rc = server_running() ? served_region(&reg, read_major.data(), nullptr)
-                            : region_sync_impl(&reg, read_major.data(), nullptr);
+  int rc = server_running() ? served_region(&reg, read_major.data(), nullptr, &prep)
+                            : region_sync_impl(&reg, read_major.data(), nullptr, &prep);
   if (rc) return fail(rc);
   *n_kept = (uint32_t)postprocess(N, H, read_major.data(), rlen.data(), out_hap_major, kept_reads);
   return RVC_OK;

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/rovaca_phmm.h"
+#include "phmm_adapter.h"
 #include "phmm_engine.h"
 #include "phmm_internal.h"
 #include "phmm_tables.h"
@@ -68,6 +69,7 @@ int engine_init(const rvc_config* cfg) {
   auto* ht = new (std::nothrow) HostTables;
   if (!ht) return set_error(RVC_E_INVALID, "out of host memory");
   build_tables(ht);
+  std::memcpy(ht->gop_cap, pcr_gop_caps(), 3 * 21);
   g_engine.host_tables.reset(ht);
   void* p = nullptr;
   const size_t bytes = sizeof(HostTables);
@@ -79,6 +81,7 @@ int engine_init(const rvc_config* cfg) {
   g_engine.tables.mm_f = base->mm_f;
   g_engine.tables.ph2pr_d = base->ph2pr_d;
   g_engine.tables.mm_d = base->mm_d;
+  g_engine.tables.gop_cap = base->gop_cap;
   g_engine.hbm_budget = (cfg && cfg->hbm_budget_bytes) ? cfg->hbm_budget_bytes : (64ull << 30);
   g_engine.max_batch_regions = (cfg && cfg->max_batch_regions) ? cfg->max_batch_regions : 4096u;
   // kernel-shape table: the one named in the config, else tuned/<device>.txt beside the library
@@ -277,7 +280,8 @@ static void append_largest_first(const std::vector<Tile>& tiles, const std::vect
   for (size_t x = 0; x < tiles.size(); ++x) dst[base + count[bucket(x)]++] = tiles[x];
 }
 
-int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Policy& pol, HostPack* hp) {
+int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Policy& pol, HostPack* hp,
+                    const DevicePrep* prep) {
   HostPack& P = *hp;
   P = HostPack();
   P.n_regions = n_regions;
@@ -316,6 +320,11 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
   P.ri.assign(read_bases + 64, 0);
   P.rd.assign(read_bases + 64, 0);
   P.rc.assign(read_bases + 64, 0);
+  P.read_prep.assign(n_read_total, 0);
+  if (prep) {
+    P.any_prep = true;
+    P.read_raw.assign(read_bases + 64, 0);
+  }
   uint32_t hcur = 0, rcur = 0;
   P.fp64_cap.assign(kNumFp64Lists, 0);
   P.waves_per_cu = pol.waves_per_cu;
@@ -360,11 +369,18 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
       const int32_t o = g.read_off[x];
       pack_seq(g.read_bases + o, len, rcur, P.read_pack, P.read_nmask);
       std::memcpy(&P.rq[rcur], g.q + o, len);
-      std::memcpy(&P.ri[rcur], g.i + o, len);
-      std::memcpy(&P.rd[rcur], g.d + o, len);
-      std::memcpy(&P.rc[rcur], g.c + o, len);
-      for (uint32_t y = 0; y < len; ++y)
-        if ((g.c[o + y] & 127) == 0) P.fold_gap = false;
+      if (prep) {
+        // raw qualities; the device fills q, i, d, c (gap continuation 10 > 0)
+        std::memcpy(&P.read_raw[rcur], g.read_bases + o, len);
+        P.read_prep[gi] = kPrepOn | ((uint32_t)(prep->pcr & 3) << 16) | ((uint32_t)(prep->min_q & 255) << 8) |
+                          (uint32_t)prep->mq[x];
+      } else {
+        std::memcpy(&P.ri[rcur], g.i + o, len);
+        std::memcpy(&P.rd[rcur], g.d + o, len);
+        std::memcpy(&P.rc[rcur], g.c + o, len);
+        for (uint32_t y = 0; y < len; ++y)
+          if ((g.c[o + y] & 127) == 0) P.fold_gap = false;
+      }
       rcur += align_up(len, kSeqAlign);
     }
     // tiles: one haplotype x up to 64/W reads of similar length
@@ -536,6 +552,7 @@ PackSizes pack_sizes(const HostPack& P) {
   z.n_tiles = P.tiles.size();
   z.n_pairs_slots = P.n_pairs_slots;
   z.n_rescue = P.fp64_base.empty() ? 0 : P.fp64_base[kNumFp64Lists];
+  z.raw_bytes = P.any_prep ? P.read_raw.size() : 0;
   return z;
 }
 
@@ -557,6 +574,8 @@ BatchLayout batch_layout(const PackSizes& z) {
   L.rlen = take(4 * z.n_reads);
   L.tiles = take(sizeof(Tile) * (z.n_tiles + 1));
   L.rbase = take(4 * kNumFp64Lists);
+  L.raw = take(z.raw_bytes);
+  L.prep = take(4 * z.n_reads);
   L.in_bytes = off;
   L.out = take(sizeof(double) * (z.n_pairs_slots + 1));
   L.flags = take(z.n_pairs_slots + 8);
@@ -582,6 +601,8 @@ void stage_pack(const HostPack& P, const BatchLayout& L, char* dst) {
   put(L.rlen, P.read_len.data(), bytes_of(P.read_len));
   put(L.tiles, P.tiles.data(), bytes_of(P.tiles));
   put(L.rbase, P.fp64_base.data(), kNumFp64Lists * sizeof(uint32_t));
+  if (P.any_prep) put(L.raw, P.read_raw.data(), P.read_raw.size());
+  put(L.prep, P.read_prep.data(), bytes_of(P.read_prep));
 }
 
 PinnedBuf::~PinnedBuf() { release(); }
@@ -633,6 +654,7 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
     P.prior_table = A.prior_table;
     P.hap_run = A.hap_run;
     P.hap_run_fixed = A.hap_run_fixed;
+    P.any_prep = P.any_prep || A.any_prep;
     std::memcpy(P.occ, A.occ, sizeof P.occ);
   }
   // tiles of the merged batch: each pack's runs split for the batch's size
@@ -653,6 +675,7 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
   z.qual_bytes = z.read_words * 16 + 64;
   z.read_nwords = z.read_words / 2 + 2;
   z.read_words += 2;
+  z.raw_bytes = P.any_prep ? z.qual_bytes : 0;
   P.fp64_base.assign(kNumFp64Lists + 1, 0);
   for (int b = 0; b < kNumFp64Lists; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
   z.n_rescue = P.fp64_base[kNumFp64Lists];
@@ -670,6 +693,8 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
   uint32_t* rlen = reinterpret_cast<uint32_t*>(dst + L.rlen);
   Tile* tiles = reinterpret_cast<Tile*>(dst + L.tiles);
   char* quals[4] = {dst + L.rq, dst + L.ri, dst + L.rd, dst + L.rc};
+  uint8_t* raw = reinterpret_cast<uint8_t*>(dst + L.raw);
+  uint32_t* prep = reinterpret_cast<uint32_t*>(dst + L.prep);
   P.bin_start.assign(kFp32Bins + 1, 0);
   for (int v = 0; v < kFp32Bins; ++v) P.bin_start[v + 1] = P.bin_start[v] + (uint32_t)bin_count[v];
   std::vector<uint32_t> bin_cur(P.bin_start.begin(), P.bin_start.end() - 1);
@@ -685,6 +710,10 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
     std::memcpy(rn + rw / 2, A.read_nmask.data(), bw / 2 * 4);
     const std::vector<uint8_t>* aq[4] = {&A.rq, &A.ri, &A.rd, &A.rc};
     for (int k = 0; k < 4; ++k) std::memcpy(quals[k] + rw * 16, aq[k]->data(), bw * 16);
+    // reads preprocessed on the device: raw bases and their prep words (zero
+    // words for reads that arrive preprocessed; their raw bytes are not read)
+    if (A.any_prep) std::memcpy(raw + rw * 16, A.read_raw.data(), bw * 16);
+    std::memcpy(prep + ridx, A.read_prep.data(), A.read_prep.size() * 4);
     const uint32_t hbase = (uint32_t)(hw * 16), rbase = (uint32_t)(rw * 16);
     for (size_t h = 0; h < A.hap_off.size(); ++h) {
       hoff[hidx + h] = A.hap_off[h] + hbase;
@@ -778,6 +807,8 @@ int DeviceBatch::upload_staged(const HostPack& P, const BatchLayout& L, const vo
   rescue_base = reinterpret_cast<const uint32_t*>(base + L.rbase);
   counters = reinterpret_cast<uint32_t*>(base + L.cnt);
   n_counters_ = kNumCounters;
+  read_raw = reinterpret_cast<const uint8_t*>(base + L.raw);
+  read_prep = reinterpret_cast<const uint32_t*>(base + L.prep);
   return RVC_OK;
 }
 
@@ -786,6 +817,21 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
   uint32_t* heads = counters;
   uint32_t* rescue_count = counters + kFp32Bins + kNumFp64Lists;
   RVC_HIP(launch_zero_u32(counters, n_counters_, s));
+  if (P.any_prep) {
+    // raw qualities -> q, i, d, c in place (idempotent: a replay gives the same)
+    PrepArgs pa;
+    pa.rq = const_cast<uint8_t*>(seqs.rq);
+    pa.ri = const_cast<uint8_t*>(seqs.ri);
+    pa.rd = const_cast<uint8_t*>(seqs.rd);
+    pa.rc = const_cast<uint8_t*>(seqs.rc);
+    pa.raw = read_raw;
+    pa.prep = read_prep;
+    pa.read_off = seqs.read_off;
+    pa.read_len = seqs.read_len;
+    pa.gop_cap = g_engine.tables.gop_cap;
+    pa.n_reads = (uint32_t)(P.read_perm.size());
+    RVC_HIP(launch_prep(pa, s));
+  }
   if (timed) RVC_HIP(hipEventRecord(ev[0], s));
   n_fp32_launches = 0;
   n_fp64_launches = 0;

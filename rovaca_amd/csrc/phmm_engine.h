@@ -81,6 +81,15 @@ FP32Policy& default_policy();
 // Reads a table written by rovaca_amd/tune.py; returns RVC_OK or an error code.
 int load_tune_table(const char* path, FP32Policy* pol);
 
+// Regions handed over with raw base qualities: the device preprocesses their
+// reads (MQ cap, min-quality floor, GOP 45 / PCR model, GCP 10) before the
+// sweep. The region's q array holds the raw qualities; i, d, c are unused.
+struct DevicePrep {
+  const uint8_t* mq;  // per read, in the region's read order
+  int32_t min_q;
+  int32_t pcr;
+};
+
 struct RegionLayout {
   uint32_t n_haps = 0, n_reads = 0;
   uint32_t hap0 = 0, read0 = 0;  // first global haplotype / read index
@@ -110,6 +119,9 @@ struct HostPack {
   uint32_t hap_run = 1;                    // longest haplotype run a tile may keep
   bool hap_run_fixed = false;              // keep runs of hap_run whatever the batch size
   uint64_t bin_units[kFp32Bins] = {};      // (read group, haplotype) pairs per FP32 bin
+  bool any_prep = false;                   // some reads are preprocessed on the device
+  std::vector<uint8_t> read_raw;           // their bases as given, at the qualities' offsets
+  std::vector<uint32_t> read_prep;         // per read: PrepArgs::prep word
   std::vector<char> staging;               // host image of the device inputs (upload)
   std::vector<double> nan_scratch;
   std::vector<uint8_t> flag_scratch;
@@ -120,6 +132,7 @@ struct PackSizes {
   uint64_t hap_words = 0, hap_nwords = 0, n_haps = 0;
   uint64_t read_words = 0, read_nwords = 0, qual_bytes = 0, n_reads = 0;
   uint64_t n_tiles = 0, n_pairs_slots = 0, n_rescue = 0;
+  uint64_t raw_bytes = 0;  // raw read bases (device preprocessing), 0 if none
 };
 PackSizes pack_sizes(const HostPack& P);
 
@@ -128,7 +141,7 @@ PackSizes pack_sizes(const HostPack& P);
 constexpr uint32_t kNumCounters = kFp32Bins + 2 * kNumFp64Lists;
 struct BatchLayout {
   size_t hpack = 0, hn = 0, hoff = 0, hlen = 0, rpack = 0, rn = 0, rq = 0, ri = 0, rd = 0, rc = 0;
-  size_t roff = 0, rlen = 0, tiles = 0, rbase = 0;
+  size_t roff = 0, rlen = 0, tiles = 0, rbase = 0, raw = 0, prep = 0;
   size_t in_bytes = 0;
   size_t out = 0, flags = 0, resc = 0, cnt = 0;
   size_t bytes = 0;
@@ -155,7 +168,8 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
 // Packs regions; tiles hold runs of up to pol.hap_run haplotypes, in
 // largest-first order per bin. Call split_runs before uploading such a pack
 // on its own (stage_merged splits on the fly).
-int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Policy& pol, HostPack* hp);
+int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Policy& pol, HostPack* hp,
+                    const DevicePrep* prep = nullptr);
 // Persistent grid of FP32 bin v's kernel for this batch.
 uint32_t fp32_grid(const HostPack& P, int v);
 // Haplotypes per tile for FP32 bin v: long runs amortise a tile's set-up,
@@ -183,6 +197,8 @@ struct DeviceBatch {
   const uint32_t* rescue_base = nullptr;
   uint32_t* counters = nullptr;
   uint32_t n_counters_ = 0;
+  const uint8_t* read_raw = nullptr;
+  const uint32_t* read_prep = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   // per-launch events of timed runs: launch x is bracketed by lev[x], lev[x+1]
   std::vector<hipEvent_t> lev;

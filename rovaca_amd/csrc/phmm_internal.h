@@ -27,6 +27,7 @@ struct DevTables {
   const float* mm_f;      // 128*129/2, index max*(max+1)/2 + min
   const double* ph2pr_d;
   const double* mm_d;
+  const uint8_t* gop_cap;  // PCR indel model caps, [model][repeat length 0..20]
 };
 
 // Packed sequences of one batch, all in device memory.
@@ -116,7 +117,25 @@ struct Fp64Args {
   uint8_t* flags;
 };
 
+// Device read preprocessing (for regions handed over with raw qualities):
+// per read, prep[r] = 1 << 31 | pcr << 16 | min_q << 8 | mq, or 0 when the
+// read's q/i/d/c are already final.
+struct PrepArgs {
+  uint8_t* rq;
+  uint8_t* ri;
+  uint8_t* rd;
+  uint8_t* rc;
+  const uint8_t* raw;        // read bases as given (ASCII), at the qualities' offsets
+  const uint32_t* prep;
+  const uint32_t* read_off;
+  const uint32_t* read_len;
+  const uint8_t* gop_cap;
+  uint32_t n_reads;
+};
+constexpr uint32_t kPrepOn = 1u << 31;
+
 // Kernel launchers (phmm_kernels.hip).
+hipError_t launch_prep(const PrepArgs& a, hipStream_t stream);
 bool fp32_supported(int W, int RPL);
 hipError_t launch_fp32(int W, int RPL, bool xf, int occ, bool pt, int grid, hipStream_t stream,
                        const Fp32Args& a);

@@ -31,6 +31,7 @@
 #include <utility>
 
 #include "phmm_internal.h"
+#include "phmm_prep.h"
 
 namespace rvc {
 namespace {
@@ -724,6 +725,36 @@ __global__ __launch_bounds__(64) void k_fp64(Fp64Args a) {
   }
 }
 
+// Read preprocessing on the device, one wave per read (the read's bases are
+// independent): q = min(qual, mq), 6 below min_q; gap open 45 lowered by the
+// PCR indel model's cap for the base's tandem-repeat length (the last base
+// keeps 45; insertions and deletions share it); gap continuation 10.
+// ph/roDV_schedule.cpp:522-608, ph/pairhmm_internal.cpp:391-429,
+// hc/genotype/read_record.cpp:184-210; the same steps as the host adapter.
+__global__ __launch_bounds__(64) void k_prep(PrepArgs a) {
+  const uint32_t r = blockIdx.x;
+  const uint32_t w = a.prep[r];
+  if (!(w & kPrepOn)) return;
+  const int mq = (int)(w & 255u), min_q = (int)((w >> 8) & 255u), pcr = (int)((w >> 16) & 3u);
+  const int R = (int)a.read_len[r];
+  const uint32_t off = a.read_off[r];
+  const uint8_t* bases = a.raw + off;
+  for (int pos = threadIdx.x; pos < R; pos += 64) {
+    const uint32_t g = off + (uint32_t)pos;
+    const int qual = a.rq[g];
+    const int capped = qual < mq ? qual : mq;
+    int gop = '-';
+    if (pcr != 0 && pos + 1 < R) {
+      const int cap = a.gop_cap[(pcr - 1) * 21 + repeat_length(bases, R, pos)];
+      gop = cap < gop ? cap : gop;
+    }
+    a.rq[g] = (uint8_t)(capped < min_q ? 6 : capped);
+    a.ri[g] = (uint8_t)gop;
+    a.rd[g] = (uint8_t)gop;
+    a.rc[g] = (uint8_t)('+' - 33);
+  }
+}
+
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -815,6 +846,12 @@ hipError_t launch_fp32(int W, int RPL, bool xf, int occ, bool pt, int grid, hipS
 hipError_t launch_fp64(int bin, bool pt, int grid, hipStream_t stream, const Fp64Args& a) {
   if (bin < 0 || bin >= kNumFp64Bins) return hipErrorInvalidValue;
   hipLaunchKernelGGL(pt ? kFp64Pt[bin] : kFp64[bin], dim3(grid), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep(const PrepArgs& a, hipStream_t stream) {
+  if (a.n_reads == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_prep, dim3(a.n_reads), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
 

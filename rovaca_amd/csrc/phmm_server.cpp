@@ -350,20 +350,14 @@ void server_stop() {
 
 bool server_running() { return g_srv != nullptr; }
 
-}  // namespace rvc
-
-using namespace rvc;
-
-extern "C" {
-
-int rvc_phmm_submit(const rvc_region* region, rvc_ticket* ticket) {
+int submit_impl(const rvc_region* region, const DevicePrep* prep, rvc_ticket* ticket) {
   if (!ticket) return set_error(RVC_E_INVALID, "null ticket");
   if (!g_engine.ready || !g_srv) return set_error(RVC_E_STATE, "serving loop is not running");
   int rc = validate_region(region);
   if (rc) return rc;
   auto* r = new (std::nothrow) Request;
   if (!r) return set_error(RVC_E_INVALID, "out of host memory");
-  rc = build_host_pack(region, 1, default_policy(), &r->pack);
+  rc = build_host_pack(region, 1, default_policy(), &r->pack, prep);
   if (rc) {
     delete r;
     return rc;
@@ -393,6 +387,14 @@ int rvc_phmm_submit(const rvc_region* region, rvc_ticket* ticket) {
   S.cv_work.notify_one();
   return RVC_OK;
 }
+
+}  // namespace rvc
+
+using namespace rvc;
+
+extern "C" {
+
+int rvc_phmm_submit(const rvc_region* region, rvc_ticket* ticket) { return submit_impl(region, nullptr, ticket); }
 
 int rvc_phmm_wait(rvc_ticket ticket, double* out_read_major, uint8_t* out_flags, int64_t timeout_ms,
                   rvc_ticket_timing* timing) {
@@ -460,9 +462,9 @@ int rvc_phmm_server_stats(rvc_server_stats* st) {
 }  // extern "C"
 
 namespace rvc {
-int served_region(const rvc_region* region, double* out, uint8_t* flags) {
+int served_region(const rvc_region* region, double* out, uint8_t* flags, const DevicePrep* prep) {
   rvc_ticket t = 0;
-  const int rc = rvc_phmm_submit(region, &t);
+  const int rc = submit_impl(region, prep, &t);
   if (rc) return rc;
   return rvc_phmm_wait(t, out, flags, -1, nullptr);
 }

@@ -11,7 +11,8 @@ void server_stop();
 bool server_running();
 // One region through the serving loop (submit + wait), as a worker thread
 // would call it.
-int served_region(const rvc_region* region, double* out, uint8_t* flags);
-int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags);
+int served_region(const rvc_region* region, double* out, uint8_t* flags, const DevicePrep* prep = nullptr);
+struct DevicePrep;
+int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags, const DevicePrep* prep = nullptr);
 
 }  // namespace rvc

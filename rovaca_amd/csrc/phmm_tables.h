@@ -14,6 +14,7 @@ struct HostTables {
   float mm_f[kMMEntries];
   double ph2pr_d[128];
   double mm_d[kMMEntries];
+  uint8_t gop_cap[64];  // PCR indel model: 3 models x 21 repeat lengths
 };
 
 // Fills t the way Rovaca's Context<float> / Context<double> do

@@ -366,6 +366,19 @@ def test_call_pairhmm_concurrent_equals_alone():
         assert np.array_equal(m0[~np.isnan(m0)], m1[~np.isnan(m1)])
 
 
+def test_call_pairhmm_served_device_prep_equals_host_prep():
+    say("served call_pairhmm with device preprocessing == host preprocessing + plain call (bit for bit)")
+    from test_gpu_prep import host_path, repeat_rich_region
+    for pcr in (rv.PCR_NONE, rv.PCR_CONSERVATIVE, rv.PCR_HOSTILE):
+        haps, reads, quals, mqs = repeat_rich_region(200 + pcr)
+        rc, mat, kept = rv.call_pairhmm(haps, reads, quals, mqs, min_quality_threshold=10, pcr=pcr)
+        assert rc == 0
+        want, wkept = host_path(haps, reads, quals, mqs, 10, pcr)
+        assert np.array_equal(kept, wkept)
+        m = ~np.isnan(want)
+        assert np.array_equal(m, ~np.isnan(mat)) and np.array_equal(mat[m], want[m])
+
+
 def test_loadgen_serves_poisson_stream():
     say("load generator: 300 regions at 3000/s from 8 producers")
     regs = [r for r in workload.batched_regions(40, seed=71) if r["n_haps"] * r["n_reads"] < 6000]
