@@ -18,7 +18,9 @@ region scheme on the host cores) on a bounded sample.
 
 Other modes: --workload stress (precision stress, 2048 regions), single (one
 8 x 32 region, latency), served (the serving loop alone), deep (served with
-1 % deep regions of 2000 reads x 128 haplotypes of 1000 bp).
+1 % deep regions of 2000 reads x 128 haplotypes of 1000 bp), calls (the
+region-level call_pairhmm from worker threads, reads preprocessed on the
+device or on the calling threads, PCR model off and CONSERVATIVE).
 """
 import argparse
 import json
@@ -234,12 +236,46 @@ def run_served(args):
                                      "rescued_pairs": st["rescued_pairs"]}}
 
 
+def run_calls(args):
+    """Rovaca's worker pattern: args.producers threads call call_pairhmm (raw
+    reads, region-level) one region at a time through the serving loop, with
+    the reads preprocessed on the device or on the calling threads."""
+    import rovaca_amd as rv
+    from rovaca_amd import workload
+    pool = workload.batched_regions(min(args.regions, 2048), seed=3)
+    rng = np.random.default_rng(17)
+    raws = [rv.raw_region(r, rng.integers(20, 61, int(r["n_reads"]))) for r in pool]
+    n_calls = max(256, int(args.calls))
+    order = rng.integers(0, len(pool), n_calls)
+    out = []
+    for pcr_name, pcr in (("NONE", rv.PCR_NONE), ("CONSERVATIVE", rv.PCR_CONSERVATIVE)):
+        for where in ("device", "host"):
+            rv.init(0, server=True, hbm_budget_bytes=64 << 30, host_prep=1 if where == "host" else 2)
+            rv.loadgen_call_pairhmm(raws, 128, order[:128], 10, pcr, args.producers)  # warm-up
+            res, lat = rv.loadgen_call_pairhmm(raws, n_calls, order, 10, pcr, args.producers)
+            rv.shutdown()
+            rec = {"pcr": pcr_name, "preprocessing": where, "calls": int(res["completed"]),
+                   "gcups": round(res["cells"] / res["wall_s"] / 1e9, 1), "wall_s": round(res["wall_s"], 3),
+                   "p50_ms": round(res["lat_p50_ms"], 3), "p99_ms": round(res["lat_p99_ms"], 3)}
+            out.append(rec)
+            log(f"pcr {pcr_name}, preprocessing on {where}: {rec['gcups']} GCUPS, p50 {rec['p50_ms']} ms, "
+                f"p99 {rec['p99_ms']} ms")
+    top = max(r["gcups"] for r in out if r["preprocessing"] == "device")
+    return {"metric": "PairHMM region calls (call_pairhmm from worker threads, served, 1 MI355X)", "value": top,
+            "unit": "GCUPS", "higher_is_better": True, "n_gpus": 1, "dtype": "f32", "rescue_dtype": "f64",
+            "data": "synthetic", "config": {"workload": "region_calls", "pool_regions": len(pool),
+                                            "workers": args.producers, "calls": n_calls, "min_quality": 10,
+                                            "mq": "U[20,60]"}, "runs": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batched", choices=["batched", "stress", "single", "served", "deep"])
+    ap.add_argument("--workload", default="batched",
+                    choices=["batched", "stress", "single", "served", "deep", "calls"])
+    ap.add_argument("--calls", type=int, default=8192, help="calls: region calls per run")
     ap.add_argument("--rates", default="2000,8000,32000", help="served: arrival rates, regions/s")
     ap.add_argument("--duration", type=float, default=1.0, help="served: seconds of arrivals per rate")
     ap.add_argument("--producers", type=int, default=32)
@@ -276,6 +312,9 @@ def main():
         return
     if args.workload == "single":
         print(json.dumps(run_single(args)), flush=True)
+        return
+    if args.workload == "calls":
+        print(json.dumps(run_calls(args)), flush=True)
         return
 
     t = time.time()

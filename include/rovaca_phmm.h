@@ -92,7 +92,10 @@ typedef struct rvc_config {
   uint32_t server_threads;   /* 0 = no serving loop; 1 = start it at init          */
   uint64_t hbm_budget_bytes; /* serving: bound on resident request buffers (0 = 64 GiB) */
   uint32_t max_batch_regions;/* serving: regions per device launch (0 = 4096)       */
-  uint32_t reserved0;
+  uint32_t host_prep;        /* where rvc_phmm_call_pairhmm preprocesses reads:     */
+                             /* 0 = on the device when the PCR model is on, else   */
+                             /* on the calling thread; 1 = always on the calling   */
+                             /* thread; 2 = always on the device                   */
   const char* tune_table;    /* kernel-shape table written by the tuner; NULL = the */
                              /* table installed beside the library, if any         */
   uint32_t reserved[6];

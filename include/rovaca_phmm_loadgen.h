@@ -37,6 +37,16 @@ typedef struct rvc_loadgen_result {
 int rvc_loadgen_run(const rvc_region* regions, uint32_t n_regions, const uint32_t* order, const double* arrival_s,
                     uint32_t n_arrivals, uint32_t n_producers, double* latency_ms, rvc_loadgen_result* result);
 
+/* Rovaca's worker pattern: each of n_workers threads calls
+ * rvc_phmm_call_pairhmm (raw reads in, hap-major matrix and kept reads out)
+ * for its share of n_calls calls, one region at a time; call i uses region
+ * order[i] (or i % n_regions). latency_ms (optional, n_calls entries)
+ * receives each call's duration. result->wall_s spans the first call to the
+ * last return; queue and lateness fields are 0. */
+int rvc_loadgen_call_pairhmm(const rvc_raw_region* regions, uint32_t n_regions, const uint32_t* order, uint32_t n_calls,
+                             int32_t min_quality_threshold, int32_t pcr_option, uint32_t n_workers, double* latency_ms,
+                             rvc_loadgen_result* result);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,7 +32,7 @@ EXPORTS = [
     "rvc_phmm_call_pairhmm", "rvc_phmm_batch_create", "rvc_phmm_batch_run", "rvc_phmm_batch_results",
     "rvc_phmm_batch_info", "rvc_phmm_batch_destroy", "rvc_phmm_submit", "rvc_phmm_wait", "rvc_phmm_poll",
     "rvc_phmm_server_stats", "rvc_phmm_preprocess_read", "rvc_phmm_postprocess", "rvc_phmm_tables",
-    "rvc_loadgen_run", "rvc_phmm_batch_kernels",
+    "rvc_loadgen_run", "rvc_loadgen_call_pairhmm", "rvc_phmm_batch_kernels",
 ]
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -56,7 +56,7 @@ class RvcRawRegion(ctypes.Structure):
 class RvcConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("server_threads", ctypes.c_uint32),
                 ("hbm_budget_bytes", ctypes.c_uint64), ("max_batch_regions", ctypes.c_uint32),
-                ("reserved0", ctypes.c_uint32), ("tune_table", ctypes.c_char_p), ("reserved", ctypes.c_uint32 * 6)]
+                ("host_prep", ctypes.c_uint32), ("tune_table", ctypes.c_char_p), ("reserved", ctypes.c_uint32 * 6)]
 
 
 class RvcBatchOpts(ctypes.Structure):
@@ -131,6 +131,9 @@ def lib():
         L.rvc_phmm_postprocess.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _f64p, _i32p, _f64p, _u32p]
         L.rvc_loadgen_run.argtypes = [ctypes.POINTER(RvcRegion), ctypes.c_uint32, _u32p, _f64p, ctypes.c_uint32,
                                       ctypes.c_uint32, _f64p, ctypes.POINTER(RvcLoadgenResult)]
+        L.rvc_loadgen_call_pairhmm.argtypes = [ctypes.POINTER(RvcRawRegion), ctypes.c_uint32, _u32p, ctypes.c_uint32,
+                                               ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, _f64p,
+                                               ctypes.POINTER(RvcLoadgenResult)]
         L.rvc_phmm_tables.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _f64p,
                                       _f64p]
         _lib = L
@@ -160,8 +163,11 @@ def make_c_region(region):
     return cr
 
 
-def init(device=0, server=False, hbm_budget_bytes=0, max_batch_regions=0, tune_table=None):
+def init(device=0, server=False, hbm_budget_bytes=0, max_batch_regions=0, tune_table=None, host_prep=0):
+    """host_prep: where call_pairhmm preprocesses reads (0 = device when the
+    PCR model is on, else host; 1 = host; 2 = device)."""
     cfg = RvcConfig()
+    cfg.host_prep = int(host_prep)
     cfg.tune_table = tune_table.encode() if tune_table else None
     cfg.device = device
     cfg.server_threads = 1 if server else 0
@@ -342,6 +348,37 @@ def loadgen_run(regions, arrival_s, order=None, n_producers=32):
     rc = lib().rvc_loadgen_run(arr, len(regions), _ptr(o, _u32p) if o is not None else None, _ptr(arrival, _f64p),
                                n, n_producers, _ptr(lat, _f64p), ctypes.byref(res))
     _check(rc)
+    return {k: getattr(res, k) for k, _ in RvcLoadgenResult._fields_}, lat
+
+
+def raw_region(region, mqs):
+    """The rvc_raw_region view of a region dict: its q array as the raw base
+    qualities and the given per-read mapping qualities (arrays kept alive on
+    the returned object)."""
+    raw = RvcRawRegion()
+    keep = [np.ascontiguousarray(region["hap_off"], np.int32), np.ascontiguousarray(region["hap_bases"], np.uint8),
+            np.ascontiguousarray(region["read_off"], np.int32), np.ascontiguousarray(region["read_bases"], np.uint8),
+            np.ascontiguousarray(region["q"], np.uint8), np.ascontiguousarray(mqs, np.uint8)]
+    raw.n_haps, raw.n_reads = int(region["n_haps"]), int(region["n_reads"])
+    raw.hap_off, raw.hap_bases = _ptr(keep[0], _i32p), _ptr(keep[1])
+    raw.read_off, raw.read_bases = _ptr(keep[2], _i32p), _ptr(keep[3])
+    raw.read_qual, raw.read_mq = _ptr(keep[4]), _ptr(keep[5])
+    raw._keep = keep
+    return raw
+
+
+def loadgen_call_pairhmm(raw_regions, n_calls, order=None, min_quality_threshold=0, pcr=PCR_NONE, n_workers=32):
+    """n_calls region calls (rvc_phmm_call_pairhmm) from n_workers threads
+    (C++), one region at a time per thread. Returns (summary, latency ms)."""
+    arr = (RvcRawRegion * len(raw_regions))()
+    for x, r in enumerate(raw_regions):
+        arr[x] = r
+    o = None if order is None else np.ascontiguousarray(order, np.uint32)
+    lat = np.zeros(n_calls, np.float64)
+    res = RvcLoadgenResult()
+    _check(lib().rvc_loadgen_call_pairhmm(arr, len(raw_regions), _ptr(o, _u32p) if o is not None else None, n_calls,
+                                          min_quality_threshold, pcr, n_workers, _ptr(lat, _f64p),
+                                          ctypes.byref(res)))
     return {k: getattr(res, k) for k, _ in RvcLoadgenResult._fields_}, lat
 
 

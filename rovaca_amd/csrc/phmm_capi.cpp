@@ -169,8 +169,24 @@ int rvc_phmm_call_pairhmm(const rvc_raw_region* rg, int32_t min_quality_threshol
     if (len < 0) return fail(set_error(RVC_E_INVALID, "read %u has negative length", r));
     rlen[r] = len;
   }
-  // the reads are preprocessed on the device (MQ cap, quality floor, GOP 45 /
-  // PCR model, GCP 10): the region carries the raw qualities
+  // read preprocessing (MQ cap, quality floor, GOP 45 / PCR model, GCP 10):
+  // on the device, where the region carries the raw qualities, or here on
+  // the calling thread. By default the device takes it when the PCR model
+  // is on (its repeat search is the costly part); without it the host steps
+  // are a few operations per base.
+  const bool on_host = g_engine.host_prep == 1 || (g_engine.host_prep == 0 && pcr_option == RVC_PCR_NONE);
+  std::vector<uint8_t> q, gop, gcp;
+  if (on_host) {
+    const int32_t total = rg->read_off[N];
+    q.resize(total);
+    gop.resize(total);
+    gcp.resize(total);
+    for (uint32_t r = 0; r < N; ++r) {
+      const int32_t o = rg->read_off[r];
+      preprocess_read(rg->read_bases + o, rg->read_qual + o, rlen[r], rg->read_mq[r], min_quality_threshold,
+                      pcr_option, q.data() + o, gop.data() + o, gcp.data() + o);
+    }
+  }
   rvc_region reg;
   reg.n_haps = H;
   reg.n_reads = N;
@@ -186,11 +202,19 @@ int rvc_phmm_call_pairhmm(const rvc_raw_region* rg, int32_t min_quality_threshol
   prep.mq = rg->read_mq;
   prep.min_q = min_quality_threshold;
   prep.pcr = pcr_option;
+  const DevicePrep* pp = &prep;
+  if (on_host) {
+    reg.q = q.data();
+    reg.i = gop.data();
+    reg.d = gop.data();
+    reg.c = gcp.data();
+    pp = nullptr;
+  }
   // with the serving loop running, concurrent callers (Rovaca's worker
   // threads) are batched together; otherwise the region runs on its own
   std::vector<double> read_major((size_t)H * N);
-  int rc = server_running() ? served_region(&reg, read_major.data(), nullptr, &prep)
-                            : region_sync_impl(&reg, read_major.data(), nullptr, &prep);
+  int rc = server_running() ? served_region(&reg, read_major.data(), nullptr, pp)
+                            : region_sync_impl(&reg, read_major.data(), nullptr, pp);
   if (rc) return fail(rc);
   *n_kept = (uint32_t)postprocess(N, H, read_major.data(), rlen.data(), out_hap_major, kept_reads);
   return RVC_OK;

@@ -121,3 +121,65 @@ extern "C" int rvc_loadgen_run(const rvc_region* regions, uint32_t n_regions, co
   return result->failed ? set_error(RVC_E_DEVICE, "%llu requests failed", (unsigned long long)result->failed)
                         : RVC_OK;
 }
+
+extern "C" int rvc_loadgen_call_pairhmm(const rvc_raw_region* regions, uint32_t n_regions, const uint32_t* order,
+                                        uint32_t n_calls, int32_t min_quality_threshold, int32_t pcr_option,
+                                        uint32_t n_workers, double* latency_ms, rvc_loadgen_result* result) {
+  using rvc::set_error;
+  if (!regions || !result || n_workers == 0 || n_regions == 0) return set_error(RVC_E_INVALID, "bad arguments");
+  for (uint32_t i = 0; i < n_calls; ++i)
+    if (order && order[i] >= n_regions) return set_error(RVC_E_INVALID, "order[%u] out of range", i);
+  std::vector<double> lat(n_calls, 0);
+  std::vector<int> rcs(n_calls, 0);
+  std::vector<uint64_t> t0(n_calls, 0), t1(n_calls, 0);
+  auto worker = [&](uint32_t w) {
+    std::vector<double> mat;
+    std::vector<uint32_t> kept;
+    for (uint32_t i = w; i < n_calls; i += n_workers) {
+      const rvc_raw_region& rg = regions[order ? order[i] : i % n_regions];
+      mat.resize((size_t)rg.n_haps * rg.n_reads);
+      kept.resize(rg.n_reads);
+      uint32_t nk = 0;
+      const Clock::time_point a = Clock::now();
+      rcs[i] = rvc_phmm_call_pairhmm(&rg, min_quality_threshold, pcr_option, mat.data(), kept.data(), &nk);
+      const Clock::time_point b = Clock::now();
+      t0[i] = ns_of(a);
+      t1[i] = ns_of(b);
+      lat[i] = std::chrono::duration<double, std::milli>(b - a).count();
+    }
+  };
+  std::vector<std::thread> th;
+  for (uint32_t w = 0; w < n_workers; ++w) th.emplace_back(worker, w);
+  for (auto& t : th) t.join();
+  std::memset(result, 0, sizeof *result);
+  std::vector<double> ok_lat;
+  uint64_t first = UINT64_MAX, last = 0;
+  for (uint32_t i = 0; i < n_calls; ++i) {
+    result->submitted++;
+    if (rcs[i] != RVC_OK) {
+      result->failed++;
+      if (latency_ms) latency_ms[i] = -1;
+      continue;
+    }
+    result->completed++;
+    const rvc_raw_region& rg = regions[order ? order[i] : i % n_regions];
+    uint64_t hs = 0, rs = 0;
+    for (uint32_t h = 0; h < rg.n_haps; ++h) hs += (uint64_t)(rg.hap_off[h + 1] - rg.hap_off[h]);
+    for (uint32_t r = 0; r < rg.n_reads; ++r) rs += (uint64_t)(rg.read_off[r + 1] - rg.read_off[r]);
+    result->cells += hs * rs;
+    ok_lat.push_back(lat[i]);
+    if (latency_ms) latency_ms[i] = lat[i];
+    first = std::min(first, t0[i]);
+    last = std::max(last, t1[i]);
+  }
+  if (!ok_lat.empty()) {
+    result->wall_s = (double)(last - first) / 1e9;
+    result->lat_p50_ms = pct(ok_lat, 0.50);
+    result->lat_p99_ms = pct(ok_lat, 0.99);
+    result->lat_max_ms = pct(ok_lat, 1.0);
+    double sum = 0;
+    for (double x : ok_lat) sum += x;
+    result->lat_mean_ms = sum / (double)ok_lat.size();
+  }
+  return result->failed ? set_error(RVC_E_DEVICE, "%llu calls failed", (unsigned long long)result->failed) : RVC_OK;
+}

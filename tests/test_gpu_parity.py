@@ -390,6 +390,15 @@ def test_loadgen_serves_poisson_stream():
     assert res["cells"] == sum(workload.cells([regs[i % len(regs)]]) for i in range(300))
 
 
+def test_loadgen_region_calls():
+    say("worker threads calling call_pairhmm (C++ load generator)")
+    rng = np.random.default_rng(91)
+    pool = [r for r in workload.batched_regions(12, seed=92) if r["n_haps"] * r["n_reads"] < 6000]
+    raws = [rv.raw_region(r, rng.integers(20, 61, int(r["n_reads"]))) for r in pool]
+    res, lat = rv.loadgen_call_pairhmm(raws, 48, None, 10, rv.PCR_CONSERVATIVE, 8)
+    assert res["completed"] == 48 and res["failed"] == 0 and (lat > 0).all()
+
+
 def test_budget_admission():
     say("admission by budget: a region larger than the whole budget is refused")
     rv.shutdown()

@@ -21,8 +21,8 @@ def say(msg):
 
 @pytest.fixture(scope="module", autouse=True)
 def engine():
-    say("init device 0 (no serving loop)")
-    rv.init(0)
+    say("init device 0 (no serving loop, reads preprocessed on the device)")
+    rv.init(0, host_prep=2)
     yield
     rv.shutdown()
 
