@@ -732,13 +732,16 @@ __global__ __launch_bounds__(64) void k_fp64(Fp64Args a) {
 // ph/roDV_schedule.cpp:522-608, ph/pairhmm_internal.cpp:391-429,
 // hc/genotype/read_record.cpp:184-210; the same steps as the host adapter.
 __global__ __launch_bounds__(64) void k_prep(PrepArgs a) {
+  __shared__ uint8_t bases[kMaxReadLen];
   const uint32_t r = blockIdx.x;
   const uint32_t w = a.prep[r];
   if (!(w & kPrepOn)) return;
   const int mq = (int)(w & 255u), min_q = (int)((w >> 8) & 255u), pcr = (int)((w >> 16) & 3u);
   const int R = (int)a.read_len[r];
   const uint32_t off = a.read_off[r];
-  const uint8_t* bases = a.raw + off;
+  // the repeat search reads each base many times: from LDS, not from L2
+  for (int x = threadIdx.x; x < R; x += 64) bases[x] = a.raw[off + x];
+  __syncthreads();
   for (int pos = threadIdx.x; pos < R; pos += 64) {
     const uint32_t g = off + (uint32_t)pos;
     const int qual = a.rq[g];
