@@ -46,6 +46,10 @@ def lib():
         L.orc_region_reference.restype = ctypes.c_int
         L.orc_preprocess_read.argtypes = [_u8p, _u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           _u8p, _u8p, _u8p, _u8p]
+        L.orc_count_repetitions.argtypes = [_u8p, ctypes.c_int, _u8p, ctypes.c_int, ctypes.c_int]
+        L.orc_count_repetitions.restype = ctypes.c_int
+        L.orc_repeat_length.argtypes = [_u8p, ctypes.c_int, ctypes.c_int]
+        L.orc_repeat_length.restype = ctypes.c_int
         L.orc_postprocess.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, _i32p, _f64p, _i32p]
         L.orc_postprocess.restype = ctypes.c_int
         _lib = L
@@ -148,6 +152,18 @@ def region_reference(region, float_width=16, double_width=8):
                                 keep[6].ctypes.data_as(_u8p), keep[7].ctypes.data_as(_u8p),
                                 float_width, double_width, out.ctypes.data_as(_f64p), route.ctypes.data_as(_u8p))
     return out.reshape(nr, nh), route.reshape(nr, nh), bool(ok)
+
+
+def count_repetitions(unit, test, leading):
+    u = np.frombuffer(bytes(unit) or b"\0", np.uint8)
+    t = np.frombuffer(bytes(test) or b"\0", np.uint8)
+    return int(lib().orc_count_repetitions(u.ctypes.data_as(_u8p), len(unit), t.ctypes.data_as(_u8p), len(test),
+                                           1 if leading else 0))
+
+
+def repeat_length(bases, offset):
+    b = np.frombuffer(bytes(bases), np.uint8)
+    return int(lib().orc_repeat_length(b.ctypes.data_as(_u8p), len(b), offset))
 
 
 def preprocess_read(bases, qual, mq, min_q, pcr):

@@ -23,6 +23,8 @@
 //                           (independent of the dynamic programme)
 //   orc_region_reference    ph/roDV_schedule.cpp:364-834 (pair list, sort,
 //                           greedy 16/8-lane grouping, three passes)
+//   orc_count_repetitions,  ph/pairhmm_internal.cpp:544-574 (copy counting) and
+//   orc_repeat_length       :260-314 (tandem-repeat length)
 //   orc_preprocess_read     ph/roDV_schedule.cpp:494-649 plus
 //                           hc/genotype/read_record.cpp:184-210 (MQ cap,
 //                           min-quality floor, GOP 45 / GCP 10 defaults) and
@@ -547,6 +549,15 @@ int tandem_repeat_units(const uint8_t* b, int num, int offset) {
   return max_rl > 20 ? 20 : max_rl;
 }
 }  // namespace
+
+// Whole copies of unit u[0 .. ul) counted from the front (leading != 0) or
+// the back of test[0 .. tl) (the counting of ph/pairhmm_internal.cpp:544-574).
+extern "C" int orc_count_repetitions(const uint8_t* u, int ul, const uint8_t* test, int tl, int leading) {
+  return n_repetitions(u, 0, ul, test, 0, tl, leading != 0);
+}
+
+// Tandem-repeat length at base 'offset' of read b[0 .. num), capped at 20.
+extern "C" int orc_repeat_length(const uint8_t* b, int num, int offset) { return tandem_repeat_units(b, num, offset); }
 
 // bases: decoded ASCII read bases; qual: raw Phred; pcr: 0 NONE, 1 HOSTILE,
 // 2 AGGRESSIVE, 3 CONSERVATIVE (hc/common/enum.h:8).

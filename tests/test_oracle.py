@@ -121,3 +121,52 @@ def test_region_routing_and_golden():
     assert np.abs(out - g["ref_intra"]).max() < 1e-5
     assert np.abs(out - g["fp64"]).max() < 1e-5
     assert set(np.unique(route)) <= {1, 2, 3, 4, 5}
+
+
+# Known answers of Rovaca's own repeat-count test
+# (genotype/test/lush_variant_context_utils_unit_test.cpp:349-370); the
+# counter there and the one the PCR indel model uses
+# (pairhmm/pairhmm_internal.cpp:510-542) walk the test string the same way.
+REPETITION_KATS = [
+    ("AT", "GATAT", False, 2), ("AT", "GATAT", True, 0), ("A", "ATATG", True, 1), ("AT", "ATATG", True, 2),
+    ("CCC", "CCCCCCCC", True, 2), ("CCC", "CCCCCCCC", False, 2),
+    ("ATG", "ATGATGATGATG", True, 4), ("G", "ATGATGATGATG", True, 0), ("T", "T", True, 1),
+    ("AT", "ATGATGATCATG", True, 1), ("CCCCCCCC", "CCC", True, 0), ("AT", "AT", True, 1), ("AT", "", True, 0),
+    ("ATG", "ATGATGATGATG", False, 4), ("G", "ATGATGATGATG", False, 1), ("T", "T", False, 1),
+    ("AT", "ATGATGATCATG", False, 0), ("CCCCCCCC", "CCC", False, 0), ("AT", "AT", False, 1), ("AT", "", False, 0),
+]
+
+
+@pytest.mark.parametrize("unit,test,leading,expected", REPETITION_KATS)
+def test_count_repetitions_kats(unit, test, leading, expected):
+    assert oracle.count_repetitions(unit.encode(), test.encode(), leading) == expected
+
+
+def test_repeat_length_closed_forms():
+    # A read that is one homopolymer of length L has repeat length min(L, 20)
+    # at every base: backward and forward runs of the one-base unit add up to L.
+    for L in (2, 5, 12, 20, 21, 33):
+        assert [oracle.repeat_length(b"A" * L, x) for x in range(L)] == [min(L, 20)] * L
+    # Dinucleotide ACACACAC. Base 0: no backward unit repeats, forward unit CA
+    # occurs 3 times and once-shifted AC has no copy in "A": 0 + 3.  Base 1:
+    # forward unit AC occurs 3 times, its backward count over "AC" is 1: 1 + 3.
+    assert oracle.repeat_length(b"ACACACAC", 0) == 3
+    assert oracle.repeat_length(b"ACACACAC", 1) == 4
+    # A run inside unrelated bases: GCTTTTTCA, base 3 (second T) sees two
+    # T's behind it (including itself) and three ahead.
+    assert oracle.repeat_length(b"GCTTTTTCA", 3) == 5
+    # No repeat at all: the forward count of the whole tail is 1.
+    assert oracle.repeat_length(b"ACGTTGCA", 0) == 1
+
+
+def test_pcr_gop_follows_repeat_length():
+    # HOSTILE caps a repeat of length 12 at GOP 10, AGGRESSIVE at 34,
+    # CONSERVATIVE at 37 (pairhmm/adapter.h:13-15); the last base keeps 45.
+    for pcr, cap in ((1, 10), (2, 34), (3, 37)):
+        _, gi, gd, _ = oracle.preprocess_read(b"A" * 12, np.full(12, 30, np.uint8), 60, 0, pcr)
+        assert list(gi) == [cap] * 11 + [45]
+        assert list(gd) == list(gi)
+    # Without a repeat (length 1 at both inner bases of ACG) HOSTILE caps at 40.
+    assert oracle.repeat_length(b"ACG", 0) == oracle.repeat_length(b"ACG", 1) == 1
+    _, gi, _, _ = oracle.preprocess_read(b"ACG", np.full(3, 30, np.uint8), 60, 0, 1)
+    assert list(gi) == [40, 40, 45]
