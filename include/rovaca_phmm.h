@@ -138,8 +138,10 @@ typedef struct rvc_batch rvc_batch;
 typedef struct rvc_batch_opts {
   uint32_t persistent_waves_per_cu; /* 0 = as many as fit                             */
   uint32_t use_graph;               /* 1 = capture the launch sequence in a hipGraph  */
-  uint32_t fp32_fold;               /* 0 = tuned choice, 1 = fold the gap-to-match    */
-                                    /* factor into I/D where allowed, 2 = never       */
+  uint32_t fp32_fold;               /* 0 = tuned choice, 1 = rescale I/D (FP32: the   */
+                                    /* 5-operation scaled form, FP64: gap-to-match    */
+                                    /* folded in) where the gap qualities allow,      */
+                                    /* 2 = never                                      */
   uint32_t rpl_round;               /* 0 = tuned choice, 1 = exact rows per lane,     */
                                     /* 2 = rows per lane in steps of 3                */
   uint32_t occupancy;               /* 0 = tuned choice, else 2/3/4 waves per SIMD    */
@@ -169,7 +171,7 @@ typedef struct rvc_batch_info {
 
 /* One kernel launch of the last timed (non-graph) run of a batch. */
 typedef struct rvc_kernel_time {
-  char name[48];          /* e.g. "k_fp32<16,10,fold,occ4>"                          */
+  char name[48];          /* e.g. "k_fp32<16,10,scaled,occ4>"                        */
   uint32_t precision;     /* 32 or 64                                                */
   uint32_t width;         /* lanes per read (W)                                      */
   uint32_t rows_per_lane; /* RPL                                                     */

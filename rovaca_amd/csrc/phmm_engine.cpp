@@ -375,12 +375,25 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
         std::memcpy(&P.read_raw[rcur], g.read_bases + o, len);
         P.read_prep[gi] = kPrepOn | ((uint32_t)(prep->pcr & 3) << 16) | ((uint32_t)(prep->min_q & 255) << 8) |
                           (uint32_t)prep->mq[x];
+        // gap open 45, or down to 10 under a PCR model; continuation 10
+        P.gop_lo = std::min<uint8_t>(P.gop_lo, prep->pcr != 0 ? 10 : 45);
+        P.gop_hi = std::max<uint8_t>(P.gop_hi, 45);
+        P.gcp_lo = std::min<uint8_t>(P.gcp_lo, 10);
       } else {
         std::memcpy(&P.ri[rcur], g.i + o, len);
         std::memcpy(&P.rd[rcur], g.d + o, len);
         std::memcpy(&P.rc[rcur], g.c + o, len);
-        for (uint32_t y = 0; y < len; ++y)
-          if ((g.c[o + y] & 127) == 0) P.fold_gap = false;
+        uint8_t lo = P.gop_lo, hi = P.gop_hi, clo = P.gcp_lo;
+        for (uint32_t y = 0; y < len; ++y) {
+          const uint8_t gi_ = g.i[o + y] & 127, gd_ = g.d[o + y] & 127, gc_ = g.c[o + y] & 127;
+          lo = std::min(lo, std::min(gi_, gd_));
+          hi = std::max(hi, std::max(gi_, gd_));
+          clo = std::min(clo, gc_);
+        }
+        P.gop_lo = lo;
+        P.gop_hi = hi;
+        P.gcp_lo = clo;
+        if (clo == 0) P.fold_gap = false;
       }
       rcur += align_up(len, kSeqAlign);
     }
@@ -446,12 +459,26 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
   return RVC_OK;
 }
 
+int scaled_exponent(const HostPack& P) {
+  // Gap open and continuation >= 6 keep tMM >= 1/2 and tDD <= 1/4. I~ = I/tMI
+  // exceeds M by at most about the largest over the smallest gap-open
+  // probability, 10^(spread/10) (at most 2^16 for a spread of 48): X lowers
+  // 2^120 by that many bits plus 2, so that no cell holds more than in the
+  // 2^120 scale of the other forms.
+  if (!P.fold_gap) return 0;
+  if (P.gop_hi < P.gop_lo) return 118;  // no read rows
+  const int spread = P.gop_hi - P.gop_lo;
+  if (P.gop_lo < 6 || P.gcp_lo < 6 || spread > 48) return 0;
+  return 118 - (int)std::ceil(spread * 0.33219280948873623);
+}
+
 uint32_t fp32_grid(const HostPack& P, int v) {
   int W, RPL;
   fp32_variant_shape(v % kFp32Variants, &W, &RPL);
-  const bool pt = v >= kFp32Variants && P.prior_table && P.fold_gap;
+  const bool sc = scaled_exponent(P) > 0;
+  const bool pt = v >= kFp32Variants && P.prior_table && sc;
   const int occ = pt ? P.occ[v] : P.occ[v % kFp32Variants];
-  int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL, P.fold_gap, occ, pt));
+  int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL, sc, occ, pt));
   if (P.waves_per_cu) per_cu = std::min<int>(per_cu, (int)P.waves_per_cu);
   return (uint32_t)per_cu * (uint32_t)g_engine.n_cu;
 }
@@ -651,6 +678,9 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
     P.n_pairs += A.n_pairs;
     P.n_cells += A.n_cells;
     P.fold_gap = P.fold_gap && A.fold_gap;
+    P.gop_lo = std::min(P.gop_lo, A.gop_lo);
+    P.gop_hi = std::max(P.gop_hi, A.gop_hi);
+    P.gcp_lo = std::min(P.gcp_lo, A.gcp_lo);
     P.waves_per_cu = A.waves_per_cu;
     P.prior_table = A.prior_table;
     P.hap_run = A.hap_run;
@@ -849,12 +879,14 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
   };
   int rc = mark(0);
   if (rc) return rc;
+  const int X = scaled_exponent(P);
+  const bool sc = X > 0;
   for (int v = 0; v < kFp32Bins; ++v) {
     const uint32_t n = P.bin_start[v + 1] - P.bin_start[v];
     if (n == 0) continue;
     int W, RPL;
     fp32_variant_shape(v % kFp32Variants, &W, &RPL);
-    const bool pt = v >= kFp32Variants && P.prior_table && P.fold_gap;
+    const bool pt = v >= kFp32Variants && P.prior_table && sc;
     Fp32Args a;
     a.s = seqs;
     a.t = g_engine.tables;
@@ -867,14 +899,16 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     a.rescue_count = rescue_count;
     a.rescue_base = rescue_base;
     a.rescue_list0 = v >= kFp32Variants ? (uint32_t)kNumFp64Bins : 0u;
+    a.init_scale = sc ? std::ldexp(1.f, X) : 0x1p120f;
+    a.back_scale = sc ? std::ldexp(1.f, 120 - X) : 1.f;
     const int occ = pt ? P.occ[v] : P.occ[v % kFp32Variants];
     const int grid = (int)std::min<uint64_t>(n, fp32_grid(P, v));
-    hipError_t e = launch_fp32(W, RPL, P.fold_gap, occ, pt, grid, s, a);
+    hipError_t e = launch_fp32(W, RPL, sc, occ, pt, grid, s, a);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP32 launch W=%d RPL=%d: %s", W, RPL, hipGetErrorString(e));
     ++n_fp32_launches;
     rvc_kernel_time kt{};
     snprintf(kt.name, sizeof kt.name, "k_fp32%s<%d,%d,%s,occ%d>", pt ? "pt" : "", W, RPL,
-             P.fold_gap ? "fold" : "plain", occ);
+             sc ? "scaled" : "plain", occ);
     kt.precision = 32; kt.width = W; kt.rows_per_lane = RPL; kt.occupancy = P.occ[v];
     kt.tiles = n; kt.cells = P.variant_cells[v];
     launches.push_back(kt);

@@ -58,7 +58,7 @@ struct FP32Policy {
   // waves per SIMD the 16-lane kernel of each RPL is compiled for (2, 3, 4)
   uint8_t occ16[17] = {4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 2, 2, 2, 2, 2, 2};
   bool rpl_steps_of_3 = true;  // 16-lane shapes only in RPL 1, 2, 4, 7, 10, 13, 16
-  bool fold = true;            // use the folded form when every gcp > 0
+  bool fold = true;            // rescaled forms where the gap qualities allow
   uint32_t waves_per_cu = 0;   // persistent grid: 0 = occupancy limit
   // the same for the prior-table kernels (W = 16 and W = 32)
   uint8_t occpt16[17] = {4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 2, 2, 2, 2, 2, 2};
@@ -113,6 +113,8 @@ struct HostPack {
   uint64_t n_pairs = 0;             // pairs with two non-empty sides
   uint64_t n_cells = 0;
   bool fold_gap = true;             // every gap-continuation quality is > 0
+  uint8_t gop_lo = 127, gop_hi = 0;  // range of the gap-open qualities (insertion and deletion)
+  uint8_t gcp_lo = 127;              // smallest gap-continuation quality
   uint32_t waves_per_cu = 0;
   uint8_t occ[kFp32Bins] = {};            // occupancy target per FP32 bin
   uint64_t variant_cells[kFp32Bins] = {};  // cells computed by each FP32 bin
@@ -127,6 +129,10 @@ struct HostPack {
   std::vector<double> nan_scratch;
   std::vector<uint8_t> flag_scratch;
 };
+
+// Row -1 scale exponent X of the FP32 pass's scaled form (2^X in place of
+// 2^120), or 0 where the batch's gap qualities rule that form out.
+int scaled_exponent(const HostPack& P);
 
 // Element counts of a batch's device arrays.
 struct PackSizes {

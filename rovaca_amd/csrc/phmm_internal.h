@@ -105,6 +105,8 @@ struct Fp32Args {
   uint32_t* rescue_count;    // kNumFp64Lists counters
   const uint32_t* rescue_base;
   uint32_t rescue_list0;     // this launch's pairs go to list rescue_list0 + fp64_bin(R)
+  float init_scale;          // row -1 deletion value times H: 2^120, or 2^X in the scaled form
+  float back_scale;          // 2^(120 - X): brings the scaled form's sum to the 2^120 scale
 };
 
 struct Fp64Args {
@@ -137,7 +139,9 @@ constexpr uint32_t kPrepOn = 1u << 31;
 // Kernel launchers (phmm_kernels.hip).
 hipError_t launch_prep(const PrepArgs& a, hipStream_t stream);
 bool fp32_supported(int W, int RPL);
-hipError_t launch_fp32(int W, int RPL, bool xf, int occ, bool pt, int grid, hipStream_t stream,
+// scaled: the 5-operation scaled form (else plain); pt: the prior-table kernel
+// (scaled form only).
+hipError_t launch_fp32(int W, int RPL, bool scaled, int occ, bool pt, int grid, hipStream_t stream,
                        const Fp32Args& a);
 // FP64 rescue of one list's bin; pt = the prior-table kernel (pairs without
 // N in the haplotype, folded form).

@@ -194,13 +194,45 @@ __device__ __forceinline__ double i_update<double>(double tmi, double um, double
   return fma(tmi, um, tii * ui);
 }
 
+// Forms of the recurrence a sweep runs (template argument F).
+//  kPlain  the model's M, I and D:
+//            M = prior * (tMM*M' + tGM*(I' + D'))
+//  kFold   I and D stored pre-multiplied by the next row's tGM (described at
+//          forward()): M = prior * (tMM*M' + I' + D'), 7 operations per cell.
+//  kScaled I and D stored divided by their own row's gap-open probability,
+//          I~ = I / tMI[r] and D~ = D / tMD[r] (row -1: D~ = D), M unchanged:
+//            M  = (prior*tMM) * (M' + rho*I~' + delta*D~')
+//            I~ = M_up + c*I~_up         D~ = M_left + tDD*D~_left
+//          with rho = tGM*tMI[r-1]/tMM, delta = tGM*tMD[r-1]/tMM and
+//          c = tII*tMI[r-1]/tMI[r]: 5 operations per cell. In this form the
+//          coefficient arrays hold tmi = rho, tmd = delta, tii = c, tdd = tDD
+//          and the priors are multiplied by tMM (row_params). I~ exceeds M by
+//          up to the ratio of the largest to the smallest gap-open
+//          probability in the batch, so the host lowers the row -1 scale
+//          2^120 by that ratio (Fp32Args::init_scale) and the sum is scaled
+//          back before the rescue test.
+constexpr int kPlain = 0, kFold = 1, kScaled = 2;
+
+// New M, I, D of one cell from the row above at column j-1 (d*), the row
+// above at column j (u*) and the same row at column j-1 (l*).
+template <typename T, int F>
+__device__ __forceinline__ void cell(T pr, T dm, T di, T dd, T um, T ui, T lm, T ld, T tmm, T tgm, T tmi, T tii,
+                                     T tmd, T tdd, T& nm, T& ni, T& nd) {
+  if constexpr (F == kScaled) {
+    nm = pr * fma(tmd, dd, fma(tmi, di, dm));
+    ni = fma(tii, ui, um);
+    nd = fma(tdd, ld, lm);
+  } else {
+    if constexpr (F == kFold) nm = pr * fma(tmm, dm, di + dd);
+    else nm = pr * fma(tmm, dm, tgm * (di + dd));
+    ni = i_update<T>(tmi, um, tii, ui);
+    nd = fma(tmd, lm, tdd * ld);
+  }
+}
+
 // One anti-diagonal step of a lane: new column j of all RPL rows from
 // column j-1 (M, I, D) and the row above (d* at j-1, u* at j).
-// XF = false: I and D are the model's values and
-//   M = prior * (tMM*M' + tGM*(I' + D')).
-// XF = true: I and D are stored pre-multiplied by the next row's tGM (see
-// forward()), so M = prior * (tMM*M' + I' + D') and one multiply per cell goes.
-template <typename T, int RPL, bool XF>
+template <typename T, int RPL, int F>
 __device__ __forceinline__ void sweep_step(const T (&M)[RPL], const T (&I)[RPL], const T (&D)[RPL], T (&nM)[RPL],
                                            T (&nI)[RPL], T (&nD)[RPL], T dM, T dI, T dD, T uM, T uI, int hc,
                                            const int (&rcode)[RPL], const T (&pm)[RPL], const T (&px)[RPL],
@@ -210,192 +242,15 @@ __device__ __forceinline__ void sweep_step(const T (&M)[RPL], const T (&I)[RPL],
   {
     const bool mt = (hc == rcode[0]) || hn;
     const T pr = mt ? pm[0] : px[0];
-    if constexpr (XF) nM[0] = pr * fma(tmm[0], dM, dI + dD);
-    else nM[0] = pr * fma(tmm[0], dM, tgm[0] * (dI + dD));
-    nI[0] = i_update(tmi[0], uM, tii[0], uI);
-    nD[0] = fma(tmd[0], M[0], tdd[0] * D[0]);
+    cell<T, F>(pr, dM, dI, dD, uM, uI, M[0], D[0], tmm[0], tgm[0], tmi[0], tii[0], tmd[0], tdd[0], nM[0], nI[0],
+               nD[0]);
   }
 #pragma unroll
   for (int r = 1; r < RPL; ++r) {
     const bool mt = (hc == rcode[r]) || hn;
     const T pr = mt ? pm[r] : px[r];
-    if constexpr (XF) nM[r] = pr * fma(tmm[r], M[r - 1], I[r - 1] + D[r - 1]);
-    else nM[r] = pr * fma(tmm[r], M[r - 1], tgm[r] * (I[r - 1] + D[r - 1]));
-    nI[r] = i_update(tmi[r], nM[r - 1], tii[r], nI[r - 1]);
-    nD[r] = fma(tmd[r], M[r], tdd[r] * D[r]);
-  }
-}
-
-// Forward sweep of one pair per segment. hb points at the segment's staged
-// haplotype codes (hb[j] = code of base j). Returns the raw sum S in the
-// segment's last lane; H is this segment's haplotype length, Hmax the
-// largest among the wave's segments.
-//
-// XF = true stores I'[k] = g[k+1]*I[k] and D'[k] = g[k+1]*D[k] with g the
-// gap-to-match probability tGM of a row (1 past the last row). Then
-//   M[k+1] = prior * (tMM*M[k] + I'[k] + D'[k])
-//   I'[k]  = (g[k+1]*tMI)*M[k-1] + (g[k+1]*tII/g[k])*I'[k-1]
-//   D'[k]  = (g[k+1]*tMD)*M[k][j-1] + tDD*D'[k][j-1]
-// which is the same recurrence with folded coefficients. It needs g > 0 for
-// every row (gap-continuation quality not 0); the host checks that per batch.
-template <typename T, int W, int RPL, bool XF>
-__device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, const uint8_t* hb, int k, uint32_t H,
-                                     uint32_t Hmax, uint32_t read) {
-  const T* ph2pr = ph2pr_tab<T>(tb);
-  const T* mmt = mm_tab<T>(tb);
-  const uint32_t R = s.read_len[read];
-  const uint32_t roff = s.read_off[read];
-
-  // Per-row parameters (ph/roDV_inter/pairhmm_inter.cpp:60-79).
-  T pm[RPL], px[RPL], tmm[RPL], tgm[RPL], tmi[RPL], tii[RPL], tmd[RPL], tdd[RPL];
-  int rcode[RPL];
-#pragma unroll
-  for (int r = 0; r < RPL; ++r) {
-    const uint32_t row = (uint32_t)k * RPL + r;
-    if (row < R) {
-      const uint32_t g = roff + row;
-      const int iq = s.rq[g] & 127, ii = s.ri[g] & 127, id = s.rd[g] & 127, ic = s.rc[g] & 127;
-      const uint32_t base = (s.read_pack[g >> 4] >> ((g & 15u) * 2u)) & 3u;
-      const uint32_t isn = (s.read_nmask[g >> 5] >> (g & 31u)) & 1u;
-      const T dist = ph2pr[iq];
-      pm[r] = (T)1 - dist;
-      px[r] = isn ? pm[r] : dist * ((T)1 / (T)3);
-      const T gc = ph2pr[ic];
-      const int mx = ii > id ? ii : id, mn = ii > id ? id : ii;
-      tmm[r] = mmt[((mx * (mx + 1)) >> 1) + mn];
-      tgm[r] = (T)1 - gc;
-      if constexpr (XF) {
-        const T gnext = (row + 1 < R) ? (T)1 - ph2pr[s.rc[g + 1] & 127] : (T)1;
-        tmi[r] = gnext * ph2pr[ii];
-        tii[r] = gnext * gc / tgm[r];
-        tmd[r] = gnext * ph2pr[id];
-        tdd[r] = gc;
-      } else {
-        tmi[r] = ph2pr[ii];
-        tii[r] = gc;
-        tmd[r] = ph2pr[id];
-        tdd[r] = gc;
-      }
-      rcode[r] = (int)base;
-    } else {
-      // padding row: M = 0, I carries M + I of the row above, D = 0
-      pm[r] = 0; px[r] = 0; tmm[r] = 0; tgm[r] = 0;
-      tmi[r] = 1; tii[r] = 1; tmd[r] = 0; tdd[r] = 1;
-      rcode[r] = 0;
-    }
-  }
-
-  // Row -1 deletion value C/H, fed to lane 0 (pre-multiplied by row 0's tGM
-  // in the folded form).
-  T initD = Num<T>::C() / (T)H;
-  if constexpr (XF) initD = ((T)1 - ph2pr[s.rc[roff] & 127]) * initD;
-  const T old_d = initD;
-  T Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
-#pragma unroll
-  for (int r = 0; r < RPL; ++r) { Ma[r] = 0; Ia[r] = 0; Da[r] = 0; }
-  // Row above this lane's first row: at column j (u*) and column j-1 (d*).
-  T uM = 0, uI = 0, uD = (k == 0) ? initD : (T)0;
-  T dM = 0, dI = 0, dD = uD;
-  T S = 0;
-  // Two steps per iteration with ping-pong registers (a -> b -> a), so the
-  // loop carries no register copies. A step past the last column only
-  // touches columns >= H, which nothing reads, so the count is rounded up.
-  const int nsteps = (int)Hmax + W - 1;
-  const int mysteps = (int)H + W - 1;
-  const uint8_t* hp = hb - k;  // hp[s] = code of column s - k
-  for (int st = 0; st < nsteps; st += 2) {
-    const int hc0 = hp[st];
-    const int hc1 = hp[st + 1];
-    sweep_step<T, RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, hc0, rcode, pm, px, tmm, tgm, tmi, tii, tmd,
-                           tdd);
-    if (st < mysteps) S += Mb[RPL - 1] + Ib[RPL - 1];
-    const T sM = lane_down0<W>(Mb[RPL - 1], k);
-    const T sI = lane_down0<W>(Ib[RPL - 1], k);
-    const T sD = lane_down<W>(Db[RPL - 1], old_d, k);
-    sweep_step<T, RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, hc1, rcode, pm, px, tmm, tgm, tmi, tii, tmd,
-                           tdd);
-    if (st + 1 < mysteps) S += Ma[RPL - 1] + Ia[RPL - 1];
-    dM = sM; dI = sI; dD = sD;
-    uM = lane_down0<W>(Ma[RPL - 1], k);
-    uI = lane_down0<W>(Ia[RPL - 1], k);
-    uD = lane_down<W>(Da[RPL - 1], old_d, k);
-  }
-  return S;
-}
-
-// Result of one FP32 pair (output slot o): log10 likelihood, or a place on
-// the FP64 list.
-__device__ __forceinline__ void finish_fp32(const Fp32Args& a, uint32_t o, uint32_t hap, uint32_t read, float S) {
-  if (S >= 1e-28f) {
-    a.out[o] = log10((double)S) - (double)36.1236000061f;
-    a.flags[o] = 1;
-  } else {
-    const int b = (int)a.rescue_list0 + fp64_bin(a.s.read_len[read]);
-    const uint32_t idx = atomicAdd(a.rescue_count + b, 1u);
-    RescueEntry e;
-    e.hap = hap; e.read = read; e.out = o; e.pad = 0;
-    a.rescue[a.rescue_base[b] + idx] = e;
-  }
-}
-
-// FP32 pass: persistent waves pull tiles (a run of haplotypes x up to 64/W
-// reads). OCC = waves per SIMD the register allocation must allow (512 / OCC
-// VGPRs per lane): the sweep is issue-latency bound, so occupancy is a tuning
-// knob.
-template <int W, int RPL, bool XF, int OCC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void k_fp32(Fp32Args a) {
-  __shared__ uint8_t hbuf[kHapBuf];
-  const int lane = threadIdx.x;
-  const int seg = seg_of<W>(lane), k = pos_of<W>(lane);
-  for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(a.head, 1u);
-    t = __builtin_amdgcn_readfirstlane(t);
-    if (t >= a.n_tiles) break;
-    const Tile tl = a.tiles[t];
-    const bool active = seg < (int)tile_nseg(tl.meta);
-    const uint32_t stride = tile_stride(tl.meta), nh = tile_nh(tl.meta);
-    const uint32_t read = tl.read0 + (active ? (uint32_t)seg : 0u);
-    for (uint32_t h = 0; h < nh; ++h) {
-      const uint32_t hap = tl.hap + h;
-      const uint32_t H = a.s.hap_len[hap];
-      __syncthreads();
-      stage_hap(a.s, hap, H, hbuf + 64, lane, 64);
-      __syncthreads();
-      const float S = forward<float, W, RPL, XF>(a.s, a.t, hbuf + 64, k, H, H, read);
-      if (active && k == W - 1) finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, S);
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Prior-table variant of the FP32 pass, for haplotypes without N.
-//
-// The prior of a cell depends only on its read row and on the haplotype base
-// (4 values), so each tile first writes, per lane and row, the prior for each
-// of the 4 bases into LDS as ptab[base][row/4][lane] (float4 = 4 rows). A step
-// then reads the 4-row groups of its column's base with one ds_read_b128 each,
-// one step ahead, instead of a compare and a select per cell. The haplotype
-// itself sits in LDS as its 2-bit stream (16 bases per word, 64 zero columns
-// in front); a funnel shift yields the codes of 4 consecutive columns.
-// ---------------------------------------------------------------------------
-constexpr int kHapWords = 112;  // 64 pad columns + 1536 bases + tail, 16 per word
-
-template <typename T, int RPL, bool XF>
-__device__ __forceinline__ void sweep_step_pt(const T (&M)[RPL], const T (&I)[RPL], const T (&D)[RPL], T (&nM)[RPL],
-                                              T (&nI)[RPL], T (&nD)[RPL], T dM, T dI, T dD, T uM, T uI, const T* pr,
-                                              const T (&tmm)[RPL], const T (&tgm)[RPL], const T (&tmi)[RPL],
-                                              const T (&tii)[RPL], const T (&tmd)[RPL], const T (&tdd)[RPL]) {
-  if constexpr (XF) nM[0] = pr[0] * fma(tmm[0], dM, dI + dD);
-  else nM[0] = pr[0] * fma(tmm[0], dM, tgm[0] * (dI + dD));
-  nI[0] = i_update<T>(tmi[0], uM, tii[0], uI);
-  nD[0] = fma(tmd[0], M[0], tdd[0] * D[0]);
-#pragma unroll
-  for (int r = 1; r < RPL; ++r) {
-    if constexpr (XF) nM[r] = pr[r] * fma(tmm[r], M[r - 1], I[r - 1] + D[r - 1]);
-    else nM[r] = pr[r] * fma(tmm[r], M[r - 1], tgm[r] * (I[r - 1] + D[r - 1]));
-    nI[r] = i_update<T>(tmi[r], nM[r - 1], tii[r], nI[r - 1]);
-    nD[r] = fma(tmd[r], M[r], tdd[r] * D[r]);
+    cell<T, F>(pr, M[r - 1], I[r - 1], D[r - 1], nM[r - 1], nI[r - 1], M[r], D[r], tmm[r], tgm[r], tmi[r], tii[r],
+               tmd[r], tdd[r], nM[r], nI[r], nD[r]);
   }
 }
 
@@ -403,8 +258,8 @@ __device__ __forceinline__ void sweep_step_pt(const T (&M)[RPL], const T (&I)[RP
 // per row (r < NR) the match and mismatch priors and the base code (-1 for
 // N, which matches every haplotype base). Rows past the read are padding
 // rows: prior 0, I carries M + I of the row above, D stays 0.
-// (ph/roDV_inter/pairhmm_inter.cpp:60-79; folded form described above.)
-template <typename T, int RPL, int NR, bool XF>
+// (ph/roDV_inter/pairhmm_inter.cpp:60-79; the forms are described above.)
+template <typename T, int RPL, int NR, int F>
 __device__ __forceinline__ void row_params(const DevSeqs& s, const T* ph2pr, const T* mmt, uint32_t read, int k,
                                            T (&tmm)[RPL], T (&tgm)[RPL], T (&tmi)[RPL], T (&tii)[RPL],
                                            T (&tmd)[RPL], T (&tdd)[RPL], T (&pm)[NR], T (&px)[NR], int (&rc)[NR]) {
@@ -428,7 +283,17 @@ __device__ __forceinline__ void row_params(const DevSeqs& s, const T* ph2pr, con
       const int mx = ii > id ? ii : id, mn = ii > id ? id : ii;
       tmm[r] = mmt[((mx * (mx + 1)) >> 1) + mn];
       tgm[r] = (T)1 - gc;
-      if constexpr (XF) {
+      if constexpr (F == kScaled) {
+        // row -1 has I = 0 and D~ = D
+        const T aprev = row > 0 ? ph2pr[s.ri[g - 1] & 127] : (T)1;
+        const T bprev = row > 0 ? ph2pr[s.rd[g - 1] & 127] : (T)1;
+        pm[r] *= tmm[r];
+        px[r] *= tmm[r];
+        tmi[r] = tgm[r] * aprev / tmm[r];
+        tmd[r] = tgm[r] * bprev / tmm[r];
+        tii[r] = gc * aprev / ph2pr[ii];
+        tdd[r] = gc;
+      } else if constexpr (F == kFold) {
         const T gnext = (row + 1 < R) ? (T)1 - ph2pr[s.rc[g + 1] & 127] : (T)1;
         tmi[r] = gnext * ph2pr[ii];
         tii[r] = gnext * gc / tgm[r];
@@ -442,13 +307,155 @@ __device__ __forceinline__ void row_params(const DevSeqs& s, const T* ph2pr, con
       }
     } else {
       tmm[r] = 0; tgm[r] = 0; tmi[r] = 1; tii[r] = 1; tmd[r] = 0; tdd[r] = 1;
+      if constexpr (F == kScaled) {
+        // the first padding row holds M + I of the read's last row: its I~
+        // (scale 1) takes that row's I~ times the row's tMI
+        tmi[r] = 0;
+        tii[r] = row == R ? ph2pr[s.ri[roff + R - 1] & 127] : (T)1;
+      }
     }
   }
+}
+
+// Forward sweep of one pair per segment. hb points at the segment's staged
+// haplotype codes (hb[j] = code of base j). Returns the raw sum S in the
+// segment's last lane; H is this segment's haplotype length, Hmax the
+// largest among the wave's segments.
+//
+// The folded form (kFold) stores I'[k] = g[k+1]*I[k] and D'[k] = g[k+1]*D[k]
+// with g the gap-to-match probability tGM of a row (1 past the last row). Then
+//   M[k+1] = prior * (tMM*M[k] + I'[k] + D'[k])
+//   I'[k]  = (g[k+1]*tMI)*M[k-1] + (g[k+1]*tII/g[k])*I'[k-1]
+//   D'[k]  = (g[k+1]*tMD)*M[k][j-1] + tDD*D'[k][j-1]
+// which is the same recurrence with folded coefficients. It needs g > 0 for
+// every row (gap-continuation quality not 0); the host checks that per batch.
+// c0 is the row -1 scale (2^120 in float, 2^1020 in double, lower in the
+// scaled form).
+template <typename T, int W, int RPL, int F>
+__device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, const uint8_t* hb, int k, uint32_t H,
+                                     uint32_t Hmax, uint32_t read, T c0) {
+  const T* ph2pr = ph2pr_tab<T>(tb);
+  const T* mmt = mm_tab<T>(tb);
+  const uint32_t roff = s.read_off[read];
+
+  T pm[RPL], px[RPL], tmm[RPL], tgm[RPL], tmi[RPL], tii[RPL], tmd[RPL], tdd[RPL];
+  int rcode[RPL];
+  row_params<T, RPL, RPL, F>(s, ph2pr, mmt, read, k, tmm, tgm, tmi, tii, tmd, tdd, pm, px, rcode);
+
+  // Row -1 deletion value c0/H, fed to lane 0 (pre-multiplied by row 0's tGM
+  // in the folded form).
+  T initD = c0 / (T)H;
+  if constexpr (F == kFold) initD = ((T)1 - ph2pr[s.rc[roff] & 127]) * initD;
+  const T old_d = initD;
+  T Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) { Ma[r] = 0; Ia[r] = 0; Da[r] = 0; }
+  // Row above this lane's first row: at column j (u*) and column j-1 (d*).
+  T uM = 0, uI = 0, uD = (k == 0) ? initD : (T)0;
+  T dM = 0, dI = 0, dD = uD;
+  T S = 0;
+  // Two steps per iteration with ping-pong registers (a -> b -> a), so the
+  // loop carries no register copies. A step past the last column only
+  // touches columns >= H, which nothing reads, so the count is rounded up.
+  const int nsteps = (int)Hmax + W - 1;
+  const int mysteps = (int)H + W - 1;
+  const uint8_t* hp = hb - k;  // hp[s] = code of column s - k
+  for (int st = 0; st < nsteps; st += 2) {
+    const int hc0 = hp[st];
+    const int hc1 = hp[st + 1];
+    sweep_step<T, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, hc0, rcode, pm, px, tmm, tgm, tmi, tii, tmd,
+                          tdd);
+    if (st < mysteps) S += Mb[RPL - 1] + Ib[RPL - 1];
+    const T sM = lane_down0<W>(Mb[RPL - 1], k);
+    const T sI = lane_down0<W>(Ib[RPL - 1], k);
+    const T sD = lane_down<W>(Db[RPL - 1], old_d, k);
+    sweep_step<T, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, hc1, rcode, pm, px, tmm, tgm, tmi, tii, tmd,
+                          tdd);
+    if (st + 1 < mysteps) S += Ma[RPL - 1] + Ia[RPL - 1];
+    dM = sM; dI = sI; dD = sD;
+    uM = lane_down0<W>(Ma[RPL - 1], k);
+    uI = lane_down0<W>(Ia[RPL - 1], k);
+    uD = lane_down<W>(Da[RPL - 1], old_d, k);
+  }
+  return S;
+}
+
+// Result of one FP32 pair (output slot o): log10 likelihood, or a place on
+// the FP64 list.
+__device__ __forceinline__ void finish_fp32(const Fp32Args& a, uint32_t o, uint32_t hap, uint32_t read, float S) {
+  if (S >= 1e-28f && S <= 0x1.fffffep127f) {
+    a.out[o] = log10((double)S) - (double)36.1236000061f;
+    a.flags[o] = 1;
+  } else {
+    const int b = (int)a.rescue_list0 + fp64_bin(a.s.read_len[read]);
+    const uint32_t idx = atomicAdd(a.rescue_count + b, 1u);
+    RescueEntry e;
+    e.hap = hap; e.read = read; e.out = o; e.pad = 0;
+    a.rescue[a.rescue_base[b] + idx] = e;
+  }
+}
+
+// FP32 pass: persistent waves pull tiles (a run of haplotypes x up to 64/W
+// reads). OCC = waves per SIMD the register allocation must allow (512 / OCC
+// VGPRs per lane): the sweep is issue-latency bound, so occupancy is a tuning
+// knob. XF selects the scaled form, else the plain one.
+template <int W, int RPL, bool XF, int OCC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void k_fp32(Fp32Args a) {
+  constexpr int F = XF ? kScaled : kPlain;
+  __shared__ uint8_t hbuf[kHapBuf];
+  const int lane = threadIdx.x;
+  const int seg = seg_of<W>(lane), k = pos_of<W>(lane);
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(a.head, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= a.n_tiles) break;
+    const Tile tl = a.tiles[t];
+    const bool active = seg < (int)tile_nseg(tl.meta);
+    const uint32_t stride = tile_stride(tl.meta), nh = tile_nh(tl.meta);
+    const uint32_t read = tl.read0 + (active ? (uint32_t)seg : 0u);
+    for (uint32_t h = 0; h < nh; ++h) {
+      const uint32_t hap = tl.hap + h;
+      const uint32_t H = a.s.hap_len[hap];
+      __syncthreads();
+      stage_hap(a.s, hap, H, hbuf + 64, lane, 64);
+      __syncthreads();
+      const float S = forward<float, W, RPL, F>(a.s, a.t, hbuf + 64, k, H, H, read, a.init_scale);
+      if (active && k == W - 1) finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, S * a.back_scale);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Prior-table variant of the FP32 pass, for haplotypes without N.
+//
+// The prior of a cell depends only on its read row and on the haplotype base
+// (4 values), so each tile first writes, per lane and row, the prior for each
+// of the 4 bases into LDS as ptab[base][row/4][lane] (float4 = 4 rows). A step
+// then reads the 4-row groups of its column's base with one ds_read_b128 each,
+// one step ahead, instead of a compare and a select per cell. The haplotype
+// itself sits in LDS as its 2-bit stream (16 bases per word, 64 zero columns
+// in front); a funnel shift yields the codes of 4 consecutive columns.
+// ---------------------------------------------------------------------------
+constexpr int kHapWords = 112;  // 64 pad columns + 1536 bases + tail, 16 per word
+
+template <typename T, int RPL, int F>
+__device__ __forceinline__ void sweep_step_pt(const T (&M)[RPL], const T (&I)[RPL], const T (&D)[RPL], T (&nM)[RPL],
+                                              T (&nI)[RPL], T (&nD)[RPL], T dM, T dI, T dD, T uM, T uI, const T* pr,
+                                              const T (&tmm)[RPL], const T (&tgm)[RPL], const T (&tmi)[RPL],
+                                              const T (&tii)[RPL], const T (&tmd)[RPL], const T (&tdd)[RPL]) {
+  cell<T, F>(pr[0], dM, dI, dD, uM, uI, M[0], D[0], tmm[0], tgm[0], tmi[0], tii[0], tmd[0], tdd[0], nM[0], nI[0],
+             nD[0]);
+#pragma unroll
+  for (int r = 1; r < RPL; ++r)
+    cell<T, F>(pr[r], M[r - 1], I[r - 1], D[r - 1], nM[r - 1], nI[r - 1], M[r], D[r], tmm[r], tgm[r], tmi[r], tii[r],
+               tmd[r], tdd[r], nM[r], nI[r], nD[r]);
 }
 
 template <int W, int RPL, bool XF, int OCC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void k_fp32pt(Fp32Args a) {
   constexpr int NQ = (RPL + 3) / 4;
+  constexpr int F = XF ? kScaled : kPlain;
   __shared__ float4 ptab[4 * NQ * 64];
   __shared__ uint32_t hpk[kHapWords];
   const int lane = threadIdx.x;
@@ -467,11 +474,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     __syncthreads();
     // per-row transition parameters (registers) and priors (LDS table), once
     // for all haplotypes of the tile
-    const uint32_t roff = a.s.read_off[read];
     float tmm[RPL], tgm[RPL], tmi[RPL], tii[RPL], tmd[RPL], tdd[RPL];
     float pm[NQ * 4], px[NQ * 4];
     int rc[NQ * 4];
-    row_params<float, RPL, NQ * 4, XF>(a.s, ph2pr, mmt, read, k, tmm, tgm, tmi, tii, tmd, tdd, pm, px, rc);
+    row_params<float, RPL, NQ * 4, F>(a.s, ph2pr, mmt, read, k, tmm, tgm, tmi, tii, tmd, tdd, pm, px, rc);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
@@ -483,7 +489,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
         v.w = (rc[4 * q + 3] < 0 || rc[4 * q + 3] == c) ? pm[4 * q + 3] : px[4 * q + 3];
         ptab[(c * NQ + q) * 64 + lane] = v;
       }
-    const float g0 = XF ? 1.f - ph2pr[a.s.rc[roff] & 127] : 1.f;
     // this lane's words x = lane, lane + 64 of a haplotype's stream (64
     // zero columns in front); the next haplotype's are loaded while the
     // current one is swept
@@ -508,8 +513,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
       __syncthreads();
       if (h + 1 < nh) hap_words(hap + 1, words);
 
-      float initD = 0x1p120f / (float)H;
-      if constexpr (XF) initD = g0 * initD;
+      const float initD = a.init_scale / (float)H;
       const float old_d = initD;
       float Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
 #pragma unroll
@@ -548,21 +552,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
         const uint32_t cw_next = window(st + 4);
         load(cw, 1, pB);
         __builtin_amdgcn_sched_barrier(0);
-        sweep_step_pt<float, RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
+        sweep_step_pt<float, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
         S += Ib[RPL - 1];
         const float s1M = lane_down0<W>(Mb[RPL - 1], k);
         const float s1I = lane_down0<W>(Ib[RPL - 1], k);
         s1D = lane_down_keep<W>(Db[RPL - 1], s1D, old_d, k);
         load(cw, 2, pA);
         __builtin_amdgcn_sched_barrier(0);
-        sweep_step_pt<float, RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, pB, tmm, tgm, tmi, tii, tmd, tdd);
+        sweep_step_pt<float, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, pB, tmm, tgm, tmi, tii, tmd, tdd);
         if (st + 1 < nsteps) S += Ia[RPL - 1];
         const float s2M = lane_down0<W>(Ma[RPL - 1], k);
         const float s2I = lane_down0<W>(Ia[RPL - 1], k);
         s2D = lane_down_keep<W>(Da[RPL - 1], s2D, old_d, k);
         load(cw, 3, pB);
         __builtin_amdgcn_sched_barrier(0);
-        sweep_step_pt<float, RPL, XF>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, pA, tmm, tgm, tmi, tii, tmd, tdd);
+        sweep_step_pt<float, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, pA, tmm, tgm, tmi, tii, tmd, tdd);
         if (st + 2 < nsteps) S += Ib[RPL - 1];
         const float s3M = lane_down0<W>(Mb[RPL - 1], k);
         const float s3I = lane_down0<W>(Ib[RPL - 1], k);
@@ -570,14 +574,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
         cw = cw_next;
         load(cw, 0, pA);
         __builtin_amdgcn_sched_barrier(0);
-        sweep_step_pt<float, RPL, XF>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, pB, tmm, tgm, tmi, tii, tmd, tdd);
+        sweep_step_pt<float, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, pB, tmm, tgm, tmi, tii, tmd, tdd);
         if (st + 3 < nsteps) S += Ia[RPL - 1];
         dM = s3M; dI = s3I; dD = s3D;
         uM = lane_down0<W>(Ma[RPL - 1], k);
         uI = lane_down0<W>(Ia[RPL - 1], k);
         uD = lane_down_keep<W>(Da[RPL - 1], uD, old_d, k);
       }
-      if (active && k == W - 1) finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, S);
+      if (active && k == W - 1)
+        finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, S * a.back_scale);
     }
   }
 }
@@ -624,7 +629,7 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
     double tmm[RPL], tgm[RPL], tmi[RPL], tii[RPL], tmd[RPL], tdd[RPL];
     double pm[NP * 2], px[NP * 2];
     int rc[NP * 2];
-    row_params<double, RPL, NP * 2, true>(a.s, ph2pr, mmt, en.read, k, tmm, tgm, tmi, tii, tmd, tdd, pm, px, rc);
+    row_params<double, RPL, NP * 2, kFold>(a.s, ph2pr, mmt, en.read, k, tmm, tgm, tmi, tii, tmd, tdd, pm, px, rc);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
@@ -669,7 +674,7 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
     for (int st = 0; st < nsteps; st += 2) {
       load(cw, (st + 1) & 3, pB);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<double, RPL, true>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
+      sweep_step_pt<double, RPL, kFold>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
       if (st < mysteps) S += Mb[RPL - 1] + Ib[RPL - 1];
       const double sM = lane_down0<W>(Mb[RPL - 1], k);
       const double sI = lane_down0<W>(Ib[RPL - 1], k);
@@ -677,7 +682,7 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
       if (((st + 2) & 3) == 0) cw = codes_at(st + 2);
       load(cw, (st + 2) & 3, pA);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<double, RPL, true>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, pB, tmm, tgm, tmi, tii, tmd, tdd);
+      sweep_step_pt<double, RPL, kFold>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, pB, tmm, tgm, tmi, tii, tmd, tdd);
       if (st + 1 < mysteps) S += Ma[RPL - 1] + Ia[RPL - 1];
       dM = sM; dI = sI; dD = sD;
       uM = lane_down0<W>(Ma[RPL - 1], k);
@@ -717,7 +722,7 @@ __global__ __launch_bounds__(64) void k_fp64(Fp64Args a) {
     __syncthreads();
     stage_hap(a.s, en.hap, H, &hbuf[seg][64], k, W);
     __syncthreads();
-    const double S = forward<double, W, RPL, false>(a.s, a.t, &hbuf[seg][64], k, H, Hmax, en.read);
+    const double S = forward<double, W, RPL, kPlain>(a.s, a.t, &hbuf[seg][64], k, H, Hmax, en.read, Num<double>::C());
     if (active && k == W - 1) {
       a.out[en.out] = log10(S) - 307.050595577260822;
       a.flags[en.out] = 3;
@@ -783,7 +788,7 @@ constexpr auto fp32pt_row(std::integer_sequence<int, R...>) {
 
 // FP32 variants: W = 16 with 1..16 rows per lane at occupancy targets 2, 3
 // and 4 waves per SIMD; W = 32 (1..16) and W = 64 (1..12) at 2 waves; each
-// with and without the folded gap-to-match factor.
+// in the plain and in the scaled form.
 using Row16 = std::array<Fp32Fn, 16>;
 const Row16 kFp32W16[2][3] = {
     {fp32_row<16, false, 2>(std::make_integer_sequence<int, 16>{}),
@@ -803,7 +808,7 @@ const std::array<Fp64Fn, kNumFp64Bins> kFp64 = {{&k_fp64<16, 4>, &k_fp64<16, 8>,
 const std::array<Fp64Fn, kNumFp64Bins> kFp64Pt = {
     {&k_fp64pt<16, 4>, &k_fp64pt<16, 8>, &k_fp64pt<32, 8>, &k_fp64pt<64, 12>}};
 
-// Prior-table variants (folded form only): W = 8, 16 and 32 at 2, 3, 4 waves
+// Prior-table variants (scaled form only): W = 8, 16 and 32 at 2, 3, 4 waves
 // per SIMD, W = 64 at 2.
 const Row16 kFp32PtW16[3] = {fp32pt_row<16, 2>(std::make_integer_sequence<int, 16>{}),
                              fp32pt_row<16, 3>(std::make_integer_sequence<int, 16>{}),

@@ -132,6 +132,59 @@ def test_rescue_both_fp64_kernels():
     b.close()
 
 
+def _gap_region(seed, gop_lo, gop_hi, gcp, n_haps=5, n_reads=6):
+    """Reads whose gap-open qualities alternate between gop_lo and gop_hi row by
+    row (the largest step between rows), gap continuation gcp."""
+    rng = np.random.default_rng(seed)
+    haps = [workload.ACGT[rng.integers(0, 4, int(rng.integers(1, 160)))] for _ in range(n_haps)]
+    haps[0] = haps[0][:1]
+    reads, quals, gi, gd, gc = [], [], [], [], []
+    for x in range(n_reads):
+        h = haps[1 + x % (n_haps - 1)]
+        L = int(rng.integers(1, min(len(h), 120) + 1))
+        r = h[:L].copy()
+        r[rng.random(L) < 0.05] = workload.ACGT[rng.integers(0, 4)]
+        reads.append(r)
+        quals.append(rng.integers(10, 41, L).astype(np.uint8))
+        alt = np.where(np.arange(L) % 2 == 0, gop_lo, gop_hi).astype(np.uint8)
+        gi.append(alt)
+        gd.append(alt[::-1].copy())
+        gc.append(np.full(L, gcp, np.uint8))
+    reg = workload.make_region(haps, reads, quals)
+    reg["i"] = np.append(np.concatenate(gi), 0).astype(np.uint8)
+    reg["d"] = np.append(np.concatenate(gd), 0).astype(np.uint8)
+    reg["c"] = np.append(np.concatenate(gc), 0).astype(np.uint8)
+    return reg
+
+
+# (gap open low, high, gap continuation, form the FP32 pass must use): the
+# scaled form takes gap qualities >= 6 with a gap-open spread up to 48
+@pytest.mark.parametrize("lo,hi,gcp,form", [(45, 45, 10, "scaled"), (10, 45, 10, "scaled"), (6, 54, 6, "scaled"),
+                                            (6, 55, 6, "plain"), (5, 45, 10, "plain"), (10, 45, 5, "plain"),
+                                            (10, 45, 0, "plain"), (30, 78, 20, "scaled")])
+def test_gap_quality_forms(lo, hi, gcp, form):
+    say(f"gap qualities {lo}..{hi}, continuation {gcp}: {form} form")
+    reg = _gap_region(lo * 131 + hi * 7 + gcp, lo, hi, gcp)
+    b = rv.Batch([reg])
+    b.run()
+    out, flags = b.results()
+    names = {k["name"] for k in b.kernels() if k["precision"] == 32}
+    assert names and all(f",{form}," in n for n in names), names
+    got = b.region_result(out, 0)
+    b.close()
+    want, route, ok = oracle.region_reference(reg)
+    assert ok
+    m = ~np.isnan(want)
+    assert np.array_equal(m, ~np.isnan(got))
+    # gap continuation 0 leaves no way back from a gap: equal infinities
+    same = got[m] == want[m]
+    assert same.any() or np.isfinite(want[m]).all()
+    assert np.all(same | (np.abs(got[m] - want[m]) <= TOL))
+    # alone through the synchronous path: the same form, bit for bit
+    alone, _ = rv.region_sync(reg)
+    assert np.array_equal(alone[m], got[m])
+
+
 @pytest.mark.parametrize("R,H", [(1, 1), (1, 40), (40, 1), (16, 16), (17, 300), (100, 350), (112, 300), (127, 200),
                                  (128, 200), (255, 200), (256, 256), (257, 100), (300, 700), (512, 512), (513, 90),
                                  (700, 1536), (767, 900), (768, 1536)])
