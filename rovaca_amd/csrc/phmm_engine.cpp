@@ -393,7 +393,6 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
         P.gop_lo = lo;
         P.gop_hi = hi;
         P.gcp_lo = clo;
-        if (clo == 0) P.fold_gap = false;
       }
       rcur += align_up(len, kSeqAlign);
     }
@@ -453,7 +452,7 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
     append_largest_first(bins[v], bin_cost[v], P.tiles);
   }
   P.bin_start[kFp32Bins] = (uint32_t)P.tiles.size();
-  if (!pol.fold) P.fold_gap = false;
+  P.rescale = pol.fold;
   P.fp64_base.assign(kNumFp64Lists + 1, 0);
   for (int b = 0; b < kNumFp64Lists; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
   return RVC_OK;
@@ -465,7 +464,7 @@ int scaled_exponent(const HostPack& P) {
   // probability, 10^(spread/10) (at most 2^16 for a spread of 48): X lowers
   // 2^120 by that many bits plus 2, so that no cell holds more than in the
   // 2^120 scale of the other forms.
-  if (!P.fold_gap) return 0;
+  if (!P.rescale) return 0;
   if (P.gop_hi < P.gop_lo) return 118;  // no read rows
   const int spread = P.gop_hi - P.gop_lo;
   if (P.gop_lo < 6 || P.gcp_lo < 6 || spread > 48) return 0;
@@ -677,7 +676,7 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
     }
     P.n_pairs += A.n_pairs;
     P.n_cells += A.n_cells;
-    P.fold_gap = P.fold_gap && A.fold_gap;
+    P.rescale = P.rescale && A.rescale;
     P.gop_lo = std::min(P.gop_lo, A.gop_lo);
     P.gop_hi = std::max(P.gop_hi, A.gop_hi);
     P.gcp_lo = std::min(P.gcp_lo, A.gcp_lo);
@@ -918,7 +917,7 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
   for (int l = 0; l < kNumFp64Lists; ++l) {
     if (P.fp64_cap[l] == 0) continue;
     const int b = l % kNumFp64Bins;
-    const bool pt = l >= kNumFp64Bins && P.prior_table && P.fold_gap;
+    const bool pt = l >= kNumFp64Bins && P.prior_table && sc;
     Fp64Args a;
     a.s = seqs;
     a.t = g_engine.tables;
@@ -927,16 +926,19 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     a.head = heads + kFp32Bins + l;
     a.out = out;
     a.flags = flags;
-    const int per_cu = std::max(1, fp64_blocks_per_cu(b, pt));
+    // the FP64 scale 2^1020 lowered like the FP32 one
+    a.init_scale = sc ? std::ldexp(1.0, X + 900) : 0x1p1020;
+    a.back_scale = sc ? std::ldexp(1.0, 120 - X) : 1.0;
+    const int per_cu = std::max(1, fp64_blocks_per_cu(b, pt, sc));
     int W, RPL;
     fp64_bin_shape(b, &W, &RPL);
     const uint64_t max_waves = (P.fp64_cap[l] + (64 / W) - 1) / (64 / W);
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(max_waves, (uint64_t)per_cu * g_engine.n_cu));
-    hipError_t e = launch_fp64(b, pt, grid, s, a);
+    hipError_t e = launch_fp64(b, pt, sc, grid, s, a);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP64 launch list %d: %s", l, hipGetErrorString(e));
     ++n_fp64_launches;
     rvc_kernel_time kt{};
-    snprintf(kt.name, sizeof kt.name, "k_fp64%s<%d,%d>", pt ? "pt" : "", W, RPL);
+    snprintf(kt.name, sizeof kt.name, "k_fp64%s<%d,%d,%s>", pt ? "pt" : "", W, RPL, sc ? "scaled" : "plain");
     kt.precision = 64; kt.width = W; kt.rows_per_lane = RPL; kt.occupancy = 0;
     kt.tiles = P.fp64_cap[l];
     launches.push_back(kt);

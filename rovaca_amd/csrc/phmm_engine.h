@@ -46,7 +46,7 @@ constexpr int kFp32Bins = 2 * kFp32Variants;
 int fp32_variant_index(int W, int RPL);
 void fp32_variant_shape(int v, int* W, int* RPL);
 int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ, bool pt);
-int fp64_blocks_per_cu(int bin, bool pt);
+int fp64_blocks_per_cu(int bin, bool pt, bool scaled);
 
 // Chooses the (W, RPL) kernel for a tile whose longest read has R bases.
 // rpl16_override[x] (for reads up to 255 bases, indexed by the minimal
@@ -58,7 +58,7 @@ struct FP32Policy {
   // waves per SIMD the 16-lane kernel of each RPL is compiled for (2, 3, 4)
   uint8_t occ16[17] = {4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 2, 2, 2, 2, 2, 2};
   bool rpl_steps_of_3 = true;  // 16-lane shapes only in RPL 1, 2, 4, 7, 10, 13, 16
-  bool fold = true;            // rescaled forms where the gap qualities allow
+  bool fold = true;            // scaled forms where the gap qualities allow
   uint32_t waves_per_cu = 0;   // persistent grid: 0 = occupancy limit
   // the same for the prior-table kernels (W = 16 and W = 32)
   uint8_t occpt16[17] = {4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 2, 2, 2, 2, 2, 2};
@@ -112,7 +112,7 @@ struct HostPack {
   uint64_t n_pairs_slots = 0;       // sum of n_haps * n_reads
   uint64_t n_pairs = 0;             // pairs with two non-empty sides
   uint64_t n_cells = 0;
-  bool fold_gap = true;             // every gap-continuation quality is > 0
+  bool rescale = true;              // the policy allows the scaled forms
   uint8_t gop_lo = 127, gop_hi = 0;  // range of the gap-open qualities (insertion and deletion)
   uint8_t gcp_lo = 127;              // smallest gap-continuation quality
   uint32_t waves_per_cu = 0;

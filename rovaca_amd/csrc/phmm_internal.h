@@ -117,6 +117,8 @@ struct Fp64Args {
   uint32_t* head;            // this launch's work-queue counter
   double* out;
   uint8_t* flags;
+  double init_scale;         // row -1 deletion value times H: 2^1020, or 2^X in the scaled form
+  double back_scale;         // 2^(1020 - X)
 };
 
 // Device read preprocessing (for regions handed over with raw qualities):
@@ -144,8 +146,8 @@ bool fp32_supported(int W, int RPL);
 hipError_t launch_fp32(int W, int RPL, bool scaled, int occ, bool pt, int grid, hipStream_t stream,
                        const Fp32Args& a);
 // FP64 rescue of one list's bin; pt = the prior-table kernel (pairs without
-// N in the haplotype, folded form).
-hipError_t launch_fp64(int bin, bool pt, int grid, hipStream_t stream, const Fp64Args& a);
+// N in the haplotype, scaled form only), scaled = the scaled form.
+hipError_t launch_fp64(int bin, bool pt, bool scaled, int grid, hipStream_t stream, const Fp64Args& a);
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t stream);
 

@@ -195,10 +195,9 @@ __device__ __forceinline__ double i_update<double>(double tmi, double um, double
 }
 
 // Forms of the recurrence a sweep runs (template argument F).
-//  kPlain  the model's M, I and D:
+//  kPlain  the model's M, I and D, 8 operations per cell:
 //            M = prior * (tMM*M' + tGM*(I' + D'))
-//  kFold   I and D stored pre-multiplied by the next row's tGM (described at
-//          forward()): M = prior * (tMM*M' + I' + D'), 7 operations per cell.
+//            I = tMI*M_up + tII*I_up      D = tMD*M_left + tDD*D_left
 //  kScaled I and D stored divided by their own row's gap-open probability,
 //          I~ = I / tMI[r] and D~ = D / tMD[r] (row -1: D~ = D), M unchanged:
 //            M  = (prior*tMM) * (M' + rho*I~' + delta*D~')
@@ -210,8 +209,9 @@ __device__ __forceinline__ double i_update<double>(double tmi, double um, double
 //          up to the ratio of the largest to the smallest gap-open
 //          probability in the batch, so the host lowers the row -1 scale
 //          2^120 by that ratio (Fp32Args::init_scale) and the sum is scaled
-//          back before the rescue test.
-constexpr int kPlain = 0, kFold = 1, kScaled = 2;
+//          back before the rescue test. Needs tMM > 0 and tMI, tMD > 0,
+//          which the host's range check on the gap qualities ensures.
+constexpr int kPlain = 0, kScaled = 1;
 
 // New M, I, D of one cell from the row above at column j-1 (d*), the row
 // above at column j (u*) and the same row at column j-1 (l*).
@@ -223,8 +223,7 @@ __device__ __forceinline__ void cell(T pr, T dm, T di, T dd, T um, T ui, T lm, T
     ni = fma(tii, ui, um);
     nd = fma(tdd, ld, lm);
   } else {
-    if constexpr (F == kFold) nm = pr * fma(tmm, dm, di + dd);
-    else nm = pr * fma(tmm, dm, tgm * (di + dd));
+    nm = pr * fma(tmm, dm, tgm * (di + dd));
     ni = i_update<T>(tmi, um, tii, ui);
     nd = fma(tmd, lm, tdd * ld);
   }
@@ -293,12 +292,6 @@ __device__ __forceinline__ void row_params(const DevSeqs& s, const T* ph2pr, con
         tmd[r] = tgm[r] * bprev / tmm[r];
         tii[r] = gc * aprev / ph2pr[ii];
         tdd[r] = gc;
-      } else if constexpr (F == kFold) {
-        const T gnext = (row + 1 < R) ? (T)1 - ph2pr[s.rc[g + 1] & 127] : (T)1;
-        tmi[r] = gnext * ph2pr[ii];
-        tii[r] = gnext * gc / tgm[r];
-        tmd[r] = gnext * ph2pr[id];
-        tdd[r] = gc;
       } else {
         tmi[r] = ph2pr[ii];
         tii[r] = gc;
@@ -317,36 +310,35 @@ __device__ __forceinline__ void row_params(const DevSeqs& s, const T* ph2pr, con
   }
 }
 
+// S sums M + f*I of the segment's last row: f = 1, or in the scaled form
+// tMI of that row when it is the read's last row (R = W * RPL, no padding
+// row; the FP64 bins allow that), since there I holds I~ = I / tMI.
+template <typename T, int W, int RPL, int F>
+__device__ __forceinline__ T sum_factor(const DevSeqs& s, const T* ph2pr, uint32_t read) {
+  const uint32_t R = s.read_len[read];
+  if (F != kScaled || R < (uint32_t)(W * RPL)) return (T)1;
+  return ph2pr[s.ri[s.read_off[read] + R - 1] & 127];
+}
+
 // Forward sweep of one pair per segment. hb points at the segment's staged
 // haplotype codes (hb[j] = code of base j). Returns the raw sum S in the
 // segment's last lane; H is this segment's haplotype length, Hmax the
-// largest among the wave's segments.
-//
-// The folded form (kFold) stores I'[k] = g[k+1]*I[k] and D'[k] = g[k+1]*D[k]
-// with g the gap-to-match probability tGM of a row (1 past the last row). Then
-//   M[k+1] = prior * (tMM*M[k] + I'[k] + D'[k])
-//   I'[k]  = (g[k+1]*tMI)*M[k-1] + (g[k+1]*tII/g[k])*I'[k-1]
-//   D'[k]  = (g[k+1]*tMD)*M[k][j-1] + tDD*D'[k][j-1]
-// which is the same recurrence with folded coefficients. It needs g > 0 for
-// every row (gap-continuation quality not 0); the host checks that per batch.
-// c0 is the row -1 scale (2^120 in float, 2^1020 in double, lower in the
-// scaled form).
+// largest among the wave's segments. c0 is the row -1 scale (2^120 in float,
+// 2^1020 in double, lower in the scaled form).
 template <typename T, int W, int RPL, int F>
 __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, const uint8_t* hb, int k, uint32_t H,
                                      uint32_t Hmax, uint32_t read, T c0) {
   const T* ph2pr = ph2pr_tab<T>(tb);
   const T* mmt = mm_tab<T>(tb);
-  const uint32_t roff = s.read_off[read];
 
   T pm[RPL], px[RPL], tmm[RPL], tgm[RPL], tmi[RPL], tii[RPL], tmd[RPL], tdd[RPL];
   int rcode[RPL];
   row_params<T, RPL, RPL, F>(s, ph2pr, mmt, read, k, tmm, tgm, tmi, tii, tmd, tdd, pm, px, rcode);
 
-  // Row -1 deletion value c0/H, fed to lane 0 (pre-multiplied by row 0's tGM
-  // in the folded form).
-  T initD = c0 / (T)H;
-  if constexpr (F == kFold) initD = ((T)1 - ph2pr[s.rc[roff] & 127]) * initD;
+  // Row -1 deletion value c0/H, fed to lane 0.
+  const T initD = c0 / (T)H;
   const T old_d = initD;
+  const T sfac = sum_factor<T, W, RPL, F>(s, ph2pr, read);
   T Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
 #pragma unroll
   for (int r = 0; r < RPL; ++r) { Ma[r] = 0; Ia[r] = 0; Da[r] = 0; }
@@ -365,13 +357,13 @@ __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, cons
     const int hc1 = hp[st + 1];
     sweep_step<T, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, hc0, rcode, pm, px, tmm, tgm, tmi, tii, tmd,
                           tdd);
-    if (st < mysteps) S += Mb[RPL - 1] + Ib[RPL - 1];
+    if (st < mysteps) S += fma(sfac, Ib[RPL - 1], Mb[RPL - 1]);
     const T sM = lane_down0<W>(Mb[RPL - 1], k);
     const T sI = lane_down0<W>(Ib[RPL - 1], k);
     const T sD = lane_down<W>(Db[RPL - 1], old_d, k);
     sweep_step<T, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, hc1, rcode, pm, px, tmm, tgm, tmi, tii, tmd,
                           tdd);
-    if (st + 1 < mysteps) S += Ma[RPL - 1] + Ia[RPL - 1];
+    if (st + 1 < mysteps) S += fma(sfac, Ia[RPL - 1], Ma[RPL - 1]);
     dM = sM; dI = sI; dD = sD;
     uM = lane_down0<W>(Ma[RPL - 1], k);
     uI = lane_down0<W>(Ia[RPL - 1], k);
@@ -588,7 +580,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
 }
 
 // FP64 rescue with the prior table, for pairs whose haplotype has no N, in
-// the folded form: the prior-table sweep in double (2 rows per ds_read_b128).
+// the scaled form: the prior-table sweep in double (2 rows per ds_read_b128).
 // The segments of a wave hold different pairs, so each segment stages its own
 // haplotype stream and the wave sweeps to the longest of them; a segment adds
 // to its sum only while its own columns last.
@@ -629,7 +621,7 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
     double tmm[RPL], tgm[RPL], tmi[RPL], tii[RPL], tmd[RPL], tdd[RPL];
     double pm[NP * 2], px[NP * 2];
     int rc[NP * 2];
-    row_params<double, RPL, NP * 2, kFold>(a.s, ph2pr, mmt, en.read, k, tmm, tgm, tmi, tii, tmd, tdd, pm, px, rc);
+    row_params<double, RPL, NP * 2, kScaled>(a.s, ph2pr, mmt, en.read, k, tmm, tgm, tmi, tii, tmd, tdd, pm, px, rc);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
@@ -641,8 +633,8 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
       }
     __syncthreads();
 
-    const uint32_t roff = a.s.read_off[en.read];
-    const double initD = (1.0 - ph2pr[a.s.rc[roff] & 127]) * (0x1p1020 / (double)H);
+    const double initD = a.init_scale / (double)H;
+    const double sfac = sum_factor<double, W, RPL, kScaled>(a.s, ph2pr, en.read);
     double Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
 #pragma unroll
     for (int r = 0; r < RPL; ++r) { Ma[r] = 0; Ia[r] = 0; Da[r] = 0; }
@@ -674,30 +666,31 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
     for (int st = 0; st < nsteps; st += 2) {
       load(cw, (st + 1) & 3, pB);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<double, RPL, kFold>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
-      if (st < mysteps) S += Mb[RPL - 1] + Ib[RPL - 1];
+      sweep_step_pt<double, RPL, kScaled>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
+      if (st < mysteps) S += fma(sfac, Ib[RPL - 1], Mb[RPL - 1]);
       const double sM = lane_down0<W>(Mb[RPL - 1], k);
       const double sI = lane_down0<W>(Ib[RPL - 1], k);
       sD = lane_down_keep<W>(Db[RPL - 1], sD, initD, k);
       if (((st + 2) & 3) == 0) cw = codes_at(st + 2);
       load(cw, (st + 2) & 3, pA);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<double, RPL, kFold>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, pB, tmm, tgm, tmi, tii, tmd, tdd);
-      if (st + 1 < mysteps) S += Ma[RPL - 1] + Ia[RPL - 1];
+      sweep_step_pt<double, RPL, kScaled>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, pB, tmm, tgm, tmi, tii, tmd, tdd);
+      if (st + 1 < mysteps) S += fma(sfac, Ia[RPL - 1], Ma[RPL - 1]);
       dM = sM; dI = sI; dD = sD;
       uM = lane_down0<W>(Ma[RPL - 1], k);
       uI = lane_down0<W>(Ia[RPL - 1], k);
       uD = lane_down_keep<W>(Da[RPL - 1], uD, initD, k);
     }
     if (active && k == W - 1) {
-      a.out[en.out] = log10(S) - 307.050595577260822;
+      a.out[en.out] = log10(S * a.back_scale) - 307.050595577260822;
       a.flags[en.out] = 3;
     }
   }
 }
 
-// FP64 rescue: persistent waves pull 64/W queued pairs at a time.
-template <int W, int RPL>
+// FP64 rescue: persistent waves pull 64/W queued pairs at a time. SC selects
+// the scaled form, else the plain one.
+template <int W, int RPL, bool SC>
 __global__ __launch_bounds__(64) void k_fp64(Fp64Args a) {
   constexpr int NSEG = 64 / W;
   __shared__ uint8_t hbuf[NSEG][kHapBuf];
@@ -722,9 +715,10 @@ __global__ __launch_bounds__(64) void k_fp64(Fp64Args a) {
     __syncthreads();
     stage_hap(a.s, en.hap, H, &hbuf[seg][64], k, W);
     __syncthreads();
-    const double S = forward<double, W, RPL, kPlain>(a.s, a.t, &hbuf[seg][64], k, H, Hmax, en.read, Num<double>::C());
+    const double S = forward<double, W, RPL, SC ? kScaled : kPlain>(a.s, a.t, &hbuf[seg][64], k, H, Hmax, en.read,
+                                                                    a.init_scale);
     if (active && k == W - 1) {
-      a.out[en.out] = log10(S) - 307.050595577260822;
+      a.out[en.out] = log10(S * a.back_scale) - 307.050595577260822;
       a.flags[en.out] = 3;
     }
   }
@@ -804,7 +798,9 @@ const std::array<Fp32Fn, 13> kFp32W64[2] = {fp32_row<64, false, 2>(std::make_int
 const Row16 kFp32W8[2] = {fp32_row<8, false, 2>(std::make_integer_sequence<int, 16>{}),
                           fp32_row<8, true, 2>(std::make_integer_sequence<int, 16>{})};
 
-const std::array<Fp64Fn, kNumFp64Bins> kFp64 = {{&k_fp64<16, 4>, &k_fp64<16, 8>, &k_fp64<32, 8>, &k_fp64<64, 12>}};
+const std::array<Fp64Fn, kNumFp64Bins> kFp64[2] = {
+    {{&k_fp64<16, 4, false>, &k_fp64<16, 8, false>, &k_fp64<32, 8, false>, &k_fp64<64, 12, false>}},
+    {{&k_fp64<16, 4, true>, &k_fp64<16, 8, true>, &k_fp64<32, 8, true>, &k_fp64<64, 12, true>}}};
 const std::array<Fp64Fn, kNumFp64Bins> kFp64Pt = {
     {&k_fp64pt<16, 4>, &k_fp64pt<16, 8>, &k_fp64pt<32, 8>, &k_fp64pt<64, 12>}};
 
@@ -851,9 +847,9 @@ hipError_t launch_fp32(int W, int RPL, bool xf, int occ, bool pt, int grid, hipS
   return hipGetLastError();
 }
 
-hipError_t launch_fp64(int bin, bool pt, int grid, hipStream_t stream, const Fp64Args& a) {
-  if (bin < 0 || bin >= kNumFp64Bins) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(pt ? kFp64Pt[bin] : kFp64[bin], dim3(grid), dim3(64), 0, stream, a);
+hipError_t launch_fp64(int bin, bool pt, bool scaled, int grid, hipStream_t stream, const Fp64Args& a) {
+  if (bin < 0 || bin >= kNumFp64Bins || (pt && !scaled)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(pt ? kFp64Pt[bin] : kFp64[scaled][bin], dim3(grid), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
 
@@ -896,14 +892,14 @@ int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ, bool pt) {
   return slot;
 }
 
-int fp64_blocks_per_cu(int bin, bool pt) {
-  static int cache[2][kNumFp64Bins] = {};
+int fp64_blocks_per_cu(int bin, bool pt, bool scaled) {
+  static int cache[2][2][kNumFp64Bins] = {};
   if (bin < 0 || bin >= kNumFp64Bins) return 1;
-  int& slot = cache[pt][bin];
+  int& slot = cache[pt][scaled][bin];
   if (slot == 0) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pt ? kFp64Pt[bin] : kFp64[bin], 64, 0) != hipSuccess ||
-        n < 1)
+    Fp64Fn f = pt ? kFp64Pt[bin] : kFp64[scaled][bin];
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64, 0) != hipSuccess || n < 1)
       n = 1;
     slot = n;
   }

@@ -185,6 +185,30 @@ def test_gap_quality_forms(lo, hi, gcp, form):
     assert np.array_equal(alone[m], got[m])
 
 
+@pytest.mark.parametrize("R", [63, 64, 65, 128, 256, 512, 767, 768])
+def test_rescue_bin_edges(R):
+    say(f"FP64 rescue at read length {R} (edges of the rescue bins' row counts)")
+    rng = np.random.default_rng(1000 + R)
+    hap = np.full(800, ord("A"), np.uint8)
+    haps = [hap, hap.copy()]
+    haps[1][5] = ord("N")  # one haplotype per FP64 kernel
+    # 30 C's in every other base of a poly-A read at quality 40 with gap
+    # continuation 20: a mismatch costs 4.5 in log10, an insertion 4.5 to open
+    # and 2 a base, so log10 L is near -120, below the FP32 range (about -64)
+    # and inside the FP64 one
+    reads, quals = [], []
+    for x in range(3):
+        r = np.full(R, ord("A"), np.uint8)
+        first = int(rng.integers(0, R - 60 + 1))
+        r[first + 1 + x % 2:first + 60:2] = ord("C")
+        reads.append(r)
+        quals.append(np.full(R, 40, np.uint8))
+    reg = workload.make_region(haps, reads, quals)
+    reg["c"][:] = 20
+    got, flags, _ = check_region(reg)
+    assert ((flags & 2) != 0).all() and np.isfinite(got).all()
+
+
 @pytest.mark.parametrize("R,H", [(1, 1), (1, 40), (40, 1), (16, 16), (17, 300), (100, 350), (112, 300), (127, 200),
                                  (128, 200), (255, 200), (256, 256), (257, 100), (300, 700), (512, 512), (513, 90),
                                  (700, 1536), (767, 900), (768, 1536)])
