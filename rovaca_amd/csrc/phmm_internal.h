@@ -76,19 +76,20 @@ struct RescueEntry {
 // that no lane holds more than 12 double rows. Few bins on purpose: the FP64
 // pass is one launch per bin that can occur, whether or not any pair of the
 // batch needs it.
-constexpr int kNumFp64Bins = 4;
+constexpr int kNumFp64Bins = 5;
 __host__ __device__ inline int fp64_bin(uint32_t R) {
   if (R <= 64) return 0;    // W=16, RPL 4
   if (R <= 128) return 1;   // W=16, RPL 8
-  if (R <= 256) return 2;   // W=32, RPL 8
-  return 3;                 // W=64, RPL 12
+  if (R <= 160) return 2;   // W=32, RPL 5 (150-base reads)
+  if (R <= 256) return 3;   // W=32, RPL 8
+  return 4;                 // W=64, RPL 12
 }
 // Rescue lists: one per FP64 bin for pairs whose haplotype has N (list b)
 // and one per bin for the others (list kNumFp64Bins + b), which the
 // prior-table FP64 kernel can take.
 constexpr int kNumFp64Lists = 2 * kNumFp64Bins;
 inline void fp64_bin_shape(int bin, int* W, int* RPL) {
-  static const int kW[kNumFp64Bins] = {16, 16, 32, 64}, kR[kNumFp64Bins] = {4, 8, 8, 12};
+  static const int kW[kNumFp64Bins] = {16, 16, 32, 32, 64}, kR[kNumFp64Bins] = {4, 8, 5, 8, 12};
   *W = kW[bin];
   *RPL = kR[bin];
 }

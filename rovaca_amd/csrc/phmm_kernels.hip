@@ -799,10 +799,12 @@ const Row16 kFp32W8[2] = {fp32_row<8, false, 2>(std::make_integer_sequence<int, 
                           fp32_row<8, true, 2>(std::make_integer_sequence<int, 16>{})};
 
 const std::array<Fp64Fn, kNumFp64Bins> kFp64[2] = {
-    {{&k_fp64<16, 4, false>, &k_fp64<16, 8, false>, &k_fp64<32, 8, false>, &k_fp64<64, 12, false>}},
-    {{&k_fp64<16, 4, true>, &k_fp64<16, 8, true>, &k_fp64<32, 8, true>, &k_fp64<64, 12, true>}}};
+    {{&k_fp64<16, 4, false>, &k_fp64<16, 8, false>, &k_fp64<32, 5, false>, &k_fp64<32, 8, false>,
+      &k_fp64<64, 12, false>}},
+    {{&k_fp64<16, 4, true>, &k_fp64<16, 8, true>, &k_fp64<32, 5, true>, &k_fp64<32, 8, true>,
+      &k_fp64<64, 12, true>}}};
 const std::array<Fp64Fn, kNumFp64Bins> kFp64Pt = {
-    {&k_fp64pt<16, 4>, &k_fp64pt<16, 8>, &k_fp64pt<32, 8>, &k_fp64pt<64, 12>}};
+    {&k_fp64pt<16, 4>, &k_fp64pt<16, 8>, &k_fp64pt<32, 5>, &k_fp64pt<32, 8>, &k_fp64pt<64, 12>}};
 
 // Prior-table variants (scaled form only): W = 8, 16 and 32 at 2, 3, 4 waves
 // per SIMD, W = 64 at 2.

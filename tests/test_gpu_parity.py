@@ -185,7 +185,7 @@ def test_gap_quality_forms(lo, hi, gcp, form):
     assert np.array_equal(alone[m], got[m])
 
 
-@pytest.mark.parametrize("R", [63, 64, 65, 128, 256, 512, 767, 768])
+@pytest.mark.parametrize("R", [63, 64, 65, 128, 129, 160, 161, 256, 512, 767, 768])
 def test_rescue_bin_edges(R):
     say(f"FP64 rescue at read length {R} (edges of the rescue bins' row counts)")
     rng = np.random.default_rng(1000 + R)
