@@ -92,7 +92,8 @@ int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags, cons
   if (!out) return set_error(RVC_E_INVALID, "null output");
   SyncWorkspace* w = nullptr;
   if ((rc = get_workspace(&w))) return rc;
-  if ((rc = build_host_pack(region, 1, policy(), &w->pack, prep))) return rc;
+  const FP32Policy pol = latency_policy(policy(), (uint64_t)region->n_haps * region->n_reads);
+  if ((rc = build_host_pack(region, 1, pol, &w->pack, prep))) return rc;
   split_runs(w->pack);
   w->batch.graph_exec = nullptr;
   w->batch.graph = nullptr;

@@ -157,6 +157,15 @@ inline uint32_t base_code(uint8_t ch, bool* is_n) {
 inline uint32_t align_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
 }  // namespace
 
+FP32Policy latency_policy(const FP32Policy& base, uint64_t pairs) {
+  FP32Policy p = base;
+  if (p.force_w == 0) {
+    if (pairs <= 512) p.force_w = 64;
+    else if (pairs <= 2048) p.force_w = 32;
+  }
+  return p;
+}
+
 FP32Policy& default_policy() {
   static FP32Policy p;
   return p;
@@ -636,6 +645,8 @@ PinnedBuf::~PinnedBuf() { release(); }
 
 void PinnedBuf::release() {
   if (ptr) hipHostFree(ptr);
+  for (char* p : retired) hipHostFree(p);
+  retired.clear();
   ptr = nullptr;
   cap = 0;
 }
@@ -643,7 +654,9 @@ void PinnedBuf::release() {
 int PinnedBuf::reserve(size_t bytes) {
   if (bytes <= cap) return RVC_OK;
   const size_t want = std::max(bytes + bytes / 4 + (1u << 20), 2 * cap);
-  release();
+  if (ptr) retired.push_back(ptr);
+  ptr = nullptr;
+  cap = 0;
   if (hipHostMalloc((void**)&ptr, want, hipHostMallocDefault) != hipSuccess) {
     ptr = nullptr;
     return set_error(RVC_E_DEVICE, "pinned host buffer of %zu bytes", want);

@@ -79,6 +79,12 @@ struct FP32Policy {
 
 // The process-wide policy (built-in defaults, replaced by a tuned table at init).
 FP32Policy& default_policy();
+
+// The policy for a region that runs alone: one too small to give every SIMD
+// a wave of 16-lane tiles (up to 2048 pairs) finishes sooner with wider
+// segments, fewer rows per lane and so a shorter chain per step (measured on
+// 8 x 32 pairs of 150 bases: 59 us at W = 16, 43 us at W = 64).
+FP32Policy latency_policy(const FP32Policy& base, uint64_t pairs);
 // Reads a table written by rovaca_amd/tune.py; returns RVC_OK or an error code.
 int load_tune_table(const char* path, FP32Policy* pol);
 
@@ -158,9 +164,13 @@ BatchLayout batch_layout(const PackSizes& z);
 void stage_pack(const HostPack& P, const BatchLayout& L, char* dst);
 
 // Grow-only pinned host buffer.
+// Pinned host buffer that grows by doubling. Growing keeps the old block
+// until release(): freeing pinned memory waits for the whole device, which
+// would stall the launches of other streams.
 struct PinnedBuf {
   char* ptr = nullptr;
   size_t cap = 0;
+  std::vector<char*> retired;
   ~PinnedBuf();
   int reserve(size_t bytes);  // keeps no contents
   void release();

@@ -49,13 +49,15 @@ struct ResultBuf {
   ~ResultBuf() {
     if (mem) hipHostFree(mem);
   }
-  int reserve(size_t n, bool* grew) {
+  // Grows by doubling; the old block goes to 'retired' (freed with the pool:
+  // freeing pinned memory waits for the whole device).
+  int reserve(size_t n, bool* grew, std::vector<char*>* retired) {
     if (n <= cap && mem) return RVC_OK;
     *grew = true;
-    if (mem) hipHostFree(mem);
+    const size_t c = std::max(n + n / 4 + 4096, 2 * cap);
+    if (mem) retired->push_back(mem);
     mem = nullptr;
     cap = 0;
-    const size_t c = std::max(n + n / 4 + 4096, 2 * cap);
     if (hipHostMalloc((void**)&mem, c * (sizeof(double) + 1) + 64 + 64, hipHostMallocDefault) != hipSuccess) {
       mem = nullptr;
       return set_error(RVC_E_DEVICE, "pinned result buffer for %zu pairs", n);
@@ -71,6 +73,16 @@ struct ResultBuf {
 struct ResultPool : std::enable_shared_from_this<ResultPool> {
   std::mutex mu;
   std::vector<std::unique_ptr<ResultBuf>> free;
+  std::vector<char*> retired;
+  ~ResultPool() {
+    for (char* p : retired) hipHostFree(p);
+  }
+  // A buffer of n pair slots in the free list from the start.
+  void add(size_t n) {
+    auto b = std::unique_ptr<ResultBuf>(new (std::nothrow) ResultBuf);
+    bool grew = false;
+    if (b && b->reserve(n, &grew, &retired) == RVC_OK) free.push_back(std::move(b));
+  }
   std::shared_ptr<ResultBuf> acquire(size_t n, int* rc, bool* grew) {
     std::unique_ptr<ResultBuf> b;
     {
@@ -89,7 +101,15 @@ struct ResultPool : std::enable_shared_from_this<ResultPool> {
       *rc = set_error(RVC_E_INVALID, "out of host memory");
       return nullptr;
     }
-    if ((*rc = b->reserve(n, grew))) return nullptr;
+    {
+      std::vector<char*> old;
+      *rc = b->reserve(n, grew, &old);
+      if (!old.empty()) {
+        std::lock_guard<std::mutex> lk(mu);
+        retired.insert(retired.end(), old.begin(), old.end());
+      }
+      if (*rc) return nullptr;
+    }
     std::shared_ptr<ResultPool> self = shared_from_this();
     return std::shared_ptr<ResultBuf>(b.release(), [self](ResultBuf* p) {
       std::lock_guard<std::mutex> lk(self->mu);
@@ -127,6 +147,7 @@ struct Slot {
   hipEvent_t done_ev = nullptr;
   void* mem = nullptr;  // device arena (grow-only)
   size_t bytes = 0;
+  std::vector<void*> retired;  // outgrown arenas, freed at stop
   PinnedBuf staging;
   HostPack meta;
   DeviceBatch batch;
@@ -160,6 +181,7 @@ struct Server {
   double busy_ms = 0;
   std::atomic<uint64_t> grows{0};
   std::atomic<uint64_t> stage_ns{0};
+  std::atomic<uint32_t> in_flight{0};  // submitted, not yet finished
 };
 
 Server* g_srv = nullptr;
@@ -183,9 +205,10 @@ int launch_slot(Slot& s) {
   if (s.staging.cap != staging_cap) ++g_srv->grows;
   for (Request* r : s.reqs) r->pack = HostPack();  // the staged image is all the launch needs
   if (!s.mem || s.bytes < L.bytes) {
-    // grow by doubling: freeing device memory waits for the device
+    // grow by doubling; the old arena is freed at stop (freeing device
+    // memory waits for the whole device; its last launch has finished)
     const size_t want = std::max(L.bytes + L.bytes / 4 + (16u << 20), 2 * s.bytes);
-    if (s.mem) hipFree(s.mem);
+    if (s.mem) s.retired.push_back(s.mem);
     s.mem = nullptr;
     s.bytes = 0;
     if (hipMalloc(&s.mem, want) != hipSuccess) {
@@ -277,6 +300,7 @@ void slot_main(Slot* sp) {
       rq->done = true;
     }
     S.completed += s.reqs.size();
+    S.in_flight.fetch_sub((uint32_t)s.reqs.size(), std::memory_order_relaxed);
     if (rc == RVC_OK) {
       S.busy_ms += ms;
       S.rescued += resc;
@@ -314,6 +338,9 @@ int server_start() {
     if (hipMalloc(&s.mem, arena) == hipSuccess) s.bytes = arena;
     else s.mem = nullptr;
     s.staging.reserve(128u << 20);
+    // result buffers for launches of about 500 typical regions
+    S->pool->add(2u << 20);
+    S->pool->add(2u << 20);
   }
   g_srv = S;
   for (Slot& s : S->slots) s.thr = std::thread(slot_main, &s);
@@ -341,6 +368,7 @@ void server_stop() {
     s.res.reset();
     s.staging.release();
     if (s.mem) hipFree(s.mem);
+    for (void* p : s.retired) hipFree(p);
     if (s.done_ev) hipEventDestroy(s.done_ev);
     if (s.stream) hipStreamDestroy(s.stream);
   }
@@ -357,7 +385,10 @@ int submit_impl(const rvc_region* region, const DevicePrep* prep, rvc_ticket* ti
   if (rc) return rc;
   auto* r = new (std::nothrow) Request;
   if (!r) return set_error(RVC_E_INVALID, "out of host memory");
-  rc = build_host_pack(region, 1, default_policy(), &r->pack, prep);
+  // with nothing else in flight the region runs alone: shapes for latency
+  const bool alone = g_srv->in_flight.load(std::memory_order_relaxed) == 0;
+  const uint64_t pairs = (uint64_t)region->n_haps * region->n_reads;
+  rc = build_host_pack(region, 1, alone ? latency_policy(default_policy(), pairs) : default_policy(), &r->pack, prep);
   if (rc) {
     delete r;
     return rc;
@@ -382,6 +413,7 @@ int submit_impl(const rvc_region* region, const DevicePrep* prep, rvc_ticket* ti
     S.tickets[r->id] = r;
     S.pending.push_back(r);
     ++S.submitted;
+    S.in_flight.fetch_add(1, std::memory_order_relaxed);
     *ticket = r->id;
   }
   S.cv_work.notify_one();
