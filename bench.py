@@ -148,7 +148,7 @@ def run_single(args):
     for _ in range(max(1, args.warmup)):
         rv.region_sync(reg)
     lat = []
-    for _ in range(max(20, args.steps)):
+    for _ in range(max(200, args.steps)):
         t0 = time.perf_counter()
         rv.region_sync(reg)
         lat.append((time.perf_counter() - t0) * 1e3)

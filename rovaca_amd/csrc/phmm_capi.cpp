@@ -113,8 +113,8 @@ int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags, cons
   if ((rc = w->results.reserve(n * (sizeof(double) + 1) + 64))) return rc;
   double* h_out = reinterpret_cast<double*>(w->results.ptr);
   uint8_t* h_flags = reinterpret_cast<uint8_t*>(w->results.ptr + n * sizeof(double));
-  hipError_t e = hipMemcpyAsync(h_out, w->batch.out, n * sizeof(double), hipMemcpyDeviceToHost, w->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(h_flags, w->batch.flags, n, hipMemcpyDeviceToHost, w->stream);
+  // the flags sit right behind the results on both sides: one copy
+  hipError_t e = hipMemcpyAsync(h_out, w->batch.out, n * (sizeof(double) + 1), hipMemcpyDeviceToHost, w->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
   if (e != hipSuccess) return set_error(RVC_E_DEVICE, "region results: %s", hipGetErrorString(e));
   unpermute_results(w->pack, h_out, h_flags, out, flags);

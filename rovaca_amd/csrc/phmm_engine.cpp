@@ -613,8 +613,9 @@ BatchLayout batch_layout(const PackSizes& z) {
   L.raw = take(z.raw_bytes);
   L.prep = take(4 * z.n_reads);
   L.in_bytes = off;
-  L.out = take(sizeof(double) * (z.n_pairs_slots + 1));
-  L.flags = take(z.n_pairs_slots + 8);
+  // flags right behind the results: one copy brings both back
+  L.out = take(sizeof(double) * z.n_pairs_slots + z.n_pairs_slots + 8);
+  L.flags = L.out + sizeof(double) * z.n_pairs_slots;
   L.resc = take(sizeof(RescueEntry) * (z.n_rescue + 1));
   L.cnt = take(4 * kNumCounters);
   L.bytes = off;
@@ -828,9 +829,7 @@ int DeviceBatch::upload_staged(const HostPack& P, const BatchLayout& L, const vo
   char* base = static_cast<char*>(mem);
   RVC_HIP(hipMemcpyAsync(base, host, L.in_bytes, hipMemcpyHostToDevice, s));
   // results start as NaN with no flags; pairs with an empty side keep that
-  const uint64_t nan_bits = 0x7ff8000000000000ull;
-  RVC_HIP(launch_fill_u64(reinterpret_cast<uint64_t*>(base + L.out), P.n_pairs_slots, nan_bits, s));
-  RVC_HIP(launch_zero_u32(reinterpret_cast<uint32_t*>(base + L.flags), (uint32_t)((P.n_pairs_slots + 3) / 4), s));
+  RVC_HIP(launch_init_results(reinterpret_cast<double*>(base + L.out), P.n_pairs_slots, s));
   seqs.hap_pack = reinterpret_cast<const uint32_t*>(base + L.hpack);
   seqs.hap_nmask = reinterpret_cast<const uint32_t*>(base + L.hn);
   seqs.hap_off = reinterpret_cast<const uint32_t*>(base + L.hoff);

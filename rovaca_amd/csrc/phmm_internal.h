@@ -150,6 +150,7 @@ hipError_t launch_fp32(int W, int RPL, bool scaled, int occ, bool pt, int grid, 
 // N in the haplotype, scaled form only), scaled = the scaled form.
 hipError_t launch_fp64(int bin, bool pt, bool scaled, int grid, hipStream_t stream, const Fp64Args& a);
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
-hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t stream);
+// out[0 .. n) = NaN and the n flag bytes behind them (rounded up to 8) = 0.
+hipError_t launch_init_results(double* out, uint64_t n, hipStream_t stream);
 
 }  // namespace rvc

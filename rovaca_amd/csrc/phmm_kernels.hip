@@ -762,10 +762,16 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   if (i < n) p[i] = 0;
 }
 
-// Grid-stride fill of n 64-bit words.
-__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t value) {
+// Results start as NaN with no flags (pairs with an empty side keep that):
+// n doubles, then n flag bytes written as ceil(n / 8) zero words.
+__global__ void k_init_results(uint64_t* out, uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = value;
+  uint64_t* flags = out + n;
+  const uint64_t nf = (n + 7) / 8;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    out[i] = 0x7ff8000000000000ull;
+    if (i < nf) flags[i] = 0;
+  }
 }
 
 using Fp32Fn = void (*)(Fp32Args);
@@ -867,10 +873,10 @@ hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t stream) {
+hipError_t launch_init_results(double* out, uint64_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, value);
+  hipLaunchKernelGGL(k_init_results, dim3((unsigned)blocks), dim3(256), 0, stream, reinterpret_cast<uint64_t*>(out), n);
   return hipGetLastError();
 }
 

@@ -43,8 +43,9 @@ constexpr int kSlots = 3;
 struct ResultBuf {
   char* mem = nullptr;
   size_t cap = 0;  // pair slots
+  size_t n = 0;    // pair slots of the launch holding it; its flags follow its results
   double* out() const { return reinterpret_cast<double*>(mem); }
-  uint8_t* flags() const { return reinterpret_cast<uint8_t*>(mem + cap * sizeof(double)); }
+  uint8_t* flags() const { return reinterpret_cast<uint8_t*>(mem + n * sizeof(double)); }
   uint32_t* rescued() const { return reinterpret_cast<uint32_t*>(mem + cap * (sizeof(double) + 1) + 64); }
   ~ResultBuf() {
     if (mem) hipHostFree(mem);
@@ -110,6 +111,7 @@ struct ResultPool : std::enable_shared_from_this<ResultPool> {
       }
       if (*rc) return nullptr;
     }
+    b->n = n;
     std::shared_ptr<ResultPool> self = shared_from_this();
     return std::shared_ptr<ResultBuf>(b.release(), [self](ResultBuf* p) {
       std::lock_guard<std::mutex> lk(self->mu);
@@ -226,8 +228,8 @@ int launch_slot(Slot& s) {
   if (grew) ++g_srv->grows;
   if ((rc = s.batch.ensure_events())) return rc;
   if ((rc = s.batch.enqueue(s.stream, true))) return rc;
-  if (hipMemcpyAsync(s.res->out(), s.batch.out, n * sizeof(double), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
-      hipMemcpyAsync(s.res->flags(), s.batch.flags, n, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+  if (hipMemcpyAsync(s.res->out(), s.batch.out, n * (sizeof(double) + 1), hipMemcpyDeviceToHost, s.stream) !=
+          hipSuccess ||
       hipMemcpyAsync(s.res->rescued(), s.batch.counters + kFp32Bins + kNumFp64Lists,
                      kNumFp64Lists * sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
       hipEventRecord(s.done_ev, s.stream) != hipSuccess)
