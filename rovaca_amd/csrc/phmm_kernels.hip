@@ -532,46 +532,90 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
           pr[4 * q + 0] = v.x; pr[4 * q + 1] = v.y; pr[4 * q + 2] = v.z; pr[4 * q + 3] = v.w;
         }
       };
-      float pA[NQ * 4], pB[NQ * 4];
-      uint32_t cw = window(0);
-      load(cw, 0, pA);
       // Four steps a -> b -> a -> b -> a per iteration. The read's last row is
       // always a padding row (the host picks RPL with R < W * RPL), which holds
       // M[R-1][j] + I[R-1][j] in I, so the sum takes one add per step.
       // sched_barrier(0) keeps each step's table reads ahead of the previous
       // step's arithmetic, so their latency overlaps a whole step.
-      for (int st = 0; st < nsteps; st += 4) {
-        const uint32_t cw_next = window(st + 4);
-        load(cw, 1, pB);
-        __builtin_amdgcn_sched_barrier(0);
-        sweep_step_pt<float, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
-        S += Ib[RPL - 1];
-        const float s1M = lane_down0<W>(Mb[RPL - 1], k);
-        const float s1I = lane_down0<W>(Ib[RPL - 1], k);
-        s1D = lane_down_keep<W>(Db[RPL - 1], s1D, old_d, k);
-        load(cw, 2, pA);
-        __builtin_amdgcn_sched_barrier(0);
-        sweep_step_pt<float, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, pB, tmm, tgm, tmi, tii, tmd, tdd);
-        if (st + 1 < nsteps) S += Ia[RPL - 1];
-        const float s2M = lane_down0<W>(Ma[RPL - 1], k);
-        const float s2I = lane_down0<W>(Ia[RPL - 1], k);
-        s2D = lane_down_keep<W>(Da[RPL - 1], s2D, old_d, k);
-        load(cw, 3, pB);
-        __builtin_amdgcn_sched_barrier(0);
-        sweep_step_pt<float, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, pA, tmm, tgm, tmi, tii, tmd, tdd);
-        if (st + 2 < nsteps) S += Ib[RPL - 1];
-        const float s3M = lane_down0<W>(Mb[RPL - 1], k);
-        const float s3I = lane_down0<W>(Ib[RPL - 1], k);
-        s3D = lane_down_keep<W>(Db[RPL - 1], s3D, old_d, k);
-        cw = cw_next;
+      if constexpr (NQ > 2) {
+        float pA[NQ * 4], pB[NQ * 4];
+        uint32_t cw = window(0);
         load(cw, 0, pA);
-        __builtin_amdgcn_sched_barrier(0);
-        sweep_step_pt<float, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, pB, tmm, tgm, tmi, tii, tmd, tdd);
-        if (st + 3 < nsteps) S += Ia[RPL - 1];
-        dM = s3M; dI = s3I; dD = s3D;
-        uM = lane_down0<W>(Ma[RPL - 1], k);
-        uI = lane_down0<W>(Ia[RPL - 1], k);
-        uD = lane_down_keep<W>(Da[RPL - 1], uD, old_d, k);
+        for (int st = 0; st < nsteps; st += 4) {
+          const uint32_t cw_next = window(st + 4);
+          load(cw, 1, pB);
+          __builtin_amdgcn_sched_barrier(0);
+          sweep_step_pt<float, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
+          S += Ib[RPL - 1];
+          const float s1M = lane_down0<W>(Mb[RPL - 1], k);
+          const float s1I = lane_down0<W>(Ib[RPL - 1], k);
+          s1D = lane_down_keep<W>(Db[RPL - 1], s1D, old_d, k);
+          load(cw, 2, pA);
+          __builtin_amdgcn_sched_barrier(0);
+          sweep_step_pt<float, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, pB, tmm, tgm, tmi, tii, tmd, tdd);
+          if (st + 1 < nsteps) S += Ia[RPL - 1];
+          const float s2M = lane_down0<W>(Ma[RPL - 1], k);
+          const float s2I = lane_down0<W>(Ia[RPL - 1], k);
+          s2D = lane_down_keep<W>(Da[RPL - 1], s2D, old_d, k);
+          load(cw, 3, pB);
+          __builtin_amdgcn_sched_barrier(0);
+          sweep_step_pt<float, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, pA, tmm, tgm, tmi, tii, tmd, tdd);
+          if (st + 2 < nsteps) S += Ib[RPL - 1];
+          const float s3M = lane_down0<W>(Mb[RPL - 1], k);
+          const float s3I = lane_down0<W>(Ib[RPL - 1], k);
+          s3D = lane_down_keep<W>(Db[RPL - 1], s3D, old_d, k);
+          cw = cw_next;
+          load(cw, 0, pA);
+          __builtin_amdgcn_sched_barrier(0);
+          sweep_step_pt<float, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, pB, tmm, tgm, tmi, tii, tmd, tdd);
+          if (st + 3 < nsteps) S += Ia[RPL - 1];
+          dM = s3M; dI = s3I; dD = s3D;
+          uM = lane_down0<W>(Ma[RPL - 1], k);
+          uI = lane_down0<W>(Ia[RPL - 1], k);
+          uD = lane_down_keep<W>(Da[RPL - 1], uD, old_d, k);
+        }
+      } else {
+        // Up to 8 rows per lane a step is too short to cover a table read:
+        // read three steps ahead into four rotating buffers.
+        float p0[NQ * 4], p1[NQ * 4], p2[NQ * 4], p3[NQ * 4];
+        uint32_t cw = window(0), cw1 = window(4);
+        load(cw, 0, p0);
+        load(cw, 1, p1);
+        load(cw, 2, p2);
+        for (int st = 0; st < nsteps; st += 4) {
+          const uint32_t cw2 = window(st + 8);
+          load(cw, 3, p3);
+          __builtin_amdgcn_sched_barrier(0);
+          sweep_step_pt<float, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, p0, tmm, tgm, tmi, tii, tmd, tdd);
+          S += Ib[RPL - 1];
+          const float s1M = lane_down0<W>(Mb[RPL - 1], k);
+          const float s1I = lane_down0<W>(Ib[RPL - 1], k);
+          s1D = lane_down_keep<W>(Db[RPL - 1], s1D, old_d, k);
+          load(cw1, 0, p0);
+          __builtin_amdgcn_sched_barrier(0);
+          sweep_step_pt<float, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, p1, tmm, tgm, tmi, tii, tmd, tdd);
+          if (st + 1 < nsteps) S += Ia[RPL - 1];
+          const float s2M = lane_down0<W>(Ma[RPL - 1], k);
+          const float s2I = lane_down0<W>(Ia[RPL - 1], k);
+          s2D = lane_down_keep<W>(Da[RPL - 1], s2D, old_d, k);
+          load(cw1, 1, p1);
+          __builtin_amdgcn_sched_barrier(0);
+          sweep_step_pt<float, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, p2, tmm, tgm, tmi, tii, tmd, tdd);
+          if (st + 2 < nsteps) S += Ib[RPL - 1];
+          const float s3M = lane_down0<W>(Mb[RPL - 1], k);
+          const float s3I = lane_down0<W>(Ib[RPL - 1], k);
+          s3D = lane_down_keep<W>(Db[RPL - 1], s3D, old_d, k);
+          load(cw1, 2, p2);
+          __builtin_amdgcn_sched_barrier(0);
+          sweep_step_pt<float, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, p3, tmm, tgm, tmi, tii, tmd, tdd);
+          if (st + 3 < nsteps) S += Ia[RPL - 1];
+          dM = s3M; dI = s3I; dD = s3D;
+          uM = lane_down0<W>(Ma[RPL - 1], k);
+          uI = lane_down0<W>(Ia[RPL - 1], k);
+          uD = lane_down_keep<W>(Da[RPL - 1], uD, old_d, k);
+          cw = cw1;
+          cw1 = cw2;
+        }
       }
       if (active && k == W - 1)
         finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, S * a.back_scale);
