@@ -209,8 +209,9 @@ def test_rescue_bin_edges(R):
     assert ((flags & 2) != 0).all() and np.isfinite(got).all()
 
 
-@pytest.mark.parametrize("R,H", [(1, 1), (1, 40), (40, 1), (16, 16), (17, 300), (100, 350), (112, 300), (127, 200),
-                                 (128, 200), (255, 200), (256, 256), (257, 100), (300, 700), (512, 512), (513, 90),
+@pytest.mark.parametrize("R,H", [(1, 1), (1, 40), (40, 1), (16, 16), (17, 300), (100, 350), (102, 300), (103, 300),
+                                 (112, 300), (127, 200), (128, 200), (158, 300), (159, 300), (254, 200), (255, 200),
+                                 (256, 256), (257, 100), (300, 700), (510, 512), (511, 512), (512, 512), (513, 90),
                                  (700, 1536), (767, 900), (768, 1536)])
 def test_shapes(R, H):
     say(f"shape R={R} H={H}")
