@@ -537,7 +537,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
       // M[R-1][j] + I[R-1][j] in I, so the sum takes one add per step.
       // sched_barrier(0) keeps each step's table reads ahead of the previous
       // step's arithmetic, so their latency overlaps a whole step.
-      if constexpr (NQ > 2) {
+      // Short steps (up to 8 rows per lane) do not cover a table read, and at
+      // two waves per SIMD (OCC 2) up to 13 rows leave registers for more
+      // buffers: those read three steps ahead (measured: 8 lanes x 13 rows
+      // 22.4 -> 21.8 ms on the batched workload; 16 x 10 at OCC 3 loses).
+      if constexpr (NQ > 2 && !(OCC == 2 && RPL <= 13)) {
         float pA[NQ * 4], pB[NQ * 4];
         uint32_t cw = window(0);
         load(cw, 0, pA);
@@ -575,8 +579,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
           uD = lane_down_keep<W>(Da[RPL - 1], uD, old_d, k);
         }
       } else {
-        // Up to 8 rows per lane a step is too short to cover a table read:
-        // read three steps ahead into four rotating buffers.
+        // three steps ahead, four rotating buffers
         float p0[NQ * 4], p1[NQ * 4], p2[NQ * 4], p3[NQ * 4];
         uint32_t cw = window(0), cw1 = window(4);
         load(cw, 0, p0);
