@@ -16,9 +16,17 @@ _u8p = ctypes.POINTER(ctypes.c_uint8)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
 _f32p = ctypes.POINTER(ctypes.c_float)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
 
 _lib = None
 _ref = None
+_ref_sw = None
+
+# Smith-Waterman scores (match, mismatch, gap open, gap extend) of Rovaca's two
+# aligner instances (common/base/lib/sw_avx/sw_avx.c:14-15)
+SW_GENOTYPE = (10, -15, -30, -5)
+SW_ASSEMBLE = (200, -150, -260, -11)
+CIGAR_OPS = "MIDNSHP=X"
 
 
 def lib():
@@ -50,6 +58,9 @@ def lib():
         L.orc_count_repetitions.restype = ctypes.c_int
         L.orc_repeat_length.argtypes = [_u8p, ctypes.c_int, ctypes.c_int]
         L.orc_repeat_length.restype = ctypes.c_int
+        L.orc_sw_align.argtypes = [_u8p, ctypes.c_int, _u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, _u32p, ctypes.c_int, _i32p, _i32p]
+        L.orc_sw_align.restype = ctypes.c_int
         L.orc_postprocess.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, _i32p, _f64p, _i32p]
         L.orc_postprocess.restype = ctypes.c_int
         _lib = L
@@ -250,3 +261,51 @@ def baseline_run(regions, n_threads, want_out=False):
         out = np.empty(sum(int(r["n_haps"]) * int(r["n_reads"]) for r in regions), np.float64)
     secs = L.orc_baseline_run(arr, len(regions), n_threads, out.ctypes.data_as(_f64p) if out is not None else None)
     return secs, out
+
+
+def _sw_out(n, cig, off, sc):
+    if n < 0:
+        return None, int(off.value), None
+    ops = [(CIGAR_OPS[int(c) & 15], int(c) >> 4) for c in cig[:n]]
+    return ops, int(off.value), int(sc.value)
+
+
+def sw_align(hap, read, assemble=False):
+    """Our restatement: (CIGAR as [(op, length)], alignment offset on the haplotype, score),
+    or (None, 1, None) where the aligner refuses the pair (a length >= 2048)."""
+    h = np.frombuffer(bytes(hap) or b"\0", np.uint8)
+    r = np.frombuffer(bytes(read) or b"\0", np.uint8)
+    cig = np.zeros(4 * 2048, np.uint32)
+    off, sc = ctypes.c_int32(0), ctypes.c_int32(0)
+    p = SW_ASSEMBLE if assemble else SW_GENOTYPE
+    n = lib().orc_sw_align(h.ctypes.data_as(_u8p), len(hap), r.ctypes.data_as(_u8p), len(read), p[0], p[1], p[2], p[3],
+                           cig.ctypes.data_as(_u32p), len(cig), ctypes.byref(off), ctypes.byref(sc))
+    return _sw_out(n, cig, off, sc)
+
+
+def ref_sw():
+    """Rovaca's own aligner built from its sources into oracle/_ref, or None."""
+    global _ref_sw
+    if _ref_sw is None:
+        path = os.path.join(_HERE, "_ref", "libref_sw.so")
+        if not os.path.exists(path):
+            return None
+        L = ctypes.CDLL(path)
+        L.ref_sw_align.argtypes = [ctypes.c_int, _u8p, ctypes.c_int, _u8p, ctypes.c_int, _u32p, ctypes.c_int, _i32p,
+                                   _i32p]
+        L.ref_sw_align.restype = ctypes.c_int
+        _ref_sw = L
+    return _ref_sw
+
+
+def ref_sw_align(hap, read, assemble=False):
+    L = ref_sw()
+    if L is None:
+        raise RuntimeError("oracle/_ref/libref_sw.so is not built")
+    h = np.frombuffer(bytes(hap) or b"\0", np.uint8)
+    r = np.frombuffer(bytes(read) or b"\0", np.uint8)
+    cig = np.zeros(4 * 2048, np.uint32)
+    off, sc = ctypes.c_int32(0), ctypes.c_int32(0)
+    n = L.ref_sw_align(1 if assemble else 0, h.ctypes.data_as(_u8p), len(hap), r.ctypes.data_as(_u8p), len(read),
+                       cig.ctypes.data_as(_u32p), len(cig), ctypes.byref(off), ctypes.byref(sc))
+    return _sw_out(n, cig, off, sc)

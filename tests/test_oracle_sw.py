@@ -1,0 +1,149 @@
+"""The Smith-Waterman oracle (read against its best haplotype) pinned against
+Rovaca's own aligner compiled from its sources (oracle/_ref), closed forms and
+brute force on tiny inputs."""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+
+ACGT = np.frombuffer(b"ACGT", np.uint8)
+
+
+def _mutate(rng, s, n_edits):
+    s = bytearray(s)
+    for _ in range(n_edits):
+        k, p = int(rng.integers(0, 3)), int(rng.integers(0, len(s)))
+        if k == 0:
+            s[p] = int(ACGT[rng.integers(0, 4)])
+        elif k == 1:
+            s.insert(p, int(ACGT[rng.integers(0, 4)]))
+        elif len(s) > 1:
+            del s[p]
+    return bytes(s)
+
+
+def random_pairs(seed, n, max_hap=400, max_read=300):
+    """Reads that are edited pieces of their haplotype (as after assembly),
+    pieces with foreign ends, and unrelated reads."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for t in range(n):
+        H = int(rng.integers(1, max_hap))
+        hap = ACGT[rng.integers(0, 4, H)].tobytes()
+        if t % 4 == 0:
+            read = ACGT[rng.integers(0, 4, int(rng.integers(1, max_read)))].tobytes()
+        else:
+            a = int(rng.integers(0, H))
+            b = int(rng.integers(a + 1, H + 1))
+            read = _mutate(rng, hap[a:b], int(rng.integers(0, 6))) or b"A"
+            if t % 4 == 3:
+                read = ACGT[rng.integers(0, 4, 5)].tobytes() + read + ACGT[rng.integers(0, 4, 7)].tobytes()
+        out.append((hap, read))
+    return out
+
+
+@pytest.mark.skipif(oracle.ref_sw() is None, reason="oracle/_ref/libref_sw.so is not built")
+@pytest.mark.parametrize("assemble", [False, True])
+def test_matches_reference_aligner(assemble):
+    for hap, read in random_pairs(11 + assemble, 1500):
+        assert oracle.sw_align(hap, read, assemble) == oracle.ref_sw_align(hap, read, assemble), (hap, read)
+
+
+@pytest.mark.skipif(oracle.ref_sw() is None, reason="oracle/_ref/libref_sw.so is not built")
+def test_matches_reference_aligner_edges():
+    # short reads (the 32-bit core below 16 bases), single bases, reads longer
+    # than the haplotype, repeats (ties), the 2048 limit
+    rng = np.random.default_rng(5)
+    cases = [(b"A", b"A"), (b"A", b"C"), (b"AC", b"A"), (b"A", b"ACGT"), (b"ACGTACGT", b"ACGTACGTACGTACGTACGT"),
+             (b"AAAAAAAAAAAAAAAAAAAA", b"AAAA"), (b"AAAAAAAAAA", b"AAAAAAAAAAAAAAAAAAAAAAAAA"),
+             (b"ACACACACACACACAC", b"CACACA"), (b"A" * 2047, b"A" * 10), (b"A" * 2048, b"A" * 10)]
+    for L in (1, 2, 7, 8, 15, 16, 17, 31, 32, 33):
+        hap = ACGT[rng.integers(0, 4, 3 * L + 5)].tobytes()
+        cases.append((hap, hap[L:2 * L]))
+        cases.append((hap, _mutate(rng, hap[L:2 * L], 1) or b"G"))
+        cases.append((hap[:L], ACGT[rng.integers(0, 4, L)].tobytes()))
+    for hap, read in cases:
+        for assemble in (False, True):
+            assert oracle.sw_align(hap, read, assemble) == oracle.ref_sw_align(hap, read, assemble), (hap, read)
+
+
+def test_closed_forms():
+    hap = b"GATTACACCGTAGCTTAGGCATCGATCGGATCCATGCA"
+    # a read equal to a piece of the haplotype: one match run at its offset
+    for a, b in [(0, len(hap)), (5, 25), (13, 21), (10, 30)]:  # pieces that occur once
+        assert oracle.sw_align(hap, hap[a:b]) == ([("M", b - a)], a, 10 * (b - a))
+    # one substitution inside long flanks stays a match run (-15 < clipping a flank)
+    read = bytearray(hap[4:34])
+    read[15] = ord("A") if read[15] != ord("A") else ord("C")
+    assert oracle.sw_align(hap, bytes(read)) == ([("M", 30)], 4, 10 * 29 - 15)
+    # one inserted and one deleted base inside 20-base flanks: gap open -30
+    ins = hap[2:22] + b"T" + hap[22:37]
+    cig, off, sc = oracle.sw_align(hap, ins)
+    assert off == 2 and sc == 10 * 35 - 30 and [op for op, _ in cig] == ["M", "I", "M"] and cig[1] == ("I", 1)
+    dele = hap[1:20] + hap[21:38]
+    cig, off, sc = oracle.sw_align(hap, dele)
+    assert off == 1 and sc == 10 * 36 - 30 and [op for op, _ in cig] == ["M", "D", "M"] and cig[1] == ("D", 1)
+    # read bases hanging over either end of the haplotype become soft clips
+    # (elsewhere the read has to be aligned: the alignment starts on row 0 or
+    # column 0 and ends on the last row or column)
+    assert oracle.sw_align(hap, b"TTTTT" + hap[:20]) == ([("S", 5), ("M", 20)], 0, 200)
+    assert oracle.sw_align(hap, hap[18:] + b"GGGG") == ([("M", len(hap) - 18), ("S", 4)], 18, 10 * (len(hap) - 18))
+    # the refused lengths
+    assert oracle.sw_align(b"A" * 2048, b"AAAA") == (None, 1, None)
+
+
+def _brute_best(hap, read, p):
+    """Best score over all alignments from a first-row/first-column cell
+    (score 0) to a last-row/last-column cell, by exhaustive path search."""
+    match, mismatch, gap_open, gap_ext = p
+    n1, n2 = len(hap), len(read)
+    memo = {}
+
+    def best_from(i, j, state):
+        # best score of a path that starts at (i, j) and ends on the last row
+        # or column; state: 0 after a match/start, 1 inside an insertion, 2 inside a deletion
+        key = (i, j, state)
+        if key in memo:
+            return memo[key]
+        res = 0 if (i == n1 or j == n2) else None
+        cands = []
+        if i < n1 and j < n2:
+            cands.append(best_from(i + 1, j + 1, 0) + (match if hap[i] == read[j] else mismatch))
+        if j < n2 and i > 0:
+            cands.append(best_from(i, j + 1, 1) + (gap_ext if state == 1 else gap_open))
+        if i < n1 and j > 0:
+            cands.append(best_from(i + 1, j, 2) + (gap_ext if state == 2 else gap_open))
+        cands = [c for c in cands if c is not None]
+        if cands:
+            res = max(cands) if res is None else max(res, max(cands))
+        memo[key] = res
+        return res
+
+    starts = [(i, 0) for i in range(n1 + 1)] + [(0, j) for j in range(1, n2 + 1)]
+    # an alignment must take at least one cell step
+    best = None
+    for i, j in starts:
+        if i < n1 and j < n2:
+            v = best_from(i + 1, j + 1, 0) + (match if hap[i] == read[j] else mismatch)
+            best = v if best is None else max(best, v)
+        if j < n2 and i > 0:
+            v = best_from(i, j + 1, 1) + gap_open
+            best = max(best, v) if best is not None else v
+        if i < n1 and j > 0:
+            v = best_from(i + 1, j, 2) + gap_open
+            best = max(best, v) if best is not None else v
+    return best
+
+
+def test_score_is_the_best_alignment_by_brute_force():
+    rng = np.random.default_rng(8)
+    for n1, n2 in itertools.product(range(1, 6), range(1, 6)):
+        for _ in range(3):
+            hap = ACGT[rng.integers(0, 2, n1)].tobytes()  # two letters: more ties and gaps
+            read = ACGT[rng.integers(0, 2, n2)].tobytes()
+            for assemble in (False, True):
+                p = oracle.SW_ASSEMBLE if assemble else oracle.SW_GENOTYPE
+                _, _, sc = oracle.sw_align(hap, read, assemble)
+                assert sc == _brute_best(hap, read, p), (hap, read, assemble)
