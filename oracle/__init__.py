@@ -294,6 +294,8 @@ def ref_sw():
         L.ref_sw_align.argtypes = [ctypes.c_int, _u8p, ctypes.c_int, _u8p, ctypes.c_int, _u32p, ctypes.c_int, _i32p,
                                    _i32p]
         L.ref_sw_align.restype = ctypes.c_int
+        L.ref_sw_align_many.argtypes = [_u8p, _i32p, _u8p, _i32p, _u32p, ctypes.c_int, ctypes.c_int]
+        L.ref_sw_align_many.restype = ctypes.c_double
         _ref_sw = L
     return _ref_sw
 
@@ -309,3 +311,20 @@ def ref_sw_align(hap, read, assemble=False):
     n = L.ref_sw_align(1 if assemble else 0, h.ctypes.data_as(_u8p), len(hap), r.ctypes.data_as(_u8p), len(read),
                        cig.ctypes.data_as(_u32p), len(cig), ctypes.byref(off), ctypes.byref(sc))
     return _sw_out(n, cig, off, sc)
+
+
+def ref_sw_align_many(haps, reads, read_hap, n_threads):
+    """Rovaca's aligner over all pairs on n_threads host threads (genotype
+    scores); returns the wall seconds."""
+    L = ref_sw()
+    if L is None:
+        raise RuntimeError("oracle/_ref/libref_sw.so is not built")
+    hap_off = np.zeros(len(haps) + 1, np.int32)
+    hap_off[1:] = np.cumsum([len(h) for h in haps])
+    read_off = np.zeros(len(reads) + 1, np.int32)
+    read_off[1:] = np.cumsum([len(r) for r in reads])
+    hb = np.frombuffer(b"".join(bytes(h) for h in haps) + b"\0", np.uint8)
+    rb = np.frombuffer(b"".join(bytes(r) for r in reads) + b"\0", np.uint8)
+    rh = np.ascontiguousarray(read_hap, np.uint32)
+    return float(L.ref_sw_align_many(hb.ctypes.data_as(_u8p), hap_off.ctypes.data_as(_i32p), rb.ctypes.data_as(_u8p),
+                                     read_off.ctypes.data_as(_i32p), rh.ctypes.data_as(_u32p), len(reads), n_threads))

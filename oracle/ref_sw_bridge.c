@@ -34,3 +34,47 @@ int ref_sw_align(int is_assemble, const unsigned char* seq1, int len1, const uns
   for (int x = 0; x < n; ++x) cigar[x] = sw->cigar[x];
   return n;
 }
+
+// CPU baseline: pairs [0, n) on n_threads threads (pair p: haps + hap_off[hap[p]],
+// reads + read_off[p]), each thread with its own aligner instance. Returns the
+// wall seconds.
+#include <pthread.h>
+#include <time.h>
+
+typedef struct {
+  const unsigned char* haps;
+  const int* hap_off;
+  const unsigned char* reads;
+  const int* read_off;
+  const unsigned int* hap;
+  int first, step, n;
+} sw_job;
+
+static void* sw_worker(void* arg) {
+  const sw_job* j = (const sw_job*)arg;
+  unsigned int cig[4096];
+  int off, sc;
+  for (int p = j->first; p < j->n; p += j->step) {
+    const unsigned int h = j->hap[p];
+    ref_sw_align(0, j->haps + j->hap_off[h], j->hap_off[h + 1] - j->hap_off[h], j->reads + j->read_off[p],
+                 j->read_off[p + 1] - j->read_off[p], cig, 4096, &off, &sc);
+  }
+  return 0;
+}
+
+double ref_sw_align_many(const unsigned char* haps, const int* hap_off, const unsigned char* reads, const int* read_off,
+                         const unsigned int* hap, int n, int n_threads) {
+  pthread_t th[256];
+  sw_job jobs[256];
+  if (n_threads > 256) n_threads = 256;
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  for (int t = 0; t < n_threads; ++t) {
+    jobs[t].haps = haps; jobs[t].hap_off = hap_off; jobs[t].reads = reads; jobs[t].read_off = read_off;
+    jobs[t].hap = hap; jobs[t].first = t; jobs[t].step = n_threads; jobs[t].n = n;
+    pthread_create(&th[t], 0, sw_worker, &jobs[t]);
+  }
+  for (int t = 0; t < n_threads; ++t) pthread_join(th[t], 0);
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}

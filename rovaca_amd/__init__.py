@@ -32,7 +32,7 @@ EXPORTS = [
     "rvc_phmm_call_pairhmm", "rvc_phmm_batch_create", "rvc_phmm_batch_run", "rvc_phmm_batch_results",
     "rvc_phmm_batch_info", "rvc_phmm_batch_destroy", "rvc_phmm_submit", "rvc_phmm_wait", "rvc_phmm_poll",
     "rvc_phmm_server_stats", "rvc_phmm_preprocess_read", "rvc_phmm_postprocess", "rvc_phmm_tables",
-    "rvc_loadgen_run", "rvc_loadgen_call_pairhmm", "rvc_phmm_batch_kernels",
+    "rvc_loadgen_run", "rvc_loadgen_call_pairhmm", "rvc_phmm_batch_kernels", "rvc_sw_align_batch",
 ]
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -386,3 +386,51 @@ def server_stats():
     st = RvcServerStats()
     _check(lib().rvc_phmm_server_stats(ctypes.byref(st)))
     return {k: getattr(st, k) for k, _ in RvcServerStats._fields_}
+
+
+# ---- batched Smith-Waterman (include/rovaca_sw.h) ---------------------------
+class RvcSwStats(ctypes.Structure):
+    _fields_ = [("fill_ms", ctypes.c_double), ("trace_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("cells", ctypes.c_uint64), ("launches", ctypes.c_uint32), ("chunks", ctypes.c_uint32)]
+
+
+_SW_OPS = "MIDNSHP=X"
+
+
+def sw_align_batch(haps, reads, read_hap, assemble=False, with_stats=False):
+    """Aligns reads[r] against haps[read_hap[r]] on the device (the aligner of
+    Rovaca's read realignment). Returns [(cigar [(op, length)], offset, score)]
+    per read, (None, 1, None) where the aligner refuses the pair; with
+    with_stats, also the call's statistics as a dict."""
+    L = lib()
+    fn = L.rvc_sw_align_batch
+    fn.argtypes = [_u8p, _i32p, ctypes.c_uint32, _u8p, _i32p, ctypes.c_uint32, _u32p, ctypes.c_int32, ctypes.c_uint32,
+                   _u32p, _i32p, _i32p, _i32p, ctypes.POINTER(RvcSwStats)]
+    hap_off = np.zeros(len(haps) + 1, np.int32)
+    hap_off[1:] = np.cumsum([len(h) for h in haps])
+    read_off = np.zeros(len(reads) + 1, np.int32)
+    read_off[1:] = np.cumsum([len(r) for r in reads])
+    hb = np.frombuffer(b"".join(bytes(h) for h in haps) + b"\0", np.uint8)
+    rb = np.frombuffer(b"".join(bytes(r) for r in reads) + b"\0", np.uint8)
+    rh = np.ascontiguousarray(read_hap, np.uint32)
+    n = len(reads)
+    max_cigar = 2 * max([len(r) for r in reads] + [1]) + 2
+    cig = np.zeros(n * max_cigar, np.uint32)
+    cnt = np.zeros(n, np.int32)
+    off = np.zeros(n, np.int32)
+    sc = np.zeros(n, np.int32)
+    st = RvcSwStats()
+    _check(fn(hb.ctypes.data_as(_u8p), hap_off.ctypes.data_as(_i32p), len(haps), rb.ctypes.data_as(_u8p),
+              read_off.ctypes.data_as(_i32p), n, rh.ctypes.data_as(_u32p), 1 if assemble else 0, max_cigar,
+              cig.ctypes.data_as(_u32p), cnt.ctypes.data_as(_i32p), off.ctypes.data_as(_i32p),
+              sc.ctypes.data_as(_i32p), ctypes.byref(st)))
+    out = []
+    for r in range(n):
+        if cnt[r] < 0:
+            out.append((None, int(off[r]), None))
+            continue
+        c = cig[r * max_cigar:r * max_cigar + cnt[r]]
+        out.append(([(_SW_OPS[int(x) & 15], int(x) >> 4) for x in c], int(off[r]), int(sc[r])))
+    if with_stats:
+        return out, {f: getattr(st, f) for f, _ in RvcSwStats._fields_}
+    return out

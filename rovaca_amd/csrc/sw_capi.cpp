@@ -1,0 +1,223 @@
+// Batched Smith-Waterman of reads against haplotypes: the C ABI of
+// include/rovaca_sw.h. Pairs are grouped by sweep shape (read length),
+// swept by k_sw_fill (one launch per shape present) and traced by
+// k_sw_trace; batches whose backtrack bytes exceed kBtBudget run in chunks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "../../include/rovaca_sw.h"
+#include "phmm_engine.h"
+#include "sw_internal.h"
+
+namespace rvc {
+namespace {
+
+constexpr uint64_t kBtBudget = 12ull << 30;  // backtrack bytes per device pass
+
+// scores of Rovaca's two aligner instances (sw_avx.c:14-15)
+constexpr int32_t kGenotype[4] = {10, -15, -30, -5};
+constexpr int32_t kAssemble[4] = {200, -150, -260, -11};
+
+inline uint64_t up256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+
+// Per-thread device workspace: a stream, three events, a grow-only buffer.
+struct SwWorkspace {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[3] = {};
+  void* mem = nullptr;
+  size_t bytes = 0;
+  ~SwWorkspace() {
+    if (mem) hipFree(mem);
+    for (hipEvent_t e : ev)
+      if (e) hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+  }
+  int ensure(size_t want) {
+    if (!stream) {
+      if (hipSetDevice(g_engine.device) != hipSuccess ||
+          hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
+        return set_error(RVC_E_DEVICE, "aligner stream");
+      for (hipEvent_t& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner events");
+    }
+    if (want <= bytes) return RVC_OK;
+    if (mem) hipFree(mem);
+    mem = nullptr;
+    bytes = 0;
+    const size_t n = std::max(want + want / 4, (size_t)64 << 20);
+    if (hipMalloc(&mem, n) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner workspace of %zu bytes", n);
+    bytes = n;
+    return RVC_OK;
+  }
+};
+thread_local SwWorkspace t_sw;
+
+}  // namespace
+
+int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_haps, const uint8_t* read_bases,
+                   const int32_t* read_off, uint32_t n_reads, const uint32_t* read_hap, int32_t assemble,
+                   uint32_t max_cigar, uint32_t* cigar, int32_t* count, int32_t* offset, int32_t* score,
+                   rvc_sw_stats* stats) {
+  const auto t_start = std::chrono::steady_clock::now();
+  if (!g_engine.ready) return set_error(RVC_E_STATE, "rvc_phmm_init has not run");
+  if (n_reads == 0) return RVC_OK;
+  if (!hap_bases || !hap_off || !read_bases || !read_off || !read_hap || !cigar || !count || !offset || !score ||
+      n_haps == 0 || max_cigar == 0)
+    return set_error(RVC_E_INVALID, "null pointer or empty haplotype list");
+  for (uint32_t h = 0; h < n_haps; ++h)
+    if (hap_off[h] < 0 || hap_off[h + 1] < hap_off[h]) return set_error(RVC_E_INVALID, "haplotype offsets");
+  for (uint32_t r = 0; r < n_reads; ++r) {
+    if (read_off[r] < 0 || read_off[r + 1] < read_off[r]) return set_error(RVC_E_INVALID, "read offsets");
+    if (read_hap[r] >= n_haps) return set_error(RVC_E_INVALID, "read %u names haplotype %u", r, read_hap[r]);
+    const int32_t n2 = read_off[r + 1] - read_off[r];
+    if (n2 > kSwMaxRead && n2 < kSwMaxLen)
+      return set_error(RVC_E_LIMIT, "read %u has %d bases (the device aligner takes up to %d)", r, n2, kSwMaxRead);
+  }
+  const int32_t* sc = assemble ? kAssemble : kGenotype;
+  if (stats) std::memset(stats, 0, sizeof *stats);
+
+  // one SwPair per read; refused pairs stay out of the sweeps
+  std::vector<SwPair> pairs(n_reads);
+  std::vector<int> shape(n_reads, -1);
+  std::vector<uint64_t> bt_bytes(n_reads, 0);
+  for (uint32_t r = 0; r < n_reads; ++r) {
+    SwPair& p = pairs[r];
+    std::memset(&p, 0, sizeof p);
+    const uint32_t h = read_hap[r];
+    p.hap_off = (uint32_t)hap_off[h];
+    p.hap_len = (uint32_t)(hap_off[h + 1] - hap_off[h]);
+    p.read_off = (uint32_t)read_off[r];
+    p.read_len = (uint32_t)(read_off[r + 1] - read_off[r]);
+    if (p.hap_len == 0 || p.read_len == 0 || p.hap_len >= (uint32_t)kSwMaxLen || p.read_len >= (uint32_t)kSwMaxLen)
+      continue;
+    int W = 0, RPL = 0;
+    shape[r] = sw_shape(p.read_len, &W, &RPL);
+    p.width = (uint16_t)W;
+    p.rpl = (uint16_t)RPL;
+    bt_bytes[r] = up256((uint64_t)(p.hap_len + W - 1) * (uint64_t)(W * RPL));
+    if (stats) stats->cells += (uint64_t)p.hap_len * p.read_len;
+  }
+  const uint64_t hap_bytes = (uint64_t)hap_off[n_haps], read_bytes = (uint64_t)read_off[n_reads];
+
+  SwWorkspace& w = t_sw;
+  uint32_t r0 = 0;
+  while (r0 < n_reads) {
+    // the chunk: reads r0 .. r1 whose backtrack bytes fit the budget
+    uint32_t r1 = r0;
+    uint64_t bt_total = 0, hrow_total = 0, hcol_total = 0;
+    while (r1 < n_reads && (r1 == r0 || bt_total + bt_bytes[r1] <= kBtBudget)) {
+      SwPair& p = pairs[r1];
+      p.bt_off = bt_total;
+      p.hrow_off = (uint32_t)hrow_total;
+      p.hcol_off = (uint32_t)hcol_total;
+      bt_total += bt_bytes[r1];
+      if (shape[r1] >= 0) {
+        hrow_total += p.hap_len;
+        hcol_total += (uint64_t)p.width * p.rpl;
+      }
+      ++r1;
+    }
+    const uint32_t n = r1 - r0;
+    // pair lists by shape
+    std::vector<uint32_t> list;
+    list.reserve(n);
+    uint32_t start[kSwShapes + 1] = {};
+    for (int s = 0; s < kSwShapes; ++s) {
+      start[s] = (uint32_t)list.size();
+      for (uint32_t x = 0; x < n; ++x)
+        if (shape[r0 + x] == s) list.push_back(x);
+    }
+    start[kSwShapes] = (uint32_t)list.size();
+    // device layout
+    size_t off = 0;
+    auto take = [&](uint64_t b) { const size_t o = off; off += up256(b); return o; };
+    const size_t o_hap = take(hap_bytes + 64), o_read = take(read_bytes + 64), o_pair = take(sizeof(SwPair) * n),
+                 o_list = take(4ull * (list.size() + 1)), o_head = take(4 * kSwShapes), o_bt = take(bt_total),
+                 o_hrow = take(4 * (hrow_total + 1)), o_hcol = take(4 * (hcol_total + 1)),
+                 o_cig = take(4ull * n * max_cigar), o_cnt = take(4ull * n), o_off = take(4ull * n),
+                 o_sc = take(4ull * n);
+    int rc = w.ensure(off);
+    if (rc) return rc;
+    char* base = static_cast<char*>(w.mem);
+    hipStream_t st = w.stream;
+    hipError_t e = hipMemcpyAsync(base + o_hap, hap_bases, hap_bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_read, read_bases, read_bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_pair, pairs.data() + r0, sizeof(SwPair) * n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !list.empty())
+      e = hipMemcpyAsync(base + o_list, list.data(), 4 * list.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(base + o_head, 0, 4 * kSwShapes, st);
+    if (e != hipSuccess) return set_error(RVC_E_DEVICE, "aligner upload: %s", hipGetErrorString(e));
+    SwArgs a{};
+    a.haps = reinterpret_cast<const uint8_t*>(base + o_hap);
+    a.reads = reinterpret_cast<const uint8_t*>(base + o_read);
+    a.pairs = reinterpret_cast<const SwPair*>(base + o_pair);
+    a.n_pairs = n;
+    a.bt = reinterpret_cast<uint8_t*>(base + o_bt);
+    a.hrow = reinterpret_cast<int32_t*>(base + o_hrow);
+    a.hcol = reinterpret_cast<int32_t*>(base + o_hcol);
+    a.match = sc[0];
+    a.mismatch = sc[1];
+    a.open = sc[2];
+    a.extend = sc[3];
+    a.cigar = reinterpret_cast<uint32_t*>(base + o_cig);
+    a.max_cigar = max_cigar;
+    a.count = reinterpret_cast<int32_t*>(base + o_cnt);
+    a.offset = reinterpret_cast<int32_t*>(base + o_off);
+    a.score = reinterpret_cast<int32_t*>(base + o_sc);
+    if (hipEventRecord(w.ev[0], st) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");
+    for (int s = 0; s < kSwShapes; ++s) {
+      const uint32_t nl = start[s + 1] - start[s];
+      if (nl == 0) continue;
+      a.list = reinterpret_cast<const uint32_t*>(base + o_list) + start[s];
+      a.n_list = nl;
+      a.head = reinterpret_cast<uint32_t*>(base + o_head) + s;
+      const uint32_t nseg = 64u / (uint32_t)kSwW[s];
+      const uint32_t waves = (nl + nseg - 1) / nseg;
+      const int grid = (int)std::max<uint32_t>(
+          1, std::min<uint32_t>(waves, (uint32_t)sw_blocks_per_cu(s) * (uint32_t)g_engine.n_cu));
+      if ((e = launch_sw_fill(s, grid, st, a)) != hipSuccess)
+        return set_error(RVC_E_DEVICE, "aligner sweep: %s", hipGetErrorString(e));
+      if (stats) ++stats->launches;
+    }
+    a.list = nullptr;
+    a.n_list = 0;
+    if (hipEventRecord(w.ev[1], st) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");
+    if ((e = launch_sw_trace(st, a)) != hipSuccess)
+      return set_error(RVC_E_DEVICE, "aligner traceback: %s", hipGetErrorString(e));
+    if (hipEventRecord(w.ev[2], st) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");
+    e = hipMemcpyAsync(cigar + (size_t)r0 * max_cigar, base + o_cig, 4ull * n * max_cigar, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(count + r0, base + o_cnt, 4ull * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(offset + r0, base + o_off, 4ull * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(score + r0, base + o_sc, 4ull * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return set_error(RVC_E_DEVICE, "aligner results: %s", hipGetErrorString(e));
+    if (stats) {
+      float f = 0, t = 0;
+      hipEventElapsedTime(&f, w.ev[0], w.ev[1]);
+      hipEventElapsedTime(&t, w.ev[1], w.ev[2]);
+      stats->fill_ms += f;
+      stats->trace_ms += t;
+      ++stats->chunks;
+    }
+    for (uint32_t x = 0; x < n; ++x)
+      if (count[r0 + x] == -2) return set_error(RVC_E_LIMIT, "read %u needs more than %u CIGAR elements", r0 + x, max_cigar);
+    r0 = r1;
+  }
+  if (stats)
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  return RVC_OK;
+}
+
+}  // namespace rvc
+
+extern "C" int rvc_sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_haps,
+                                  const uint8_t* read_bases, const int32_t* read_off, uint32_t n_reads,
+                                  const uint32_t* read_hap, int32_t assemble, uint32_t max_cigar, uint32_t* cigar,
+                                  int32_t* count, int32_t* offset, int32_t* score, rvc_sw_stats* stats) {
+  return rvc::sw_align_batch(hap_bases, hap_off, n_haps, read_bases, read_off, n_reads, read_hap, assemble, max_cigar,
+                             cigar, count, offset, score, stats);
+}

@@ -1,0 +1,66 @@
+// Device-side types of the batched Smith-Waterman (read against haplotype).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rvc {
+
+constexpr int kSwMaxLen = 2048;       // the reference aligner refuses lengths >= this (sw_avx.c:18, sw_core_template.h:341)
+constexpr int kSwMaxRead = 1024;      // longest read the sweep shapes hold
+constexpr int kSwShapes = 8;          // (W, RPL): see sw_shape()
+constexpr int kSwLow = -(1 << 28);    // "no gap" at row and column 0
+constexpr int kSwBig = -(1 << 26);    // disables a move in the rows past the read
+constexpr uint32_t kSwOpM = 0, kSwOpI = 1, kSwOpD = 2, kSwOpS = 4;  // BAM CIGAR operations
+
+// Sweep shapes (segment width W, rows per lane RPL): 64 .. 1024 rows. Up to
+// 12 rows per lane a kernel keeps two waves per SIMD.
+constexpr int kSwW[kSwShapes] = {16, 16, 16, 32, 32, 64, 64, 64};
+constexpr int kSwR[kSwShapes] = {4, 8, 12, 8, 12, 8, 12, 16};
+
+// The shape for a read of n2 bases (the fewest rows with W * RPL >= n2), or -1.
+inline int sw_shape(uint32_t n2, int* W, int* RPL) {
+  for (int s = 0; s < kSwShapes; ++s)
+    if ((uint32_t)(kSwW[s] * kSwR[s]) >= n2) {
+      *W = kSwW[s];
+      *RPL = kSwR[s];
+      return s;
+    }
+  return -1;
+}
+
+struct SwPair {
+  uint32_t hap_off, hap_len;
+  uint32_t read_off, read_len;
+  uint64_t bt_off;      // backtrack bytes: (hap_len + W - 1) steps x W * RPL rows
+  uint32_t hrow_off;    // hap_len ints: H of the read's last row per column
+  uint32_t hcol_off;    // W * RPL ints: H of the haplotype's last column per row
+  uint16_t width, rpl;
+  uint32_t pad;
+};
+
+struct SwArgs {
+  const uint8_t* haps;
+  const uint8_t* reads;
+  const SwPair* pairs;
+  uint32_t n_pairs;
+  const uint32_t* list;  // pairs of this launch's shape
+  uint32_t n_list;
+  uint32_t* head;        // this launch's work-queue counter
+  uint8_t* bt;
+  int32_t* hrow;
+  int32_t* hcol;
+  int32_t match, mismatch, open, extend;
+  uint32_t* cigar;       // max_cigar elements per pair
+  uint32_t max_cigar;
+  int32_t* count;
+  int32_t* offset;
+  int32_t* score;
+};
+
+hipError_t launch_sw_fill(int shape, int grid, hipStream_t stream, const SwArgs& a);
+hipError_t launch_sw_trace(hipStream_t stream, const SwArgs& a);
+int sw_blocks_per_cu(int shape);
+
+}  // namespace rvc

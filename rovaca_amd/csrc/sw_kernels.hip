@@ -1,0 +1,291 @@
+// Smith-Waterman of reads against haplotypes on gfx950: Rovaca's
+// read-to-best-haplotype aligner (the ro_sw library behind sw_avx_init /
+// avx_function, common/base/lib/ro_sw/sw_core_template.h:113-334 and
+// sw_cigar.c:20-171), batched.
+//
+// Matrix: rows j = 1..n2 are read bases, columns i = 1..n1 haplotype bases
+// (the reference's i and j, transposed for the sweep). Per cell
+//   E = max(H[i][j-1] + open, E[i][j-1] + extend)   (insertion, down a column)
+//   F = max(F[i-1][j] + extend, H[i-1][j] + open)   (deletion, along a row)
+//   H = max(max(cut, H[i-1][j-1] + s(hap[i], read[j])), E, F)
+// with the reference's tie order (E replaces M only when greater, F replaces
+// both only when greater) and a backtrack byte: 0 match, 1 insertion,
+// 2 deletion, | 4 if E extended, | 8 if F extended. Row 0 and column 0 hold
+// H = 0 and no open gap.
+//
+// The sweep is the PairHMM one: a segment of W lanes takes one pair, lane k
+// holds read rows k*RPL .. k*RPL+RPL-1 and at step s computes haplotype
+// column s-k+1 of all of them; H and E of a lane's last row move one lane down
+// with DPP, F stays in the lane. Rows past the read carry H of the read's last
+// row down unchanged (E open 0, no match, no deletion), so the segment's last
+// row holds H[i][n2] for the best-end search. Backtrack bytes are stored per
+// step in lane order (coalesced); the traceback kernel maps (i, j) back.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "sw_internal.h"
+
+namespace rvc {
+namespace {
+
+constexpr int kHapRow = 2176;  // LDS bytes per segment: 64 zeros, <= 2047 bases, drain columns
+
+template <int W>
+__device__ __forceinline__ int down0(int v, int k) {
+  if constexpr (W == 16) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);
+  } else if constexpr (W == 64) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true);
+  } else {
+    const int t = __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true);
+    return k == 0 ? 0 : t;
+  }
+}
+
+template <int W>
+__device__ __forceinline__ int down_or(int v, int old, int k) {
+  if constexpr (W == 16) {
+    return __builtin_amdgcn_update_dpp(old, v, 0x111, 0xf, 0xf, false);
+  } else {
+    const int t = __builtin_amdgcn_update_dpp(old, v, 0x138, 0xf, 0xf, false);
+    return k == 0 ? old : t;
+  }
+}
+
+// One step of a lane over its RPL rows at haplotype base hc.
+// uH, uE: H and E of the row above in this column; dH: H of the row above in
+// the previous column.
+template <int RPL>
+__device__ __forceinline__ void sw_step(int (&Hp)[RPL], int (&F)[RPL], int (&Hn)[RPL], int& lastE, uint32_t (&bt)[RPL / 4],
+                                        int uH, int uE, int dH, int hc, const int (&rc)[RPL], const int (&oE)[RPL],
+                                        const int (&oF)[RPL], const int (&mis)[RPL], int match, int ext, int cut) {
+  int aH = uH, aE = uE, dg = dH;
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const int oh = aH + oE[r], eh = aE + ext;
+    const int e = max(oh, eh);
+    const int ov = Hp[r] + oF[r], ev = F[r] + ext;
+    const int f = max(ev, ov);
+    const int m = dg + (hc == rc[r] ? match : mis[r]);
+    int h = max(cut, m);
+    int b = e > h ? 1 : 0;
+    h = max(h, e);
+    b = f > h ? 2 : b;
+    h = max(h, f);
+    b |= (oh > eh ? 0 : 4) | (ov > ev ? 0 : 8);
+    if ((r & 3) == 0) bt[r >> 2] = (uint32_t)b;
+    else bt[r >> 2] |= (uint32_t)b << (8 * (r & 3));
+    dg = Hp[r];
+    Hn[r] = h;
+    F[r] = f;
+    aH = h;
+    aE = e;
+  }
+  lastE = aE;
+}
+
+template <int W, int RPL>
+__global__ __launch_bounds__(64) void k_sw_fill(SwArgs a) {
+  constexpr int NSEG = 64 / W;
+  constexpr int ROWS = W * RPL;
+  __shared__ uint8_t hb[NSEG][kHapRow];
+  const int lane = threadIdx.x;
+  const int seg = lane / W, k = lane % W;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(a.head, (uint32_t)NSEG);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= a.n_list) break;
+    const bool active = t + (uint32_t)seg < a.n_list;
+    const uint32_t p = a.list[active ? t + seg : t];
+    const SwPair pr = a.pairs[p];
+    const int n1 = (int)pr.hap_len, n2 = (int)pr.read_len;
+    // the wave sweeps to its longest haplotype
+    int nmax = n1;
+#pragma unroll
+    for (int g = 0; g < NSEG; ++g) {
+      const int o = __shfl(n1, g * W);
+      nmax = o > nmax ? o : nmax;
+    }
+    __syncthreads();
+    for (int x = k; x < kHapRow; x += W) {
+      const int i = x - 64;
+      hb[seg][x] = (i >= 0 && i < n1) ? a.haps[pr.hap_off + i] : 0;
+    }
+    __syncthreads();
+    int rc[RPL], oE[RPL], oF[RPL], mis[RPL], Hp[RPL], F[RPL], Hn[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      const int j = k * RPL + r;
+      const bool real = j < n2;
+      rc[r] = real ? (int)a.reads[pr.read_off + j] : 0x1ff;
+      oE[r] = real ? a.open : 0;
+      oF[r] = real ? a.open : kSwBig;
+      mis[r] = real ? a.mismatch : kSwBig;
+      Hp[r] = 0;
+      F[r] = kSwLow;
+    }
+    // the reference's 16-bit core floors H at INT16_MIN / 4 (ro_sw.c:19-32,
+    // sw_internal.h:19-22); its 32-bit core at -100000000
+    const bool use32 = n2 < 16 || a.match <= 0 || (long long)(n1 < n2 ? n1 : n2) * a.match > 32767;
+    const int cut = use32 ? -100000000 : -8192;
+    uint8_t* btp = a.bt + pr.bt_off + (uint32_t)(k * RPL);
+    int32_t* hrow = a.hrow + pr.hrow_off;
+    int32_t* hcol = a.hcol + pr.hcol_off;
+    int uH = 0, uE = kSwLow, dH = 0;
+    const int own_steps = n1 + W - 1;
+    const int nsteps = nmax + W - 1;
+    auto step = [&](int s, auto fill) {
+      const int col = s - k + 1;  // haplotype column of this lane
+      const int hc = hb[seg][64 + s - k];
+      uint32_t bt[RPL / 4];
+      int lastE;
+      sw_step<RPL>(Hp, F, Hn, lastE, bt, uH, uE, dH, hc, rc, oE, oF, mis, a.match, a.extend, cut);
+      if constexpr (decltype(fill)::value) {
+        if (col < 1) {
+          // before the first column: keep column 0 (H = 0, no deletion)
+#pragma unroll
+          for (int r = 0; r < RPL; ++r) { Hn[r] = 0; F[r] = kSwLow; }
+        }
+      }
+      if (active && s < own_steps) {
+        uint32_t* dst = reinterpret_cast<uint32_t*>(btp + (size_t)s * ROWS);
+#pragma unroll
+        for (int q = 0; q < RPL / 4; ++q) dst[q] = bt[q];
+        if (col == n1) {
+#pragma unroll
+          for (int r = 0; r < RPL; ++r) hcol[k * RPL + r] = Hn[r];
+        }
+        if (k == W - 1 && col >= 1 && col <= n1) hrow[col - 1] = Hn[RPL - 1];
+      }
+      dH = uH;
+      uH = down0<W>(Hn[RPL - 1], k);
+      uE = down_or<W>(lastE, kSwLow, k);
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) Hp[r] = Hn[r];
+    };
+    // the first W - 1 steps start lanes one by one; after them every lane
+    // is past column 0
+    int s = 0;
+    for (; s < W - 1 && s < nsteps; ++s) step(s, std::true_type{});
+    for (; s < nsteps; ++s) step(s, std::false_type{});
+  }
+}
+
+// One thread per pair: the best end cell (last haplotype column, then last
+// read row, one anti-diagonal at a time, with the reference's tie rules),
+// then the traceback into CIGAR runs, written start to end.
+__global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n_pairs) return;
+  const SwPair pr = a.pairs[p];
+  const int n1 = (int)pr.hap_len, n2 = (int)pr.read_len;
+  uint32_t* out = a.cigar + (size_t)p * a.max_cigar;
+  if (n1 >= kSwMaxLen || n2 >= kSwMaxLen || n1 == 0 || n2 == 0) {
+    a.count[p] = -1;
+    a.offset[p] = 1;
+    a.score[p] = 0;
+    return;
+  }
+  const int32_t* hrow = a.hrow + pr.hrow_off;  // H[i][n2], i = 1..n1
+  const int32_t* hcol = a.hcol + pr.hcol_off;  // H[n1][j], j = 1..n2
+  const bool use32 = n2 < 16 || a.match <= 0 || (long long)(n1 < n2 ? n1 : n2) * a.match > 32767;
+  int best = use32 ? INT32_MIN : -32768, bi = 0, bj = 0;
+  for (int d = 1; d <= n1 + n2; ++d) {
+    if (d >= n1 + 1) {
+      const int i = n1, j = d - n1, v = hcol[j - 1];
+      if (best < v || (best == v && abs(i - j) < abs(bi - bj))) { best = v; bi = i; bj = j; }
+    }
+    if (d >= n2 + 1) {
+      const int i = d - n2, j = n2, v = hrow[i - 1];
+      if (best < v || (best == v && (bj == n2 || abs(i - j) <= abs(bi - bj)))) { best = v; bi = i; bj = j; }
+    }
+  }
+  a.score[p] = best;
+  const int rows = (int)pr.width * (int)pr.rpl;
+  const uint8_t* bt = a.bt + pr.bt_off;
+  // runs end to start, merged as they come
+  int n = 0, cur_op = -1;
+  uint32_t cur_len = 0;
+  bool overflow = false;
+  auto push = [&](int op, uint32_t len) {
+    if (op == cur_op) { cur_len += len; return; }
+    if (cur_op >= 0) {
+      if (n < (int)a.max_cigar) out[n] = (cur_len << 4) | (uint32_t)cur_op;
+      else overflow = true;
+      ++n;
+    }
+    cur_op = op;
+    cur_len = len;
+  };
+  int i = bi, j = bj;
+  if (j < n2) push(kSwOpS, (uint32_t)(n2 - j));
+  int state = 0;
+  while (i > 0 && j > 0) {
+    const int kk = (j - 1) / (int)pr.rpl;
+    const int b = bt[(size_t)(i - 1 + kk) * rows + (j - 1)];
+    if (state == 4) {
+      --j;
+      push(kSwOpI, 1);
+      state = b & 4;
+    } else if (state == 8) {
+      --i;
+      push(kSwOpD, 1);
+      state = b & 8;
+    } else {
+      switch (b & 3) {
+        case 0: --i; --j; push(kSwOpM, 1); state = 0; break;
+        case 1: --j; push(kSwOpI, 1); state = b & 4; break;
+        default: --i; push(kSwOpD, 1); state = b & 8; break;
+      }
+    }
+  }
+  if (j > 0) push(kSwOpS, (uint32_t)j);
+  push(-2, 0);  // flush the last run
+  if (overflow) {
+    a.count[p] = -2;
+    a.offset[p] = i;
+    return;
+  }
+  for (int x = 0, y = n - 1; x < y; ++x, --y) {
+    const uint32_t tmp = out[x];
+    out[x] = out[y];
+    out[y] = tmp;
+  }
+  a.count[p] = n;
+  a.offset[p] = i;
+}
+
+using SwFn = void (*)(SwArgs);
+const SwFn kSwFill[kSwShapes] = {&k_sw_fill<kSwW[0], kSwR[0]>, &k_sw_fill<kSwW[1], kSwR[1]>,
+                                 &k_sw_fill<kSwW[2], kSwR[2]>, &k_sw_fill<kSwW[3], kSwR[3]>,
+                                 &k_sw_fill<kSwW[4], kSwR[4]>, &k_sw_fill<kSwW[5], kSwR[5]>,
+                                 &k_sw_fill<kSwW[6], kSwR[6]>, &k_sw_fill<kSwW[7], kSwR[7]>};
+}  // namespace
+
+hipError_t launch_sw_fill(int shape, int grid, hipStream_t stream, const SwArgs& a) {
+  if (shape < 0 || shape >= kSwShapes) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(kSwFill[shape], dim3(grid), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_sw_trace(hipStream_t stream, const SwArgs& a) {
+  if (a.n_pairs == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sw_trace, dim3((a.n_pairs + 63) / 64), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
+int sw_blocks_per_cu(int shape) {
+  static int cache[kSwShapes] = {};
+  if (shape < 0 || shape >= kSwShapes) return 1;
+  if (cache[shape] == 0) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kSwFill[shape], 64, 0) != hipSuccess || n < 1) n = 1;
+    cache[shape] = n;
+  }
+  return cache[shape];
+}
+
+}  // namespace rvc

@@ -1,0 +1,97 @@
+"""GPU parity of the batched Smith-Waterman (read against haplotype) with the
+oracle, which the CPU tests pin to Rovaca's own aligner: CIGAR, offset and
+score identical."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import rovaca_amd as rv
+from test_oracle_sw import ACGT, _mutate, random_pairs
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def engine():
+    say("init device 0")
+    rv.init(0)
+    yield
+    say("shutdown")
+    rv.shutdown()
+
+
+def check(haps, reads, read_hap, assemble=False):
+    got = rv.sw_align_batch(haps, reads, read_hap, assemble)
+    for r, (g, read) in enumerate(zip(got, reads)):
+        want = oracle.sw_align(haps[read_hap[r]], read, assemble)
+        assert g == want, (r, len(haps[read_hap[r]]), len(read), g, want)
+    return got
+
+
+@pytest.mark.parametrize("assemble", [False, True])
+def test_random_pairs(assemble):
+    say(f"random pairs, assemble={assemble}")
+    pairs = random_pairs(21 + assemble, 600)
+    haps = [h for h, _ in pairs]
+    reads = [r for _, r in pairs]
+    check(haps, reads, list(range(len(pairs))), assemble)
+
+
+def test_every_shape_and_its_edges():
+    say("read lengths at the sweep shapes' edges")
+    rng = np.random.default_rng(3)
+    haps, reads, read_hap = [], [], []
+    for L in (1, 2, 15, 16, 17, 63, 64, 65, 127, 128, 129, 191, 192, 193, 255, 256, 257, 383, 384, 385, 511, 512, 513,
+              767, 768, 769, 1000, 1023, 1024):
+        hap = ACGT[rng.integers(0, 4, L + 200)].tobytes()
+        haps.append(hap)
+        for _ in range(2):
+            a = int(rng.integers(0, 150))
+            reads.append(_mutate(rng, hap[a:a + L], int(rng.integers(0, 4))) or b"C")
+            read_hap.append(len(haps) - 1)
+        reads.append(ACGT[rng.integers(0, 4, L)].tobytes())
+        read_hap.append(len(haps) - 1)
+    check(haps, reads, read_hap)
+
+
+def test_haplotype_lengths_and_refusals():
+    say("haplotypes up to 2047 bases; 2048 refused")
+    rng = np.random.default_rng(4)
+    haps = [ACGT[rng.integers(0, 4, n)].tobytes() for n in (1, 2, 500, 2047, 2048)]
+    reads = [haps[2][10:160], haps[3][1900:2040], b"ACGTACGT", haps[3][:100], haps[4][:100], b"A", b"G" * 30]
+    read_hap = [2, 3, 0, 3, 4, 1, 3]
+    got = check(haps, reads, read_hap)
+    assert got[4] == (None, 1, None)
+
+
+def test_shared_haplotypes_and_batch_independence():
+    say("many reads per haplotype; results do not depend on the batch")
+    rng = np.random.default_rng(5)
+    haps = [ACGT[rng.integers(0, 4, int(rng.integers(300, 600)))].tobytes() for _ in range(8)]
+    reads, read_hap = [], []
+    for _ in range(400):
+        h = int(rng.integers(0, 8))
+        L = int(rng.choice([100, 150, 250]))
+        a = int(rng.integers(0, len(haps[h]) - L))
+        reads.append(_mutate(rng, haps[h][a:a + L], int(rng.integers(0, 5))) or b"T")
+        read_hap.append(h)
+    whole = check(haps, reads, read_hap)
+    part = rv.sw_align_batch(haps, reads[123:170], read_hap[123:170])
+    assert part == whole[123:170]
+
+
+def test_limits():
+    say("limit errors")
+    with pytest.raises(rv.EngineError) as e:
+        rv.sw_align_batch([b"A" * 1200], [b"A" * 1025], [0])
+    assert e.value.code == rv.RVC_E_LIMIT
+    with pytest.raises(rv.EngineError) as e:
+        rv.sw_align_batch([b"ACGT"], [b"AC"], [1])
+    assert e.value.code == rv.RVC_E_INVALID
