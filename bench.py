@@ -320,14 +320,18 @@ def run_sw(args):
            "parity": {"sample_pairs": len(sample), "mismatches_vs_oracle": mism}}
     if not args.no_cpu and oracle.ref_sw() is not None:
         cores = min(16, os.cpu_count() or 1)
-        n_cpu = min(len(reads), 60000)
-        cpu_cells = sum(len(haps[read_hap[i]]) * len(reads[i]) for i in range(n_cpu))
-        secs = oracle.ref_sw_align_many(haps, reads[:n_cpu], read_hap[:n_cpu], cores)
+        n_cpu = len(reads)
+        cpu_cells = cells
+        # all pairs, repeated for about 5 s; the median pass
+        passes = []
+        while len(passes) < 3 or (sum(passes) < 5.0 and len(passes) < 40):
+            passes.append(oracle.ref_sw_align_many(haps, reads, read_hap, cores))
+        secs = float(np.median(passes))
         out["cpu_baseline"] = {"value": round(cpu_cells / secs / 1e9, 2), "unit": "GCUPS", "cores": cores,
                                "kind": "reference",
-                               "sample": f"{n_cpu} of the pairs ({cpu_cells / 1e9:.1f} Gcells), Rovaca's aligner "
+                               "sample": f"all {n_cpu} pairs ({cpu_cells / 1e9:.1f} Gcells), Rovaca's aligner "
                                          f"(oracle/_ref/libref_sw.so, its AVX-512 16-bit core), {cores} threads, "
-                                         f"{secs:.1f} s wall"}
+                                         f"median of {len(passes)} passes, {secs:.3f} s each"}
         out["gpu_over_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
     return out
 

@@ -17,6 +17,7 @@ namespace rvc {
 namespace {
 
 constexpr uint64_t kBtBudget = 12ull << 30;  // backtrack bytes per device pass
+constexpr uint32_t kPackCols = 64;           // CIGAR elements per pair copied back packed
 
 // scores of Rovaca's two aligner instances (sw_avx.c:14-15)
 constexpr int32_t kGenotype[4] = {10, -15, -30, -5};
@@ -30,6 +31,7 @@ struct SwWorkspace {
   hipEvent_t ev[3] = {};
   void* mem = nullptr;
   size_t bytes = 0;
+  std::vector<uint32_t> host;  // CIGARs as copied back
   ~SwWorkspace() {
     if (mem) hipFree(mem);
     for (hipEvent_t e : ev)
@@ -139,7 +141,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
                  o_list = take(4ull * (list.size() + 1)), o_head = take(4 * kSwShapes), o_bt = take(bt_total),
                  o_hrow = take(4 * (hrow_total + 1)), o_hcol = take(4 * (hcol_total + 1)),
                  o_cig = take(4ull * n * max_cigar), o_cnt = take(4ull * n), o_off = take(4ull * n),
-                 o_sc = take(4ull * n);
+                 o_sc = take(4ull * n), o_pack = take(4ull * n * kPackCols);
     int rc = w.ensure(off);
     if (rc) return rc;
     char* base = static_cast<char*>(w.mem);
@@ -189,11 +191,34 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     if ((e = launch_sw_trace(st, a)) != hipSuccess)
       return set_error(RVC_E_DEVICE, "aligner traceback: %s", hipGetErrorString(e));
     if (hipEventRecord(w.ev[2], st) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");
-    e = hipMemcpyAsync(cigar + (size_t)r0 * max_cigar, base + o_cig, 4ull * n * max_cigar, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(count + r0, base + o_cnt, 4ull * n, hipMemcpyDeviceToHost, st);
+    // counts first, then of every pair's CIGAR slot only the columns the
+    // longest CIGAR of the chunk uses
+    e = hipMemcpyAsync(count + r0, base + o_cnt, 4ull * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(offset + r0, base + o_off, 4ull * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(score + r0, base + o_sc, 4ull * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
+    int32_t widest = 0;
+    for (uint32_t x = 0; x < n && e == hipSuccess; ++x) widest = std::max(widest, count[r0 + x]);
+    widest = std::min<int32_t>(widest, (int32_t)max_cigar);
+    if (e == hipSuccess && widest > 0) {
+      // the used columns packed on the device (pitch 'widest') when they fit
+      // the pack area, else the whole slots; one copy, then spread into the
+      // caller's slots
+      const bool packed = widest <= kPackCols;
+      const uint32_t pitch = packed ? (uint32_t)widest : max_cigar;
+      if (packed)
+        e = hipMemcpy2DAsync(base + o_pack, 4ull * pitch, base + o_cig, 4ull * max_cigar, 4ull * pitch, n,
+                             hipMemcpyDeviceToDevice, st);
+      w.host.resize((size_t)n * pitch);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(w.host.data(), base + (packed ? o_pack : o_cig), 4ull * n * pitch, hipMemcpyDeviceToHost,
+                           st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      for (uint32_t x = 0; x < n && e == hipSuccess; ++x)
+        if (count[r0 + x] > 0)
+          std::memcpy(cigar + (size_t)(r0 + x) * max_cigar, w.host.data() + (size_t)x * pitch,
+                      4ull * (uint32_t)count[r0 + x]);
+    }
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "aligner results: %s", hipGetErrorString(e));
     if (stats) {
       float f = 0, t = 0;

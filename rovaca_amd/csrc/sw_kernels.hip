@@ -69,11 +69,13 @@ __device__ __forceinline__ void sw_step(int (&Hp)[RPL], int (&F)[RPL], int (&Hn)
     const int ov = Hp[r] + oF[r], ev = F[r] + ext;
     const int f = max(ev, ov);
     const int m = dg + (hc == rc[r] ? match : mis[r]);
-    int h = max(cut, m);
-    int b = e > h ? 1 : 0;
-    h = max(h, e);
-    b = f > h ? 2 : b;
-    h = max(h, f);
+    const int h0 = max(cut, m);
+    // H = max(M, E, F); E comes from the row above in this column, so only
+    // the last max waits for it
+    const int h = max(max(h0, f), e);
+    const int h1 = max(h0, e);
+    int b = e > h0 ? 1 : 0;
+    b = f > h1 ? 2 : b;
     b |= (oh > eh ? 0 : 4) | (ov > ev ? 0 : 8);
     if ((r & 3) == 0) bt[r >> 2] = (uint32_t)b;
     else bt[r >> 2] |= (uint32_t)b << (8 * (r & 3));
@@ -137,9 +139,10 @@ __global__ __launch_bounds__(64) void k_sw_fill(SwArgs a) {
     int uH = 0, uE = kSwLow, dH = 0;
     const int own_steps = n1 + W - 1;
     const int nsteps = nmax + W - 1;
+    int hc = hb[seg][64 - k];  // this step's haplotype base, read a step ahead
     auto step = [&](int s, auto fill) {
       const int col = s - k + 1;  // haplotype column of this lane
-      const int hc = hb[seg][64 + s - k];
+      const int hc_next = hb[seg][64 + s + 1 - k];
       uint32_t bt[RPL / 4];
       int lastE;
       sw_step<RPL>(Hp, F, Hn, lastE, bt, uH, uE, dH, hc, rc, oE, oF, mis, a.match, a.extend, cut);
@@ -165,6 +168,7 @@ __global__ __launch_bounds__(64) void k_sw_fill(SwArgs a) {
       uE = down_or<W>(lastE, kSwLow, k);
 #pragma unroll
       for (int r = 0; r < RPL; ++r) Hp[r] = Hn[r];
+      hc = hc_next;
     };
     // the first W - 1 steps start lanes one by one; after them every lane
     // is past column 0
