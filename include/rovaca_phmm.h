@@ -138,10 +138,9 @@ typedef struct rvc_batch rvc_batch;
 typedef struct rvc_batch_opts {
   uint32_t persistent_waves_per_cu; /* 0 = as many as fit                             */
   uint32_t use_graph;               /* 1 = capture the launch sequence in a hipGraph  */
-  uint32_t fp32_fold;               /* 0 = tuned choice, 1 = rescale I/D (FP32: the   */
-                                    /* 5-operation scaled form, FP64: gap-to-match    */
-                                    /* folded in) where the gap qualities allow,      */
-                                    /* 2 = never                                      */
+  uint32_t fp32_fold;               /* 0 = tuned choice, 1 = rescale I/D (the         */
+                                    /* 5-operation scaled form, FP32 and FP64) where  */
+                                    /* the gap qualities allow, 2 = never             */
   uint32_t rpl_round;               /* 0 = tuned choice, 1 = exact rows per lane,     */
                                     /* 2 = rows per lane in steps of 3                */
   uint32_t occupancy;               /* 0 = tuned choice, else 2/3/4 waves per SIMD    */

@@ -174,7 +174,7 @@ FP32Policy& default_policy() {
 // Table format (text): lines "occ16 <17 ints>", "occpt16 <17 ints>",
 // "occpt32 <17 ints>", "occpt8 <17 ints>" (waves per SIMD), "shape16 <17
 // ints>" (0 or W*256+RPL, RPL 0 = fewest rows of width W),
-// "steps3 0|1", "fold 0|1", "table 0|1", "hap_run <1..64>"; '#' starts a
+// "steps3 0|1", "fold 0|1" (scaled forms allowed), "table 0|1", "hap_run <1..64>"; '#' starts a
 // comment.
 int load_tune_table(const char* path, FP32Policy* pol) {
   FILE* f = fopen(path, "r");

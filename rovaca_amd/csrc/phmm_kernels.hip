@@ -179,10 +179,10 @@ __device__ __forceinline__ void stage_hap(const DevSeqs& s, uint32_t hap, uint32
   }
 }
 
-// I = tMI*M_up + tII*I_up. In float the product off the row-to-row chain
-// (tMI*M_up) is rounded and tII*I_up is fused, so the chain down a lane's rows
-// is one FMA per row; in double the other way round keeps the FP64 kernels
-// within 256 registers (two waves per SIMD).
+// I = tMI*M_up + tII*I_up in the plain form. In float the product off the
+// row-to-row chain (tMI*M_up) is rounded and tII*I_up is fused, so the chain
+// down a lane's rows is one FMA per row; in double the other way round keeps
+// the plain FP64 kernels within 256 registers (two waves per SIMD).
 template <typename T>
 __device__ __forceinline__ T i_update(T tmi, T um, T tii, T ui);
 template <>
