@@ -100,7 +100,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     shape[r] = sw_shape(p.read_len, &W, &RPL);
     p.width = (uint16_t)W;
     p.rpl = (uint16_t)RPL;
-    bt_bytes[r] = up256((uint64_t)(p.hap_len + W - 1) * (uint64_t)(W * RPL));
+    bt_bytes[r] = up256((uint64_t)(p.hap_len + W - 1) * (uint64_t)(W * RPL / 2));  // a nibble per cell
     if (stats) stats->cells += (uint64_t)p.hap_len * p.read_len;
   }
   const uint64_t hap_bytes = (uint64_t)hap_off[n_haps], read_bytes = (uint64_t)read_off[n_reads];

@@ -33,7 +33,7 @@ inline int sw_shape(uint32_t n2, int* W, int* RPL) {
 struct SwPair {
   uint32_t hap_off, hap_len;
   uint32_t read_off, read_len;
-  uint64_t bt_off;      // backtrack bytes: (hap_len + W - 1) steps x W * RPL rows
+  uint64_t bt_off;      // backtrack nibbles: (hap_len + W - 1) steps x W * RPL rows
   uint32_t hrow_off;    // hap_len ints: H of the read's last row per column
   uint32_t hcol_off;    // W * RPL ints: H of the haplotype's last column per row
   uint16_t width, rpl;

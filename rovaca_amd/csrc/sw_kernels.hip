@@ -9,17 +9,19 @@
 //   F = max(F[i-1][j] + extend, H[i-1][j] + open)   (deletion, along a row)
 //   H = max(max(cut, H[i-1][j-1] + s(hap[i], read[j])), E, F)
 // with the reference's tie order (E replaces M only when greater, F replaces
-// both only when greater) and a backtrack byte: 0 match, 1 insertion,
-// 2 deletion, | 4 if E extended, | 8 if F extended. Row 0 and column 0 hold
-// H = 0 and no open gap.
+// both only when greater) and a backtrack nibble: bit 0 E > M (insertion),
+// bit 1 F > max(M, E) (deletion, wins over bit 0), bit 2 E was opened here
+// (not extended), bit 3 F was opened here. Row 0 and column 0 hold H = 0 and
+// no open gap.
 //
 // The sweep is the PairHMM one: a segment of W lanes takes one pair, lane k
 // holds read rows k*RPL .. k*RPL+RPL-1 and at step s computes haplotype
 // column s-k+1 of all of them; H and E of a lane's last row move one lane down
 // with DPP, F stays in the lane. Rows past the read carry H of the read's last
 // row down unchanged (E open 0, no match, no deletion), so the segment's last
-// row holds H[i][n2] for the best-end search. Backtrack bytes are stored per
-// step in lane order (coalesced); the traceback kernel maps (i, j) back.
+// row holds H[i][n2] for the best-end search. Backtrack nibbles are stored per
+// step in lane order (coalesced, 16 bits per 4 rows); the traceback kernel
+// maps (i, j) back.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -74,11 +76,15 @@ __device__ __forceinline__ void sw_step(int (&Hp)[RPL], int (&F)[RPL], int (&Hn)
     // the last max waits for it
     const int h = max(max(h0, f), e);
     const int h1 = max(h0, e);
-    int b = e > h0 ? 1 : 0;
-    b = f > h1 ? 2 : b;
-    b |= (oh > eh ? 0 : 4) | (ov > ev ? 0 : 8);
-    if ((r & 3) == 0) bt[r >> 2] = (uint32_t)b;
-    else bt[r >> 2] |= (uint32_t)b << (8 * (r & 3));
+    // the four decisions as sign bits of differences (all well inside int32),
+    // shifted into the group's 16 bits: row q of a 4-row group ends at bits
+    // 4 * (3 - q) .. 4 * (3 - q) + 3
+    uint32_t acc = (r & 3) == 0 ? 0u : bt[r >> 2];
+    acc = __builtin_amdgcn_alignbit(acc, (uint32_t)(ev - ov), 31);  // bit 3: F opened (ov > ev)
+    acc = __builtin_amdgcn_alignbit(acc, (uint32_t)(eh - oh), 31);  // bit 2: E opened (oh > eh)
+    acc = __builtin_amdgcn_alignbit(acc, (uint32_t)(h1 - f), 31);   // bit 1: f > max(M, E)
+    acc = __builtin_amdgcn_alignbit(acc, (uint32_t)(h0 - e), 31);   // bit 0: e > M
+    bt[r >> 2] = acc;
     dg = Hp[r];
     Hn[r] = h;
     F[r] = f;
@@ -133,7 +139,7 @@ __global__ __launch_bounds__(64) void k_sw_fill(SwArgs a) {
     // sw_internal.h:19-22); its 32-bit core at -100000000
     const bool use32 = n2 < 16 || a.match <= 0 || (long long)(n1 < n2 ? n1 : n2) * a.match > 32767;
     const int cut = use32 ? -100000000 : -8192;
-    uint8_t* btp = a.bt + pr.bt_off + (uint32_t)(k * RPL);
+    uint8_t* btp = a.bt + pr.bt_off + (uint32_t)(k * RPL / 2);
     int32_t* hrow = a.hrow + pr.hrow_off;
     int32_t* hcol = a.hcol + pr.hcol_off;
     int uH = 0, uE = kSwLow, dH = 0;
@@ -154,9 +160,9 @@ __global__ __launch_bounds__(64) void k_sw_fill(SwArgs a) {
         }
       }
       if (active && s < own_steps) {
-        uint32_t* dst = reinterpret_cast<uint32_t*>(btp + (size_t)s * ROWS);
+        uint16_t* dst = reinterpret_cast<uint16_t*>(btp + (size_t)s * (ROWS / 2));
 #pragma unroll
-        for (int q = 0; q < RPL / 4; ++q) dst[q] = bt[q];
+        for (int q = 0; q < RPL / 4; ++q) dst[q] = (uint16_t)bt[q];
         if (col == n1) {
 #pragma unroll
           for (int r = 0; r < RPL; ++r) hcol[k * RPL + r] = Hn[r];
@@ -209,7 +215,7 @@ __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
   }
   a.score[p] = best;
   const int rows = (int)pr.width * (int)pr.rpl;
-  const uint8_t* bt = a.bt + pr.bt_off;
+  const uint16_t* bt = reinterpret_cast<const uint16_t*>(a.bt + pr.bt_off);
   // runs end to start, merged as they come
   int n = 0, cur_op = -1;
   uint32_t cur_len = 0;
@@ -229,21 +235,32 @@ __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
   int state = 0;
   while (i > 0 && j > 0) {
     const int kk = (j - 1) / (int)pr.rpl;
-    const int b = bt[(size_t)(i - 1 + kk) * rows + (j - 1)];
+    const size_t nib = (size_t)(i - 1 + kk) * rows + (j - 1);
+    const int v = (bt[nib >> 2] >> (4 * (3 - (int)(nib & 3)))) & 15;
+    // state 4: inside an insertion, 8: inside a deletion (the gap goes on
+    // where this cell's E / F was extended, not opened)
+    const int ins_ext = (v & 4) ? 0 : 4, del_ext = (v & 8) ? 0 : 8;
     if (state == 4) {
       --j;
       push(kSwOpI, 1);
-      state = b & 4;
+      state = ins_ext;
     } else if (state == 8) {
       --i;
       push(kSwOpD, 1);
-      state = b & 8;
+      state = del_ext;
+    } else if (v & 2) {
+      --i;
+      push(kSwOpD, 1);
+      state = del_ext;
+    } else if (v & 1) {
+      --j;
+      push(kSwOpI, 1);
+      state = ins_ext;
     } else {
-      switch (b & 3) {
-        case 0: --i; --j; push(kSwOpM, 1); state = 0; break;
-        case 1: --j; push(kSwOpI, 1); state = b & 4; break;
-        default: --i; push(kSwOpD, 1); state = b & 8; break;
-      }
+      --i;
+      --j;
+      push(kSwOpM, 1);
+      state = 0;
     }
   }
   if (j > 0) push(kSwOpS, (uint32_t)j);
