@@ -397,40 +397,60 @@ class RvcSwStats(ctypes.Structure):
 _SW_OPS = "MIDNSHP=X"
 
 
+class SwBatch:
+    """A batch of reads to align against their haplotypes on the device,
+    packed once, with output arrays kept for repeated runs (a caller that
+    aligns batch after batch reuses its result buffers the same way)."""
+
+    def __init__(self, haps, reads, read_hap, assemble=False):
+        self.hap_off = np.zeros(len(haps) + 1, np.int32)
+        self.hap_off[1:] = np.cumsum([len(h) for h in haps])
+        self.read_off = np.zeros(len(reads) + 1, np.int32)
+        self.read_off[1:] = np.cumsum([len(r) for r in reads])
+        self.hb = np.frombuffer(b"".join(bytes(h) for h in haps) + b"\0", np.uint8)
+        self.rb = np.frombuffer(b"".join(bytes(r) for r in reads) + b"\0", np.uint8)
+        self.rh = np.ascontiguousarray(read_hap, np.uint32)
+        self.n_haps, self.n = len(haps), len(reads)
+        self.assemble = 1 if assemble else 0
+        self.max_cigar = 2 * max([len(r) for r in reads] + [1]) + 2
+        self.cig = np.zeros(self.n * self.max_cigar, np.uint32)
+        self.cnt = np.zeros(self.n, np.int32)
+        self.off = np.zeros(self.n, np.int32)
+        self.sc = np.zeros(self.n, np.int32)
+
+    def run(self):
+        """Aligns every read; returns the call's statistics as a dict."""
+        fn = lib().rvc_sw_align_batch
+        fn.argtypes = [_u8p, _i32p, ctypes.c_uint32, _u8p, _i32p, ctypes.c_uint32, _u32p, ctypes.c_int32,
+                       ctypes.c_uint32, _u32p, _i32p, _i32p, _i32p, ctypes.POINTER(RvcSwStats)]
+        st = RvcSwStats()
+        _check(fn(self.hb.ctypes.data_as(_u8p), self.hap_off.ctypes.data_as(_i32p), self.n_haps,
+                  self.rb.ctypes.data_as(_u8p), self.read_off.ctypes.data_as(_i32p), self.n,
+                  self.rh.ctypes.data_as(_u32p), self.assemble, self.max_cigar, self.cig.ctypes.data_as(_u32p),
+                  self.cnt.ctypes.data_as(_i32p), self.off.ctypes.data_as(_i32p), self.sc.ctypes.data_as(_i32p),
+                  ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in RvcSwStats._fields_}
+
+    def results(self):
+        """[(cigar [(op, length)], offset, score)] per read of the last run,
+        (None, 1, None) where the aligner refuses the pair."""
+        out = []
+        for r in range(self.n):
+            k = int(self.cnt[r])
+            if k < 0:
+                out.append((None, int(self.off[r]), None))
+                continue
+            c = self.cig[r * self.max_cigar:r * self.max_cigar + k]
+            out.append(([(_SW_OPS[int(x) & 15], int(x) >> 4) for x in c], int(self.off[r]), int(self.sc[r])))
+        return out
+
+
 def sw_align_batch(haps, reads, read_hap, assemble=False, with_stats=False):
     """Aligns reads[r] against haps[read_hap[r]] on the device (the aligner of
     Rovaca's read realignment). Returns [(cigar [(op, length)], offset, score)]
     per read, (None, 1, None) where the aligner refuses the pair; with
     with_stats, also the call's statistics as a dict."""
-    L = lib()
-    fn = L.rvc_sw_align_batch
-    fn.argtypes = [_u8p, _i32p, ctypes.c_uint32, _u8p, _i32p, ctypes.c_uint32, _u32p, ctypes.c_int32, ctypes.c_uint32,
-                   _u32p, _i32p, _i32p, _i32p, ctypes.POINTER(RvcSwStats)]
-    hap_off = np.zeros(len(haps) + 1, np.int32)
-    hap_off[1:] = np.cumsum([len(h) for h in haps])
-    read_off = np.zeros(len(reads) + 1, np.int32)
-    read_off[1:] = np.cumsum([len(r) for r in reads])
-    hb = np.frombuffer(b"".join(bytes(h) for h in haps) + b"\0", np.uint8)
-    rb = np.frombuffer(b"".join(bytes(r) for r in reads) + b"\0", np.uint8)
-    rh = np.ascontiguousarray(read_hap, np.uint32)
-    n = len(reads)
-    max_cigar = 2 * max([len(r) for r in reads] + [1]) + 2
-    cig = np.zeros(n * max_cigar, np.uint32)
-    cnt = np.zeros(n, np.int32)
-    off = np.zeros(n, np.int32)
-    sc = np.zeros(n, np.int32)
-    st = RvcSwStats()
-    _check(fn(hb.ctypes.data_as(_u8p), hap_off.ctypes.data_as(_i32p), len(haps), rb.ctypes.data_as(_u8p),
-              read_off.ctypes.data_as(_i32p), n, rh.ctypes.data_as(_u32p), 1 if assemble else 0, max_cigar,
-              cig.ctypes.data_as(_u32p), cnt.ctypes.data_as(_i32p), off.ctypes.data_as(_i32p),
-              sc.ctypes.data_as(_i32p), ctypes.byref(st)))
-    out = []
-    for r in range(n):
-        if cnt[r] < 0:
-            out.append((None, int(off[r]), None))
-            continue
-        c = cig[r * max_cigar:r * max_cigar + cnt[r]]
-        out.append(([(_SW_OPS[int(x) & 15], int(x) >> 4) for x in c], int(off[r]), int(sc[r])))
-    if with_stats:
-        return out, {f: getattr(st, f) for f, _ in RvcSwStats._fields_}
-    return out
+    b = SwBatch(haps, reads, read_hap, assemble)
+    st = b.run()
+    out = b.results()
+    return (out, st) if with_stats else out

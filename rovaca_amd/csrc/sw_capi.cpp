@@ -124,16 +124,18 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
       ++r1;
     }
     const uint32_t n = r1 - r0;
-    // pair lists by shape
-    std::vector<uint32_t> list;
-    list.reserve(n);
+    // pair lists by shape (a counting sort, stable within a shape)
     uint32_t start[kSwShapes + 1] = {};
-    for (int s = 0; s < kSwShapes; ++s) {
-      start[s] = (uint32_t)list.size();
+    for (uint32_t x = 0; x < n; ++x)
+      if (shape[r0 + x] >= 0) ++start[shape[r0 + x] + 1];
+    for (int s = 0; s < kSwShapes; ++s) start[s + 1] += start[s];
+    std::vector<uint32_t> list(start[kSwShapes]);
+    {
+      uint32_t fill[kSwShapes];
+      std::copy(start, start + kSwShapes, fill);
       for (uint32_t x = 0; x < n; ++x)
-        if (shape[r0 + x] == s) list.push_back(x);
+        if (shape[r0 + x] >= 0) list[fill[shape[r0 + x]]++] = x;
     }
-    start[kSwShapes] = (uint32_t)list.size();
     // device layout
     size_t off = 0;
     auto take = [&](uint64_t b) { const size_t o = off; off += up256(b); return o; };
