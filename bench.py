@@ -312,7 +312,9 @@ def run_sw(args):
     mism = sum(1 for i in sample if res[i] != oracle.sw_align(haps[read_hap[i]], reads[i]))
     out = {"metric": "Smith-Waterman realignment GCUPS (reads vs best haplotype, 1 MI355X)",
            "value": round(cells / (dev_ms * 1e-3) / 1e9, 1), "unit": "GCUPS", "higher_is_better": True, "n_gpus": 1,
-           "dtype": "int32", "data": "synthetic",
+           # genotype scores on reads of 16+ bases: the packed int16 sweep (two
+           # pairs per lane); shorter reads take the int32 sweep
+           "dtype": "int16", "data": "synthetic",
            "config": {"workload": f"sw_realign_{len(regions)}_regions", "pairs": len(reads), "cells": cells,
                       "scores": "genotype 10/-15/-30/-5"},
            "device_ms": {"sweep": round(best["fill_ms"], 3), "traceback": round(best["trace_ms"], 3)},

@@ -1,7 +1,9 @@
 // Batched Smith-Waterman of reads against haplotypes: the C ABI of
-// include/rovaca_sw.h. Pairs are grouped by sweep shape (read length),
-// swept by k_sw_fill (one launch per shape present) and traced by
-// k_sw_trace; batches whose backtrack bytes exceed kBtBudget run in chunks.
+// include/rovaca_sw.h. Pairs are grouped by sweep shape (read length); the
+// pairs sw_narrow() admits are ordered by haplotype length and swept two per
+// lane by k_sw_fill16 (duos of neighbours in that order), the rest by
+// k_sw_fill, one launch per shape and sweep present; k_sw_trace traces all.
+// Batches whose backtrack bytes exceed kBtBudget run in chunks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,6 +34,7 @@ struct SwWorkspace {
   void* mem = nullptr;
   size_t bytes = 0;
   std::vector<uint32_t> host;  // CIGARs as copied back
+  std::vector<uint32_t> keys, ord, list;
   ~SwWorkspace() {
     if (mem) hipFree(mem);
     for (hipEvent_t e : ev)
@@ -85,7 +88,8 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
   // one SwPair per read; refused pairs stay out of the sweeps
   std::vector<SwPair> pairs(n_reads);
   std::vector<int> shape(n_reads, -1);
-  std::vector<uint64_t> bt_bytes(n_reads, 0);
+  std::vector<uint8_t> narrow(n_reads, 0);
+  std::vector<uint64_t> bt_est(n_reads, 0);  // bounds the backtrack bytes a pair adds, alone or in a duo
   for (uint32_t r = 0; r < n_reads; ++r) {
     SwPair& p = pairs[r];
     std::memset(&p, 0, sizeof p);
@@ -100,7 +104,12 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     shape[r] = sw_shape(p.read_len, &W, &RPL);
     p.width = (uint16_t)W;
     p.rpl = (uint16_t)RPL;
-    bt_bytes[r] = up256((uint64_t)(p.hap_len + W - 1) * (uint64_t)(W * RPL / 2));  // a nibble per cell
+    bt_est[r] = up256((uint64_t)(p.hap_len + W - 1) * (uint64_t)(W * RPL / 2));  // a nibble per cell
+    if (sw_narrow(p.hap_len, p.read_len, sc[0], sc[1], sc[2], sc[3])) {
+      // a duo's steps run to its longer haplotype: at most twice its own share
+      narrow[r] = 1;
+      bt_est[r] *= 2;
+    }
     if (stats) stats->cells += (uint64_t)p.hap_len * p.read_len;
   }
   const uint64_t hap_bytes = (uint64_t)hap_off[n_haps], read_bytes = (uint64_t)read_off[n_reads];
@@ -110,13 +119,12 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
   while (r0 < n_reads) {
     // the chunk: reads r0 .. r1 whose backtrack bytes fit the budget
     uint32_t r1 = r0;
-    uint64_t bt_total = 0, hrow_total = 0, hcol_total = 0;
-    while (r1 < n_reads && (r1 == r0 || bt_total + bt_bytes[r1] <= kBtBudget)) {
+    uint64_t est = 0, hrow_total = 0, hcol_total = 0;
+    while (r1 < n_reads && (r1 == r0 || est + bt_est[r1] <= kBtBudget)) {
       SwPair& p = pairs[r1];
-      p.bt_off = bt_total;
       p.hrow_off = (uint32_t)hrow_total;
       p.hcol_off = (uint32_t)hcol_total;
-      bt_total += bt_bytes[r1];
+      est += bt_est[r1];
       if (shape[r1] >= 0) {
         hrow_total += p.hap_len;
         hcol_total += (uint64_t)p.width * p.rpl;
@@ -124,23 +132,78 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
       ++r1;
     }
     const uint32_t n = r1 - r0;
-    // pair lists by shape (a counting sort, stable within a shape)
-    uint32_t start[kSwShapes + 1] = {};
-    for (uint32_t x = 0; x < n; ++x)
-      if (shape[r0 + x] >= 0) ++start[shape[r0 + x] + 1];
-    for (int s = 0; s < kSwShapes; ++s) start[s + 1] += start[s];
-    std::vector<uint32_t> list(start[kSwShapes]);
-    {
-      uint32_t fill[kSwShapes];
-      std::copy(start, start + kSwShapes, fill);
-      for (uint32_t x = 0; x < n; ++x)
-        if (shape[r0 + x] >= 0) list[fill[shape[r0 + x]]++] = x;
+    // order (a counting sort): packed-sweep pairs by shape and haplotype
+    // length, then int32-sweep pairs by shape
+    constexpr uint32_t kKeys = kSwShapes * kSwMaxLen + kSwShapes;
+    auto key = [&](uint32_t x) -> uint32_t {
+      const uint32_t r = r0 + x;
+      if (shape[r] < 0) return kKeys;
+      return narrow[r] ? (uint32_t)shape[r] * kSwMaxLen + pairs[r].hap_len : kSwShapes * kSwMaxLen + (uint32_t)shape[r];
+    };
+    std::vector<uint32_t>& cnt = w.keys;
+    cnt.assign(kKeys + 1, 0);
+    for (uint32_t x = 0; x < n; ++x) {
+      const uint32_t k = key(x);
+      if (k < kKeys) ++cnt[k + 1];
+    }
+    for (uint32_t k = 0; k < kKeys; ++k) cnt[k + 1] += cnt[k];
+    std::vector<uint32_t>& ord = w.ord;
+    ord.resize(cnt[kKeys]);
+    for (uint32_t x = 0; x < n; ++x) {
+      const uint32_t k = key(x);
+      if (k < kKeys) ord[cnt[k]++] = x;
+    }
+    // work lists and backtrack areas: duos of neighbours in that order (two
+    // entries each; a lone last pair gets kSwNoPair), then single pairs
+    std::vector<uint32_t>& list = w.list;
+    list.clear();
+    uint32_t dstart[kSwShapes] = {}, dcount[kSwShapes] = {}, sstart[kSwShapes] = {}, scount[kSwShapes] = {};
+    uint64_t bt_total = 0;
+    size_t i = 0;
+    const size_t m = ord.size();
+    for (int s = 0; s < kSwShapes; ++s) {
+      const uint64_t rows = (uint64_t)kSwW[s] * (uint64_t)kSwR[s];
+      dstart[s] = (uint32_t)list.size();
+      while (i < m && narrow[r0 + ord[i]] && shape[r0 + ord[i]] == s) {
+        const uint32_t xa = ord[i++];
+        const uint32_t xb = (i < m && narrow[r0 + ord[i]] && shape[r0 + ord[i]] == s) ? ord[i++] : kSwNoPair;
+        SwPair& pa = pairs[r0 + xa];
+        uint32_t n1 = pa.hap_len;
+        pa.bt_off = bt_total;
+        pa.bt_lanes = 2;
+        pa.bt_half = 0;
+        if (xb != kSwNoPair) {
+          SwPair& pb = pairs[r0 + xb];
+          n1 = std::max(n1, pb.hap_len);
+          pb.bt_off = bt_total;
+          pb.bt_lanes = 2;
+          pb.bt_half = 1;
+        }
+        bt_total += up256((uint64_t)(n1 + kSwW[s] - 1) * rows);  // two nibbles per row and step
+        list.push_back(xa);
+        list.push_back(xb);
+        ++dcount[s];
+      }
+    }
+    for (int s = 0; s < kSwShapes; ++s) {
+      const uint64_t rows = (uint64_t)kSwW[s] * (uint64_t)kSwR[s];
+      sstart[s] = (uint32_t)list.size();
+      while (i < m && !narrow[r0 + ord[i]] && shape[r0 + ord[i]] == s) {
+        const uint32_t x = ord[i++];
+        SwPair& p = pairs[r0 + x];
+        p.bt_off = bt_total;
+        p.bt_lanes = 1;
+        p.bt_half = 0;
+        bt_total += up256((uint64_t)(p.hap_len + kSwW[s] - 1) * rows / 2);
+        list.push_back(x);
+        ++scount[s];
+      }
     }
     // device layout
     size_t off = 0;
     auto take = [&](uint64_t b) { const size_t o = off; off += up256(b); return o; };
     const size_t o_hap = take(hap_bytes + 64), o_read = take(read_bytes + 64), o_pair = take(sizeof(SwPair) * n),
-                 o_list = take(4ull * (list.size() + 1)), o_head = take(4 * kSwShapes), o_bt = take(bt_total),
+                 o_list = take(4ull * (list.size() + 1)), o_head = take(4 * 2 * kSwShapes), o_bt = take(bt_total),
                  o_hrow = take(4 * (hrow_total + 1)), o_hcol = take(4 * (hcol_total + 1)),
                  o_cig = take(4ull * n * max_cigar), o_cnt = take(4ull * n), o_off = take(4ull * n),
                  o_sc = take(4ull * n), o_pack = take(4ull * n * kPackCols);
@@ -153,7 +216,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     if (e == hipSuccess) e = hipMemcpyAsync(base + o_pair, pairs.data() + r0, sizeof(SwPair) * n, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && !list.empty())
       e = hipMemcpyAsync(base + o_list, list.data(), 4 * list.size(), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemsetAsync(base + o_head, 0, 4 * kSwShapes, st);
+    if (e == hipSuccess) e = hipMemsetAsync(base + o_head, 0, 4 * 2 * kSwShapes, st);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "aligner upload: %s", hipGetErrorString(e));
     SwArgs a{};
     a.haps = reinterpret_cast<const uint8_t*>(base + o_hap);
@@ -173,19 +236,22 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     a.offset = reinterpret_cast<int32_t*>(base + o_off);
     a.score = reinterpret_cast<int32_t*>(base + o_sc);
     if (hipEventRecord(w.ev[0], st) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");
-    for (int s = 0; s < kSwShapes; ++s) {
-      const uint32_t nl = start[s + 1] - start[s];
-      if (nl == 0) continue;
-      a.list = reinterpret_cast<const uint32_t*>(base + o_list) + start[s];
-      a.n_list = nl;
-      a.head = reinterpret_cast<uint32_t*>(base + o_head) + s;
-      const uint32_t nseg = 64u / (uint32_t)kSwW[s];
-      const uint32_t waves = (nl + nseg - 1) / nseg;
-      const int grid = (int)std::max<uint32_t>(
-          1, std::min<uint32_t>(waves, (uint32_t)sw_blocks_per_cu(s) * (uint32_t)g_engine.n_cu));
-      if ((e = launch_sw_fill(s, grid, st, a)) != hipSuccess)
-        return set_error(RVC_E_DEVICE, "aligner sweep: %s", hipGetErrorString(e));
-      if (stats) ++stats->launches;
+    // packed sweeps (a work item is a duo), then int32 sweeps
+    for (int packed = 1; packed >= 0; --packed) {
+      for (int s = 0; s < kSwShapes; ++s) {
+        const uint32_t nl = packed ? dcount[s] : scount[s];
+        if (nl == 0) continue;
+        a.list = reinterpret_cast<const uint32_t*>(base + o_list) + (packed ? dstart[s] : sstart[s]);
+        a.n_list = nl;
+        a.head = reinterpret_cast<uint32_t*>(base + o_head) + (packed ? s : kSwShapes + s);
+        const uint32_t nseg = 64u / (uint32_t)kSwW[s];
+        const uint32_t waves = (nl + nseg - 1) / nseg;
+        const int per_cu = packed ? sw16_blocks_per_cu(s) : sw_blocks_per_cu(s);
+        const int grid = (int)std::max<uint32_t>(1, std::min<uint32_t>(waves, (uint32_t)per_cu * (uint32_t)g_engine.n_cu));
+        if ((e = packed ? launch_sw_fill16(s, grid, st, a) : launch_sw_fill(s, grid, st, a)) != hipSuccess)
+          return set_error(RVC_E_DEVICE, "aligner sweep: %s", hipGetErrorString(e));
+        if (stats) ++stats->launches;
+      }
     }
     a.list = nullptr;
     a.n_list = 0;

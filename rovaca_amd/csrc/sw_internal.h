@@ -13,6 +13,20 @@ constexpr int kSwShapes = 8;          // (W, RPL): see sw_shape()
 constexpr int kSwLow = -(1 << 28);    // "no gap" at row and column 0
 constexpr int kSwBig = -(1 << 26);    // disables a move in the rows past the read
 constexpr uint32_t kSwOpM = 0, kSwOpI = 1, kSwOpD = 2, kSwOpS = 4;  // BAM CIGAR operations
+constexpr uint32_t kSwNoPair = 0xffffffffu;  // the empty half of a duo
+
+// The packed int16 sweep (two pairs per lane, one in each 16-bit half) takes
+// pairs on the 16-bit core whose scores stay small: every H is in
+// [-8192, kSwNarrowMax], every E and F above -8192 - 300, so with the
+// sentinels below all sums and differences fit in int16.
+constexpr int kSwNarrowMax = 12000;
+constexpr int kSwLow16 = -16384;  // "no gap" at row and column 0
+constexpr int kSwBig16 = -10240;  // disables a move in the rows past the read
+inline bool sw_narrow(uint32_t n1, uint32_t n2, int match, int mismatch, int open, int extend) {
+  const uint32_t mn = n1 < n2 ? n1 : n2;
+  return n2 >= 16 && match > 0 && (int64_t)mn * match <= kSwNarrowMax && mismatch <= 0 && mismatch >= -300 &&
+         open <= 0 && open >= -300 && extend <= 0 && extend >= -300;
+}
 
 // Sweep shapes (segment width W, rows per lane RPL): 64 .. 1024 rows. Up to
 // 12 rows per lane a kernel keeps two waves per SIMD.
@@ -37,7 +51,9 @@ struct SwPair {
   uint32_t hrow_off;    // hap_len ints: H of the read's last row per column
   uint32_t hcol_off;    // W * RPL ints: H of the haplotype's last column per row
   uint16_t width, rpl;
-  uint32_t pad;
+  uint8_t bt_lanes;     // 1: the pair's own 16 bits per 4 rows; 2: 32-bit words shared with a duo partner
+  uint8_t bt_half;      // with bt_lanes 2: 0 the low, 1 the high 16 bits
+  uint16_t pad;
 };
 
 struct SwArgs {
@@ -45,8 +61,8 @@ struct SwArgs {
   const uint8_t* reads;
   const SwPair* pairs;
   uint32_t n_pairs;
-  const uint32_t* list;  // pairs of this launch's shape
-  uint32_t n_list;
+  const uint32_t* list;  // pairs of this launch's shape (duos: two entries each)
+  uint32_t n_list;       // pairs (duos)
   uint32_t* head;        // this launch's work-queue counter
   uint8_t* bt;
   int32_t* hrow;
@@ -60,7 +76,9 @@ struct SwArgs {
 };
 
 hipError_t launch_sw_fill(int shape, int grid, hipStream_t stream, const SwArgs& a);
+hipError_t launch_sw_fill16(int shape, int grid, hipStream_t stream, const SwArgs& a);
 hipError_t launch_sw_trace(hipStream_t stream, const SwArgs& a);
 int sw_blocks_per_cu(int shape);
+int sw16_blocks_per_cu(int shape);
 
 }  // namespace rvc

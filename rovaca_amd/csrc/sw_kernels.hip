@@ -184,6 +184,168 @@ __global__ __launch_bounds__(64) void k_sw_fill(SwArgs a) {
   }
 }
 
+// ---- the packed int16 sweep: two pairs per lane ---------------------------
+// The same recurrence, tie order and backtrack nibbles on pairs that
+// sw_narrow() admits, with pair A in the low and pair B in the high 16 bits
+// of every register (v_pk_* operations). The match score is
+// (dg + match) + min(hc ^ rc, 1) * (mismatch - match), byte equality as in
+// the int32 sweep; rows past a read hold rc = 0x1ff (never equal), mismatch
+// and deletion open kSwBig16, insertion open 0. Each 4-row group leaves one
+// 32-bit word per step: A's 16 bits low, B's high.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ s16x2 pmax(s16x2 a, s16x2 b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ uint32_t bits(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ s16x2 halves(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
+__device__ __forceinline__ s16x2 splat(int v) { return s16x2{(short)v, (short)v}; }
+// the sign bits of both halves at bits 0 and 16
+__device__ __forceinline__ uint32_t signs(s16x2 d) {
+  return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, d) >> (u16x2)15);
+}
+
+template <int RPL>
+__device__ __forceinline__ void sw_step16(s16x2 (&Hp)[RPL], s16x2 (&F)[RPL], s16x2 (&Hn)[RPL], s16x2& lastE,
+                                          uint32_t (&bt)[RPL / 4], s16x2 uH, s16x2 uE, s16x2 dH, u16x2 hc,
+                                          const u16x2 (&rc)[RPL], const s16x2 (&oE)[RPL], const s16x2 (&oF)[RPL],
+                                          const s16x2 (&dl)[RPL], s16x2 match, s16x2 ext, s16x2 cut, u16x2 one) {
+  s16x2 aH = uH, aE = uE, dg = dH;
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const s16x2 oh = aH + oE[r], eh = aE + ext;
+    const s16x2 e = pmax(oh, eh);
+    const s16x2 ov = Hp[r] + oF[r], ev = F[r] + ext;
+    const s16x2 f = pmax(ev, ov);
+    const s16x2 t = __builtin_bit_cast(s16x2, __builtin_elementwise_min(hc ^ rc[r], one));
+    const s16x2 m = t * dl[r] + (dg + match);
+    const s16x2 h0 = pmax(cut, m);
+    const s16x2 h1 = pmax(h0, e);
+    const s16x2 h = pmax(h1, f);
+    // bit order as in sw_step; each shift moves both halves' bits up by one
+    uint32_t acc = (r & 3) == 0 ? 0u : bt[r >> 2];
+    acc = (acc << 1) | signs(ev - ov);
+    acc = (acc << 1) | signs(eh - oh);
+    acc = (acc << 1) | signs(h1 - f);
+    acc = (acc << 1) | signs(h0 - e);
+    bt[r >> 2] = acc;
+    dg = Hp[r];
+    Hn[r] = h;
+    F[r] = f;
+    aH = h;
+    aE = e;
+  }
+  lastE = aE;
+}
+
+template <int W, int RPL>
+__global__ __launch_bounds__(64) void k_sw_fill16(SwArgs a) {
+  constexpr int NSEG = 64 / W;
+  constexpr int ROWS = W * RPL;
+  // haplotype bases of the duo, A in the low and B in the high byte
+  __shared__ uint16_t hb[NSEG][kHapRow];
+  const int lane = threadIdx.x;
+  const int seg = lane / W, k = lane % W;
+  const s16x2 low = splat(kSwLow16);
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(a.head, (uint32_t)NSEG);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= a.n_list) break;
+    const bool active = t + (uint32_t)seg < a.n_list;
+    const uint32_t d = active ? t + seg : t;
+    const uint32_t pa = a.list[2 * d], pb0 = a.list[2 * d + 1];
+    const bool has_b = active && pb0 != kSwNoPair;
+    const uint32_t pb = pb0 != kSwNoPair ? pb0 : pa;
+    const SwPair A = a.pairs[pa], B = a.pairs[pb];
+    const int n1a = (int)A.hap_len, n1b = (int)B.hap_len;
+    const int n2a = (int)A.read_len, n2b = (int)B.read_len;
+    const int n1 = n1a > n1b ? n1a : n1b;
+    int nmax = n1;
+#pragma unroll
+    for (int g = 0; g < NSEG; ++g) {
+      const int o = __shfl(n1, g * W);
+      nmax = o > nmax ? o : nmax;
+    }
+    __syncthreads();
+    for (int x = k; x < kHapRow; x += W) {
+      const int i = x - 64;
+      const uint32_t ba = (i >= 0 && i < n1a) ? a.haps[A.hap_off + i] : 0u;
+      const uint32_t bb = (i >= 0 && i < n1b) ? a.haps[B.hap_off + i] : 0u;
+      hb[seg][x] = (uint16_t)(ba | bb << 8);
+    }
+    __syncthreads();
+    u16x2 rc[RPL];
+    s16x2 oE[RPL], oF[RPL], dl[RPL], Hp[RPL], F[RPL], Hn[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      const int j = k * RPL + r;
+      const bool ra = j < n2a, rb = j < n2b;
+      rc[r] = u16x2{(unsigned short)(ra ? a.reads[A.read_off + j] : 0x1ff),
+                    (unsigned short)(rb ? a.reads[B.read_off + j] : 0x1ff)};
+      oE[r] = s16x2{(short)(ra ? a.open : 0), (short)(rb ? a.open : 0)};
+      oF[r] = s16x2{(short)(ra ? a.open : kSwBig16), (short)(rb ? a.open : kSwBig16)};
+      dl[r] = s16x2{(short)((ra ? a.mismatch : kSwBig16) - a.match), (short)((rb ? a.mismatch : kSwBig16) - a.match)};
+      Hp[r] = splat(0);
+      F[r] = low;
+    }
+    const s16x2 match = splat(a.match), ext = splat(a.extend), cut = splat(-8192);
+    // 1 in both halves, kept opaque: min(x, 1) stays one v_pk_min_u16 rather
+    // than a compare and select per half
+    uint32_t one_bits = 0x00010001u;
+    asm volatile("" : "+v"(one_bits));
+    const u16x2 one = __builtin_bit_cast(u16x2, one_bits);
+    uint32_t* btp = reinterpret_cast<uint32_t*>(a.bt + A.bt_off) + k * (RPL / 4);
+    int32_t* hrow_a = a.hrow + A.hrow_off;
+    int32_t* hcol_a = a.hcol + A.hcol_off;
+    int32_t* hrow_b = a.hrow + B.hrow_off;
+    int32_t* hcol_b = a.hcol + B.hcol_off;
+    s16x2 uH = splat(0), uE = low, dH = splat(0);
+    const int own_steps = n1 + W - 1;
+    const int nsteps = nmax + W - 1;
+    // this step's haplotype bases (read a step ahead), one per half
+    uint32_t hc = __builtin_amdgcn_perm(0u, (uint32_t)hb[seg][64 - k], 0x0c010c00u);
+    auto step = [&](int s, auto fill) {
+      const int col = s - k + 1;
+      const uint32_t hc_next = __builtin_amdgcn_perm(0u, (uint32_t)hb[seg][64 + s + 1 - k], 0x0c010c00u);
+      uint32_t bt[RPL / 4];
+      s16x2 lastE;
+      sw_step16<RPL>(Hp, F, Hn, lastE, bt, uH, uE, dH, __builtin_bit_cast(u16x2, hc), rc, oE, oF, dl, match, ext, cut, one);
+      if constexpr (decltype(fill)::value) {
+        if (col < 1) {
+#pragma unroll
+          for (int r = 0; r < RPL; ++r) { Hn[r] = splat(0); F[r] = low; }
+        }
+      }
+      if (active && s < own_steps) {
+        uint32_t* dst = btp + (size_t)s * (ROWS / 4);
+#pragma unroll
+        for (int q = 0; q < RPL / 4; ++q) dst[q] = bt[q];
+        if (col == n1a) {
+#pragma unroll
+          for (int r = 0; r < RPL; ++r) hcol_a[k * RPL + r] = Hn[r].x;
+        }
+        if (has_b && col == n1b) {
+#pragma unroll
+          for (int r = 0; r < RPL; ++r) hcol_b[k * RPL + r] = Hn[r].y;
+        }
+        if (k == W - 1 && col >= 1) {
+          if (col <= n1a) hrow_a[col - 1] = Hn[RPL - 1].x;
+          if (has_b && col <= n1b) hrow_b[col - 1] = Hn[RPL - 1].y;
+        }
+      }
+      dH = uH;
+      uH = halves((uint32_t)down0<W>((int)bits(Hn[RPL - 1]), k));
+      uE = halves((uint32_t)down_or<W>((int)bits(lastE), (int)bits(low), k));
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) Hp[r] = Hn[r];
+      hc = hc_next;
+    };
+    int s = 0;
+    for (; s < W - 1 && s < nsteps; ++s) step(s, std::true_type{});
+    for (; s < nsteps; ++s) step(s, std::false_type{});
+  }
+}
+
 // One thread per pair: the best end cell (last haplotype column, then last
 // read row, one anti-diagonal at a time, with the reference's tie rules),
 // then the traceback into CIGAR runs, written start to end.
@@ -215,7 +377,9 @@ __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
   }
   a.score[p] = best;
   const int rows = (int)pr.width * (int)pr.rpl;
-  const uint16_t* bt = reinterpret_cast<const uint16_t*>(a.bt + pr.bt_off);
+  // 16-bit words: the pair's own, or every other one of a duo's 32-bit words
+  const uint16_t* bt = reinterpret_cast<const uint16_t*>(a.bt + pr.bt_off) + pr.bt_half;
+  const int bt_lanes = pr.bt_lanes;
   // runs end to start, merged as they come
   int n = 0, cur_op = -1;
   uint32_t cur_len = 0;
@@ -236,7 +400,7 @@ __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
   while (i > 0 && j > 0) {
     const int kk = (j - 1) / (int)pr.rpl;
     const size_t nib = (size_t)(i - 1 + kk) * rows + (j - 1);
-    const int v = (bt[nib >> 2] >> (4 * (3 - (int)(nib & 3)))) & 15;
+    const int v = (bt[(nib >> 2) * bt_lanes] >> (4 * (3 - (int)(nib & 3)))) & 15;
     // state 4: inside an insertion, 8: inside a deletion (the gap goes on
     // where this cell's E / F was extended, not opened)
     const int ins_ext = (v & 4) ? 0 : 4, del_ext = (v & 8) ? 0 : 8;
@@ -284,6 +448,20 @@ const SwFn kSwFill[kSwShapes] = {&k_sw_fill<kSwW[0], kSwR[0]>, &k_sw_fill<kSwW[1
                                  &k_sw_fill<kSwW[2], kSwR[2]>, &k_sw_fill<kSwW[3], kSwR[3]>,
                                  &k_sw_fill<kSwW[4], kSwR[4]>, &k_sw_fill<kSwW[5], kSwR[5]>,
                                  &k_sw_fill<kSwW[6], kSwR[6]>, &k_sw_fill<kSwW[7], kSwR[7]>};
+const SwFn kSwFill16[kSwShapes] = {&k_sw_fill16<kSwW[0], kSwR[0]>, &k_sw_fill16<kSwW[1], kSwR[1]>,
+                                   &k_sw_fill16<kSwW[2], kSwR[2]>, &k_sw_fill16<kSwW[3], kSwR[3]>,
+                                   &k_sw_fill16<kSwW[4], kSwR[4]>, &k_sw_fill16<kSwW[5], kSwR[5]>,
+                                   &k_sw_fill16<kSwW[6], kSwR[6]>, &k_sw_fill16<kSwW[7], kSwR[7]>};
+
+int blocks_per_cu(const SwFn* table, int* cache, int shape) {
+  if (shape < 0 || shape >= kSwShapes) return 1;
+  if (cache[shape] == 0) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, table[shape], 64, 0) != hipSuccess || n < 1) n = 1;
+    cache[shape] = n;
+  }
+  return cache[shape];
+}
 }  // namespace
 
 hipError_t launch_sw_fill(int shape, int grid, hipStream_t stream, const SwArgs& a) {
@@ -298,15 +476,20 @@ hipError_t launch_sw_trace(hipStream_t stream, const SwArgs& a) {
   return hipGetLastError();
 }
 
+hipError_t launch_sw_fill16(int shape, int grid, hipStream_t stream, const SwArgs& a) {
+  if (shape < 0 || shape >= kSwShapes) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(kSwFill16[shape], dim3(grid), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
 int sw_blocks_per_cu(int shape) {
   static int cache[kSwShapes] = {};
-  if (shape < 0 || shape >= kSwShapes) return 1;
-  if (cache[shape] == 0) {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kSwFill[shape], 64, 0) != hipSuccess || n < 1) n = 1;
-    cache[shape] = n;
-  }
-  return cache[shape];
+  return blocks_per_cu(kSwFill, cache, shape);
+}
+
+int sw16_blocks_per_cu(int shape) {
+  static int cache[kSwShapes] = {};
+  return blocks_per_cu(kSwFill16, cache, shape);
 }
 
 }  // namespace rvc

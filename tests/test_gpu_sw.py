@@ -95,3 +95,27 @@ def test_limits():
     with pytest.raises(rv.EngineError) as e:
         rv.sw_align_batch([b"ACGT"], [b"AC"], [1])
     assert e.value.code == rv.RVC_E_INVALID
+
+
+@pytest.mark.parametrize("assemble", [False, True])
+def test_packed_and_int32_sweeps_mixed(assemble):
+    # pairs two to a lane (the packed int16 sweep) next to pairs swept alone:
+    # haplotype lengths far apart within a shape, odd counts per shape, reads
+    # of 15 / 16 bases (the 16-bit core's edge) and, with the assembly scores,
+    # short pairs (packed) beside long ones (int32)
+    say(f"packed and int32 sweeps mixed, assemble={assemble}")
+    rng = np.random.default_rng(11 + assemble)
+    haps, reads, read_hap = [], [], []
+    for L in (15, 16, 17, 40, 59, 60, 61, 100, 130, 250, 400):
+        for t in range(7 if L % 2 else 4):
+            H = int(rng.choice([L + 1, L + 30, 3 * L, 1500, 2000]))
+            hap = ACGT[rng.integers(0, 4, H)].tobytes()
+            haps.append(hap)
+            a = int(rng.integers(0, max(1, H - L)))
+            piece = hap[a:a + L]
+            reads.append(_mutate(rng, piece, int(rng.integers(0, 3))) if t % 3 else ACGT[rng.integers(0, 4, L)].tobytes())
+            read_hap.append(len(haps) - 1)
+    for h in rng.permutation(len(reads))[:9]:  # extra reads on haplotypes already used
+        reads.append(haps[read_hap[h]][:len(reads[h])] or b"A")
+        read_hap.append(read_hap[h])
+    check(haps, reads, read_hap, assemble)
