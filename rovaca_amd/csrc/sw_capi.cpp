@@ -206,7 +206,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
                  o_list = take(4ull * (list.size() + 1)), o_head = take(4 * 2 * kSwShapes), o_bt = take(bt_total),
                  o_hrow = take(4 * (hrow_total + 1)), o_hcol = take(4 * (hcol_total + 1)),
                  o_cig = take(4ull * n * max_cigar), o_cnt = take(4ull * n), o_off = take(4ull * n),
-                 o_sc = take(4ull * n), o_pack = take(4ull * n * kPackCols);
+                 o_sc = take(4ull * n), o_best = take(8ull * n), o_pack = take(4ull * n * kPackCols);
     int rc = w.ensure(off);
     if (rc) return rc;
     char* base = static_cast<char*>(w.mem);
@@ -235,6 +235,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     a.count = reinterpret_cast<int32_t*>(base + o_cnt);
     a.offset = reinterpret_cast<int32_t*>(base + o_off);
     a.score = reinterpret_cast<int32_t*>(base + o_sc);
+    a.best_ij = reinterpret_cast<int32_t*>(base + o_best);
     if (hipEventRecord(w.ev[0], st) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");
     // packed sweeps (a work item is a duo), then int32 sweeps
     for (int packed = 1; packed >= 0; --packed) {

@@ -73,6 +73,7 @@ struct SwArgs {
   int32_t* count;
   int32_t* offset;
   int32_t* score;
+  int32_t* best_ij;      // per pair: the best end cell (i, j)
 };
 
 hipError_t launch_sw_fill(int shape, int grid, hipStream_t stream, const SwArgs& a);

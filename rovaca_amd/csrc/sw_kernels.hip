@@ -346,9 +346,66 @@ __global__ __launch_bounds__(64) void k_sw_fill16(SwArgs a) {
   }
 }
 
-// One thread per pair: the best end cell (last haplotype column, then last
-// read row, one anti-diagonal at a time, with the reference's tie rules),
-// then the traceback into CIGAR runs, written start to end.
+// One wave per pair: the best end cell over the last haplotype column
+// (H[n1][j]) and the last read row (H[i][n2]), scanned one anti-diagonal
+// d = i + j at a time, column candidate before row candidate, with the
+// reference's replacement rules (sw_core_template.h:332-348): a candidate
+// replaces the best if greater, or if equal and closer to the diagonal
+// (a row candidate also when equally close, or when the best is on the last
+// row). Only candidates equal to the maximum can be the result, and the first
+// of them in scan order replaces whatever came before, so the wave takes the
+// maximum with coalesced loads, then applies the rules to the equal ones in
+// scan order.
+__global__ __launch_bounds__(256) void k_sw_best(SwArgs a) {
+  const uint32_t p = blockIdx.x * 4 + threadIdx.x / 64;
+  const int l = threadIdx.x % 64;
+  if (p >= a.n_pairs) return;  // whole waves
+  const SwPair pr = a.pairs[p];
+  const int n1 = (int)pr.hap_len, n2 = (int)pr.read_len;
+  if (n1 >= kSwMaxLen || n2 >= kSwMaxLen || n1 == 0 || n2 == 0) return;
+  const int32_t* hrow = a.hrow + pr.hrow_off;  // H[i][n2], i = 1..n1
+  const int32_t* hcol = a.hcol + pr.hcol_off;  // H[n1][j], j = 1..n2
+  const int d_lo = (n1 < n2 ? n1 : n2) + 1, d_hi = n1 + n2;
+  int vmax = INT32_MIN;
+  for (int d0 = d_lo; d0 <= d_hi; d0 += 64) {
+    const int d = d0 + l;
+    if (d <= d_hi) {
+      if (d >= n1 + 1) vmax = max(vmax, hcol[d - n1 - 1]);
+      if (d >= n2 + 1) vmax = max(vmax, hrow[d - n2 - 1]);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) vmax = max(vmax, __shfl_xor(vmax, o));
+  int bi = 0, bj = 0;
+  bool have = false;
+  for (int d0 = d_lo; d0 <= d_hi; d0 += 64) {
+    const int d = d0 + l;
+    const bool c = d <= d_hi && d >= n1 + 1 && hcol[d - n1 - 1] == vmax;
+    const bool r = d <= d_hi && d >= n2 + 1 && hrow[d - n2 - 1] == vmax;
+    uint64_t mask = __ballot(c || r);
+    while (mask) {
+      const int src = __builtin_ctzll(mask);
+      mask &= mask - 1;
+      const int dd = d0 + src;
+      if (__shfl(c ? 1 : 0, src)) {
+        const int i = n1, j = dd - n1;
+        if (!have || abs(i - j) < abs(bi - bj)) { bi = i; bj = j; have = true; }
+      }
+      if (__shfl(r ? 1 : 0, src)) {
+        const int i = dd - n2, j = n2;
+        if (!have || bj == n2 || abs(i - j) <= abs(bi - bj)) { bi = i; bj = j; have = true; }
+      }
+    }
+  }
+  if (l == 0) {
+    a.score[p] = vmax;
+    a.best_ij[2 * p] = bi;
+    a.best_ij[2 * p + 1] = bj;
+  }
+}
+
+// One thread per pair: the traceback from the best end cell into CIGAR runs,
+// written start to end.
 __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.n_pairs) return;
@@ -361,21 +418,7 @@ __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
     a.score[p] = 0;
     return;
   }
-  const int32_t* hrow = a.hrow + pr.hrow_off;  // H[i][n2], i = 1..n1
-  const int32_t* hcol = a.hcol + pr.hcol_off;  // H[n1][j], j = 1..n2
-  const bool use32 = n2 < 16 || a.match <= 0 || (long long)(n1 < n2 ? n1 : n2) * a.match > 32767;
-  int best = use32 ? INT32_MIN : -32768, bi = 0, bj = 0;
-  for (int d = 1; d <= n1 + n2; ++d) {
-    if (d >= n1 + 1) {
-      const int i = n1, j = d - n1, v = hcol[j - 1];
-      if (best < v || (best == v && abs(i - j) < abs(bi - bj))) { best = v; bi = i; bj = j; }
-    }
-    if (d >= n2 + 1) {
-      const int i = d - n2, j = n2, v = hrow[i - 1];
-      if (best < v || (best == v && (bj == n2 || abs(i - j) <= abs(bi - bj)))) { best = v; bi = i; bj = j; }
-    }
-  }
-  a.score[p] = best;
+  const int bi = a.best_ij[2 * p], bj = a.best_ij[2 * p + 1];
   const int rows = (int)pr.width * (int)pr.rpl;
   // 16-bit words: the pair's own, or every other one of a duo's 32-bit words
   const uint16_t* bt = reinterpret_cast<const uint16_t*>(a.bt + pr.bt_off) + pr.bt_half;
@@ -472,6 +515,7 @@ hipError_t launch_sw_fill(int shape, int grid, hipStream_t stream, const SwArgs&
 
 hipError_t launch_sw_trace(hipStream_t stream, const SwArgs& a) {
   if (a.n_pairs == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sw_best, dim3((a.n_pairs + 3) / 4), dim3(256), 0, stream, a);
   hipLaunchKernelGGL(k_sw_trace, dim3((a.n_pairs + 63) / 64), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
