@@ -27,14 +27,47 @@ constexpr int32_t kAssemble[4] = {200, -150, -260, -11};
 
 inline uint64_t up256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 
-// Per-thread device workspace: a stream, three events, a grow-only buffer.
+// A grow-only pinned host buffer (copies from and to it need no staging).
+struct PinnedArea {
+  void* p = nullptr;
+  size_t bytes = 0;
+  PinnedArea() = default;
+  PinnedArea(const PinnedArea&) = delete;
+  PinnedArea& operator=(const PinnedArea&) = delete;
+  ~PinnedArea() {
+    if (p) hipHostFree(p);
+  }
+  template <class T>
+  T* get(size_t n) {
+    const size_t want = std::max<size_t>(n, 1) * sizeof(T);
+    if (want > bytes) {
+      if (p) hipHostFree(p);
+      p = nullptr;
+      bytes = 0;
+      const size_t grow = want + want / 4;
+      if (hipHostMalloc(&p, grow, hipHostMallocDefault) != hipSuccess) {
+        p = nullptr;
+        return nullptr;
+      }
+      bytes = grow;
+    }
+    return static_cast<T*>(p);
+  }
+};
+
+// Per-thread workspace: a stream, three events, a grow-only device buffer,
+// pinned host buffers for the pair table, the work lists and the results,
+// and the per-read host arrays, all kept across calls.
 struct SwWorkspace {
   hipStream_t stream = nullptr;
   hipEvent_t ev[3] = {};
   void* mem = nullptr;
   size_t bytes = 0;
-  std::vector<uint32_t> host;  // CIGARs as copied back
-  std::vector<uint32_t> keys, ord, list;
+  PinnedArea pairs, list, res, cig;
+  std::vector<int> shape;
+  std::vector<uint8_t> narrow;
+  std::vector<uint64_t> bt_est;
+  std::vector<uint32_t> keys, ord;
   ~SwWorkspace() {
     if (mem) hipFree(mem);
     for (hipEvent_t e : ev)
@@ -86,10 +119,20 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
   if (stats) std::memset(stats, 0, sizeof *stats);
 
   // one SwPair per read; refused pairs stay out of the sweeps
-  std::vector<SwPair> pairs(n_reads);
-  std::vector<int> shape(n_reads, -1);
-  std::vector<uint8_t> narrow(n_reads, 0);
-  std::vector<uint64_t> bt_est(n_reads, 0);  // bounds the backtrack bytes a pair adds, alone or in a duo
+  SwWorkspace& w = t_sw;
+  // a call that failed part way may have left copies from the pinned
+  // buffers in flight
+  if (w.stream && hipStreamSynchronize(w.stream) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner stream");
+  SwPair* pairs = w.pairs.get<SwPair>(n_reads);
+  uint32_t* list = w.list.get<uint32_t>(2ull * n_reads + 2);
+  int32_t* res = w.res.get<int32_t>(3ull * n_reads);
+  if (!pairs || !list || !res) return set_error(RVC_E_DEVICE, "aligner pinned host buffers for %u reads", n_reads);
+  std::vector<int>& shape = w.shape;
+  std::vector<uint8_t>& narrow = w.narrow;
+  std::vector<uint64_t>& bt_est = w.bt_est;  // bounds the backtrack bytes a pair adds, alone or in a duo
+  shape.assign(n_reads, -1);
+  narrow.assign(n_reads, 0);
+  bt_est.assign(n_reads, 0);
   for (uint32_t r = 0; r < n_reads; ++r) {
     SwPair& p = pairs[r];
     std::memset(&p, 0, sizeof p);
@@ -114,7 +157,6 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
   }
   const uint64_t hap_bytes = (uint64_t)hap_off[n_haps], read_bytes = (uint64_t)read_off[n_reads];
 
-  SwWorkspace& w = t_sw;
   uint32_t r0 = 0;
   while (r0 < n_reads) {
     // the chunk: reads r0 .. r1 whose backtrack bytes fit the budget
@@ -155,15 +197,14 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     }
     // work lists and backtrack areas: duos of neighbours in that order (two
     // entries each; a lone last pair gets kSwNoPair), then single pairs
-    std::vector<uint32_t>& list = w.list;
-    list.clear();
+    uint32_t nlist = 0;
     uint32_t dstart[kSwShapes] = {}, dcount[kSwShapes] = {}, sstart[kSwShapes] = {}, scount[kSwShapes] = {};
     uint64_t bt_total = 0;
     size_t i = 0;
     const size_t m = ord.size();
     for (int s = 0; s < kSwShapes; ++s) {
       const uint64_t rows = (uint64_t)kSwW[s] * (uint64_t)kSwR[s];
-      dstart[s] = (uint32_t)list.size();
+      dstart[s] = nlist;
       while (i < m && narrow[r0 + ord[i]] && shape[r0 + ord[i]] == s) {
         const uint32_t xa = ord[i++];
         const uint32_t xb = (i < m && narrow[r0 + ord[i]] && shape[r0 + ord[i]] == s) ? ord[i++] : kSwNoPair;
@@ -180,14 +221,14 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
           pb.bt_half = 1;
         }
         bt_total += up256((uint64_t)(n1 + kSwW[s] - 1) * rows);  // two nibbles per row and step
-        list.push_back(xa);
-        list.push_back(xb);
+        list[nlist++] = xa;
+        list[nlist++] = xb;
         ++dcount[s];
       }
     }
     for (int s = 0; s < kSwShapes; ++s) {
       const uint64_t rows = (uint64_t)kSwW[s] * (uint64_t)kSwR[s];
-      sstart[s] = (uint32_t)list.size();
+      sstart[s] = nlist;
       while (i < m && !narrow[r0 + ord[i]] && shape[r0 + ord[i]] == s) {
         const uint32_t x = ord[i++];
         SwPair& p = pairs[r0 + x];
@@ -195,7 +236,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
         p.bt_lanes = 1;
         p.bt_half = 0;
         bt_total += up256((uint64_t)(p.hap_len + kSwW[s] - 1) * rows / 2);
-        list.push_back(x);
+        list[nlist++] = x;
         ++scount[s];
       }
     }
@@ -203,7 +244,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     size_t off = 0;
     auto take = [&](uint64_t b) { const size_t o = off; off += up256(b); return o; };
     const size_t o_hap = take(hap_bytes + 64), o_read = take(read_bytes + 64), o_pair = take(sizeof(SwPair) * n),
-                 o_list = take(4ull * (list.size() + 1)), o_head = take(4 * 2 * kSwShapes), o_bt = take(bt_total),
+                 o_list = take(4ull * (nlist + 1)), o_head = take(4 * 2 * kSwShapes), o_bt = take(bt_total),
                  o_hrow = take(4 * (hrow_total + 1)), o_hcol = take(4 * (hcol_total + 1)),
                  o_cig = take(4ull * n * max_cigar), o_cnt = take(4ull * n), o_off = take(4ull * n),
                  o_sc = take(4ull * n), o_best = take(8ull * n), o_pack = take(4ull * n * kPackCols);
@@ -213,9 +254,9 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     hipStream_t st = w.stream;
     hipError_t e = hipMemcpyAsync(base + o_hap, hap_bases, hap_bytes, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(base + o_read, read_bases, read_bytes, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(base + o_pair, pairs.data() + r0, sizeof(SwPair) * n, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && !list.empty())
-      e = hipMemcpyAsync(base + o_list, list.data(), 4 * list.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_pair, pairs + r0, sizeof(SwPair) * n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && nlist > 0)
+      e = hipMemcpyAsync(base + o_list, list, 4ull * nlist, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(base + o_head, 0, 4 * 2 * kSwShapes, st);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "aligner upload: %s", hipGetErrorString(e));
     SwArgs a{};
@@ -262,10 +303,16 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     if (hipEventRecord(w.ev[2], st) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");
     // counts first, then of every pair's CIGAR slot only the columns the
     // longest CIGAR of the chunk uses
-    e = hipMemcpyAsync(count + r0, base + o_cnt, 4ull * n, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(offset + r0, base + o_off, 4ull * n, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(score + r0, base + o_sc, 4ull * n, hipMemcpyDeviceToHost, st);
+    // (count, offset and score sit one after another on the device)
+    e = hipMemcpyAsync(res, base + o_cnt, 4ull * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(res + n, base + o_off, 4ull * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(res + 2ull * n, base + o_sc, 4ull * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) {
+      std::memcpy(count + r0, res, 4ull * n);
+      std::memcpy(offset + r0, res + n, 4ull * n);
+      std::memcpy(score + r0, res + 2ull * n, 4ull * n);
+    }
     int32_t widest = 0;
     for (uint32_t x = 0; x < n && e == hipSuccess; ++x) widest = std::max(widest, count[r0 + x]);
     widest = std::min<int32_t>(widest, (int32_t)max_cigar);
@@ -278,15 +325,14 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
       if (packed)
         e = hipMemcpy2DAsync(base + o_pack, 4ull * pitch, base + o_cig, 4ull * max_cigar, 4ull * pitch, n,
                              hipMemcpyDeviceToDevice, st);
-      w.host.resize((size_t)n * pitch);
+      uint32_t* host = w.cig.get<uint32_t>((size_t)n * pitch);
+      if (!host) return set_error(RVC_E_DEVICE, "aligner pinned host buffer of %zu CIGAR elements", (size_t)n * pitch);
       if (e == hipSuccess)
-        e = hipMemcpyAsync(w.host.data(), base + (packed ? o_pack : o_cig), 4ull * n * pitch, hipMemcpyDeviceToHost,
-                           st);
+        e = hipMemcpyAsync(host, base + (packed ? o_pack : o_cig), 4ull * n * pitch, hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       for (uint32_t x = 0; x < n && e == hipSuccess; ++x)
         if (count[r0 + x] > 0)
-          std::memcpy(cigar + (size_t)(r0 + x) * max_cigar, w.host.data() + (size_t)x * pitch,
-                      4ull * (uint32_t)count[r0 + x]);
+          std::memcpy(cigar + (size_t)(r0 + x) * max_cigar, host + (size_t)x * pitch, 4ull * (uint32_t)count[r0 + x]);
     }
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "aligner results: %s", hipGetErrorString(e));
     if (stats) {
