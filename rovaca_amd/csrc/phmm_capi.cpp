@@ -15,6 +15,7 @@
 #include "phmm_adapter.h"
 #include "phmm_engine.h"
 #include "phmm_server.h"
+#include "sw_internal.h"
 
 using namespace rvc;
 
@@ -140,6 +141,7 @@ int rvc_phmm_init(const rvc_config* cfg) {
 int rvc_phmm_shutdown(void) {
   server_stop();
   release_workspaces();
+  sw_release_workspaces();
   return engine_shutdown();
 }
 

@@ -7,8 +7,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
+#include <mutex>
+#include <new>
 #include <vector>
 
 #include "../../include/rovaca_sw.h"
@@ -57,7 +60,8 @@ struct PinnedArea {
 
 // Per-thread workspace: a stream, three events, a grow-only device buffer,
 // pinned host buffers for the pair table, the work lists and the results,
-// and the per-read host arrays, all kept across calls.
+// and the per-read host arrays, all kept across calls. Workspaces are owned
+// by a registry and released by rvc_phmm_shutdown, never by thread exit.
 struct SwWorkspace {
   hipStream_t stream = nullptr;
   hipEvent_t ev[3] = {};
@@ -92,9 +96,32 @@ struct SwWorkspace {
     return RVC_OK;
   }
 };
-thread_local SwWorkspace t_sw;
+std::mutex g_sw_mu;
+std::vector<SwWorkspace*> g_sw_all;
+std::atomic<uint64_t> g_sw_generation{1};
+thread_local SwWorkspace* t_sw = nullptr;
+thread_local uint64_t t_sw_generation = 0;
+
+SwWorkspace* sw_workspace() {
+  if (!t_sw || t_sw_generation != g_sw_generation.load(std::memory_order_acquire)) {
+    auto* w = new (std::nothrow) SwWorkspace;
+    if (!w) return nullptr;
+    std::lock_guard<std::mutex> lk(g_sw_mu);
+    g_sw_all.push_back(w);
+    t_sw = w;
+    t_sw_generation = g_sw_generation.load(std::memory_order_relaxed);
+  }
+  return t_sw;
+}
 
 }  // namespace
+
+void sw_release_workspaces() {
+  std::lock_guard<std::mutex> lk(g_sw_mu);
+  for (SwWorkspace* w : g_sw_all) delete w;
+  g_sw_all.clear();
+  g_sw_generation.fetch_add(1, std::memory_order_release);
+}
 
 int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_haps, const uint8_t* read_bases,
                    const int32_t* read_off, uint32_t n_reads, const uint32_t* read_hap, int32_t assemble,
@@ -119,7 +146,9 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
   if (stats) std::memset(stats, 0, sizeof *stats);
 
   // one SwPair per read; refused pairs stay out of the sweeps
-  SwWorkspace& w = t_sw;
+  SwWorkspace* wp = sw_workspace();
+  if (!wp) return set_error(RVC_E_INVALID, "out of host memory");
+  SwWorkspace& w = *wp;
   // a call that failed part way may have left copies from the pinned
   // buffers in flight
   if (w.stream && hipStreamSynchronize(w.stream) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner stream");

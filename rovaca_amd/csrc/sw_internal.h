@@ -81,5 +81,6 @@ hipError_t launch_sw_fill16(int shape, int grid, hipStream_t stream, const SwArg
 hipError_t launch_sw_trace(hipStream_t stream, const SwArgs& a);
 int sw_blocks_per_cu(int shape);
 int sw16_blocks_per_cu(int shape);
+void sw_release_workspaces();  // the aligner's per-thread workspaces (engine shutdown)
 
 }  // namespace rvc

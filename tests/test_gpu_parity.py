@@ -179,7 +179,9 @@ def test_gap_quality_forms(lo, hi, gcp, form):
     # gap continuation 0 leaves no way back from a gap: equal infinities
     same = got[m] == want[m]
     assert same.any() or np.isfinite(want[m]).all()
-    assert np.all(same | (np.abs(got[m] - want[m]) <= TOL))
+    with np.errstate(invalid="ignore"):  # inf - inf where both are the same infinity
+        close = np.abs(got[m] - want[m]) <= TOL
+    assert np.all(same | close)
     # alone through the synchronous path: the same form, bit for bit
     alone, _ = rv.region_sync(reg)
     assert np.array_equal(alone[m], got[m])

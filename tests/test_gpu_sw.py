@@ -119,3 +119,16 @@ def test_packed_and_int32_sweeps_mixed(assemble):
         reads.append(haps[read_hap[h]][:len(reads[h])] or b"A")
         read_hap.append(read_hap[h])
     check(haps, reads, read_hap, assemble)
+
+
+def test_engine_restart_releases_and_rebuilds_workspaces():
+    # shutdown releases the aligner's per-thread workspace; the next call
+    # after init builds a new one
+    say("aligner across an engine restart")
+    pairs = random_pairs(31, 40)
+    haps = [h for h, _ in pairs]
+    reads = [r for _, r in pairs]
+    first = check(haps, reads, list(range(len(pairs))))
+    rv.shutdown()
+    rv.init(0)
+    assert rv.sw_align_batch(haps, reads, list(range(len(pairs)))) == first
