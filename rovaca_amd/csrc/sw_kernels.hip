@@ -191,7 +191,8 @@ __global__ __launch_bounds__(64) void k_sw_fill(SwArgs a) {
 // (dg + match) + min(hc ^ rc, 1) * (mismatch - match), byte equality as in
 // the int32 sweep; rows past a read hold rc = 0x1ff (never equal), mismatch
 // and deletion open kSwBig16, insertion open 0. Each 4-row group leaves one
-// 32-bit word per step: A's 16 bits low, B's high.
+// 32-bit word per step, A's 16 bits low and B's high, with its own bit order
+// (see sw_step16; k_sw_trace reads both orders).
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
@@ -199,10 +200,6 @@ __device__ __forceinline__ s16x2 pmax(s16x2 a, s16x2 b) { return __builtin_eleme
 __device__ __forceinline__ uint32_t bits(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ s16x2 halves(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
 __device__ __forceinline__ s16x2 splat(int v) { return s16x2{(short)v, (short)v}; }
-// the sign bits of both halves at bits 0 and 16
-__device__ __forceinline__ uint32_t signs(s16x2 d) {
-  return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, d) >> (u16x2)15);
-}
 
 template <int RPL>
 __device__ __forceinline__ void sw_step16(s16x2 (&Hp)[RPL], s16x2 (&F)[RPL], s16x2 (&Hn)[RPL], s16x2& lastE,
@@ -221,12 +218,15 @@ __device__ __forceinline__ void sw_step16(s16x2 (&Hp)[RPL], s16x2 (&F)[RPL], s16
     const s16x2 h0 = pmax(cut, m);
     const s16x2 h1 = pmax(h0, e);
     const s16x2 h = pmax(h1, f);
-    // bit order as in sw_step; each shift moves both halves' bits up by one
+    // the high bytes of the four differences (signs at bit 7 of each byte)
+    // gathered two at a time, then shifted into the group's word: in each
+    // half, row q of the group ends with F opened at bit 2q, deletion at
+    // 2q + 1, E opened at 8 + 2q, insertion at 9 + 2q
+    const uint32_t p01 = __builtin_amdgcn_perm(bits(eh - oh), bits(ev - ov), 0x07030501u);
+    const uint32_t p23 = __builtin_amdgcn_perm(bits(h0 - e), bits(h1 - f), 0x07030501u);
     uint32_t acc = (r & 3) == 0 ? 0u : bt[r >> 2];
-    acc = (acc << 1) | signs(ev - ov);
-    acc = (acc << 1) | signs(eh - oh);
-    acc = (acc << 1) | signs(h1 - f);
-    acc = (acc << 1) | signs(h0 - e);
+    acc = (p01 & 0x80808080u) | ((acc >> 1) & 0x7f7f7f7fu);
+    acc = (p23 & 0x80808080u) | ((acc >> 1) & 0x7f7f7f7fu);
     bt[r >> 2] = acc;
     dg = Hp[r];
     Hn[r] = h;
@@ -443,7 +443,12 @@ __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
   while (i > 0 && j > 0) {
     const int kk = (j - 1) / (int)pr.rpl;
     const size_t nib = (size_t)(i - 1 + kk) * rows + (j - 1);
-    const int v = (bt[(nib >> 2) * bt_lanes] >> (4 * (3 - (int)(nib & 3)))) & 15;
+    const uint32_t w16 = bt[(nib >> 2) * bt_lanes];
+    const int q = (int)(nib & 3);
+    // bit 0 insertion, 1 deletion, 2 E opened, 3 F opened
+    const int v = bt_lanes == 1 ? (int)(w16 >> (4 * (3 - q))) & 15
+                                : (int)(((w16 >> (9 + 2 * q)) & 1) | ((w16 >> (2 * q)) & 2) |
+                                        ((w16 >> (6 + 2 * q)) & 4) | ((w16 << 3 >> (2 * q)) & 8));
     // state 4: inside an insertion, 8: inside a deletion (the gap goes on
     // where this cell's E / F was extended, not opened)
     const int ins_ext = (v & 4) ? 0 : 4, del_ext = (v & 8) ? 0 : 8;
