@@ -12,6 +12,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/rovaca_sw.h"
@@ -67,6 +68,11 @@ struct SwWorkspace {
   hipEvent_t ev[3] = {};
   void* mem = nullptr;
   size_t bytes = 0;
+  // the caller's bases, uploaded from a helper thread on their own stream
+  hipStream_t up_stream = nullptr;
+  hipEvent_t up_done = nullptr;
+  void* in_mem = nullptr;
+  size_t in_bytes = 0;
   PinnedArea pairs, list, res, cig;
   std::vector<int> shape;
   std::vector<uint8_t> narrow;
@@ -74,18 +80,36 @@ struct SwWorkspace {
   std::vector<uint32_t> keys, ord;
   ~SwWorkspace() {
     if (mem) hipFree(mem);
+    if (in_mem) hipFree(in_mem);
     for (hipEvent_t e : ev)
       if (e) hipEventDestroy(e);
+    if (up_done) hipEventDestroy(up_done);
     if (stream) hipStreamDestroy(stream);
+    if (up_stream) hipStreamDestroy(up_stream);
+  }
+  int streams() {
+    if (stream) return RVC_OK;
+    if (hipSetDevice(g_engine.device) != hipSuccess ||
+        hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&up_stream, hipStreamNonBlocking) != hipSuccess)
+      return set_error(RVC_E_DEVICE, "aligner streams");
+    for (hipEvent_t& e : ev)
+      if (hipEventCreate(&e) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner events");
+    if (hipEventCreateWithFlags(&up_done, hipEventDisableTiming) != hipSuccess)
+      return set_error(RVC_E_DEVICE, "aligner events");
+    return RVC_OK;
+  }
+  int ensure_in(size_t want) {
+    if (want <= in_bytes) return RVC_OK;
+    if (in_mem) hipFree(in_mem);
+    in_mem = nullptr;
+    in_bytes = 0;
+    const size_t n = std::max(want + want / 4, (size_t)16 << 20);
+    if (hipMalloc(&in_mem, n) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner input buffer of %zu bytes", n);
+    in_bytes = n;
+    return RVC_OK;
   }
   int ensure(size_t want) {
-    if (!stream) {
-      if (hipSetDevice(g_engine.device) != hipSuccess ||
-          hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
-        return set_error(RVC_E_DEVICE, "aligner stream");
-      for (hipEvent_t& e : ev)
-        if (hipEventCreate(&e) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner events");
-    }
     if (want <= bytes) return RVC_OK;
     if (mem) hipFree(mem);
     mem = nullptr;
@@ -145,13 +169,34 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
   const int32_t* sc = assemble ? kAssemble : kGenotype;
   if (stats) std::memset(stats, 0, sizeof *stats);
 
-  // one SwPair per read; refused pairs stay out of the sweeps
   SwWorkspace* wp = sw_workspace();
   if (!wp) return set_error(RVC_E_INVALID, "out of host memory");
   SwWorkspace& w = *wp;
   // a call that failed part way may have left copies from the pinned
   // buffers in flight
   if (w.stream && hipStreamSynchronize(w.stream) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner stream");
+  if (int rc = w.streams()) return rc;
+  // the caller's bases (pageable memory, staged by the runtime) go up from a
+  // helper thread while this one builds the pair table and the work lists
+  const uint64_t hap_bytes = (uint64_t)hap_off[n_haps], read_bytes = (uint64_t)read_off[n_reads];
+  const size_t o_read_in = (size_t)up256(hap_bytes + 64);
+  if (int rc = w.ensure_in(o_read_in + up256(read_bytes + 64))) return rc;
+  char* in = static_cast<char*>(w.in_mem);
+  hipError_t up_err = hipSuccess;
+  std::thread uploader([&] {
+    hipError_t e = hipSetDevice(g_engine.device);
+    if (e == hipSuccess) e = hipMemcpyAsync(in, hap_bases, hap_bytes, hipMemcpyHostToDevice, w.up_stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(in + o_read_in, read_bases, read_bytes, hipMemcpyHostToDevice, w.up_stream);
+    if (e == hipSuccess) e = hipEventRecord(w.up_done, w.up_stream);
+    up_err = e;
+  });
+  struct Join {
+    std::thread& t;
+    ~Join() {
+      if (t.joinable()) t.join();
+    }
+  } join_uploader{uploader};
+  // one SwPair per read; refused pairs stay out of the sweeps
   SwPair* pairs = w.pairs.get<SwPair>(n_reads);
   uint32_t* list = w.list.get<uint32_t>(2ull * n_reads + 2);
   int32_t* res = w.res.get<int32_t>(3ull * n_reads);
@@ -184,7 +229,6 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     }
     if (stats) stats->cells += (uint64_t)p.hap_len * p.read_len;
   }
-  const uint64_t hap_bytes = (uint64_t)hap_off[n_haps], read_bytes = (uint64_t)read_off[n_reads];
 
   uint32_t r0 = 0;
   while (r0 < n_reads) {
@@ -272,7 +316,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     // device layout
     size_t off = 0;
     auto take = [&](uint64_t b) { const size_t o = off; off += up256(b); return o; };
-    const size_t o_hap = take(hap_bytes + 64), o_read = take(read_bytes + 64), o_pair = take(sizeof(SwPair) * n),
+    const size_t o_pair = take(sizeof(SwPair) * n),
                  o_list = take(4ull * (nlist + 1)), o_head = take(4 * 2 * kSwShapes), o_bt = take(bt_total),
                  o_hrow = take(4 * (hrow_total + 1)), o_hcol = take(4 * (hcol_total + 1)),
                  o_cig = take(4ull * n * max_cigar), o_cnt = take(4ull * n), o_off = take(4ull * n),
@@ -281,16 +325,19 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     if (rc) return rc;
     char* base = static_cast<char*>(w.mem);
     hipStream_t st = w.stream;
-    hipError_t e = hipMemcpyAsync(base + o_hap, hap_bases, hap_bytes, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(base + o_read, read_bases, read_bytes, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(base + o_pair, pairs + r0, sizeof(SwPair) * n, hipMemcpyHostToDevice, st);
+    hipError_t e = hipMemcpyAsync(base + o_pair, pairs + r0, sizeof(SwPair) * n, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && nlist > 0)
       e = hipMemcpyAsync(base + o_list, list, 4ull * nlist, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(base + o_head, 0, 4 * 2 * kSwShapes, st);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "aligner upload: %s", hipGetErrorString(e));
+    if (uploader.joinable()) {
+      uploader.join();
+      if (up_err == hipSuccess) up_err = hipStreamWaitEvent(st, w.up_done, 0);
+      if (up_err != hipSuccess) return set_error(RVC_E_DEVICE, "aligner upload of the bases: %s", hipGetErrorString(up_err));
+    }
     SwArgs a{};
-    a.haps = reinterpret_cast<const uint8_t*>(base + o_hap);
-    a.reads = reinterpret_cast<const uint8_t*>(base + o_read);
+    a.haps = reinterpret_cast<const uint8_t*>(in);
+    a.reads = reinterpret_cast<const uint8_t*>(in + o_read_in);
     a.pairs = reinterpret_cast<const SwPair*>(base + o_pair);
     a.n_pairs = n;
     a.bt = reinterpret_cast<uint8_t*>(base + o_bt);
