@@ -98,7 +98,11 @@ typedef struct rvc_config {
                              /* thread; 2 = always on the device                   */
   const char* tune_table;    /* kernel-shape table written by the tuner; NULL = the */
                              /* table installed beside the library, if any         */
-  uint32_t reserved[6];
+  uint32_t sw_min_launch_pairs; /* aligner (rovaca_sw.h): a sweep shape with fewer */
+                             /* pairs runs with the next larger one launched       */
+                             /* anyway (0 = enough to fill the device once;        */
+                             /* 1 = every shape launches on its own)               */
+  uint32_t reserved[5];
 } rvc_config;
 
 /* ---- lifecycle -------------------------------------------------------------- */

@@ -56,7 +56,8 @@ class RvcRawRegion(ctypes.Structure):
 class RvcConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("server_threads", ctypes.c_uint32),
                 ("hbm_budget_bytes", ctypes.c_uint64), ("max_batch_regions", ctypes.c_uint32),
-                ("host_prep", ctypes.c_uint32), ("tune_table", ctypes.c_char_p), ("reserved", ctypes.c_uint32 * 6)]
+                ("host_prep", ctypes.c_uint32), ("tune_table", ctypes.c_char_p),
+                ("sw_min_launch_pairs", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 5)]
 
 
 class RvcBatchOpts(ctypes.Structure):
@@ -163,11 +164,15 @@ def make_c_region(region):
     return cr
 
 
-def init(device=0, server=False, hbm_budget_bytes=0, max_batch_regions=0, tune_table=None, host_prep=0):
+def init(device=0, server=False, hbm_budget_bytes=0, max_batch_regions=0, tune_table=None, host_prep=0,
+         sw_min_launch_pairs=0):
     """host_prep: where call_pairhmm preprocesses reads (0 = device when the
-    PCR model is on, else host; 1 = host; 2 = device)."""
+    PCR model is on, else host; 1 = host; 2 = device). sw_min_launch_pairs:
+    the aligner runs a sweep shape with fewer pairs together with the next
+    larger one (0 = enough pairs to fill the device once; 1 = never)."""
     cfg = RvcConfig()
     cfg.host_prep = int(host_prep)
+    cfg.sw_min_launch_pairs = int(sw_min_launch_pairs)
     cfg.tune_table = tune_table.encode() if tune_table else None
     cfg.device = device
     cfg.server_threads = 1 if server else 0

@@ -85,6 +85,7 @@ int engine_init(const rvc_config* cfg) {
   g_engine.hbm_budget = (cfg && cfg->hbm_budget_bytes) ? cfg->hbm_budget_bytes : (64ull << 30);
   g_engine.max_batch_regions = (cfg && cfg->max_batch_regions) ? cfg->max_batch_regions : 4096u;
   g_engine.host_prep = cfg ? std::min<uint32_t>(cfg->host_prep, 2) : 0;
+  g_engine.sw_min_launch = cfg ? cfg->sw_min_launch_pairs : 0;
   // kernel-shape table: the one named in the config, else tuned/<device>.txt beside the library
   std::string table;
   if (cfg && cfg->tune_table) {

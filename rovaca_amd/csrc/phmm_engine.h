@@ -30,6 +30,7 @@ struct Engine {
   uint64_t hbm_budget = 0;
   uint32_t max_batch_regions = 0;
   uint32_t host_prep = 0;   // call_pairhmm preprocessing: 0 auto, 1 host, 2 device
+  uint32_t sw_min_launch = 0;  // aligner: fewest pairs a sweep shape launches with (0 auto)
   std::string tune_source;  // where the kernel-shape table came from ("" = built-in defaults)
 };
 extern Engine g_engine;

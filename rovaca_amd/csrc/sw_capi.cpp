@@ -219,15 +219,45 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
       continue;
     int W = 0, RPL = 0;
     shape[r] = sw_shape(p.read_len, &W, &RPL);
-    p.width = (uint16_t)W;
-    p.rpl = (uint16_t)RPL;
-    bt_est[r] = up256((uint64_t)(p.hap_len + W - 1) * (uint64_t)(W * RPL / 2));  // a nibble per cell
-    if (sw_narrow(p.hap_len, p.read_len, sc[0], sc[1], sc[2], sc[3])) {
-      // a duo's steps run to its longer haplotype: at most twice its own share
-      narrow[r] = 1;
-      bt_est[r] *= 2;
-    }
+    narrow[r] = sw_narrow(p.hap_len, p.read_len, sc[0], sc[1], sc[2], sc[3]) ? 1 : 0;
     if (stats) stats->cells += (uint64_t)p.hap_len * p.read_len;
+  }
+  // a shape with too few pairs to fill the device once (or fewer than the
+  // engine's sw_min_launch) joins the next larger shape launched anyway (of
+  // the same sweep, at most 1.5 times its rows): a launch of its own would
+  // cost about one work item's time for a handful of pairs
+  int promote[2][kSwShapes];
+  {
+    uint64_t cnt[2][kSwShapes] = {};
+    for (uint32_t r = 0; r < n_reads; ++r)
+      if (shape[r] >= 0) ++cnt[narrow[r]][shape[r]];
+    for (int nw = 0; nw < 2; ++nw) {
+      int target = -1;  // the smallest larger shape that is launched
+      for (int s = kSwShapes - 1; s >= 0; --s) {
+        promote[nw][s] = s;
+        if (cnt[nw][s] == 0) continue;
+        const uint64_t fill =
+            g_engine.sw_min_launch ? (uint64_t)g_engine.sw_min_launch
+                                   : (uint64_t)g_engine.n_cu * (uint64_t)(nw ? sw16_blocks_per_cu(s) : sw_blocks_per_cu(s)) *
+                                         (uint64_t)(64 / kSwW[s]) * (nw ? 2u : 1u);
+        if (target >= 0 && cnt[nw][s] < fill && 2 * kSwW[target] * kSwR[target] <= 3 * kSwW[s] * kSwR[s])
+          promote[nw][s] = target;
+        else
+          target = s;
+      }
+    }
+  }
+  for (uint32_t r = 0; r < n_reads; ++r) {
+    if (shape[r] < 0) continue;
+    SwPair& p = pairs[r];
+    const int s = promote[narrow[r]][shape[r]];
+    shape[r] = s;
+    p.width = (uint16_t)kSwW[s];
+    p.rpl = (uint16_t)kSwR[s];
+    // a 16-bit word per 4 rows and step; a duo's steps run to its longer
+    // haplotype: at most twice this pair's own share
+    bt_est[r] = up256((uint64_t)(p.hap_len + kSwW[s] - 1) * (uint64_t)(kSwW[s] * sw_groups(kSwR[s])) * 2);
+    if (narrow[r]) bt_est[r] *= 2;
   }
 
   uint32_t r0 = 0;
@@ -276,7 +306,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     size_t i = 0;
     const size_t m = ord.size();
     for (int s = 0; s < kSwShapes; ++s) {
-      const uint64_t rows = (uint64_t)kSwW[s] * (uint64_t)kSwR[s];
+      const uint64_t groups = (uint64_t)kSwW[s] * (uint64_t)sw_groups(kSwR[s]);
       dstart[s] = nlist;
       while (i < m && narrow[r0 + ord[i]] && shape[r0 + ord[i]] == s) {
         const uint32_t xa = ord[i++];
@@ -293,14 +323,14 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
           pb.bt_lanes = 2;
           pb.bt_half = 1;
         }
-        bt_total += up256((uint64_t)(n1 + kSwW[s] - 1) * rows);  // two nibbles per row and step
+        bt_total += up256((uint64_t)(n1 + kSwW[s] - 1) * groups * 4);  // a 32-bit word per 4 rows and step
         list[nlist++] = xa;
         list[nlist++] = xb;
         ++dcount[s];
       }
     }
     for (int s = 0; s < kSwShapes; ++s) {
-      const uint64_t rows = (uint64_t)kSwW[s] * (uint64_t)kSwR[s];
+      const uint64_t groups = (uint64_t)kSwW[s] * (uint64_t)sw_groups(kSwR[s]);
       sstart[s] = nlist;
       while (i < m && !narrow[r0 + ord[i]] && shape[r0 + ord[i]] == s) {
         const uint32_t x = ord[i++];
@@ -308,7 +338,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
         p.bt_off = bt_total;
         p.bt_lanes = 1;
         p.bt_half = 0;
-        bt_total += up256((uint64_t)(p.hap_len + kSwW[s] - 1) * rows / 2);
+        bt_total += up256((uint64_t)(p.hap_len + kSwW[s] - 1) * groups * 2);
         list[nlist++] = x;
         ++scount[s];
       }

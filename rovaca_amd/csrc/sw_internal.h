@@ -9,7 +9,7 @@ namespace rvc {
 
 constexpr int kSwMaxLen = 2048;       // the reference aligner refuses lengths >= this (sw_avx.c:18, sw_core_template.h:341)
 constexpr int kSwMaxRead = 1024;      // longest read the sweep shapes hold
-constexpr int kSwShapes = 8;          // (W, RPL): see sw_shape()
+constexpr int kSwShapes = 12;         // (W, RPL): see sw_shape()
 constexpr int kSwLow = -(1 << 28);    // "no gap" at row and column 0
 constexpr int kSwBig = -(1 << 26);    // disables a move in the rows past the read
 constexpr uint32_t kSwOpM = 0, kSwOpI = 1, kSwOpD = 2, kSwOpS = 4;  // BAM CIGAR operations
@@ -28,10 +28,16 @@ inline bool sw_narrow(uint32_t n1, uint32_t n2, int match, int mismatch, int ope
          open <= 0 && open >= -300 && extend <= 0 && extend >= -300;
 }
 
-// Sweep shapes (segment width W, rows per lane RPL): 64 .. 1024 rows. Up to
-// 12 rows per lane a kernel keeps two waves per SIMD.
-constexpr int kSwW[kSwShapes] = {16, 16, 16, 32, 32, 64, 64, 64};
-constexpr int kSwR[kSwShapes] = {4, 8, 12, 8, 12, 8, 12, 16};
+// Sweep shapes (segment width W, rows per lane RPL): 64 .. 1024 rows, with
+// steps of 32-64 rows up to 384 so that common read lengths (100, 150, 250
+// bases) leave few rows idle. Up to 12 rows per lane a kernel keeps two waves
+// per SIMD.
+constexpr int kSwW[kSwShapes] = {16, 16, 16, 16, 16, 32, 32, 32, 32, 64, 64, 64};
+constexpr int kSwR[kSwShapes] = {4, 7, 8, 10, 12, 7, 8, 10, 12, 8, 12, 16};
+
+// Backtrack groups per lane: 4 rows each, the last one possibly shorter
+// (its bits sit where a full group's would).
+constexpr int sw_groups(int rpl) { return (rpl + 3) / 4; }
 
 // The shape for a read of n2 bases (the fewest rows with W * RPL >= n2), or -1.
 inline int sw_shape(uint32_t n2, int* W, int* RPL) {
@@ -47,7 +53,7 @@ inline int sw_shape(uint32_t n2, int* W, int* RPL) {
 struct SwPair {
   uint32_t hap_off, hap_len;
   uint32_t read_off, read_len;
-  uint64_t bt_off;      // backtrack nibbles: (hap_len + W - 1) steps x W * RPL rows
+  uint64_t bt_off;      // backtrack words: (hap_len + W - 1) steps x W lanes x sw_groups(RPL)
   uint32_t hrow_off;    // hap_len ints: H of the read's last row per column
   uint32_t hcol_off;    // W * RPL ints: H of the haplotype's last column per row
   uint16_t width, rpl;

@@ -60,7 +60,8 @@ __device__ __forceinline__ int down_or(int v, int old, int k) {
 // uH, uE: H and E of the row above in this column; dH: H of the row above in
 // the previous column.
 template <int RPL>
-__device__ __forceinline__ void sw_step(int (&Hp)[RPL], int (&F)[RPL], int (&Hn)[RPL], int& lastE, uint32_t (&bt)[RPL / 4],
+__device__ __forceinline__ void sw_step(int (&Hp)[RPL], int (&F)[RPL], int (&Hn)[RPL], int& lastE,
+                                        uint32_t (&bt)[sw_groups(RPL)],
                                         int uH, int uE, int dH, int hc, const int (&rc)[RPL], const int (&oE)[RPL],
                                         const int (&oF)[RPL], const int (&mis)[RPL], int match, int ext, int cut) {
   int aH = uH, aE = uE, dg = dH;
@@ -91,13 +92,14 @@ __device__ __forceinline__ void sw_step(int (&Hp)[RPL], int (&F)[RPL], int (&Hn)
     aH = h;
     aE = e;
   }
+  // a short last group: its nibbles to the places of a full group's
+  if constexpr (RPL % 4 != 0) bt[RPL / 4] <<= 4 * (4 - RPL % 4);
   lastE = aE;
 }
 
 template <int W, int RPL>
 __global__ __launch_bounds__(64) void k_sw_fill(SwArgs a) {
   constexpr int NSEG = 64 / W;
-  constexpr int ROWS = W * RPL;
   __shared__ uint8_t hb[NSEG][kHapRow];
   const int lane = threadIdx.x;
   const int seg = lane / W, k = lane % W;
@@ -139,7 +141,8 @@ __global__ __launch_bounds__(64) void k_sw_fill(SwArgs a) {
     // sw_internal.h:19-22); its 32-bit core at -100000000
     const bool use32 = n2 < 16 || a.match <= 0 || (long long)(n1 < n2 ? n1 : n2) * a.match > 32767;
     const int cut = use32 ? -100000000 : -8192;
-    uint8_t* btp = a.bt + pr.bt_off + (uint32_t)(k * RPL / 2);
+    constexpr int G = sw_groups(RPL);
+    uint16_t* btp = reinterpret_cast<uint16_t*>(a.bt + pr.bt_off) + k * G;
     int32_t* hrow = a.hrow + pr.hrow_off;
     int32_t* hcol = a.hcol + pr.hcol_off;
     int uH = 0, uE = kSwLow, dH = 0;
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(64) void k_sw_fill(SwArgs a) {
     auto step = [&](int s, auto fill) {
       const int col = s - k + 1;  // haplotype column of this lane
       const int hc_next = hb[seg][64 + s + 1 - k];
-      uint32_t bt[RPL / 4];
+      uint32_t bt[G];
       int lastE;
       sw_step<RPL>(Hp, F, Hn, lastE, bt, uH, uE, dH, hc, rc, oE, oF, mis, a.match, a.extend, cut);
       if constexpr (decltype(fill)::value) {
@@ -160,9 +163,9 @@ __global__ __launch_bounds__(64) void k_sw_fill(SwArgs a) {
         }
       }
       if (active && s < own_steps) {
-        uint16_t* dst = reinterpret_cast<uint16_t*>(btp + (size_t)s * (ROWS / 2));
+        uint16_t* dst = btp + (size_t)s * (W * G);
 #pragma unroll
-        for (int q = 0; q < RPL / 4; ++q) dst[q] = (uint16_t)bt[q];
+        for (int q = 0; q < G; ++q) dst[q] = (uint16_t)bt[q];
         if (col == n1) {
 #pragma unroll
           for (int r = 0; r < RPL; ++r) hcol[k * RPL + r] = Hn[r];
@@ -203,7 +206,7 @@ __device__ __forceinline__ s16x2 splat(int v) { return s16x2{(short)v, (short)v}
 
 template <int RPL>
 __device__ __forceinline__ void sw_step16(s16x2 (&Hp)[RPL], s16x2 (&F)[RPL], s16x2 (&Hn)[RPL], s16x2& lastE,
-                                          uint32_t (&bt)[RPL / 4], s16x2 uH, s16x2 uE, s16x2 dH, u16x2 hc,
+                                          uint32_t (&bt)[sw_groups(RPL)], s16x2 uH, s16x2 uE, s16x2 dH, u16x2 hc,
                                           const u16x2 (&rc)[RPL], const s16x2 (&oE)[RPL], const s16x2 (&oF)[RPL],
                                           const s16x2 (&dl)[RPL], s16x2 match, s16x2 ext, s16x2 cut, u16x2 one) {
   s16x2 aH = uH, aE = uE, dg = dH;
@@ -234,13 +237,18 @@ __device__ __forceinline__ void sw_step16(s16x2 (&Hp)[RPL], s16x2 (&F)[RPL], s16
     aH = h;
     aE = e;
   }
+  // a short last group: its bits down to the places of a full group's
+  if constexpr (RPL % 4 != 0) {
+    constexpr int sh = 2 * (4 - RPL % 4);
+    constexpr uint32_t keep = (0xffu >> sh) * 0x01010101u;
+    bt[RPL / 4] = (bt[RPL / 4] >> sh) & keep;
+  }
   lastE = aE;
 }
 
 template <int W, int RPL>
 __global__ __launch_bounds__(64) void k_sw_fill16(SwArgs a) {
   constexpr int NSEG = 64 / W;
-  constexpr int ROWS = W * RPL;
   // haplotype bases of the duo, A in the low and B in the high byte
   __shared__ uint16_t hb[NSEG][kHapRow];
   const int lane = threadIdx.x;
@@ -294,7 +302,8 @@ __global__ __launch_bounds__(64) void k_sw_fill16(SwArgs a) {
     uint32_t one_bits = 0x00010001u;
     asm volatile("" : "+v"(one_bits));
     const u16x2 one = __builtin_bit_cast(u16x2, one_bits);
-    uint32_t* btp = reinterpret_cast<uint32_t*>(a.bt + A.bt_off) + k * (RPL / 4);
+    constexpr int G = sw_groups(RPL);
+    uint32_t* btp = reinterpret_cast<uint32_t*>(a.bt + A.bt_off) + k * G;
     int32_t* hrow_a = a.hrow + A.hrow_off;
     int32_t* hcol_a = a.hcol + A.hcol_off;
     int32_t* hrow_b = a.hrow + B.hrow_off;
@@ -307,7 +316,7 @@ __global__ __launch_bounds__(64) void k_sw_fill16(SwArgs a) {
     auto step = [&](int s, auto fill) {
       const int col = s - k + 1;
       const uint32_t hc_next = __builtin_amdgcn_perm(0u, (uint32_t)hb[seg][64 + s + 1 - k], 0x0c010c00u);
-      uint32_t bt[RPL / 4];
+      uint32_t bt[G];
       s16x2 lastE;
       sw_step16<RPL>(Hp, F, Hn, lastE, bt, uH, uE, dH, __builtin_bit_cast(u16x2, hc), rc, oE, oF, dl, match, ext, cut, one);
       if constexpr (decltype(fill)::value) {
@@ -317,9 +326,9 @@ __global__ __launch_bounds__(64) void k_sw_fill16(SwArgs a) {
         }
       }
       if (active && s < own_steps) {
-        uint32_t* dst = btp + (size_t)s * (ROWS / 4);
+        uint32_t* dst = btp + (size_t)s * (W * G);
 #pragma unroll
-        for (int q = 0; q < RPL / 4; ++q) dst[q] = bt[q];
+        for (int q = 0; q < G; ++q) dst[q] = bt[q];
         if (col == n1a) {
 #pragma unroll
           for (int r = 0; r < RPL; ++r) hcol_a[k * RPL + r] = Hn[r].x;
@@ -419,7 +428,7 @@ __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
     return;
   }
   const int bi = a.best_ij[2 * p], bj = a.best_ij[2 * p + 1];
-  const int rows = (int)pr.width * (int)pr.rpl;
+  const int rpl = (int)pr.rpl, gpl = sw_groups(rpl), wg = (int)pr.width * gpl;
   // 16-bit words: the pair's own, or every other one of a duo's 32-bit words
   const uint16_t* bt = reinterpret_cast<const uint16_t*>(a.bt + pr.bt_off) + pr.bt_half;
   const int bt_lanes = pr.bt_lanes;
@@ -441,10 +450,11 @@ __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
   if (j < n2) push(kSwOpS, (uint32_t)(n2 - j));
   int state = 0;
   while (i > 0 && j > 0) {
-    const int kk = (j - 1) / (int)pr.rpl;
-    const size_t nib = (size_t)(i - 1 + kk) * rows + (j - 1);
-    const uint32_t w16 = bt[(nib >> 2) * bt_lanes];
-    const int q = (int)(nib & 3);
+    const int kk = (j - 1) / rpl;  // the lane of read row j
+    const int rr = (j - 1) - kk * rpl;
+    const size_t word = (size_t)(i - 1 + kk) * wg + (size_t)(kk * gpl + (rr >> 2));
+    const uint32_t w16 = bt[word * bt_lanes];
+    const int q = rr & 3;
     // bit 0 insertion, 1 deletion, 2 E opened, 3 F opened
     const int v = bt_lanes == 1 ? (int)(w16 >> (4 * (3 - q))) & 15
                                 : (int)(((w16 >> (9 + 2 * q)) & 1) | ((w16 >> (2 * q)) & 2) |
@@ -492,14 +502,8 @@ __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
 }
 
 using SwFn = void (*)(SwArgs);
-const SwFn kSwFill[kSwShapes] = {&k_sw_fill<kSwW[0], kSwR[0]>, &k_sw_fill<kSwW[1], kSwR[1]>,
-                                 &k_sw_fill<kSwW[2], kSwR[2]>, &k_sw_fill<kSwW[3], kSwR[3]>,
-                                 &k_sw_fill<kSwW[4], kSwR[4]>, &k_sw_fill<kSwW[5], kSwR[5]>,
-                                 &k_sw_fill<kSwW[6], kSwR[6]>, &k_sw_fill<kSwW[7], kSwR[7]>};
-const SwFn kSwFill16[kSwShapes] = {&k_sw_fill16<kSwW[0], kSwR[0]>, &k_sw_fill16<kSwW[1], kSwR[1]>,
-                                   &k_sw_fill16<kSwW[2], kSwR[2]>, &k_sw_fill16<kSwW[3], kSwR[3]>,
-                                   &k_sw_fill16<kSwW[4], kSwR[4]>, &k_sw_fill16<kSwW[5], kSwR[5]>,
-                                   &k_sw_fill16<kSwW[6], kSwR[6]>, &k_sw_fill16<kSwW[7], kSwR[7]>};
+const SwFn kSwFill[kSwShapes] = {&k_sw_fill<kSwW[0], kSwR[0]>, &k_sw_fill<kSwW[1], kSwR[1]>, &k_sw_fill<kSwW[2], kSwR[2]>, &k_sw_fill<kSwW[3], kSwR[3]>, &k_sw_fill<kSwW[4], kSwR[4]>, &k_sw_fill<kSwW[5], kSwR[5]>, &k_sw_fill<kSwW[6], kSwR[6]>, &k_sw_fill<kSwW[7], kSwR[7]>, &k_sw_fill<kSwW[8], kSwR[8]>, &k_sw_fill<kSwW[9], kSwR[9]>, &k_sw_fill<kSwW[10], kSwR[10]>, &k_sw_fill<kSwW[11], kSwR[11]>};
+const SwFn kSwFill16[kSwShapes] = {&k_sw_fill16<kSwW[0], kSwR[0]>, &k_sw_fill16<kSwW[1], kSwR[1]>, &k_sw_fill16<kSwW[2], kSwR[2]>, &k_sw_fill16<kSwW[3], kSwR[3]>, &k_sw_fill16<kSwW[4], kSwR[4]>, &k_sw_fill16<kSwW[5], kSwR[5]>, &k_sw_fill16<kSwW[6], kSwR[6]>, &k_sw_fill16<kSwW[7], kSwR[7]>, &k_sw_fill16<kSwW[8], kSwR[8]>, &k_sw_fill16<kSwW[9], kSwR[9]>, &k_sw_fill16<kSwW[10], kSwR[10]>, &k_sw_fill16<kSwW[11], kSwR[11]>};
 
 int blocks_per_cu(const SwFn* table, int* cache, int shape) {
   if (shape < 0 || shape >= kSwShapes) return 1;

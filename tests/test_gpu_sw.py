@@ -20,8 +20,10 @@ def say(msg):
 
 @pytest.fixture(scope="module", autouse=True)
 def engine():
+    # every sweep shape launches on its own, however few its pairs, so that
+    # small batches reach every kernel
     say("init device 0")
-    rv.init(0)
+    rv.init(0, sw_min_launch_pairs=1)
     yield
     say("shutdown")
     rv.shutdown()
@@ -48,13 +50,15 @@ def test_every_shape_and_its_edges():
     say("read lengths at the sweep shapes' edges")
     rng = np.random.default_rng(3)
     haps, reads, read_hap = [], [], []
-    for L in (1, 2, 15, 16, 17, 63, 64, 65, 127, 128, 129, 191, 192, 193, 255, 256, 257, 383, 384, 385, 511, 512, 513,
-              767, 768, 769, 1000, 1023, 1024):
+    # the sweep shapes hold 64, 112, 128, 160, 192, 224, 256, 320, 384, 512,
+    # 768 and 1024 rows
+    edges = (64, 112, 128, 160, 192, 224, 256, 320, 384, 512, 768)
+    for L in (1, 2, 15, 16, 17) + tuple(x + d for x in edges for d in (-1, 0, 1)) + (1000, 1023, 1024):
         hap = ACGT[rng.integers(0, 4, L + 200)].tobytes()
         haps.append(hap)
         for _ in range(2):
             a = int(rng.integers(0, 150))
-            reads.append(_mutate(rng, hap[a:a + L], int(rng.integers(0, 4))) or b"C")
+            reads.append((_mutate(rng, hap[a:a + L], int(rng.integers(0, 4))) or b"C")[:L])  # insertions stay within L
             read_hap.append(len(haps) - 1)
         reads.append(ACGT[rng.integers(0, 4, L)].tobytes())
         read_hap.append(len(haps) - 1)
@@ -121,14 +125,17 @@ def test_packed_and_int32_sweeps_mixed(assemble):
     check(haps, reads, read_hap, assemble)
 
 
-def test_engine_restart_releases_and_rebuilds_workspaces():
+def test_engine_restart_and_folded_shapes():
     # shutdown releases the aligner's per-thread workspace; the next call
-    # after init builds a new one
-    say("aligner across an engine restart")
-    pairs = random_pairs(31, 40)
+    # after init builds a new one. With the default setting the sparse shapes
+    # of a small batch run folded into larger ones: the same results
+    say("aligner across an engine restart; sparse shapes folded")
+    pairs = random_pairs(31, 400)
     haps = [h for h, _ in pairs]
     reads = [r for _, r in pairs]
     first = check(haps, reads, list(range(len(pairs))))
     rv.shutdown()
     rv.init(0)
     assert rv.sw_align_batch(haps, reads, list(range(len(pairs)))) == first
+    for assemble in (False, True):
+        check(haps, reads, list(range(len(pairs))), assemble)
