@@ -48,12 +48,13 @@ def build(verbose=False):
 
 
 def build_oracle():
-    """Builds oracle/liboracle.so and, where the Rovaca tree exists, oracle/_ref."""
+    """Builds oracle/liboracle.so and, where the Rovaca tree exists, oracle/_ref,
+    from scratch every time (make -B)."""
     odir = os.path.join(ROOT, "oracle")
-    subprocess.run(["make", "-s", "-C", odir, "all"], check=True)
+    subprocess.run(["make", "-B", "-s", "-j4", "-C", odir, "all"], check=True)
     ref = os.environ.get("ROVACA_REF", "/root/reference/src/haplotypecaller/pairhmm")
     if os.path.isdir(ref):
-        subprocess.run(["make", "-s", "-C", odir, "ref", f"REF={ref}"], check=True)
+        subprocess.run(["make", "-B", "-s", "-j4", "-C", odir, "ref", f"REF={ref}"], check=True)
 
 
 if __name__ == "__main__":
