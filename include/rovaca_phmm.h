@@ -102,7 +102,9 @@ typedef struct rvc_config {
                              /* pairs runs with the next larger one launched       */
                              /* anyway (0 = enough to fill the device once;        */
                              /* 1 = every shape launches on its own)               */
-  uint32_t reserved[5];
+  uint32_t sw_bt_budget_mib; /* aligner: backtrack bytes per device pass, in MiB;   */
+                             /* larger batches run in several passes (0 = 12 GiB)  */
+  uint32_t reserved[4];
 } rvc_config;
 
 /* ---- lifecycle -------------------------------------------------------------- */

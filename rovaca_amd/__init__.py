@@ -57,7 +57,8 @@ class RvcConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("server_threads", ctypes.c_uint32),
                 ("hbm_budget_bytes", ctypes.c_uint64), ("max_batch_regions", ctypes.c_uint32),
                 ("host_prep", ctypes.c_uint32), ("tune_table", ctypes.c_char_p),
-                ("sw_min_launch_pairs", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 5)]
+                ("sw_min_launch_pairs", ctypes.c_uint32), ("sw_bt_budget_mib", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 4)]
 
 
 class RvcBatchOpts(ctypes.Structure):
@@ -165,14 +166,16 @@ def make_c_region(region):
 
 
 def init(device=0, server=False, hbm_budget_bytes=0, max_batch_regions=0, tune_table=None, host_prep=0,
-         sw_min_launch_pairs=0):
+         sw_min_launch_pairs=0, sw_bt_budget_mib=0):
     """host_prep: where call_pairhmm preprocesses reads (0 = device when the
     PCR model is on, else host; 1 = host; 2 = device). sw_min_launch_pairs:
     the aligner runs a sweep shape with fewer pairs together with the next
-    larger one (0 = enough pairs to fill the device once; 1 = never)."""
+    larger one (0 = enough pairs to fill the device once; 1 = never).
+    sw_bt_budget_mib: the aligner's backtrack bytes per device pass (0 = 12 GiB)."""
     cfg = RvcConfig()
     cfg.host_prep = int(host_prep)
     cfg.sw_min_launch_pairs = int(sw_min_launch_pairs)
+    cfg.sw_bt_budget_mib = int(sw_bt_budget_mib)
     cfg.tune_table = tune_table.encode() if tune_table else None
     cfg.device = device
     cfg.server_threads = 1 if server else 0
@@ -407,7 +410,7 @@ class SwBatch:
     packed once, with output arrays kept for repeated runs (a caller that
     aligns batch after batch reuses its result buffers the same way)."""
 
-    def __init__(self, haps, reads, read_hap, assemble=False):
+    def __init__(self, haps, reads, read_hap, assemble=False, max_cigar=None):
         self.hap_off = np.zeros(len(haps) + 1, np.int32)
         self.hap_off[1:] = np.cumsum([len(h) for h in haps])
         self.read_off = np.zeros(len(reads) + 1, np.int32)
@@ -417,7 +420,8 @@ class SwBatch:
         self.rh = np.ascontiguousarray(read_hap, np.uint32)
         self.n_haps, self.n = len(haps), len(reads)
         self.assemble = 1 if assemble else 0
-        self.max_cigar = 2 * max([len(r) for r in reads] + [1]) + 2
+        # 2 * longest read + 2 elements always suffice
+        self.max_cigar = int(max_cigar) if max_cigar else 2 * max([len(r) for r in reads] + [1]) + 2
         self.cig = np.zeros(self.n * self.max_cigar, np.uint32)
         self.cnt = np.zeros(self.n, np.int32)
         self.off = np.zeros(self.n, np.int32)

@@ -31,6 +31,7 @@ struct Engine {
   uint32_t max_batch_regions = 0;
   uint32_t host_prep = 0;   // call_pairhmm preprocessing: 0 auto, 1 host, 2 device
   uint32_t sw_min_launch = 0;  // aligner: fewest pairs a sweep shape launches with (0 auto)
+  uint64_t sw_bt_budget = 12ull << 30;  // aligner: backtrack bytes per device pass
   std::string tune_source;  // where the kernel-shape table came from ("" = built-in defaults)
 };
 extern Engine g_engine;

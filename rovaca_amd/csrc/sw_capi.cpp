@@ -3,7 +3,8 @@
 // pairs sw_narrow() admits are ordered by haplotype length and swept two per
 // lane by k_sw_fill16 (duos of neighbours in that order), the rest by
 // k_sw_fill, one launch per shape and sweep present; k_sw_trace traces all.
-// Batches whose backtrack bytes exceed kBtBudget run in chunks.
+// Batches whose backtrack bytes exceed the engine's budget (12 GiB unless
+// rvc_config says otherwise) run in chunks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,7 +23,6 @@
 namespace rvc {
 namespace {
 
-constexpr uint64_t kBtBudget = 12ull << 30;  // backtrack bytes per device pass
 constexpr uint32_t kPackCols = 64;           // CIGAR elements per pair copied back packed
 
 // scores of Rovaca's two aligner instances (sw_avx.c:14-15)
@@ -265,7 +265,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     // the chunk: reads r0 .. r1 whose backtrack bytes fit the budget
     uint32_t r1 = r0;
     uint64_t est = 0, hrow_total = 0, hcol_total = 0;
-    while (r1 < n_reads && (r1 == r0 || est + bt_est[r1] <= kBtBudget)) {
+    while (r1 < n_reads && (r1 == r0 || est + bt_est[r1] <= g_engine.sw_bt_budget)) {
       SwPair& p = pairs[r1];
       p.hrow_off = (uint32_t)hrow_total;
       p.hcol_off = (uint32_t)hcol_total;

@@ -125,6 +125,47 @@ def test_packed_and_int32_sweeps_mixed(assemble):
     check(haps, reads, read_hap, assemble)
 
 
+def test_passes_under_a_small_backtrack_budget():
+    # a 1 MiB backtrack budget splits the batch into many device passes (one
+    # pair alone where its own area exceeds it): the same results as the
+    # oracle, read by read
+    say("many device passes under a 1 MiB backtrack budget")
+    rv.shutdown()
+    rv.init(0, sw_min_launch_pairs=1, sw_bt_budget_mib=1)
+    try:
+        rng = np.random.default_rng(41)
+        pairs = random_pairs(41, 300)
+        haps = [h for h, _ in pairs]
+        reads = [r for _, r in pairs]
+        hap = ACGT[rng.integers(0, 4, 2000)].tobytes()
+        haps.append(hap)
+        for L in (700, 1024):  # areas of 1-2 MiB each
+            reads.append(hap[100:100 + L])
+        read_hap = list(range(len(pairs))) + [len(haps) - 1] * 2
+        b = rv.SwBatch(haps, reads, read_hap)
+        st = b.run()
+        assert st["chunks"] > 3, st
+        got = b.results()
+        for r, read in enumerate(reads):
+            assert got[r] == oracle.sw_align(haps[read_hap[r]], read), r
+    finally:
+        rv.shutdown()
+        rv.init(0, sw_min_launch_pairs=1)
+
+
+def test_cigar_longer_than_its_slot():
+    say("a CIGAR longer than max_cigar is an error")
+    rng = np.random.default_rng(42)
+    hap = ACGT[rng.integers(0, 4, 300)].tobytes()
+    read = hap[10:60] + b"T" + hap[60:110]  # M I M: three elements
+    ok = rv.SwBatch([hap], [read], [0], max_cigar=3)
+    ok.run()
+    assert ok.results()[0] == oracle.sw_align(hap, read)
+    with pytest.raises(rv.EngineError) as e:
+        rv.SwBatch([hap], [read], [0], max_cigar=2).run()
+    assert e.value.code == rv.RVC_E_LIMIT
+
+
 def test_engine_restart_and_folded_shapes():
     # shutdown releases the aligner's per-thread workspace; the next call
     # after init builds a new one. With the default setting the sparse shapes
