@@ -69,8 +69,8 @@ struct SwWorkspace {
   void* mem = nullptr;
   size_t bytes = 0;
   // the caller's bases, uploaded from a helper thread on their own stream
-  hipStream_t up_stream = nullptr;
-  hipEvent_t up_done = nullptr;
+  hipStream_t up_stream = nullptr, up_stream2 = nullptr;
+  hipEvent_t up_done = nullptr, up_done2 = nullptr;
   void* in_mem = nullptr;
   size_t in_bytes = 0;
   PinnedArea pairs, list, res, cig;
@@ -84,18 +84,22 @@ struct SwWorkspace {
     for (hipEvent_t e : ev)
       if (e) hipEventDestroy(e);
     if (up_done) hipEventDestroy(up_done);
+    if (up_done2) hipEventDestroy(up_done2);
     if (stream) hipStreamDestroy(stream);
     if (up_stream) hipStreamDestroy(up_stream);
+    if (up_stream2) hipStreamDestroy(up_stream2);
   }
   int streams() {
     if (stream) return RVC_OK;
     if (hipSetDevice(g_engine.device) != hipSuccess ||
         hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&up_stream, hipStreamNonBlocking) != hipSuccess)
+        hipStreamCreateWithFlags(&up_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&up_stream2, hipStreamNonBlocking) != hipSuccess)
       return set_error(RVC_E_DEVICE, "aligner streams");
     for (hipEvent_t& e : ev)
       if (hipEventCreate(&e) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner events");
-    if (hipEventCreateWithFlags(&up_done, hipEventDisableTiming) != hipSuccess)
+    if (hipEventCreateWithFlags(&up_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&up_done2, hipEventDisableTiming) != hipSuccess)
       return set_error(RVC_E_DEVICE, "aligner events");
     return RVC_OK;
   }
@@ -182,20 +186,29 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
   const size_t o_read_in = (size_t)up256(hap_bytes + 64);
   if (int rc = w.ensure_in(o_read_in + up256(read_bytes + 64))) return rc;
   char* in = static_cast<char*>(w.in_mem);
-  hipError_t up_err = hipSuccess;
+  hipError_t up_err = hipSuccess, up_err2 = hipSuccess;
+  const uint64_t half = read_bytes / 2;
   std::thread uploader([&] {
     hipError_t e = hipSetDevice(g_engine.device);
     if (e == hipSuccess) e = hipMemcpyAsync(in, hap_bases, hap_bytes, hipMemcpyHostToDevice, w.up_stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(in + o_read_in, read_bases, read_bytes, hipMemcpyHostToDevice, w.up_stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(in + o_read_in, read_bases, half, hipMemcpyHostToDevice, w.up_stream);
     if (e == hipSuccess) e = hipEventRecord(w.up_done, w.up_stream);
     up_err = e;
+  });
+  std::thread uploader2([&] {
+    hipError_t e = hipSetDevice(g_engine.device);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(in + o_read_in + half, read_bases + half, read_bytes - half, hipMemcpyHostToDevice,
+                         w.up_stream2);
+    if (e == hipSuccess) e = hipEventRecord(w.up_done2, w.up_stream2);
+    up_err2 = e;
   });
   struct Join {
     std::thread& t;
     ~Join() {
       if (t.joinable()) t.join();
     }
-  } join_uploader{uploader};
+  } join_uploader{uploader}, join_uploader2{uploader2};
   // one SwPair per read; refused pairs stay out of the sweeps
   SwPair* pairs = w.pairs.get<SwPair>(n_reads);
   uint32_t* list = w.list.get<uint32_t>(2ull * n_reads + 2);
@@ -362,7 +375,10 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "aligner upload: %s", hipGetErrorString(e));
     if (uploader.joinable()) {
       uploader.join();
+      uploader2.join();
+      if (up_err == hipSuccess) up_err = up_err2;
       if (up_err == hipSuccess) up_err = hipStreamWaitEvent(st, w.up_done, 0);
+      if (up_err == hipSuccess) up_err = hipStreamWaitEvent(st, w.up_done2, 0);
       if (up_err != hipSuccess) return set_error(RVC_E_DEVICE, "aligner upload of the bases: %s", hipGetErrorString(up_err));
     }
     SwArgs a{};
