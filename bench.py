@@ -370,9 +370,8 @@ def main():
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
-              "(regions are independent and would be scattered to per-GPU queues, see DESIGN.md)",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (regions are independent: "
+              "more GPUs would each take a share of them from a queue of their own)", file=sys.stderr)
         sys.exit(2)
 
     import rovaca_amd as rv
