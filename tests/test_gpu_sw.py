@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 import rovaca_amd as rv
-from test_oracle_sw import ACGT, _mutate, random_pairs
+from test_oracle_sw import ACGT, _mutate, random_pairs, reference_vectors
 
 pytestmark = pytest.mark.gpu
 
@@ -44,6 +44,17 @@ def test_random_pairs(assemble):
     haps = [h for h, _ in pairs]
     reads = [r for _, r in pairs]
     check(haps, reads, list(range(len(pairs))), assemble)
+
+
+@pytest.mark.parametrize("assemble", [False, True])
+def test_committed_reference_vectors(assemble):
+    # the device against the outputs of Rovaca's own aligner stored with
+    # their inputs (no oracle involved)
+    say(f"committed reference vectors, assemble={assemble}")
+    haps, reads, want = reference_vectors(assemble)
+    got = rv.sw_align_batch(haps, reads, list(range(len(reads))), assemble)
+    for r, (g, w) in enumerate(zip(got, want)):
+        assert g == w, (r, len(haps[r]), len(reads[r]), g, w)
 
 
 def test_every_shape_and_its_edges():

@@ -147,3 +147,32 @@ def test_score_is_the_best_alignment_by_brute_force():
                 p = oracle.SW_ASSEMBLE if assemble else oracle.SW_GENOTYPE
                 _, _, sc = oracle.sw_align(hap, read, assemble)
                 assert sc == _brute_best(hap, read, p), (hap, read, assemble)
+
+
+def reference_vectors(assemble):
+    """The committed fixture (tests/golden/make_golden_sw.py): pairs and the
+    outputs of Rovaca's own aligner, as (haps, reads, expected results)."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "sw_reference_pairs.npz"))
+    hb, ho, rb, ro = z["hap_bases"].tobytes(), z["hap_off"], z["read_bases"].tobytes(), z["read_off"]
+    haps = [hb[ho[k]:ho[k + 1]] for k in range(len(ho) - 1)]
+    reads = [rb[ro[k]:ro[k + 1]] for k in range(len(ro) - 1)]
+    a = int(bool(assemble))
+    cig, cnt, off, sc = z[f"cigar_{a}"], z[f"count_{a}"], z[f"offset_{a}"], z[f"score_{a}"]
+    want, pos = [], 0
+    for k in range(len(reads)):
+        if cnt[k] < 0:
+            want.append((None, int(off[k]), None))
+            continue
+        c = cig[pos:pos + cnt[k]]
+        pos += int(cnt[k])
+        want.append(([("MIDNSHP=X"[int(x) & 15], int(x) >> 4) for x in c], int(off[k]), int(sc[k])))
+    return haps, reads, want
+
+
+@pytest.mark.parametrize("assemble", [False, True])
+def test_matches_committed_reference_vectors(assemble):
+    haps, reads, want = reference_vectors(assemble)
+    assert len(want) > 200
+    for h, r, w in zip(haps, reads, want):
+        assert oracle.sw_align(h, r, assemble) == w, (h, r)
