@@ -290,15 +290,15 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
       ++r1;
     }
     const uint32_t n = r1 - r0;
-    // order (a counting sort): packed-sweep pairs by shape and haplotype
-    // length, longest first (the persistent waves end on short items), then
-    // int32-sweep pairs by shape
-    constexpr uint32_t kKeys = kSwShapes * kSwMaxLen + kSwShapes;
+    // order (a counting sort): packed-sweep pairs, then int32-sweep pairs,
+    // each by shape and haplotype length, longest first (the persistent
+    // waves end on short items)
+    constexpr uint32_t kKeys = 2 * kSwShapes * kSwMaxLen;
     auto key = [&](uint32_t x) -> uint32_t {
       const uint32_t r = r0 + x;
       if (shape[r] < 0) return kKeys;
-      return narrow[r] ? (uint32_t)shape[r] * kSwMaxLen + (kSwMaxLen - 1 - pairs[r].hap_len)
-                       : kSwShapes * kSwMaxLen + (uint32_t)shape[r];
+      return ((narrow[r] ? 0u : (uint32_t)kSwShapes) + (uint32_t)shape[r]) * kSwMaxLen +
+             (kSwMaxLen - 1 - pairs[r].hap_len);
     };
     std::vector<uint32_t>& cnt = w.keys;
     cnt.assign(kKeys + 1, 0);
