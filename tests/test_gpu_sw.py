@@ -177,6 +177,48 @@ def test_cigar_longer_than_its_slot():
     assert e.value.code == rv.RVC_E_LIMIT
 
 
+def test_concurrent_callers_equal_alone():
+    # workers align from their own threads (each gets its own workspace) while
+    # others run call_pairhmm on the same device
+    say("aligner from 6 threads and call_pairhmm from 2 at once equal each call alone")
+    import threading
+
+    from rovaca_amd import workload
+    batches = []
+    for t in range(6):
+        pairs = random_pairs(60 + t, 150)
+        batches.append(([h for h, _ in pairs], [r for _, r in pairs], list(range(len(pairs))), bool(t % 2)))
+    calls = []
+    rng = np.random.default_rng(74)
+    for reg in [r for r in workload.batched_regions(12, seed=73) if r["n_haps"] * r["n_reads"] < 4000][:2]:
+        H, N = reg["n_haps"], reg["n_reads"]
+        haps = [reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]] for h in range(H)]
+        reads = [reg["read_bases"][reg["read_off"][r]:reg["read_off"][r + 1]] for r in range(N)]
+        quals = [reg["q"][reg["read_off"][r]:reg["read_off"][r + 1]] for r in range(N)]
+        calls.append((haps, reads, quals, rng.integers(20, 61, N).astype(np.uint8)))
+    alone_sw = [rv.sw_align_batch(*b) for b in batches]
+    alone_ph = [rv.call_pairhmm(*c) for c in calls]
+    got_sw, got_ph = [None] * len(batches), [None] * len(calls)
+
+    def sw(x):
+        got_sw[x] = rv.sw_align_batch(*batches[x])
+
+    def ph(x):
+        got_ph[x] = rv.call_pairhmm(*calls[x])
+
+    th = [threading.Thread(target=sw, args=(x,)) for x in range(len(batches))]
+    th += [threading.Thread(target=ph, args=(x,)) for x in range(len(calls))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert got_sw == alone_sw
+    for (rc0, m0, k0), (rc1, m1, k1) in zip(alone_ph, got_ph):
+        assert rc0 == 0 and rc1 == 0 and np.array_equal(k0, k1)
+        assert np.array_equal(np.isnan(m0), np.isnan(m1))
+        assert np.array_equal(m0[~np.isnan(m0)], m1[~np.isnan(m1)])
+
+
 def test_engine_restart_and_folded_shapes():
     # shutdown releases the aligner's per-thread workspace; the next call
     # after init builds a new one. With the default setting the sparse shapes
