@@ -443,7 +443,7 @@ def main():
     dom = max((k for k in kernels if k["precision"] == 32), key=lambda k: k["ms"])
     dom_tflops = dom["cells"] * FLOPS_PER_CELL / (dom["ms"] * 1e-3) / 1e12
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{wname.replace('_regions', '')}.json")
+    tpath = os.path.join(ROOT, "profiles", f"k_fp32_{wname.replace('_regions', '')}_hbm_traffic.json")
     if os.path.exists(tpath):
         t = json.load(open(tpath))["kernels"].get(dom["name"])
         if t:
