@@ -268,12 +268,10 @@ def run_calls(args):
                                             "mq": "U[20,60]"}, "runs": out}
 
 
-def run_sw(args):
-    """Realignment of reads to their best haplotype (the next kernel after
-    PairHMM on the same region): every read of the first regions of the
-    batched workload against the haplotype the PairHMM matrix ranks first,
-    through the batched device aligner. The CPU baseline is Rovaca's own
-    aligner built from its sources (oracle/_ref) on the host's cores."""
+def realign_inputs(args):
+    """The first regions of the batched workload and, per region, its
+    haplotypes, its reads and each read's best haplotype (the PairHMM matrix's
+    largest entry in the read's row; reads without one are left out)."""
     import rovaca_amd as rv
     from rovaca_amd import workload
     regions = workload.batched_regions(min(args.regions, 1024), seed=1)
@@ -281,19 +279,68 @@ def run_sw(args):
     b = rv.Batch(regions)
     b.run()
     lk_all, _ = b.results()
-    haps, reads, read_hap = [], [], []
+    per_region = []
     for x, reg in enumerate(regions):
         lk = b.region_result(lk_all, x)  # [n_reads][n_haps]
-        h0 = len(haps)
-        for h in range(reg["n_haps"]):
-            haps.append(reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]].tobytes())
+        haps = [reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]].tobytes() for h in range(reg["n_haps"])]
+        reads, best = [], []
         for r in range(reg["n_reads"]):
             row = lk[r]
             if np.all(np.isnan(row)):
                 continue
             reads.append(reg["read_bases"][reg["read_off"][r]:reg["read_off"][r + 1]].tobytes())
-            read_hap.append(h0 + int(np.nanargmax(row)))
+            best.append(int(np.nanargmax(row)))
+        per_region.append((haps, reads, best))
     b.close()
+    return regions, per_region
+
+
+def run_sw_calls(args):
+    """Realignment the way Rovaca's workers would call it: one call per region
+    (its reads against their best haplotypes) from 'producers' native worker
+    threads (rvc_loadgen_sw_align), each region's batch packed beforehand,
+    through the served call (concurrent calls merged into shared device
+    passes) and, for comparison, the batched call made per region. GCUPS over
+    the whole run and per-call latency; the best of 'steps' runs after an
+    untimed one."""
+    import rovaca_amd as rv
+    regions, per_region = realign_inputs(args)
+    batches = [rv.SwBatch(h, r, b) for h, r, b in per_region if r]
+    n_thr = max(1, args.producers)
+    log(f"sw_calls: {len(batches)} region calls from {n_thr} threads")
+
+    def stream(served):
+        runs = [rv.loadgen_sw_align(batches, served, n_thr) for _ in range(max(1, args.steps) + 1)][1:]
+        res, lat = min(runs, key=lambda r: r[0]["wall_s"])
+        return {"gcups": round(res["cells"] / res["wall_s"] / 1e9, 1), "wall_ms": round(res["wall_s"] * 1e3, 2),
+                "p50_ms": round(res["lat_p50_ms"], 3), "p99_ms": round(res["lat_p99_ms"], 3)}, res["cells"]
+
+    served, cells = stream(True)
+    direct, _ = stream(False)
+    rv.shutdown()
+    return {"metric": f"Smith-Waterman realignment, one served call per region from {n_thr} threads (1 MI355X)",
+            "value": served["gcups"], "unit": "GCUPS", "higher_is_better": True, "n_gpus": 1,
+            "dtype": "int16", "data": "synthetic",
+            "config": {"workload": f"sw_realign_calls_{len(regions)}_regions", "calls": len(batches), "cells": cells,
+                       "producers": n_thr},
+            "served": served, "batched_call_per_region": direct,
+            "mean_reads_per_call": round(sum(len(r) for _, r, _ in per_region) / max(1, len(batches)), 1)}
+
+
+def run_sw(args):
+    """Realignment of reads to their best haplotype (the next kernel after
+    PairHMM on the same region): every read of the first regions of the
+    batched workload against the haplotype the PairHMM matrix ranks first,
+    through the batched device aligner. The CPU baseline is Rovaca's own
+    aligner built from its sources (oracle/_ref) on the host's cores."""
+    import rovaca_amd as rv
+    regions, per_region = realign_inputs(args)
+    haps, reads, read_hap = [], [], []
+    for rh, rr, rb in per_region:
+        h0 = len(haps)
+        haps += rh
+        reads += rr
+        read_hap += [h0 + x for x in rb]
     cells = sum(len(haps[h]) * len(r) for h, r in zip(read_hap, reads))
     log(f"sw: {len(reads)} reads of {len(regions)} regions against their best haplotype, {cells / 1e9:.1f} Gcells")
     # the batch packed once and run repeatedly with the same result buffers
@@ -345,7 +392,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "stress", "single", "served", "deep", "calls", "sw"])
+                    choices=["batched", "stress", "single", "served", "deep", "calls", "sw", "sw_calls"])
     ap.add_argument("--calls", type=int, default=8192, help="calls: region calls per run")
     ap.add_argument("--rates", default="2000,8000,32000", help="served: arrival rates, regions/s")
     ap.add_argument("--duration", type=float, default=1.0, help="served: seconds of arrivals per rate")
@@ -388,6 +435,9 @@ def main():
         return
     if args.workload == "sw":
         print(json.dumps(run_sw(args)), flush=True)
+        return
+    if args.workload == "sw_calls":
+        print(json.dumps(run_sw_calls(args)), flush=True)
         return
 
     t = time.time()

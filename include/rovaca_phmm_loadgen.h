@@ -4,7 +4,8 @@
  * No counterpart in Rovaca: it stands in for Rovaca's worker threads (the
  * boost::asio pool that calls call_pairhmm once per active region,
  * hc/haplotypecaller_engine.cpp:119,254) so the serving loop can be measured
- * under an arrival process. Used by the benchmark and the tests.
+ * under an arrival process, and the aligner's served form under Rovaca's
+ * worker pattern. Used by the benchmark and the tests.
  */
 #ifndef ROVACA_PHMM_LOADGEN_H_
 #define ROVACA_PHMM_LOADGEN_H_
@@ -12,6 +13,7 @@
 #include <stdint.h>
 
 #include "rovaca_phmm.h"
+#include "rovaca_sw.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -46,6 +48,33 @@ int rvc_loadgen_run(const rvc_region* regions, uint32_t n_regions, const uint32_
 int rvc_loadgen_call_pairhmm(const rvc_raw_region* regions, uint32_t n_regions, const uint32_t* order, uint32_t n_calls,
                              int32_t min_quality_threshold, int32_t pcr_option, uint32_t n_workers, double* latency_ms,
                              rvc_loadgen_result* result);
+
+/* One region's realignment call: the arguments of rvc_sw_align_batch. */
+typedef struct rvc_sw_region {
+  const uint8_t* hap_bases;
+  const int32_t* hap_off;
+  uint32_t n_haps;
+  const uint8_t* read_bases;
+  const int32_t* read_off;
+  uint32_t n_reads;
+  const uint32_t* read_hap;
+  uint32_t max_cigar;
+  uint32_t* cigar;
+  int32_t* count;
+  int32_t* offset;
+  int32_t* score;
+} rvc_sw_region;
+
+/* Rovaca's realignment pattern (realign_reads_to_their_best_haplotype once
+ * per region, hc/haplotypecaller_engine.cpp:268, from every worker thread):
+ * n_workers threads take the n_regions regions one at a time and align each
+ * with one call, rvc_sw_align_served (served != 0) or rvc_sw_align_batch,
+ * with the genotyping or (assemble != 0) the assembly scores. latency_ms
+ * (optional, n_regions entries) receives each call's duration;
+ * result->cells sums haplotype length x read length over the aligned pairs,
+ * result->wall_s spans the first call to the last return. */
+int rvc_loadgen_sw_align(const rvc_sw_region* regions, uint32_t n_regions, int32_t assemble, int32_t served,
+                         uint32_t n_workers, double* latency_ms, rvc_loadgen_result* result);
 
 #ifdef __cplusplus
 }

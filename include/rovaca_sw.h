@@ -49,6 +49,19 @@ int rvc_sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_
                        uint32_t max_cigar, uint32_t* cigar, int32_t* count, int32_t* offset, int32_t* score,
                        rvc_sw_stats* stats);
 
+/* The same call for callers that realign one region at a time from many
+ * threads, as Rovaca's workers do (realign_reads_to_their_best_haplotype,
+ * hc/genotype/utils/assembly_based_caller_utils.h:48, called once per region
+ * at hc/haplotypecaller_engine.cpp:268 from each worker thread): same
+ * arguments, results and errors as rvc_sw_align_batch. Concurrent calls are
+ * merged into shared device passes by the engine's serving threads; the call
+ * returns when its own results are in place. A call whose CIGAR does not fit
+ * max_cigar fails alone (RVC_E_LIMIT); calls during rvc_phmm_shutdown get
+ * RVC_E_STATE. */
+int rvc_sw_align_served(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_haps, const uint8_t* read_bases,
+                        const int32_t* read_off, uint32_t n_reads, const uint32_t* read_hap, int32_t assemble,
+                        uint32_t max_cigar, uint32_t* cigar, int32_t* count, int32_t* offset, int32_t* score);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,7 +32,7 @@ EXPORTS = [
     "rvc_phmm_call_pairhmm", "rvc_phmm_batch_create", "rvc_phmm_batch_run", "rvc_phmm_batch_results",
     "rvc_phmm_batch_info", "rvc_phmm_batch_destroy", "rvc_phmm_submit", "rvc_phmm_wait", "rvc_phmm_poll",
     "rvc_phmm_server_stats", "rvc_phmm_preprocess_read", "rvc_phmm_postprocess", "rvc_phmm_tables",
-    "rvc_loadgen_run", "rvc_loadgen_call_pairhmm", "rvc_phmm_batch_kernels", "rvc_sw_align_batch",
+    "rvc_loadgen_run", "rvc_loadgen_call_pairhmm", "rvc_phmm_batch_kernels", "rvc_sw_align_batch", "rvc_sw_align_served", "rvc_loadgen_sw_align",
 ]
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -427,17 +427,25 @@ class SwBatch:
         self.off = np.zeros(self.n, np.int32)
         self.sc = np.zeros(self.n, np.int32)
 
-    def run(self):
-        """Aligns every read; returns the call's statistics as a dict."""
+    def run(self, served=False):
+        """Aligns every read; returns the call's statistics as a dict (the
+        batched call), or None (served: rvc_sw_align_served, merged with
+        other threads' concurrent calls)."""
+        args = [self.hb.ctypes.data_as(_u8p), self.hap_off.ctypes.data_as(_i32p), self.n_haps,
+                self.rb.ctypes.data_as(_u8p), self.read_off.ctypes.data_as(_i32p), self.n,
+                self.rh.ctypes.data_as(_u32p), self.assemble, self.max_cigar, self.cig.ctypes.data_as(_u32p),
+                self.cnt.ctypes.data_as(_i32p), self.off.ctypes.data_as(_i32p), self.sc.ctypes.data_as(_i32p)]
+        types = [_u8p, _i32p, ctypes.c_uint32, _u8p, _i32p, ctypes.c_uint32, _u32p, ctypes.c_int32,
+                 ctypes.c_uint32, _u32p, _i32p, _i32p, _i32p]
+        if served:
+            fn = lib().rvc_sw_align_served
+            fn.argtypes = types
+            _check(fn(*args))
+            return None
         fn = lib().rvc_sw_align_batch
-        fn.argtypes = [_u8p, _i32p, ctypes.c_uint32, _u8p, _i32p, ctypes.c_uint32, _u32p, ctypes.c_int32,
-                       ctypes.c_uint32, _u32p, _i32p, _i32p, _i32p, ctypes.POINTER(RvcSwStats)]
+        fn.argtypes = types + [ctypes.POINTER(RvcSwStats)]
         st = RvcSwStats()
-        _check(fn(self.hb.ctypes.data_as(_u8p), self.hap_off.ctypes.data_as(_i32p), self.n_haps,
-                  self.rb.ctypes.data_as(_u8p), self.read_off.ctypes.data_as(_i32p), self.n,
-                  self.rh.ctypes.data_as(_u32p), self.assemble, self.max_cigar, self.cig.ctypes.data_as(_u32p),
-                  self.cnt.ctypes.data_as(_i32p), self.off.ctypes.data_as(_i32p), self.sc.ctypes.data_as(_i32p),
-                  ctypes.byref(st)))
+        _check(fn(*args, ctypes.byref(st)))
         return {f: getattr(st, f) for f, _ in RvcSwStats._fields_}
 
     def results(self):
@@ -454,12 +462,35 @@ class SwBatch:
         return out
 
 
-def sw_align_batch(haps, reads, read_hap, assemble=False, with_stats=False):
+class RvcSwRegion(ctypes.Structure):
+    _fields_ = [("hap_bases", _u8p), ("hap_off", _i32p), ("n_haps", ctypes.c_uint32), ("read_bases", _u8p),
+                ("read_off", _i32p), ("n_reads", ctypes.c_uint32), ("read_hap", _u32p), ("max_cigar", ctypes.c_uint32),
+                ("cigar", _u32p), ("count", _i32p), ("offset", _i32p), ("score", _i32p)]
+
+
+def loadgen_sw_align(batches, served=True, n_workers=32, assemble=False):
+    """One aligner call per SwBatch (a region's reads) from n_workers C++
+    threads, served (rvc_sw_align_served) or batched per call. Results land
+    in each SwBatch. Returns (summary, latency ms per call)."""
+    arr = (RvcSwRegion * len(batches))()
+    for x, b in enumerate(batches):
+        arr[x] = RvcSwRegion(b.hb.ctypes.data_as(_u8p), b.hap_off.ctypes.data_as(_i32p), b.n_haps,
+                             b.rb.ctypes.data_as(_u8p), b.read_off.ctypes.data_as(_i32p), b.n,
+                             b.rh.ctypes.data_as(_u32p), b.max_cigar, b.cig.ctypes.data_as(_u32p),
+                             b.cnt.ctypes.data_as(_i32p), b.off.ctypes.data_as(_i32p), b.sc.ctypes.data_as(_i32p))
+    lat = np.zeros(len(batches), np.float64)
+    res = RvcLoadgenResult()
+    _check(lib().rvc_loadgen_sw_align(arr, len(batches), 1 if assemble else 0, 1 if served else 0, n_workers,
+                                      _ptr(lat, _f64p), ctypes.byref(res)))
+    return {k: getattr(res, k) for k, _ in RvcLoadgenResult._fields_}, lat
+
+
+def sw_align_batch(haps, reads, read_hap, assemble=False, with_stats=False, served=False):
     """Aligns reads[r] against haps[read_hap[r]] on the device (the aligner of
     Rovaca's read realignment). Returns [(cigar [(op, length)], offset, score)]
     per read, (None, 1, None) where the aligner refuses the pair; with
-    with_stats, also the call's statistics as a dict."""
+    with_stats, also the call's statistics as a dict (None when served)."""
     b = SwBatch(haps, reads, read_hap, assemble)
-    st = b.run()
+    st = b.run(served)
     out = b.results()
     return (out, st) if with_stats else out

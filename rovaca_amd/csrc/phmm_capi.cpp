@@ -141,6 +141,7 @@ int rvc_phmm_init(const rvc_config* cfg) {
 int rvc_phmm_shutdown(void) {
   server_stop();
   release_workspaces();
+  sw_server_stop();
   sw_release_workspaces();
   return engine_shutdown();
 }

@@ -1,4 +1,4 @@
-// Load generator for the serving loop (see include/rovaca_phmm_loadgen.h).
+// Load generator for the serving loop and the served aligner (see include/rovaca_phmm_loadgen.h).
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -167,6 +167,67 @@ extern "C" int rvc_loadgen_call_pairhmm(const rvc_raw_region* regions, uint32_t 
     for (uint32_t h = 0; h < rg.n_haps; ++h) hs += (uint64_t)(rg.hap_off[h + 1] - rg.hap_off[h]);
     for (uint32_t r = 0; r < rg.n_reads; ++r) rs += (uint64_t)(rg.read_off[r + 1] - rg.read_off[r]);
     result->cells += hs * rs;
+    ok_lat.push_back(lat[i]);
+    if (latency_ms) latency_ms[i] = lat[i];
+    first = std::min(first, t0[i]);
+    last = std::max(last, t1[i]);
+  }
+  if (!ok_lat.empty()) {
+    result->wall_s = (double)(last - first) / 1e9;
+    result->lat_p50_ms = pct(ok_lat, 0.50);
+    result->lat_p99_ms = pct(ok_lat, 0.99);
+    result->lat_max_ms = pct(ok_lat, 1.0);
+    double sum = 0;
+    for (double x : ok_lat) sum += x;
+    result->lat_mean_ms = sum / (double)ok_lat.size();
+  }
+  return result->failed ? set_error(RVC_E_DEVICE, "%llu calls failed", (unsigned long long)result->failed) : RVC_OK;
+}
+
+extern "C" int rvc_loadgen_sw_align(const rvc_sw_region* regions, uint32_t n_regions, int32_t assemble, int32_t served,
+                                    uint32_t n_workers, double* latency_ms, rvc_loadgen_result* result) {
+  using rvc::set_error;
+  if (!regions || !result || n_workers == 0) return set_error(RVC_E_INVALID, "bad arguments");
+  std::vector<double> lat(n_regions, 0);
+  std::vector<int> rcs(n_regions, 0);
+  std::vector<uint64_t> t0(n_regions, 0), t1(n_regions, 0);
+  std::atomic<uint32_t> next{0};
+  auto worker = [&] {
+    for (;;) {
+      const uint32_t i = next.fetch_add(1, std::memory_order_relaxed);
+      if (i >= n_regions) return;
+      const rvc_sw_region& g = regions[i];
+      const Clock::time_point a = Clock::now();
+      rcs[i] = served ? rvc_sw_align_served(g.hap_bases, g.hap_off, g.n_haps, g.read_bases, g.read_off, g.n_reads,
+                                            g.read_hap, assemble, g.max_cigar, g.cigar, g.count, g.offset, g.score)
+                      : rvc_sw_align_batch(g.hap_bases, g.hap_off, g.n_haps, g.read_bases, g.read_off, g.n_reads,
+                                           g.read_hap, assemble, g.max_cigar, g.cigar, g.count, g.offset, g.score,
+                                           nullptr);
+      const Clock::time_point b = Clock::now();
+      t0[i] = ns_of(a);
+      t1[i] = ns_of(b);
+      lat[i] = std::chrono::duration<double, std::milli>(b - a).count();
+    }
+  };
+  std::vector<std::thread> th;
+  for (uint32_t w = 0; w < n_workers; ++w) th.emplace_back(worker);
+  for (auto& t : th) t.join();
+  std::memset(result, 0, sizeof *result);
+  std::vector<double> ok_lat;
+  uint64_t first = UINT64_MAX, last = 0;
+  for (uint32_t i = 0; i < n_regions; ++i) {
+    result->submitted++;
+    if (rcs[i] != RVC_OK) {
+      result->failed++;
+      if (latency_ms) latency_ms[i] = -1;
+      continue;
+    }
+    result->completed++;
+    const rvc_sw_region& g = regions[i];
+    for (uint32_t r = 0; r < g.n_reads; ++r) {
+      const uint32_t h = g.read_hap[r];
+      result->cells += (uint64_t)(g.hap_off[h + 1] - g.hap_off[h]) * (uint64_t)(g.read_off[r + 1] - g.read_off[r]);
+    }
     ok_lat.push_back(lat[i]);
     if (latency_ms) latency_ms[i] = lat[i];
     first = std::min(first, t0[i]);

@@ -24,6 +24,7 @@ namespace rvc {
 namespace {
 
 constexpr uint32_t kPackCols = 64;           // CIGAR elements per pair copied back packed
+constexpr uint64_t kThreadedUpload = 1u << 20;  // bases from which helper threads upload them
 
 // scores of Rovaca's two aligner instances (sw_avx.c:14-15)
 constexpr int32_t kGenotype[4] = {10, -15, -30, -5};
@@ -151,25 +152,40 @@ void sw_release_workspaces() {
   g_sw_generation.fetch_add(1, std::memory_order_release);
 }
 
-int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_haps, const uint8_t* read_bases,
-                   const int32_t* read_off, uint32_t n_reads, const uint32_t* read_hap, int32_t assemble,
-                   uint32_t max_cigar, uint32_t* cigar, int32_t* count, int32_t* offset, int32_t* score,
-                   rvc_sw_stats* stats) {
-  const auto t_start = std::chrono::steady_clock::now();
+int sw_validate(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_haps, const uint8_t* read_bases,
+                const int32_t* read_off, uint32_t n_reads, const uint32_t* read_hap, uint32_t max_cigar,
+                const uint32_t* cigar, const int32_t* count, const int32_t* offset, const int32_t* score,
+                int32_t* longest) {
   if (!g_engine.ready) return set_error(RVC_E_STATE, "rvc_phmm_init has not run");
+  if (longest) *longest = 0;
   if (n_reads == 0) return RVC_OK;
   if (!hap_bases || !hap_off || !read_bases || !read_off || !read_hap || !cigar || !count || !offset || !score ||
       n_haps == 0 || max_cigar == 0)
     return set_error(RVC_E_INVALID, "null pointer or empty haplotype list");
   for (uint32_t h = 0; h < n_haps; ++h)
     if (hap_off[h] < 0 || hap_off[h + 1] < hap_off[h]) return set_error(RVC_E_INVALID, "haplotype offsets");
+  int32_t most = 0;
   for (uint32_t r = 0; r < n_reads; ++r) {
     if (read_off[r] < 0 || read_off[r + 1] < read_off[r]) return set_error(RVC_E_INVALID, "read offsets");
     if (read_hap[r] >= n_haps) return set_error(RVC_E_INVALID, "read %u names haplotype %u", r, read_hap[r]);
     const int32_t n2 = read_off[r + 1] - read_off[r];
     if (n2 > kSwMaxRead && n2 < kSwMaxLen)
       return set_error(RVC_E_LIMIT, "read %u has %d bases (the device aligner takes up to %d)", r, n2, kSwMaxRead);
+    most = std::max(most, n2);
   }
+  if (longest) *longest = most;
+  return RVC_OK;
+}
+
+int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_haps, const uint8_t* read_bases,
+                   const int32_t* read_off, uint32_t n_reads, const uint32_t* read_hap, int32_t assemble,
+                   uint32_t max_cigar, uint32_t* cigar, int32_t* count, int32_t* offset, int32_t* score,
+                   rvc_sw_stats* stats) {
+  const auto t_start = std::chrono::steady_clock::now();
+  if (int rc = sw_validate(hap_bases, hap_off, n_haps, read_bases, read_off, n_reads, read_hap, max_cigar, cigar,
+                           count, offset, score, nullptr))
+    return rc;
+  if (n_reads == 0) return RVC_OK;
   const int32_t* sc = assemble ? kAssemble : kGenotype;
   if (stats) std::memset(stats, 0, sizeof *stats);
 
@@ -186,23 +202,35 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
   const size_t o_read_in = (size_t)up256(hap_bytes + 64);
   if (int rc = w.ensure_in(o_read_in + up256(read_bytes + 64))) return rc;
   char* in = static_cast<char*>(w.in_mem);
+  // (small batches, such as one region's reads, go up from this thread: a
+  // thread costs more than their copy)
   hipError_t up_err = hipSuccess, up_err2 = hipSuccess;
   const uint64_t half = read_bytes / 2;
-  std::thread uploader([&] {
+  auto upload_first = [&] {
     hipError_t e = hipSetDevice(g_engine.device);
     if (e == hipSuccess) e = hipMemcpyAsync(in, hap_bases, hap_bytes, hipMemcpyHostToDevice, w.up_stream);
     if (e == hipSuccess) e = hipMemcpyAsync(in + o_read_in, read_bases, half, hipMemcpyHostToDevice, w.up_stream);
     if (e == hipSuccess) e = hipEventRecord(w.up_done, w.up_stream);
     up_err = e;
-  });
-  std::thread uploader2([&] {
+  };
+  auto upload_second = [&] {
     hipError_t e = hipSetDevice(g_engine.device);
     if (e == hipSuccess)
       e = hipMemcpyAsync(in + o_read_in + half, read_bases + half, read_bytes - half, hipMemcpyHostToDevice,
                          w.up_stream2);
     if (e == hipSuccess) e = hipEventRecord(w.up_done2, w.up_stream2);
     up_err2 = e;
-  });
+  };
+  const bool threaded = hap_bytes + read_bytes >= kThreadedUpload;
+  std::thread uploader, uploader2;
+  if (threaded) {
+    uploader = std::thread(upload_first);
+    uploader2 = std::thread(upload_second);
+  } else {
+    upload_first();
+    upload_second();
+  }
+  bool uploads_pending = true;
   struct Join {
     std::thread& t;
     ~Join() {
@@ -375,9 +403,10 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
       e = hipMemcpyAsync(base + o_list, list, 4ull * nlist, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(base + o_head, 0, 4 * 2 * kSwShapes, st);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "aligner upload: %s", hipGetErrorString(e));
-    if (uploader.joinable()) {
-      uploader.join();
-      uploader2.join();
+    if (uploads_pending) {
+      uploads_pending = false;
+      if (uploader.joinable()) uploader.join();
+      if (uploader2.joinable()) uploader2.join();
       if (up_err == hipSuccess) up_err = up_err2;
       if (up_err == hipSuccess) up_err = hipStreamWaitEvent(st, w.up_done, 0);
       if (up_err == hipSuccess) up_err = hipStreamWaitEvent(st, w.up_done2, 0);

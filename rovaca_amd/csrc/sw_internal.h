@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "../../include/rovaca_sw.h"
+
 namespace rvc {
 
 constexpr int kSwMaxLen = 2048;       // the reference aligner refuses lengths >= this (sw_avx.c:18, sw_core_template.h:341)
@@ -88,5 +90,23 @@ hipError_t launch_sw_trace(hipStream_t stream, const SwArgs& a);
 int sw_blocks_per_cu(int shape);
 int sw16_blocks_per_cu(int shape);
 void sw_release_workspaces();  // the aligner's per-thread workspaces (engine shutdown)
+
+// Host side of the aligner (sw_capi.cpp, sw_server.cpp). sw_validate checks
+// a call's arguments as the batched entry point does and reports its longest
+// read; sw_align_batch is rvc_sw_align_batch; sw_align_served is
+// rvc_sw_align_served; sw_server_stop ends the serving threads (engine
+// shutdown, before the workspaces are released).
+int sw_validate(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_haps, const uint8_t* read_bases,
+                const int32_t* read_off, uint32_t n_reads, const uint32_t* read_hap, uint32_t max_cigar,
+                const uint32_t* cigar, const int32_t* count, const int32_t* offset, const int32_t* score,
+                int32_t* longest);
+int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_haps, const uint8_t* read_bases,
+                   const int32_t* read_off, uint32_t n_reads, const uint32_t* read_hap, int32_t assemble,
+                   uint32_t max_cigar, uint32_t* cigar, int32_t* count, int32_t* offset, int32_t* score,
+                   rvc_sw_stats* stats);
+int sw_align_served(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_haps, const uint8_t* read_bases,
+                    const int32_t* read_off, uint32_t n_reads, const uint32_t* read_hap, int32_t assemble,
+                    uint32_t max_cigar, uint32_t* cigar, int32_t* count, int32_t* offset, int32_t* score);
+void sw_server_stop();
 
 }  // namespace rvc

@@ -219,6 +219,84 @@ def test_concurrent_callers_equal_alone():
         assert np.array_equal(m0[~np.isnan(m0)], m1[~np.isnan(m1)])
 
 
+def test_served_calls_equal_batch():
+    # one region per call from many threads, merged by the serving threads
+    say("served calls from 12 threads equal the batched calls")
+    import threading
+    batches = []
+    for t in range(12):
+        pairs = random_pairs(80 + t, 40 + 13 * t)
+        batches.append(([h for h, _ in pairs], [r for _, r in pairs], list(range(len(pairs))), t % 3 == 0))
+    want = [rv.sw_align_batch(*b) for b in batches]
+    got = [None] * len(batches)
+
+    def worker(x):
+        got[x] = rv.sw_align_batch(*batches[x], served=True)
+
+    th = [threading.Thread(target=worker, args=(x,)) for x in range(len(batches))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert got == want
+    for (haps, reads, rh, asm), g in zip(batches[:3], got[:3]):
+        for r in range(len(reads)):
+            assert g[r] == oracle.sw_align(haps[rh[r]], reads[r], asm)
+
+
+def test_served_errors_stay_with_their_call():
+    say("a served call that fails does not fail the calls merged with it")
+    import threading
+    rng = np.random.default_rng(43)
+    hap = ACGT[rng.integers(0, 4, 300)].tobytes()
+    gapped = hap[10:60] + b"T" + hap[60:110]  # three CIGAR elements
+    good = [random_pairs(90 + t, 60) for t in range(6)]
+    out, errs = [None] * 8, [None] * 8
+
+    def ok(x):
+        p = good[x]
+        out[x] = rv.sw_align_batch([h for h, _ in p], [r for _, r in p], list(range(len(p))), served=True)
+
+    def too_long_cigar(x):
+        try:
+            rv.SwBatch([hap], [gapped], [0], max_cigar=2).run(served=True)
+        except rv.EngineError as e:
+            errs[x] = e.code
+
+    def too_long_read(x):
+        try:
+            rv.sw_align_batch([b"A" * 1200], [b"A" * 1030], [0], served=True)
+        except rv.EngineError as e:
+            errs[x] = e.code
+
+    th = [threading.Thread(target=ok, args=(x,)) for x in range(6)]
+    th += [threading.Thread(target=too_long_cigar, args=(6,)), threading.Thread(target=too_long_read, args=(7,))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert errs[6] == rv.RVC_E_LIMIT and errs[7] == rv.RVC_E_LIMIT
+    for x in range(6):
+        p = good[x]
+        assert out[x] == [oracle.sw_align(h, r) for h, r in p]
+
+
+def test_native_workers_served_and_batched():
+    # Rovaca's worker pattern from C++ threads: one call per region, served
+    # and batched; every region's results equal its own batched call
+    say("64 region calls from 16 native workers, served and batched")
+    batches, want = [], []
+    for t in range(64):
+        pairs = random_pairs(200 + t, 20 + t)
+        haps, reads = [h for h, _ in pairs], [r for _, r in pairs]
+        batches.append(rv.SwBatch(haps, reads, list(range(len(pairs)))))
+        want.append(rv.sw_align_batch(haps, reads, list(range(len(pairs)))))
+    for served in (True, False):
+        res, lat = rv.loadgen_sw_align(batches, served=served, n_workers=16)
+        assert res["completed"] == 64 and res["failed"] == 0 and (lat > 0).all()
+        assert [b.results() for b in batches] == want
+
+
 def test_engine_restart_and_folded_shapes():
     # shutdown releases the aligner's per-thread workspace; the next call
     # after init builds a new one. With the default setting the sparse shapes
@@ -228,8 +306,10 @@ def test_engine_restart_and_folded_shapes():
     haps = [h for h, _ in pairs]
     reads = [r for _, r in pairs]
     first = check(haps, reads, list(range(len(pairs))))
+    rv.sw_align_batch(haps, reads, list(range(len(pairs))), served=True)  # the serving threads started
     rv.shutdown()
     rv.init(0)
     assert rv.sw_align_batch(haps, reads, list(range(len(pairs)))) == first
+    assert rv.sw_align_batch(haps, reads, list(range(len(pairs))), served=True) == first
     for assemble in (False, True):
         check(haps, reads, list(range(len(pairs))), assemble)
