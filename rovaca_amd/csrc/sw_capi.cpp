@@ -25,6 +25,7 @@ namespace {
 
 constexpr uint32_t kPackCols = 64;           // CIGAR elements per pair copied back packed
 constexpr uint64_t kThreadedUpload = 1u << 20;  // bases from which helper threads upload them
+constexpr int kSubStreams = 3;                   // streams the launches of a pass are forked over
 
 // scores of Rovaca's two aligner instances (sw_avx.c:14-15)
 constexpr int32_t kGenotype[4] = {10, -15, -30, -5};
@@ -67,6 +68,11 @@ struct PinnedArea {
 struct SwWorkspace {
   hipStream_t stream = nullptr;
   hipEvent_t ev[3] = {};
+  // the launches of one pass (a sweep, then its pairs' best end and
+  // traceback) forked over streams of their own, so that small passes run
+  // their shapes at once
+  hipStream_t sub[kSubStreams] = {};
+  hipEvent_t sub_fill[kSubStreams] = {}, sub_done[kSubStreams] = {};
   void* mem = nullptr;
   size_t bytes = 0;
   // the caller's bases, uploaded from a helper thread on their own stream
@@ -86,6 +92,11 @@ struct SwWorkspace {
       if (e) hipEventDestroy(e);
     if (up_done) hipEventDestroy(up_done);
     if (up_done2) hipEventDestroy(up_done2);
+    for (int k = 0; k < kSubStreams; ++k) {
+      if (sub_fill[k]) hipEventDestroy(sub_fill[k]);
+      if (sub_done[k]) hipEventDestroy(sub_done[k]);
+      if (sub[k]) hipStreamDestroy(sub[k]);
+    }
     if (stream) hipStreamDestroy(stream);
     if (up_stream) hipStreamDestroy(up_stream);
     if (up_stream2) hipStreamDestroy(up_stream2);
@@ -102,6 +113,11 @@ struct SwWorkspace {
     if (hipEventCreateWithFlags(&up_done, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&up_done2, hipEventDisableTiming) != hipSuccess)
       return set_error(RVC_E_DEVICE, "aligner events");
+    for (int k = 0; k < kSubStreams; ++k)
+      if (hipStreamCreateWithFlags(&sub[k], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreate(&sub_fill[k]) != hipSuccess ||
+          hipEventCreateWithFlags(&sub_done[k], hipEventDisableTiming) != hipSuccess)
+        return set_error(RVC_E_DEVICE, "aligner streams");
     return RVC_OK;
   }
   int ensure_in(size_t want) {
@@ -430,12 +446,21 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     a.offset = reinterpret_cast<int32_t*>(base + o_off);
     a.score = reinterpret_cast<int32_t*>(base + o_sc);
     a.best_ij = reinterpret_cast<int32_t*>(base + o_best);
+    // each launch (a shape's sweep, then its pairs' best end and traceback)
+    // on the next of the sub streams, forked from and joined back into st
     if (hipEventRecord(w.ev[0], st) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");
-    // packed sweeps (a work item is a duo), then int32 sweeps
+    bool used[kSubStreams] = {};
+    int next_sub = 0;
     for (int packed = 1; packed >= 0; --packed) {
       for (int s = 0; s < kSwShapes; ++s) {
         const uint32_t nl = packed ? dcount[s] : scount[s];
         if (nl == 0) continue;
+        const int k = next_sub;
+        next_sub = (next_sub + 1) % kSubStreams;
+        hipStream_t ss = w.sub[k];
+        if (!used[k] && (e = hipStreamWaitEvent(ss, w.ev[0], 0)) != hipSuccess)
+          return set_error(RVC_E_DEVICE, "aligner fork: %s", hipGetErrorString(e));
+        used[k] = true;
         a.list = reinterpret_cast<const uint32_t*>(base + o_list) + (packed ? dstart[s] : sstart[s]);
         a.n_list = nl;
         a.head = reinterpret_cast<uint32_t*>(base + o_head) + (packed ? s : kSwShapes + s);
@@ -443,16 +468,23 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
         const uint32_t waves = (nl + nseg - 1) / nseg;
         const int per_cu = packed ? sw16_blocks_per_cu(s) : sw_blocks_per_cu(s);
         const int grid = (int)std::max<uint32_t>(1, std::min<uint32_t>(waves, (uint32_t)per_cu * (uint32_t)g_engine.n_cu));
-        if ((e = packed ? launch_sw_fill16(s, grid, st, a) : launch_sw_fill(s, grid, st, a)) != hipSuccess)
+        if ((e = packed ? launch_sw_fill16(s, grid, ss, a) : launch_sw_fill(s, grid, ss, a)) != hipSuccess)
           return set_error(RVC_E_DEVICE, "aligner sweep: %s", hipGetErrorString(e));
+        if (hipEventRecord(w.sub_fill[k], ss) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");
+        a.n_list = packed ? 2 * nl : nl;  // a duo's two entries
+        if ((e = launch_sw_trace(ss, a)) != hipSuccess)
+          return set_error(RVC_E_DEVICE, "aligner traceback: %s", hipGetErrorString(e));
         if (stats) ++stats->launches;
       }
     }
+    for (int k = 0; k < kSubStreams; ++k) {
+      if (!used[k]) continue;
+      if (hipEventRecord(w.sub_done[k], w.sub[k]) != hipSuccess ||
+          hipStreamWaitEvent(st, w.sub_done[k], 0) != hipSuccess)
+        return set_error(RVC_E_DEVICE, "aligner join");
+    }
     a.list = nullptr;
     a.n_list = 0;
-    if (hipEventRecord(w.ev[1], st) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");
-    if ((e = launch_sw_trace(st, a)) != hipSuccess)
-      return set_error(RVC_E_DEVICE, "aligner traceback: %s", hipGetErrorString(e));
     if (hipEventRecord(w.ev[2], st) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");
     // counts first, then of every pair's CIGAR slot only the columns the
     // longest CIGAR of the chunk uses
@@ -465,6 +497,13 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
       std::memcpy(count + r0, res, 4ull * n);
       std::memcpy(offset + r0, res + n, 4ull * n);
       std::memcpy(score + r0, res + 2ull * n, 4ull * n);
+      // the pairs the library refuses (no sweep): count -1, offset 1
+      for (uint32_t x = 0; x < n; ++x)
+        if (shape[r0 + x] < 0) {
+          count[r0 + x] = -1;
+          offset[r0 + x] = 1;
+          score[r0 + x] = 0;
+        }
     }
     int32_t widest = 0;
     for (uint32_t x = 0; x < n && e == hipSuccess; ++x) widest = std::max(widest, count[r0 + x]);
@@ -489,11 +528,15 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     }
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "aligner results: %s", hipGetErrorString(e));
     if (stats) {
-      float f = 0, t = 0;
-      hipEventElapsedTime(&f, w.ev[0], w.ev[1]);
-      hipEventElapsedTime(&t, w.ev[1], w.ev[2]);
+      // sweeps: fork to the last sweep's end; best end and traceback: the rest
+      float f = 0, all = 0;
+      for (int k = 0; k < kSubStreams; ++k) {
+        float x = 0;
+        if (used[k] && hipEventElapsedTime(&x, w.ev[0], w.sub_fill[k]) == hipSuccess) f = std::max(f, x);
+      }
+      hipEventElapsedTime(&all, w.ev[0], w.ev[2]);
       stats->fill_ms += f;
-      stats->trace_ms += t;
+      stats->trace_ms += std::max(0.f, all - f);
       ++stats->chunks;
     }
     for (uint32_t x = 0; x < n; ++x)

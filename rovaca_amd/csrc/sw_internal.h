@@ -86,6 +86,8 @@ struct SwArgs {
 
 hipError_t launch_sw_fill(int shape, int grid, hipStream_t stream, const SwArgs& a);
 hipError_t launch_sw_fill16(int shape, int grid, hipStream_t stream, const SwArgs& a);
+// best end and traceback of the pairs a.list names (a.n_list entries,
+// kSwNoPair skipped)
 hipError_t launch_sw_trace(hipStream_t stream, const SwArgs& a);
 int sw_blocks_per_cu(int shape);
 int sw16_blocks_per_cu(int shape);

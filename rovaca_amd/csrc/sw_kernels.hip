@@ -366,9 +366,11 @@ __global__ __launch_bounds__(64) void k_sw_fill16(SwArgs a) {
 // maximum with coalesced loads, then applies the rules to the equal ones in
 // scan order.
 __global__ __launch_bounds__(256) void k_sw_best(SwArgs a) {
-  const uint32_t p = blockIdx.x * 4 + threadIdx.x / 64;
+  const uint32_t x = blockIdx.x * 4 + threadIdx.x / 64;
   const int l = threadIdx.x % 64;
-  if (p >= a.n_pairs) return;  // whole waves
+  if (x >= a.n_list) return;  // whole waves
+  const uint32_t p = a.list[x];
+  if (p == kSwNoPair) return;
   const SwPair pr = a.pairs[p];
   const int n1 = (int)pr.hap_len, n2 = (int)pr.read_len;
   if (n1 >= kSwMaxLen || n2 >= kSwMaxLen || n1 == 0 || n2 == 0) return;
@@ -416,17 +418,13 @@ __global__ __launch_bounds__(256) void k_sw_best(SwArgs a) {
 // One thread per pair: the traceback from the best end cell into CIGAR runs,
 // written start to end.
 __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n_pairs) return;
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.n_list) return;
+  const uint32_t p = a.list[x];
+  if (p == kSwNoPair) return;
   const SwPair pr = a.pairs[p];
-  const int n1 = (int)pr.hap_len, n2 = (int)pr.read_len;
+  const int n2 = (int)pr.read_len;
   uint32_t* out = a.cigar + (size_t)p * a.max_cigar;
-  if (n1 >= kSwMaxLen || n2 >= kSwMaxLen || n1 == 0 || n2 == 0) {
-    a.count[p] = -1;
-    a.offset[p] = 1;
-    a.score[p] = 0;
-    return;
-  }
   const int bi = a.best_ij[2 * p], bj = a.best_ij[2 * p + 1];
   const int rpl = (int)pr.rpl, gpl = sw_groups(rpl), wg = (int)pr.width * gpl;
   // 16-bit words: the pair's own, or every other one of a duo's 32-bit words
@@ -523,9 +521,9 @@ hipError_t launch_sw_fill(int shape, int grid, hipStream_t stream, const SwArgs&
 }
 
 hipError_t launch_sw_trace(hipStream_t stream, const SwArgs& a) {
-  if (a.n_pairs == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sw_best, dim3((a.n_pairs + 3) / 4), dim3(256), 0, stream, a);
-  hipLaunchKernelGGL(k_sw_trace, dim3((a.n_pairs + 63) / 64), dim3(64), 0, stream, a);
+  if (a.n_list == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sw_best, dim3((a.n_list + 3) / 4), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(k_sw_trace, dim3((a.n_list + 63) / 64), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
 
