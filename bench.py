@@ -13,8 +13,10 @@ the dominant FP32 kernel against MI355X's 157.3 TFLOP/s FP32 peak at 12 flops
 per cell, the plain one-region-per-launch GPU form on a sample, accuracy
 against the FP64 oracle and the reference routing on a sample, the same
 regions served one by one (Poisson arrivals from 32 producer threads, GCUPS
-with p50/p99 region latency per arrival rate), and the CPU baseline (Rovaca's
-region scheme on the host cores) on a bounded sample.
+with p50/p99 region latency per arrival rate), the CPU baseline (Rovaca's
+region scheme on the host cores) on a bounded sample, and the next kernel on
+the same regions: every read realigned to its best haplotype by the device
+aligner, as one batch and as one served call per region (`realign`).
 
 Other modes: --workload stress (precision stress, 2048 regions), single (one
 8 x 32 region, latency), served (the serving loop alone), deep (served with
@@ -554,6 +556,22 @@ def main():
                     result["served"]["rates"][-1]["gcups"] / result["cpu_baseline"]["value"], 1)
         except Exception as e:  # noqa: BLE001
             result["cpu_baseline"] = {"error": str(e)}
+    if not args.no_extras and args.workload == "batched":
+        # the next kernel on the same regions: every read realigned to its
+        # best haplotype, as one batch and as one served call per region
+        log("realignment aligner")
+        try:
+            sw = run_sw(argparse.Namespace(regions=1024, steps=3, no_cpu=args.no_cpu))
+            calls = run_sw_calls(argparse.Namespace(regions=1024, steps=2, producers=32))
+            result["realign"] = {"workload": sw["config"]["workload"], "pairs": sw["config"]["pairs"],
+                                 "cells": sw["config"]["cells"], "dtype": sw["dtype"],
+                                 "gcups_device": sw["value"], "device_ms": sw["device_ms"],
+                                 "gcups_with_copies": sw["gcups_with_copies"], "parity": sw["parity"],
+                                 "cpu_baseline": sw.get("cpu_baseline"),
+                                 "served_call_per_region_32_workers": calls["served"],
+                                 "batched_call_per_region_32_workers": calls["batched_call_per_region"]}
+        except Exception as e:  # noqa: BLE001
+            result["realign"] = {"error": str(e)}
     print(json.dumps(result), flush=True)
 
 
