@@ -1,10 +1,13 @@
 // Batched Smith-Waterman of reads against haplotypes: the C ABI of
-// include/rovaca_sw.h. Pairs are grouped by sweep shape (read length); the
-// pairs sw_narrow() admits are ordered by haplotype length and swept two per
-// lane by k_sw_fill16 (duos of neighbours in that order), the rest by
-// k_sw_fill, one launch per shape and sweep present; k_sw_trace traces all.
-// Batches whose backtrack bytes exceed the engine's budget (12 GiB unless
-// rvc_config says otherwise) run in chunks.
+// include/rovaca_sw.h (rvc_sw_align_batch; the served form is in
+// sw_server.cpp). Pairs are grouped by sweep shape (read length); the pairs
+// sw_narrow() admits are ordered by haplotype length, longest first, and
+// swept two per lane by k_sw_fill16 (duos of neighbours in that order), the
+// rest by k_sw_fill. Each shape and sweep present is one launch on one of a
+// few streams forked from the call's stream, followed on that stream by
+// k_sw_best and k_sw_trace for its pairs. Batches whose backtrack bytes
+// exceed the engine's budget (12 GiB unless rvc_config says otherwise) run in
+// chunks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

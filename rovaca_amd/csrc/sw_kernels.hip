@@ -21,7 +21,8 @@
 // row down unchanged (E open 0, no match, no deletion), so the segment's last
 // row holds H[i][n2] for the best-end search. Backtrack nibbles are stored per
 // step in lane order (coalesced, 16 bits per 4 rows); the traceback kernel
-// maps (i, j) back.
+// maps (i, j) back. The packed int16 sweep (k_sw_fill16, below) runs the same
+// recurrence two pairs per lane.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
