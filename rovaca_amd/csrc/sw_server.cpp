@@ -70,10 +70,39 @@ SwServer& server() {
 // CIGAR elements that always suffice for a read of n bases
 inline uint64_t slot_for(int32_t longest) { return 2ull * (uint64_t)std::max(longest, 1) + 2; }
 
+// Grow-only pinned bytes for a pass's merged bases: the batched path's
+// uploads from them go straight to the device, without staging.
+struct PinnedBytes {
+  uint8_t* p = nullptr;
+  size_t cap = 0, n = 0;
+  ~PinnedBytes() {
+    if (p) hipHostFree(p);
+  }
+  bool reserve(size_t want) {
+    if (want <= cap) return true;
+    uint8_t* q = nullptr;
+    const size_t c = std::max(want + want / 2, (size_t)1 << 20);
+    if (hipHostMalloc((void**)&q, c, hipHostMallocDefault) != hipSuccess) return false;
+    if (p) {
+      std::memcpy(q, p, n);
+      hipHostFree(p);
+    }
+    p = q;
+    cap = c;
+    return true;
+  }
+  bool append(const uint8_t* src, size_t len) {
+    if (!reserve(n + len + 1)) return false;
+    std::memcpy(p + n, src, len);
+    n += len;
+    return true;
+  }
+};
+
 void serve() {
   SwServer& S = server();
   // the merged batch and its results, kept across passes
-  std::vector<uint8_t> hb, rb;
+  PinnedBytes hb, rb;
   std::vector<int32_t> ho, ro, cnt, off, sc;
   std::vector<uint32_t> rh, cig;
   std::vector<SwRequest*> take;
@@ -105,16 +134,17 @@ void serve() {
     // appended; its offsets moved to where that span lands (each request's
     // first haplotype and read start where the previous request's last ones
     // end), its reads' haplotype indexes by the haplotypes before it
-    hb.clear();
-    rb.clear();
+    hb.n = 0;
+    rb.n = 0;
     ho.assign(1, 0);
     ro.assign(1, 0);
     rh.clear();
+    bool room = true;
     for (SwRequest* q : take) {
-      const int32_t hbase = (int32_t)hb.size(), rbase = (int32_t)rb.size(), h0 = (int32_t)ho.size() - 1;
+      const int32_t hbase = (int32_t)hb.n, rbase = (int32_t)rb.n, h0 = (int32_t)ho.size() - 1;
       const int32_t hfirst = q->hap_off[0], rfirst = q->read_off[0];
-      hb.insert(hb.end(), q->hap_bases + hfirst, q->hap_bases + q->hap_off[q->n_haps]);
-      rb.insert(rb.end(), q->read_bases + rfirst, q->read_bases + q->read_off[q->n_reads]);
+      room = room && hb.append(q->hap_bases + hfirst, (size_t)(q->hap_off[q->n_haps] - hfirst)) &&
+             rb.append(q->read_bases + rfirst, (size_t)(q->read_off[q->n_reads] - rfirst));
       for (uint32_t h = 1; h <= q->n_haps; ++h) ho.push_back(hbase + (q->hap_off[h] - hfirst));
       for (uint32_t r = 1; r <= q->n_reads; ++r) ro.push_back(rbase + (q->read_off[r] - rfirst));
       for (uint32_t r = 0; r < q->n_reads; ++r) rh.push_back((uint32_t)h0 + q->read_hap[r]);
@@ -125,10 +155,10 @@ void serve() {
     off.resize(cnt.size());
     sc.resize(cnt.size());
     cig.resize((size_t)std::max<uint32_t>(n_reads, 1) * slot);
-    hb.push_back(0);  // non-empty base arrays
-    rb.push_back(0);
-    const int rc = sw_align_batch(hb.data(), ho.data(), n_haps, rb.data(), ro.data(), n_reads, rh.data(), assemble,
-                                  slot, cig.data(), cnt.data(), off.data(), sc.data(), nullptr);
+    room = room && hb.reserve(1) && rb.reserve(1);  // non-empty base arrays
+    const int rc = room ? sw_align_batch(hb.p, ho.data(), n_haps, rb.p, ro.data(), n_reads, rh.data(), assemble, slot,
+                                         cig.data(), cnt.data(), off.data(), sc.data(), nullptr)
+                        : set_error(RVC_E_DEVICE, "pinned host memory for a merged pass");
     const std::string err = rc ? last_error() : std::string();
     // spread the results into the callers' slots
     uint32_t r0 = 0;
