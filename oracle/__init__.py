@@ -294,7 +294,7 @@ def ref_sw():
         L.ref_sw_align.argtypes = [ctypes.c_int, _u8p, ctypes.c_int, _u8p, ctypes.c_int, _u32p, ctypes.c_int, _i32p,
                                    _i32p]
         L.ref_sw_align.restype = ctypes.c_int
-        L.ref_sw_align_many.argtypes = [_u8p, _i32p, _u8p, _i32p, _u32p, ctypes.c_int, ctypes.c_int]
+        L.ref_sw_align_many.argtypes = [ctypes.c_int, _u8p, _i32p, _u8p, _i32p, _u32p, ctypes.c_int, ctypes.c_int]
         L.ref_sw_align_many.restype = ctypes.c_double
         _ref_sw = L
     return _ref_sw
@@ -313,9 +313,9 @@ def ref_sw_align(hap, read, assemble=False):
     return _sw_out(n, cig, off, sc)
 
 
-def ref_sw_align_many(haps, reads, read_hap, n_threads):
-    """Rovaca's aligner over all pairs on n_threads host threads (genotype
-    scores); returns the wall seconds."""
+def ref_sw_align_many(haps, reads, read_hap, n_threads, assemble=False):
+    """Rovaca's aligner over all pairs on n_threads host threads (genotyping
+    or, with assemble, assembly scores); returns the wall seconds."""
     L = ref_sw()
     if L is None:
         raise RuntimeError("oracle/_ref/libref_sw.so is not built")
@@ -326,5 +326,6 @@ def ref_sw_align_many(haps, reads, read_hap, n_threads):
     hb = np.frombuffer(b"".join(bytes(h) for h in haps) + b"\0", np.uint8)
     rb = np.frombuffer(b"".join(bytes(r) for r in reads) + b"\0", np.uint8)
     rh = np.ascontiguousarray(read_hap, np.uint32)
-    return float(L.ref_sw_align_many(hb.ctypes.data_as(_u8p), hap_off.ctypes.data_as(_i32p), rb.ctypes.data_as(_u8p),
-                                     read_off.ctypes.data_as(_i32p), rh.ctypes.data_as(_u32p), len(reads), n_threads))
+    return float(L.ref_sw_align_many(1 if assemble else 0, hb.ctypes.data_as(_u8p), hap_off.ctypes.data_as(_i32p),
+                                     rb.ctypes.data_as(_u8p), read_off.ctypes.data_as(_i32p),
+                                     rh.ctypes.data_as(_u32p), len(reads), n_threads))

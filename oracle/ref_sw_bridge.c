@@ -36,8 +36,8 @@ int ref_sw_align(int is_assemble, const unsigned char* seq1, int len1, const uns
 }
 
 // CPU baseline: pairs [0, n) on n_threads threads (pair p: haps + hap_off[hap[p]],
-// reads + read_off[p]), each thread with its own aligner instance. Returns the
-// wall seconds.
+// reads + read_off[p]), each thread with its own aligner instance of the
+// genotyping or (is_assemble) the assembly scores. Returns the wall seconds.
 #include <pthread.h>
 #include <time.h>
 
@@ -47,7 +47,7 @@ typedef struct {
   const unsigned char* reads;
   const int* read_off;
   const unsigned int* hap;
-  int first, step, n;
+  int first, step, n, is_assemble;
 } sw_job;
 
 static void* sw_worker(void* arg) {
@@ -56,14 +56,14 @@ static void* sw_worker(void* arg) {
   int off, sc;
   for (int p = j->first; p < j->n; p += j->step) {
     const unsigned int h = j->hap[p];
-    ref_sw_align(0, j->haps + j->hap_off[h], j->hap_off[h + 1] - j->hap_off[h], j->reads + j->read_off[p],
+    ref_sw_align(j->is_assemble, j->haps + j->hap_off[h], j->hap_off[h + 1] - j->hap_off[h], j->reads + j->read_off[p],
                  j->read_off[p + 1] - j->read_off[p], cig, 4096, &off, &sc);
   }
   return 0;
 }
 
-double ref_sw_align_many(const unsigned char* haps, const int* hap_off, const unsigned char* reads, const int* read_off,
-                         const unsigned int* hap, int n, int n_threads) {
+double ref_sw_align_many(int is_assemble, const unsigned char* haps, const int* hap_off, const unsigned char* reads,
+                         const int* read_off, const unsigned int* hap, int n, int n_threads) {
   pthread_t th[256];
   sw_job jobs[256];
   if (n_threads > 256) n_threads = 256;
@@ -72,6 +72,7 @@ double ref_sw_align_many(const unsigned char* haps, const int* hap_off, const un
   for (int t = 0; t < n_threads; ++t) {
     jobs[t].haps = haps; jobs[t].hap_off = hap_off; jobs[t].reads = reads; jobs[t].read_off = read_off;
     jobs[t].hap = hap; jobs[t].first = t; jobs[t].step = n_threads; jobs[t].n = n;
+    jobs[t].is_assemble = is_assemble;
     pthread_create(&th[t], 0, sw_worker, &jobs[t]);
   }
   for (int t = 0; t < n_threads; ++t) pthread_join(th[t], 0);
