@@ -22,7 +22,10 @@ Other modes: --workload stress (precision stress, 2048 regions), single (one
 8 x 32 region, latency), served (the serving loop alone), deep (served with
 1 % deep regions of 2000 reads x 128 haplotypes of 1000 bp), calls (the
 region-level call_pairhmm from worker threads, reads preprocessed on the
-device or on the calling threads, PCR model off and CONSERVATIVE).
+device or on the calling threads, PCR model off and CONSERVATIVE), sw (the
+realignment aligner on one batch of 1024 regions' reads, --assemble for the
+assembly scores) and sw_calls (the aligner called once per region from
+worker threads, served and batched).
 """
 import argparse
 import json
