@@ -18,7 +18,7 @@ region scheme on the host cores) on a bounded sample, and the next kernel on
 the same regions: every read realigned to its best haplotype by the device
 aligner, as one batch and as one served call per region (`realign`).
 
-Other modes: --workload stress (precision stress, 2048 regions), single (one
+Other modes: --workload stress (precision stress, 4096 regions), single (one
 8 x 32 region, latency), served (the serving loop alone), deep (served with
 1 % deep regions of 2000 reads x 128 haplotypes of 1000 bp), calls (the
 region-level call_pairhmm from worker threads, reads preprocessed on the
