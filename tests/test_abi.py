@@ -1,6 +1,7 @@
 """The C ABI library: loads, exports every declared symbol, and its host-only
 steps (tables, read preprocessing, post-processing) match the oracle. No
 device work here."""
+import ctypes
 import os
 import re
 
@@ -108,3 +109,36 @@ def test_calls_fail_cleanly_without_init():
     with pytest.raises(rv.EngineError) as e:
         rv.region_sync(reg)
     assert e.value.code == rv.RVC_E_STATE
+
+
+def test_struct_layouts_match_the_headers():
+    # the ctypes mirrors of the ABI's structs against the C compiler's view of
+    # the headers: same field names, offsets and sizes
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"rvc_region": rv.RvcRegion, "rvc_raw_region": rv.RvcRawRegion, "rvc_config": rv.RvcConfig,
+               "rvc_batch_opts": rv.RvcBatchOpts, "rvc_batch_info": rv.RvcBatchInfo,
+               "rvc_kernel_time": rv.RvcKernelTime, "rvc_server_stats": rv.RvcServerStats,
+               "rvc_loadgen_result": rv.RvcLoadgenResult, "rvc_sw_stats": rv.RvcSwStats,
+               "rvc_sw_region": rv.RvcSwRegion}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "rovaca_phmm_loadgen.h"', "int main(void) {"]
+    for name, py in structs.items():
+        lines.append(f'  printf("{name} %zu\\n", sizeof({name}));')
+        for field, _ in py._fields_:
+            lines.append(f'  printf("{name}.{field} %zu\\n", offsetof({name}, {field}));')
+    lines += ["  return 0;", "}"]
+    out_dir = os.path.join(ROOT, "rovaca_amd", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    src, exe = os.path.join(out_dir, "abi_layout.c"), os.path.join(out_dir, "abi_layout")
+    with open(src, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    subprocess.run([cc, "-std=c11", "-I", os.path.join(ROOT, "include"), "-o", exe, src], check=True)
+    c = dict(line.rsplit(" ", 1) for line in subprocess.run([exe], check=True, capture_output=True,
+                                                              text=True).stdout.split("\n") if line)
+    for name, py in structs.items():
+        assert int(c[name]) == ctypes.sizeof(py), name
+        for field, _ in py._fields_:
+            assert int(c[f"{name}.{field}"]) == getattr(py, field).offset, (name, field)
