@@ -64,13 +64,15 @@ struct PinnedArea {
   }
 };
 
-// Per-thread workspace: a stream, three events, a grow-only device buffer,
-// pinned host buffers for the pair table, the work lists and the results,
-// and the per-read host arrays, all kept across calls. Workspaces are owned
-// by a registry and released by rvc_phmm_shutdown, never by thread exit.
+// Per-thread workspace: the call's stream and its fork/join events, the
+// upload and launch streams, a grow-only device buffer for the bases and one
+// for the rest, pinned host buffers for the pair table, the work lists and
+// the results, and the per-read host arrays, all kept across calls.
+// Workspaces are owned by a registry and released by rvc_phmm_shutdown,
+// never by thread exit.
 struct SwWorkspace {
   hipStream_t stream = nullptr;
-  hipEvent_t ev[3] = {};
+  hipEvent_t ev[2] = {};  // a pass's fork and join
   // the launches of one pass (a sweep, then its pairs' best end and
   // traceback) forked over streams of their own, so that small passes run
   // their shapes at once
@@ -488,7 +490,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     }
     a.list = nullptr;
     a.n_list = 0;
-    if (hipEventRecord(w.ev[2], st) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");
+    if (hipEventRecord(w.ev[1], st) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");
     // counts first, then of every pair's CIGAR slot only the columns the
     // longest CIGAR of the chunk uses
     // (count, offset and score sit one after another on the device)
@@ -537,7 +539,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
         float x = 0;
         if (used[k] && hipEventElapsedTime(&x, w.ev[0], w.sub_fill[k]) == hipSuccess) f = std::max(f, x);
       }
-      hipEventElapsedTime(&all, w.ev[0], w.ev[2]);
+      hipEventElapsedTime(&all, w.ev[0], w.ev[1]);
       stats->fill_ms += f;
       stats->trace_ms += std::max(0.f, all - f);
       ++stats->chunks;
