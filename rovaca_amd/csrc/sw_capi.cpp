@@ -125,6 +125,13 @@ struct SwWorkspace {
         return set_error(RVC_E_DEVICE, "aligner streams");
     return RVC_OK;
   }
+  int quiesce() {
+    hipStream_t all[3 + kSubStreams] = {stream, up_stream, up_stream2};
+    for (int k = 0; k < kSubStreams; ++k) all[3 + k] = sub[k];
+    for (hipStream_t x : all)
+      if (x && hipStreamSynchronize(x) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner stream");
+    return RVC_OK;
+  }
   int ensure_in(size_t want) {
     if (want <= in_bytes) return RVC_OK;
     if (in_mem) hipFree(in_mem);
@@ -213,9 +220,9 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
   SwWorkspace* wp = sw_workspace();
   if (!wp) return set_error(RVC_E_INVALID, "out of host memory");
   SwWorkspace& w = *wp;
-  // a call that failed part way may have left copies from the pinned
-  // buffers in flight
-  if (w.stream && hipStreamSynchronize(w.stream) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner stream");
+  // a call that failed part way may have left copies or kernels in flight
+  // on any of the workspace's streams
+  if (int rc = w.quiesce()) return rc;
   if (int rc = w.streams()) return rc;
   // the caller's bases (pageable memory, staged by the runtime) go up from a
   // helper thread while this one builds the pair table and the work lists
@@ -245,8 +252,17 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
   const bool threaded = hap_bytes + read_bytes >= kThreadedUpload;
   std::thread uploader, uploader2;
   if (threaded) {
-    uploader = std::thread(upload_first);
-    uploader2 = std::thread(upload_second);
+    // (where a thread cannot be had, this thread uploads)
+    try {
+      uploader = std::thread(upload_first);
+    } catch (...) {
+      upload_first();
+    }
+    try {
+      uploader2 = std::thread(upload_second);
+    } catch (...) {
+      upload_second();
+    }
   } else {
     upload_first();
     upload_second();
