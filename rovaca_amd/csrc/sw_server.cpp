@@ -99,6 +99,10 @@ struct PinnedBytes {
   }
 };
 
+void pass(const std::vector<SwRequest*>& take, int32_t assemble, int32_t longest, PinnedBytes& hb, PinnedBytes& rb,
+          std::vector<int32_t>& ho, std::vector<int32_t>& ro, std::vector<int32_t>& cnt, std::vector<int32_t>& off,
+          std::vector<int32_t>& sc, std::vector<uint32_t>& rh, std::vector<uint32_t>& cig);
+
 void serve() {
   SwServer& S = server();
   // the merged batch and its results, kept across passes
@@ -130,64 +134,78 @@ void serve() {
       }
       if (!S.queue.empty()) S.work.notify_one();  // the other score set, for another server thread if any
     }
-    // merge: the span of each request's bases that its offsets cover,
-    // appended; its offsets moved to where that span lands (each request's
-    // first haplotype and read start where the previous request's last ones
-    // end), its reads' haplotype indexes by the haplotypes before it
-    hb.n = 0;
-    rb.n = 0;
-    ho.assign(1, 0);
-    ro.assign(1, 0);
-    rh.clear();
-    bool room = true;
-    for (SwRequest* q : take) {
-      const int32_t hbase = (int32_t)hb.n, rbase = (int32_t)rb.n, h0 = (int32_t)ho.size() - 1;
-      const int32_t hfirst = q->hap_off[0], rfirst = q->read_off[0];
-      room = room && hb.append(q->hap_bases + hfirst, (size_t)(q->hap_off[q->n_haps] - hfirst)) &&
-             rb.append(q->read_bases + rfirst, (size_t)(q->read_off[q->n_reads] - rfirst));
-      for (uint32_t h = 1; h <= q->n_haps; ++h) ho.push_back(hbase + (q->hap_off[h] - hfirst));
-      for (uint32_t r = 1; r <= q->n_reads; ++r) ro.push_back(rbase + (q->read_off[r] - rfirst));
-      for (uint32_t r = 0; r < q->n_reads; ++r) rh.push_back((uint32_t)h0 + q->read_hap[r]);
-    }
-    const uint32_t n_haps = (uint32_t)ho.size() - 1, n_reads = (uint32_t)rh.size();
-    const uint32_t slot = (uint32_t)slot_for(longest);
-    cnt.resize(std::max<size_t>(n_reads, 1));
-    off.resize(cnt.size());
-    sc.resize(cnt.size());
-    cig.resize((size_t)std::max<uint32_t>(n_reads, 1) * slot);
-    room = room && hb.reserve(1) && rb.reserve(1);  // non-empty base arrays
-    const int rc = room ? sw_align_batch(hb.p, ho.data(), n_haps, rb.p, ro.data(), n_reads, rh.data(), assemble, slot,
-                                         cig.data(), cnt.data(), off.data(), sc.data(), nullptr)
-                        : set_error(RVC_E_DEVICE, "pinned host memory for a merged pass");
-    const std::string err = rc ? last_error() : std::string();
-    // spread the results into the callers' slots
-    uint32_t r0 = 0;
-    for (SwRequest* q : take) {
-      q->rc = rc;
-      q->err = err;
-      if (rc == RVC_OK) {
-        std::memcpy(q->count, cnt.data() + r0, 4ull * q->n_reads);
-        std::memcpy(q->offset, off.data() + r0, 4ull * q->n_reads);
-        std::memcpy(q->score, sc.data() + r0, 4ull * q->n_reads);
-        for (uint32_t r = 0; r < q->n_reads; ++r) {
-          const int32_t k = cnt[r0 + r];
-          if (k <= 0) continue;
-          if ((uint32_t)k > q->max_cigar) {
-            q->rc = RVC_E_LIMIT;
-            q->err = "read " + std::to_string(r) + " needs more than " + std::to_string(q->max_cigar) +
-                     " CIGAR elements";
-            continue;
-          }
-          std::memcpy(q->cigar + (size_t)r * q->max_cigar, cig.data() + (size_t)(r0 + r) * slot, 4ull * (uint32_t)k);
-        }
+    try {
+      pass(take, assemble, longest, hb, rb, ho, ro, cnt, off, sc, rh, cig);
+    } catch (...) {  // host memory for the merged pass: its calls fail, the server goes on
+      for (SwRequest* q : take) {
+        q->rc = RVC_E_INVALID;
+        q->err = "out of host memory for a merged aligner pass";
       }
-      r0 += q->n_reads;
     }
     {
       std::lock_guard<std::mutex> lk(S.mu);
       for (SwRequest* q : take) q->done = true;
     }
     S.done.notify_all();
+  }
+}
+
+// One merged pass over the requests in 'take' (results into their slots).
+void pass(const std::vector<SwRequest*>& take, int32_t assemble, int32_t longest, PinnedBytes& hb, PinnedBytes& rb,
+          std::vector<int32_t>& ho, std::vector<int32_t>& ro, std::vector<int32_t>& cnt, std::vector<int32_t>& off,
+          std::vector<int32_t>& sc, std::vector<uint32_t>& rh, std::vector<uint32_t>& cig) {
+  // merge: the span of each request's bases that its offsets cover,
+  // appended; its offsets moved to where that span lands (each request's
+  // first haplotype and read start where the previous request's last ones
+  // end), its reads' haplotype indexes by the haplotypes before it
+  hb.n = 0;
+  rb.n = 0;
+  ho.assign(1, 0);
+  ro.assign(1, 0);
+  rh.clear();
+  bool room = true;
+  for (SwRequest* q : take) {
+    const int32_t hbase = (int32_t)hb.n, rbase = (int32_t)rb.n, h0 = (int32_t)ho.size() - 1;
+    const int32_t hfirst = q->hap_off[0], rfirst = q->read_off[0];
+    room = room && hb.append(q->hap_bases + hfirst, (size_t)(q->hap_off[q->n_haps] - hfirst)) &&
+           rb.append(q->read_bases + rfirst, (size_t)(q->read_off[q->n_reads] - rfirst));
+    for (uint32_t h = 1; h <= q->n_haps; ++h) ho.push_back(hbase + (q->hap_off[h] - hfirst));
+    for (uint32_t r = 1; r <= q->n_reads; ++r) ro.push_back(rbase + (q->read_off[r] - rfirst));
+    for (uint32_t r = 0; r < q->n_reads; ++r) rh.push_back((uint32_t)h0 + q->read_hap[r]);
+  }
+  const uint32_t n_haps = (uint32_t)ho.size() - 1, n_reads = (uint32_t)rh.size();
+  const uint32_t slot = (uint32_t)slot_for(longest);
+  cnt.resize(std::max<size_t>(n_reads, 1));
+  off.resize(cnt.size());
+  sc.resize(cnt.size());
+  cig.resize((size_t)std::max<uint32_t>(n_reads, 1) * slot);
+  room = room && hb.reserve(1) && rb.reserve(1);  // non-empty base arrays
+  const int rc = room ? sw_align_batch(hb.p, ho.data(), n_haps, rb.p, ro.data(), n_reads, rh.data(), assemble, slot,
+                                       cig.data(), cnt.data(), off.data(), sc.data(), nullptr)
+                      : set_error(RVC_E_DEVICE, "pinned host memory for a merged pass");
+  const std::string err = rc ? last_error() : std::string();
+  // spread the results into the callers' slots
+  uint32_t r0 = 0;
+  for (SwRequest* q : take) {
+    q->rc = rc;
+    q->err = err;
+    if (rc == RVC_OK) {
+      std::memcpy(q->count, cnt.data() + r0, 4ull * q->n_reads);
+      std::memcpy(q->offset, off.data() + r0, 4ull * q->n_reads);
+      std::memcpy(q->score, sc.data() + r0, 4ull * q->n_reads);
+      for (uint32_t r = 0; r < q->n_reads; ++r) {
+        const int32_t k = cnt[r0 + r];
+        if (k <= 0) continue;
+        if ((uint32_t)k > q->max_cigar) {
+          q->rc = RVC_E_LIMIT;
+          q->err = "read " + std::to_string(r) + " needs more than " + std::to_string(q->max_cigar) +
+                   " CIGAR elements";
+          continue;
+        }
+        std::memcpy(q->cigar + (size_t)r * q->max_cigar, cig.data() + (size_t)(r0 + r) * slot, 4ull * (uint32_t)k);
+      }
+    }
+    r0 += q->n_reads;
   }
 }
 
