@@ -274,8 +274,13 @@ void slot_main(Slot* sp) {
     const uint64_t t = now_ns();
     for (Request* r : s.reqs) r->t_launch = t;
     lk.unlock();
-    int rc = launch_slot(s);
-    if (rc == RVC_OK) rc = wait_slot(s);
+    int rc = RVC_OK;
+    try {  // host allocations of a launch: its requests fail, the loop goes on
+      rc = launch_slot(s);
+      if (rc == RVC_OK) rc = wait_slot(s);
+    } catch (...) {
+      rc = set_error(RVC_E_INVALID, "out of host memory for a serving launch");
+    }
     std::shared_ptr<BatchResult> res;
     float ms = 0;
     uint64_t resc = 0;
