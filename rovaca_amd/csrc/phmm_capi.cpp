@@ -126,220 +126,246 @@ int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags, cons
 extern "C" {
 
 int rvc_phmm_init(const rvc_config* cfg) {
-  int rc = engine_init(cfg);
-  if (rc) return rc;
-  if (cfg && cfg->server_threads) {
-    rc = server_start();
-    if (rc) {
-      engine_shutdown();
-      return rc;
+  return rvc::guarded([&]() -> int {
+    int rc = engine_init(cfg);
+    if (rc) return rc;
+    if (cfg && cfg->server_threads) {
+      rc = server_start();
+      if (rc) {
+        engine_shutdown();
+        return rc;
+      }
     }
-  }
-  return RVC_OK;
+    return RVC_OK;
+  });
 }
 
 int rvc_phmm_shutdown(void) {
-  server_stop();
-  release_workspaces();
-  sw_server_stop();
-  sw_release_workspaces();
-  return engine_shutdown();
+  return rvc::guarded([&]() -> int {
+    server_stop();
+    release_workspaces();
+    sw_server_stop();
+    sw_release_workspaces();
+    return engine_shutdown();
+  });
 }
 
 const char* rvc_phmm_last_error(void) { return last_error(); }
 
 int rvc_phmm_region_sync(const rvc_region* region, double* out_read_major, uint8_t* out_flags) {
-  return region_sync_impl(region, out_read_major, out_flags);
+  return rvc::guarded([&]() -> int {
+    return region_sync_impl(region, out_read_major, out_flags);
+  });
 }
 
 int rvc_phmm_call_pairhmm(const rvc_raw_region* rg, int32_t min_quality_threshold, int32_t pcr_option,
                           double* out_hap_major, uint32_t* kept_reads, uint32_t* n_kept) {
-  if (!rg || !out_hap_major || !kept_reads || !n_kept) return set_error(RVC_E_INVALID, "null argument");
-  const double nan = std::numeric_limits<double>::quiet_NaN();
-  const uint32_t H = rg->n_haps, N = rg->n_reads;
-  auto fail = [&](int code) {
-    for (size_t x = 0; x < (size_t)H * N; ++x) out_hap_major[x] = nan;
-    for (uint32_t r = 0; r < N; ++r) kept_reads[r] = r;
-    *n_kept = N;
-    return code;
-  };
-  if (H == 0 || N == 0 || !rg->hap_off || !rg->read_off || !rg->read_qual || !rg->read_mq)
-    return fail(set_error(RVC_E_INVALID, "empty or incomplete region"));
-  if (pcr_option < RVC_PCR_NONE || pcr_option > RVC_PCR_CONSERVATIVE)
-    return fail(set_error(RVC_E_INVALID, "unknown PCR indel model %d", pcr_option));
-  std::vector<int32_t> rlen(N);
-  for (uint32_t r = 0; r < N; ++r) {
-    const int32_t len = rg->read_off[r + 1] - rg->read_off[r];
-    if (len < 0) return fail(set_error(RVC_E_INVALID, "read %u has negative length", r));
-    rlen[r] = len;
-  }
-  // read preprocessing (MQ cap, quality floor, GOP 45 / PCR model, GCP 10):
-  // on the device, where the region carries the raw qualities, or here on
-  // the calling thread. By default the device takes it when the PCR model
-  // is on (its repeat search is the costly part); without it the host steps
-  // are a few operations per base.
-  const bool on_host = g_engine.host_prep == 1 || (g_engine.host_prep == 0 && pcr_option == RVC_PCR_NONE);
-  std::vector<uint8_t> q, gop, gcp;
-  if (on_host) {
-    const int32_t total = rg->read_off[N];
-    q.resize(total);
-    gop.resize(total);
-    gcp.resize(total);
+  return rvc::guarded([&]() -> int {
+    if (!rg || !out_hap_major || !kept_reads || !n_kept) return set_error(RVC_E_INVALID, "null argument");
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    const uint32_t H = rg->n_haps, N = rg->n_reads;
+    auto fail = [&](int code) {
+      for (size_t x = 0; x < (size_t)H * N; ++x) out_hap_major[x] = nan;
+      for (uint32_t r = 0; r < N; ++r) kept_reads[r] = r;
+      *n_kept = N;
+      return code;
+    };
+    if (H == 0 || N == 0 || !rg->hap_off || !rg->read_off || !rg->read_qual || !rg->read_mq)
+      return fail(set_error(RVC_E_INVALID, "empty or incomplete region"));
+    if (pcr_option < RVC_PCR_NONE || pcr_option > RVC_PCR_CONSERVATIVE)
+      return fail(set_error(RVC_E_INVALID, "unknown PCR indel model %d", pcr_option));
+    std::vector<int32_t> rlen(N);
     for (uint32_t r = 0; r < N; ++r) {
-      const int32_t o = rg->read_off[r];
-      preprocess_read(rg->read_bases + o, rg->read_qual + o, rlen[r], rg->read_mq[r], min_quality_threshold,
-                      pcr_option, q.data() + o, gop.data() + o, gcp.data() + o);
+      const int32_t len = rg->read_off[r + 1] - rg->read_off[r];
+      if (len < 0) return fail(set_error(RVC_E_INVALID, "read %u has negative length", r));
+      rlen[r] = len;
     }
-  }
-  rvc_region reg;
-  reg.n_haps = H;
-  reg.n_reads = N;
-  reg.hap_off = rg->hap_off;
-  reg.hap_bases = rg->hap_bases;
-  reg.read_off = rg->read_off;
-  reg.read_bases = rg->read_bases;
-  reg.q = rg->read_qual;
-  reg.i = rg->read_qual;  // not read
-  reg.d = rg->read_qual;
-  reg.c = rg->read_qual;
-  DevicePrep prep;
-  prep.mq = rg->read_mq;
-  prep.min_q = min_quality_threshold;
-  prep.pcr = pcr_option;
-  const DevicePrep* pp = &prep;
-  if (on_host) {
-    reg.q = q.data();
-    reg.i = gop.data();
-    reg.d = gop.data();
-    reg.c = gcp.data();
-    pp = nullptr;
-  }
-  // with the serving loop running, concurrent callers (Rovaca's worker
-  // threads) are batched together; otherwise the region runs on its own
-  std::vector<double> read_major((size_t)H * N);
-  int rc = server_running() ? served_region(&reg, read_major.data(), nullptr, pp)
-                            : region_sync_impl(&reg, read_major.data(), nullptr, pp);
-  if (rc) return fail(rc);
-  *n_kept = (uint32_t)postprocess(N, H, read_major.data(), rlen.data(), out_hap_major, kept_reads);
-  return RVC_OK;
+    // read preprocessing (MQ cap, quality floor, GOP 45 / PCR model, GCP 10):
+    // on the device, where the region carries the raw qualities, or here on
+    // the calling thread. By default the device takes it when the PCR model
+    // is on (its repeat search is the costly part); without it the host steps
+    // are a few operations per base.
+    const bool on_host = g_engine.host_prep == 1 || (g_engine.host_prep == 0 && pcr_option == RVC_PCR_NONE);
+    std::vector<uint8_t> q, gop, gcp;
+    if (on_host) {
+      const int32_t total = rg->read_off[N];
+      q.resize(total);
+      gop.resize(total);
+      gcp.resize(total);
+      for (uint32_t r = 0; r < N; ++r) {
+        const int32_t o = rg->read_off[r];
+        preprocess_read(rg->read_bases + o, rg->read_qual + o, rlen[r], rg->read_mq[r], min_quality_threshold,
+                        pcr_option, q.data() + o, gop.data() + o, gcp.data() + o);
+      }
+    }
+    rvc_region reg;
+    reg.n_haps = H;
+    reg.n_reads = N;
+    reg.hap_off = rg->hap_off;
+    reg.hap_bases = rg->hap_bases;
+    reg.read_off = rg->read_off;
+    reg.read_bases = rg->read_bases;
+    reg.q = rg->read_qual;
+    reg.i = rg->read_qual;  // not read
+    reg.d = rg->read_qual;
+    reg.c = rg->read_qual;
+    DevicePrep prep;
+    prep.mq = rg->read_mq;
+    prep.min_q = min_quality_threshold;
+    prep.pcr = pcr_option;
+    const DevicePrep* pp = &prep;
+    if (on_host) {
+      reg.q = q.data();
+      reg.i = gop.data();
+      reg.d = gop.data();
+      reg.c = gcp.data();
+      pp = nullptr;
+    }
+    // with the serving loop running, concurrent callers (Rovaca's worker
+    // threads) are batched together; otherwise the region runs on its own
+    std::vector<double> read_major((size_t)H * N);
+    int rc = server_running() ? served_region(&reg, read_major.data(), nullptr, pp)
+                              : region_sync_impl(&reg, read_major.data(), nullptr, pp);
+    if (rc) return fail(rc);
+    *n_kept = (uint32_t)postprocess(N, H, read_major.data(), rlen.data(), out_hap_major, kept_reads);
+    return RVC_OK;
+  });
 }
 
 int rvc_phmm_batch_create(const rvc_region* regions, uint32_t n_regions, const rvc_batch_opts* opts,
                           rvc_batch** out) {
-  if (!out) return set_error(RVC_E_INVALID, "null output handle");
-  *out = nullptr;
-  if (!g_engine.ready) return set_error(RVC_E_STATE, "rvc_phmm_init has not run");
-  if (!regions || n_regions == 0) return set_error(RVC_E_INVALID, "no regions");
-  for (uint32_t r = 0; r < n_regions; ++r) {
-    int rc = validate_region(&regions[r]);
-    if (rc) return rc;
-  }
-  auto* b = new (std::nothrow) rvc_batch;
-  if (!b) return set_error(RVC_E_INVALID, "out of host memory");
-  std::unique_ptr<rvc_batch> guard(b);
-  FP32Policy pol = policy();
-  if (opts) {
-    if (opts->fp32_fold == 1) pol.fold = true;
-    if (opts->fp32_fold == 2) pol.fold = false;
-    if (opts->rpl_round == 1) pol.rpl_steps_of_3 = false;
-    if (opts->rpl_round == 2) pol.rpl_steps_of_3 = true;
-    pol.waves_per_cu = opts->persistent_waves_per_cu;
-    if (opts->occupancy) pol.force_occ = (uint8_t)std::min(4u, std::max(2u, opts->occupancy));
-    if (opts->segment_width == 8 || opts->segment_width == 16 || opts->segment_width == 32 ||
-        opts->segment_width == 64)
-      pol.force_w = (uint8_t)opts->segment_width;
-    if (opts->prior_table == 1) pol.prior_table = true;
-    if (opts->prior_table == 2) pol.prior_table = false;
-    if (opts->hap_run) {
-      pol.hap_run = std::min<uint32_t>(kMaxHapRun, opts->hap_run);
-      pol.hap_run_fixed = true;
+  return rvc::guarded([&]() -> int {
+    if (!out) return set_error(RVC_E_INVALID, "null output handle");
+    *out = nullptr;
+    if (!g_engine.ready) return set_error(RVC_E_STATE, "rvc_phmm_init has not run");
+    if (!regions || n_regions == 0) return set_error(RVC_E_INVALID, "no regions");
+    for (uint32_t r = 0; r < n_regions; ++r) {
+      int rc = validate_region(&regions[r]);
+      if (rc) return rc;
     }
-  }
-  int rc = build_host_pack(regions, n_regions, pol, &b->pack);
-  if (rc) return rc;
-  split_runs(b->pack);
-  if (hipSetDevice(g_engine.device) != hipSuccess || hipStreamCreateWithFlags(&b->dev.stream, hipStreamNonBlocking) != hipSuccess)
-    return set_error(RVC_E_DEVICE, "batch stream");
-  b->dev.own_stream = true;
-  if ((rc = b->dev.upload(b->pack, b->dev.stream, nullptr, 0))) return rc;
-  if (hipStreamSynchronize(b->dev.stream) != hipSuccess) return set_error(RVC_E_DEVICE, "batch upload");
-  std::vector<char>().swap(b->pack.staging);
-  b->use_graph = opts && opts->use_graph;
-  *out = guard.release();
-  return RVC_OK;
+    auto* b = new (std::nothrow) rvc_batch;
+    if (!b) return set_error(RVC_E_INVALID, "out of host memory");
+    std::unique_ptr<rvc_batch> guard(b);
+    FP32Policy pol = policy();
+    if (opts) {
+      if (opts->fp32_fold == 1) pol.fold = true;
+      if (opts->fp32_fold == 2) pol.fold = false;
+      if (opts->rpl_round == 1) pol.rpl_steps_of_3 = false;
+      if (opts->rpl_round == 2) pol.rpl_steps_of_3 = true;
+      pol.waves_per_cu = opts->persistent_waves_per_cu;
+      if (opts->occupancy) pol.force_occ = (uint8_t)std::min(4u, std::max(2u, opts->occupancy));
+      if (opts->segment_width == 8 || opts->segment_width == 16 || opts->segment_width == 32 ||
+          opts->segment_width == 64)
+        pol.force_w = (uint8_t)opts->segment_width;
+      if (opts->prior_table == 1) pol.prior_table = true;
+      if (opts->prior_table == 2) pol.prior_table = false;
+      if (opts->hap_run) {
+        pol.hap_run = std::min<uint32_t>(kMaxHapRun, opts->hap_run);
+        pol.hap_run_fixed = true;
+      }
+    }
+    int rc = build_host_pack(regions, n_regions, pol, &b->pack);
+    if (rc) return rc;
+    split_runs(b->pack);
+    if (hipSetDevice(g_engine.device) != hipSuccess || hipStreamCreateWithFlags(&b->dev.stream, hipStreamNonBlocking) != hipSuccess)
+      return set_error(RVC_E_DEVICE, "batch stream");
+    b->dev.own_stream = true;
+    if ((rc = b->dev.upload(b->pack, b->dev.stream, nullptr, 0))) return rc;
+    if (hipStreamSynchronize(b->dev.stream) != hipSuccess) return set_error(RVC_E_DEVICE, "batch upload");
+    std::vector<char>().swap(b->pack.staging);
+    b->use_graph = opts && opts->use_graph;
+    *out = guard.release();
+    return RVC_OK;
+  });
 }
 
 int rvc_phmm_batch_run(rvc_batch* b) {
-  if (!b) return set_error(RVC_E_INVALID, "null batch");
-  int rc = b->dev.run(b->use_graph);
-  if (rc == RVC_OK) b->ran = true;
-  return rc;
+  return rvc::guarded([&]() -> int {
+    if (!b) return set_error(RVC_E_INVALID, "null batch");
+    int rc = b->dev.run(b->use_graph);
+    if (rc == RVC_OK) b->ran = true;
+    return rc;
+  });
 }
 
 int rvc_phmm_batch_results(rvc_batch* b, double* out, uint8_t* flags) {
-  if (!b || !out) return set_error(RVC_E_INVALID, "null argument");
-  if (!b->ran) return set_error(RVC_E_STATE, "batch has not run");
-  return b->dev.download(out, flags, b->dev.stream);
+  return rvc::guarded([&]() -> int {
+    if (!b || !out) return set_error(RVC_E_INVALID, "null argument");
+    if (!b->ran) return set_error(RVC_E_STATE, "batch has not run");
+    return b->dev.download(out, flags, b->dev.stream);
+  });
 }
 
 int rvc_phmm_batch_info(const rvc_batch* b, rvc_batch_info* info) {
-  if (!b || !info) return set_error(RVC_E_INVALID, "null argument");
-  std::memset(info, 0, sizeof *info);
-  info->n_pairs = b->pack.n_pairs;
-  info->n_cells = b->pack.n_cells;
-  info->n_rescued = b->dev.n_rescued;
-  info->device_bytes = b->dev.bytes;
-  info->fp32_ms = b->dev.fp32_ms;
-  info->fp64_ms = b->dev.fp64_ms;
-  info->total_ms = b->dev.total_ms;
-  info->n_fp32_launches = b->dev.n_fp32_launches;
-  info->n_fp64_launches = b->dev.n_fp64_launches;
-  return RVC_OK;
+  return rvc::guarded([&]() -> int {
+    if (!b || !info) return set_error(RVC_E_INVALID, "null argument");
+    std::memset(info, 0, sizeof *info);
+    info->n_pairs = b->pack.n_pairs;
+    info->n_cells = b->pack.n_cells;
+    info->n_rescued = b->dev.n_rescued;
+    info->device_bytes = b->dev.bytes;
+    info->fp32_ms = b->dev.fp32_ms;
+    info->fp64_ms = b->dev.fp64_ms;
+    info->total_ms = b->dev.total_ms;
+    info->n_fp32_launches = b->dev.n_fp32_launches;
+    info->n_fp64_launches = b->dev.n_fp64_launches;
+    return RVC_OK;
+  });
 }
 
 int rvc_phmm_batch_kernels(const rvc_batch* b, rvc_kernel_time* out, uint32_t max, uint32_t* n) {
-  if (!b || !n || (max && !out)) return set_error(RVC_E_INVALID, "null argument");
-  const auto& L = b->dev.launches;
-  *n = (uint32_t)L.size();
-  for (uint32_t x = 0; x < max && x < L.size(); ++x) out[x] = L[x];
-  return RVC_OK;
+  return rvc::guarded([&]() -> int {
+    if (!b || !n || (max && !out)) return set_error(RVC_E_INVALID, "null argument");
+    const auto& L = b->dev.launches;
+    *n = (uint32_t)L.size();
+    for (uint32_t x = 0; x < max && x < L.size(); ++x) out[x] = L[x];
+    return RVC_OK;
+  });
 }
 
 int rvc_phmm_batch_destroy(rvc_batch* b) {
-  if (!b) return set_error(RVC_E_INVALID, "null batch");
-  delete b;
-  return RVC_OK;
+  return rvc::guarded([&]() -> int {
+    if (!b) return set_error(RVC_E_INVALID, "null batch");
+    delete b;
+    return RVC_OK;
+  });
 }
 
 int rvc_phmm_preprocess_read(const uint8_t* bases, const uint8_t* qual, int32_t len, uint8_t mq,
                              int32_t min_quality_threshold, int32_t pcr_option, uint8_t* q, uint8_t* gop,
                              uint8_t* gcp) {
-  if (len < 0 || (len > 0 && (!bases || !qual || !q || !gop || !gcp)))
-    return set_error(RVC_E_INVALID, "bad read arguments");
-  if (pcr_option < RVC_PCR_NONE || pcr_option > RVC_PCR_CONSERVATIVE)
-    return set_error(RVC_E_INVALID, "unknown PCR indel model %d", pcr_option);
-  preprocess_read(bases, qual, len, mq, min_quality_threshold, pcr_option, q, gop, gcp);
-  return RVC_OK;
+  return rvc::guarded([&]() -> int {
+    if (len < 0 || (len > 0 && (!bases || !qual || !q || !gop || !gcp)))
+      return set_error(RVC_E_INVALID, "bad read arguments");
+    if (pcr_option < RVC_PCR_NONE || pcr_option > RVC_PCR_CONSERVATIVE)
+      return set_error(RVC_E_INVALID, "unknown PCR indel model %d", pcr_option);
+    preprocess_read(bases, qual, len, mq, min_quality_threshold, pcr_option, q, gop, gcp);
+    return RVC_OK;
+  });
 }
 
 int rvc_phmm_postprocess(uint32_t n_reads, uint32_t n_haps, double* read_major, const int32_t* read_len,
                          double* out_hap_major, uint32_t* kept_reads) {
-  if (n_reads == 0 || n_haps == 0 || !read_major || !read_len || !out_hap_major || !kept_reads)
-    return -set_error(RVC_E_INVALID, "bad post-processing arguments");
-  return postprocess(n_reads, n_haps, read_major, read_len, out_hap_major, kept_reads);
+  return rvc::guarded([&]() -> int {
+    if (n_reads == 0 || n_haps == 0 || !read_major || !read_len || !out_hap_major || !kept_reads)
+      return -set_error(RVC_E_INVALID, "bad post-processing arguments");
+    return postprocess(n_reads, n_haps, read_major, read_len, out_hap_major, kept_reads);
+  });
 }
 
 int rvc_phmm_tables(float* ph2pr_f32, float* mm_f32, double* ph2pr_f64, double* mm_f64) {
-  static HostTables* t = [] {
-    auto* p = new HostTables;
-    build_tables(p);
-    return p;
-  }();
-  if (ph2pr_f32) std::memcpy(ph2pr_f32, t->ph2pr_f, sizeof t->ph2pr_f);
-  if (mm_f32) std::memcpy(mm_f32, t->mm_f, sizeof t->mm_f);
-  if (ph2pr_f64) std::memcpy(ph2pr_f64, t->ph2pr_d, sizeof t->ph2pr_d);
-  if (mm_f64) std::memcpy(mm_f64, t->mm_d, sizeof t->mm_d);
-  return RVC_OK;
+  return rvc::guarded([&]() -> int {
+    static HostTables* t = [] {
+      auto* p = new HostTables;
+      build_tables(p);
+      return p;
+    }();
+    if (ph2pr_f32) std::memcpy(ph2pr_f32, t->ph2pr_f, sizeof t->ph2pr_f);
+    if (mm_f32) std::memcpy(mm_f32, t->mm_f, sizeof t->mm_f);
+    if (ph2pr_f64) std::memcpy(ph2pr_f64, t->ph2pr_d, sizeof t->ph2pr_d);
+    if (mm_f64) std::memcpy(mm_f64, t->mm_d, sizeof t->mm_d);
+    return RVC_OK;
+  });
 }
 
 }  // extern "C"

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -18,6 +19,19 @@ namespace rvc {
 
 int set_error(int code, const char* fmt, ...);
 const char* last_error();
+
+// Runs the body of a C-ABI entry point: a C++ exception (host memory) must
+// not cross into the caller's C or C++ code, so it becomes an error code.
+template <class F>
+int guarded(F&& body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return set_error(RVC_E_INVALID, "out of host memory");
+  } catch (...) {
+    return set_error(RVC_E_INVALID, "unexpected host error");
+  }
+}
 
 struct Engine {
   std::mutex mu;

@@ -433,69 +433,77 @@ using namespace rvc;
 
 extern "C" {
 
-int rvc_phmm_submit(const rvc_region* region, rvc_ticket* ticket) { return submit_impl(region, nullptr, ticket); }
+int rvc_phmm_submit(const rvc_region* region, rvc_ticket* ticket) {
+  return rvc::guarded([&]() -> int { return submit_impl(region, nullptr, ticket); });
+}
 
 int rvc_phmm_wait(rvc_ticket ticket, double* out_read_major, uint8_t* out_flags, int64_t timeout_ms,
                   rvc_ticket_timing* timing) {
-  if (!g_srv) return set_error(RVC_E_STATE, "serving loop is not running");
-  Server& S = *g_srv;
-  std::unique_lock<std::mutex> lk(S.mu);
-  auto it = S.tickets.find(ticket);
-  if (it == S.tickets.end()) return set_error(RVC_E_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
-  Request* r = it->second;
-  if (timeout_ms < 0) {
-    S.cv_done.wait(lk, [&] { return r->done; });
-  } else if (!S.cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return r->done; })) {
-    return set_error(RVC_E_TIMEOUT, "ticket %llu not done after %lld ms", (unsigned long long)ticket,
-                     (long long)timeout_ms);
-  }
-  S.tickets.erase(it);
-  lk.unlock();
-  const int st = r->status;
-  if (timing) {
-    timing->submit_ns = r->t_submit;
-    timing->launch_ns = r->t_launch;
-    timing->done_ns = r->t_done;
-  }
-  if (st == RVC_OK && (out_read_major || out_flags)) {
-    const BatchResult& B = *r->result;
-    const RegionLayout& L = B.region[r->result_region];
-    const double* bo = B.buf->out();
-    const uint8_t* bf = B.buf->flags();
-    for (uint32_t sx = 0; sx < L.n_reads; ++sx) {
-      const uint32_t x = B.read_perm[L.read0 + sx];
-      const size_t src = L.pair0 + (size_t)sx * L.n_haps, dst = (size_t)x * L.n_haps;
-      if (out_read_major) std::memcpy(out_read_major + dst, bo + src, L.n_haps * sizeof(double));
-      if (out_flags) std::memcpy(out_flags + dst, bf + src, L.n_haps);
+  return rvc::guarded([&]() -> int {
+    if (!g_srv) return set_error(RVC_E_STATE, "serving loop is not running");
+    Server& S = *g_srv;
+    std::unique_lock<std::mutex> lk(S.mu);
+    auto it = S.tickets.find(ticket);
+    if (it == S.tickets.end()) return set_error(RVC_E_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
+    Request* r = it->second;
+    if (timeout_ms < 0) {
+      S.cv_done.wait(lk, [&] { return r->done; });
+    } else if (!S.cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return r->done; })) {
+      return set_error(RVC_E_TIMEOUT, "ticket %llu not done after %lld ms", (unsigned long long)ticket,
+                       (long long)timeout_ms);
     }
-  }
-  delete r;
-  return st == RVC_OK ? RVC_OK : set_error(st, "request failed with code %d", st);
+    S.tickets.erase(it);
+    lk.unlock();
+    const int st = r->status;
+    if (timing) {
+      timing->submit_ns = r->t_submit;
+      timing->launch_ns = r->t_launch;
+      timing->done_ns = r->t_done;
+    }
+    if (st == RVC_OK && (out_read_major || out_flags)) {
+      const BatchResult& B = *r->result;
+      const RegionLayout& L = B.region[r->result_region];
+      const double* bo = B.buf->out();
+      const uint8_t* bf = B.buf->flags();
+      for (uint32_t sx = 0; sx < L.n_reads; ++sx) {
+        const uint32_t x = B.read_perm[L.read0 + sx];
+        const size_t src = L.pair0 + (size_t)sx * L.n_haps, dst = (size_t)x * L.n_haps;
+        if (out_read_major) std::memcpy(out_read_major + dst, bo + src, L.n_haps * sizeof(double));
+        if (out_flags) std::memcpy(out_flags + dst, bf + src, L.n_haps);
+      }
+    }
+    delete r;
+    return st == RVC_OK ? RVC_OK : set_error(st, "request failed with code %d", st);
+  });
 }
 
 int rvc_phmm_poll(rvc_ticket ticket) {
-  if (!g_srv) return -RVC_E_STATE;
-  std::lock_guard<std::mutex> lk(g_srv->mu);
-  auto it = g_srv->tickets.find(ticket);
-  if (it == g_srv->tickets.end()) return -RVC_E_INVALID;
-  return it->second->done ? 1 : 0;
+  return rvc::guarded([&]() -> int {
+    if (!g_srv) return -RVC_E_STATE;
+    std::lock_guard<std::mutex> lk(g_srv->mu);
+    auto it = g_srv->tickets.find(ticket);
+    if (it == g_srv->tickets.end()) return -RVC_E_INVALID;
+    return it->second->done ? 1 : 0;
+  });
 }
 
 int rvc_phmm_server_stats(rvc_server_stats* st) {
-  if (!st) return set_error(RVC_E_INVALID, "null stats");
-  if (!g_srv) return set_error(RVC_E_STATE, "serving loop is not running");
-  std::lock_guard<std::mutex> lk(g_srv->mu);
-  st->submitted = g_srv->submitted;
-  st->completed = g_srv->completed;
-  st->launches = g_srv->launches;
-  st->cells = g_srv->cells;
-  st->rescued_pairs = g_srv->rescued;
-  st->resident_bytes = g_srv->resident;
-  st->peak_resident_bytes = g_srv->peak_resident;
-  st->busy_ms = g_srv->busy_ms;
-  st->stage_ms = (double)g_srv->stage_ns.load() * 1e-6;
-  st->buffer_grows = g_srv->grows.load();
-  return RVC_OK;
+  return rvc::guarded([&]() -> int {
+    if (!st) return set_error(RVC_E_INVALID, "null stats");
+    if (!g_srv) return set_error(RVC_E_STATE, "serving loop is not running");
+    std::lock_guard<std::mutex> lk(g_srv->mu);
+    st->submitted = g_srv->submitted;
+    st->completed = g_srv->completed;
+    st->launches = g_srv->launches;
+    st->cells = g_srv->cells;
+    st->rescued_pairs = g_srv->rescued;
+    st->resident_bytes = g_srv->resident;
+    st->peak_resident_bytes = g_srv->peak_resident;
+    st->busy_ms = g_srv->busy_ms;
+    st->stage_ms = (double)g_srv->stage_ns.load() * 1e-6;
+    st->buffer_grows = g_srv->grows.load();
+    return RVC_OK;
+  });
 }
 
 }  // extern "C"

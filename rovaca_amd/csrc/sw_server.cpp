@@ -265,6 +265,8 @@ extern "C" int rvc_sw_align_served(const uint8_t* hap_bases, const int32_t* hap_
                                    const uint8_t* read_bases, const int32_t* read_off, uint32_t n_reads,
                                    const uint32_t* read_hap, int32_t assemble, uint32_t max_cigar, uint32_t* cigar,
                                    int32_t* count, int32_t* offset, int32_t* score) {
-  return rvc::sw_align_served(hap_bases, hap_off, n_haps, read_bases, read_off, n_reads, read_hap, assemble,
-                              max_cigar, cigar, count, offset, score);
+  return rvc::guarded([&]() -> int {
+    return rvc::sw_align_served(hap_bases, hap_off, n_haps, read_bases, read_off, n_reads, read_hap, assemble,
+                                max_cigar, cigar, count, offset, score);
+  });
 }
