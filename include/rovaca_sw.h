@@ -7,7 +7,7 @@
  * seq2 = read without soft clips, sw->avx_function(sw), then cigar,
  * cigar_count and alignment_offset), i.e. sw_avx_init(is_assemble) +
  * avx_function (hc/common/base/lib/sw_avx/sw_avx.c:402-451, the ro_sw core
- * common/base/lib/ro_sw/sw_core_template.h:113-356 and sw_cigar.c:20-171),
+ * common/base/lib/ro_sw/sw_core_template.h:110-361 and sw_cigar.c:20-171),
  * with the same results. Uses the engine initialised by rvc_phmm_init
  * (rovaca_phmm.h) and its error codes. */
 #ifndef ROVACA_SW_H_

@@ -3,10 +3,13 @@
 // sw_avx_init(0) / avx_function:
 //   scores          common/base/lib/sw_avx/sw_avx.c:14-15 (genotype 10/-15/-30/-5,
 //                   assembly 200/-150/-260/-11)
-//   core choice     common/base/lib/ro_sw/ro_sw.c:19-32 (16-bit scores unless the
-//                   read is shorter than 16 or min length x match > INT16_MAX)
-//   recurrence      common/base/lib/ro_sw/sw_core_template.h:113-316
-//   best end cell   sw_core_template.h:319-334 (last row, then last column,
+//   core choice     common/base/lib/ro_sw/ro_sw.c:26-40 (16-bit scores unless the
+//                   read is shorter than 16 or min length x match > INT16_MAX;
+//                   the AVX2 build switches below 8 bases, ro_sw.c:10-24, which
+//                   gives the same results: under 16 bases no H comes near the
+//                   16-bit floor of -8192)
+//   recurrence      common/base/lib/ro_sw/sw_core_template.h:113-330
+//   best end cell   sw_core_template.h:332-348 (last row, then last column,
 //                   one anti-diagonal at a time)
 //   traceback/CIGAR common/base/lib/ro_sw/sw_cigar.c:20-171
 // Rows i = 1..len1 are haplotype bases (seq1), columns j = 1..len2 read bases

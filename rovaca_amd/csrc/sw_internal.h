@@ -9,7 +9,7 @@
 
 namespace rvc {
 
-constexpr int kSwMaxLen = 2048;       // the reference aligner refuses lengths >= this (sw_avx.c:18, sw_core_template.h:341)
+constexpr int kSwMaxLen = 2048;       // the reference aligner refuses lengths >= this (sw_avx.c:18, sw_core_template.h:358-361)
 constexpr int kSwMaxRead = 1024;      // longest read the sweep shapes hold
 constexpr int kSwShapes = 12;         // (W, RPL): see sw_shape()
 constexpr int kSwLow = -(1 << 28);    // "no gap" at row and column 0

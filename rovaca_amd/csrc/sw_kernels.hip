@@ -1,6 +1,6 @@
 // Smith-Waterman of reads against haplotypes on gfx950: Rovaca's
 // read-to-best-haplotype aligner (the ro_sw library behind sw_avx_init /
-// avx_function, common/base/lib/ro_sw/sw_core_template.h:113-334 and
+// avx_function, common/base/lib/ro_sw/sw_core_template.h:110-361 and
 // sw_cigar.c:20-171), batched.
 //
 // Matrix: rows j = 1..n2 are read bases, columns i = 1..n1 haplotype bases
@@ -138,8 +138,8 @@ __global__ __launch_bounds__(64) void k_sw_fill(SwArgs a) {
       Hp[r] = 0;
       F[r] = kSwLow;
     }
-    // the reference's 16-bit core floors H at INT16_MIN / 4 (ro_sw.c:19-32,
-    // sw_internal.h:19-22); its 32-bit core at -100000000
+    // the reference's 16-bit core floors H at INT16_MIN / 4 (ro_sw.c:26-40,
+    // sw_internal.h:20-23); its 32-bit core at -100000000
     const bool use32 = n2 < 16 || a.match <= 0 || (long long)(n1 < n2 ? n1 : n2) * a.match > 32767;
     const int cut = use32 ? -100000000 : -8192;
     constexpr int G = sw_groups(RPL);
