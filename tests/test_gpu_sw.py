@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 import rovaca_amd as rv
-from test_oracle_sw import ACGT, _mutate, random_pairs, reference_vectors
+from test_oracle_sw import ACGT, _mutate, pairs_with_other_bytes, random_pairs, reference_vectors
 
 pytestmark = pytest.mark.gpu
 
@@ -313,3 +313,10 @@ def test_engine_restart_and_folded_shapes():
     assert rv.sw_align_batch(haps, reads, list(range(len(pairs))), served=True) == first
     for assemble in (False, True):
         check(haps, reads, list(range(len(pairs))), assemble)
+
+
+@pytest.mark.parametrize("assemble", [False, True])
+def test_reads_with_n_bases(assemble):
+    say(f"reads and haplotypes with N bases, assemble={assemble}")
+    pairs = pairs_with_other_bytes(41 + assemble, 400)
+    check([h for h, _ in pairs], [r for _, r in pairs], list(range(len(pairs))), assemble)

@@ -176,3 +176,26 @@ def test_matches_committed_reference_vectors(assemble):
     assert len(want) > 200
     for h, r, w in zip(haps, reads, want):
         assert oracle.sw_align(h, r, assemble) == w, (h, r)
+
+
+def pairs_with_other_bytes(seed, n):
+    """Pairs whose reads and haplotypes carry N (and a few other bytes) as
+    real reads do: the aligner compares raw bytes, so N matches N only."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for hap, read in random_pairs(seed, n):
+        h = np.frombuffer(hap, np.uint8).copy()
+        r = np.frombuffer(read, np.uint8).copy()
+        h[rng.random(len(h)) < 0.02] = ord("N")
+        r[rng.random(len(r)) < 0.05] = ord("N")
+        if len(r) > 3 and rng.random() < 0.3:
+            r[int(rng.integers(0, len(r)))] = ord("a")  # a lowercase base: no match to 'A'
+        out.append((h.tobytes(), r.tobytes()))
+    return out
+
+
+@pytest.mark.skipif(oracle.ref_sw() is None, reason="oracle/_ref/libref_sw.so is not built")
+@pytest.mark.parametrize("assemble", [False, True])
+def test_matches_reference_aligner_with_n_bases(assemble):
+    for hap, read in pairs_with_other_bytes(31 + assemble, 400):
+        assert oracle.sw_align(hap, read, assemble) == oracle.ref_sw_align(hap, read, assemble), (hap, read)
