@@ -13,6 +13,8 @@
  *   FP32 path: log10((double)S) - 36.1236000061f   (ph/roDV_inter/pairhmm_inter_api.h:10-12)
  *   FP64 path: log10(S) - 307.050595577260822       (ph/adapter.h:25-26)
  * An 'N' in either sequence matches every base (ph/roDV_intra/pairhmm_impl.cpp:13-44).
+ * Bases must be A, C, G, T or N: any other byte makes the call return
+ * RVC_E_BASES without computing anything.
  */
 #ifndef ROVACA_PHMM_H_
 #define ROVACA_PHMM_H_
@@ -35,6 +37,15 @@ extern "C" {
 #define RVC_E_BUDGET 5    /* one region needs more device memory than the whole budget  */
 #define RVC_E_TIMEOUT 6   /* rvc_phmm_wait gave up before the result was ready          */
 #define RVC_E_SHUTDOWN 7  /* the server stopped before this request finished            */
+#define RVC_E_BASES 8     /* a base other than A, C, G, T or N (an IUPAC code, '=', a    */
+                          /* lowercase letter). Rovaca's result for such a byte depends */
+                          /* on its kernel routing: its inter kernel compares raw bytes */
+                          /* (ph/roDV_inter/pairhmm_inter.cpp:167,287), its intra kernel */
+                          /* maps B..T to 'A' and reads past its table above 'T'        */
+                          /* (ph/roDV_common/common.h:42-56), and which kernel a pair   */
+                          /* takes follows its greedy grouping (ph/roDV_schedule.cpp:    */
+                          /* 86-142, 417-441). Nothing is computed; the caller keeps    */
+                          /* Rovaca's own CPU PairHMM for the region.                   */
 
 /* per-pair flags written beside each result */
 #define RVC_PAIR_COMPUTED 0x1u /* the pair had two non-empty sequences               */

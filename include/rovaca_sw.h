@@ -39,7 +39,11 @@ typedef struct rvc_sw_stats {
  * BAM encoding (length << 4 | op; M 0, I 1, D 2, S 4), offset[r] the
  * alignment offset on the haplotype (the library's alignment_offset),
  * score[r] the best score. Where the library refuses the pair (a length
- * >= 2048, or an empty sequence) count[r] = -1 and offset[r] = 1.
+ * >= 2048) count[r] = -1 and offset[r] = 1. An empty haplotype gives one
+ * element S<read length>, offset 0, score 0; an empty read against n1 bases
+ * gives no element, offset n1, score 0 (both empty: score INT32_MIN), as a
+ * fresh instance of the library does (sw_core_template.h:356-361 refuses
+ * only the lengths >= 2048).
  * max_cigar = 2 * longest read + 2 always suffices.
  * Returns RVC_OK, RVC_E_INVALID (null pointer, bad index), RVC_E_LIMIT (a
  * read of 1025 .. 2047 bases, beyond the device shapes; or more CIGAR

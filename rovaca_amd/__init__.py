@@ -22,6 +22,7 @@ RVC_E_STATE = 4
 RVC_E_BUDGET = 5
 RVC_E_TIMEOUT = 6
 RVC_E_SHUTDOWN = 7
+RVC_E_BASES = 8
 PAIR_COMPUTED = 1
 PAIR_FP64 = 2
 PCR_NONE, PCR_HOSTILE, PCR_AGGRESSIVE, PCR_CONSERVATIVE = 0, 1, 2, 3

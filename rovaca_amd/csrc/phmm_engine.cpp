@@ -124,6 +124,25 @@ int engine_shutdown() {
 // ---------------------------------------------------------------------------
 // validation and packing
 // ---------------------------------------------------------------------------
+namespace {
+// 1 for the bytes the device format holds (A C G T, and N as the wildcard).
+struct AcgtnTable {
+  uint8_t ok[256] = {};
+  AcgtnTable() { ok['A'] = ok['C'] = ok['G'] = ok['T'] = ok['N'] = 1; }
+};
+const AcgtnTable kAcgtn;
+
+// Index of the first byte of s[0..n) outside A, C, G, T, N, or -1.
+int64_t first_other_base(const uint8_t* s, int32_t n) {
+  uint8_t all = 1;
+  for (int32_t x = 0; x < n; ++x) all &= kAcgtn.ok[s[x]];
+  if (all) return -1;
+  for (int32_t x = 0; x < n; ++x)
+    if (!kAcgtn.ok[s[x]]) return x;
+  return -1;
+}
+}  // namespace
+
 int validate_region(const rvc_region* r) {
   if (!r) return set_error(RVC_E_INVALID, "null region");
   if (r->n_haps == 0 || r->n_reads == 0) return set_error(RVC_E_INVALID, "region without haplotypes or reads");
@@ -143,6 +162,22 @@ int validate_region(const rvc_region* r) {
     if (l > kMaxReadLen) return set_error(RVC_E_LIMIT, "read %u has %d bases (limit %d)", x, l, kMaxReadLen);
     if (l > 0 && !r->read_bases) return set_error(RVC_E_INVALID, "null read_bases");
   }
+  // Bytes other than A, C, G, T, N have no single meaning in the reference
+  // (raw-byte compare in its inter kernel, ph/roDV_inter/pairhmm_inter.cpp:167;
+  // 'A' or out of table in its intra kernel, ph/roDV_common/common.h:42-56;
+  // the kernel follows its grouping): refused, the caller keeps its CPU path.
+  for (uint32_t h = 0; h < r->n_haps; ++h) {
+    const int64_t x = first_other_base(r->hap_bases + r->hap_off[h], r->hap_off[h + 1] - r->hap_off[h]);
+    if (x >= 0)
+      return set_error(RVC_E_BASES, "haplotype %u base %lld is byte %u, not A, C, G, T or N", h, (long long)x,
+                       (unsigned)r->hap_bases[r->hap_off[h] + x]);
+  }
+  for (uint32_t y = 0; y < r->n_reads; ++y) {
+    const int64_t x = first_other_base(r->read_bases + r->read_off[y], r->read_off[y + 1] - r->read_off[y]);
+    if (x >= 0)
+      return set_error(RVC_E_BASES, "read %u base %lld is byte %u, not A, C, G, T or N", y, (long long)x,
+                       (unsigned)r->read_bases[r->read_off[y] + x]);
+  }
   return RVC_OK;
 }
 
@@ -153,7 +188,7 @@ inline uint32_t base_code(uint8_t ch, bool* is_n) {
     case 'C': *is_n = false; return 1;
     case 'G': *is_n = false; return 2;
     case 'T': *is_n = false; return 3;
-    default: *is_n = true; return 0;  // 'N' and any other byte match every base
+    default: *is_n = true; return 0;  // 'N' (validate_region refuses every other byte)
   }
 }
 inline uint32_t align_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <type_traits>
 #include <cstdint>
 #include <utility>
@@ -932,33 +933,35 @@ hipError_t launch_init_results(double* out, uint64_t n, hipStream_t stream) {
 namespace rvc {
 
 // Resident 64-lane blocks per CU for a kernel (register and LDS limited).
+// The occupancy queries are cached per kernel. Serving threads and callers of
+// the one-region path ask concurrently: every slot is an atomic, and two
+// threads that both find it empty store the same value.
 int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ, bool pt) {
-  static int cache[2][2][3][4][17] = {};
+  static std::atomic<int> cache[2][2][3][4][17] = {};
   const int w = W == 16 ? 0 : W == 32 ? 1 : W == 64 ? 2 : 3;
   const int o = std::min(4, std::max(2, occ)) - 2;
   if (RPL < 1 || RPL > 16) return 1;
-  int& slot = cache[pt][xf][o][w][RPL];
-  if (slot == 0) {
-    int n = 0;
+  std::atomic<int>& slot = cache[pt][xf][o][w][RPL];
+  int n = slot.load(std::memory_order_relaxed);
+  if (n == 0) {
     Fp32Fn f = fp32_fn(W, RPL, xf, occ, pt);
     if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64, 0) != hipSuccess || n < 1) n = 1;
-    slot = n;
+    slot.store(n, std::memory_order_relaxed);
   }
-  return slot;
+  return n;
 }
 
 int fp64_blocks_per_cu(int bin, bool pt, bool scaled) {
-  static int cache[2][2][kNumFp64Bins] = {};
+  static std::atomic<int> cache[2][2][kNumFp64Bins] = {};
   if (bin < 0 || bin >= kNumFp64Bins) return 1;
-  int& slot = cache[pt][scaled][bin];
-  if (slot == 0) {
-    int n = 0;
+  std::atomic<int>& slot = cache[pt][scaled][bin];
+  int n = slot.load(std::memory_order_relaxed);
+  if (n == 0) {
     Fp64Fn f = pt ? kFp64Pt[bin] : kFp64[scaled][bin];
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64, 0) != hipSuccess || n < 1)
-      n = 1;
-    slot = n;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64, 0) != hipSuccess || n < 1) n = 1;
+    slot.store(n, std::memory_order_relaxed);
   }
-  return slot;
+  return n;
 }
 
 }  // namespace rvc

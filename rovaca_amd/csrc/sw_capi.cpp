@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -518,13 +519,31 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
       std::memcpy(count + r0, res, 4ull * n);
       std::memcpy(offset + r0, res + n, 4ull * n);
       std::memcpy(score + r0, res + 2ull * n, 4ull * n);
-      // the pairs the library refuses (no sweep): count -1, offset 1
-      for (uint32_t x = 0; x < n; ++x)
-        if (shape[r0 + x] < 0) {
+      // pairs without a sweep: the library refuses a length >= 2048 (count
+      // -1, offset 1, sw_core_template.h:356-361); an empty haplotype leaves
+      // every read base soft-clipped at offset 0 with score 0; an empty read
+      // against n1 bases ends at offset n1 with no CIGAR element and score 0
+      // (both empty: score INT32_MIN, the best-end search finds no cell). The
+      // library's own instance keeps the previous call's CIGAR count for an
+      // empty read (its count is not reset); these are a fresh instance's.
+      for (uint32_t x = 0; x < n; ++x) {
+        if (shape[r0 + x] >= 0) continue;
+        const SwPair& p = pairs[r0 + x];
+        if (p.hap_len >= (uint32_t)kSwMaxLen || p.read_len >= (uint32_t)kSwMaxLen) {
           count[r0 + x] = -1;
           offset[r0 + x] = 1;
           score[r0 + x] = 0;
+        } else if (p.hap_len == 0 && p.read_len > 0) {
+          count[r0 + x] = 1;  // max_cigar >= 1 (validated)
+          cigar[(size_t)(r0 + x) * max_cigar] = (p.read_len << 4) | 4u;  // S
+          offset[r0 + x] = 0;
+          score[r0 + x] = 0;
+        } else {
+          count[r0 + x] = 0;
+          offset[r0 + x] = (int32_t)p.hap_len;
+          score[r0 + x] = p.hap_len == 0 ? INT32_MIN : 0;
         }
+      }
     }
     int32_t widest = 0;
     for (uint32_t x = 0; x < n && e == hipSuccess; ++x) widest = std::max(widest, count[r0 + x]);
@@ -544,7 +563,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
         e = hipMemcpyAsync(host, base + (packed ? o_pack : o_cig), 4ull * n * pitch, hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       for (uint32_t x = 0; x < n && e == hipSuccess; ++x)
-        if (count[r0 + x] > 0)
+        if (count[r0 + x] > 0 && shape[r0 + x] >= 0)
           std::memcpy(cigar + (size_t)(r0 + x) * max_cigar, host + (size_t)x * pitch, 4ull * (uint32_t)count[r0 + x]);
     }
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "aligner results: %s", hipGetErrorString(e));

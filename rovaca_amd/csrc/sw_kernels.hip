@@ -25,6 +25,8 @@
 // recurrence two pairs per lane.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cstdint>
 #include <type_traits>
 
@@ -504,14 +506,16 @@ using SwFn = void (*)(SwArgs);
 const SwFn kSwFill[kSwShapes] = {&k_sw_fill<kSwW[0], kSwR[0]>, &k_sw_fill<kSwW[1], kSwR[1]>, &k_sw_fill<kSwW[2], kSwR[2]>, &k_sw_fill<kSwW[3], kSwR[3]>, &k_sw_fill<kSwW[4], kSwR[4]>, &k_sw_fill<kSwW[5], kSwR[5]>, &k_sw_fill<kSwW[6], kSwR[6]>, &k_sw_fill<kSwW[7], kSwR[7]>, &k_sw_fill<kSwW[8], kSwR[8]>, &k_sw_fill<kSwW[9], kSwR[9]>, &k_sw_fill<kSwW[10], kSwR[10]>, &k_sw_fill<kSwW[11], kSwR[11]>};
 const SwFn kSwFill16[kSwShapes] = {&k_sw_fill16<kSwW[0], kSwR[0]>, &k_sw_fill16<kSwW[1], kSwR[1]>, &k_sw_fill16<kSwW[2], kSwR[2]>, &k_sw_fill16<kSwW[3], kSwR[3]>, &k_sw_fill16<kSwW[4], kSwR[4]>, &k_sw_fill16<kSwW[5], kSwR[5]>, &k_sw_fill16<kSwW[6], kSwR[6]>, &k_sw_fill16<kSwW[7], kSwR[7]>, &k_sw_fill16<kSwW[8], kSwR[8]>, &k_sw_fill16<kSwW[9], kSwR[9]>, &k_sw_fill16<kSwW[10], kSwR[10]>, &k_sw_fill16<kSwW[11], kSwR[11]>};
 
-int blocks_per_cu(const SwFn* table, int* cache, int shape) {
+// Occupancy per shape, cached; asked from several threads at once (each
+// slot is atomic, and racing threads store the same value).
+int blocks_per_cu(const SwFn* table, std::atomic<int>* cache, int shape) {
   if (shape < 0 || shape >= kSwShapes) return 1;
-  if (cache[shape] == 0) {
-    int n = 0;
+  int n = cache[shape].load(std::memory_order_relaxed);
+  if (n == 0) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, table[shape], 64, 0) != hipSuccess || n < 1) n = 1;
-    cache[shape] = n;
+    cache[shape].store(n, std::memory_order_relaxed);
   }
-  return cache[shape];
+  return n;
 }
 }  // namespace
 
@@ -535,12 +539,12 @@ hipError_t launch_sw_fill16(int shape, int grid, hipStream_t stream, const SwArg
 }
 
 int sw_blocks_per_cu(int shape) {
-  static int cache[kSwShapes] = {};
+  static std::atomic<int> cache[kSwShapes] = {};
   return blocks_per_cu(kSwFill, cache, shape);
 }
 
 int sw16_blocks_per_cu(int shape) {
-  static int cache[kSwShapes] = {};
+  static std::atomic<int> cache[kSwShapes] = {};
   return blocks_per_cu(kSwFill16, cache, shape);
 }
 

@@ -100,6 +100,52 @@ def test_postprocess_closed_form():
     assert np.isnan(out[:, 1]).all()
 
 
+def test_postprocess_drops_a_middle_read_and_packs_the_rest():
+    # The reference erases a dropped read's row from the read-major matrix
+    # (collect_poorly_modelled_indices, ph/pairhmm_internal.cpp:489-497) before
+    # transpose_likelihood_matrix (:499-508) copies the remaining rows into the
+    # first columns of the NaN-initialised [H][N] result (:581, :613): kept
+    # reads are packed to the front, the last columns stay NaN.
+    rm = np.array([[-1.0, -2.0], [-20.0, -30.0], [-3.0, -1.5]])
+    for impl in (rv.postprocess, oracle.postprocess):
+        out, kept = impl(rm, np.array([100, 100, 100], np.int32))
+        assert list(kept) == [0, 2]
+        assert out.shape == (2, 3)
+        assert list(out[:, 0]) == [-1.0, -2.0]      # read 0 (nothing below best - 4.5 = -5.5)
+        assert list(out[:, 1]) == [-3.0, -1.5]      # read 2, moved into column 1
+        assert np.isnan(out[:, 2]).all()            # the dropped read's column is the last one
+
+
+def test_postprocess_threshold_for_short_reads():
+    # min(2, ceil(0.02 R)) * -4.0 (ph/pairhmm_internal.cpp:38-41, ph/adapter.h:20-22):
+    # -4.0 up to R = 50, -8.0 from R = 51; an empty read (all NaN) has best -inf
+    # below its threshold 0 and is dropped
+    cases = [(50, -4.01, False), (50, -3.99, True), (1, -4.01, False), (51, -7.99, True), (51, -8.01, False),
+             (300, -7.99, True)]
+    for R, best, keep in cases:
+        rm = np.array([[best, best - 1.0]])
+        for impl in (rv.postprocess, oracle.postprocess):
+            out, kept = impl(rm, np.array([R], np.int32))
+            assert (list(kept) == [0]) == keep, (R, best)
+    rm = np.array([[np.nan, np.nan], [-1.0, -2.0]])
+    for impl in (rv.postprocess, oracle.postprocess):
+        out, kept = impl(rm, np.array([0, 10], np.int32))
+        assert list(kept) == [1]
+
+
+def test_postprocess_nan_first_row():
+    # normalize takes the best with std::max_element (ph/pairhmm_internal.cpp:248-258):
+    # a NaN in front compares false against everything and stays the "best",
+    # so nothing in that row is floored; a NaN further on is skipped. The
+    # drop test ignores NaNs either way (:33-37).
+    rm = np.array([[np.nan, -10.0, -1.0], [-10.0, np.nan, -1.0]])
+    for impl in (rv.postprocess, oracle.postprocess):
+        out, kept = impl(rm, np.array([100, 100], np.int32))
+        assert list(kept) == [0, 1]
+        assert np.isnan(out[0, 0]) and out[1, 0] == -10.0 and out[2, 0] == -1.0
+        assert out[0, 1] == -5.5 and np.isnan(out[1, 1]) and out[2, 1] == -1.0
+
+
 def test_calls_fail_cleanly_without_init():
     reg = {"n_haps": 1, "n_reads": 1, "hap_off": np.array([0, 4], np.int32),
            "hap_bases": np.frombuffer(b"ACGT", np.uint8).copy(), "read_off": np.array([0, 4], np.int32),

@@ -256,6 +256,61 @@ def test_limits():
         assert e.value.code == code
 
 
+@pytest.mark.parametrize("byte", [b"R", b"Y", b"=", b"a", b"n", b"."])
+def test_other_bytes_are_refused(byte):
+    # The reference's result for a byte other than A, C, G, T, N depends on
+    # its routing (raw-byte compare in the inter kernel, 'A' or an index past
+    # its table in the intra kernel): every entry point refuses the region
+    # with RVC_E_BASES and computes nothing; the caller keeps its CPU path.
+    say(f"byte {byte!r} in a read and in a haplotype: RVC_E_BASES")
+    base = workload.single_region(3)
+    for where in ("read_bases", "hap_bases"):
+        reg = dict(base)
+        arr = reg[where].copy()
+        arr[7] = byte[0]
+        reg[where] = arr
+        with pytest.raises(rv.EngineError) as e:
+            rv.region_sync(reg)
+        assert e.value.code == rv.RVC_E_BASES
+        with pytest.raises(rv.EngineError) as e:
+            rv.submit(reg)
+        assert e.value.code == rv.RVC_E_BASES
+        with pytest.raises(rv.EngineError) as e:
+            rv.Batch([base, reg])
+        assert e.value.code == rv.RVC_E_BASES
+        H, N = reg["n_haps"], reg["n_reads"]
+        haps = [reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]] for h in range(H)]
+        reads = [reg["read_bases"][reg["read_off"][r]:reg["read_off"][r + 1]] for r in range(N)]
+        quals = [reg["q"][reg["read_off"][r]:reg["read_off"][r + 1]] for r in range(N)]
+        rc, mat, kept = rv.call_pairhmm(haps, reads, quals, np.full(N, 60, np.uint8))
+        assert rc == rv.RVC_E_BASES and np.isnan(mat).all() and list(kept) == list(range(N))
+    # the same region with only A, C, G, T, N still runs
+    check_region(base)
+
+
+def test_call_pairhmm_drops_a_middle_read():
+    say("call_pairhmm: a poorly modelled read in the middle is dropped, the others packed to the front")
+    reg = workload.single_region(5)
+    H, N = reg["n_haps"], reg["n_reads"]
+    haps = [reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]] for h in range(H)]
+    reads = [reg["read_bases"][reg["read_off"][r]:reg["read_off"][r + 1]].copy() for r in range(N)]
+    quals = [reg["q"][reg["read_off"][r]:reg["read_off"][r + 1]].copy() for r in range(N)]
+    rng = np.random.default_rng(5)
+    bad = N // 2
+    reads[bad] = workload.ACGT[rng.integers(0, 4, len(reads[bad]))]  # unrelated to every haplotype
+    quals[bad][:] = 40
+    rc, mat, kept = rv.call_pairhmm(haps, reads, quals, np.full(N, 60, np.uint8))
+    assert rc == 0
+    assert bad not in set(kept.tolist()) and len(kept) < N
+    assert np.isnan(mat[:, len(kept):]).all() and not np.isnan(mat[:, :len(kept)]).any()
+    # the kept columns are the reads' normalised rows, in order
+    oreg = workload.make_region(haps, reads, quals)
+    rm, _, ok = oracle.region_reference(oreg)
+    want, wkept = oracle.postprocess(rm, np.array([len(r) for r in reads], np.int32))
+    assert np.array_equal(kept, wkept)
+    assert np.abs(mat[:, :len(kept)] - want[:, :len(kept)]).max() < TOL
+
+
 def test_many_haplotypes():
     say("4096 haplotypes x 2 reads")
     rng = np.random.default_rng(4)

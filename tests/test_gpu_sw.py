@@ -86,6 +86,18 @@ def test_haplotype_lengths_and_refusals():
     assert got[4] == (None, 1, None)
 
 
+def test_empty_haplotypes_and_reads():
+    say("empty haplotypes and reads: not refused, as the reference library")
+    haps = [b"", b"ACGTACGTAC", b"A" * 2047]
+    reads = [b"ACGT", b"", b"", b"ACGTACGTACGTACGTACGTAA", b"ACGTAC", b"", b"GT" * 40]
+    read_hap = [0, 1, 0, 0, 1, 2, 0]
+    for served in (False, True):
+        got = rv.sw_align_batch(haps, reads, read_hap, served=served)
+        for r, g in enumerate(got):
+            assert g == oracle.sw_align(haps[read_hap[r]], reads[r]), (r, g)
+    assert got[0] == ([("S", 4)], 0, 0) and got[1] == ([], 10, 0)
+
+
 def test_shared_haplotypes_and_batch_independence():
     say("many reads per haplotype; results do not depend on the batch")
     rng = np.random.default_rng(5)

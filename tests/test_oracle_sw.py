@@ -2,11 +2,15 @@
 Rovaca's own aligner compiled from its sources (oracle/_ref), closed forms and
 brute force on tiny inputs."""
 import itertools
+import os
+import sys
 
 import numpy as np
 import pytest
 
 import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 ACGT = np.frombuffer(b"ACGT", np.uint8)
 
@@ -67,6 +71,26 @@ def test_matches_reference_aligner_edges():
     for hap, read in cases:
         for assemble in (False, True):
             assert oracle.sw_align(hap, read, assemble) == oracle.ref_sw_align(hap, read, assemble), (hap, read)
+
+
+@pytest.mark.skipif(oracle.ref_sw() is None, reason="oracle/_ref/libref_sw.so is not built")
+def test_empty_sequences_match_a_fresh_reference_aligner():
+    # An empty haplotype or read is not refused by Rovaca's aligner (only a
+    # length >= 2048 is, sw_core_template.h:356-361). Its instance keeps the
+    # previous call's CIGAR count when the read is empty, so each case runs in
+    # a process of its own, on a fresh instance.
+    import subprocess
+    cases = [(b"", b"ACGT"), (b"", b"ACGTACGTACGTACGTACGTAA"), (b"", b"A" * 1500), (b"ACGT", b""),
+             (b"A" * 2047, b""), (b"", b""), (b"A" * 2048, b"")]
+    for hap, read in cases:
+        for assemble in (False, True):
+            code = (f"import oracle; print(repr(oracle.ref_sw_align({hap!r}, {read!r}, {assemble})))")
+            out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, check=True, capture_output=True,
+                                 text=True).stdout.strip()
+            assert repr(oracle.sw_align(hap, read, assemble)) == out, (len(hap), len(read), assemble)
+    # closed forms: every read base soft-clipped against an empty haplotype
+    assert oracle.sw_align(b"", b"ACGTA") == ([("S", 5)], 0, 0)
+    assert oracle.sw_align(b"ACG", b"") == ([], 3, 0)
 
 
 def test_closed_forms():
