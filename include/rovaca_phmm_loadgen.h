@@ -35,9 +35,13 @@ typedef struct rvc_loadgen_result {
  * in order as they are submitted.
  * The serving loop must be running (rvc_config.server_threads = 1).
  * latency_ms (optional, n_arrivals entries) receives each request's
- * submit -> results-ready time. */
+ * submit -> results-ready time. out_read_major (optional, n_arrivals
+ * entries, each NULL or room for that arrival's region's n_reads * n_haps
+ * doubles) receives each arrival's read-major matrix as rvc_phmm_wait
+ * returns it. */
 int rvc_loadgen_run(const rvc_region* regions, uint32_t n_regions, const uint32_t* order, const double* arrival_s,
-                    uint32_t n_arrivals, uint32_t n_producers, double* latency_ms, rvc_loadgen_result* result);
+                    uint32_t n_arrivals, uint32_t n_producers, double* latency_ms, double* const* out_read_major,
+                    rvc_loadgen_result* result);
 
 /* Rovaca's worker pattern: each of n_workers threads calls
  * rvc_phmm_call_pairhmm (raw reads in, hap-major matrix and kept reads out)

@@ -133,7 +133,8 @@ def lib():
                                                ctypes.c_int32, _u8p, _u8p, _u8p]
         L.rvc_phmm_postprocess.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _f64p, _i32p, _f64p, _u32p]
         L.rvc_loadgen_run.argtypes = [ctypes.POINTER(RvcRegion), ctypes.c_uint32, _u32p, _f64p, ctypes.c_uint32,
-                                      ctypes.c_uint32, _f64p, ctypes.POINTER(RvcLoadgenResult)]
+                                      ctypes.c_uint32, _f64p, ctypes.POINTER(_f64p),
+                                      ctypes.POINTER(RvcLoadgenResult)]
         L.rvc_loadgen_call_pairhmm.argtypes = [ctypes.POINTER(RvcRawRegion), ctypes.c_uint32, _u32p, ctypes.c_uint32,
                                                ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, _f64p,
                                                ctypes.POINTER(RvcLoadgenResult)]
@@ -343,9 +344,10 @@ def tables():
     return pf, mf, pd, md
 
 
-def loadgen_run(regions, arrival_s, order=None, n_producers=32):
+def loadgen_run(regions, arrival_s, order=None, n_producers=32, outputs=False):
     """Submits regions at the given arrival times from n_producers threads (C++)
-    and waits for all of them. Returns (summary dict, per-request latency ms)."""
+    and waits for all of them. Returns (summary dict, per-request latency ms),
+    and with outputs=True also each arrival's read-major matrix."""
     arr = (RvcRegion * len(regions))()
     for x, r in enumerate(regions):
         arr[x] = make_c_region(r)
@@ -354,10 +356,16 @@ def loadgen_run(regions, arrival_s, order=None, n_producers=32):
     o = None if order is None else np.ascontiguousarray(order, np.uint32)
     lat = np.zeros(n, np.float64)
     res = RvcLoadgenResult()
+    outs, optr = None, None
+    if outputs:
+        idx = o if o is not None else np.arange(n) % len(regions)
+        outs = [np.empty((int(regions[i]["n_reads"]), int(regions[i]["n_haps"])), np.float64) for i in idx]
+        optr = (_f64p * n)(*[_ptr(x, _f64p) for x in outs])
     rc = lib().rvc_loadgen_run(arr, len(regions), _ptr(o, _u32p) if o is not None else None, _ptr(arrival, _f64p),
-                               n, n_producers, _ptr(lat, _f64p), ctypes.byref(res))
+                               n, n_producers, _ptr(lat, _f64p), optr, ctypes.byref(res))
     _check(rc)
-    return {k: getattr(res, k) for k, _ in RvcLoadgenResult._fields_}, lat
+    summary = {k: getattr(res, k) for k, _ in RvcLoadgenResult._fields_}
+    return (summary, lat, outs) if outputs else (summary, lat)
 
 
 def raw_region(region, mqs):

@@ -29,7 +29,7 @@ double pct(std::vector<double> v, double p) {
 
 extern "C" int rvc_loadgen_run(const rvc_region* regions, uint32_t n_regions, const uint32_t* order,
                                const double* arrival_s, uint32_t n_arrivals, uint32_t n_producers,
-                               double* latency_ms, rvc_loadgen_result* result) {
+                               double* latency_ms, double* const* out_read_major, rvc_loadgen_result* result) {
   return rvc::guarded([&]() -> int {
     using rvc::set_error;
     if (!regions || !arrival_s || !result || n_producers == 0 || n_regions == 0)
@@ -57,7 +57,8 @@ extern "C" int rvc_loadgen_run(const rvc_region* regions, uint32_t n_regions, co
             cv.wait(lk, [&] { return n_sub > k; });
           }
           if (sub_rc[i] != RVC_OK) continue;
-          wait_rc[i] = rvc_phmm_wait(tickets[i], nullptr, nullptr, -1, &timing[i]);
+          wait_rc[i] = rvc_phmm_wait(tickets[i], out_read_major ? out_read_major[i] : nullptr, nullptr, -1,
+                                     &timing[i]);
         }
       });
       for (uint32_t i = p; i < n_arrivals; i += n_producers) {

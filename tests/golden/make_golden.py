@@ -7,6 +7,11 @@ Outputs per pair:
              FP32 then FP64 below 1e-28), built from the reference sources into
              oracle/_ref by oracle/Makefile
   fp64       the oracle's FP64 evaluation (C = 2^1020, N wildcard)
+For the 8 real regions of Rovaca's own schedule test data
+(pairhmm_debug.txt, ph/resource/) it writes pairhmm_debug_ref.npz: per region
+the ref_intra and fp64 matrices, flattened region after region (read-major),
+as the reference test computes its expected values
+(ph/test/pairhmm_unitest_from_assemble.cpp:69-107).
 Run from the repository root: python tests/golden/make_golden.py
 """
 import os
@@ -16,7 +21,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 import oracle  # noqa: E402
+from conftest import load_debug_regions  # noqa: E402
 from rovaca_amd import workload  # noqa: E402
 
 KEYS = ("hap_off", "hap_bases", "read_off", "read_bases", "q", "i", "d", "c")
@@ -55,6 +62,13 @@ def main():
     stress = [r for r in workload.stress_regions(60, seed=202) if r["n_haps"] * r["n_reads"] <= 1500][:3]
     for x, r in enumerate(stress):
         save(f"stress_seed202_{x}.npz", r)
+    refs, f64s = [], []
+    for reg in load_debug_regions():
+        ref, f64 = outputs(reg)
+        refs.append(ref.ravel())
+        f64s.append(f64.ravel())
+    np.savez_compressed(os.path.join(ROOT, "tests", "golden", "pairhmm_debug_ref.npz"),
+                        ref_intra=np.concatenate(refs), fp64=np.concatenate(f64s))
 
 
 if __name__ == "__main__":

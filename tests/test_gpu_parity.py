@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 import rovaca_amd as rv
-from conftest import GOLDEN
+from conftest import GOLDEN, load_debug_regions
 from rovaca_amd import workload
 
 pytestmark = pytest.mark.gpu
@@ -295,7 +295,7 @@ def test_call_pairhmm_drops_a_middle_read():
     haps = [reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]] for h in range(H)]
     reads = [reg["read_bases"][reg["read_off"][r]:reg["read_off"][r + 1]].copy() for r in range(N)]
     quals = [reg["q"][reg["read_off"][r]:reg["read_off"][r + 1]].copy() for r in range(N)]
-    rng = np.random.default_rng(5)
+    rng = np.random.default_rng(12345)  # not the region's own seed (its first draws are haplotype 0)
     bad = N // 2
     reads[bad] = workload.ACGT[rng.integers(0, 4, len(reads[bad]))]  # unrelated to every haplotype
     quals[bad][:] = 40
@@ -376,7 +376,7 @@ def test_segment_width_equals_default(width):
         reads, quals = [], []
         for r in range(N):
             o = reg["read_off"][r]
-            n = int(rng.integers(20, 121))
+            n = min(int(rng.integers(20, 121)), int(reg["read_off"][-1] - o))  # bases of the region only
             reads.append(reg["read_bases"][o:o + n])
             quals.append(reg["q"][o:o + n])
         if x == 0:  # N in a haplotype: compare-and-select kernel
@@ -554,3 +554,139 @@ def test_budget_admission():
     finally:
         rv.shutdown()
         rv.init(0, server=True)
+
+
+def test_real_regions_against_reference_fixture():
+    # Rovaca's own schedule test data: 8 real chrY regions, expected values from
+    # Rovaca's intra kernel, tolerance 1e-5 as ph/test/pairhmm_unitest_from_assemble.cpp:158
+    say("8 real regions (ph/resource/pairhmm_debug.txt) against Rovaca's intra kernel, 1e-5")
+    regs = load_debug_regions()
+    ref = np.load(os.path.join(GOLDEN, "pairhmm_debug_ref.npz"))["ref_intra"]
+    b = rv.Batch(regs)
+    b.run()
+    bout, _ = b.results()
+    o, worst = 0, 0.0
+    for x, reg in enumerate(regs):
+        n = reg["n_haps"] * reg["n_reads"]
+        want = ref[o:o + n].reshape(reg["n_reads"], reg["n_haps"])
+        o += n
+        got, flags = rv.region_sync(reg)
+        assert np.isfinite(got).all() and (flags & 1).all()
+        worst = max(worst, float(np.abs(got - want).max()))
+        assert np.array_equal(got, b.region_result(bout, x))
+    b.close()
+    assert o == len(ref)
+    assert worst <= 1e-5, worst
+    say(f"real regions: worst |d| {worst:.2e}")
+
+
+def _rescue_check(got, flags, route):
+    # the FP64 flag against the reference routing's FP64 routes (2: inter
+    # FP64, 4/5: intra FP64); a disagreement is allowed only within 1e-3 of
+    # the threshold log10(1e-28f) - 36.1236000061f = -64.1236
+    resc = (flags & rv.PAIR_FP64) != 0
+    ref_resc = np.isin(route, (2, 4, 5))
+    diff = resc != ref_resc
+    assert (np.abs(got[diff] + 64.1236000061) < 1e-3).all(), got[diff]
+    return int(resc.sum()), int(diff.sum())
+
+
+def test_rescue_membership_matches_reference_routing():
+    say("FP64 rescue membership against the reference routing on stress regions")
+    regs = [r for r in workload.stress_regions(24, seed=91) if r["n_haps"] * r["n_reads"] < 6000][:8]
+    for name in sorted(f for f in os.listdir(GOLDEN) if f.startswith("stress_")):
+        g = np.load(os.path.join(GOLDEN, name))
+        reg = {k: g[k] for k in ("hap_off", "hap_bases", "read_off", "read_bases", "q", "i", "d", "c")}
+        reg["n_haps"], reg["n_reads"] = len(reg["hap_off"]) - 1, len(reg["read_off"]) - 1
+        regs.append(reg)
+    n_resc = n_diff = 0
+    for reg in regs:
+        got, flags = rv.region_sync(reg)
+        want, route, ok = oracle.region_reference(reg)
+        a, d = _rescue_check(got, flags, route)
+        n_resc += a
+        n_diff += d
+    assert n_resc > 100
+    say(f"rescued {n_resc} pairs, {n_diff} within 1e-3 of the threshold on the other side")
+
+
+def test_deep_region_config4():
+    # 2000 reads of 250 bases x 128 haplotypes of 1000 bases (6.4e10 cells)
+    say("deep region 2000 x 128 (250 / 1000 bp): batch, plain and served mixed with small regions")
+    deep = workload.deep_region(seed=4)
+    b = rv.Batch([deep])
+    b.run()
+    out, flags = b.results()
+    got = b.region_result(out, 0)
+    fl = b.region_result(flags, 0)
+    b.close()
+    # every pair by property: finite, <= 0, FP64 exactly below the threshold
+    assert np.isfinite(got).all() and (got <= 0).all() and (fl & 1).all()
+    resc = (fl & rv.PAIR_FP64) != 0
+    assert (got[resc] < -64.1236 + 0.2).all() and (got[~resc] > -64.1236 - 0.2).all()
+    # 512 seeded pairs against the FP64 oracle
+    rng = np.random.default_rng(44)
+    rr = rng.integers(0, deep["n_reads"], 512)
+    hh = rng.integers(0, deep["n_haps"], 512)
+    worst = 0.0
+    for r, h in zip(rr, hh):
+        o0, o1 = deep["read_off"][r], deep["read_off"][r + 1]
+        hb = deep["hap_bases"][deep["hap_off"][h]:deep["hap_off"][h + 1]]
+        want = oracle.pair_log10_f64(hb, deep["read_bases"][o0:o1], *[deep[k][o0:o1] for k in "qidc"])
+        worst = max(worst, abs(got[r, h] - want))
+    assert worst < 1e-4, worst
+    say(f"deep region: 512 pairs vs FP64 oracle, worst |d| {worst:.2e}; {int(resc.sum())} rescued")
+    # the plain one-region path gives the same bits
+    alone, af = rv.region_sync(deep)
+    assert np.array_equal(alone, got) and np.array_equal(af, fl)
+    # served, mixed into a stream of small regions from 8 threads: same bits
+    small = [r for r in workload.batched_regions(24, seed=45) if r["n_haps"] * r["n_reads"] < 6000][:16]
+    small_alone = [rv.region_sync(r)[0] for r in small]
+    jobs = [("deep", deep)] + [(x, r) for x, r in enumerate(small)]
+    results, errors = {}, []
+
+    def worker(items):
+        try:
+            for key, reg in items:
+                t = rv.submit(reg)
+                results[key] = rv.wait(t, reg)[0]
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    th = [threading.Thread(target=worker, args=(jobs[k::8],)) for k in range(8)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not errors
+    assert np.array_equal(results["deep"], got)
+    for x, a in enumerate(small_alone):
+        m = results[x]
+        assert np.array_equal(np.isnan(a), np.isnan(m)) and np.array_equal(a[~np.isnan(a)], m[~np.isnan(m)])
+
+
+def test_served_poisson_stream_32_producers():
+    # the served stream at its stated shape: Poisson arrivals at 8k regions/s
+    # for 0.5 s from 32 producer threads, regions of every size of the batched
+    # workload (up to 64 haplotypes x 500 reads); every returned matrix equal
+    # to the region alone, bit for bit
+    say("served stream: 4000 Poisson arrivals at 8000/s from 32 producers, each result == alone")
+    pool = workload.batched_regions(48, seed=77)
+    assert max(r["n_haps"] * r["n_reads"] for r in pool) > 6000
+    alone = [rv.region_sync(r)[0] for r in pool]
+    rng = np.random.default_rng(78)
+    n = 4000
+    arr = np.cumsum(rng.exponential(1 / 8000.0, n))
+    order = rng.integers(0, len(pool), n).astype(np.uint32)
+    res, lat, outs = rv.loadgen_run(pool, arr, order, 32, outputs=True)
+    assert res["completed"] == n and res["failed"] == 0
+    for i in range(n):
+        a, m = alone[order[i]], outs[i]
+        assert np.array_equal(np.isnan(a), np.isnan(m)), i
+        assert np.array_equal(a[~np.isnan(a)], m[~np.isnan(m)]), i
+    # a sample of the regions against the oracle's reference routing
+    for x in [x for x, r in enumerate(pool) if r["n_haps"] * r["n_reads"] < 1500][:3]:
+        want, route, ok = oracle.region_reference(pool[x])
+        m = ~np.isnan(want)
+        assert np.abs(alone[x][m] - want[m]).max() < TOL
+    say(f"stream: p50 {res['lat_p50_ms']:.2f} ms, p99 {res['lat_p99_ms']:.2f} ms")

@@ -1,10 +1,13 @@
 """The oracle checked against what pins it: Rovaca's known-answer tests and
 closed forms, brute-force path enumeration, an unscaled long-double
 evaluation, and (where it was built) Rovaca's own intra kernel."""
+import os
+
 import numpy as np
 import pytest
 
 import oracle
+from conftest import GOLDEN
 from rovaca_amd import workload
 
 
@@ -170,3 +173,23 @@ def test_pcr_gop_follows_repeat_length():
     assert oracle.repeat_length(b"ACG", 0) == oracle.repeat_length(b"ACG", 1) == 1
     _, gi, _, _ = oracle.preprocess_read(b"ACG", np.full(3, 30, np.uint8), 60, 0, 1)
     assert list(gi) == [40, 40, 45]
+
+
+def test_oracle_on_real_regions_against_reference_fixture():
+    # the 8 real chrY regions of Rovaca's schedule test (ph/resource/pairhmm_debug.txt):
+    # the oracle's reference routing against Rovaca's intra kernel at 1e-5
+    # (ph/test/pairhmm_unitest_from_assemble.cpp:158)
+    from conftest import load_debug_regions
+    regs = load_debug_regions()
+    ref = np.load(os.path.join(GOLDEN, "pairhmm_debug_ref.npz"))
+    o = 0
+    for reg in regs:
+        n = reg["n_haps"] * reg["n_reads"]
+        want = ref["ref_intra"][o:o + n].reshape(reg["n_reads"], reg["n_haps"])
+        f64 = ref["fp64"][o:o + n].reshape(reg["n_reads"], reg["n_haps"])
+        o += n
+        got, route, ok = oracle.region_reference(reg)
+        assert ok and np.isfinite(got).all()
+        assert np.abs(got - want).max() <= 1e-5
+        assert np.abs(f64 - want).max() <= 1e-5
+    assert o == len(ref["ref_intra"])
