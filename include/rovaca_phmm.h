@@ -135,6 +135,15 @@ const char* rvc_phmm_last_error(void);
  * RVC_PAIR_* bits. Thread-safe; each call launches its own kernels. */
 int rvc_phmm_region_sync(const rvc_region* region, double* out_read_major, uint8_t* out_flags);
 
+/* No counterpart in the reference: the same call and results as
+ * rvc_phmm_region_sync for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It allocates no device or pinned host memory: the region's
+ * device image lives in a fixed 32 MiB device array set up by rvc_phmm_init;
+ * a region whose image exceeds it gets RVC_E_LIMIT. Calls are serialised
+ * (they share the array); each call clears what it reads with kernels. */
+int rvc_phmm_region_sync_noalloc(const rvc_region* region, double* out_read_major, uint8_t* out_flags);
+
 /* Replaces call_roDV_pairhmm_scheduled (ph/pairhmm_internal.cpp:576-623):
  * read preprocessing (MQ cap, min-quality floor, GOP/GCP defaults, PCR indel
  * model: ph/roDV_schedule.cpp:494-649, ph/pairhmm_internal.cpp:260-476),

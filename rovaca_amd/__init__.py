@@ -30,6 +30,7 @@ PCR_NONE, PCR_HOSTILE, PCR_AGGRESSIVE, PCR_CONSERVATIVE = 0, 1, 2, 3
 # Every symbol include/rovaca_phmm.h declares.
 EXPORTS = [
     "rvc_phmm_init", "rvc_phmm_shutdown", "rvc_phmm_last_error", "rvc_phmm_region_sync",
+    "rvc_phmm_region_sync_noalloc",
     "rvc_phmm_call_pairhmm", "rvc_phmm_batch_create", "rvc_phmm_batch_run", "rvc_phmm_batch_results",
     "rvc_phmm_batch_info", "rvc_phmm_batch_destroy", "rvc_phmm_submit", "rvc_phmm_wait", "rvc_phmm_poll",
     "rvc_phmm_server_stats", "rvc_phmm_preprocess_read", "rvc_phmm_postprocess", "rvc_phmm_tables",
@@ -116,6 +117,7 @@ def lib():
         L.rvc_phmm_init.argtypes = [ctypes.POINTER(RvcConfig)]
         L.rvc_phmm_last_error.restype = ctypes.c_char_p
         L.rvc_phmm_region_sync.argtypes = [ctypes.POINTER(RvcRegion), _f64p, _u8p]
+        L.rvc_phmm_region_sync_noalloc.argtypes = [ctypes.POINTER(RvcRegion), _f64p, _u8p]
         L.rvc_phmm_call_pairhmm.argtypes = [ctypes.POINTER(RvcRawRegion), ctypes.c_int32, ctypes.c_int32, _f64p,
                                             _u32p, _u32p]
         L.rvc_phmm_batch_create.argtypes = [ctypes.POINTER(RvcRegion), ctypes.c_uint32,
@@ -197,6 +199,17 @@ def region_sync(region):
     flags = np.zeros(nr * nh, np.uint8)
     cr = make_c_region(region)
     _check(lib().rvc_phmm_region_sync(ctypes.byref(cr), _ptr(out, _f64p), _ptr(flags)))
+    return out.reshape(nr, nh), flags.reshape(nr, nh)
+
+
+def region_sync_noalloc(region):
+    """region_sync through the entry point that allocates no device memory
+    (rvc_phmm_region_sync_noalloc): the same matrix and flags."""
+    nr, nh = int(region["n_reads"]), int(region["n_haps"])
+    out = np.empty(nr * nh, np.float64)
+    flags = np.zeros(nr * nh, np.uint8)
+    cr = make_c_region(region)
+    _check(lib().rvc_phmm_region_sync_noalloc(ctypes.byref(cr), _ptr(out, _f64p), _ptr(flags)))
     return out.reshape(nr, nh), flags.reshape(nr, nh)
 
 

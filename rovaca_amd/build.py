@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librovaca_phmm.so")
 SOURCES = ["phmm_kernels.hip", "phmm_engine.cpp", "phmm_tables.cpp", "phmm_capi.cpp", "phmm_adapter.cpp",
-           "phmm_server.cpp", "phmm_loadgen.cpp", "sw_kernels.hip", "sw_capi.cpp",
+           "phmm_server.cpp", "phmm_loadgen.cpp", "phmm_noalloc.cpp", "sw_kernels.hip", "sw_capi.cpp",
            "sw_server.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: keep the compiler from forming v_pk_{add,mul,fma}_f32.

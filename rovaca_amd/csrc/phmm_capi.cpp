@@ -129,9 +129,14 @@ int rvc_phmm_init(const rvc_config* cfg) {
   return rvc::guarded([&]() -> int {
     int rc = engine_init(cfg);
     if (rc) return rc;
+    if ((rc = noalloc_init())) {
+      engine_shutdown();
+      return rc;
+    }
     if (cfg && cfg->server_threads) {
       rc = server_start();
       if (rc) {
+        noalloc_shutdown();
         engine_shutdown();
         return rc;
       }
@@ -143,6 +148,7 @@ int rvc_phmm_init(const rvc_config* cfg) {
 int rvc_phmm_shutdown(void) {
   return rvc::guarded([&]() -> int {
     server_stop();
+    noalloc_shutdown();
     release_workspaces();
     sw_server_stop();
     sw_release_workspaces();
