@@ -174,24 +174,33 @@ def run_single(args):
     return res
 
 
-def served_stream(pool, deep, rates, duration, producers):
+def served_stream(pool, deep, rates, duration, producers, warmup_s=0.3):
     """Poisson arrivals of pool regions (and 1 % deep ones if given) from
-    'producers' host threads at each rate, through the serving loop."""
+    'producers' host threads at each rate, through the serving loop. Before
+    each timed window a short untimed stream at the same rate (and with the
+    same mix) sizes the slot arenas, staging and result buffers, so that the
+    timed window reallocates nothing (buffer_grows reports it)."""
     import rovaca_amd as rv
     regions = pool + deep
     rv.init(0, server=True, hbm_budget_bytes=64 << 30, max_batch_regions=4096)
     rng = np.random.default_rng(9)
-    # warm-up: a short stream at the lowest rate
-    arr = np.cumsum(rng.exponential(1.0 / rates[0], 200))
-    rv.loadgen_run(regions, arr, np.arange(200) % len(pool), producers)
-    out = []
-    for rate in rates:
-        n = max(64, int(rate * duration))
+
+    def arrivals(rate, n):
         arr = np.cumsum(rng.exponential(1.0 / rate, n))
         order = rng.integers(0, len(pool), n)
         if deep:
             is_deep = rng.random(n) < 0.01
             order[is_deep] = len(pool) + rng.integers(0, len(deep), int(is_deep.sum()))
+        return arr, order
+
+    out = []
+    for rate in rates:
+        wa, wo = arrivals(rate, max(64, int(rate * warmup_s)))
+        if deep:
+            wo[:len(deep)] = len(pool) + np.arange(len(deep))  # every deep region once in the warm-up
+        rv.loadgen_run(regions, wa, wo, producers)
+        n = max(64, int(rate * duration))
+        arr, order = arrivals(rate, n)
         st0 = rv.server_stats()
         res, lat = rv.loadgen_run(regions, arr, order, producers)
         st1 = rv.server_stats()
@@ -210,12 +219,177 @@ def served_stream(pool, deep, rates, duration, producers):
             dl = lat[order >= len(pool)]
             rec["deep_regions"] = int((order >= len(pool)).sum())
             rec["deep_p50_ms"] = round(float(np.median(dl)), 3) if len(dl) else None
+            rec["deep_p99_ms"] = round(float(np.percentile(dl, 99)), 3) if len(dl) else None
+            rec["cells"] = int(res["cells"])
         out.append(rec)
         log(f"rate {rate:.0f}/s: {rec['gcups']} GCUPS, p50 {rec['p50_ms']} ms, p99 {rec['p99_ms']} ms, "
-            f"{rec['launches']} launches")
+            f"{rec['launches']} launches, {rec['buffer_grows']} buffer grows")
     st = rv.server_stats()
     rv.shutdown()
     return out, st
+
+
+def highest_rate_under(rates, p99_ms=10.0):
+    ok = [r["rate_per_s"] for r in rates if r["p99_ms"] < p99_ms]
+    return max(ok) if ok else None
+
+
+def single_region_config(no_cpu, reps=400):
+    """The plain single active region (8 haplotypes x 32 reads): per-call
+    latency of the one-region path, and Rovaca's region scheme on one host
+    thread for the same region."""
+    import rovaca_amd as rv
+    from rovaca_amd import workload
+    reg = workload.single_region(0)
+    cells = workload.cells([reg])
+    rv.init(0)
+    for _ in range(20):
+        rv.region_sync(reg)
+    lat = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        rv.region_sync(reg)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    rv.shutdown()
+    lat = np.array(lat)
+    out = {"workload": "single_region_8x32", "cells": cells, "calls": reps,
+           "median_ms": round(float(np.median(lat)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+           "gcups_at_median": round(cells / (np.median(lat) * 1e-3) / 1e9, 2)}
+    if not no_cpu:
+        import oracle
+        oracle.baseline_run([reg], 1)
+        secs = min(oracle.baseline_run([reg], 1)[0] for _ in range(5))
+        out["cpu_baseline"] = {"value": round(secs * 1e3, 4), "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": "the same region, one worker thread, best of 5"}
+        out["cpu_over_gpu_latency"] = round(secs * 1e3 / out["median_ms"], 1)
+    return out
+
+
+def fixture_accuracy():
+    """The device against Rovaca's own intra kernel on the committed fixtures
+    (inputs and reference outputs stored together): max/mean |delta log10 L|."""
+    import rovaca_amd as rv
+    d, n = [], 0
+    gold = os.path.join(ROOT, "tests", "golden")
+    for name in sorted(os.listdir(gold)):
+        if not (name.endswith(".npz") and name.startswith(("single_", "batched_", "stress_"))):
+            continue
+        g = np.load(os.path.join(gold, name))
+        reg = {k: g[k] for k in ("hap_off", "hap_bases", "read_off", "read_bases", "q", "i", "d", "c")}
+        reg["n_haps"], reg["n_reads"] = len(reg["hap_off"]) - 1, len(reg["read_off"]) - 1
+        got, _ = rv.region_sync(reg)
+        m = ~np.isnan(g["ref_intra"])
+        d.append(np.abs(got[m] - g["ref_intra"][m]))
+        n += 1
+    d = np.concatenate(d)
+    return {"fixtures": n, "pairs": int(len(d)), "max_abs_vs_reference_intra": float(d.max()),
+            "mean_abs_vs_reference_intra": float(d.mean())}
+
+
+def stress_config(args):
+    """Precision stress (2048 regions, 30 % of reads 250 bp with 40 %
+    substitutions and Q 2-10): batched GCUPS, FP32/FP64 split, rescued
+    fraction, accuracy against the FP64 oracle and Rovaca's intra kernel
+    (committed fixtures), CPU baseline, and the plain and served forms."""
+    import rovaca_amd as rv
+    from rovaca_amd import workload
+    regions = workload.stress_regions(2048, seed=2)
+    cells = workload.cells(regions)
+    rv.init(0)
+    b = rv.Batch(regions, use_graph=True)
+    b.run()
+    t0 = time.perf_counter()
+    steps = 4
+    for _ in range(steps):
+        b.run()
+    wall = time.perf_counter() - t0
+    b.close()
+    e = rv.Batch(regions)
+    e.run()
+    split = []
+    for _ in range(2):
+        e.run()
+        split.append(e.info())
+    out_v, flags = e.results()
+    rescued_cells = 0
+    for x, reg in enumerate(regions):
+        fl = e.region_result(flags, x)
+        if (fl & 2).any():
+            hl, rl = np.diff(reg["hap_off"]), np.diff(reg["read_off"])
+            rr, hh = np.nonzero(fl & 2)
+            rescued_cells += int((rl[rr].astype(np.int64) * hl[hh]).sum())
+    fp32_ms = float(np.mean([x["fp32_ms"] for x in split]))
+    fp64_ms = float(np.mean([x["fp64_ms"] for x in split]))
+    n_pairs = int(split[-1]["n_pairs"])
+    n_resc = int(split[-1]["n_rescued"])
+    value = cells * steps / wall / 1e9
+    rec = {"workload": "precision_stress_2048_regions", "regions": len(regions), "pairs": n_pairs, "cells": cells,
+           "gcups": round(value, 1), "ms_per_step": round(wall / steps * 1e3, 3),
+           "device_ms": {"fp32_pass": round(fp32_ms, 3), "fp64_rescue": round(fp64_ms, 3)},
+           "rescue": {"pairs": n_resc, "fraction": round(n_resc / max(1, n_pairs), 4), "cells": rescued_cells,
+                      "fp64_tflops": round(rescued_cells * FLOPS_PER_CELL / (fp64_ms * 1e-3) / 1e12, 2),
+                      "fp64_frac": round(rescued_cells * FLOPS_PER_CELL / (fp64_ms * 1e-3) / 1e12
+                                         / FP64_PEAK_TFLOPS, 4)}}
+    log(f"stress: {rec['gcups']} GCUPS, FP32 {fp32_ms:.1f} ms + FP64 {fp64_ms:.1f} ms, {n_resc} rescued")
+    small = [x for x, r in enumerate(regions) if r["n_haps"] * r["n_reads"] <= 1000][:8]
+    if small:
+        rec["accuracy"] = accuracy(regions, e, out_v, small)
+    rec["accuracy_vs_reference_fixtures"] = fixture_accuracy()
+    e.close()
+    rec["plain_gpu"] = plain_gpu(regions, 128)
+    rec["batched_over_plain"] = round(value / rec["plain_gpu"]["value"], 2)
+    rv.shutdown()
+    srv, _ = served_stream(regions, [], [16000.0], 0.5, args.producers)
+    rec["served"] = srv[0]
+    rec["served_over_plain"] = round(srv[0]["gcups"] / rec["plain_gpu"]["value"], 2)
+    if not args.no_cpu:
+        try:
+            rec["cpu_baseline"] = cpu_baseline(regions, 10.0)
+            rec["gpu_over_cpu"] = round(value / rec["cpu_baseline"]["value"], 1)
+            rec["served_over_cpu"] = round(srv[0]["gcups"] / rec["cpu_baseline"]["value"], 1)
+        except Exception as ex:  # noqa: BLE001
+            rec["cpu_baseline"] = {"error": str(ex)}
+    return rec
+
+
+def deep_config(pool, args):
+    """Deep-coverage tail under serving: 1 % of the served stream replaced by
+    regions of 2000 reads (250 bp) x 128 haplotypes (1000 bp), at each rate,
+    with the plain per-region form and Rovaca's region scheme on the host
+    cores on a sample of the same mix (99 pool regions + 1 deep region)."""
+    from rovaca_amd import workload
+    deep = [workload.deep_region(seed=40 + x) for x in range(4)]
+    rates = [float(x) for x in args.rates.split(",")]
+    srv, st = served_stream(pool, deep, rates, args.duration, args.producers)
+    rec = {"workload": "served_with_1pct_deep_regions", "deep_region": "2000 reads x 250 bp, 128 haplotypes x 1000 bp",
+           "deep_cells": workload.cells(deep[:1]), "pool_regions": len(pool), "producers": args.producers,
+           "rates": srv, "highest_rate_p99_under_10ms": highest_rate_under(srv),
+           "peak_resident_gib": round(st["peak_resident_bytes"] / 2**30, 3)}
+    import rovaca_amd as rv
+    mix = pool[:99] + deep[:1]
+    rv.init(0)
+    rv.region_sync(mix[0])
+    t0 = time.perf_counter()
+    for r in mix:
+        rv.region_sync(r)
+    secs = time.perf_counter() - t0
+    rv.shutdown()
+    mix_cells = workload.cells(mix)
+    rec["plain_gpu"] = {"value": round(mix_cells / secs / 1e9, 1), "unit": "GCUPS",
+                        "sample": "99 pool regions + 1 deep region, one region per launch"}
+    top = srv[-1]["gcups"]
+    rec["served_over_plain"] = round(top / rec["plain_gpu"]["value"], 2)
+    if not args.no_cpu:
+        import oracle
+        threads = max(1, min(16, os.cpu_count() or 1))
+        oracle.baseline_run(mix[:8], threads)
+        runs = sorted(oracle.baseline_run(mix, threads)[0] for _ in range(3))
+        rec["cpu_baseline"] = {"value": round(mix_cells / runs[1] / 1e9, 2), "unit": "GCUPS", "cores": threads,
+                               "kind": "port",
+                               "sample": f"99 pool regions + 1 deep region ({mix_cells / 1e9:.1f} Gcells), "
+                                         f"median of 3 runs ({', '.join(f'{x:.2f}' for x in runs)} s wall)"}
+        rec["served_over_cpu"] = round(top / rec["cpu_baseline"]["value"], 1)
+    return rec
 
 
 def run_served(args):
@@ -224,21 +398,24 @@ def run_served(args):
     from rovaca_amd import workload
     t = time.time()
     pool = workload.batched_regions(min(args.regions, 2048), seed=3)
-    deep = []
+    log(f"pool of {len(pool)} regions generated in {time.time() - t:.1f} s")
     if args.workload == "deep":
-        deep = [workload.deep_region(seed=40 + x) for x in range(4)]
-    log(f"pool of {len(pool)} regions (+{len(deep)} deep) generated in {time.time() - t:.1f} s")
-    rates = [float(x) for x in args.rates.split(",")] if args.workload == "served" else [2000.0]
-    out, st = served_stream(pool, deep, rates, args.duration, args.producers)
+        rec = deep_config(pool, args)
+        top = rec["rates"][-1]
+        return {"metric": "PairHMM served_with_1pct_deep_regions GCUPS + p99 region latency (1 MI355X)",
+                "value": top["gcups"], "unit": "GCUPS", "higher_is_better": True, "n_gpus": 1, "dtype": "f32",
+                "rescue_dtype": "f64", "data": "synthetic", "config": {"workload": rec["workload"]}, **rec}
+    rates = [float(x) for x in args.rates.split(",")]
+    out, st = served_stream(pool, [], rates, args.duration, args.producers)
     top = out[-1]
-    name = "served" if args.workload == "served" else "served_with_1pct_deep_regions"
-    return {"metric": f"PairHMM {name} GCUPS + p99 region latency (1 MI355X)", "value": top["gcups"],
+    return {"metric": "PairHMM served GCUPS + p99 region latency (1 MI355X)", "value": top["gcups"],
             "unit": "GCUPS", "higher_is_better": True, "n_gpus": 1, "dtype": "f32", "rescue_dtype": "f64",
-            "data": "synthetic", "config": {"workload": name, "pool_regions": len(pool), "deep_regions": len(deep),
+            "data": "synthetic", "config": {"workload": "served", "pool_regions": len(pool),
                                             "producers": args.producers, "hbm_budget_gib": 64,
                                             "arrivals": "poisson", "duration_s_per_rate": args.duration},
-            "rates": out, "server": {"launches": st["launches"], "peak_resident_gib": round(st["peak_resident_bytes"] / 2**30, 3),
-                                     "rescued_pairs": st["rescued_pairs"]}}
+            "rates": out, "highest_rate_p99_under_10ms": highest_rate_under(out),
+            "server": {"launches": st["launches"], "peak_resident_gib": round(st["peak_resident_bytes"] / 2**30, 3),
+                       "rescued_pairs": st["rescued_pairs"]}}
 
 
 def run_calls(args):
@@ -402,7 +579,7 @@ def main():
     ap.add_argument("--workload", default="batched",
                     choices=["batched", "stress", "single", "served", "deep", "calls", "sw", "sw_calls"])
     ap.add_argument("--calls", type=int, default=8192, help="calls: region calls per run")
-    ap.add_argument("--rates", default="2000,8000,32000", help="served: arrival rates, regions/s")
+    ap.add_argument("--rates", default="2000,8000,16000,24000,32000", help="served: arrival rates, regions/s")
     ap.add_argument("--duration", type=float, default=1.0, help="served: seconds of arrivals per rate")
     ap.add_argument("--producers", type=int, default=32)
     ap.add_argument("--assemble", action="store_true", help="sw workload: the assembly scores (the int32 sweep)")
@@ -412,6 +589,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip plain-GPU and accuracy samples")
     ap.add_argument("--no-served", action="store_true", help="skip the served stream of the batched workload")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configs in the default line (single region, precision stress, deep tail)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (known bytes) to scale HBM byte counters")
     args = ap.parse_args()
@@ -550,7 +729,9 @@ def main():
         rates = [float(x) for x in args.rates.split(",")]
         srv, _ = served_stream(regions, [], rates, args.duration, args.producers)
         result["served"] = {"pool": wname, "producers": args.producers, "arrivals": "poisson",
-                            "duration_s_per_rate": args.duration, "hbm_budget_gib": 64, "rates": srv}
+                            "duration_s_per_rate": args.duration, "hbm_budget_gib": 64,
+                            "warmup": "0.3 s of arrivals at each rate before its window", "rates": srv,
+                            "highest_rate_p99_under_10ms": highest_rate_under(srv)}
         if "plain_gpu" in result:
             result["served"]["top_rate_over_plain"] = round(srv[-1]["gcups"] / result["plain_gpu"]["value"], 2)
     if not args.no_cpu:
@@ -563,6 +744,16 @@ def main():
                     result["served"]["rates"][-1]["gcups"] / result["cpu_baseline"]["value"], 1)
         except Exception as e:  # noqa: BLE001
             result["cpu_baseline"] = {"error": str(e)}
+    if not args.no_configs and args.workload == "batched":
+        # the other configs of the benchmark, each with its own baselines
+        log("config: single region 8 x 32")
+        result["single_region"] = single_region_config(args.no_cpu)
+        log("config: precision stress, 2048 regions")
+        result["precision_stress"] = stress_config(args)
+        log("config: deep-coverage tail regions under serving")
+        pool = [r for r in regions[:2048]]
+        result["deep_tail"] = deep_config(pool, argparse.Namespace(rates="2000,8000,32000", duration=args.duration,
+                                                                    producers=args.producers, no_cpu=args.no_cpu))
     if not args.no_extras and args.workload == "batched":
         # the next kernel on the same regions: every read realigned to its
         # best haplotype, as one batch and as one served call per region

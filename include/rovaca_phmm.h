@@ -158,6 +158,50 @@ int rvc_phmm_region_sync_noalloc(const rvc_region* region, double* out_read_majo
 int rvc_phmm_call_pairhmm(const rvc_raw_region* region, int32_t min_quality_threshold, int32_t pcr_option,
                           double* out_hap_major, uint32_t* kept_reads, uint32_t* n_kept);
 
+/* A queued request of the serving loop (rvc_phmm_submit and
+ * rvc_phmm_call_pairhmm_submit). */
+typedef uint64_t rvc_ticket;
+
+/* The variant sites of a region, for haplotype -> allele marginalisation at
+ * the genotyping boundary (AlleleLikelihoods::marginalize,
+ * hc/genotype/allele_likelihoods.hpp:189-217, 701-732, called per site at
+ * hc/genotype/germline_genotying_engine.cpp:143 with the allele mapper of
+ * AssemblyBasedCallerUtils::create_allele_mapper, which depends only on the
+ * haplotypes). hap_allele[s * n_haps + h] is the allele of haplotype h at
+ * site s, or -1 where the haplotype maps to none (each haplotype maps to at
+ * most one allele of a site, as old_to_new_allele_index_map requires). */
+typedef struct rvc_sites {
+  uint32_t n_sites;
+  const uint32_t* n_alleles;  /* n_sites entries, each 1 .. 32767 */
+  const int32_t* hap_allele;  /* n_sites * n_haps entries */
+} rvc_sites;
+
+/* The same region-level call in two halves, so that a worker can go on with
+ * its next region (assembly, the CPU steps before PairHMM) while this one is
+ * on the device; Rovaca's worker otherwise blocks on call_pairhmm for every
+ * region (hc/haplotypecaller_engine.cpp:149-289, the call at :254).
+ * rvc_phmm_call_pairhmm_submit preprocesses the reads (on the calling thread
+ * or, as configured, on the device), packs the region and queues it on the
+ * serving loop (rvc_config.server_threads = 1, else RVC_E_STATE); it returns
+ * at once, and the caller's buffers are free when it does. A region the
+ * engine cannot take (RVC_E_BASES, RVC_E_LIMIT, RVC_E_BUDGET ...) is refused
+ * here, with nothing queued. rvc_phmm_poll(ticket) tells whether it is done. */
+int rvc_phmm_call_pairhmm_submit(const rvc_raw_region* region, int32_t min_quality_threshold, int32_t pcr_option,
+                                 const rvc_sites* sites, rvc_ticket* ticket);
+/* Waits for a submitted region call (timeout_ms < 0: no limit) and finishes
+ * it as rvc_phmm_call_pairhmm does: normalize, removal of poorly modelled
+ * reads and transpose into out_hap_major / kept_reads / n_kept; on failure
+ * all NaN with every read kept. RVC_E_TIMEOUT leaves the ticket valid.
+ * With sites given at submit, the device also marginalises the normalised
+ * matrix while it is resident (allele_likelihoods.hpp:701-732: per read the
+ * largest likelihood of the haplotypes mapped to the allele, -inf for an
+ * allele no haplotype maps to, a NaN never the largest) and out_alleles
+ * receives, site after site, [n_alleles[s]][n_reads] values whose first
+ * *n_kept columns belong to kept_reads (as the reference marginalises the
+ * kept reads) and the rest NaN. out_alleles may be NULL. */
+int rvc_phmm_call_pairhmm_wait(rvc_ticket ticket, double* out_hap_major, uint32_t* kept_reads, uint32_t* n_kept,
+                               double* out_alleles, int64_t timeout_ms);
+
 /* ---- batched: many regions in one launch sequence -------------------------- */
 typedef struct rvc_batch rvc_batch;
 
@@ -227,7 +271,6 @@ int rvc_phmm_batch_destroy(rvc_batch* b);
  * pinned staging and returns at once; the serving loop admits requests while
  * their resident buffers fit the HBM budget, launches them together with
  * whatever else is waiting, and completes each ticket on its own. */
-typedef uint64_t rvc_ticket;
 
 /* Steady-clock nanoseconds of a request's life: submitted, launched in a
  * batch, results ready on the host. */

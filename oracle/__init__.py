@@ -63,6 +63,9 @@ def lib():
         L.orc_sw_align.restype = ctypes.c_int
         L.orc_postprocess.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, _i32p, _f64p, _i32p]
         L.orc_postprocess.restype = ctypes.c_int
+        L.orc_marginalize.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_int, _u32p, _i32p,
+                                      _f64p]
+        L.orc_marginalize.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -197,6 +200,25 @@ def postprocess(read_major, read_len):
     nk = lib().orc_postprocess(n, h, rm.ctypes.data_as(_f64p), rl.ctypes.data_as(_i32p),
                                hap_major.ctypes.data_as(_f64p), kept.ctypes.data_as(_i32p))
     return hap_major, kept[:nk].copy()
+
+
+def marginalize(hap_major, n_cols, n_alleles, hap_allele):
+    """Per site [n_alleles[s]][N] marginals of a hap-major [H][N] matrix over
+    its first n_cols columns (hap_allele: [n_sites][H], -1 = no allele)."""
+    hm = np.ascontiguousarray(hap_major, dtype=np.float64)
+    H, N = hm.shape
+    na = np.ascontiguousarray(n_alleles, dtype=np.uint32)
+    ha = np.ascontiguousarray(hap_allele, dtype=np.int32).reshape(len(na), H)
+    out = np.empty(int(na.sum()) * N, np.float64)
+    rc = lib().orc_marginalize(H, N, hm.ctypes.data_as(_f64p), int(n_cols), len(na), na.ctypes.data_as(_u32p),
+                               ha.ctypes.data_as(_i32p), out.ctypes.data_as(_f64p))
+    if rc != 0:
+        raise ValueError("allele map out of range")
+    res, o = [], 0
+    for A in na:
+        res.append(out[o:o + int(A) * N].reshape(int(A), N))
+        o += int(A) * N
+    return res
 
 
 class Region(ctypes.Structure):

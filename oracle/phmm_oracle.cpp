@@ -33,6 +33,10 @@
 //   orc_postprocess         ph/pairhmm_internal.cpp:22-44 (poorly modelled
 //                           reads), :248-258 (normalize), :489-508 (transpose),
 //                           :576-623 (wrapper order), constants ph/adapter.h:20-23
+//   orc_marginalize         hc/genotype/allele_likelihoods.hpp:701-732
+//                           (marginal_likelihoods_direct) with the index map of
+//                           :735-752, on the first n_cols columns (the evidence
+//                           count) of a hap-major [H][N] matrix
 //
 // Compiled with -ffp-contract=off so every product and sum rounds exactly as
 // written; the reference's GCC build may fuse some of them (FMA), which moves
@@ -609,4 +613,37 @@ extern "C" int orc_postprocess(int n, int h, double* read_major, const int32_t* 
     kept[nk++] = r;
   }
   return nk;
+}
+
+// Haplotype -> allele marginalisation of one sample's [H][N] hap-major matrix
+// for n_sites sites, as AlleleLikelihoods::marginalize does it
+// (hc/genotype/allele_likelihoods.hpp:189-217): per new allele a and read r <
+// n_cols the running value starts at -infinity and every old allele (here a
+// haplotype) mapped to a replaces it when strictly larger (:713-729); old
+// alleles mapped to none are skipped (:717-719). out holds, site after site,
+// [n_alleles[s]][N] values; columns n_cols .. N-1 are NaN. Returns 0, or -1
+// if a map entry is out of range (the reference exits there, :746-749).
+extern "C" int orc_marginalize(int H, int N, const double* hap_major, int n_cols, int n_sites,
+                               const uint32_t* n_alleles, const int32_t* hap_allele, double* out) {
+  size_t o = 0;
+  for (int s = 0; s < n_sites; s++) {
+    const int A = (int)n_alleles[s];
+    const int32_t* old2new = hap_allele + (size_t)s * H;
+    for (int h = 0; h < H; h++)
+      if (old2new[h] < -1 || old2new[h] >= A) return -1;
+    std::vector<double> nv((size_t)A * N, -std::numeric_limits<double>::infinity());
+    for (int ai = 0; ai < H; ai++) {
+      const int na = old2new[ai];
+      if (na == -1) continue;
+      for (int ri = 0; ri < n_cols; ri++) {
+        const double lk = hap_major[(size_t)ai * N + ri];
+        if (lk > nv[(size_t)na * N + ri]) nv[(size_t)na * N + ri] = lk;
+      }
+    }
+    for (int a = 0; a < A; a++)
+      for (int r = 0; r < N; r++)
+        out[o + (size_t)a * N + r] = r < n_cols ? nv[(size_t)a * N + r] : std::numeric_limits<double>::quiet_NaN();
+    o += (size_t)A * N;
+  }
+  return 0;
 }

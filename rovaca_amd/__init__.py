@@ -31,7 +31,7 @@ PCR_NONE, PCR_HOSTILE, PCR_AGGRESSIVE, PCR_CONSERVATIVE = 0, 1, 2, 3
 EXPORTS = [
     "rvc_phmm_init", "rvc_phmm_shutdown", "rvc_phmm_last_error", "rvc_phmm_region_sync",
     "rvc_phmm_region_sync_noalloc",
-    "rvc_phmm_call_pairhmm", "rvc_phmm_batch_create", "rvc_phmm_batch_run", "rvc_phmm_batch_results",
+    "rvc_phmm_call_pairhmm", "rvc_phmm_call_pairhmm_submit", "rvc_phmm_call_pairhmm_wait", "rvc_phmm_batch_create", "rvc_phmm_batch_run", "rvc_phmm_batch_results",
     "rvc_phmm_batch_info", "rvc_phmm_batch_destroy", "rvc_phmm_submit", "rvc_phmm_wait", "rvc_phmm_poll",
     "rvc_phmm_server_stats", "rvc_phmm_preprocess_read", "rvc_phmm_postprocess", "rvc_phmm_tables",
     "rvc_loadgen_run", "rvc_loadgen_call_pairhmm", "rvc_phmm_batch_kernels", "rvc_sw_align_batch", "rvc_sw_align_served", "rvc_loadgen_sw_align",
@@ -53,6 +53,10 @@ class RvcRawRegion(ctypes.Structure):
     _fields_ = [("n_haps", ctypes.c_uint32), ("n_reads", ctypes.c_uint32), ("hap_off", _i32p),
                 ("hap_bases", _u8p), ("read_off", _i32p), ("read_bases", _u8p), ("read_qual", _u8p),
                 ("read_mq", _u8p)]
+
+
+class RvcSites(ctypes.Structure):
+    _fields_ = [("n_sites", ctypes.c_uint32), ("n_alleles", _u32p), ("hap_allele", _i32p)]
 
 
 class RvcConfig(ctypes.Structure):
@@ -120,6 +124,9 @@ def lib():
         L.rvc_phmm_region_sync_noalloc.argtypes = [ctypes.POINTER(RvcRegion), _f64p, _u8p]
         L.rvc_phmm_call_pairhmm.argtypes = [ctypes.POINTER(RvcRawRegion), ctypes.c_int32, ctypes.c_int32, _f64p,
                                             _u32p, _u32p]
+        L.rvc_phmm_call_pairhmm_submit.argtypes = [ctypes.POINTER(RvcRawRegion), ctypes.c_int32, ctypes.c_int32,
+                                                   ctypes.POINTER(RvcSites), ctypes.POINTER(ctypes.c_uint64)]
+        L.rvc_phmm_call_pairhmm_wait.argtypes = [ctypes.c_uint64, _f64p, _u32p, _u32p, _f64p, ctypes.c_int64]
         L.rvc_phmm_batch_create.argtypes = [ctypes.POINTER(RvcRegion), ctypes.c_uint32,
                                             ctypes.POINTER(RvcBatchOpts), ctypes.POINTER(ctypes.c_void_p)]
         L.rvc_phmm_batch_run.argtypes = [ctypes.c_void_p]
@@ -213,8 +220,7 @@ def region_sync_noalloc(region):
     return out.reshape(nr, nh), flags.reshape(nr, nh)
 
 
-def call_pairhmm(haps, reads, quals, mqs, min_quality_threshold=0, pcr=PCR_NONE):
-    """The region-level call of the reference (raw reads in, [H][N] matrix and kept reads out)."""
+def _raw_arrays(haps, reads, quals, mqs):
     hap_off = np.zeros(len(haps) + 1, np.int32)
     hap_off[1:] = np.cumsum([len(h) for h in haps])
     read_off = np.zeros(len(reads) + 1, np.int32)
@@ -228,6 +234,13 @@ def call_pairhmm(haps, reads, quals, mqs, min_quality_threshold=0, pcr=PCR_NONE)
     raw.hap_off, raw.hap_bases = _ptr(hap_off, _i32p), _ptr(hb)
     raw.read_off, raw.read_bases = _ptr(read_off, _i32p), _ptr(rb)
     raw.read_qual, raw.read_mq = _ptr(rq), _ptr(mq)
+    raw._keep = (hap_off, read_off, hb, rb, rq, mq)
+    return raw
+
+
+def call_pairhmm(haps, reads, quals, mqs, min_quality_threshold=0, pcr=PCR_NONE):
+    """The region-level call of the reference (raw reads in, [H][N] matrix and kept reads out)."""
+    raw = _raw_arrays(haps, reads, quals, mqs)
     H, N = len(haps), len(reads)
     out = np.empty(max(H * N, 1), np.float64)
     kept = np.zeros(max(N, 1), np.uint32)
@@ -235,6 +248,50 @@ def call_pairhmm(haps, reads, quals, mqs, min_quality_threshold=0, pcr=PCR_NONE)
     rc = lib().rvc_phmm_call_pairhmm(ctypes.byref(raw), min_quality_threshold, pcr, _ptr(out, _f64p),
                                      _ptr(kept, _u32p), ctypes.byref(nk))
     return rc, out[:H * N].reshape(H, N), kept[:nk.value].copy()
+
+
+def call_pairhmm_submit(haps, reads, quals, mqs, min_quality_threshold=0, pcr=PCR_NONE, sites=None):
+    """First half of the region-level call: returns (rc, ticket) at once.
+    sites = (n_alleles per site, hap_allele [n_sites][n_haps] with -1 for none)
+    asks the device to marginalise the region's matrix per site."""
+    raw = _raw_arrays(haps, reads, quals, mqs)
+    t = ctypes.c_uint64(0)
+    sp = None
+    if sites is not None:
+        na = np.ascontiguousarray(sites[0], np.uint32)
+        ha = np.ascontiguousarray(sites[1], np.int32).reshape(-1)
+        st = RvcSites(len(na), _ptr(na, _u32p), _ptr(ha, _i32p))
+        st._keep = (na, ha)
+        sp = ctypes.byref(st)
+    rc = lib().rvc_phmm_call_pairhmm_submit(ctypes.byref(raw), min_quality_threshold, pcr, sp, ctypes.byref(t))
+    return rc, t.value
+
+
+def call_pairhmm_wait(ticket, n_haps, n_reads, timeout_ms=-1, n_alleles=None):
+    """Second half: (rc, [H][N] matrix, kept reads), as call_pairhmm returns
+    them; with n_alleles (the sites given at submit) also the per-site
+    [n_alleles[s]][N] marginals (kept reads' columns first, the rest NaN)."""
+    out = np.empty(max(n_haps * n_reads, 1), np.float64)
+    kept = np.zeros(max(n_reads, 1), np.uint32)
+    nk = ctypes.c_uint32(0)
+    marg = None
+    if n_alleles is not None:
+        marg = np.empty(max(int(np.sum(n_alleles)) * n_reads, 1), np.float64)
+    rc = lib().rvc_phmm_call_pairhmm_wait(ticket, _ptr(out, _f64p), _ptr(kept, _u32p), ctypes.byref(nk),
+                                          _ptr(marg, _f64p) if marg is not None else None, timeout_ms)
+    res = (rc, out[:n_haps * n_reads].reshape(n_haps, n_reads), kept[:nk.value].copy())
+    if n_alleles is None:
+        return res
+    per, o = [], 0
+    for A in n_alleles:
+        per.append(marg[o:o + int(A) * n_reads].reshape(int(A), n_reads))
+        o += int(A) * n_reads
+    return res + (per,)
+
+
+def poll(ticket):
+    """1 if the ticket is done, 0 if not yet, negative for an unknown ticket."""
+    return lib().rvc_phmm_poll(ticket)
 
 
 class Batch:

@@ -9,6 +9,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/rovaca_phmm.h"
@@ -123,6 +124,98 @@ int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags, cons
 }
 }  // namespace rvc
 
+namespace {
+
+// A region-level call after read preprocessing, ready to be computed: the
+// region as the engine takes it, with the host-preprocessed arrays it points
+// into (host path) or the device-preprocessing arguments (device path).
+struct PreparedCall {
+  uint32_t H = 0, N = 0;
+  std::vector<int32_t> rlen;
+  std::vector<uint8_t> q, gop, gcp;
+  rvc_region reg{};
+  DevicePrep prep{};
+  bool on_device = false;
+  const DevicePrep* prep_ptr() const { return on_device ? &prep : nullptr; }
+};
+
+int prepare_call(const rvc_raw_region* rg, int32_t min_quality_threshold, int32_t pcr_option, PreparedCall* pc) {
+  const uint32_t H = rg->n_haps, N = rg->n_reads;
+  pc->H = H;
+  pc->N = N;
+  if (H == 0 || N == 0 || !rg->hap_off || !rg->read_off || !rg->read_qual || !rg->read_mq)
+    return set_error(RVC_E_INVALID, "empty or incomplete region");
+  if (pcr_option < RVC_PCR_NONE || pcr_option > RVC_PCR_CONSERVATIVE)
+    return set_error(RVC_E_INVALID, "unknown PCR indel model %d", pcr_option);
+  pc->rlen.resize(N);
+  for (uint32_t r = 0; r < N; ++r) {
+    const int32_t len = rg->read_off[r + 1] - rg->read_off[r];
+    if (len < 0) return set_error(RVC_E_INVALID, "read %u has negative length", r);
+    pc->rlen[r] = len;
+  }
+  // read preprocessing (MQ cap, quality floor, GOP 45 / PCR model, GCP 10):
+  // on the device, where the region carries the raw qualities, or here on
+  // the calling thread. By default the device takes it when the PCR model
+  // is on (its repeat search is the costly part); without it the host steps
+  // are a few operations per base.
+  const bool on_host = g_engine.host_prep == 1 || (g_engine.host_prep == 0 && pcr_option == RVC_PCR_NONE);
+  rvc_region& reg = pc->reg;
+  reg.n_haps = H;
+  reg.n_reads = N;
+  reg.hap_off = rg->hap_off;
+  reg.hap_bases = rg->hap_bases;
+  reg.read_off = rg->read_off;
+  reg.read_bases = rg->read_bases;
+  if (on_host) {
+    const int32_t total = rg->read_off[N];
+    pc->q.resize(total);
+    pc->gop.resize(total);
+    pc->gcp.resize(total);
+    for (uint32_t r = 0; r < N; ++r) {
+      const int32_t o = rg->read_off[r];
+      preprocess_read(rg->read_bases + o, rg->read_qual + o, pc->rlen[r], rg->read_mq[r], min_quality_threshold,
+                      pcr_option, pc->q.data() + o, pc->gop.data() + o, pc->gcp.data() + o);
+    }
+    reg.q = pc->q.data();
+    reg.i = pc->gop.data();
+    reg.d = pc->gop.data();
+    reg.c = pc->gcp.data();
+    pc->on_device = false;
+  } else {
+    reg.q = rg->read_qual;  // raw qualities; the device writes q, i, d, c
+    reg.i = rg->read_qual;  // (not read)
+    reg.d = rg->read_qual;
+    reg.c = rg->read_qual;
+    pc->prep.mq = rg->read_mq;
+    pc->prep.min_q = min_quality_threshold;
+    pc->prep.pcr = pcr_option;
+    pc->on_device = true;
+  }
+  return RVC_OK;
+}
+
+// The reference's error path (ph/pairhmm_internal.cpp:585-588, 605-609):
+// every entry NaN and every read kept.
+int fail_call(uint32_t H, uint32_t N, double* out_hap_major, uint32_t* kept_reads, uint32_t* n_kept, int code) {
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  for (size_t x = 0; x < (size_t)H * N; ++x) out_hap_major[x] = nan;
+  for (uint32_t r = 0; r < N; ++r) kept_reads[r] = r;
+  *n_kept = N;
+  return code;
+}
+
+// Region-level calls submitted and not yet waited for: what post-processing
+// needs at wait (the reads' lengths), by ticket.
+struct PendingCall {
+  uint32_t H = 0, N = 0;
+  std::vector<int32_t> rlen;
+  std::vector<uint32_t> site_alleles;  // alleles per site, when the call marginalises
+};
+std::mutex g_calls_mu;
+std::unordered_map<rvc_ticket, PendingCall> g_calls;
+
+}  // namespace
+
 extern "C" {
 
 int rvc_phmm_init(const rvc_config* cfg) {
@@ -148,6 +241,10 @@ int rvc_phmm_init(const rvc_config* cfg) {
 int rvc_phmm_shutdown(void) {
   return rvc::guarded([&]() -> int {
     server_stop();
+    {
+      std::lock_guard<std::mutex> lk(g_calls_mu);
+      g_calls.clear();  // their tickets died with the serving loop
+    }
     noalloc_shutdown();
     release_workspaces();
     sw_server_stop();
@@ -168,72 +265,88 @@ int rvc_phmm_call_pairhmm(const rvc_raw_region* rg, int32_t min_quality_threshol
                           double* out_hap_major, uint32_t* kept_reads, uint32_t* n_kept) {
   return rvc::guarded([&]() -> int {
     if (!rg || !out_hap_major || !kept_reads || !n_kept) return set_error(RVC_E_INVALID, "null argument");
-    const double nan = std::numeric_limits<double>::quiet_NaN();
-    const uint32_t H = rg->n_haps, N = rg->n_reads;
-    auto fail = [&](int code) {
-      for (size_t x = 0; x < (size_t)H * N; ++x) out_hap_major[x] = nan;
-      for (uint32_t r = 0; r < N; ++r) kept_reads[r] = r;
-      *n_kept = N;
-      return code;
-    };
-    if (H == 0 || N == 0 || !rg->hap_off || !rg->read_off || !rg->read_qual || !rg->read_mq)
-      return fail(set_error(RVC_E_INVALID, "empty or incomplete region"));
-    if (pcr_option < RVC_PCR_NONE || pcr_option > RVC_PCR_CONSERVATIVE)
-      return fail(set_error(RVC_E_INVALID, "unknown PCR indel model %d", pcr_option));
-    std::vector<int32_t> rlen(N);
-    for (uint32_t r = 0; r < N; ++r) {
-      const int32_t len = rg->read_off[r + 1] - rg->read_off[r];
-      if (len < 0) return fail(set_error(RVC_E_INVALID, "read %u has negative length", r));
-      rlen[r] = len;
-    }
-    // read preprocessing (MQ cap, quality floor, GOP 45 / PCR model, GCP 10):
-    // on the device, where the region carries the raw qualities, or here on
-    // the calling thread. By default the device takes it when the PCR model
-    // is on (its repeat search is the costly part); without it the host steps
-    // are a few operations per base.
-    const bool on_host = g_engine.host_prep == 1 || (g_engine.host_prep == 0 && pcr_option == RVC_PCR_NONE);
-    std::vector<uint8_t> q, gop, gcp;
-    if (on_host) {
-      const int32_t total = rg->read_off[N];
-      q.resize(total);
-      gop.resize(total);
-      gcp.resize(total);
-      for (uint32_t r = 0; r < N; ++r) {
-        const int32_t o = rg->read_off[r];
-        preprocess_read(rg->read_bases + o, rg->read_qual + o, rlen[r], rg->read_mq[r], min_quality_threshold,
-                        pcr_option, q.data() + o, gop.data() + o, gcp.data() + o);
-      }
-    }
-    rvc_region reg;
-    reg.n_haps = H;
-    reg.n_reads = N;
-    reg.hap_off = rg->hap_off;
-    reg.hap_bases = rg->hap_bases;
-    reg.read_off = rg->read_off;
-    reg.read_bases = rg->read_bases;
-    reg.q = rg->read_qual;
-    reg.i = rg->read_qual;  // not read
-    reg.d = rg->read_qual;
-    reg.c = rg->read_qual;
-    DevicePrep prep;
-    prep.mq = rg->read_mq;
-    prep.min_q = min_quality_threshold;
-    prep.pcr = pcr_option;
-    const DevicePrep* pp = &prep;
-    if (on_host) {
-      reg.q = q.data();
-      reg.i = gop.data();
-      reg.d = gop.data();
-      reg.c = gcp.data();
-      pp = nullptr;
-    }
+    PreparedCall pc;
+    int rc = prepare_call(rg, min_quality_threshold, pcr_option, &pc);
+    if (rc) return fail_call(rg->n_haps, rg->n_reads, out_hap_major, kept_reads, n_kept, rc);
     // with the serving loop running, concurrent callers (Rovaca's worker
     // threads) are batched together; otherwise the region runs on its own
-    std::vector<double> read_major((size_t)H * N);
-    int rc = server_running() ? served_region(&reg, read_major.data(), nullptr, pp)
-                              : region_sync_impl(&reg, read_major.data(), nullptr, pp);
-    if (rc) return fail(rc);
-    *n_kept = (uint32_t)postprocess(N, H, read_major.data(), rlen.data(), out_hap_major, kept_reads);
+    std::vector<double> read_major((size_t)pc.H * pc.N);
+    rc = server_running() ? served_region(&pc.reg, read_major.data(), nullptr, pc.prep_ptr())
+                          : region_sync_impl(&pc.reg, read_major.data(), nullptr, pc.prep_ptr());
+    if (rc) return fail_call(pc.H, pc.N, out_hap_major, kept_reads, n_kept, rc);
+    *n_kept = (uint32_t)postprocess(pc.N, pc.H, read_major.data(), pc.rlen.data(), out_hap_major, kept_reads);
+    return RVC_OK;
+  });
+}
+
+int rvc_phmm_call_pairhmm_submit(const rvc_raw_region* rg, int32_t min_quality_threshold, int32_t pcr_option,
+                                 const rvc_sites* sites, rvc_ticket* ticket) {
+  return rvc::guarded([&]() -> int {
+    if (!rg || !ticket) return set_error(RVC_E_INVALID, "null argument");
+    if (!server_running()) return set_error(RVC_E_STATE, "serving loop is not running");
+    auto pc = std::make_unique<PreparedCall>();
+    int rc = prepare_call(rg, min_quality_threshold, pcr_option, pc.get());
+    if (rc) return rc;
+    // the region is packed (copied) by submit: the caller's buffers and the
+    // host-preprocessed arrays are free once it returns
+    rvc_ticket t = 0;
+    if ((rc = submit_impl(&pc->reg, pc->prep_ptr(), sites, &t))) return rc;
+    PendingCall p;
+    p.H = pc->H;
+    p.N = pc->N;
+    if (sites) p.site_alleles.assign(sites->n_alleles, sites->n_alleles + sites->n_sites);
+    p.rlen = std::move(pc->rlen);
+    std::lock_guard<std::mutex> lk(g_calls_mu);
+    g_calls[t] = std::move(p);
+    *ticket = t;
+    return RVC_OK;
+  });
+}
+
+int rvc_phmm_call_pairhmm_wait(rvc_ticket ticket, double* out_hap_major, uint32_t* kept_reads, uint32_t* n_kept,
+                               double* out_alleles, int64_t timeout_ms) {
+  return rvc::guarded([&]() -> int {
+    if (!out_hap_major || !kept_reads || !n_kept) return set_error(RVC_E_INVALID, "null argument");
+    PendingCall p;
+    {
+      std::lock_guard<std::mutex> lk(g_calls_mu);
+      auto it = g_calls.find(ticket);
+      if (it == g_calls.end())
+        return set_error(RVC_E_INVALID, "unknown region-call ticket %llu", (unsigned long long)ticket);
+      p.H = it->second.H;
+      p.N = it->second.N;
+    }
+    std::vector<double> read_major((size_t)p.H * p.N), marg;
+    const int rc = wait_impl(ticket, read_major.data(), nullptr, timeout_ms, nullptr, &marg);
+    if (rc == RVC_E_TIMEOUT) return rc;  // the ticket stays valid: wait again
+    {
+      std::lock_guard<std::mutex> lk(g_calls_mu);
+      auto it = g_calls.find(ticket);
+      if (it == g_calls.end())  // another thread waited for the same ticket
+        return set_error(RVC_E_INVALID, "region-call ticket %llu already waited for", (unsigned long long)ticket);
+      p = std::move(it->second);
+      g_calls.erase(it);
+    }
+    size_t n_marg = 0;
+    for (uint32_t al : p.site_alleles) n_marg += (size_t)al * p.N;
+    if (rc) {
+      if (out_alleles)
+        for (size_t x = 0; x < n_marg; ++x) out_alleles[x] = std::numeric_limits<double>::quiet_NaN();
+      return fail_call(p.H, p.N, out_hap_major, kept_reads, n_kept, rc);
+    }
+    *n_kept = (uint32_t)postprocess(p.N, p.H, read_major.data(), p.rlen.data(), out_hap_major, kept_reads);
+    if (out_alleles && n_marg) {
+      // the kept reads' columns, packed to the front like the matrix's
+      const double nan = std::numeric_limits<double>::quiet_NaN();
+      size_t row = 0;
+      for (uint32_t al : p.site_alleles)
+        for (uint32_t a = 0; a < al; ++a, ++row) {
+          const double* src = marg.data() + row * p.N;
+          double* dst = out_alleles + row * p.N;
+          for (uint32_t k = 0; k < *n_kept; ++k) dst[k] = src[kept_reads[k]];
+          for (uint32_t k = *n_kept; k < p.N; ++k) dst[k] = nan;
+        }
+    }
     return RVC_OK;
   });
 }

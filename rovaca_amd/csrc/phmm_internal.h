@@ -139,7 +139,33 @@ struct PrepArgs {
 };
 constexpr uint32_t kPrepOn = 1u << 31;
 
+// Haplotype -> allele marginalisation of one region's read-major results on
+// the device (allele_likelihoods.hpp:189-217, 701-732, on the matrix that the
+// region call normalises, ph/pairhmm_internal.cpp:248-258): per read and site,
+// the largest normalised likelihood among the haplotypes mapped to each
+// allele, -inf where no haplotype maps to it.
+struct MargRegion {
+  uint64_t pair0;     // first result slot of the region (device read order)
+  uint32_t n_haps, n_reads;
+  uint32_t read0;     // the region's entries of read_perm (device slot -> input read)
+  uint32_t n_sites;
+  uint32_t site0;     // first site of the region in n_alleles / allele0 / map0
+  uint32_t pad;
+};
+struct MargArgs {
+  const double* results;     // the batch's read-major results
+  const MargRegion* regions;
+  const uint32_t* read_perm;
+  const uint32_t* n_alleles;  // per site
+  const uint64_t* allele0;    // per site: first output row (rows of n_reads values, input read order)
+  const uint64_t* map0;       // per site: first entry of its haplotype -> allele map in 'map'
+  const int16_t* map;         // allele of each haplotype at the site, -1 = none
+  double* out;
+  uint32_t n_regions;
+};
+
 // Kernel launchers (phmm_kernels.hip).
+hipError_t launch_marginalize(const MargArgs& a, hipStream_t stream);
 hipError_t launch_prep(const PrepArgs& a, hipStream_t stream);
 bool fp32_supported(int W, int RPL);
 // scaled: the 5-operation scaled form (else plain); pt: the prior-table kernel

@@ -805,6 +805,39 @@ __global__ __launch_bounds__(64) void k_prep(PrepArgs a) {
   }
 }
 
+// One block per region, a thread per read: normalise the read's row as
+// ph/pairhmm_internal.cpp:248-258 does (best found as std::max_element finds
+// it, so a leading NaN leaves the row as it is; values below best - 4.5 are
+// raised to it), then for every site and allele the largest of the values of
+// the haplotypes mapped to it, starting from -inf, with a strict compare so
+// that a NaN never wins (allele_likelihoods.hpp:713-729).
+__global__ __launch_bounds__(256) void k_marginalize(MargArgs a) {
+  const MargRegion g = a.regions[blockIdx.x];
+  const uint32_t H = g.n_haps;
+  for (uint32_t sx = threadIdx.x; sx < g.n_reads; sx += blockDim.x) {
+    const double* row = a.results + g.pair0 + (uint64_t)sx * H;
+    double best = row[0];
+    for (uint32_t h = 1; h < H; ++h)
+      if (best < row[h]) best = row[h];
+    const double floor_v = best + (-4.5);
+    const uint32_t x = a.read_perm[g.read0 + sx];
+    for (uint32_t s = g.site0; s < g.site0 + g.n_sites; ++s) {
+      const int16_t* m = a.map + a.map0[s];
+      const uint32_t A = a.n_alleles[s];
+      for (uint32_t al = 0; al < A; ++al) {
+        double v = -__builtin_huge_val();
+        for (uint32_t h = 0; h < H; ++h) {
+          if (m[h] != (int16_t)al) continue;
+          double w = row[h];
+          w = w < floor_v ? floor_v : w;
+          if (w > v) v = w;
+        }
+        a.out[a.allele0[s] + (uint64_t)al * g.n_reads + x] = v;
+      }
+    }
+  }
+}
+
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -918,6 +951,12 @@ hipError_t launch_prep(const PrepArgs& a, hipStream_t stream) {
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_marginalize(const MargArgs& a, hipStream_t stream) {
+  if (a.n_regions == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_marginalize, dim3(a.n_regions), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 

@@ -11,8 +11,9 @@
 // copy, the FP32 pass, the FP64 rescue and the copy of the results on its
 // stream. Slots work concurrently, so requests that arrive while earlier
 // launches are on the device are staged and started while those finish.
-// Results land in pinned buffers that the finished requests share; each waiter
-// copies its own region out.
+// Results land in the slot's pinned result buffer; when the launch finishes,
+// the slot thread copies each request's region out (in input read order) and
+// the buffer goes back to the pool, so a waiter that comes late holds none.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -68,9 +69,8 @@ struct ResultBuf {
   }
 };
 
-// Result buffers are recycled: a finished launch hands its buffer to the
-// requests it served, and the buffer comes back when the last of them has
-// been waited for (or dropped at shutdown).
+// Result buffers are recycled: a slot takes one for a launch and gives it
+// back once its requests have copied their regions out.
 struct ResultPool : std::enable_shared_from_this<ResultPool> {
   std::mutex mu;
   std::vector<std::unique_ptr<ResultBuf>> free;
@@ -120,20 +120,21 @@ struct ResultPool : std::enable_shared_from_this<ResultPool> {
   }
 };
 
-// Results of one finished launch, shared by its requests.
-struct BatchResult {
-  std::shared_ptr<ResultBuf> buf;
-  std::vector<RegionLayout> region;
-  std::vector<uint32_t> read_perm;
-};
-
 struct Request {
   uint64_t id = 0;
   HostPack pack;  // the region, packed by the submitting thread
   uint64_t resident = 0;
   uint32_t max_hap = 0;
-  std::shared_ptr<const BatchResult> result;
-  uint32_t result_region = 0;
+  // the region's read-major results and flags in input read order, copied
+  // out of the launch's pinned buffer by the slot thread when it finishes
+  std::vector<double> out;
+  std::vector<uint8_t> flags;
+  // sites to marginalise on the device (optional): alleles per site, the
+  // allele of each haplotype per site (-1 = none), and the marginals
+  // ([site][allele][read], input read order) copied out like the results
+  std::vector<uint32_t> site_alleles;
+  std::vector<int16_t> site_map;
+  std::vector<double> marg;
   int status = 0;
   bool done = false;
   uint64_t t_submit = 0, t_launch = 0, t_done = 0;
@@ -156,7 +157,14 @@ struct Slot {
   std::shared_ptr<ResultBuf> res;
   std::vector<Request*> reqs;
   uint64_t resident = 0;
+  // marginalisation of the requests that carry sites: pinned inputs
+  // (regions, read order, site tables) and outputs, device offsets
+  PinnedBuf marg_in, marg_out;
+  size_t marg_out_bytes = 0;
+  std::vector<uint64_t> marg_base;  // per request: first output value
 };
+
+size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Device bytes a packed region holds while in flight: its share of the
 // layout DeviceBatch::upload_staged builds (sequences, tiles, results, rescue
@@ -206,10 +214,30 @@ int launch_slot(Slot& s) {
   g_srv->stage_ns += now_ns() - t0;
   if (s.staging.cap != staging_cap) ++g_srv->grows;
   for (Request* r : s.reqs) r->pack = HostPack();  // the staged image is all the launch needs
-  if (!s.mem || s.bytes < L.bytes) {
+  // marginalisation inputs of the requests with sites, after the batch in
+  // the arena: region records, the batch's read order, per-site tables
+  uint32_t n_mreg = 0, n_sites = 0;
+  size_t map_entries = 0, marg_values = 0;
+  s.marg_base.assign(s.reqs.size(), 0);
+  for (uint32_t x = 0; x < (uint32_t)s.reqs.size(); ++x) {
+    const Request* r = s.reqs[x];
+    if (r->site_alleles.empty()) continue;
+    ++n_mreg;
+    n_sites += (uint32_t)r->site_alleles.size();
+    map_entries += r->site_map.size();
+    s.marg_base[x] = marg_values;
+    for (uint32_t al : r->site_alleles) marg_values += (size_t)al * s.meta.region[x].n_reads;
+  }
+  const size_t n_reads_all = s.meta.read_perm.size();
+  const size_t o_reg = 0, o_perm = up256(sizeof(MargRegion) * n_mreg), o_nal = o_perm + up256(4 * n_reads_all),
+               o_al0 = o_nal + up256(4ull * n_sites), o_map0 = o_al0 + up256(8ull * n_sites),
+               o_map = o_map0 + up256(8ull * n_sites), marg_in_bytes = o_map + up256(2 * map_entries + 2);
+  const size_t marg_dev = n_mreg ? up256(L.bytes) : 0;
+  const size_t need = n_mreg ? marg_dev + marg_in_bytes + 8 * marg_values : L.bytes;
+  if (!s.mem || s.bytes < need) {
     // grow by doubling; the old arena is freed at stop (freeing device
     // memory waits for the whole device; its last launch has finished)
-    const size_t want = std::max(L.bytes + L.bytes / 4 + (16u << 20), 2 * s.bytes);
+    const size_t want = std::max(need + need / 4 + (16u << 20), 2 * s.bytes);
     if (s.mem) s.retired.push_back(s.mem);
     s.mem = nullptr;
     s.bytes = 0;
@@ -228,6 +256,59 @@ int launch_slot(Slot& s) {
   if (grew) ++g_srv->grows;
   if ((rc = s.batch.ensure_events())) return rc;
   if ((rc = s.batch.enqueue(s.stream, true))) return rc;
+  s.marg_out_bytes = 8 * marg_values;
+  if (n_mreg) {
+    const size_t cap_in = s.marg_in.cap, cap_out = s.marg_out.cap;
+    if ((rc = s.marg_in.reserve(marg_in_bytes)) || (rc = s.marg_out.reserve(8 * marg_values + 8))) return rc;
+    if (s.marg_in.cap != cap_in) ++g_srv->grows;
+    if (s.marg_out.cap != cap_out) ++g_srv->grows;
+    char* h = s.marg_in.ptr;
+    char* d = static_cast<char*>(s.mem) + marg_dev;
+    MargRegion* mr = reinterpret_cast<MargRegion*>(h + o_reg);
+    uint32_t* nal = reinterpret_cast<uint32_t*>(h + o_nal);
+    uint64_t* al0 = reinterpret_cast<uint64_t*>(h + o_al0);
+    uint64_t* mp0 = reinterpret_cast<uint64_t*>(h + o_map0);
+    int16_t* mp = reinterpret_cast<int16_t*>(h + o_map);
+    std::memcpy(h + o_perm, s.meta.read_perm.data(), 4 * n_reads_all);
+    uint32_t k = 0, site = 0;
+    uint64_t moff = 0;
+    for (uint32_t x = 0; x < (uint32_t)s.reqs.size(); ++x) {
+      const Request* r = s.reqs[x];
+      if (r->site_alleles.empty()) continue;
+      const RegionLayout& G = s.meta.region[x];
+      MargRegion& g = mr[k++];
+      g.pair0 = G.pair0;
+      g.n_haps = G.n_haps;
+      g.n_reads = G.n_reads;
+      g.read0 = G.read0;
+      g.n_sites = (uint32_t)r->site_alleles.size();
+      g.site0 = site;
+      g.pad = 0;
+      uint64_t vo = s.marg_base[x];
+      for (uint32_t q = 0; q < g.n_sites; ++q, ++site) {
+        nal[site] = r->site_alleles[q];
+        al0[site] = vo;
+        vo += (uint64_t)r->site_alleles[q] * G.n_reads;
+        mp0[site] = moff + (uint64_t)q * G.n_haps;
+      }
+      std::memcpy(mp + moff, r->site_map.data(), 2 * r->site_map.size());
+      moff += r->site_map.size();
+    }
+    MargArgs ma;
+    ma.results = s.batch.out;
+    ma.regions = reinterpret_cast<const MargRegion*>(d + o_reg);
+    ma.read_perm = reinterpret_cast<const uint32_t*>(d + o_perm);
+    ma.n_alleles = reinterpret_cast<const uint32_t*>(d + o_nal);
+    ma.allele0 = reinterpret_cast<const uint64_t*>(d + o_al0);
+    ma.map0 = reinterpret_cast<const uint64_t*>(d + o_map0);
+    ma.map = reinterpret_cast<const int16_t*>(d + o_map);
+    ma.out = reinterpret_cast<double*>(d + marg_in_bytes);
+    ma.n_regions = n_mreg;
+    if (hipMemcpyAsync(d, h, marg_in_bytes, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
+        launch_marginalize(ma, s.stream) != hipSuccess ||
+        hipMemcpyAsync(s.marg_out.ptr, ma.out, 8 * marg_values, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+      return set_error(RVC_E_DEVICE, "marginalisation launch");
+  }
   if (hipMemcpyAsync(s.res->out(), s.batch.out, n * (sizeof(double) + 1), hipMemcpyDeviceToHost, s.stream) !=
           hipSuccess ||
       hipMemcpyAsync(s.res->rescued(), s.batch.counters + kFp32Bins + kNumFp64Lists,
@@ -281,27 +362,47 @@ void slot_main(Slot* sp) {
     } catch (...) {
       rc = set_error(RVC_E_INVALID, "out of host memory for a serving launch");
     }
-    std::shared_ptr<BatchResult> res;
     float ms = 0;
     uint64_t resc = 0;
     if (rc == RVC_OK) {
-      res = std::make_shared<BatchResult>();
-      res->buf = std::move(s.res);
-      res->region = std::move(s.meta.region);
-      res->read_perm = std::move(s.meta.read_perm);
-      if (hipEventElapsedTime(&ms, s.batch.ev[0], s.batch.ev[2]) != hipSuccess) ms = 0;
-      for (int b = 0; b < kNumFp64Lists; ++b) resc += res->buf->rescued()[b];
+      // each request takes its own copy (input read order), so the launch's
+      // pinned buffer goes back to the pool now rather than when the last
+      // waiter has come: a slow waiter holds no launch buffer
+      try {
+        const ResultBuf& B = *s.res;
+        for (uint32_t r = 0; r < (uint32_t)s.reqs.size(); ++r) {
+          Request* rq = s.reqs[r];
+          const RegionLayout& L = s.meta.region[r];
+          rq->out.resize((size_t)L.n_reads * L.n_haps);
+          rq->flags.resize((size_t)L.n_reads * L.n_haps);
+          for (uint32_t sx = 0; sx < L.n_reads; ++sx) {
+            const uint32_t x = s.meta.read_perm[L.read0 + sx];
+            const size_t src = L.pair0 + (size_t)sx * L.n_haps, dst = (size_t)x * L.n_haps;
+            std::memcpy(rq->out.data() + dst, B.out() + src, L.n_haps * sizeof(double));
+            std::memcpy(rq->flags.data() + dst, B.flags() + src, L.n_haps);
+          }
+          if (!rq->site_alleles.empty()) {
+            size_t nv = 0;
+            for (uint32_t al : rq->site_alleles) nv += (size_t)al * L.n_reads;
+            rq->marg.resize(nv);
+            std::memcpy(rq->marg.data(), reinterpret_cast<const double*>(s.marg_out.ptr) + s.marg_base[r],
+                        nv * sizeof(double));
+          }
+        }
+        if (hipEventElapsedTime(&ms, s.batch.ev[0], s.batch.ev[2]) != hipSuccess) ms = 0;
+        for (int b = 0; b < kNumFp64Lists; ++b) resc += B.rescued()[b];
+      } catch (...) {
+        rc = set_error(RVC_E_INVALID, "out of host memory for a serving launch's results");
+      }
     } else {
       hipStreamSynchronize(s.stream);
-      s.res.reset();
     }
+    s.res.reset();
     const uint64_t cells = s.meta.n_cells;
     lk.lock();
     const uint64_t t_done = now_ns();
     for (uint32_t r = 0; r < (uint32_t)s.reqs.size(); ++r) {
       Request* rq = s.reqs[r];
-      rq->result = res;
-      rq->result_region = r;
       rq->status = rc;
       rq->t_done = t_done;
       rq->done = true;
@@ -345,8 +446,7 @@ int server_start() {
     if (hipMalloc(&s.mem, arena) == hipSuccess) s.bytes = arena;
     else s.mem = nullptr;
     s.staging.reserve(128u << 20);
-    // result buffers for launches of about 500 typical regions
-    S->pool->add(2u << 20);
+    // a result buffer per slot for launches of about 500 typical regions
     S->pool->add(2u << 20);
   }
   g_srv = S;
@@ -385,13 +485,30 @@ void server_stop() {
 
 bool server_running() { return g_srv != nullptr; }
 
-int submit_impl(const rvc_region* region, const DevicePrep* prep, rvc_ticket* ticket) {
+int submit_impl(const rvc_region* region, const DevicePrep* prep, const rvc_sites* sites, rvc_ticket* ticket) {
   if (!ticket) return set_error(RVC_E_INVALID, "null ticket");
   if (!g_engine.ready || !g_srv) return set_error(RVC_E_STATE, "serving loop is not running");
   int rc = validate_region(region);
   if (rc) return rc;
+  if (sites && sites->n_sites) {
+    if (!sites->n_alleles || !sites->hap_allele) return set_error(RVC_E_INVALID, "null site table");
+    for (uint32_t s = 0; s < sites->n_sites; ++s) {
+      const uint32_t A = sites->n_alleles[s];
+      if (A == 0 || A > 32767) return set_error(RVC_E_INVALID, "site %u has %u alleles", s, A);
+      for (uint32_t h = 0; h < region->n_haps; ++h) {
+        const int32_t v = sites->hap_allele[(size_t)s * region->n_haps + h];
+        if (v < -1 || v >= (int32_t)A)
+          return set_error(RVC_E_INVALID, "site %u maps haplotype %u to allele %d of %u", s, h, v, A);
+      }
+    }
+  }
   auto* r = new (std::nothrow) Request;
   if (!r) return set_error(RVC_E_INVALID, "out of host memory");
+  if (sites && sites->n_sites) {
+    r->site_alleles.assign(sites->n_alleles, sites->n_alleles + sites->n_sites);
+    r->site_map.resize((size_t)sites->n_sites * region->n_haps);
+    for (size_t x = 0; x < r->site_map.size(); ++x) r->site_map[x] = (int16_t)sites->hap_allele[x];
+  }
   // with nothing else in flight the region runs alone: shapes for latency
   const bool alone = g_srv->in_flight.load(std::memory_order_relaxed) == 0;
   const uint64_t pairs = (uint64_t)region->n_haps * region->n_reads;
@@ -401,6 +518,7 @@ int submit_impl(const rvc_region* region, const DevicePrep* prep, rvc_ticket* ti
     return rc;
   }
   r->resident = footprint(r->pack);
+  for (uint32_t al : r->site_alleles) r->resident += 8ull * al * region->n_reads + 8 * 3 + 2ull * region->n_haps;
   for (uint32_t h : r->pack.hap_len) r->max_hap = std::max(r->max_hap, h);
   if (r->resident > g_engine.hbm_budget) {
     const uint64_t need = r->resident;
@@ -427,6 +545,37 @@ int submit_impl(const rvc_region* region, const DevicePrep* prep, rvc_ticket* ti
   return RVC_OK;
 }
 
+int wait_impl(rvc_ticket ticket, double* out_read_major, uint8_t* out_flags, int64_t timeout_ms,
+              rvc_ticket_timing* timing, std::vector<double>* marg) {
+  if (!g_srv) return set_error(RVC_E_STATE, "serving loop is not running");
+  Server& S = *g_srv;
+  std::unique_lock<std::mutex> lk(S.mu);
+  auto it = S.tickets.find(ticket);
+  if (it == S.tickets.end()) return set_error(RVC_E_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
+  Request* r = it->second;
+  if (timeout_ms < 0) {
+    S.cv_done.wait(lk, [&] { return r->done; });
+  } else if (!S.cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return r->done; })) {
+    return set_error(RVC_E_TIMEOUT, "ticket %llu not done after %lld ms", (unsigned long long)ticket,
+                     (long long)timeout_ms);
+  }
+  S.tickets.erase(it);
+  lk.unlock();
+  std::unique_ptr<Request> own(r);
+  const int st = r->status;
+  if (timing) {
+    timing->submit_ns = r->t_submit;
+    timing->launch_ns = r->t_launch;
+    timing->done_ns = r->t_done;
+  }
+  if (st == RVC_OK) {
+    if (out_read_major) std::memcpy(out_read_major, r->out.data(), r->out.size() * sizeof(double));
+    if (out_flags) std::memcpy(out_flags, r->flags.data(), r->flags.size());
+    if (marg) marg->swap(r->marg);
+  }
+  return st == RVC_OK ? RVC_OK : set_error(st, "request failed with code %d", st);
+}
+
 }  // namespace rvc
 
 using namespace rvc;
@@ -434,47 +583,12 @@ using namespace rvc;
 extern "C" {
 
 int rvc_phmm_submit(const rvc_region* region, rvc_ticket* ticket) {
-  return rvc::guarded([&]() -> int { return submit_impl(region, nullptr, ticket); });
+  return rvc::guarded([&]() -> int { return submit_impl(region, nullptr, nullptr, ticket); });
 }
 
 int rvc_phmm_wait(rvc_ticket ticket, double* out_read_major, uint8_t* out_flags, int64_t timeout_ms,
                   rvc_ticket_timing* timing) {
-  return rvc::guarded([&]() -> int {
-    if (!g_srv) return set_error(RVC_E_STATE, "serving loop is not running");
-    Server& S = *g_srv;
-    std::unique_lock<std::mutex> lk(S.mu);
-    auto it = S.tickets.find(ticket);
-    if (it == S.tickets.end()) return set_error(RVC_E_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
-    Request* r = it->second;
-    if (timeout_ms < 0) {
-      S.cv_done.wait(lk, [&] { return r->done; });
-    } else if (!S.cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return r->done; })) {
-      return set_error(RVC_E_TIMEOUT, "ticket %llu not done after %lld ms", (unsigned long long)ticket,
-                       (long long)timeout_ms);
-    }
-    S.tickets.erase(it);
-    lk.unlock();
-    const int st = r->status;
-    if (timing) {
-      timing->submit_ns = r->t_submit;
-      timing->launch_ns = r->t_launch;
-      timing->done_ns = r->t_done;
-    }
-    if (st == RVC_OK && (out_read_major || out_flags)) {
-      const BatchResult& B = *r->result;
-      const RegionLayout& L = B.region[r->result_region];
-      const double* bo = B.buf->out();
-      const uint8_t* bf = B.buf->flags();
-      for (uint32_t sx = 0; sx < L.n_reads; ++sx) {
-        const uint32_t x = B.read_perm[L.read0 + sx];
-        const size_t src = L.pair0 + (size_t)sx * L.n_haps, dst = (size_t)x * L.n_haps;
-        if (out_read_major) std::memcpy(out_read_major + dst, bo + src, L.n_haps * sizeof(double));
-        if (out_flags) std::memcpy(out_flags + dst, bf + src, L.n_haps);
-      }
-    }
-    delete r;
-    return st == RVC_OK ? RVC_OK : set_error(st, "request failed with code %d", st);
-  });
+  return rvc::guarded([&]() -> int { return wait_impl(ticket, out_read_major, out_flags, timeout_ms, timing, nullptr); });
 }
 
 int rvc_phmm_poll(rvc_ticket ticket) {
@@ -511,7 +625,7 @@ int rvc_phmm_server_stats(rvc_server_stats* st) {
 namespace rvc {
 int served_region(const rvc_region* region, double* out, uint8_t* flags, const DevicePrep* prep) {
   rvc_ticket t = 0;
-  const int rc = submit_impl(region, prep, &t);
+  const int rc = submit_impl(region, prep, nullptr, &t);
   if (rc) return rc;
   return rvc_phmm_wait(t, out, flags, -1, nullptr);
 }

@@ -690,3 +690,86 @@ def test_served_poisson_stream_32_producers():
         m = ~np.isnan(want)
         assert np.abs(alone[x][m] - want[m]).max() < TOL
     say(f"stream: p50 {res['lat_p50_ms']:.2f} ms, p99 {res['lat_p99_ms']:.2f} ms")
+
+
+def test_call_pairhmm_submit_wait_equals_call_and_overlaps_cpu_work():
+    # Rovaca's worker blocks on call_pairhmm per region
+    # (hc/haplotypecaller_engine.cpp:149-289); the two-half call lets it go on
+    # with CPU work between submit and wait. Results equal the one-call form.
+    say("call_pairhmm_submit / _wait: equal to call_pairhmm; the worker works while the region is on the device")
+    rng = np.random.default_rng(101)
+    regs = [r for r in workload.batched_regions(16, seed=102) if r["n_haps"] * r["n_reads"] < 6000][:6]
+    calls = []
+    for reg in regs:
+        H, N = reg["n_haps"], reg["n_reads"]
+        haps = [reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]] for h in range(H)]
+        reads = [reg["read_bases"][reg["read_off"][r]:reg["read_off"][r + 1]] for r in range(N)]
+        quals = [reg["q"][reg["read_off"][r]:reg["read_off"][r + 1]] for r in range(N)]
+        calls.append((haps, reads, quals, rng.integers(20, 61, N).astype(np.uint8)))
+    for pcr in (rv.PCR_NONE, rv.PCR_CONSERVATIVE):
+        want = [rv.call_pairhmm(*c, min_quality_threshold=10, pcr=pcr) for c in calls]
+        tickets = []
+        for c in calls:
+            rc, t = rv.call_pairhmm_submit(*c, min_quality_threshold=10, pcr=pcr)
+            assert rc == 0
+            tickets.append(t)
+        for (rc0, m0, k0), t, c in zip(want, tickets, calls):
+            rc1, m1, k1 = rv.call_pairhmm_wait(t, len(c[0]), len(c[1]))
+            assert rc0 == 0 and rc1 == 0 and np.array_equal(k0, k1)
+            assert np.array_equal(np.isnan(m0), np.isnan(m1)) and np.array_equal(m0[~np.isnan(m0)], m1[~np.isnan(m1)])
+    # a deep region: submit returns before the device is done, the worker
+    # does CPU work of its own, then waits
+    deep = workload.deep_region(seed=103)
+    H, N = deep["n_haps"], deep["n_reads"]
+    haps = [deep["hap_bases"][deep["hap_off"][h]:deep["hap_off"][h + 1]] for h in range(H)]
+    reads = [deep["read_bases"][deep["read_off"][r]:deep["read_off"][r + 1]] for r in range(N)]
+    quals = [deep["q"][deep["read_off"][r]:deep["read_off"][r + 1]] for r in range(N)]
+    mqs = np.full(N, 60, np.uint8)
+    rc, t = rv.call_pairhmm_submit(haps, reads, quals, mqs)
+    assert rc == 0
+    assert rv.poll(t) == 0          # 6.4e10 cells: not done the moment submit returns
+    work = sum(int(x) for x in range(200000))  # the worker's own CPU work meanwhile
+    assert work > 0
+    rc1, m1, k1 = rv.call_pairhmm_wait(t, H, N)
+    rc0, m0, k0 = rv.call_pairhmm(haps, reads, quals, mqs)
+    assert rc0 == 0 and rc1 == 0 and np.array_equal(k0, k1)
+    assert np.array_equal(np.isnan(m0), np.isnan(m1)) and np.array_equal(m0[~np.isnan(m0)], m1[~np.isnan(m1)])
+    # a refused region is refused at submit, with nothing queued
+    bad = [h.copy() for h in calls[0][1]]
+    bad[0][0] = ord("R")
+    rc, t = rv.call_pairhmm_submit(calls[0][0], bad, calls[0][2], calls[0][3])
+    assert rc == rv.RVC_E_BASES
+
+
+def test_call_pairhmm_marginalises_sites_on_the_device():
+    # the region call with its variant sites: the device marginalises the
+    # normalised matrix per site (allele_likelihoods.hpp:701-732); the kept
+    # reads' columns equal the oracle's marginalisation of the call's own
+    # matrix, exactly (a max is exact)
+    say("call_pairhmm with sites: haplotype -> allele marginals from the device")
+    rng = np.random.default_rng(111)
+    regs = [r for r in workload.batched_regions(20, seed=112) if r["n_haps"] * r["n_reads"] < 8000][:8]
+    regs.append(workload.single_region(113))
+    pending = []
+    for reg in regs:
+        H, N = reg["n_haps"], reg["n_reads"]
+        haps = [reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]] for h in range(H)]
+        reads = [reg["read_bases"][reg["read_off"][r]:reg["read_off"][r + 1]] for r in range(N)]
+        quals = [reg["q"][reg["read_off"][r]:reg["read_off"][r + 1]] for r in range(N)]
+        mqs = rng.integers(20, 61, N).astype(np.uint8)
+        n_sites = int(rng.integers(1, 6))
+        n_alleles = rng.integers(1, 5, n_sites).astype(np.uint32)
+        hap_allele = np.stack([rng.integers(-1, int(a), H) for a in n_alleles]).astype(np.int32)
+        rc, t = rv.call_pairhmm_submit(haps, reads, quals, mqs, min_quality_threshold=10, sites=(n_alleles, hap_allele))
+        assert rc == 0
+        pending.append((t, H, N, n_alleles, hap_allele, (haps, reads, quals, mqs)))
+    for t, H, N, n_alleles, hap_allele, call in pending:
+        rc, mat, kept, marg = rv.call_pairhmm_wait(t, H, N, n_alleles=n_alleles)
+        assert rc == 0
+        rc0, mat0, kept0 = rv.call_pairhmm(*call, min_quality_threshold=10)
+        assert np.array_equal(kept, kept0)
+        assert np.array_equal(np.isnan(mat), np.isnan(mat0)) and np.array_equal(mat[~np.isnan(mat)], mat0[~np.isnan(mat0)])
+        want = oracle.marginalize(mat, len(kept), n_alleles, hap_allele)
+        for g, w in zip(marg, want):
+            assert np.array_equal(np.isnan(g), np.isnan(w))
+            assert np.array_equal(g[~np.isnan(g)], w[~np.isnan(w)])

@@ -193,3 +193,34 @@ def test_oracle_on_real_regions_against_reference_fixture():
         assert np.abs(got - want).max() <= 1e-5
         assert np.abs(f64 - want).max() <= 1e-5
     assert o == len(ref["ref_intra"])
+
+
+def test_marginalize_closed_form():
+    # allele_likelihoods.hpp:701-732: per read, the largest likelihood of the
+    # haplotypes mapped to the allele, -inf for an allele no haplotype maps
+    # to, a NaN never the largest; unmapped haplotypes skipped; columns past
+    # the evidence count (dropped reads) NaN
+    hm = np.array([[-1.0, -5.0, np.nan], [-2.0, np.nan, -3.0], [-0.5, -7.0, -1.0]])
+    s0, s1 = oracle.marginalize(hm, 2, [2, 3], [[0, 1, 0], [-1, 0, 0]])
+    assert s0[0, :2].tolist() == [-0.5, -5.0] and s0[1, :2].tolist() == [-2.0, -np.inf]
+    assert s1[0, :2].tolist() == [-0.5, -7.0] and np.isneginf(s1[1:, :2]).all()
+    assert np.isnan(s0[:, 2]).all() and np.isnan(s1[:, 2]).all()
+    with pytest.raises(ValueError):
+        oracle.marginalize(hm, 2, [2], [[0, 2, 0]])
+
+
+def test_marginalize_matches_the_reference_tests_property():
+    # the check of AlleleLikelihoodsUnitTest.testMarginalization
+    # (genotype/test/allele_likelihoods_unit_test.cpp:210-244): each new value
+    # equals the max over its old alleles, on random likelihoods
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        H, N = int(rng.integers(1, 12)), int(rng.integers(1, 20))
+        hm = rng.uniform(-30, 0, (H, N))
+        A = int(rng.integers(1, 5))
+        m = rng.integers(-1, A, H)
+        (got,) = oracle.marginalize(hm, N, [A], [m])
+        for a in range(A):
+            old = [h for h in range(H) if m[h] == a]
+            want = hm[old].max(axis=0) if old else np.full(N, -np.inf)
+            assert np.array_equal(got[a], want)
