@@ -145,33 +145,11 @@ def plain_gpu(regions, n):
 
 def run_single(args):
     """One 8 x 32 region through the plain form, repeated: latency and GCUPS."""
-    import rovaca_amd as rv
-    from rovaca_amd import workload
-    reg = workload.single_region(0)
-    cells = workload.cells([reg])
-    rv.init(0)
-    for _ in range(max(1, args.warmup)):
-        rv.region_sync(reg)
-    lat = []
-    for _ in range(max(200, args.steps)):
-        t0 = time.perf_counter()
-        rv.region_sync(reg)
-        lat.append((time.perf_counter() - t0) * 1e3)
-    rv.shutdown()
-    lat = np.array(lat)
-    res = {"metric": "PairHMM single-region latency (8 haplotypes x 32 reads, 1 MI355X)",
-           "value": round(float(np.median(lat)), 4), "unit": "ms", "higher_is_better": False, "n_gpus": 1,
-           "steps": len(lat), "warmup": args.warmup, "dtype": "f32", "rescue_dtype": "f64", "data": "synthetic",
-           "config": {"workload": "single_region_8x32", "cells": cells},
-           "gcups_at_median": round(cells / (np.median(lat) * 1e-3) / 1e9, 2),
-           "p99_ms": round(float(np.percentile(lat, 99)), 4)}
-    if not args.no_cpu:
-        import oracle
-        oracle.baseline_run([reg], 1)
-        secs = min(oracle.baseline_run([reg], 1)[0] for _ in range(5))
-        res["cpu_baseline"] = {"value": round(secs * 1e3, 4), "unit": "ms", "cores": 1, "kind": "port",
-                               "sample": "the same region, one worker thread, best of 5"}
-    return res
+    rec = single_region_config(args.no_cpu, reps=max(200, args.steps))
+    return {"metric": "PairHMM single-region latency (8 haplotypes x 32 reads, 1 MI355X)",
+            "value": rec["median_ms"], "unit": "ms", "higher_is_better": False, "n_gpus": 1,
+            "steps": rec["calls"], "warmup": 20, "dtype": "f32", "rescue_dtype": "f64", "data": "synthetic",
+            "config": {"workload": "single_region_8x32", "cells": rec["cells"]}, **rec}
 
 
 def served_stream(pool, deep, rates, duration, producers, warmup_s=0.3):
@@ -236,13 +214,19 @@ def highest_rate_under(rates, p99_ms=10.0):
 
 def single_region_config(no_cpu, reps=400):
     """The plain single active region (8 haplotypes x 32 reads): per-call
-    latency of the one-region path, and Rovaca's region scheme on one host
-    thread for the same region."""
+    latency of the one-region path as a C++ caller sees it (the calls made
+    from native code, rvc_loadgen_region_latency) and through Python, the
+    entry point that allocates nothing, the region's launch sequence with
+    resident inputs eager and as a hipGraph replay, and Rovaca's region
+    scheme on one host thread for the same region."""
     import rovaca_amd as rv
     from rovaca_amd import workload
     reg = workload.single_region(0)
     cells = workload.cells([reg])
     rv.init(0)
+    rv.region_latency(reg, 50)
+    native = rv.region_latency(reg, reps)
+    native_na = rv.region_latency(reg, reps, noalloc=True)
     for _ in range(20):
         rv.region_sync(reg)
     lat = []
@@ -250,11 +234,29 @@ def single_region_config(no_cpu, reps=400):
         t0 = time.perf_counter()
         rv.region_sync(reg)
         lat.append((time.perf_counter() - t0) * 1e3)
+    # the region's launch sequence with its inputs resident, queued eagerly
+    # and replayed as a hipGraph (rvc_phmm_batch with use_graph)
+    seq = {}
+    for mode in ("eager", "graph"):
+        b = rv.Batch([reg], use_graph=(mode == "graph"), untimed=True)
+        for _ in range(20):
+            b.run()
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            b.run()
+            t.append((time.perf_counter() - t0) * 1e3)
+        seq[mode] = round(float(np.median(t)), 4)
+        b.close()
     rv.shutdown()
     lat = np.array(lat)
     out = {"workload": "single_region_8x32", "cells": cells, "calls": reps,
-           "median_ms": round(float(np.median(lat)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
-           "gcups_at_median": round(cells / (np.median(lat) * 1e-3) / 1e9, 2)}
+           "median_ms": round(float(np.median(native)), 4), "p99_ms": round(float(np.percentile(native, 99)), 4),
+           "gcups_at_median": round(cells / (np.median(native) * 1e-3) / 1e9, 2),
+           "caller": "C++ (rvc_loadgen_region_latency)",
+           "noalloc_median_ms": round(float(np.median(native_na)), 4),
+           "python_median_ms": round(float(np.median(lat)), 4), "python_p99_ms": round(float(np.percentile(lat, 99)), 4),
+           "resident_launch_sequence_ms": {"eager": seq["eager"], "hipgraph_replay": seq["graph"]}}
     if not no_cpu:
         import oracle
         oracle.baseline_run([reg], 1)

@@ -139,9 +139,10 @@ int rvc_phmm_region_sync(const rvc_region* region, double* out_read_major, uint8
  * rvc_phmm_region_sync for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc). It allocates no device or pinned host memory: the region's
- * device image lives in a fixed 32 MiB device array set up by rvc_phmm_init;
- * a region whose image exceeds it gets RVC_E_LIMIT. Calls are serialised
- * (they share the array); each call clears what it reads with kernels. */
+ * device image lives in a fixed 32 MiB device array, and its host image in a
+ * 32 MiB pinned buffer, both set up by rvc_phmm_init; a region whose image
+ * exceeds them gets RVC_E_LIMIT. Calls are serialised (they share the
+ * array); each call uploads its starting results and zero counters. */
 int rvc_phmm_region_sync_noalloc(const rvc_region* region, double* out_read_major, uint8_t* out_flags);
 
 /* Replaces call_roDV_pairhmm_scheduled (ph/pairhmm_internal.cpp:576-623):
@@ -223,6 +224,9 @@ typedef struct rvc_batch_opts {
                                     /* once and sweeps a run of haplotypes. 0 = up to */
                                     /* the tuned maximum, fewer for small batches;    */
                                     /* 1..64 = exactly that many                      */
+  uint32_t untimed;                 /* 1 = eager runs queue the launches without the  */
+                                    /* per-launch events (rvc_phmm_batch_kernels and  */
+                                    /* the FP32/FP64 split are then not measured)     */
 } rvc_batch_opts;
 
 typedef struct rvc_batch_info {

@@ -53,6 +53,12 @@ int rvc_loadgen_call_pairhmm(const rvc_raw_region* regions, uint32_t n_regions, 
                              int32_t min_quality_threshold, int32_t pcr_option, uint32_t n_workers, double* latency_ms,
                              rvc_loadgen_result* result);
 
+/* Latency of the one-region call as a C++ caller sees it: n_calls calls of
+ * rvc_phmm_region_sync (noalloc = 0) or rvc_phmm_region_sync_noalloc
+ * (noalloc = 1) on one region from this thread, one after another;
+ * latency_ms[i] receives call i's duration. */
+int rvc_loadgen_region_latency(const rvc_region* region, uint32_t n_calls, int32_t noalloc, double* latency_ms);
+
 /* One region's realignment call: the arguments of rvc_sw_align_batch. */
 typedef struct rvc_sw_region {
   const uint8_t* hap_bases;

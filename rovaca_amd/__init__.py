@@ -34,7 +34,7 @@ EXPORTS = [
     "rvc_phmm_call_pairhmm", "rvc_phmm_call_pairhmm_submit", "rvc_phmm_call_pairhmm_wait", "rvc_phmm_batch_create", "rvc_phmm_batch_run", "rvc_phmm_batch_results",
     "rvc_phmm_batch_info", "rvc_phmm_batch_destroy", "rvc_phmm_submit", "rvc_phmm_wait", "rvc_phmm_poll",
     "rvc_phmm_server_stats", "rvc_phmm_preprocess_read", "rvc_phmm_postprocess", "rvc_phmm_tables",
-    "rvc_loadgen_run", "rvc_loadgen_call_pairhmm", "rvc_phmm_batch_kernels", "rvc_sw_align_batch", "rvc_sw_align_served", "rvc_loadgen_sw_align",
+    "rvc_loadgen_run", "rvc_loadgen_call_pairhmm", "rvc_loadgen_region_latency", "rvc_phmm_batch_kernels", "rvc_sw_align_batch", "rvc_sw_align_served", "rvc_loadgen_sw_align",
 ]
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -71,7 +71,7 @@ class RvcBatchOpts(ctypes.Structure):
     _fields_ = [("persistent_waves_per_cu", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
                 ("fp32_fold", ctypes.c_uint32), ("rpl_round", ctypes.c_uint32), ("occupancy", ctypes.c_uint32),
                 ("segment_width", ctypes.c_uint32), ("prior_table", ctypes.c_uint32),
-                ("hap_run", ctypes.c_uint32)]
+                ("hap_run", ctypes.c_uint32), ("untimed", ctypes.c_uint32)]
 
 
 class RvcBatchInfo(ctypes.Structure):
@@ -147,6 +147,7 @@ def lib():
         L.rvc_loadgen_call_pairhmm.argtypes = [ctypes.POINTER(RvcRawRegion), ctypes.c_uint32, _u32p, ctypes.c_uint32,
                                                ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, _f64p,
                                                ctypes.POINTER(RvcLoadgenResult)]
+        L.rvc_loadgen_region_latency.argtypes = [ctypes.POINTER(RvcRegion), ctypes.c_uint32, ctypes.c_int32, _f64p]
         L.rvc_phmm_tables.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _f64p,
                                       _f64p]
         _lib = L
@@ -298,7 +299,7 @@ class Batch:
     """Many regions packed, uploaded and launched together."""
 
     def __init__(self, regions, use_graph=False, fp32_fold=0, rpl_round=0, waves_per_cu=0, occupancy=0,
-                 segment_width=0, prior_table=0, hap_run=0):
+                 segment_width=0, prior_table=0, hap_run=0, untimed=False):
         self._regions = regions
         arr = (RvcRegion * len(regions))()
         for x, r in enumerate(regions):
@@ -313,6 +314,7 @@ class Batch:
         opts.segment_width = segment_width
         opts.prior_table = prior_table
         opts.hap_run = hap_run
+        opts.untimed = 1 if untimed else 0
         h = ctypes.c_void_p()
         _check(lib().rvc_phmm_batch_create(arr, len(regions), ctypes.byref(opts), ctypes.byref(h)))
         self.h = h
@@ -436,6 +438,15 @@ def loadgen_run(regions, arrival_s, order=None, n_producers=32, outputs=False):
     _check(rc)
     summary = {k: getattr(res, k) for k, _ in RvcLoadgenResult._fields_}
     return (summary, lat, outs) if outputs else (summary, lat)
+
+
+def region_latency(region, n_calls, noalloc=False):
+    """Per-call latency (ms) of n_calls one-region calls made from C++
+    (rvc_loadgen_region_latency), without Python in the timed loop."""
+    cr = make_c_region(region)
+    lat = np.zeros(n_calls, np.float64)
+    _check(lib().rvc_loadgen_region_latency(ctypes.byref(cr), n_calls, 1 if noalloc else 0, _ptr(lat, _f64p)))
+    return lat
 
 
 def raw_region(region, mqs):

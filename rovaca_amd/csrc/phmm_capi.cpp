@@ -100,25 +100,23 @@ int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags, cons
   w->batch.graph_exec = nullptr;
   w->batch.graph = nullptr;
   const BatchLayout L = batch_layout(pack_sizes(w->pack));
-  if ((rc = w->staging.reserve(L.in_bytes))) return rc;
+  if ((rc = w->staging.reserve(L.cnt + 4 * kNumCounters))) return rc;
   stage_pack(w->pack, L, w->staging.ptr);
-  if ((rc = w->batch.upload_staged(w->pack, L, w->staging.ptr, w->stream, w->mem, w->bytes))) return rc;
+  stage_results(w->pack, L, w->staging.ptr);  // uploaded with the inputs: no begin kernel
+  if ((rc = w->batch.upload_staged(w->pack, L, w->staging.ptr, w->stream, w->mem, w->bytes, true))) return rc;
   if (w->batch.own_mem) {  // the workspace was too small: keep the new allocation
     if (w->mem) hipFree(w->mem);
     w->mem = w->batch.mem;
     w->bytes = w->batch.bytes;
     w->batch.own_mem = false;
   }
-  // kernels, then one copy of results and flags into pinned memory, one wait
-  if ((rc = w->batch.enqueue(w->stream, false))) return rc;
+  // the FP32 pass, one copy of results, flags and counters into pinned
+  // memory, one wait; the FP64 launches only where pairs need them
   const size_t n = w->pack.n_pairs_slots;
-  if ((rc = w->results.reserve(n * (sizeof(double) + 1) + 64))) return rc;
-  double* h_out = reinterpret_cast<double*>(w->results.ptr);
-  uint8_t* h_flags = reinterpret_cast<uint8_t*>(w->results.ptr + n * sizeof(double));
-  // the flags sit right behind the results on both sides: one copy
-  hipError_t e = hipMemcpyAsync(h_out, w->batch.out, n * (sizeof(double) + 1), hipMemcpyDeviceToHost, w->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
-  if (e != hipSuccess) return set_error(RVC_E_DEVICE, "region results: %s", hipGetErrorString(e));
+  if ((rc = w->results.reserve(w->batch.fetch_bytes() + 64))) return rc;
+  if ((rc = w->batch.run_and_fetch(w->stream, w->results.ptr))) return rc;
+  const double* h_out = reinterpret_cast<const double*>(w->results.ptr);
+  const uint8_t* h_flags = reinterpret_cast<const uint8_t*>(w->results.ptr + n * sizeof(double));
   unpermute_results(w->pack, h_out, h_flags, out, flags);
   return RVC_OK;
 }
@@ -393,6 +391,7 @@ int rvc_phmm_batch_create(const rvc_region* regions, uint32_t n_regions, const r
     if (hipStreamSynchronize(b->dev.stream) != hipSuccess) return set_error(RVC_E_DEVICE, "batch upload");
     std::vector<char>().swap(b->pack.staging);
     b->use_graph = opts && opts->use_graph;
+    b->dev.untimed = opts && opts->untimed;
     *out = guard.release();
     return RVC_OK;
   });

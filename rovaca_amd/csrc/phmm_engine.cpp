@@ -288,16 +288,61 @@ void FP32Policy::pick(uint32_t R, int* W, int* RPL) const {
 
 // Appends the packed bases of seq[0..n) at base offset off (a multiple of
 // kSeqAlign) of the pack / N-mask word arrays.
+namespace {
+// 2-bit code (bits 0-1) and N flag (bit 2) of every byte, from base_code.
+struct CodeTable {
+  uint8_t c[256];
+  CodeTable() {
+    for (int x = 0; x < 256; ++x) {
+      bool is_n;
+      const uint32_t code = base_code((uint8_t)x, &is_n);
+      c[x] = (uint8_t)(code | (is_n ? 4u : 0u));
+    }
+  }
+};
+const CodeTable kCodes;
+}  // namespace
+
 static void pack_seq(const uint8_t* seq, uint32_t n, uint32_t off, std::vector<uint32_t>& pack,
                      std::vector<uint32_t>& nmask) {
-  for (uint32_t x = 0; x < n; ++x) {
-    bool is_n;
-    const uint32_t code = base_code(seq[x], &is_n);
-    const uint32_t g = off + x;
-    pack[g >> 4] |= code << ((g & 15u) * 2u);
-    if (is_n) nmask[g >> 5] |= 1u << (g & 31u);
+  // off is a multiple of 32: whole 2-bit and N-mask words, assembled in registers
+  uint32_t* pw = pack.data() + (off >> 4);
+  uint32_t* nw = nmask.data() + (off >> 5);
+  uint32_t x = 0;
+  for (; x + 32 <= n; x += 32) {
+    uint32_t lo = 0, hi = 0, nm = 0;
+    for (uint32_t y = 0; y < 16; ++y) {
+      const uint32_t c0 = kCodes.c[seq[x + y]], c1 = kCodes.c[seq[x + 16 + y]];
+      lo |= (c0 & 3u) << (2 * y);
+      hi |= (c1 & 3u) << (2 * y);
+      nm |= ((c0 >> 2) << y) | ((c1 >> 2) << (16 + y));
+    }
+    pw[x >> 4] |= lo;
+    pw[(x >> 4) + 1] |= hi;
+    nw[x >> 5] |= nm;
+  }
+  for (; x < n; ++x) {  // the last, partial word
+    const uint32_t c = kCodes.c[seq[x]];
+    pw[x >> 4] |= (c & 3u) << ((x & 15u) * 2u);
+    nw[x >> 5] |= (c >> 2) << (x & 31u);
   }
 }
+
+namespace {
+// Per-thread working storage of the packing steps, kept from call to call.
+struct PackScratch {
+  std::vector<std::vector<Tile>> bins = std::vector<std::vector<Tile>>(kFp32Bins);
+  std::vector<std::vector<uint32_t>> bin_cost = std::vector<std::vector<uint32_t>>(kFp32Bins);
+  std::vector<uint8_t> hap_has_n;
+  std::vector<uint32_t> order, idx;
+  std::vector<Tile> out, bin;
+  std::vector<uint32_t> start, cost;
+};
+PackScratch& pack_scratch() {
+  thread_local PackScratch s;
+  return s;
+}
+}  // namespace
 
 // Appends tiles to dst ordered by cost (anti-diagonal steps of the tile),
 // largest first, so that the persistent waves end on small tiles. Costs are
@@ -308,7 +353,8 @@ static void append_largest_first(const std::vector<Tile>& tiles, const std::vect
   const size_t base = dst.size();
   dst.resize(base + tiles.size());
   if (tiles.size() < 512) {
-    std::vector<uint32_t> idx(tiles.size());
+    std::vector<uint32_t>& idx = pack_scratch().idx;
+    idx.resize(tiles.size());
     for (uint32_t x = 0; x < idx.size(); ++x) idx[x] = x;
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t p, uint32_t q) { return (cost[p] >> 4) > (cost[q] >> 4); });
     for (size_t x = 0; x < idx.size(); ++x) dst[base + x] = tiles[idx[x]];
@@ -327,10 +373,43 @@ static void append_largest_first(const std::vector<Tile>& tiles, const std::vect
   for (size_t x = 0; x < tiles.size(); ++x) dst[base + count[bucket(x)]++] = tiles[x];
 }
 
+void HostPack::reset() {
+  HostPack fresh;  // the scalars of a new pack
+  auto keep = [](auto& v, auto& into) {
+    v.clear();
+    into.swap(v);
+  };
+  keep(region, fresh.region);
+  keep(hap_pack, fresh.hap_pack);
+  keep(hap_nmask, fresh.hap_nmask);
+  keep(hap_off, fresh.hap_off);
+  keep(hap_len, fresh.hap_len);
+  keep(read_pack, fresh.read_pack);
+  keep(read_nmask, fresh.read_nmask);
+  keep(read_off, fresh.read_off);
+  keep(read_len, fresh.read_len);
+  keep(rq, fresh.rq);
+  keep(ri, fresh.ri);
+  keep(rd, fresh.rd);
+  keep(rc, fresh.rc);
+  keep(read_perm, fresh.read_perm);
+  keep(tiles, fresh.tiles);
+  keep(bin_start, fresh.bin_start);
+  keep(fp64_cap, fresh.fp64_cap);
+  keep(fp64_base, fresh.fp64_base);
+  keep(read_raw, fresh.read_raw);
+  keep(read_prep, fresh.read_prep);
+  keep(staging, fresh.staging);
+  keep(nan_scratch, fresh.nan_scratch);
+  keep(flag_scratch, fresh.flag_scratch);
+  *this = std::move(fresh);
+}
+
 int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Policy& pol, HostPack* hp,
                     const DevicePrep* prep) {
   HostPack& P = *hp;
-  P = HostPack();
+  P.reset();
+  PackScratch& W = pack_scratch();
   P.n_regions = n_regions;
   P.region.resize(n_regions);
   uint64_t n_hap_total = 0, n_read_total = 0, hap_bases = 0, read_bases = 0, pairs = 0;
@@ -383,10 +462,14 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
     fp32_variant_shape(b % kFp32Variants, &W, &RPL);
     P.occ[b] = (uint8_t)pol.occupancy(W, RPL, b >= kFp32Variants);
   }
-  std::vector<std::vector<Tile>> bins(kFp32Bins);
-  std::vector<std::vector<uint32_t>> bin_cost(kFp32Bins);
+  std::vector<std::vector<Tile>>& bins = W.bins;
+  std::vector<std::vector<uint32_t>>& bin_cost = W.bin_cost;
+  for (int v = 0; v < kFp32Bins; ++v) {
+    bins[v].clear();
+    bin_cost[v].clear();
+  }
   const uint32_t run_max = P.hap_run;
-  std::vector<uint8_t> hap_has_n;
+  std::vector<uint8_t>& hap_has_n = W.hap_has_n;
   for (uint32_t r = 0; r < n_regions; ++r) {
     const rvc_region& g = regions[r];
     RegionLayout& L = P.region[r];
@@ -401,7 +484,8 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
       hcur += align_up(len, kSeqAlign);
     }
     // reads of a region in descending length, so that tiles group equal lengths
-    std::vector<uint32_t> order(g.n_reads);
+    std::vector<uint32_t>& order = W.order;
+    order.resize(g.n_reads);
     for (uint32_t x = 0; x < g.n_reads; ++x) order[x] = x;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
       return (g.read_off[a + 1] - g.read_off[a]) > (g.read_off[b + 1] - g.read_off[b]);
@@ -552,10 +636,13 @@ void split_tile(const Tile& t, uint32_t run, F&& emit) {
 }  // namespace
 
 void split_runs(HostPack& P) {
-  std::vector<Tile> out;
-  std::vector<uint32_t> start(kFp32Bins + 1, 0);
-  std::vector<Tile> bin;
-  std::vector<uint32_t> cost;
+  PackScratch& S = pack_scratch();
+  std::vector<Tile>& out = S.out;
+  std::vector<uint32_t>& start = S.start;
+  std::vector<Tile>& bin = S.bin;
+  std::vector<uint32_t>& cost = S.cost;
+  out.clear();
+  start.assign(kFp32Bins + 1, 0);
   for (int v = 0; v < kFp32Bins; ++v) {
     start[v] = (uint32_t)out.size();
     const uint32_t run = run_for_bin(P, v, P.bin_units[v]);
@@ -651,10 +738,13 @@ BatchLayout batch_layout(const PackSizes& z) {
   L.prep = take(4 * z.n_reads);
   L.in_bytes = off;
   // flags right behind the results: one copy brings both back
-  L.out = take(sizeof(double) * z.n_pairs_slots + z.n_pairs_slots + 8);
+  // results, flags (rounded up to 8 bytes) and the counters one after
+  // another: one copy brings all three back
+  const size_t flag_bytes = (z.n_pairs_slots + 7) & ~size_t(7);
+  L.out = take(sizeof(double) * z.n_pairs_slots + flag_bytes + 4 * kNumCounters);
   L.flags = L.out + sizeof(double) * z.n_pairs_slots;
+  L.cnt = L.flags + flag_bytes;
   L.resc = take(sizeof(RescueEntry) * (z.n_rescue + 1));
-  L.cnt = take(4 * kNumCounters);
   L.bytes = off;
   return L;
 }
@@ -677,6 +767,16 @@ void stage_pack(const HostPack& P, const BatchLayout& L, char* dst) {
   put(L.rbase, P.fp64_base.data(), kNumFp64Lists * sizeof(uint32_t));
   if (P.any_prep) put(L.raw, P.read_raw.data(), P.read_raw.size());
   put(L.prep, P.read_prep.data(), bytes_of(P.read_prep));
+}
+
+size_t stage_results(const HostPack& P, const BatchLayout& L, char* dst) {
+  const size_t n = P.n_pairs_slots;
+  const size_t flag_bytes = (n + 7) & ~size_t(7);
+  const uint64_t nan = 0x7ff8000000000000ull;
+  uint64_t* o = reinterpret_cast<uint64_t*>(dst + L.out);
+  for (size_t x = 0; x < n; ++x) o[x] = nan;
+  std::memset(dst + L.flags, 0, flag_bytes + 4 * kNumCounters);
+  return L.cnt + 4 * kNumCounters;
 }
 
 PinnedBuf::~PinnedBuf() { release(); }
@@ -852,7 +952,7 @@ int DeviceBatch::upload(const HostPack& P, hipStream_t s, void* reuse_mem, size_
 }
 
 int DeviceBatch::upload_staged(const HostPack& P, const BatchLayout& L, const void* host, hipStream_t s,
-                               void* reuse_mem, size_t reuse_bytes) {
+                               void* reuse_mem, size_t reuse_bytes, bool with_results) {
   pack = &P;
   layout = L;
   bytes = L.bytes;
@@ -864,9 +964,13 @@ int DeviceBatch::upload_staged(const HostPack& P, const BatchLayout& L, const vo
     own_mem = true;
   }
   char* base = static_cast<char*>(mem);
-  RVC_HIP(hipMemcpyAsync(base, host, L.in_bytes, hipMemcpyHostToDevice, s));
-  // results start as NaN with no flags; pairs with an empty side keep that
-  RVC_HIP(launch_init_results(reinterpret_cast<double*>(base + L.out), P.n_pairs_slots, s));
+  // with_results: the host image carries the starting results and zero
+  // counters too (stage_results); else the first launch sequence's begin
+  // kernel sets results to NaN with no flags (pairs with an empty side keep
+  // that) and clears the counters
+  RVC_HIP(hipMemcpyAsync(base, host, with_results ? L.cnt + 4 * kNumCounters : L.in_bytes, hipMemcpyHostToDevice, s));
+  init_pending = !with_results;
+  counters_clear = with_results;
   seqs.hap_pack = reinterpret_cast<const uint32_t*>(base + L.hpack);
   seqs.hap_nmask = reinterpret_cast<const uint32_t*>(base + L.hn);
   seqs.hap_off = reinterpret_cast<const uint32_t*>(base + L.hoff);
@@ -891,11 +995,16 @@ int DeviceBatch::upload_staged(const HostPack& P, const BatchLayout& L, const vo
   return RVC_OK;
 }
 
-int DeviceBatch::enqueue(hipStream_t s, bool timed) {
+int DeviceBatch::enqueue(hipStream_t s, bool timed, const uint32_t* rescued) {
   const HostPack& P = *pack;
   uint32_t* heads = counters;
   uint32_t* rescue_count = counters + kFp32Bins + kNumFp64Lists;
-  RVC_HIP(launch_zero_u32(counters, n_counters_, s));
+  // one kernel: counters to zero and, after an upload, results to NaN / flags 0
+  // (none where the upload brought both)
+  if (init_pending || !counters_clear)
+    RVC_HIP(launch_begin(counters, n_counters_, out, init_pending ? P.n_pairs_slots : 0, s));
+  init_pending = false;
+  counters_clear = false;
   if (P.any_prep) {
     // raw qualities -> q, i, d, c in place (idempotent: a replay gives the same)
     PrepArgs pa;
@@ -963,8 +1072,35 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     if ((rc = mark((int)launches.size()))) return rc;
   }
   if (timed) RVC_HIP(hipEventRecord(ev[1], s));
+  if (rescued == kNoFp64) {
+    if (timed) RVC_HIP(hipEventRecord(ev[2], s));
+    return RVC_OK;
+  }
+  return enqueue_fp64(s, timed, rescued);
+}
+
+int DeviceBatch::enqueue_fp64(hipStream_t s, bool timed, const uint32_t* rescued) {
+  const HostPack& P = *pack;
+  uint32_t* heads = counters;
+  uint32_t* rescue_count = counters + kFp32Bins + kNumFp64Lists;
+  const int X = scaled_exponent(P);
+  const bool sc = X > 0;
+  int rc = RVC_OK;
+  auto mark = [&](int idx) -> int {
+    if (!timed) return RVC_OK;
+    while ((int)lev.size() <= idx) {
+      hipEvent_t e;
+      RVC_HIP(hipEventCreate(&e));
+      lev.push_back(e);
+    }
+    RVC_HIP(hipEventRecord(lev[idx], s));
+    return RVC_OK;
+  };
   for (int l = 0; l < kNumFp64Lists; ++l) {
-    if (P.fp64_cap[l] == 0) continue;
+    // every list that can hold pairs, or (with the counts read back after the
+    // FP32 pass) only the lists that do
+    const uint64_t n_list = rescued ? rescued[l] : P.fp64_cap[l];
+    if (n_list == 0) continue;
     const int b = l % kNumFp64Bins;
     const bool pt = l >= kNumFp64Bins && P.prior_table && sc;
     Fp64Args a;
@@ -981,7 +1117,7 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     const int per_cu = std::max(1, fp64_blocks_per_cu(b, pt, sc));
     int W, RPL;
     fp64_bin_shape(b, &W, &RPL);
-    const uint64_t max_waves = (P.fp64_cap[l] + (64 / W) - 1) / (64 / W);
+    const uint64_t max_waves = (n_list + (64 / W) - 1) / (64 / W);
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(max_waves, (uint64_t)per_cu * g_engine.n_cu));
     hipError_t e = launch_fp64(b, pt, sc, grid, s, a);
     if (e != hipSuccess) return set_error(RVC_E_DEVICE, "FP64 launch list %d: %s", l, hipGetErrorString(e));
@@ -994,6 +1130,31 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed) {
     if ((rc = mark((int)launches.size()))) return rc;
   }
   if (timed) RVC_HIP(hipEventRecord(ev[2], s));
+  return RVC_OK;
+}
+
+size_t DeviceBatch::fetch_bytes() const {
+  const size_t n = pack->n_pairs_slots;
+  return sizeof(double) * n + ((n + 7) & ~size_t(7)) + 4 * kNumCounters;
+}
+
+int DeviceBatch::run_and_fetch(hipStream_t s, char* host) {
+  const size_t n = pack->n_pairs_slots;
+  int rc = enqueue(s, false, kNoFp64);
+  if (rc) return rc;
+  RVC_HIP(hipMemcpyAsync(host, out, fetch_bytes(), hipMemcpyDeviceToHost, s));
+  RVC_HIP(hipStreamSynchronize(s));
+  const uint32_t* cnt = reinterpret_cast<const uint32_t*>(host + sizeof(double) * n + ((n + 7) & ~size_t(7)));
+  uint32_t rescued[kNumFp64Lists];
+  std::memcpy(rescued, cnt + kFp32Bins + kNumFp64Lists, sizeof rescued);
+  bool any = false;
+  for (uint32_t c : rescued) any = any || c != 0;
+  n_rescued = 0;
+  for (uint32_t c : rescued) n_rescued += c;
+  if (!any) return RVC_OK;
+  if ((rc = enqueue_fp64(s, false, rescued))) return rc;
+  RVC_HIP(hipMemcpyAsync(host, out, n * (sizeof(double) + 1), hipMemcpyDeviceToHost, s));
+  RVC_HIP(hipStreamSynchronize(s));
   return RVC_OK;
 }
 
@@ -1020,6 +1181,18 @@ int DeviceBatch::run(bool use_graph_) {
     }
     RVC_HIP(hipEventRecord(ev[0], stream));
     RVC_HIP(hipGraphLaunch(graph_exec, stream));
+    RVC_HIP(hipEventRecord(ev[2], stream));
+    RVC_HIP(hipEventSynchronize(ev[2]));
+    float ms = 0;
+    RVC_HIP(hipEventElapsedTime(&ms, ev[0], ev[2]));
+    total_ms = ms;
+    fp32_ms = fp64_ms = -1;
+  } else if (untimed) {
+    // the same launches as the graph, queued one by one, with no events
+    // between them
+    RVC_HIP(hipEventRecord(ev[0], stream));
+    rc = enqueue(stream, false);
+    if (rc) return rc;
     RVC_HIP(hipEventRecord(ev[2], stream));
     RVC_HIP(hipEventSynchronize(ev[2]));
     float ms = 0;

@@ -150,6 +150,9 @@ struct HostPack {
   std::vector<char> staging;               // host image of the device inputs (upload)
   std::vector<double> nan_scratch;
   std::vector<uint8_t> flag_scratch;
+  // Back to the state of a new pack, keeping the vectors' storage (packing
+  // one region after another on a thread then allocates nothing new).
+  void reset();
 };
 
 // Row -1 scale exponent X of the FP32 pass's scaled form (2^X in place of
@@ -178,6 +181,10 @@ struct BatchLayout {
 BatchLayout batch_layout(const PackSizes& z);
 // Writes the input section of P at dst (in_bytes of L).
 void stage_pack(const HostPack& P, const BatchLayout& L, char* dst);
+// Writes the starting results (NaN, flags 0) and zero counters behind the
+// inputs at dst, so that one upload of [0, out_end) replaces the begin
+// kernel; returns out_end.
+size_t stage_results(const HostPack& P, const BatchLayout& L, char* dst);
 
 // Grow-only pinned host buffer.
 // Pinned host buffer that grows by doubling. Growing keeps the old block
@@ -215,6 +222,9 @@ void split_runs(HostPack& P);
 void unpermute_results(const HostPack& P, const double* dev_out, const uint8_t* dev_flags, double* host_out,
                        uint8_t* host_flags);
 
+// enqueue() argument: queue the FP32 pass only.
+inline const uint32_t* const kNoFp64 = reinterpret_cast<const uint32_t*>(uintptr_t(1));
+
 struct DeviceBatch {
   const HostPack* pack = nullptr;
   void* mem = nullptr;
@@ -241,6 +251,9 @@ struct DeviceBatch {
   double fp32_ms = 0, fp64_ms = 0, total_ms = 0;
   uint64_t n_rescued = 0;
   uint32_t n_fp32_launches = 0, n_fp64_launches = 0;
+  bool init_pending = false;  // results not yet set to NaN since the last upload
+  bool counters_clear = false;  // the upload brought zeroed counters: no begin kernel
+  bool untimed = false;       // eager runs record no per-launch events
 
   ~DeviceBatch();
   void release();
@@ -251,9 +264,21 @@ struct DeviceBatch {
   // The same for inputs already staged in device layout at 'host' (pinned
   // memory makes the copy asynchronous); 'meta' describes the batch.
   int upload_staged(const HostPack& meta, const BatchLayout& L, const void* host, hipStream_t s, void* reuse_mem,
-                    size_t reuse_bytes);
-  // Queues counter reset, FP32 launches and FP64 rescue launches on s.
-  int enqueue(hipStream_t s, bool timed);
+                    size_t reuse_bytes, bool with_results = false);
+  // Queues counter reset, FP32 launches and FP64 rescue launches on s: one
+  // FP64 launch per list that can hold pairs (rescued == nullptr), none
+  // (rescued == kNoFp64), or one per list whose count in rescued[] is not 0.
+  int enqueue(hipStream_t s, bool timed, const uint32_t* rescued = nullptr);
+  // The FP64 launches alone (as enqueue queues them after the FP32 pass).
+  int enqueue_fp64(hipStream_t s, bool timed, const uint32_t* rescued);
+  // Bytes of results, flags and counters as they sit on the device.
+  size_t fetch_bytes() const;
+  // The whole launch sequence for a caller that waits for the results: the
+  // FP32 pass, one copy of results, flags and counters into host (fetch_bytes
+  // of room, pinned for an asynchronous copy), a wait; then, only if the
+  // counters show pairs below 1e-28, the FP64 launches of those lists and a
+  // second copy of results and flags. Returns when host holds the results.
+  int run_and_fetch(hipStream_t s, char* host);
   int ensure_events();
   // Runs the launch sequence on 'stream' (optionally as a hipGraph) and waits.
   int run(bool use_graph);

@@ -178,5 +178,8 @@ hipError_t launch_fp64(int bin, bool pt, bool scaled, int grid, hipStream_t stre
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
 // out[0 .. n) = NaN and the n flag bytes behind them (rounded up to 8) = 0.
 hipError_t launch_init_results(double* out, uint64_t n, hipStream_t stream);
+// The start of a launch sequence: p[0 .. n) = 0 and, for n_out > 0, the
+// results as launch_init_results sets them, in one kernel.
+hipError_t launch_begin(uint32_t* p, uint32_t n, double* out, uint64_t n_out, hipStream_t stream);
 
 }  // namespace rvc

@@ -960,6 +960,27 @@ hipError_t launch_marginalize(const MargArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// Counters to zero and results to NaN with flags 0, one grid-stride pass.
+__global__ void k_begin(uint32_t* p, uint32_t n, uint64_t* out, uint64_t n_out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint64_t i = i0; i < n; i += stride) p[i] = 0;
+  uint64_t* flags = out + n_out;
+  const uint64_t nf = (n_out + 7) / 8;
+  for (uint64_t i = i0; i < n_out; i += stride) {
+    out[i] = 0x7ff8000000000000ull;
+    if (i < nf) flags[i] = 0;
+  }
+}
+
+hipError_t launch_begin(uint32_t* p, uint32_t n, double* out, uint64_t n_out, hipStream_t stream) {
+  const uint64_t items = std::max<uint64_t>(n, n_out);
+  const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((items + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_begin, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, reinterpret_cast<uint64_t*>(out),
+                     n_out);
+  return hipGetLastError();
+}
+
 hipError_t launch_init_results(double* out, uint64_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);

@@ -125,6 +125,24 @@ extern "C" int rvc_loadgen_run(const rvc_region* regions, uint32_t n_regions, co
   });
 }
 
+extern "C" int rvc_loadgen_region_latency(const rvc_region* region, uint32_t n_calls, int32_t noalloc,
+                                          double* latency_ms) {
+  return rvc::guarded([&]() -> int {
+    using rvc::set_error;
+    if (!region || !latency_ms) return set_error(RVC_E_INVALID, "bad arguments");
+    std::vector<double> out((size_t)region->n_haps * region->n_reads);
+    std::vector<uint8_t> flags(out.size());
+    for (uint32_t i = 0; i < n_calls; ++i) {
+      const Clock::time_point a = Clock::now();
+      const int rc = noalloc ? rvc_phmm_region_sync_noalloc(region, out.data(), flags.data())
+                             : rvc_phmm_region_sync(region, out.data(), flags.data());
+      latency_ms[i] = std::chrono::duration<double, std::milli>(Clock::now() - a).count();
+      if (rc) return rc;
+    }
+    return RVC_OK;
+  });
+}
+
 extern "C" int rvc_loadgen_call_pairhmm(const rvc_raw_region* regions, uint32_t n_regions, const uint32_t* order,
                                         uint32_t n_calls, int32_t min_quality_threshold, int32_t pcr_option,
                                         uint32_t n_workers, double* latency_ms, rvc_loadgen_result* result) {

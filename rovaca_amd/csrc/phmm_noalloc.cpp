@@ -8,14 +8,13 @@
 // The array lives for the whole process on the device the engine uses. Its
 // device address is looked up once at rvc_phmm_init (hipGetSymbolAddress);
 // host code only ever uses that address, never the symbol itself. Calls are
-// serialised by a mutex (they share the one array); counters and results are
-// cleared by kernels on every call (no hipMemsetAsync), so nothing of an
-// earlier call's contents is read.
+// serialised by a mutex (they share the one array); every call uploads its
+// starting results and zero counters with its inputs (no hipMemsetAsync), so
+// nothing of an earlier call's contents is read.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
 #include <mutex>
-#include <vector>
 
 #include "../../include/rovaca_phmm.h"
 #include "phmm_engine.h"
@@ -40,8 +39,10 @@ struct NoAllocState {
   int device = -1;
   hipStream_t stream = nullptr;
   HostPack pack;
-  std::vector<char> staging;    // host image of the inputs
-  std::vector<char> results;    // results and flags as copied back
+  // pinned host buffer of the same size as the device array, allocated at
+  // set-up: the staged inputs go up from it and the results come back into
+  // it (stream order keeps the two apart), so the copies are asynchronous
+  char* host = nullptr;
   DeviceBatch batch;
 };
 NoAllocState g_na;
@@ -60,6 +61,10 @@ int noalloc_init() {
     g_na.stream = nullptr;
     return set_error(RVC_E_DEVICE, "stream of the no-allocation call");
   }
+  if (!g_na.host && hipHostMalloc((void**)&g_na.host, kNoAllocScratchBytes, hipHostMallocDefault) != hipSuccess) {
+    g_na.host = nullptr;
+    return set_error(RVC_E_DEVICE, "pinned buffer of the no-allocation call");
+  }
   g_na.dev = p;
   g_na.device = g_engine.device;
   g_na.batch.stream = g_na.stream;
@@ -74,6 +79,8 @@ void noalloc_shutdown() {
     hipStreamDestroy(g_na.stream);
   }
   g_na.stream = nullptr;
+  if (g_na.host) hipHostFree(g_na.host);
+  g_na.host = nullptr;
   g_na.batch.stream = nullptr;
   g_na.dev = nullptr;
   g_na.device = -1;
@@ -94,23 +101,20 @@ int region_sync_noalloc(const rvc_region* region, double* out, uint8_t* flags) {
   if (L.bytes > kNoAllocScratchBytes)
     return set_error(RVC_E_LIMIT, "region needs %zu bytes of device scratch; the no-allocation call holds %zu",
                      L.bytes, kNoAllocScratchBytes);
-  g_na.staging.resize(L.in_bytes);
-  stage_pack(g_na.pack, L, g_na.staging.data());
+  // (the staged image and the fetched results both fit: each is at most L.bytes)
+  stage_pack(g_na.pack, L, g_na.host);
+  stage_results(g_na.pack, L, g_na.host);  // uploaded with the inputs: no begin kernel
   DeviceBatch& b = g_na.batch;
   // the image goes into the scratch array (reuse_mem covers it: no hipMalloc)
-  if ((rc = b.upload_staged(g_na.pack, L, g_na.staging.data(), g_na.stream, g_na.dev, kNoAllocScratchBytes)))
+  if ((rc = b.upload_staged(g_na.pack, L, g_na.host, g_na.stream, g_na.dev, kNoAllocScratchBytes, true)))
     return rc;
   if (b.own_mem) return set_error(RVC_E_INVALID, "device image did not fit the scratch array");
-  if ((rc = b.enqueue(g_na.stream, false))) return rc;
   const size_t n = g_na.pack.n_pairs_slots;
-  g_na.results.resize(n * (sizeof(double) + 1));
-  // results with the flags right behind them: one copy back, one wait
-  hipError_t e = hipMemcpyAsync(g_na.results.data(), b.out, n * (sizeof(double) + 1), hipMemcpyDeviceToHost,
-                                g_na.stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(g_na.stream);
-  if (e != hipSuccess) return set_error(RVC_E_DEVICE, "region results: %s", hipGetErrorString(e));
-  const double* h_out = reinterpret_cast<const double*>(g_na.results.data());
-  const uint8_t* h_flags = reinterpret_cast<const uint8_t*>(g_na.results.data() + n * sizeof(double));
+  // the FP32 pass, one copy back of results, flags and counters, one wait;
+  // the FP64 launches only where pairs need them
+  if ((rc = b.run_and_fetch(g_na.stream, g_na.host))) return rc;
+  const double* h_out = reinterpret_cast<const double*>(g_na.host);
+  const uint8_t* h_flags = reinterpret_cast<const uint8_t*>(g_na.host + n * sizeof(double));
   unpermute_results(g_na.pack, h_out, h_flags, out, flags);
   b.mem = nullptr;  // the batch never owns the scratch array
   return RVC_OK;

@@ -39,15 +39,19 @@ namespace {
 using Clock = std::chrono::steady_clock;
 constexpr int kSlots = 3;
 
-// Pinned host memory for one launch's results: log10 L and flags per pair
-// slot, plus the FP64 rescue counters.
+// Pinned host memory for one launch's results in their device layout: log10
+// L per pair slot, the flags (rounded up to 8 bytes) and the counters, whose
+// last kNumFp64Lists are the FP64 rescue counts.
 struct ResultBuf {
   char* mem = nullptr;
   size_t cap = 0;  // pair slots
-  size_t n = 0;    // pair slots of the launch holding it; its flags follow its results
+  size_t n = 0;    // pair slots of the launch holding it
   double* out() const { return reinterpret_cast<double*>(mem); }
   uint8_t* flags() const { return reinterpret_cast<uint8_t*>(mem + n * sizeof(double)); }
-  uint32_t* rescued() const { return reinterpret_cast<uint32_t*>(mem + cap * (sizeof(double) + 1) + 64); }
+  const uint32_t* rescued() const {
+    return reinterpret_cast<const uint32_t*>(mem + n * sizeof(double) + ((n + 7) & ~size_t(7))) + kFp32Bins +
+           kNumFp64Lists;
+  }
   ~ResultBuf() {
     if (mem) hipHostFree(mem);
   }
@@ -60,7 +64,8 @@ struct ResultBuf {
     if (mem) retired->push_back(mem);
     mem = nullptr;
     cap = 0;
-    if (hipHostMalloc((void**)&mem, c * (sizeof(double) + 1) + 64 + 64, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&mem, c * (sizeof(double) + 1) + 8 + 4 * kNumCounters + 64, hipHostMallocDefault) !=
+        hipSuccess) {
       mem = nullptr;
       return set_error(RVC_E_DEVICE, "pinned result buffer for %zu pairs", n);
     }
@@ -309,10 +314,9 @@ int launch_slot(Slot& s) {
         hipMemcpyAsync(s.marg_out.ptr, ma.out, 8 * marg_values, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
       return set_error(RVC_E_DEVICE, "marginalisation launch");
   }
-  if (hipMemcpyAsync(s.res->out(), s.batch.out, n * (sizeof(double) + 1), hipMemcpyDeviceToHost, s.stream) !=
+  // results, flags and counters sit one after another: one copy
+  if (hipMemcpyAsync(s.res->out(), s.batch.out, s.batch.fetch_bytes(), hipMemcpyDeviceToHost, s.stream) !=
           hipSuccess ||
-      hipMemcpyAsync(s.res->rescued(), s.batch.counters + kFp32Bins + kNumFp64Lists,
-                     kNumFp64Lists * sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
       hipEventRecord(s.done_ev, s.stream) != hipSuccess)
     return set_error(RVC_E_DEVICE, "result copy");
   return RVC_OK;
