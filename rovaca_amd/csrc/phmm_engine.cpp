@@ -239,10 +239,12 @@ int load_tune_table(const char* path, FP32Policy* pol) {
     } else if (!strcmp(key, "shape16") && cnt == 17) {
       for (int x = 0; x < 17; ++x) {
         const int w = v[x] >> 8, r = v[x] & 0xff;
-        // RPL 0: the fewest rows of width w for the longest read of class x
-        const int rr = r ? r : (x * 16 + w - 1) / std::max(1, w);
+        // RPL 0: the fewest rows of width w for each read of class x; the
+        // shortest must have a kernel (longer ones without one keep the
+        // 16-lane default, FP32Policy::pick)
+        const int rr = r ? r : ((x - 1) * 16 + 1 + w - 1) / std::max(1, w);
         const bool ok = v[x] == 0 || ((w == 8 || w == 16 || w == 32 || w == 64) && rr >= 1 &&
-                                      rr * w >= x * 16 && fp32_supported(w, rr));
+                                      (r == 0 || rr * w >= x * 16) && fp32_supported(w, rr));
         if (!ok) { rc = set_error(RVC_E_INVALID, "tune table %s: bad shape16 entry %d", path, v[x]); break; }
         p.rpl16_override[x] = (uint16_t)v[x];
       }
@@ -262,7 +264,7 @@ int load_tune_table(const char* path, FP32Policy* pol) {
   return rc;
 }
 
-void FP32Policy::pick(uint32_t R, int* W, int* RPL) const {
+void FP32Policy::pick(uint32_t R, int* W, int* RPL, bool cmp) const {
   // rows for R + 1: the segment always ends in a padding row, whose I holds
   // M + I of the read's last row (the prior-table kernel sums it directly)
   const uint32_t Rp = R + 1;
@@ -273,16 +275,25 @@ void FP32Policy::pick(uint32_t R, int* W, int* RPL) const {
   if (Rp <= 256) { *W = 16; *RPL = rpl_steps_of_3 ? kRpl16[exact] : exact; }
   else if (Rp <= 512) { *W = 32; *RPL = (int)((Rp + 31) / 32); }
   else { *W = 64; *RPL = (int)((Rp + 63) / 64); }
+  bool tuned = false;
   if (Rp <= 256 && force_w) {
     *W = force_w;
     *RPL = (int)std::max<uint32_t>(1, (Rp + force_w - 1) / force_w);
   } else if (Rp <= 256 && rpl16_override[exact] != 0) {
+    tuned = true;
     *W = rpl16_override[exact] >> 8;
     *RPL = rpl16_override[exact] & 0xff;
     if (*RPL == 0) {  // fewest rows of width W, in steps of 3 like the 16-lane shapes
       *RPL = (int)((Rp + *W - 1) / *W);
       if (rpl_steps_of_3 && *RPL <= 16) *RPL = kRpl16[*RPL];
     }
+  }
+  // a shape no kernel has (8 lanes past 19 rows) or, chosen by the tuned
+  // table, one past 16 rows for the compare kernel (it spills there) keeps
+  // the 16-lane default
+  if (Rp <= 256 && (!fp32_supported(*W, *RPL) || (tuned && cmp && *RPL > 16))) {
+    *W = 16;
+    *RPL = rpl_steps_of_3 ? kRpl16[exact] : exact;
   }
 }
 
@@ -526,53 +537,60 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
       }
       rcur += align_up(len, kSeqAlign);
     }
-    // tiles: one haplotype x up to 64/W reads of similar length
-    uint32_t sx = 0;
-    while (sx < g.n_reads) {
-      const uint32_t R0 = P.read_len[L.read0 + sx];
-      if (R0 == 0) break;  // the rest are empty reads
-      int W, RPL;
-      pol.pick(R0, &W, &RPL);
-      const uint32_t nseg_max = 64u / (uint32_t)W;
-      uint32_t nseg = 0;
-      while (nseg < nseg_max && sx + nseg < g.n_reads && P.read_len[L.read0 + sx + nseg] > 0) ++nseg;
-      const int v0 = fp32_variant_index(W, RPL);
-      // runs of consecutive non-empty haplotypes of the same kind (with or
-      // without N), at most run_max per tile
-      uint32_t h = 0;
-      while (h < g.n_haps) {
-        if (P.hap_len[L.hap0 + h] == 0) {
-          ++h;
-          continue;
-        }
-        const uint8_t kind = hap_has_n[h];
-        uint32_t nh = 0, cost = 0;
-        while (nh < run_max && h + nh < g.n_haps && P.hap_len[L.hap0 + h + nh] > 0 && hap_has_n[h + nh] == kind) {
-          cost += P.hap_len[L.hap0 + h + nh] + (uint32_t)W - 1;
-          ++nh;
-        }
-        const int v = v0 + (kind ? 0 : kFp32Variants);
-        Tile t;
-        t.hap = L.hap0 + h;
-        t.read0 = L.read0 + sx;
-        t.out0 = (uint32_t)(L.pair0 + (uint64_t)sx * g.n_haps + h);
-        t.meta = tile_meta(nseg, g.n_haps, nh);
-        bins[v].push_back(t);
-        bin_cost[v].push_back(cost);
-        P.bin_units[v] += nh;
-        for (uint32_t hh = h; hh < h + nh; ++hh) {
-          const uint32_t hl = P.hap_len[L.hap0 + hh];
-          for (uint32_t s = 0; s < nseg; ++s) {
-            const uint32_t rl = P.read_len[L.read0 + sx + s];
-            P.fp64_cap[(kind ? 0 : kNumFp64Bins) + fp64_bin(rl)] += 1;
-            P.n_pairs += 1;
-            P.n_cells += (uint64_t)rl * hl;
-            P.variant_cells[v] += (uint64_t)rl * hl;
+    // tiles: one haplotype x up to 64/W reads of similar length, for the
+    // haplotypes without N (prior-table kernel) and then those with N
+    // (compare kernel), whose shapes may differ
+    bool kinds[2] = {false, false};
+    for (uint32_t h = 0; h < g.n_haps; ++h)
+      if (P.hap_len[L.hap0 + h] > 0) kinds[hap_has_n[h]] = true;
+    for (uint8_t kind = 0; kind < 2; ++kind) {
+      if (!kinds[kind]) continue;
+      uint32_t sx = 0;
+      while (sx < g.n_reads) {
+        const uint32_t R0 = P.read_len[L.read0 + sx];
+        if (R0 == 0) break;  // the rest are empty reads
+        int W, RPL;
+        pol.pick(R0, &W, &RPL, kind != 0);
+        const uint32_t nseg_max = 64u / (uint32_t)W;
+        uint32_t nseg = 0;
+        while (nseg < nseg_max && sx + nseg < g.n_reads && P.read_len[L.read0 + sx + nseg] > 0) ++nseg;
+        const int v0 = fp32_variant_index(W, RPL);
+        // runs of consecutive non-empty haplotypes of this kind, at most
+        // run_max per tile
+        uint32_t h = 0;
+        while (h < g.n_haps) {
+          if (P.hap_len[L.hap0 + h] == 0 || hap_has_n[h] != kind) {
+            ++h;
+            continue;
           }
+          uint32_t nh = 0, cost = 0;
+          while (nh < run_max && h + nh < g.n_haps && P.hap_len[L.hap0 + h + nh] > 0 && hap_has_n[h + nh] == kind) {
+            cost += P.hap_len[L.hap0 + h + nh] + (uint32_t)W - 1;
+            ++nh;
+          }
+          const int v = v0 + (kind ? 0 : kFp32Variants);
+          Tile t;
+          t.hap = L.hap0 + h;
+          t.read0 = L.read0 + sx;
+          t.out0 = (uint32_t)(L.pair0 + (uint64_t)sx * g.n_haps + h);
+          t.meta = tile_meta(nseg, g.n_haps, nh);
+          bins[v].push_back(t);
+          bin_cost[v].push_back(cost);
+          P.bin_units[v] += nh;
+          for (uint32_t hh = h; hh < h + nh; ++hh) {
+            const uint32_t hl = P.hap_len[L.hap0 + hh];
+            for (uint32_t s = 0; s < nseg; ++s) {
+              const uint32_t rl = P.read_len[L.read0 + sx + s];
+              P.fp64_cap[(kind ? 0 : kNumFp64Bins) + fp64_bin(rl)] += 1;
+              P.n_pairs += 1;
+              P.n_cells += (uint64_t)rl * hl;
+              P.variant_cells[v] += (uint64_t)rl * hl;
+            }
+          }
+          h += nh;
         }
-        h += nh;
+        sx += nseg;
       }
-      sx += nseg;
     }
   }
   P.tiles.clear();
@@ -666,12 +684,12 @@ void split_runs(HostPack& P) {
 
 int fp32_variant_index(int W, int RPL) {
   const int w = W == 16 ? 0 : W == 32 ? 1 : W == 64 ? 2 : 3;
-  return w * 16 + (RPL - 1);
+  return w * kMaxRpl + (RPL - 1);
 }
 void fp32_variant_shape(int v, int* W, int* RPL) {
   static const int kW[4] = {16, 32, 64, 8};
-  *W = kW[v / 16];
-  *RPL = v % 16 + 1;
+  *W = kW[v / kMaxRpl];
+  *RPL = v % kMaxRpl + 1;
 }
 
 // ---------------------------------------------------------------------------

@@ -54,10 +54,11 @@ int engine_init(const rvc_config* cfg);
 int engine_shutdown();
 int validate_region(const rvc_region* r);
 
-// FP32 kernel shapes: W in {16, 32, 64, 8} x RPL in 1..16. Tile bins are
-// [table][shape]: table = 1 for haplotypes without N (prior-table kernel
-// allowed), 0 otherwise.
-constexpr int kFp32Variants = 64;
+// FP32 kernel shapes: W in {16, 32, 64, 8} x RPL in 1..kMaxRpl (W = 64 up to
+// 13, W = 16 and 32 up to 16, W = 8 up to 19). Tile bins are [table][shape]:
+// table = 1 for haplotypes without N (prior-table kernel allowed), 0
+// otherwise.
+constexpr int kFp32Variants = 4 * kMaxRpl;
 constexpr int kFp32Bins = 2 * kFp32Variants;
 int fp32_variant_index(int W, int RPL);
 void fp32_variant_shape(int v, int* W, int* RPL);
@@ -71,22 +72,27 @@ int fp64_blocks_per_cu(int bin, bool pt, bool scaled);
 // filled by the tile tuner.
 struct FP32Policy {
   uint16_t rpl16_override[17] = {};
-  // waves per SIMD the 16-lane kernel of each RPL is compiled for (2, 3, 4)
-  uint8_t occ16[17] = {4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 2, 2, 2, 2, 2, 2};
+  // waves per SIMD the 16-lane kernel of each RPL is compiled for (2, 3, 4);
+  // rows per lane past 16 (W = 8 only) run at 2
+  uint8_t occ16[kMaxRpl + 1] = {4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
   bool rpl_steps_of_3 = true;  // 16-lane shapes only in RPL 1, 2, 4, 7, 10, 13, 16
   bool fold = true;            // scaled forms where the gap qualities allow
   uint32_t waves_per_cu = 0;   // persistent grid: 0 = occupancy limit
   // the same for the prior-table kernels (W = 16 and W = 32)
-  uint8_t occpt16[17] = {4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 2, 2, 2, 2, 2, 2};
-  uint8_t occpt32[17] = {4, 4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 3, 2, 2, 2, 2};
-  uint8_t occpt8[17] = {4, 4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 3, 4, 4, 4, 4};
+  uint8_t occpt16[kMaxRpl + 1] = {4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
+  uint8_t occpt32[kMaxRpl + 1] = {4, 4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 3, 2, 2, 2, 2, 2, 2, 2, 2};
+  uint8_t occpt8[kMaxRpl + 1] = {4, 4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 3, 4, 4, 4, 4, 2, 2, 2, 2};
   bool prior_table = true;     // use the prior-table kernel where the haplotype has no N
   uint8_t force_occ = 0;       // batch option: 2/3/4 for every 16-lane kernel
   uint8_t force_w = 0;         // batch option: 16/32/64 lanes per read for reads <= 256
   uint32_t hap_run = 16;       // most haplotypes per tile (consecutive, of one region)
   bool hap_run_fixed = false;  // batch option: exactly hap_run per tile, whatever the batch size
-  void pick(uint32_t R, int* W, int* RPL) const;
+  // cmp: for the compare-and-select kernel (haplotypes with N), which has no
+  // 8-lane shape past 16 rows per lane (it would spill): those tiles keep
+  // the 16-lane shape
+  void pick(uint32_t R, int* W, int* RPL, bool cmp = false) const;
   int occupancy(int W, int RPL, bool pt) const {
+    if (RPL > 16) return 2;
     if (force_occ && (W == 16 || (pt && (W == 32 || W == 8)))) return force_occ;
     if (pt) return W == 16 ? occpt16[RPL] : W == 32 ? occpt32[RPL] : W == 8 ? occpt8[RPL] : 2;
     return W == 16 ? occ16[RPL] : 2;

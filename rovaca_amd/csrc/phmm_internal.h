@@ -11,6 +11,8 @@ namespace rvc {
 constexpr int kMaxHaps = 4096;
 constexpr int kMaxHapLen = 1536;
 constexpr int kMaxReadLen = 768;
+// Largest rows per lane an FP32 kernel shape may have (8-lane shapes go to 19).
+constexpr int kMaxRpl = 20;
 
 // Every packed sequence starts on a 32-base boundary, so its 2-bit words
 // (16 bases per uint32) and N-mask words (32 bases per uint32) are aligned.

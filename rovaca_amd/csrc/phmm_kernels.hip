@@ -449,7 +449,13 @@ template <int W, int RPL, bool XF, int OCC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void k_fp32pt(Fp32Args a) {
   constexpr int NQ = (RPL + 3) / 4;
   constexpr int F = XF ? kScaled : kPlain;
-  __shared__ float4 ptab[4 * NQ * 64];
+  // Past 16 rows per lane the last, partial group of 4 rows is stored as
+  // single floats (TR of them): 19 rows then take 19.5 KB of table instead
+  // of 20.5 KB, and eight waves fit in a CU's 160 KB of LDS.
+  constexpr int TR = RPL > 16 ? RPL % 4 : 0;  // rows in the single-float tail
+  constexpr int NF = TR ? RPL / 4 : NQ;       // float4 groups
+  __shared__ float4 ptab[4 * NF * 64];
+  __shared__ float ptail[TR ? 4 * TR * 64 : 1];
   __shared__ uint32_t hpk[kHapWords];
   const int lane = threadIdx.x;
   const int seg = seg_of<W>(lane), k = pos_of<W>(lane);
@@ -472,16 +478,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     int rc[NQ * 4];
     row_params<float, RPL, NQ * 4, F>(a.s, ph2pr, mmt, read, k, tmm, tgm, tmi, tii, tmd, tdd, pm, px, rc);
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < 4; ++c) {
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
+      for (int q = 0; q < NF; ++q) {
         float4 v;
         v.x = (rc[4 * q + 0] < 0 || rc[4 * q + 0] == c) ? pm[4 * q + 0] : px[4 * q + 0];
         v.y = (rc[4 * q + 1] < 0 || rc[4 * q + 1] == c) ? pm[4 * q + 1] : px[4 * q + 1];
         v.z = (rc[4 * q + 2] < 0 || rc[4 * q + 2] == c) ? pm[4 * q + 2] : px[4 * q + 2];
         v.w = (rc[4 * q + 3] < 0 || rc[4 * q + 3] == c) ? pm[4 * q + 3] : px[4 * q + 3];
-        ptab[(c * NQ + q) * 64 + lane] = v;
+        ptab[(c * NF + q) * 64 + lane] = v;
       }
+#pragma unroll
+      for (int t = 0; t < TR; ++t) {
+        const int r = 4 * NF + t;
+        ptail[(c * TR + t) * 64 + lane] = (rc[r] < 0 || rc[r] == c) ? pm[r] : px[r];
+      }
+    }
     // this lane's words x = lane, lane + 64 of a haplotype's stream (64
     // zero columns in front); the next haplotype's are loaded while the
     // current one is swept
@@ -526,12 +538,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
       };
       auto load = [&](uint32_t codes, int i, float (&pr)[NQ * 4]) {
         const uint32_t c = (codes >> (2 * i)) & 3u;
-        const float4* b = reinterpret_cast<const float4*>(tab + c * (uint32_t)(NQ * 1024));
+        const float4* b = reinterpret_cast<const float4*>(tab + c * (uint32_t)(NF * 1024));
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
+        for (int q = 0; q < NF; ++q) {
           const float4 v = b[q * 64];
           pr[4 * q + 0] = v.x; pr[4 * q + 1] = v.y; pr[4 * q + 2] = v.z; pr[4 * q + 3] = v.w;
         }
+#pragma unroll
+        for (int t = 0; t < TR; ++t) pr[4 * NF + t] = ptail[(c * TR + t) * 64 + lane];
       };
       // Four steps a -> b -> a -> b -> a per iteration. The read's last row is
       // always a padding row (the host picks RPL with R < W * RPL), which holds
@@ -884,6 +898,13 @@ const std::array<Fp32Fn, 13> kFp32W64[2] = {fp32_row<64, false, 2>(std::make_int
                                             fp32_row<64, true, 2>(std::make_integer_sequence<int, 13>{})};
 const Row16 kFp32W8[2] = {fp32_row<8, false, 2>(std::make_integer_sequence<int, 16>{}),
                           fp32_row<8, true, 2>(std::make_integer_sequence<int, 16>{})};
+// W = 8 with 17-19 rows per lane (reads of 135-151 bases in 152 rows): the
+// compare kernel spills at these sizes and is there only for batches that
+// cannot take the prior-table kernel (gap qualities outside the scaled form)
+const std::array<Fp32Fn, 3> kFp32W8Long[2] = {{{&k_fp32<8, 17, false, 2>, &k_fp32<8, 18, false, 2>,
+                                                &k_fp32<8, 19, false, 2>}},
+                                              {{&k_fp32<8, 17, true, 2>, &k_fp32<8, 18, true, 2>,
+                                                &k_fp32<8, 19, true, 2>}}};
 
 const std::array<Fp64Fn, kNumFp64Bins> kFp64[2] = {
     {{&k_fp64<16, 4, false>, &k_fp64<16, 8, false>, &k_fp64<32, 5, false>, &k_fp64<32, 8, false>,
@@ -905,6 +926,10 @@ const std::array<Fp32Fn, 13> kFp32PtW64 = fp32pt_row<64, 2>(std::make_integer_se
 const Row16 kFp32PtW8[3] = {fp32pt_row<8, 2>(std::make_integer_sequence<int, 16>{}),
                             fp32pt_row<8, 3>(std::make_integer_sequence<int, 16>{}),
                             fp32pt_row<8, 4>(std::make_integer_sequence<int, 16>{})};
+// 8 lanes x 17-19 rows at 2 waves per SIMD (19 rows: 249 VGPRs, no spill;
+// the 20 KB prior table leaves room for 7 waves per CU)
+const std::array<Fp32Fn, 3> kFp32PtW8Long = {{&k_fp32pt<8, 17, true, 2>, &k_fp32pt<8, 18, true, 2>,
+                                              &k_fp32pt<8, 19, true, 2>}};
 
 Fp32Fn fp32pt_fn(int W, int RPL, int occ) {
   if (RPL < 1) return nullptr;
@@ -913,6 +938,7 @@ Fp32Fn fp32pt_fn(int W, int RPL, int occ) {
   if (W == 32 && RPL <= 16) return kFp32PtW32[o][RPL - 1];
   if (W == 64 && RPL <= 13) return kFp32PtW64[RPL - 1];
   if (W == 8 && RPL <= 16) return kFp32PtW8[o][RPL - 1];
+  if (W == 8 && RPL <= 19) return kFp32PtW8Long[RPL - 17];
   return nullptr;
 }
 
@@ -923,6 +949,7 @@ Fp32Fn fp32_fn(int W, int RPL, bool xf, int occ, bool pt) {
   if (W == 32 && RPL <= 16) return kFp32W32[xf][RPL - 1];
   if (W == 64 && RPL <= 13) return kFp32W64[xf][RPL - 1];
   if (W == 8 && RPL <= 16) return kFp32W8[xf][RPL - 1];
+  if (W == 8 && RPL <= 19) return kFp32W8Long[xf][RPL - 17];
   return nullptr;
 }
 }  // namespace
@@ -997,10 +1024,10 @@ namespace rvc {
 // the one-region path ask concurrently: every slot is an atomic, and two
 // threads that both find it empty store the same value.
 int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ, bool pt) {
-  static std::atomic<int> cache[2][2][3][4][17] = {};
+  static std::atomic<int> cache[2][2][3][4][kMaxRpl + 1] = {};
   const int w = W == 16 ? 0 : W == 32 ? 1 : W == 64 ? 2 : 3;
   const int o = std::min(4, std::max(2, occ)) - 2;
-  if (RPL < 1 || RPL > 16) return 1;
+  if (RPL < 1 || RPL > kMaxRpl) return 1;
   std::atomic<int>& slot = cache[pt][xf][o][w][RPL];
   int n = slot.load(std::memory_order_relaxed);
   if (n == 0) {

@@ -57,7 +57,7 @@ def main():
     pool = base_pool(args.regions)
     print(f"[tune] pool of {len(pool)} regions in {time.time() - t0:.1f} s", flush=True)
     rv.init(0, tune_table=os.devnull)  # start from the built-in defaults
-    occ16, occpt16, occpt32, occpt8 = [4] * 17, [4] * 17, [4] * 17, [4] * 17
+    occ16, occpt16, occpt32, occpt8 = [4] * 17, [4] * 17, [4] * 17, [4] * 17  # rows past 16 run at 2
     shape16 = [0] * 17
     report = []
 
@@ -109,6 +109,21 @@ def main():
             shape16[rpl] = 8 * 256  # rows per lane follow the read length
         elif w != 16:
             shape16[rpl] = w * 256 + (-(-(R + 1) // w))
+        # 8-lane shapes of 17-19 rows (prior-table kernel, 2 waves per SIMD)
+        # hold reads up to 151 bases: where the class has such reads, time
+        # them against the chosen shape on the class's reads that fit
+        R8 = min(R, 8 * 19 - 1)
+        if w == 16 and 16 * (rpl - 1) < R8 + 1 and -(-(R8 + 1) // 8) > 16:
+            regs8 = truncated(pool, R8)
+            cells8 = workload.cells(regs8)
+            ms8, g8 = time_batch(regs8, cells8, occupancy=2, segment_width=8, prior_table=1)
+            ms16, g16 = time_batch(regs8, cells8, occupancy=occpt16[rpl], segment_width=16, prior_table=1)
+            rows = -(-(R8 + 1) // 8)
+            report.append((rpl, "table", 8, 2, ms8, g8))
+            print(f"[tune] R={R8:3d} table W= 8 ({rows} rows) occ=2: {ms8:8.3f} ms {g8:8.1f} GCUPS "
+                  f"(W=16: {ms16:8.3f} ms)", flush=True)
+            if ms8 < ms16:
+                shape16[rpl] = 8 * 256
     occ16[0], occpt16[0], occpt32[0] = occ16[1], occpt16[1], occpt32[1]
     # 8-lane shapes were timed at even row counts; odd ones take the next one's
     for r in range(15, 0, -2):

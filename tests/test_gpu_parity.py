@@ -773,3 +773,48 @@ def test_call_pairhmm_marginalises_sites_on_the_device():
         for g, w in zip(marg, want):
             assert np.array_equal(np.isnan(g), np.isnan(w))
             assert np.array_equal(g[~np.isnan(g)], w[~np.isnan(w)])
+
+
+@pytest.mark.parametrize("lo,hi", [(128, 151), (100, 160)])
+def test_eight_lane_long_shapes_equal_default(lo, hi):
+    # 8-lane segments of 17-19 rows per lane (reads of 128-151 bases), on the
+    # prior-table kernel and, for a haplotype with N, the compare kernel; reads
+    # past 151 bases fall back to the 16-lane shape: all the same bits as the
+    # default shapes
+    say(f"8-lane shapes for reads of {lo}-{hi} bases give the default bits")
+    rng = np.random.default_rng(lo + hi)
+    regs = []
+    for x in range(4):
+        base = workload.batched_regions(1, seed=300 + x)[0]
+        H = base["n_haps"]
+        haps = [base["hap_bases"][base["hap_off"][h]:base["hap_off"][h + 1]] for h in range(H)]
+        haps = [h if len(h) >= hi else np.concatenate([h, h, h])[:hi + 40] for h in haps]
+        if x == 0:
+            haps[1] = haps[1].copy()
+            haps[1][3] = ord("N")
+        reads, quals = [], []
+        for r in range(24):
+            n = int(rng.integers(lo, hi + 1))
+            src = haps[int(rng.integers(0, H))]
+            o = int(rng.integers(0, len(src) - n + 1))
+            rd = src[o:o + n].copy()
+            rd[rng.random(n) < 0.02] = workload.ACGT[rng.integers(0, 4)]
+            reads.append(rd)
+            quals.append(rng.integers(10, 41, n).astype(np.uint8))
+        regs.append(workload.make_region(haps, reads, quals))
+    b = rv.Batch(regs, segment_width=8)
+    b.run()
+    out, flags = b.results()
+    names = {k["name"] for k in b.kernels() if k["precision"] == 32}
+    assert any(f"<8,{r}," in n for n in names for r in (17, 18, 19)), names
+    for x, reg in enumerate(regs):
+        alone, af = rv.region_sync(reg)
+        mixed = b.region_result(out, x)
+        assert np.array_equal(np.isnan(alone), np.isnan(mixed))
+        assert np.array_equal(alone[~np.isnan(alone)], mixed[~np.isnan(mixed)])
+        assert np.array_equal(af, b.region_result(flags, x))
+    b.close()
+    want, route, ok = oracle.region_reference(regs[1])
+    got = rv.region_sync(regs[1])[0]
+    m = ~np.isnan(want)
+    assert np.abs(got[m] - want[m]).max() < TOL
