@@ -45,9 +45,11 @@ typedef struct rvc_sw_stats {
  * fresh instance of the library does (sw_core_template.h:356-361 refuses
  * only the lengths >= 2048).
  * max_cigar = 2 * longest read + 2 always suffices.
- * Returns RVC_OK, RVC_E_INVALID (null pointer, bad index), RVC_E_LIMIT (a
- * read of 1025 .. 2047 bases, beyond the device shapes; or more CIGAR
- * elements than max_cigar), RVC_E_STATE or RVC_E_DEVICE. stats may be NULL. */
+ * Reads and haplotypes of up to 2047 bases are aligned (reads of 1025-2047
+ * bases take a slow one-wave-per-SIMD sweep; they are rare in short-read
+ * data). Returns RVC_OK, RVC_E_INVALID (null pointer, bad index), RVC_E_LIMIT
+ * (more CIGAR elements than max_cigar), RVC_E_STATE or RVC_E_DEVICE. stats
+ * may be NULL. */
 int rvc_sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_haps, const uint8_t* read_bases,
                        const int32_t* read_off, uint32_t n_reads, const uint32_t* read_hap, int32_t assemble,
                        uint32_t max_cigar, uint32_t* cigar, int32_t* count, int32_t* offset, int32_t* score,

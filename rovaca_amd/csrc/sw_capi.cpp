@@ -198,7 +198,7 @@ int sw_validate(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_hap
     if (read_off[r] < 0 || read_off[r + 1] < read_off[r]) return set_error(RVC_E_INVALID, "read offsets");
     if (read_hap[r] >= n_haps) return set_error(RVC_E_INVALID, "read %u names haplotype %u", r, read_hap[r]);
     const int32_t n2 = read_off[r + 1] - read_off[r];
-    if (n2 > kSwMaxRead && n2 < kSwMaxLen)
+    if (n2 > kSwMaxRead && n2 < kSwMaxLen)  // (none: the largest shape holds every read the reference aligns)
       return set_error(RVC_E_LIMIT, "read %u has %d bases (the device aligner takes up to %d)", r, n2, kSwMaxRead);
     most = std::max(most, n2);
   }
@@ -298,7 +298,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
       continue;
     int W = 0, RPL = 0;
     shape[r] = sw_shape(p.read_len, &W, &RPL);
-    narrow[r] = sw_narrow(p.hap_len, p.read_len, sc[0], sc[1], sc[2], sc[3]) ? 1 : 0;
+    narrow[r] = shape[r] < kSwPackedShapes && sw_narrow(p.hap_len, p.read_len, sc[0], sc[1], sc[2], sc[3]) ? 1 : 0;
     if (stats) stats->cells += (uint64_t)p.hap_len * p.read_len;
   }
   // a shape with too few pairs to fill the device once (or fewer than the

@@ -10,8 +10,9 @@
 namespace rvc {
 
 constexpr int kSwMaxLen = 2048;       // the reference aligner refuses lengths >= this (sw_avx.c:18, sw_core_template.h:358-361)
-constexpr int kSwMaxRead = 1024;      // longest read the sweep shapes hold
-constexpr int kSwShapes = 12;         // (W, RPL): see sw_shape()
+constexpr int kSwMaxRead = 2048;      // rows of the largest sweep shape (reads up to 2047 bases)
+constexpr int kSwShapes = 13;         // (W, RPL): see sw_shape()
+constexpr int kSwPackedShapes = 12;   // the packed int16 sweep has the first 12 shapes
 constexpr int kSwLow = -(1 << 28);    // "no gap" at row and column 0
 constexpr int kSwBig = -(1 << 26);    // disables a move in the rows past the read
 constexpr uint32_t kSwOpM = 0, kSwOpI = 1, kSwOpD = 2, kSwOpS = 4;  // BAM CIGAR operations
@@ -30,12 +31,14 @@ inline bool sw_narrow(uint32_t n1, uint32_t n2, int match, int mismatch, int ope
          open <= 0 && open >= -300 && extend <= 0 && extend >= -300;
 }
 
-// Sweep shapes (segment width W, rows per lane RPL): 64 .. 1024 rows, with
+// Sweep shapes (segment width W, rows per lane RPL): 64 .. 2048 rows, with
 // steps of 32-64 rows up to 384 so that common read lengths (100, 150, 250
 // bases) leave few rows idle. Up to 12 rows per lane a kernel keeps two waves
-// per SIMD.
-constexpr int kSwW[kSwShapes] = {16, 16, 16, 16, 16, 32, 32, 32, 32, 64, 64, 64};
-constexpr int kSwR[kSwShapes] = {4, 7, 8, 10, 12, 7, 8, 10, 12, 8, 12, 16};
+// per SIMD. The last shape (64 x 32 rows, int32 sweep only) holds the rare
+// reads of 1025-2047 bases that the reference still aligns: one wave per
+// SIMD, its registers spilled to the accumulation registers (no scratch).
+constexpr int kSwW[kSwShapes] = {16, 16, 16, 16, 16, 32, 32, 32, 32, 64, 64, 64, 64};
+constexpr int kSwR[kSwShapes] = {4, 7, 8, 10, 12, 7, 8, 10, 12, 8, 12, 16, 32};
 
 // Backtrack groups per lane: 4 rows each, the last one possibly shorter
 // (its bits sit where a full group's would).

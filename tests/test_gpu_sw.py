@@ -114,11 +114,36 @@ def test_shared_haplotypes_and_batch_independence():
     assert part == whole[123:170]
 
 
+@pytest.mark.parametrize("assemble", [False, True])
+def test_long_reads_1025_to_2047(assemble):
+    # the reference aligns every length below 2048 (sw_core_template.h:356-361):
+    # reads past the 1024 rows of the regular shapes take the 2048-row one
+    say(f"reads of 1025-2047 bases, assemble={assemble}")
+    rng = np.random.default_rng(1025 + assemble)
+    haps = [ACGT[rng.integers(0, 4, n)].tobytes() for n in (2047, 1800, 1100, 600)]
+    reads, read_hap = [], []
+    for n in (1024, 1025, 1026, 1300, 1536, 2000, 2047):
+        h = int(rng.integers(0, 2))
+        src = haps[h]
+        o = int(rng.integers(0, len(src) - min(n, len(src)) + 1))
+        rd = bytearray(src[o:o + n])
+        rd += ACGT[rng.integers(0, 4, n - len(rd))].tobytes()
+        for x in rng.integers(0, n, n // 50):
+            rd[x] = ACGT[rng.integers(0, 4)]
+        reads.append(bytes(rd))
+        read_hap.append(h)
+    reads.append(haps[3][100:500] + ACGT[rng.integers(0, 4, 900)].tobytes())  # longer than its haplotype
+    read_hap.append(3)
+    reads.append(b"A" * 1100)                                                   # poly-A against a random one
+    read_hap.append(2)
+    got = check(haps, reads, read_hap, assemble)
+    assert got[-1][0] is not None
+    served = rv.sw_align_batch(haps, reads, read_hap, assemble, served=True)
+    assert served == got
+
+
 def test_limits():
     say("limit errors")
-    with pytest.raises(rv.EngineError) as e:
-        rv.sw_align_batch([b"A" * 1200], [b"A" * 1025], [0])
-    assert e.value.code == rv.RVC_E_LIMIT
     with pytest.raises(rv.EngineError) as e:
         rv.sw_align_batch([b"ACGT"], [b"AC"], [1])
     assert e.value.code == rv.RVC_E_INVALID
@@ -275,19 +300,19 @@ def test_served_errors_stay_with_their_call():
         except rv.EngineError as e:
             errs[x] = e.code
 
-    def too_long_read(x):
+    def bad_index(x):
         try:
-            rv.sw_align_batch([b"A" * 1200], [b"A" * 1030], [0], served=True)
+            rv.sw_align_batch([b"A" * 1200], [b"A" * 1030], [1], served=True)
         except rv.EngineError as e:
             errs[x] = e.code
 
     th = [threading.Thread(target=ok, args=(x,)) for x in range(6)]
-    th += [threading.Thread(target=too_long_cigar, args=(6,)), threading.Thread(target=too_long_read, args=(7,))]
+    th += [threading.Thread(target=too_long_cigar, args=(6,)), threading.Thread(target=bad_index, args=(7,))]
     for t in th:
         t.start()
     for t in th:
         t.join()
-    assert errs[6] == rv.RVC_E_LIMIT and errs[7] == rv.RVC_E_LIMIT
+    assert errs[6] == rv.RVC_E_LIMIT and errs[7] == rv.RVC_E_INVALID
     for x in range(6):
         p = good[x]
         assert out[x] == [oracle.sw_align(h, r) for h, r in p]

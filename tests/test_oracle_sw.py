@@ -74,6 +74,21 @@ def test_matches_reference_aligner_edges():
 
 
 @pytest.mark.skipif(oracle.ref_sw() is None, reason="oracle/_ref/libref_sw.so is not built")
+def test_long_reads_match_reference_aligner():
+    # reads of 1025-2047 bases (the device's 2048-row shape) against the
+    # reference library, both score sets
+    rng = np.random.default_rng(2047)
+    for n1, n2 in ((2047, 1025), (1500, 2047), (2047, 2047), (900, 1400)):
+        hap = ACGT[rng.integers(0, 4, n1)].tobytes()
+        o = int(rng.integers(0, max(1, n1 - n2)))
+        read = bytearray(hap[o:o + n2]) + ACGT[rng.integers(0, 4, max(0, n2 - (n1 - o)))].tobytes()
+        for x in rng.integers(0, n2, n2 // 40):
+            read[x] = ACGT[rng.integers(0, 4)]
+        for assemble in (False, True):
+            assert oracle.sw_align(hap, bytes(read), assemble) == oracle.ref_sw_align(hap, bytes(read), assemble)
+
+
+@pytest.mark.skipif(oracle.ref_sw() is None, reason="oracle/_ref/libref_sw.so is not built")
 def test_empty_sequences_match_a_fresh_reference_aligner():
     # An empty haplotype or read is not refused by Rovaca's aligner (only a
     # length >= 2048 is, sw_core_template.h:356-361). Its instance keeps the
