@@ -471,7 +471,7 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
   for (int b = 0; b < kFp32Bins; ++b) {
     int W, RPL;
     fp32_variant_shape(b % kFp32Variants, &W, &RPL);
-    P.occ[b] = (uint8_t)pol.occupancy(W, RPL, b >= kFp32Variants);
+    P.occ[b] = (uint8_t)pol.occupancy(W, RPL, bin_class(b) == kClassTable);
   }
   std::vector<std::vector<Tile>>& bins = W.bins;
   std::vector<std::vector<uint32_t>>& bin_cost = W.bin_cost;
@@ -501,6 +501,9 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
       return (g.read_off[a + 1] - g.read_off[a]) > (g.read_off[b + 1] - g.read_off[b]);
     });
+    // range of the region's gap-open qualities (insertion and deletion) and
+    // its smallest gap-continuation quality: they fix the region's form
+    uint8_t gop_lo = 127, gop_hi = 0, gcp_lo = 127;
     for (uint32_t sx = 0; sx < g.n_reads; ++sx) {
       const uint32_t x = order[sx];
       const uint32_t len = (uint32_t)(g.read_off[x + 1] - g.read_off[x]);
@@ -517,40 +520,44 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
         P.read_prep[gi] = kPrepOn | ((uint32_t)(prep->pcr & 3) << 16) | ((uint32_t)(prep->min_q & 255) << 8) |
                           (uint32_t)prep->mq[x];
         // gap open 45, or down to 10 under a PCR model; continuation 10
-        P.gop_lo = std::min<uint8_t>(P.gop_lo, prep->pcr != 0 ? 10 : 45);
-        P.gop_hi = std::max<uint8_t>(P.gop_hi, 45);
-        P.gcp_lo = std::min<uint8_t>(P.gcp_lo, 10);
+        gop_lo = std::min<uint8_t>(gop_lo, prep->pcr != 0 ? 10 : 45);
+        gop_hi = std::max<uint8_t>(gop_hi, 45);
+        gcp_lo = std::min<uint8_t>(gcp_lo, 10);
       } else {
         std::memcpy(&P.ri[rcur], g.i + o, len);
         std::memcpy(&P.rd[rcur], g.d + o, len);
         std::memcpy(&P.rc[rcur], g.c + o, len);
-        uint8_t lo = P.gop_lo, hi = P.gop_hi, clo = P.gcp_lo;
+        uint8_t lo = gop_lo, hi = gop_hi, clo = gcp_lo;
         for (uint32_t y = 0; y < len; ++y) {
           const uint8_t gi_ = g.i[o + y] & 127, gd_ = g.d[o + y] & 127, gc_ = g.c[o + y] & 127;
           lo = std::min(lo, std::min(gi_, gd_));
           hi = std::max(hi, std::max(gi_, gd_));
           clo = std::min(clo, gc_);
         }
-        P.gop_lo = lo;
-        P.gop_hi = hi;
-        P.gcp_lo = clo;
+        gop_lo = lo;
+        gop_hi = hi;
+        gcp_lo = clo;
       }
       rcur += align_up(len, kSeqAlign);
     }
     // tiles: one haplotype x up to 64/W reads of similar length, for the
     // haplotypes without N (prior-table kernel) and then those with N
-    // (compare kernel), whose shapes may differ
+    // (compare kernel), whose shapes may differ; all in the plain class where
+    // the region's gap qualities rule the scaled form out
+    const int X = scale_exponent(gop_lo, gop_hi, gcp_lo, pol.fold);
+    const bool plain = X == kPlainExponent;
     bool kinds[2] = {false, false};
     for (uint32_t h = 0; h < g.n_haps; ++h)
       if (P.hap_len[L.hap0 + h] > 0) kinds[hap_has_n[h]] = true;
     for (uint8_t kind = 0; kind < 2; ++kind) {
       if (!kinds[kind]) continue;
+      const int cls = plain ? kClassPlain : kind ? kClassN : kClassTable;
       uint32_t sx = 0;
       while (sx < g.n_reads) {
         const uint32_t R0 = P.read_len[L.read0 + sx];
         if (R0 == 0) break;  // the rest are empty reads
         int W, RPL;
-        pol.pick(R0, &W, &RPL, kind != 0);
+        pol.pick(R0, &W, &RPL, cls != kClassTable);
         const uint32_t nseg_max = 64u / (uint32_t)W;
         uint32_t nseg = 0;
         while (nseg < nseg_max && sx + nseg < g.n_reads && P.read_len[L.read0 + sx + nseg] > 0) ++nseg;
@@ -568,12 +575,12 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
             cost += P.hap_len[L.hap0 + h + nh] + (uint32_t)W - 1;
             ++nh;
           }
-          const int v = v0 + (kind ? 0 : kFp32Variants);
+          const int v = v0 + cls * kFp32Variants;
           Tile t;
           t.hap = L.hap0 + h;
           t.read0 = L.read0 + sx;
           t.out0 = (uint32_t)(L.pair0 + (uint64_t)sx * g.n_haps + h);
-          t.meta = tile_meta(nseg, g.n_haps, nh);
+          t.meta = tile_meta(nseg, g.n_haps, nh, (uint32_t)X);
           bins[v].push_back(t);
           bin_cost[v].push_back(cost);
           P.bin_units[v] += nh;
@@ -581,7 +588,7 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
             const uint32_t hl = P.hap_len[L.hap0 + hh];
             for (uint32_t s = 0; s < nseg; ++s) {
               const uint32_t rl = P.read_len[L.read0 + sx + s];
-              P.fp64_cap[(kind ? 0 : kNumFp64Bins) + fp64_bin(rl)] += 1;
+              P.fp64_cap[cls * kNumFp64Bins + fp64_bin(rl)] += 1;
               P.n_pairs += 1;
               P.n_cells += (uint64_t)rl * hl;
               P.variant_cells[v] += (uint64_t)rl * hl;
@@ -600,30 +607,29 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
     append_largest_first(bins[v], bin_cost[v], P.tiles);
   }
   P.bin_start[kFp32Bins] = (uint32_t)P.tiles.size();
-  P.rescale = pol.fold;
   P.fp64_base.assign(kNumFp64Lists + 1, 0);
   for (int b = 0; b < kNumFp64Lists; ++b) P.fp64_base[b + 1] = P.fp64_base[b] + P.fp64_cap[b];
   return RVC_OK;
 }
 
-int scaled_exponent(const HostPack& P) {
+int scale_exponent(int gop_lo, int gop_hi, int gcp_lo, bool allow_scaled) {
   // Gap open and continuation >= 6 keep tMM >= 1/2 and tDD <= 1/4. I~ = I/tMI
   // exceeds M by at most about the largest over the smallest gap-open
   // probability, 10^(spread/10) (at most 2^16 for a spread of 48): X lowers
   // 2^120 by that many bits plus 2, so that no cell holds more than in the
-  // 2^120 scale of the other forms.
-  if (!P.rescale) return 0;
-  if (P.gop_hi < P.gop_lo) return 118;  // no read rows
-  const int spread = P.gop_hi - P.gop_lo;
-  if (P.gop_lo < 6 || P.gcp_lo < 6 || spread > 48) return 0;
+  // 2^120 scale of the plain form.
+  if (!allow_scaled) return kPlainExponent;
+  if (gop_hi < gop_lo) return 118;  // no read rows
+  const int spread = gop_hi - gop_lo;
+  if (gop_lo < 6 || gcp_lo < 6 || spread > 48) return kPlainExponent;
   return 118 - (int)std::ceil(spread * 0.33219280948873623);
 }
 
 uint32_t fp32_grid(const HostPack& P, int v) {
   int W, RPL;
   fp32_variant_shape(v % kFp32Variants, &W, &RPL);
-  const bool sc = scaled_exponent(P) > 0;
-  const bool pt = v >= kFp32Variants && P.prior_table && sc;
+  const bool sc = bin_class(v) != kClassPlain;
+  const bool pt = bin_class(v) == kClassTable && P.prior_table;
   const int occ = pt ? P.occ[v] : P.occ[v % kFp32Variants];
   int per_cu = std::max(1, fp32_blocks_per_cu(W, RPL, sc, occ, pt));
   if (P.waves_per_cu) per_cu = std::min<int>(per_cu, (int)P.waves_per_cu);
@@ -642,12 +648,12 @@ namespace {
 // Appends the pieces of tile t with at most 'run' haplotypes each.
 template <typename F>
 void split_tile(const Tile& t, uint32_t run, F&& emit) {
-  const uint32_t nh = tile_nh(t.meta), nseg = tile_nseg(t.meta), stride = tile_stride(t.meta);
+  const uint32_t nh = tile_nh(t.meta);
   for (uint32_t h = 0; h < nh; h += run) {
     Tile p = t;
     p.hap = t.hap + h;
     p.out0 = t.out0 + h;
-    p.meta = tile_meta(nseg, stride, std::min(run, nh - h));
+    p.meta = tile_with_nh(t.meta, std::min(run, nh - h));
     emit(p);
   }
 }
@@ -845,10 +851,6 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
     }
     P.n_pairs += A.n_pairs;
     P.n_cells += A.n_cells;
-    P.rescale = P.rescale && A.rescale;
-    P.gop_lo = std::min(P.gop_lo, A.gop_lo);
-    P.gop_hi = std::max(P.gop_hi, A.gop_hi);
-    P.gcp_lo = std::min(P.gcp_lo, A.gcp_lo);
     P.waves_per_cu = A.waves_per_cu;
     P.prior_table = A.prior_table;
     P.hap_run = A.hap_run;
@@ -1054,14 +1056,13 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed, const uint32_t* rescued) {
   };
   int rc = mark(0);
   if (rc) return rc;
-  const int X = scaled_exponent(P);
-  const bool sc = X > 0;
   for (int v = 0; v < kFp32Bins; ++v) {
     const uint32_t n = P.bin_start[v + 1] - P.bin_start[v];
     if (n == 0) continue;
     int W, RPL;
     fp32_variant_shape(v % kFp32Variants, &W, &RPL);
-    const bool pt = v >= kFp32Variants && P.prior_table && sc;
+    const bool sc = bin_class(v) != kClassPlain;
+    const bool pt = bin_class(v) == kClassTable && P.prior_table;
     Fp32Args a;
     a.s = seqs;
     a.t = g_engine.tables;
@@ -1073,9 +1074,7 @@ int DeviceBatch::enqueue(hipStream_t s, bool timed, const uint32_t* rescued) {
     a.rescue = rescue;
     a.rescue_count = rescue_count;
     a.rescue_base = rescue_base;
-    a.rescue_list0 = v >= kFp32Variants ? (uint32_t)kNumFp64Bins : 0u;
-    a.init_scale = sc ? std::ldexp(1.f, X) : 0x1p120f;
-    a.back_scale = sc ? std::ldexp(1.f, 120 - X) : 1.f;
+    a.rescue_list0 = (uint32_t)(bin_class(v) * kNumFp64Bins);
     const int occ = pt ? P.occ[v] : P.occ[v % kFp32Variants];
     const int grid = (int)std::min<uint64_t>(n, fp32_grid(P, v));
     hipError_t e = launch_fp32(W, RPL, sc, occ, pt, grid, s, a);
@@ -1101,8 +1100,6 @@ int DeviceBatch::enqueue_fp64(hipStream_t s, bool timed, const uint32_t* rescued
   const HostPack& P = *pack;
   uint32_t* heads = counters;
   uint32_t* rescue_count = counters + kFp32Bins + kNumFp64Lists;
-  const int X = scaled_exponent(P);
-  const bool sc = X > 0;
   int rc = RVC_OK;
   auto mark = [&](int idx) -> int {
     if (!timed) return RVC_OK;
@@ -1120,7 +1117,9 @@ int DeviceBatch::enqueue_fp64(hipStream_t s, bool timed, const uint32_t* rescued
     const uint64_t n_list = rescued ? rescued[l] : P.fp64_cap[l];
     if (n_list == 0) continue;
     const int b = l % kNumFp64Bins;
-    const bool pt = l >= kNumFp64Bins && P.prior_table && sc;
+    const int cls = l / kNumFp64Bins;
+    const bool sc = cls != kClassPlain;
+    const bool pt = cls == kClassTable && P.prior_table;
     Fp64Args a;
     a.s = seqs;
     a.t = g_engine.tables;
@@ -1129,9 +1128,6 @@ int DeviceBatch::enqueue_fp64(hipStream_t s, bool timed, const uint32_t* rescued
     a.head = heads + kFp32Bins + l;
     a.out = out;
     a.flags = flags;
-    // the FP64 scale 2^1020 lowered like the FP32 one
-    a.init_scale = sc ? std::ldexp(1.0, X + 900) : 0x1p1020;
-    a.back_scale = sc ? std::ldexp(1.0, 120 - X) : 1.0;
     const int per_cu = std::max(1, fp64_blocks_per_cu(b, pt, sc));
     int W, RPL;
     fp64_bin_shape(b, &W, &RPL);

@@ -55,11 +55,24 @@ int engine_shutdown();
 int validate_region(const rvc_region* r);
 
 // FP32 kernel shapes: W in {16, 32, 64, 8} x RPL in 1..kMaxRpl (W = 64 up to
-// 13, W = 16 and 32 up to 16, W = 8 up to 19). Tile bins are [table][shape]:
-// table = 1 for haplotypes without N (prior-table kernel allowed), 0
-// otherwise.
+// 13, W = 16 and 32 up to 16, W = 8 up to 19). Tile bins are [class][shape],
+// the class fixed per region: kClassN for haplotypes with N (compare kernel,
+// scaled form), kClassTable for those without (prior-table kernel, scaled
+// form), kClassPlain for every haplotype of a region whose gap qualities rule
+// the scaled form out (compare kernel, plain form). A region's results thus
+// never depend on the other regions of its batch. The FP64 rescue lists are
+// [class][FP64 bin] in the same way.
+constexpr int kClassN = 0, kClassTable = 1, kClassPlain = 2, kNumClasses = 3;
 constexpr int kFp32Variants = 4 * kMaxRpl;
-constexpr int kFp32Bins = 2 * kFp32Variants;
+constexpr int kFp32Bins = kNumClasses * kFp32Variants;
+static_assert(kNumFp64Lists == kNumClasses * kNumFp64Bins, "one FP64 list per class and bin");
+inline int bin_class(int v) { return v / kFp32Variants; }
+// Scale exponent of the plain form: row -1 at 2^120, nothing to bring back.
+constexpr int kPlainExponent = 120;
+// Row -1 scale exponent X of a region (2^X in place of 2^120), from the
+// range of its gap-open qualities and its smallest gap-continuation quality,
+// or kPlainExponent where they rule the scaled form out (or the policy does).
+int scale_exponent(int gop_lo, int gop_hi, int gcp_lo, bool allow_scaled);
 int fp32_variant_index(int W, int RPL);
 void fp32_variant_shape(int v, int* W, int* RPL);
 int fp32_blocks_per_cu(int W, int RPL, bool xf, int occ, bool pt);
@@ -140,9 +153,6 @@ struct HostPack {
   uint64_t n_pairs_slots = 0;       // sum of n_haps * n_reads
   uint64_t n_pairs = 0;             // pairs with two non-empty sides
   uint64_t n_cells = 0;
-  bool rescale = true;              // the policy allows the scaled forms
-  uint8_t gop_lo = 127, gop_hi = 0;  // range of the gap-open qualities (insertion and deletion)
-  uint8_t gcp_lo = 127;              // smallest gap-continuation quality
   uint32_t waves_per_cu = 0;
   uint8_t occ[kFp32Bins] = {};            // occupancy target per FP32 bin
   uint64_t variant_cells[kFp32Bins] = {};  // cells computed by each FP32 bin
@@ -160,10 +170,6 @@ struct HostPack {
   // one region after another on a thread then allocates nothing new).
   void reset();
 };
-
-// Row -1 scale exponent X of the FP32 pass's scaled form (2^X in place of
-// 2^120), or 0 where the batch's gap qualities rule that form out.
-int scaled_exponent(const HostPack& P);
 
 // Element counts of a batch's device arrays.
 struct PackSizes {

@@ -59,19 +59,25 @@ struct Tile {
   uint32_t meta;  // bits 0-3: segments in use; bits 4-16: stride; bits 17-23: nh
 };
 constexpr uint32_t kMaxHapRun = 64;
-__host__ __device__ inline uint32_t tile_meta(uint32_t nseg, uint32_t stride, uint32_t nh) {
-  return nseg | (stride << 4) | (nh << 17);
+// x: the region's scale exponent X (row -1 scale 2^X, sum brought back by
+// 2^(120 - X)); 120 in the plain form
+__host__ __device__ inline uint32_t tile_meta(uint32_t nseg, uint32_t stride, uint32_t nh, uint32_t x) {
+  return nseg | (stride << 4) | (nh << 17) | (x << 24);
 }
 __host__ __device__ inline uint32_t tile_nseg(uint32_t meta) { return meta & 0xfu; }
 __host__ __device__ inline uint32_t tile_stride(uint32_t meta) { return (meta >> 4) & 0x1fffu; }
-__host__ __device__ inline uint32_t tile_nh(uint32_t meta) { return meta >> 17; }
+__host__ __device__ inline uint32_t tile_nh(uint32_t meta) { return (meta >> 17) & 0x7fu; }
+__host__ __device__ inline uint32_t tile_x(uint32_t meta) { return meta >> 24; }
+__host__ __device__ inline uint32_t tile_with_nh(uint32_t meta, uint32_t nh) {
+  return (meta & ~(0x7fu << 17)) | (nh << 17);
+}
 
 // One pair queued for the FP64 recomputation.
 struct RescueEntry {
   uint32_t hap;
   uint32_t read;
   uint32_t out;
-  uint32_t pad;
+  uint32_t x;  // the pair's scale exponent, as in its tile
 };
 
 // FP64 bins: (segment width, rows per lane) chosen from the read length so
@@ -86,10 +92,11 @@ __host__ __device__ inline int fp64_bin(uint32_t R) {
   if (R <= 256) return 3;   // W=32, RPL 8
   return 4;                 // W=64, RPL 12
 }
-// Rescue lists: one per FP64 bin for pairs whose haplotype has N (list b)
-// and one per bin for the others (list kNumFp64Bins + b), which the
-// prior-table FP64 kernel can take.
-constexpr int kNumFp64Lists = 2 * kNumFp64Bins;
+// Rescue lists: per FP64 bin, one for pairs whose haplotype has N (list b),
+// one for the others of scaled-form regions (list kNumFp64Bins + b), which
+// the prior-table FP64 kernel can take, and one for the pairs of plain-form
+// regions (list 2 kNumFp64Bins + b).
+constexpr int kNumFp64Lists = 3 * kNumFp64Bins;
 inline void fp64_bin_shape(int bin, int* W, int* RPL) {
   static const int kW[kNumFp64Bins] = {16, 16, 32, 32, 64}, kR[kNumFp64Bins] = {4, 8, 5, 8, 12};
   *W = kW[bin];
@@ -108,8 +115,6 @@ struct Fp32Args {
   uint32_t* rescue_count;    // kNumFp64Lists counters
   const uint32_t* rescue_base;
   uint32_t rescue_list0;     // this launch's pairs go to list rescue_list0 + fp64_bin(R)
-  float init_scale;          // row -1 deletion value times H: 2^120, or 2^X in the scaled form
-  float back_scale;          // 2^(120 - X): brings the scaled form's sum to the 2^120 scale
 };
 
 struct Fp64Args {
@@ -120,8 +125,6 @@ struct Fp64Args {
   uint32_t* head;            // this launch's work-queue counter
   double* out;
   uint8_t* flags;
-  double init_scale;         // row -1 deletion value times H: 2^1020, or 2^X in the scaled form
-  double back_scale;         // 2^(1020 - X)
 };
 
 // Device read preprocessing (for regions handed over with raw qualities):

@@ -208,11 +208,18 @@ __device__ __forceinline__ double i_update<double>(double tmi, double um, double
 //          coefficient arrays hold tmi = rho, tmd = delta, tii = c, tdd = tDD
 //          and the priors are multiplied by tMM (row_params). I~ exceeds M by
 //          up to the ratio of the largest to the smallest gap-open
-//          probability in the batch, so the host lowers the row -1 scale
-//          2^120 by that ratio (Fp32Args::init_scale) and the sum is scaled
-//          back before the rescue test. Needs tMM > 0 and tMI, tMD > 0,
+//          probability in the region, so the host lowers the row -1 scale
+//          2^120 by that ratio to 2^X (X per region, carried by each tile and
+//          rescue entry) and the sum is scaled back by 2^(120 - X), exactly,
+//          before the rescue test. Needs tMM > 0 and tMI, tMD > 0,
 //          which the host's range check on the gap qualities ensures.
 constexpr int kPlain = 0, kScaled = 1;
+
+// 2^e, exactly, for e in the normal range of the type.
+__device__ __forceinline__ float pow2f(int e) { return __builtin_bit_cast(float, (uint32_t)(e + 127) << 23); }
+__device__ __forceinline__ double pow2d(int e) {
+  return __builtin_bit_cast(double, (uint64_t)(e + 1023) << 52);
+}
 
 // New M, I, D of one cell from the row above at column j-1 (d*), the row
 // above at column j (u*) and the same row at column j-1 (l*).
@@ -375,7 +382,9 @@ __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, cons
 
 // Result of one FP32 pair (output slot o): log10 likelihood, or a place on
 // the FP64 list.
-__device__ __forceinline__ void finish_fp32(const Fp32Args& a, uint32_t o, uint32_t hap, uint32_t read, float S) {
+// x is the pair's scale exponent (S already brought back to the 2^120 scale).
+__device__ __forceinline__ void finish_fp32(const Fp32Args& a, uint32_t o, uint32_t hap, uint32_t read, uint32_t x,
+                                            float S) {
   if (S >= 1e-28f && S <= 0x1.fffffep127f) {
     a.out[o] = log10((double)S) - (double)36.1236000061f;
     a.flags[o] = 1;
@@ -383,7 +392,7 @@ __device__ __forceinline__ void finish_fp32(const Fp32Args& a, uint32_t o, uint3
     const int b = (int)a.rescue_list0 + fp64_bin(a.s.read_len[read]);
     const uint32_t idx = atomicAdd(a.rescue_count + b, 1u);
     RescueEntry e;
-    e.hap = hap; e.read = read; e.out = o; e.pad = 0;
+    e.hap = hap; e.read = read; e.out = o; e.x = x;
     a.rescue[a.rescue_base[b] + idx] = e;
   }
 }
@@ -407,14 +416,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     const bool active = seg < (int)tile_nseg(tl.meta);
     const uint32_t stride = tile_stride(tl.meta), nh = tile_nh(tl.meta);
     const uint32_t read = tl.read0 + (active ? (uint32_t)seg : 0u);
+    const uint32_t X = tile_x(tl.meta);
+    const float init = pow2f((int)X), back = pow2f(120 - (int)X);
     for (uint32_t h = 0; h < nh; ++h) {
       const uint32_t hap = tl.hap + h;
       const uint32_t H = a.s.hap_len[hap];
       __syncthreads();
       stage_hap(a.s, hap, H, hbuf + 64, lane, 64);
       __syncthreads();
-      const float S = forward<float, W, RPL, F>(a.s, a.t, hbuf + 64, k, H, H, read, a.init_scale);
-      if (active && k == W - 1) finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, S * a.back_scale);
+      const float S = forward<float, W, RPL, F>(a.s, a.t, hbuf + 64, k, H, H, read, init);
+      if (active && k == W - 1) finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, X, S * back);
     }
   }
 }
@@ -468,7 +479,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     if (t >= a.n_tiles) break;
     const Tile tl = a.tiles[t];
     const bool active = seg < (int)tile_nseg(tl.meta);
-    const uint32_t stride = tile_stride(tl.meta), nh = tile_nh(tl.meta);
+    const uint32_t stride = tile_stride(tl.meta), nh = tile_nh(tl.meta), X = tile_x(tl.meta);
     const uint32_t read = tl.read0 + (active ? (uint32_t)seg : 0u);
     __syncthreads();
     // per-row transition parameters (registers) and priors (LDS table), once
@@ -518,7 +529,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
       __syncthreads();
       if (h + 1 < nh) hap_words(hap + 1, words);
 
-      const float initD = a.init_scale / (float)H;
+      const float initD = pow2f((int)X) / (float)H;
       const float old_d = initD;
       float Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
 #pragma unroll
@@ -636,7 +647,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
         }
       }
       if (active && k == W - 1)
-        finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, S * a.back_scale);
+        finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, X, S * pow2f(120 - (int)X));
     }
   }
 }
@@ -695,7 +706,7 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
       }
     __syncthreads();
 
-    const double initD = a.init_scale / (double)H;
+    const double initD = pow2d((int)en.x + 900) / (double)H;
     const double sfac = sum_factor<double, W, RPL, kScaled>(a.s, ph2pr, en.read);
     double Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
 #pragma unroll
@@ -744,7 +755,7 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
       uD = lane_down_keep<W>(Da[RPL - 1], uD, initD, k);
     }
     if (active && k == W - 1) {
-      a.out[en.out] = log10(S * a.back_scale) - 307.050595577260822;
+      a.out[en.out] = log10(S * pow2d(120 - (int)en.x)) - 307.050595577260822;
       a.flags[en.out] = 3;
     }
   }
@@ -778,9 +789,9 @@ __global__ __launch_bounds__(64) void k_fp64(Fp64Args a) {
     stage_hap(a.s, en.hap, H, &hbuf[seg][64], k, W);
     __syncthreads();
     const double S = forward<double, W, RPL, SC ? kScaled : kPlain>(a.s, a.t, &hbuf[seg][64], k, H, Hmax, en.read,
-                                                                    a.init_scale);
+                                                                    pow2d((int)en.x + 900));
     if (active && k == W - 1) {
-      a.out[en.out] = log10(S * a.back_scale) - 307.050595577260822;
+      a.out[en.out] = log10(S * pow2d(120 - (int)en.x)) - 307.050595577260822;
       a.flags[en.out] = 3;
     }
   }

@@ -187,6 +187,51 @@ def test_gap_quality_forms(lo, hi, gcp, form):
     assert np.array_equal(alone[m], got[m])
 
 
+def test_mixed_forms_equal_alone():
+    say("plain-form regions among scaled ones: each equals the region alone, batched and served")
+    base = [r for r in workload.batched_regions(10, seed=61) if r["n_haps"] * r["n_reads"] < 6000]
+    # two regions the scaled form cannot take (gap open 5; spread 49) and two
+    # it takes at other exponents than the batch's default regions (spreads
+    # 48, the lowest exponent, and 48 from gap open 30 up)
+    odd = [_gap_region(901, 5, 45, 10), _gap_region(902, 6, 55, 6), _gap_region(903, 6, 54, 6),
+           _gap_region(904, 30, 78, 20)]
+    regs = base[:4] + odd[:2] + base[4:7] + odd[2:] + base[7:]
+    alone = [rv.region_sync(r) for r in regs]
+    b = rv.Batch(regs)
+    b.run()
+    out, flags = b.results()
+    names = {k["name"] for k in b.kernels() if k["precision"] == 32}
+    assert any(",plain," in n for n in names) and any(",scaled," in n for n in names), names
+    for x, reg in enumerate(regs):
+        a, af = alone[x]
+        m = b.region_result(out, x)
+        assert np.array_equal(np.isnan(a), np.isnan(m)), x
+        assert np.array_equal(a[~np.isnan(a)], m[~np.isnan(m)]), x
+        assert np.array_equal(af, b.region_result(flags, x)), x
+    b.close()
+    # the serving loop batches concurrent submissions together
+    results = [None] * len(regs)
+    errors = []
+
+    def worker(idx):
+        try:
+            for x in idx:
+                t = rv.submit(regs[x])
+                results[x] = rv.wait(t, regs[x])[0]
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    th = [threading.Thread(target=worker, args=(list(range(k, len(regs), 8)),)) for k in range(8)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not errors
+    for (a, _), m in zip(alone, results):
+        assert np.array_equal(np.isnan(a), np.isnan(m))
+        assert np.array_equal(a[~np.isnan(a)], m[~np.isnan(m)])
+
+
 @pytest.mark.parametrize("R", [63, 64, 65, 128, 129, 160, 161, 256, 512, 767, 768])
 def test_rescue_bin_edges(R):
     say(f"FP64 rescue at read length {R} (edges of the rescue bins' row counts)")
