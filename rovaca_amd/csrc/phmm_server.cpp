@@ -38,6 +38,13 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 constexpr int kSlots = 3;
+// Bounds on one launch: device bytes of its requests and pair slots. A slot
+// stops taking queued requests at either (a request larger than both still
+// goes alone). The slot buffers are sized for a launch at the bounds when the
+// server starts, so that launches under load reallocate nothing; a launch of
+// 8M pairs runs for tens of milliseconds, long enough to keep the device busy.
+constexpr uint64_t kLaunchBytes = 256ull << 20;
+constexpr uint64_t kLaunchPairs = 8ull << 20;
 
 // Pinned host memory for one launch's results in their device layout: log10
 // L per pair slot, the flags (rounded up to 8 bytes) and the counters, whose
@@ -344,11 +351,13 @@ void slot_main(Slot* sp) {
              (!S.pending.empty() && S.resident + S.pending.front()->resident <= g_engine.hbm_budget);
     });
     if (S.pending.empty()) break;  // stopping and drained
-    uint64_t add = 0;
+    uint64_t add = 0, pairs = 0;
     while (!S.pending.empty() && s.reqs.size() < g_engine.max_batch_regions) {
       Request* r = S.pending.front();
       if (S.resident + add + r->resident > g_engine.hbm_budget) break;
+      if (!s.reqs.empty() && (add + r->resident > kLaunchBytes || pairs + r->pack.n_pairs_slots > kLaunchPairs)) break;
       add += r->resident;
+      pairs += r->pack.n_pairs_slots;
       s.reqs.push_back(r);
       S.pending.pop_front();
     }
@@ -444,14 +453,16 @@ int server_start() {
     }
     s.batch.stream = s.stream;
     s.batch.own_stream = false;
-    // starting sizes that hold a few hundred typical regions, so that the
-    // first launches do not reallocate (each reallocation waits for the device)
-    const size_t arena = std::min<uint64_t>(g_engine.hbm_budget / kSlots + (16u << 20), 512ull << 20);
+    // sizes that hold a launch at the bounds (the arena as launch_slot would
+    // grow it for one), so that launches do not reallocate (each
+    // reallocation waits for the device)
+    const uint64_t cap = std::min<uint64_t>(g_engine.hbm_budget, kLaunchBytes);
+    const size_t arena = cap + cap / 4 + (16u << 20);
     if (hipMalloc(&s.mem, arena) == hipSuccess) s.bytes = arena;
     else s.mem = nullptr;
-    s.staging.reserve(128u << 20);
-    // a result buffer per slot for launches of about 500 typical regions
-    S->pool->add(2u << 20);
+    s.staging.reserve(cap);
+    // a result buffer per slot
+    S->pool->add(kLaunchPairs);
   }
   g_srv = S;
   for (Slot& s : S->slots) s.thr = std::thread(slot_main, &s);
