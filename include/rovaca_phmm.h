@@ -227,6 +227,10 @@ typedef struct rvc_batch_opts {
   uint32_t untimed;                 /* 1 = eager runs queue the launches without the  */
                                     /* per-launch events (rvc_phmm_batch_kernels and  */
                                     /* the FP32/FP64 split are then not measured)     */
+  uint32_t shape_fold;              /* 0 = tiles of a shape with few cells are swept  */
+                                    /* by the next larger shape of their width (at    */
+                                    /* most a third more rows; same results), 1 = every */
+                                    /* tile keeps its own shape                       */
 } rvc_batch_opts;
 
 typedef struct rvc_batch_info {

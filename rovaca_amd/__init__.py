@@ -71,7 +71,7 @@ class RvcBatchOpts(ctypes.Structure):
     _fields_ = [("persistent_waves_per_cu", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
                 ("fp32_fold", ctypes.c_uint32), ("rpl_round", ctypes.c_uint32), ("occupancy", ctypes.c_uint32),
                 ("segment_width", ctypes.c_uint32), ("prior_table", ctypes.c_uint32),
-                ("hap_run", ctypes.c_uint32), ("untimed", ctypes.c_uint32)]
+                ("hap_run", ctypes.c_uint32), ("untimed", ctypes.c_uint32), ("shape_fold", ctypes.c_uint32)]
 
 
 class RvcBatchInfo(ctypes.Structure):
@@ -299,7 +299,7 @@ class Batch:
     """Many regions packed, uploaded and launched together."""
 
     def __init__(self, regions, use_graph=False, fp32_fold=0, rpl_round=0, waves_per_cu=0, occupancy=0,
-                 segment_width=0, prior_table=0, hap_run=0, untimed=False):
+                 segment_width=0, prior_table=0, hap_run=0, untimed=False, shape_fold=True):
         self._regions = regions
         arr = (RvcRegion * len(regions))()
         for x, r in enumerate(regions):
@@ -315,6 +315,7 @@ class Batch:
         opts.prior_table = prior_table
         opts.hap_run = hap_run
         opts.untimed = 1 if untimed else 0
+        opts.shape_fold = 0 if shape_fold else 1
         h = ctypes.c_void_p()
         _check(lib().rvc_phmm_batch_create(arr, len(regions), ctypes.byref(opts), ctypes.byref(h)))
         self.h = h

@@ -374,6 +374,7 @@ int rvc_phmm_batch_create(const rvc_region* regions, uint32_t n_regions, const r
       if (opts->segment_width == 8 || opts->segment_width == 16 || opts->segment_width == 32 ||
           opts->segment_width == 64)
         pol.force_w = (uint8_t)opts->segment_width;
+      if (opts->shape_fold == 1) pol.fold_small_bins = false;
       if (opts->prior_table == 1) pol.prior_table = true;
       if (opts->prior_table == 2) pol.prior_table = false;
       if (opts->hap_run) {

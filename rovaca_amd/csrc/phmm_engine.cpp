@@ -468,6 +468,7 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
   P.prior_table = pol.prior_table;
   P.hap_run = std::max<uint32_t>(1, std::min<uint32_t>(kMaxHapRun, pol.hap_run));
   P.hap_run_fixed = pol.hap_run_fixed;
+  P.no_fold = !pol.fold_small_bins;
   for (int b = 0; b < kFp32Bins; ++b) {
     int W, RPL;
     fp32_variant_shape(b % kFp32Variants, &W, &RPL);
@@ -659,12 +660,48 @@ void split_tile(const Tile& t, uint32_t run, F&& emit) {
 }
 }  // namespace
 
+void fold_bins(HostPack& P, int (&to)[kFp32Bins]) {
+  // A bin of fewer cells than this runs well below the large bins' rate (on
+  // MI355X: bins of 2e9-1.4e10 cells at 5.0-6.2 k GCUPS, bins of 2e11+ at
+  // 9 k): its launch's ramp and drain, and runs of one haplotype per tile.
+  // The next larger shape of its class and width sweeps its tiles with at
+  // most a third more rows (padding rows, which leave the results as they
+  // are), at least as fast.
+  const uint64_t small = 80000000ull * (uint64_t)std::max(1, g_engine.n_cu);
+  for (int v = 0; v < kFp32Bins; ++v) to[v] = v;
+  for (int cls = 0; cls < kNumClasses; ++cls)
+    for (int w = 0; w < 4; ++w) {
+      int big = -1, big_rpl = 0;
+      for (int rpl = kMaxRpl; rpl >= 1; --rpl) {
+        const int v = cls * kFp32Variants + w * kMaxRpl + rpl - 1;
+        if (P.bin_units[v] == 0) continue;
+        if (big >= 0 && 3 * big_rpl <= 4 * rpl && P.variant_cells[v] < small) {
+          to[v] = big;
+          continue;
+        }
+        big = v;
+        big_rpl = rpl;
+      }
+    }
+  if (P.no_fold)
+    for (int v = 0; v < kFp32Bins; ++v) to[v] = v;
+  for (int v = 0; v < kFp32Bins; ++v)
+    if (to[v] != v) {
+      P.variant_cells[to[v]] += P.variant_cells[v];
+      P.bin_units[to[v]] += P.bin_units[v];
+      P.variant_cells[v] = 0;
+      P.bin_units[v] = 0;
+    }
+}
+
 void split_runs(HostPack& P) {
   PackScratch& S = pack_scratch();
   std::vector<Tile>& out = S.out;
   std::vector<uint32_t>& start = S.start;
   std::vector<Tile>& bin = S.bin;
   std::vector<uint32_t>& cost = S.cost;
+  int to[kFp32Bins];
+  fold_bins(P, to);
   out.clear();
   start.assign(kFp32Bins + 1, 0);
   for (int v = 0; v < kFp32Bins; ++v) {
@@ -674,13 +711,17 @@ void split_runs(HostPack& P) {
     fp32_variant_shape(v % kFp32Variants, &W, &RPL);
     bin.clear();
     cost.clear();
-    for (uint32_t x = P.bin_start[v]; x < P.bin_start[v + 1]; ++x)
-      split_tile(P.tiles[x], run, [&](const Tile& p) {
-        uint32_t c = 0;
-        for (uint32_t h = 0; h < tile_nh(p.meta); ++h) c += P.hap_len[p.hap + h] + (uint32_t)W - 1;
-        bin.push_back(p);
-        cost.push_back(c);
-      });
+    // this bin's own tiles and those of the bins folded into it
+    for (int u = 0; u < kFp32Bins; ++u) {
+      if (to[u] != v) continue;
+      for (uint32_t x = P.bin_start[u]; x < P.bin_start[u + 1]; ++x)
+        split_tile(P.tiles[x], run, [&](const Tile& p) {
+          uint32_t c = 0;
+          for (uint32_t h = 0; h < tile_nh(p.meta); ++h) c += P.hap_len[p.hap + h] + (uint32_t)W - 1;
+          bin.push_back(p);
+          cost.push_back(c);
+        });
+    }
     append_largest_first(bin, cost, out);
   }
   start[kFp32Bins] = (uint32_t)out.size();
@@ -858,15 +899,19 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
     P.any_prep = P.any_prep || A.any_prep;
     std::memcpy(P.occ, A.occ, sizeof P.occ);
   }
-  // tiles of the merged batch: each pack's runs split for the batch's size
+  // tiles of the merged batch: small bins folded into larger shapes, each
+  // pack's runs split for the batch's size
+  P.no_fold = n > 0 && packs[0]->no_fold;
+  int to[kFp32Bins];
+  fold_bins(P, to);
   uint32_t run[kFp32Bins];
   std::vector<uint64_t> bin_count(kFp32Bins, 0);
-  for (int v = 0; v < kFp32Bins; ++v) run[v] = run_for_bin(P, v, P.bin_units[v]);
+  for (int v = 0; v < kFp32Bins; ++v) run[v] = run_for_bin(P, to[v], P.bin_units[to[v]]);
   for (uint32_t x = 0; x < n; ++x) {
     const HostPack& A = *packs[x];
     for (int v = 0; v < kFp32Bins; ++v)
       for (uint32_t k = A.bin_start[v]; k < A.bin_start[v + 1]; ++k)
-        bin_count[v] += (tile_nh(A.tiles[k].meta) + run[v] - 1) / run[v];
+        bin_count[to[v]] += (tile_nh(A.tiles[k].meta) + run[v] - 1) / run[v];
   }
   for (int v = 0; v < kFp32Bins; ++v) z.n_tiles += bin_count[v];
   if (z.hap_words * 16 > 0xffffffffull || z.read_words * 16 > 0xffffffffull || z.n_pairs_slots > 0xffffffffull)
@@ -933,7 +978,7 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
       P.region.push_back(R);
     }
     for (int v = 0; v < kFp32Bins; ++v) {
-      Tile* t = tiles + bin_cur[v];
+      Tile* t = tiles + bin_cur[to[v]];
       for (uint32_t k = A.bin_start[v]; k < A.bin_start[v + 1]; ++k) {
         Tile b = A.tiles[k];
         b.hap += (uint32_t)hidx;
@@ -941,7 +986,7 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
         b.out0 += (uint32_t)pbase;
         split_tile(b, run[v], [&](const Tile& q) { *t++ = q; });
       }
-      bin_cur[v] = (uint32_t)(t - tiles);
+      bin_cur[to[v]] = (uint32_t)(t - tiles);
     }
     hw += aw;
     rw += bw;

@@ -100,6 +100,7 @@ struct FP32Policy {
   uint8_t force_w = 0;         // batch option: 16/32/64 lanes per read for reads <= 256
   uint32_t hap_run = 16;       // most haplotypes per tile (consecutive, of one region)
   bool hap_run_fixed = false;  // batch option: exactly hap_run per tile, whatever the batch size
+  bool fold_small_bins = true; // sweep the tiles of small shape bins with a larger shape (fold_bins)
   // cmp: for the compare-and-select kernel (haplotypes with N), which has no
   // 8-lane shape past 16 rows per lane (it would spill): those tiles keep
   // the 16-lane shape
@@ -159,6 +160,7 @@ struct HostPack {
   bool prior_table = true;                 // launch table bins with the prior-table kernel
   uint32_t hap_run = 1;                    // longest haplotype run a tile may keep
   bool hap_run_fixed = false;              // keep runs of hap_run whatever the batch size
+  bool no_fold = false;                    // every tile keeps its shape (no fold_bins)
   uint64_t bin_units[kFp32Bins] = {};      // (read group, haplotype) pairs per FP32 bin
   bool any_prep = false;                   // some reads are preprocessed on the device
   std::vector<uint8_t> read_raw;           // their bases as given, at the qualities' offsets
@@ -227,7 +229,12 @@ uint32_t fp32_grid(const HostPack& P, int v);
 // Haplotypes per tile for FP32 bin v: long runs amortise a tile's set-up,
 // but every persistent wave needs a few dozen tiles so that all end together.
 uint32_t run_for_bin(const HostPack& P, int v, uint64_t units);
-// Splits the pack's tiles into runs of run_for_bin haplotypes, largest first.
+// Where each FP32 bin's tiles go (to[v]): a bin of few cells into the next
+// larger non-empty shape of its class and width, if that has at most a third
+// more rows; moves the bins' cell and unit counts along.
+void fold_bins(HostPack& P, int (&to)[kFp32Bins]);
+// Splits the pack's tiles into runs of run_for_bin haplotypes, largest first
+// (after fold_bins).
 void split_runs(HostPack& P);
 
 // Copies a batch's results (device order, as downloaded) to input read order.

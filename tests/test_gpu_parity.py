@@ -408,6 +408,43 @@ def test_hap_runs_equal_alone(run):
     b.close()
 
 
+def test_small_bins_fold_into_larger_shapes():
+    say("tiles of small shape bins swept by the next larger shape: fewer launches, the same bits")
+    rng = np.random.default_rng(71)
+    regs = []
+    for x in range(8):
+        base = workload.batched_regions(1, seed=700 + x)[0]
+        H = base["n_haps"]
+        haps = [base["hap_bases"][base["hap_off"][h]:base["hap_off"][h + 1]] for h in range(H)]
+        haps = [h if len(h) >= 260 else np.concatenate([h, h, h])[:300] for h in haps]
+        reads, quals = [], []
+        for r in range(30):
+            n = int(rng.choice([100, 120, 130, 140, 150, 155, 180, 200, 250]))  # many shape bins
+            src = haps[int(rng.integers(0, H))]
+            o = int(rng.integers(0, len(src) - n + 1))
+            reads.append(src[o:o + n].copy())
+            quals.append(rng.integers(10, 41, n).astype(np.uint8))
+        regs.append(workload.make_region(haps, reads, quals))
+    runs = {}
+    for fold in (True, False):
+        b = rv.Batch(regs, shape_fold=fold)
+        b.run()
+        out, flags = b.results()
+        runs[fold] = ([b.region_result(out, x) for x in range(len(regs))],
+                      [b.region_result(flags, x) for x in range(len(regs))],
+                      [k for k in b.kernels() if k["precision"] == 32])
+        b.close()
+    (o1, f1, k1), (o0, f0, k0) = runs[True], runs[False]
+    assert len(k1) < len(k0), ([k["name"] for k in k1], [k["name"] for k in k0])
+    assert sum(k["cells"] for k in k1) == sum(k["cells"] for k in k0)
+    for a, b_, fa, fb in zip(o1, o0, f1, f0):
+        assert np.array_equal(np.isnan(a), np.isnan(b_)) and np.array_equal(a[~np.isnan(a)], b_[~np.isnan(b_)])
+        assert np.array_equal(fa, fb)
+    alone, _ = rv.region_sync(regs[0])
+    assert np.array_equal(np.isnan(alone), np.isnan(o1[0])) and np.array_equal(alone[~np.isnan(alone)],
+                                                                               o1[0][~np.isnan(o1[0])])
+
+
 @pytest.mark.parametrize("width", [8, 32])
 def test_segment_width_equals_default(width):
     say(f"{width}-lane segments give the same bits as the default shapes")
@@ -428,7 +465,7 @@ def test_segment_width_equals_default(width):
             haps[1] = haps[1].copy()
             haps[1][3] = ord("N")
         regs.append(workload.make_region(haps, reads, quals))
-    b = rv.Batch(regs, segment_width=width)
+    b = rv.Batch(regs, segment_width=width, shape_fold=False)
     b.run()
     out, flags = b.results()
     names = {k["name"] for k in b.kernels() if k["precision"] == 32}
@@ -847,7 +884,7 @@ def test_eight_lane_long_shapes_equal_default(lo, hi):
             reads.append(rd)
             quals.append(rng.integers(10, 41, n).astype(np.uint8))
         regs.append(workload.make_region(haps, reads, quals))
-    b = rv.Batch(regs, segment_width=8)
+    b = rv.Batch(regs, segment_width=8, shape_fold=False)  # each tile in its own shape
     b.run()
     out, flags = b.results()
     names = {k["name"] for k in b.kernels() if k["precision"] == 32}
