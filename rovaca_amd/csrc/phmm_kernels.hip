@@ -9,7 +9,9 @@
 // W = 16, wave_shr:1 for W = 32/64), which feeds the first row of the next
 // lane one step later; lane 0 of a segment takes the row -1 boundary
 // (M = I = 0, D = C/H) from the DPP 'old' operand. The haplotype is staged in
-// LDS as one code byte per base; lane k reads byte (s - k) each step.
+// LDS as one code byte per base; lane k reads byte (s - k) each step. The
+// FP32 prior-table kernel moves two values instead: the lane above mixes its
+// last row with the receiving lane's row-0 coefficients (sweep_step_xy).
 //
 // Rows past the end of a read are padding rows that carry the sum
 // M[R-1][j] + I[R-1][j] down unchanged (prior 0, I <- M + I), so the result
@@ -100,19 +102,6 @@ __device__ __forceinline__ float lane_down(float v, float old, int k) {
   }
 }
 
-// The same, where lane 0 of a segment keeps the value 'keep' already holds:
-// with a loop-carried 'keep' that starts as the boundary value, the DPP move
-// writes only lanes 1.. and no register copy is needed per step. W = 32 must
-// still select lane 32's boundary explicitly.
-template <int W>
-__device__ __forceinline__ float lane_down_keep(float v, float keep, float old, int k) {
-  if constexpr (W == 32) {
-    return lane_down<W>(v, old, k);
-  } else {
-    return __int_as_float(
-        __builtin_amdgcn_update_dpp(__float_as_int(keep), __float_as_int(v), kDownCtrl<W>, 0xf, 0xf, false));
-  }
-}
 
 template <int W>
 __device__ __forceinline__ double lane_down(double v, double old, int k) {
@@ -456,6 +445,36 @@ __device__ __forceinline__ void sweep_step_pt(const T (&M)[RPL], const T (&I)[RP
                tmd[r], tdd[r], nM[r], nI[r], nD[r]);
 }
 
+// One step of the prior-table sweep where row 0 of a lane takes its inputs
+// premixed by the lane above, with this lane's row-0 coefficients:
+//   Y = M' + rho*I~' + delta*D~'   (the lane above's last row, two steps old: the diagonal)
+//   X = M_up + c*I~_up            (one step old: the row above in this column)
+// so M[0] = prior * Y and I~[0] = X are exactly the operations of the full
+// cell, done by the sender, and a step moves two values between lanes
+// instead of three (M, I and D).
+template <int RPL>
+__device__ __forceinline__ void sweep_step_xy(const float (&M)[RPL], const float (&I)[RPL], const float (&D)[RPL],
+                                              float (&nM)[RPL], float (&nI)[RPL], float (&nD)[RPL], float yd,
+                                              float xu, const float* pr, const float (&tmm)[RPL],
+                                              const float (&tgm)[RPL], const float (&tmi)[RPL],
+                                              const float (&tii)[RPL], const float (&tmd)[RPL],
+                                              const float (&tdd)[RPL]) {
+  nM[0] = pr[0] * yd;
+  nI[0] = xu;
+  nD[0] = fmaf(tdd[0], D[0], M[0]);
+#pragma unroll
+  for (int r = 1; r < RPL; ++r)
+    cell<float, kScaled>(pr[r], M[r - 1], I[r - 1], D[r - 1], nM[r - 1], nI[r - 1], M[r], D[r], tmm[r], tgm[r],
+                         tmi[r], tii[r], tmd[r], tdd[r], nM[r], nI[r], nD[r]);
+}
+
+// v of lane (lane + delta) mod 64, through the LDS crossbar (ds_bpermute:
+// no LDS memory is touched, every lane reads its source lane's register).
+__device__ __forceinline__ float lane_from(float v, int delta) {
+  const int addr = ((int)(threadIdx.x + delta) & 63) << 2;
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+
 template <int W, int RPL, bool XF, int OCC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void k_fp32pt(Fp32Args a) {
   constexpr int NQ = (RPL + 3) / 4;
@@ -505,6 +524,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
         ptail[(c * TR + t) * 64 + lane] = (rc[r] < 0 || rc[r] == c) ? pm[r] : px[r];
       }
     }
+    // Row-0 coefficients of the lane below, which receives this lane's X and
+    // Y. A segment's last lane feeds the next segment's first lane (or wraps
+    // to lane 0), which must see row -1 (M = I = 0, D~ = initD): its last row
+    // is a padding row (M = 0, D~ held at initD from the start), so with
+    // c = rho = 0 it sends X = 0 and Y = delta * initD. (The lane permutes
+    // run in every lane: each reads its source lane's register.)
+    constexpr int SH = W == 8 ? 2 : 1;  // lane distance to the next position of a segment
+    const float cB = lane_from(tii[0], SH), rB = lane_from(tmi[0], SH);
+    const float dN = lane_from(tmd[0], SH);
+    const float cN = k == W - 1 ? 0.f : cB;
+    const float rN = k == W - 1 ? 0.f : rB;
     // this lane's words x = lane, lane + 64 of a haplotype's stream (64
     // zero columns in front); the next haplotype's are loaded while the
     // current one is swept
@@ -530,15 +560,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
       if (h + 1 < nh) hap_words(hap + 1, words);
 
       const float initD = pow2f((int)X) / (float)H;
-      const float old_d = initD;
       float Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
 #pragma unroll
       for (int r = 0; r < RPL; ++r) { Ma[r] = 0.f; Ia[r] = 0.f; Da[r] = 0.f; }
-      float uM = 0.f, uI = 0.f, uD = (k == 0) ? initD : 0.f;
-      float dM = 0.f, dI = 0.f, dD = uD;
-      // D moved down a lane: lane 0 keeps the row -1 value that these
-      // loop-carried registers already hold (no register copy per step)
-      float s1D = old_d, s2D = old_d, s3D = old_d;
+      Da[RPL - 1] = k == W - 1 ? initD : 0.f;
+      // Y of the two steps before step 0 and X of the step before: row -1
+      // for a segment's first lane, zeros elsewhere (columns < 0)
+      const float y0 = k == 0 ? tmd[0] * initD : 0.f;
+      float yd = y0, yn = y0, xu = 0.f;
       float S = 0.f;
       const int nsteps = (int)H + W - 1;
       const char* tab = reinterpret_cast<const char*>(ptab) + lane * 16;
@@ -558,6 +587,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
 #pragma unroll
         for (int t = 0; t < TR; ++t) pr[4 * NF + t] = ptail[(c * TR + t) * 64 + lane];
       };
+      // After a step: this lane's X and Y for the lane below, moved there.
+      // 16-lane segments move them with LDS lane permutes, 8-lane segments
+      // (and 32/64) with DPP, where a segment's first lane takes X = 0 (bound
+      // control) and keeps Y = y0, which the register of two steps before
+      // (dead after this step) already holds there (measured on the batched
+      // workload: 16 x 16 49.2 -> 46.7 ms with permutes, 47.6 with DPP;
+      // 8 x 19 36.1 -> 34.3 ms with DPP, 36.0 with permutes).
+      auto send = [&](const float (&nM)[RPL], const float (&nI)[RPL], const float (&nD)[RPL]) {
+        const float x = fmaf(cN, nI[RPL - 1], nM[RPL - 1]);
+        const float y = fmaf(dN, nD[RPL - 1], fmaf(rN, nI[RPL - 1], nM[RPL - 1]));
+        if constexpr (W == 16) {
+          xu = lane_from(x, -SH);
+          yd = yn;
+          yn = lane_from(y, -SH);
+        } else {
+          xu = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kDownCtrl<W>, 0xf, 0xf, true));
+          const float t = __int_as_float(
+              __builtin_amdgcn_update_dpp(__float_as_int(yd), __float_as_int(y), kDownCtrl<W>, 0xf, 0xf, false));
+          yd = yn;
+          yn = t;
+        }
+      };
       // Four steps a -> b -> a -> b -> a per iteration. The read's last row is
       // always a padding row (the host picks RPL with R < W * RPL), which holds
       // M[R-1][j] + I[R-1][j] in I, so the sum takes one add per step.
@@ -575,34 +626,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
           const uint32_t cw_next = window(st + 4);
           load(cw, 1, pB);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_pt<float, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
           S += Ib[RPL - 1];
-          const float s1M = lane_down0<W>(Mb[RPL - 1], k);
-          const float s1I = lane_down0<W>(Ib[RPL - 1], k);
-          s1D = lane_down_keep<W>(Db[RPL - 1], s1D, old_d, k);
+          send(Mb, Ib, Db);
           load(cw, 2, pA);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_pt<float, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, pB, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
           if (st + 1 < nsteps) S += Ia[RPL - 1];
-          const float s2M = lane_down0<W>(Ma[RPL - 1], k);
-          const float s2I = lane_down0<W>(Ia[RPL - 1], k);
-          s2D = lane_down_keep<W>(Da[RPL - 1], s2D, old_d, k);
+          send(Ma, Ia, Da);
           load(cw, 3, pB);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_pt<float, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, pA, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
           if (st + 2 < nsteps) S += Ib[RPL - 1];
-          const float s3M = lane_down0<W>(Mb[RPL - 1], k);
-          const float s3I = lane_down0<W>(Ib[RPL - 1], k);
-          s3D = lane_down_keep<W>(Db[RPL - 1], s3D, old_d, k);
+          send(Mb, Ib, Db);
           cw = cw_next;
           load(cw, 0, pA);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_pt<float, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, pB, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
           if (st + 3 < nsteps) S += Ia[RPL - 1];
-          dM = s3M; dI = s3I; dD = s3D;
-          uM = lane_down0<W>(Ma[RPL - 1], k);
-          uI = lane_down0<W>(Ia[RPL - 1], k);
-          uD = lane_down_keep<W>(Da[RPL - 1], uD, old_d, k);
+          send(Ma, Ia, Da);
         }
       } else {
         // three steps ahead, four rotating buffers
@@ -615,33 +657,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
           const uint32_t cw2 = window(st + 8);
           load(cw, 3, p3);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_pt<float, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, p0, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, p0, tmm, tgm, tmi, tii, tmd, tdd);
           S += Ib[RPL - 1];
-          const float s1M = lane_down0<W>(Mb[RPL - 1], k);
-          const float s1I = lane_down0<W>(Ib[RPL - 1], k);
-          s1D = lane_down_keep<W>(Db[RPL - 1], s1D, old_d, k);
+          send(Mb, Ib, Db);
           load(cw1, 0, p0);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_pt<float, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, s1M, s1I, p1, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, p1, tmm, tgm, tmi, tii, tmd, tdd);
           if (st + 1 < nsteps) S += Ia[RPL - 1];
-          const float s2M = lane_down0<W>(Ma[RPL - 1], k);
-          const float s2I = lane_down0<W>(Ia[RPL - 1], k);
-          s2D = lane_down_keep<W>(Da[RPL - 1], s2D, old_d, k);
+          send(Ma, Ia, Da);
           load(cw1, 1, p1);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_pt<float, RPL, F>(Ma, Ia, Da, Mb, Ib, Db, s1M, s1I, s1D, s2M, s2I, p2, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, p2, tmm, tgm, tmi, tii, tmd, tdd);
           if (st + 2 < nsteps) S += Ib[RPL - 1];
-          const float s3M = lane_down0<W>(Mb[RPL - 1], k);
-          const float s3I = lane_down0<W>(Ib[RPL - 1], k);
-          s3D = lane_down_keep<W>(Db[RPL - 1], s3D, old_d, k);
+          send(Mb, Ib, Db);
           load(cw1, 2, p2);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_pt<float, RPL, F>(Mb, Ib, Db, Ma, Ia, Da, s2M, s2I, s2D, s3M, s3I, p3, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, p3, tmm, tgm, tmi, tii, tmd, tdd);
           if (st + 3 < nsteps) S += Ia[RPL - 1];
-          dM = s3M; dI = s3I; dD = s3D;
-          uM = lane_down0<W>(Ma[RPL - 1], k);
-          uI = lane_down0<W>(Ia[RPL - 1], k);
-          uD = lane_down_keep<W>(Da[RPL - 1], uD, old_d, k);
+          send(Ma, Ia, Da);
           cw = cw1;
           cw1 = cw2;
         }
