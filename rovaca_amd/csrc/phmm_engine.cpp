@@ -702,18 +702,33 @@ void split_runs(HostPack& P) {
   std::vector<uint32_t>& cost = S.cost;
   int to[kFp32Bins];
   fold_bins(P, to);
+  // the bins holding tiles, grouped by the bin they are swept in (a counting
+  // sort, increasing within a group): a one-region call touches a handful of
+  // the bins, so nothing here may cost bins x bins
+  int first[kFp32Bins + 1] = {};
+  int src[kFp32Bins];
+  for (int u = 0; u < kFp32Bins; ++u)
+    if (P.bin_start[u] < P.bin_start[u + 1]) ++first[to[u] + 1];
+  for (int v = 0; v < kFp32Bins; ++v) first[v + 1] += first[v];
+  {
+    int fill[kFp32Bins];
+    std::memcpy(fill, first, sizeof fill);
+    for (int u = 0; u < kFp32Bins; ++u)
+      if (P.bin_start[u] < P.bin_start[u + 1]) src[fill[to[u]]++] = u;
+  }
   out.clear();
   start.assign(kFp32Bins + 1, 0);
   for (int v = 0; v < kFp32Bins; ++v) {
     start[v] = (uint32_t)out.size();
+    if (first[v] == first[v + 1]) continue;
     const uint32_t run = run_for_bin(P, v, P.bin_units[v]);
     int W, RPL;
     fp32_variant_shape(v % kFp32Variants, &W, &RPL);
     bin.clear();
     cost.clear();
     // this bin's own tiles and those of the bins folded into it
-    for (int u = 0; u < kFp32Bins; ++u) {
-      if (to[u] != v) continue;
+    for (int i = first[v]; i < first[v + 1]; ++i) {
+      const int u = src[i];
       for (uint32_t x = P.bin_start[u]; x < P.bin_start[u + 1]; ++x)
         split_tile(P.tiles[x], run, [&](const Tile& p) {
           uint32_t c = 0;
