@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <emmintrin.h>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
@@ -132,12 +133,21 @@ struct AcgtnTable {
 };
 const AcgtnTable kAcgtn;
 
-// Index of the first byte of s[0..n) outside A, C, G, T, N, or -1.
+// Index of the first byte of s[0..n) outside A, C, G, T, N, or -1 (16 bytes
+// per step against the five letters, then byte by byte).
 int64_t first_other_base(const uint8_t* s, int32_t n) {
-  uint8_t all = 1;
-  for (int32_t x = 0; x < n; ++x) all &= kAcgtn.ok[s[x]];
-  if (all) return -1;
-  for (int32_t x = 0; x < n; ++x)
+  int32_t x = 0;
+  const __m128i a = _mm_set1_epi8('A'), c = _mm_set1_epi8('C'), g = _mm_set1_epi8('G'), t = _mm_set1_epi8('T'),
+                nn = _mm_set1_epi8('N');
+  for (; x + 16 <= n; x += 16) {
+    const __m128i v = _mm_loadu_si128(reinterpret_cast<const __m128i*>(s + x));
+    const __m128i ok = _mm_or_si128(_mm_or_si128(_mm_or_si128(_mm_cmpeq_epi8(v, a), _mm_cmpeq_epi8(v, c)),
+                                                 _mm_or_si128(_mm_cmpeq_epi8(v, g), _mm_cmpeq_epi8(v, t))),
+                                    _mm_cmpeq_epi8(v, nn));
+    const int m = _mm_movemask_epi8(ok);
+    if (m != 0xFFFF) return x + __builtin_ctz(~m & 0xFFFF);
+  }
+  for (; x < n; ++x)
     if (!kAcgtn.ok[s[x]]) return x;
   return -1;
 }
@@ -314,6 +324,70 @@ struct CodeTable {
 const CodeTable kCodes;
 }  // namespace
 
+namespace {
+inline uint64_t load8(const uint8_t* p) {
+  uint64_t v;
+  std::memcpy(&v, p, 8);
+  return v;
+}
+// The 2-bit codes of 8 bases (one per byte, first base in the low byte)
+// packed into 16 bits, base i at bits 2i: ((c >> 1) ^ (c >> 2)) & 3 is 0, 1,
+// 2, 3 for 'A', 'C', 'G', 'T' (0 for 'N'), then three shift-or steps close
+// the gaps between the bytes' fields.
+inline uint32_t codes8(uint64_t v) {
+  uint64_t x = ((v >> 1) ^ (v >> 2)) & 0x0303030303030303ull;
+  x = (x | (x >> 6)) & 0x000F000F000F000Full;
+  x = (x | (x >> 12)) & 0x000000FF000000FFull;
+  return (uint32_t)((x | (x >> 24)) & 0xFFFFull);
+}
+// Bit i set where byte i is 'N': an exact zero-byte test of v ^ "NNNNNNNN"
+// (high bit of each byte), gathered the same way.
+inline uint32_t nbits8(uint64_t v) {
+  const uint64_t t = v ^ 0x4E4E4E4E4E4E4E4Eull;
+  uint64_t z = ~(((t & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) | t | 0x7F7F7F7F7F7F7F7Full) >> 7;
+  z = (z | (z >> 7)) & 0x0003000300030003ull;
+  z = (z | (z >> 14)) & 0x0000000F0000000Full;
+  return (uint32_t)((z | (z >> 28)) & 0xFFull);
+}
+}  // namespace
+
+// Widens [lo, hi] to the gap-open qualities (insertion and deletion) of one
+// read and lowers clo to its smallest gap-continuation quality (7-bit
+// values), 16 bases per step.
+static void gap_ranges(const uint8_t* gi, const uint8_t* gd, const uint8_t* gc, uint32_t n, uint8_t* lo, uint8_t* hi,
+                       uint8_t* clo) {
+  const __m128i m7 = _mm_set1_epi8(127);
+  __m128i vlo = _mm_set1_epi8((char)*lo), vhi = _mm_set1_epi8((char)*hi), vc = _mm_set1_epi8((char)*clo);
+  uint32_t y = 0;
+  for (; y + 16 <= n; y += 16) {
+    const __m128i a = _mm_and_si128(_mm_loadu_si128(reinterpret_cast<const __m128i*>(gi + y)), m7);
+    const __m128i b = _mm_and_si128(_mm_loadu_si128(reinterpret_cast<const __m128i*>(gd + y)), m7);
+    const __m128i c = _mm_and_si128(_mm_loadu_si128(reinterpret_cast<const __m128i*>(gc + y)), m7);
+    vlo = _mm_min_epu8(vlo, _mm_min_epu8(a, b));
+    vhi = _mm_max_epu8(vhi, _mm_max_epu8(a, b));
+    vc = _mm_min_epu8(vc, c);
+  }
+  alignas(16) uint8_t l[16], h[16], q[16];
+  _mm_store_si128(reinterpret_cast<__m128i*>(l), vlo);
+  _mm_store_si128(reinterpret_cast<__m128i*>(h), vhi);
+  _mm_store_si128(reinterpret_cast<__m128i*>(q), vc);
+  uint8_t L = l[0], Hh = h[0], C = q[0];
+  for (int k = 1; k < 16; ++k) {
+    L = std::min(L, l[k]);
+    Hh = std::max(Hh, h[k]);
+    C = std::min(C, q[k]);
+  }
+  for (; y < n; ++y) {
+    const uint8_t a = gi[y] & 127, b = gd[y] & 127, c = gc[y] & 127;
+    L = std::min(L, std::min(a, b));
+    Hh = std::max(Hh, std::max(a, b));
+    C = std::min(C, c);
+  }
+  *lo = L;
+  *hi = Hh;
+  *clo = C;
+}
+
 static void pack_seq(const uint8_t* seq, uint32_t n, uint32_t off, std::vector<uint32_t>& pack,
                      std::vector<uint32_t>& nmask) {
   // off is a multiple of 32: whole 2-bit and N-mask words, assembled in registers
@@ -321,16 +395,10 @@ static void pack_seq(const uint8_t* seq, uint32_t n, uint32_t off, std::vector<u
   uint32_t* nw = nmask.data() + (off >> 5);
   uint32_t x = 0;
   for (; x + 32 <= n; x += 32) {
-    uint32_t lo = 0, hi = 0, nm = 0;
-    for (uint32_t y = 0; y < 16; ++y) {
-      const uint32_t c0 = kCodes.c[seq[x + y]], c1 = kCodes.c[seq[x + 16 + y]];
-      lo |= (c0 & 3u) << (2 * y);
-      hi |= (c1 & 3u) << (2 * y);
-      nm |= ((c0 >> 2) << y) | ((c1 >> 2) << (16 + y));
-    }
-    pw[x >> 4] |= lo;
-    pw[(x >> 4) + 1] |= hi;
-    nw[x >> 5] |= nm;
+    const uint64_t v0 = load8(seq + x), v1 = load8(seq + x + 8), v2 = load8(seq + x + 16), v3 = load8(seq + x + 24);
+    pw[x >> 4] |= codes8(v0) | (codes8(v1) << 16);
+    pw[(x >> 4) + 1] |= codes8(v2) | (codes8(v3) << 16);
+    nw[x >> 5] |= nbits8(v0) | (nbits8(v1) << 8) | (nbits8(v2) << 16) | (nbits8(v3) << 24);
   }
   for (; x < n; ++x) {  // the last, partial word
     const uint32_t c = kCodes.c[seq[x]];
@@ -363,6 +431,13 @@ static void append_largest_first(const std::vector<Tile>& tiles, const std::vect
   if (tiles.empty()) return;
   const size_t base = dst.size();
   dst.resize(base + tiles.size());
+  // already in order (one region's tiles often all cost the same): a copy
+  bool ordered = true;
+  for (size_t x = 1; ordered && x < tiles.size(); ++x) ordered = (cost[x - 1] >> 4) >= (cost[x] >> 4);
+  if (ordered) {
+    std::copy(tiles.begin(), tiles.end(), dst.begin() + base);
+    return;
+  }
   if (tiles.size() < 512) {
     std::vector<uint32_t>& idx = pack_scratch().idx;
     idx.resize(tiles.size());
@@ -528,16 +603,7 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
         std::memcpy(&P.ri[rcur], g.i + o, len);
         std::memcpy(&P.rd[rcur], g.d + o, len);
         std::memcpy(&P.rc[rcur], g.c + o, len);
-        uint8_t lo = gop_lo, hi = gop_hi, clo = gcp_lo;
-        for (uint32_t y = 0; y < len; ++y) {
-          const uint8_t gi_ = g.i[o + y] & 127, gd_ = g.d[o + y] & 127, gc_ = g.c[o + y] & 127;
-          lo = std::min(lo, std::min(gi_, gd_));
-          hi = std::max(hi, std::max(gi_, gd_));
-          clo = std::min(clo, gc_);
-        }
-        gop_lo = lo;
-        gop_hi = hi;
-        gcp_lo = clo;
+        gap_ranges(g.i + o, g.d + o, g.c + o, len, &gop_lo, &gop_hi, &gcp_lo);
       }
       rcur += align_up(len, kSeqAlign);
     }
