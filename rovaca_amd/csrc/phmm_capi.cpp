@@ -103,7 +103,7 @@ int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags, cons
   if ((rc = w->staging.reserve(L.cnt + 4 * kNumCounters))) return rc;
   stage_pack(w->pack, L, w->staging.ptr);
   stage_results(w->pack, L, w->staging.ptr);  // uploaded with the inputs: no begin kernel
-  if ((rc = w->batch.upload_staged(w->pack, L, w->staging.ptr, w->stream, w->mem, w->bytes, true))) return rc;
+  if ((rc = w->batch.upload_staged(w->pack, L, w->staging.ptr, w->stream, w->mem, w->bytes, true, true))) return rc;
   if (w->batch.own_mem) {  // the workspace was too small: keep the new allocation
     if (w->mem) hipFree(w->mem);
     w->mem = w->batch.mem;

@@ -1089,6 +1089,8 @@ int stage_merged(const HostPack* const* packs, uint32_t n, HostPack* meta, Batch
   return RVC_OK;
 }
 
+constexpr size_t kKernelUploadMax = 4u << 20;
+
 int DeviceBatch::upload(const HostPack& P, hipStream_t s, void* reuse_mem, size_t reuse_bytes) {
   const BatchLayout L = batch_layout(pack_sizes(P));
   std::vector<char>& img = const_cast<HostPack&>(P).staging;
@@ -1098,7 +1100,7 @@ int DeviceBatch::upload(const HostPack& P, hipStream_t s, void* reuse_mem, size_
 }
 
 int DeviceBatch::upload_staged(const HostPack& P, const BatchLayout& L, const void* host, hipStream_t s,
-                               void* reuse_mem, size_t reuse_bytes, bool with_results) {
+                               void* reuse_mem, size_t reuse_bytes, bool with_results, bool src_pinned) {
   pack = &P;
   layout = L;
   bytes = L.bytes;
@@ -1114,7 +1116,12 @@ int DeviceBatch::upload_staged(const HostPack& P, const BatchLayout& L, const vo
   // counters too (stage_results); else the first launch sequence's begin
   // kernel sets results to NaN with no flags (pairs with an empty side keep
   // that) and clears the counters
-  RVC_HIP(hipMemcpyAsync(base, host, with_results ? L.cnt + 4 * kNumCounters : L.in_bytes, hipMemcpyHostToDevice, s));
+  const size_t up = with_results ? L.cnt + 4 * kNumCounters : L.in_bytes;
+  // one region's image: a kernel reads the pinned staging (about 3 us sooner
+  // than a copy engine whose end the sweep must wait for); larger images go
+  // by DMA
+  if (src_pinned && up <= kKernelUploadMax) RVC_HIP(launch_upload(base, host, up, s));
+  else RVC_HIP(hipMemcpyAsync(base, host, up, hipMemcpyHostToDevice, s));
   init_pending = !with_results;
   counters_clear = with_results;
   seqs.hap_pack = reinterpret_cast<const uint32_t*>(base + L.hpack);

@@ -281,9 +281,12 @@ struct DeviceBatch {
   // enough) and queues the host-to-device copy on s.
   int upload(const HostPack& P, hipStream_t s, void* reuse_mem, size_t reuse_bytes);
   // The same for inputs already staged in device layout at 'host' (pinned
-  // memory makes the copy asynchronous); 'meta' describes the batch.
+  // memory makes the copy asynchronous); 'meta' describes the batch. Where
+  // 'host' is pinned (hipHostMalloc) and the image small, src_pinned lets a
+  // kernel on s read it instead of a copy engine: the sweep that follows
+  // then starts without waiting for another engine's signal.
   int upload_staged(const HostPack& meta, const BatchLayout& L, const void* host, hipStream_t s, void* reuse_mem,
-                    size_t reuse_bytes, bool with_results = false);
+                    size_t reuse_bytes, bool with_results = false, bool src_pinned = false);
   // Queues counter reset, FP32 launches and FP64 rescue launches on s: one
   // FP64 launch per list that can hold pairs (rescued == nullptr), none
   // (rescued == kNoFp64), or one per list whose count in rescued[] is not 0.

@@ -106,7 +106,7 @@ int region_sync_noalloc(const rvc_region* region, double* out, uint8_t* flags) {
   stage_results(g_na.pack, L, g_na.host);  // uploaded with the inputs: no begin kernel
   DeviceBatch& b = g_na.batch;
   // the image goes into the scratch array (reuse_mem covers it: no hipMalloc)
-  if ((rc = b.upload_staged(g_na.pack, L, g_na.host, g_na.stream, g_na.dev, kNoAllocScratchBytes, true)))
+  if ((rc = b.upload_staged(g_na.pack, L, g_na.host, g_na.stream, g_na.dev, kNoAllocScratchBytes, true, true)))
     return rc;
   if (b.own_mem) return set_error(RVC_E_INVALID, "device image did not fit the scratch array");
   const size_t n = g_na.pack.n_pairs_slots;
