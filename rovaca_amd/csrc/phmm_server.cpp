@@ -260,7 +260,8 @@ int launch_slot(Slot& s) {
     s.bytes = want;
     ++g_srv->grows;
   }
-  if ((rc = s.batch.upload_staged(s.meta, L, s.staging.ptr, s.stream, s.mem, s.bytes))) return rc;
+  // the staging buffer is pinned and is not restaged before this launch ends
+  if ((rc = s.batch.upload_staged(s.meta, L, s.staging.ptr, s.stream, s.mem, s.bytes, false, true))) return rc;
   const size_t n = s.meta.n_pairs_slots;
   bool grew = false;
   s.res = g_srv->pool->acquire(n, &rc, &grew);
