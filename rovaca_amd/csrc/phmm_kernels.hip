@@ -452,20 +452,18 @@ __device__ __forceinline__ void sweep_step_pt(const T (&M)[RPL], const T (&I)[RP
 // so M[0] = prior * Y and I~[0] = X are exactly the operations of the full
 // cell, done by the sender, and a step moves two values between lanes
 // instead of three (M, I and D).
-template <int RPL>
-__device__ __forceinline__ void sweep_step_xy(const float (&M)[RPL], const float (&I)[RPL], const float (&D)[RPL],
-                                              float (&nM)[RPL], float (&nI)[RPL], float (&nD)[RPL], float yd,
-                                              float xu, const float* pr, const float (&tmm)[RPL],
-                                              const float (&tgm)[RPL], const float (&tmi)[RPL],
-                                              const float (&tii)[RPL], const float (&tmd)[RPL],
-                                              const float (&tdd)[RPL]) {
+template <typename T, int RPL>
+__device__ __forceinline__ void sweep_step_xy(const T (&M)[RPL], const T (&I)[RPL], const T (&D)[RPL], T (&nM)[RPL],
+                                              T (&nI)[RPL], T (&nD)[RPL], T yd, T xu, const T* pr,
+                                              const T (&tmm)[RPL], const T (&tgm)[RPL], const T (&tmi)[RPL],
+                                              const T (&tii)[RPL], const T (&tmd)[RPL], const T (&tdd)[RPL]) {
   nM[0] = pr[0] * yd;
   nI[0] = xu;
-  nD[0] = fmaf(tdd[0], D[0], M[0]);
+  nD[0] = fma(tdd[0], D[0], M[0]);
 #pragma unroll
   for (int r = 1; r < RPL; ++r)
-    cell<float, kScaled>(pr[r], M[r - 1], I[r - 1], D[r - 1], nM[r - 1], nI[r - 1], M[r], D[r], tmm[r], tgm[r],
-                         tmi[r], tii[r], tmd[r], tdd[r], nM[r], nI[r], nD[r]);
+    cell<T, kScaled>(pr[r], M[r - 1], I[r - 1], D[r - 1], nM[r - 1], nI[r - 1], M[r], D[r], tmm[r], tgm[r], tmi[r],
+                     tii[r], tmd[r], tdd[r], nM[r], nI[r], nD[r]);
 }
 
 // v of lane (lane + delta) mod 64, through the LDS crossbar (ds_bpermute:
@@ -473,6 +471,13 @@ __device__ __forceinline__ void sweep_step_xy(const float (&M)[RPL], const float
 __device__ __forceinline__ float lane_from(float v, int delta) {
   const int addr = ((int)(threadIdx.x + delta) & 63) << 2;
   return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_from(double v, int delta) {
+  const int addr = ((int)(threadIdx.x + delta) & 63) << 2;
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+  return __longlong_as_double(((long long)(unsigned)lo) | ((long long)hi << 32));
 }
 
 template <int W, int RPL, bool XF, int OCC>
@@ -626,23 +631,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
           const uint32_t cw_next = window(st + 4);
           load(cw, 1, pB);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_xy<RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<float, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
           S += Ib[RPL - 1];
           send(Mb, Ib, Db);
           load(cw, 2, pA);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_xy<RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<float, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
           if (st + 1 < nsteps) S += Ia[RPL - 1];
           send(Ma, Ia, Da);
           load(cw, 3, pB);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_xy<RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<float, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
           if (st + 2 < nsteps) S += Ib[RPL - 1];
           send(Mb, Ib, Db);
           cw = cw_next;
           load(cw, 0, pA);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_xy<RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<float, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
           if (st + 3 < nsteps) S += Ia[RPL - 1];
           send(Ma, Ia, Da);
         }
@@ -657,22 +662,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
           const uint32_t cw2 = window(st + 8);
           load(cw, 3, p3);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_xy<RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, p0, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<float, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, p0, tmm, tgm, tmi, tii, tmd, tdd);
           S += Ib[RPL - 1];
           send(Mb, Ib, Db);
           load(cw1, 0, p0);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_xy<RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, p1, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<float, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, p1, tmm, tgm, tmi, tii, tmd, tdd);
           if (st + 1 < nsteps) S += Ia[RPL - 1];
           send(Ma, Ia, Da);
           load(cw1, 1, p1);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_xy<RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, p2, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<float, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, p2, tmm, tgm, tmi, tii, tmd, tdd);
           if (st + 2 < nsteps) S += Ib[RPL - 1];
           send(Mb, Ib, Db);
           load(cw1, 2, p2);
           __builtin_amdgcn_sched_barrier(0);
-          sweep_step_xy<RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, p3, tmm, tgm, tmi, tii, tmd, tdd);
+          sweep_step_xy<float, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, p3, tmm, tgm, tmi, tii, tmd, tdd);
           if (st + 3 < nsteps) S += Ia[RPL - 1];
           send(Ma, Ia, Da);
           cw = cw1;
@@ -741,12 +746,18 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
 
     const double initD = pow2d((int)en.x + 900) / (double)H;
     const double sfac = sum_factor<double, W, RPL, kScaled>(a.s, ph2pr, en.read);
+    // Row 0 takes X and Y premixed by the lane above with this lane's row-0
+    // coefficients (as in the FP32 kernel): cN, rN, dN are the next lane's.
+    // A segment's first lane sees row -1 (X = 0, Y = delta * initD of its
+    // own pair): DPP bound control and the kept register give it that in
+    // 16- and 64-lane segments; at lane 32 of a 32-lane pair the wave-wide
+    // move must be replaced (lane_down0 / lane_down_keep select there).
+    const double cN = lane_from(tii[0], 1), rN = lane_from(tmi[0], 1), dN = lane_from(tmd[0], 1);
     double Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
 #pragma unroll
     for (int r = 0; r < RPL; ++r) { Ma[r] = 0; Ia[r] = 0; Da[r] = 0; }
-    double uM = 0, uI = 0, uD = (k == 0) ? initD : 0.0;
-    double dM = 0, dI = 0, dD = uD;
-    double sD = initD;  // loop-carried: lane 0 keeps the row -1 value
+    const double y0 = k == 0 ? tmd[0] * initD : 0.0;
+    double yd = y0, yn = y0, xu = 0.0;
     double S = 0;
     const int nsteps = (int)Hmax + W - 1;
     const int mysteps = (int)H + W - 1;
@@ -765,27 +776,42 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
         pr[2 * q + 0] = v.x; pr[2 * q + 1] = v.y;
       }
     };
+    auto send = [&](const double (&nM)[RPL], const double (&nI)[RPL], const double (&nD)[RPL]) {
+      const double x = fma(cN, nI[RPL - 1], nM[RPL - 1]);
+      const double y = fma(dN, nD[RPL - 1], fma(rN, nI[RPL - 1], nM[RPL - 1]));
+      xu = lane_down0<W>(x, k);
+      const double t = lane_down_keep<W>(y, yd, y0, k);  // yd (two steps old) dies here
+      yd = yn;
+      yn = t;
+    };
     double pA[NP * 2], pB[NP * 2];
     uint32_t cw = codes_at(0);
     load(cw, 0, pA);
-    // two steps a -> b -> a per iteration, the table reads one step ahead
-    for (int st = 0; st < nsteps; st += 2) {
-      load(cw, (st + 1) & 3, pB);
+    // four steps a -> b -> a -> b -> a per iteration, the table reads one
+    // step ahead, the next codes word read once per iteration
+    for (int st = 0; st < nsteps; st += 4) {
+      const uint32_t cw_next = codes_at(st + 4);
+      load(cw, 1, pB);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<double, RPL, kScaled>(Ma, Ia, Da, Mb, Ib, Db, dM, dI, dD, uM, uI, pA, tmm, tgm, tmi, tii, tmd, tdd);
+      sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
       if (st < mysteps) S += fma(sfac, Ib[RPL - 1], Mb[RPL - 1]);
-      const double sM = lane_down0<W>(Mb[RPL - 1], k);
-      const double sI = lane_down0<W>(Ib[RPL - 1], k);
-      sD = lane_down_keep<W>(Db[RPL - 1], sD, initD, k);
-      if (((st + 2) & 3) == 0) cw = codes_at(st + 2);
-      load(cw, (st + 2) & 3, pA);
+      send(Mb, Ib, Db);
+      load(cw, 2, pA);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_pt<double, RPL, kScaled>(Mb, Ib, Db, Ma, Ia, Da, uM, uI, uD, sM, sI, pB, tmm, tgm, tmi, tii, tmd, tdd);
+      sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
       if (st + 1 < mysteps) S += fma(sfac, Ia[RPL - 1], Ma[RPL - 1]);
-      dM = sM; dI = sI; dD = sD;
-      uM = lane_down0<W>(Ma[RPL - 1], k);
-      uI = lane_down0<W>(Ia[RPL - 1], k);
-      uD = lane_down_keep<W>(Da[RPL - 1], uD, initD, k);
+      send(Ma, Ia, Da);
+      load(cw, 3, pB);
+      __builtin_amdgcn_sched_barrier(0);
+      sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
+      if (st + 2 < mysteps) S += fma(sfac, Ib[RPL - 1], Mb[RPL - 1]);
+      send(Mb, Ib, Db);
+      cw = cw_next;
+      load(cw, 0, pA);
+      __builtin_amdgcn_sched_barrier(0);
+      sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
+      if (st + 3 < mysteps) S += fma(sfac, Ia[RPL - 1], Ma[RPL - 1]);
+      send(Ma, Ia, Da);
     }
     if (active && k == W - 1) {
       a.out[en.out] = log10(S * pow2d(120 - (int)en.x)) - 307.050595577260822;
