@@ -174,6 +174,10 @@ struct Slot {
   PinnedBuf marg_in, marg_out;
   size_t marg_out_bytes = 0;
   std::vector<uint64_t> marg_base;  // per request: first output value
+  MargArgs marg{};                  // its launch, queued after the FP64 pass
+  size_t marg_in_bytes = 0;
+  char* marg_dev = nullptr;
+  bool marg_pending = false;
 };
 
 size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -268,7 +272,8 @@ int launch_slot(Slot& s) {
   if (!s.res) return rc;
   if (grew) ++g_srv->grows;
   if ((rc = s.batch.ensure_events())) return rc;
-  if ((rc = s.batch.enqueue(s.stream, true))) return rc;
+  // the FP32 pass; the FP64 launches follow once its rescue counts are known
+  if ((rc = s.batch.enqueue(s.stream, true, kNoFp64))) return rc;
   s.marg_out_bytes = 8 * marg_values;
   if (n_mreg) {
     const size_t cap_in = s.marg_in.cap, cap_out = s.marg_out.cap;
@@ -317,16 +322,48 @@ int launch_slot(Slot& s) {
     ma.map = reinterpret_cast<const int16_t*>(d + o_map);
     ma.out = reinterpret_cast<double*>(d + marg_in_bytes);
     ma.n_regions = n_mreg;
-    if (hipMemcpyAsync(d, h, marg_in_bytes, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
-        launch_marginalize(ma, s.stream) != hipSuccess ||
-        hipMemcpyAsync(s.marg_out.ptr, ma.out, 8 * marg_values, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
-      return set_error(RVC_E_DEVICE, "marginalisation launch");
+    s.marg = ma;
+    s.marg_in_bytes = marg_in_bytes;
+    s.marg_dev = d;
+    s.marg_pending = true;
   }
   // results, flags and counters sit one after another: one copy
   if (hipMemcpyAsync(s.res->out(), s.batch.out, s.batch.fetch_bytes(), hipMemcpyDeviceToHost, s.stream) !=
           hipSuccess ||
       hipEventRecord(s.done_ev, s.stream) != hipSuccess)
     return set_error(RVC_E_DEVICE, "result copy");
+  return RVC_OK;
+}
+
+// The second phase of a launch, after its FP32 results, flags and rescue
+// counts are in the slot's result buffer: the FP64 launches of the lists that
+// hold pairs (grids sized by their counts) with a second copy of results and
+// flags, then the marginalisation of the requests with sites. *queued says
+// whether anything went on the stream (most launches need neither).
+int launch_tail(Slot& s, bool* queued) {
+  *queued = false;
+  uint32_t rescued[kNumFp64Lists];
+  std::memcpy(rescued, s.res->rescued(), sizeof rescued);
+  bool any = false;
+  for (uint32_t c : rescued) any = any || c != 0;
+  int rc;
+  if (any) {
+    if ((rc = s.batch.enqueue_fp64(s.stream, true, rescued))) return rc;
+    const size_t n = s.meta.n_pairs_slots;
+    if (hipMemcpyAsync(s.res->out(), s.batch.out, n * (sizeof(double) + 1), hipMemcpyDeviceToHost, s.stream) !=
+        hipSuccess)
+      return set_error(RVC_E_DEVICE, "result copy");
+    *queued = true;
+  }
+  if (s.marg_pending) {
+    s.marg_pending = false;
+    if (hipMemcpyAsync(s.marg_dev, s.marg_in.ptr, s.marg_in_bytes, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
+        launch_marginalize(s.marg, s.stream) != hipSuccess ||
+        hipMemcpyAsync(s.marg_out.ptr, s.marg.out, s.marg_out_bytes, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+      return set_error(RVC_E_DEVICE, "marginalisation launch");
+    *queued = true;
+  }
+  if (*queued && hipEventRecord(s.done_ev, s.stream) != hipSuccess) return set_error(RVC_E_DEVICE, "launch event");
   return RVC_OK;
 }
 
@@ -371,8 +408,12 @@ void slot_main(Slot* sp) {
     lk.unlock();
     int rc = RVC_OK;
     try {  // host allocations of a launch: its requests fail, the loop goes on
+      s.marg_pending = false;
       rc = launch_slot(s);
       if (rc == RVC_OK) rc = wait_slot(s);
+      bool more = false;
+      if (rc == RVC_OK) rc = launch_tail(s, &more);
+      if (rc == RVC_OK && more) rc = wait_slot(s);
     } catch (...) {
       rc = set_error(RVC_E_INVALID, "out of host memory for a serving launch");
     }
