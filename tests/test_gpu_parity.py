@@ -525,6 +525,39 @@ def test_served_results_equal_alone():
     assert st["completed"] >= len(regs)
 
 
+def test_served_rescue_equals_alone():
+    # stress regions among ordinary ones from 8 threads: a served launch whose
+    # FP32 pass flags pairs runs the FP64 lists in its second phase, one
+    # without flagged pairs skips them; each result equals the region alone
+    say("serving with FP64 rescue: stress and ordinary regions from 8 threads, each equals the region alone")
+    stress = [r for r in workload.stress_regions(16, seed=71) if r["n_haps"] * r["n_reads"] < 5000][:6]
+    plain = [r for r in workload.batched_regions(12, seed=72) if r["n_haps"] * r["n_reads"] < 5000]
+    regs = [r for pair in zip(stress, plain) for r in pair] + plain[len(stress):]
+    alone = [rv.region_sync(r) for r in regs]
+    assert sum(int(((f & 2) != 0).sum()) for _, f in alone) > 0
+    results = [None] * len(regs)
+    errors = []
+
+    def worker(idx):
+        try:
+            for x in idx:
+                t = rv.submit(regs[x])
+                results[x] = rv.wait(t, regs[x])[0]
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    for rep in range(2):
+        th = [threading.Thread(target=worker, args=(list(range(k, len(regs), 8)),)) for k in range(8)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join(timeout=120)
+        assert not errors
+        for (a, _), m in zip(alone, results):
+            assert np.array_equal(np.isnan(a), np.isnan(m)), rep
+            assert np.array_equal(a[~np.isnan(a)], m[~np.isnan(m)]), rep
+
+
 def test_call_pairhmm_region_adapter():
     say("call_pairhmm adapter: preprocessing + likelihoods + normalize/filter/transpose")
     rng = np.random.default_rng(61)
