@@ -749,15 +749,22 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
     // Row 0 takes X and Y premixed by the lane above with this lane's row-0
     // coefficients (as in the FP32 kernel): cN, rN, dN are the next lane's.
     // A segment's first lane sees row -1 (X = 0, Y = delta * initD of its
-    // own pair): DPP bound control and the kept register give it that in
-    // 16- and 64-lane segments; at lane 32 of a 32-lane pair the wave-wide
-    // move must be replaced (lane_down0 / lane_down_keep select there).
-    const double cN = lane_from(tii[0], 1), rN = lane_from(tmi[0], 1), dN = lane_from(tmd[0], 1);
+    // own pair). In 16- and 64-lane segments DPP bound control gives it
+    // X = 0 and the kept register Y. In 32-lane segments the wave-wide move
+    // takes lane 31's values to lane 32, the second pair's first lane: there
+    // lane 31 sends zeros (its M masked, its next-lane coefficients 0) and
+    // every first lane adds its own row -1 term y0 to the Y it receives
+    // (Y + 0 is Y itself, so the results stay bit-identical), which costs
+    // two FP64 operations a step instead of four selects and two copies.
+    double cN = lane_from(tii[0], 1), rN = lane_from(tmi[0], 1), dN = lane_from(tmd[0], 1);
+    constexpr bool kMaskLast = W == 32;
+    if (kMaskLast && k == W - 1) { cN = 0.0; rN = 0.0; dN = 0.0; }
+    const double mLast = kMaskLast && k == W - 1 ? 0.0 : 1.0;
     double Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
 #pragma unroll
     for (int r = 0; r < RPL; ++r) { Ma[r] = 0; Ia[r] = 0; Da[r] = 0; }
     const double y0 = k == 0 ? tmd[0] * initD : 0.0;
-    double yd = y0, yn = y0, xu = 0.0;
+    double yd = kMaskLast ? 0.0 : y0, yn = yd, xu = 0.0;
     double S = 0;
     const int nsteps = (int)Hmax + W - 1;
     const int mysteps = (int)H + W - 1;
@@ -777,13 +784,25 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
       }
     };
     auto send = [&](const double (&nM)[RPL], const double (&nI)[RPL], const double (&nD)[RPL]) {
-      const double x = fma(cN, nI[RPL - 1], nM[RPL - 1]);
-      const double y = fma(dN, nD[RPL - 1], fma(rN, nI[RPL - 1], nM[RPL - 1]));
-      xu = lane_down0<W>(x, k);
-      const double t = lane_down_keep<W>(y, yd, y0, k);  // yd (two steps old) dies here
-      yd = yn;
-      yn = t;
+      if constexpr (kMaskLast) {
+        const double m = nM[RPL - 1] * mLast;
+        const double x = fma(cN, nI[RPL - 1], m);
+        const double y = fma(dN, nD[RPL - 1], fma(rN, nI[RPL - 1], m));
+        xu = lane_down0<64>(x, k);  // wave_shr:1, lane 0 receives 0
+        const double t = lane_down0<64>(y, k);
+        yd = yn;
+        yn = t;
+      } else {
+        const double x = fma(cN, nI[RPL - 1], nM[RPL - 1]);
+        const double y = fma(dN, nD[RPL - 1], fma(rN, nI[RPL - 1], nM[RPL - 1]));
+        xu = lane_down0<W>(x, k);
+        const double t = lane_down_keep<W>(y, yd, y0, k);  // yd (two steps old) dies here
+        yd = yn;
+        yn = t;
+      }
     };
+    // Y as row 0 of this lane takes it: with y0 added where lane 31 sent zeros
+    auto ydue = [&]() -> double { return kMaskLast ? yd + y0 : yd; };
     double pA[NP * 2], pB[NP * 2];
     uint32_t cw = codes_at(0);
     load(cw, 0, pA);
@@ -793,23 +812,23 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
       const uint32_t cw_next = codes_at(st + 4);
       load(cw, 1, pB);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
+      sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, ydue(), xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
       if (st < mysteps) S += fma(sfac, Ib[RPL - 1], Mb[RPL - 1]);
       send(Mb, Ib, Db);
       load(cw, 2, pA);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
+      sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, ydue(), xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
       if (st + 1 < mysteps) S += fma(sfac, Ia[RPL - 1], Ma[RPL - 1]);
       send(Ma, Ia, Da);
       load(cw, 3, pB);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
+      sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, ydue(), xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
       if (st + 2 < mysteps) S += fma(sfac, Ib[RPL - 1], Mb[RPL - 1]);
       send(Mb, Ib, Db);
       cw = cw_next;
       load(cw, 0, pA);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
+      sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, ydue(), xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
       if (st + 3 < mysteps) S += fma(sfac, Ia[RPL - 1], Ma[RPL - 1]);
       send(Ma, Ia, Da);
     }
