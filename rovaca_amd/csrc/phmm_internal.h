@@ -83,14 +83,16 @@ struct RescueEntry {
 // FP64 bins: (segment width, rows per lane) chosen from the read length so
 // that no lane holds more than 12 double rows. Few bins on purpose: the FP64
 // pass is one launch per bin that can occur, whether or not any pair of the
-// batch needs it.
+// batch needs it. Bins 0-3 take reads of fewer than W * RPL bases, so their
+// last row is always a padding row (the prior-table kernel relies on it);
+// bin 4 takes reads up to the 768-base limit.
 constexpr int kNumFp64Bins = 5;
 __host__ __device__ inline int fp64_bin(uint32_t R) {
-  if (R <= 64) return 0;    // W=16, RPL 4
-  if (R <= 128) return 1;   // W=16, RPL 8
-  if (R <= 160) return 2;   // W=32, RPL 5 (150-base reads)
-  if (R <= 256) return 3;   // W=32, RPL 8
-  return 4;                 // W=64, RPL 12
+  if (R < 64) return 0;    // W=16, RPL 4
+  if (R < 128) return 1;   // W=16, RPL 8
+  if (R < 160) return 2;   // W=32, RPL 5 (150-base reads)
+  if (R < 256) return 3;   // W=32, RPL 8 (250-base reads)
+  return 4;                // W=64, RPL 12
 }
 // Rescue lists: per FP64 bin, one for pairs whose haplotype has N (list b),
 // one for the others of scaled-form regions (list kNumFp64Bins + b), which

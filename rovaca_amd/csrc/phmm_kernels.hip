@@ -29,8 +29,8 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
-#include <type_traits>
 #include <cstdint>
+#include <type_traits>
 #include <utility>
 
 #include "phmm_internal.h"
@@ -142,6 +142,24 @@ __device__ __forceinline__ double lane_down_keep(double v, double keep, double o
     const int hi = __builtin_amdgcn_update_dpp((int)(kb >> 32), (int)(vb >> 32), kDownCtrl<W>, 0xf, 0xf, false);
     return __longlong_as_double(((long long)(unsigned)lo) | ((long long)hi << 32));
   }
+}
+
+// v moved one lane down by the W-lane DPP control (row_shr for 8 and 16,
+// wave_shr:1 for 32 and 64, which crosses from lane 31 to lane 32); a lane
+// without a source lane receives 0 (bound) or 'keep'.
+template <int W>
+__device__ __forceinline__ double dpp_down(double v, double keep, bool bound) {
+  const long long vb = __double_as_longlong(v), kb = __double_as_longlong(keep);
+  int lo, hi;
+  if (bound) {
+    lo = __builtin_amdgcn_update_dpp(0, (int)(vb & 0xffffffffLL), kDownCtrl<W>, 0xf, 0xf, true);
+    hi = __builtin_amdgcn_update_dpp(0, (int)(vb >> 32), kDownCtrl<W>, 0xf, 0xf, true);
+  } else {
+    lo = __builtin_amdgcn_update_dpp((int)(kb & 0xffffffffLL), (int)(vb & 0xffffffffLL), kDownCtrl<W>, 0xf, 0xf,
+                                     false);
+    hi = __builtin_amdgcn_update_dpp((int)(kb >> 32), (int)(vb >> 32), kDownCtrl<W>, 0xf, 0xf, false);
+  }
+  return __longlong_as_double(((long long)(unsigned)lo) | ((long long)hi << 32));
 }
 
 template <typename T>
@@ -715,11 +733,12 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
     const bool active = e < count;
     const RescueEntry en = a.list[active ? e : t];
     const uint32_t H = a.s.hap_len[en.hap];
-    uint32_t Hmax = H;
+    uint32_t Hmax = H, Hmin = H;
 #pragma unroll
     for (int g = 0; g < NSEG; ++g) {
       const uint32_t h2 = (uint32_t)__shfl((int)H, g * W);
       Hmax = h2 > Hmax ? h2 : Hmax;
+      Hmin = h2 < Hmin ? h2 : Hmin;
     }
     __syncthreads();
     {
@@ -745,26 +764,30 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
     __syncthreads();
 
     const double initD = pow2d((int)en.x + 900) / (double)H;
-    const double sfac = sum_factor<double, W, RPL, kScaled>(a.s, ph2pr, en.read);
+    // The 16- and 32-lane bins hold reads of fewer than W * RPL bases, so a
+    // segment's last row is a padding row: M = 0, I~ carries M + I of the
+    // read's last row (its sum term), D~ holds what it starts with. The
+    // 64-lane bin takes reads of up to W * RPL bases and sums M + f * I.
+    constexpr bool kPad = W < 64;
+    const double sfac = kPad ? 1.0 : sum_factor<double, W, RPL, kScaled>(a.s, ph2pr, en.read);
     // Row 0 takes X and Y premixed by the lane above with this lane's row-0
     // coefficients (as in the FP32 kernel): cN, rN, dN are the next lane's.
     // A segment's first lane sees row -1 (X = 0, Y = delta * initD of its
-    // own pair). In 16- and 64-lane segments DPP bound control gives it
-    // X = 0 and the kept register Y. In 32-lane segments the wave-wide move
-    // takes lane 31's values to lane 32, the second pair's first lane: there
-    // lane 31 sends zeros (its M masked, its next-lane coefficients 0) and
-    // every first lane adds its own row -1 term y0 to the Y it receives
-    // (Y + 0 is Y itself, so the results stay bit-identical), which costs
-    // two FP64 operations a step instead of four selects and two copies.
-    double cN = lane_from(tii[0], 1), rN = lane_from(tmi[0], 1), dN = lane_from(tmd[0], 1);
-    constexpr bool kMaskLast = W == 32;
-    if (kMaskLast && k == W - 1) { cN = 0.0; rN = 0.0; dN = 0.0; }
-    const double mLast = kMaskLast && k == W - 1 ? 0.0 : 1.0;
+    // own pair). DPP bound control gives lane 0 of the move X = 0 and the
+    // kept register Y (16-lane rows, the 64-lane wave). In the 32-lane
+    // wave-wide move lane 32 receives lane 31's values instead: lane 31's
+    // padding row, with M = 0, next-lane c = rho = 0 and D~ held at the
+    // second pair's initD, sends exactly X = 0 and Y = delta * initD.
+    const double initDn = lane_from(initD, 1);
+    double cN = lane_from(tii[0], 1), rN = lane_from(tmi[0], 1);
+    const double dN = lane_from(tmd[0], 1);
+    if (kPad && k == W - 1) { cN = 0.0; rN = 0.0; }
     double Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
 #pragma unroll
     for (int r = 0; r < RPL; ++r) { Ma[r] = 0; Ia[r] = 0; Da[r] = 0; }
+    if (kPad && k == W - 1) Da[RPL - 1] = initDn;
     const double y0 = k == 0 ? tmd[0] * initD : 0.0;
-    double yd = kMaskLast ? 0.0 : y0, yn = yd, xu = 0.0;
+    double yd = y0, yn = y0, xu = 0.0;
     double S = 0;
     const int nsteps = (int)Hmax + W - 1;
     const int mysteps = (int)H + W - 1;
@@ -784,54 +807,59 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
       }
     };
     auto send = [&](const double (&nM)[RPL], const double (&nI)[RPL], const double (&nD)[RPL]) {
-      if constexpr (kMaskLast) {
-        const double m = nM[RPL - 1] * mLast;
-        const double x = fma(cN, nI[RPL - 1], m);
-        const double y = fma(dN, nD[RPL - 1], fma(rN, nI[RPL - 1], m));
-        xu = lane_down0<64>(x, k);  // wave_shr:1, lane 0 receives 0
-        const double t = lane_down0<64>(y, k);
+      const double x = fma(cN, nI[RPL - 1], nM[RPL - 1]);
+      const double y = fma(dN, nD[RPL - 1], fma(rN, nI[RPL - 1], nM[RPL - 1]));
+      if constexpr (kPad) {
+        xu = dpp_down<W>(x, 0.0, true);
+        const double t = dpp_down<W>(y, yd, false);  // yd (two steps old) dies here
         yd = yn;
         yn = t;
       } else {
-        const double x = fma(cN, nI[RPL - 1], nM[RPL - 1]);
-        const double y = fma(dN, nD[RPL - 1], fma(rN, nI[RPL - 1], nM[RPL - 1]));
         xu = lane_down0<W>(x, k);
-        const double t = lane_down_keep<W>(y, yd, y0, k);  // yd (two steps old) dies here
+        const double t = lane_down_keep<W>(y, yd, y0, k);
         yd = yn;
         yn = t;
       }
     };
-    // Y as row 0 of this lane takes it: with y0 added where lane 31 sent zeros
-    auto ydue = [&]() -> double { return kMaskLast ? yd + y0 : yd; };
+    // the sum term of the last row: I~ of the padding row, or M + f * I
+    auto term = [&](const double (&nM)[RPL], const double (&nI)[RPL]) -> double {
+      return kPad ? nI[RPL - 1] : fma(sfac, nI[RPL - 1], nM[RPL - 1]);
+    };
     double pA[NP * 2], pB[NP * 2];
     uint32_t cw = codes_at(0);
     load(cw, 0, pA);
     // four steps a -> b -> a -> b -> a per iteration, the table reads one
-    // step ahead, the next codes word read once per iteration
-    for (int st = 0; st < nsteps; st += 4) {
+    // step ahead, the next codes word read once per iteration; while every
+    // segment of the wave is inside its own columns the sums need no test
+    auto iter4 = [&](int st, auto guarded) {
+      constexpr bool G = decltype(guarded)::value;
       const uint32_t cw_next = codes_at(st + 4);
       load(cw, 1, pB);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, ydue(), xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
-      if (st < mysteps) S += fma(sfac, Ib[RPL - 1], Mb[RPL - 1]);
+      sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
+      if (!G || st < mysteps) S += term(Mb, Ib);
       send(Mb, Ib, Db);
       load(cw, 2, pA);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, ydue(), xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
-      if (st + 1 < mysteps) S += fma(sfac, Ia[RPL - 1], Ma[RPL - 1]);
+      sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
+      if (!G || st + 1 < mysteps) S += term(Ma, Ia);
       send(Ma, Ia, Da);
       load(cw, 3, pB);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, ydue(), xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
-      if (st + 2 < mysteps) S += fma(sfac, Ib[RPL - 1], Mb[RPL - 1]);
+      sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
+      if (!G || st + 2 < mysteps) S += term(Mb, Ib);
       send(Mb, Ib, Db);
       cw = cw_next;
       load(cw, 0, pA);
       __builtin_amdgcn_sched_barrier(0);
-      sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, ydue(), xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
-      if (st + 3 < mysteps) S += fma(sfac, Ia[RPL - 1], Ma[RPL - 1]);
+      sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
+      if (!G || st + 3 < mysteps) S += term(Ma, Ia);
       send(Ma, Ia, Da);
-    }
+    };
+    const int minsteps = (int)Hmin + W - 1;
+    int st = 0;
+    for (; st + 4 <= minsteps; st += 4) iter4(st, std::false_type{});
+    for (; st < nsteps; st += 4) iter4(st, std::true_type{});
     if (active && k == W - 1) {
       a.out[en.out] = log10(S * pow2d(120 - (int)en.x)) - 307.050595577260822;
       a.flags[en.out] = 3;
