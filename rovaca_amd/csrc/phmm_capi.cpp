@@ -114,7 +114,7 @@ int region_sync_impl(const rvc_region* region, double* out, uint8_t* flags, cons
   // memory, one wait; the FP64 launches only where pairs need them
   const size_t n = w->pack.n_pairs_slots;
   if ((rc = w->results.reserve(w->batch.fetch_bytes() + 64))) return rc;
-  if ((rc = w->batch.run_and_fetch(w->stream, w->results.ptr))) return rc;
+  if ((rc = w->batch.run_and_fetch(w->stream, w->results.ptr, true))) return rc;
   const double* h_out = reinterpret_cast<const double*>(w->results.ptr);
   const uint8_t* h_flags = reinterpret_cast<const uint8_t*>(w->results.ptr + n * sizeof(double));
   unpermute_results(w->pack, h_out, h_flags, out, flags);

@@ -1120,7 +1120,7 @@ int DeviceBatch::upload_staged(const HostPack& P, const BatchLayout& L, const vo
   // one region's image: a kernel reads the pinned staging (about 3 us sooner
   // than a copy engine whose end the sweep must wait for); larger images go
   // by DMA
-  if (src_pinned && up <= kKernelUploadMax) RVC_HIP(launch_upload(base, host, up, s));
+  if (src_pinned && up <= kKernelUploadMax) RVC_HIP(launch_copy(base, host, up, s));
   else RVC_HIP(hipMemcpyAsync(base, host, up, hipMemcpyHostToDevice, s));
   init_pending = !with_results;
   counters_clear = with_results;
@@ -1285,11 +1285,12 @@ size_t DeviceBatch::fetch_bytes() const {
   return sizeof(double) * n + ((n + 7) & ~size_t(7)) + 4 * kNumCounters;
 }
 
-int DeviceBatch::run_and_fetch(hipStream_t s, char* host) {
+int DeviceBatch::run_and_fetch(hipStream_t s, char* host, bool host_pinned) {
   const size_t n = pack->n_pairs_slots;
   int rc = enqueue(s, false, kNoFp64);
   if (rc) return rc;
-  RVC_HIP(hipMemcpyAsync(host, out, fetch_bytes(), hipMemcpyDeviceToHost, s));
+  if (host_pinned) RVC_HIP(launch_copy(host, out, fetch_bytes(), s));
+  else RVC_HIP(hipMemcpyAsync(host, out, fetch_bytes(), hipMemcpyDeviceToHost, s));
   RVC_HIP(hipStreamSynchronize(s));
   const uint32_t* cnt = reinterpret_cast<const uint32_t*>(host + sizeof(double) * n + ((n + 7) & ~size_t(7)));
   uint32_t rescued[kNumFp64Lists];
@@ -1300,7 +1301,8 @@ int DeviceBatch::run_and_fetch(hipStream_t s, char* host) {
   for (uint32_t c : rescued) n_rescued += c;
   if (!any) return RVC_OK;
   if ((rc = enqueue_fp64(s, false, rescued))) return rc;
-  RVC_HIP(hipMemcpyAsync(host, out, n * (sizeof(double) + 1), hipMemcpyDeviceToHost, s));
+  if (host_pinned) RVC_HIP(launch_copy(host, out, n * (sizeof(double) + 1), s));
+  else RVC_HIP(hipMemcpyAsync(host, out, n * (sizeof(double) + 1), hipMemcpyDeviceToHost, s));
   RVC_HIP(hipStreamSynchronize(s));
   return RVC_OK;
 }

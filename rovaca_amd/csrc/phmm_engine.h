@@ -297,10 +297,12 @@ struct DeviceBatch {
   size_t fetch_bytes() const;
   // The whole launch sequence for a caller that waits for the results: the
   // FP32 pass, one copy of results, flags and counters into host (fetch_bytes
-  // of room, pinned for an asynchronous copy), a wait; then, only if the
-  // counters show pairs below 1e-28, the FP64 launches of those lists and a
-  // second copy of results and flags. Returns when host holds the results.
-  int run_and_fetch(hipStream_t s, char* host);
+  // of room), a wait; then, only if the counters show pairs below 1e-28, the
+  // FP64 launches of those lists and a second copy of results and flags.
+  // Returns when host holds the results. host_pinned: host is pinned
+  // (hipHostMalloc), and a kernel on s writes the copies there (launch_copy);
+  // otherwise they go through a copy engine.
+  int run_and_fetch(hipStream_t s, char* host, bool host_pinned);
   int ensure_events();
   // Runs the launch sequence on 'stream' (optionally as a hipGraph) and waits.
   int run(bool use_graph);

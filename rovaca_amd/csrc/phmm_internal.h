@@ -183,8 +183,12 @@ hipError_t launch_fp32(int W, int RPL, bool scaled, int occ, bool pt, int grid, 
 // N in the haplotype, scaled form only), scaled = the scaled form.
 hipError_t launch_fp64(int bin, bool pt, bool scaled, int grid, hipStream_t stream, const Fp64Args& a);
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
-// n bytes from pinned host memory src to device memory dst, by a kernel on stream.
-hipError_t launch_upload(void* dst, const void* src, uint64_t n, hipStream_t stream);
+// n bytes from src to dst by a kernel on stream; each side is device memory
+// or pinned host memory (mapped into the device's address space). Unlike a
+// DMA copy it waits on no other stream's copies: the copy engines serve
+// their queues in order, so a DMA copy queued behind another stream's copy
+// that waits on that stream's kernels would wait for those kernels too.
+hipError_t launch_copy(void* dst, const void* src, uint64_t n, hipStream_t stream);
 // out[0 .. n) = NaN and the n flag bytes behind them (rounded up to 8) = 0.
 hipError_t launch_init_results(double* out, uint64_t n, hipStream_t stream);
 // The start of a launch sequence: p[0 .. n) = 0 and, for n_out > 0, the

@@ -971,11 +971,16 @@ __global__ __launch_bounds__(256) void k_marginalize(MargArgs a) {
 
 // Host-to-device copy of a small staged image by a kernel that reads pinned
 // host memory (16 bytes per lane, then the tail bytes).
-__global__ void k_upload(char* dst, const char* src, uint64_t n) {
-  const uint64_t n16 = n >> 4;
+__global__ void k_copy(char* dst, const char* src, uint64_t n) {
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = tid; i < n16; i += stride) reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(src)[i];
-  for (uint64_t i = (n16 << 4) + tid; i < n; i += stride) dst[i] = src[i];
+  uint64_t done = 0;
+  if (((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15) == 0) {
+    const uint64_t n16 = n >> 4;
+    for (uint64_t i = tid; i < n16; i += stride)
+      reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(src)[i];
+    done = n16 << 4;
+  }
+  for (uint64_t i = done + tid; i < n; i += stride) dst[i] = src[i];
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -1082,10 +1087,11 @@ Fp32Fn fp32_fn(int W, int RPL, bool xf, int occ, bool pt) {
 
 bool fp32_supported(int W, int RPL) { return fp32_fn(W, RPL, false, 2, false) != nullptr; }
 
-hipError_t launch_upload(void* dst, const void* src, uint64_t n, hipStream_t stream) {
+hipError_t launch_copy(void* dst, const void* src, uint64_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
   const uint64_t n16 = (n + 15) >> 4;
   const unsigned blocks = (unsigned)std::min<uint64_t>(1024, std::max<uint64_t>(1, (n16 + 255) / 256));
-  hipLaunchKernelGGL(k_upload, dim3(blocks), dim3(256), 0, stream, static_cast<char*>(dst),
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, stream, static_cast<char*>(dst),
                      static_cast<const char*>(src), n);
   return hipGetLastError();
 }

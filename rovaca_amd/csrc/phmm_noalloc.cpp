@@ -112,7 +112,7 @@ int region_sync_noalloc(const rvc_region* region, double* out, uint8_t* flags) {
   const size_t n = g_na.pack.n_pairs_slots;
   // the FP32 pass, one copy back of results, flags and counters, one wait;
   // the FP64 launches only where pairs need them
-  if ((rc = b.run_and_fetch(g_na.stream, g_na.host))) return rc;
+  if ((rc = b.run_and_fetch(g_na.stream, g_na.host, true))) return rc;
   const double* h_out = reinterpret_cast<const double*>(g_na.host);
   const uint8_t* h_flags = reinterpret_cast<const uint8_t*>(g_na.host + n * sizeof(double));
   unpermute_results(g_na.pack, h_out, h_flags, out, flags);

@@ -327,9 +327,10 @@ int launch_slot(Slot& s) {
     s.marg_dev = d;
     s.marg_pending = true;
   }
-  // results, flags and counters sit one after another: one copy
-  if (hipMemcpyAsync(s.res->out(), s.batch.out, s.batch.fetch_bytes(), hipMemcpyDeviceToHost, s.stream) !=
-          hipSuccess ||
+  // results, flags and counters sit one after another: one copy, by a
+  // kernel into the pinned result buffer (a DMA copy could wait behind
+  // another slot's copy, and so for that slot's kernels)
+  if (launch_copy(s.res->out(), s.batch.out, s.batch.fetch_bytes(), s.stream) != hipSuccess ||
       hipEventRecord(s.done_ev, s.stream) != hipSuccess)
     return set_error(RVC_E_DEVICE, "result copy");
   return RVC_OK;
@@ -350,16 +351,15 @@ int launch_tail(Slot& s, bool* queued) {
   if (any) {
     if ((rc = s.batch.enqueue_fp64(s.stream, true, rescued))) return rc;
     const size_t n = s.meta.n_pairs_slots;
-    if (hipMemcpyAsync(s.res->out(), s.batch.out, n * (sizeof(double) + 1), hipMemcpyDeviceToHost, s.stream) !=
-        hipSuccess)
+    if (launch_copy(s.res->out(), s.batch.out, n * (sizeof(double) + 1), s.stream) != hipSuccess)
       return set_error(RVC_E_DEVICE, "result copy");
     *queued = true;
   }
   if (s.marg_pending) {
     s.marg_pending = false;
-    if (hipMemcpyAsync(s.marg_dev, s.marg_in.ptr, s.marg_in_bytes, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
+    if (launch_copy(s.marg_dev, s.marg_in.ptr, s.marg_in_bytes, s.stream) != hipSuccess ||
         launch_marginalize(s.marg, s.stream) != hipSuccess ||
-        hipMemcpyAsync(s.marg_out.ptr, s.marg.out, s.marg_out_bytes, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+        launch_copy(s.marg_out.ptr, s.marg.out, s.marg_out_bytes, s.stream) != hipSuccess)
       return set_error(RVC_E_DEVICE, "marginalisation launch");
     *queued = true;
   }
