@@ -227,7 +227,7 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
 // Persistent grid of FP32 bin v's kernel for this batch.
 uint32_t fp32_grid(const HostPack& P, int v);
 // Haplotypes per tile for FP32 bin v: long runs amortise a tile's set-up,
-// but every persistent wave needs a few dozen tiles so that all end together.
+// but every persistent wave needs about a dozen tiles so that all end together.
 uint32_t run_for_bin(const HostPack& P, int v, uint64_t units);
 // Where each FP32 bin's tiles go (to[v]): a bin of few cells into the next
 // larger non-empty shape of its class and width, if that has at most a third
