@@ -232,7 +232,9 @@ def test_mixed_forms_equal_alone():
         assert np.array_equal(a[~np.isnan(a)], m[~np.isnan(m)])
 
 
-@pytest.mark.parametrize("R", [63, 64, 65, 128, 129, 160, 161, 256, 512, 767, 768])
+# the FP64 bins hold reads of fewer than 64, 128, 160 and 256 bases (their
+# last row is a padding row; one at 63, 127, 159, 255) and up to 768
+@pytest.mark.parametrize("R", [63, 64, 65, 127, 128, 129, 159, 160, 161, 255, 256, 257, 512, 767, 768])
 def test_rescue_bin_edges(R):
     say(f"FP64 rescue at read length {R} (edges of the rescue bins' row counts)")
     rng = np.random.default_rng(1000 + R)
