@@ -9,13 +9,17 @@
 // W = 16, wave_shr:1 for W = 32/64), which feeds the first row of the next
 // lane one step later; lane 0 of a segment takes the row -1 boundary
 // (M = I = 0, D = C/H) from the DPP 'old' operand. The haplotype is staged in
-// LDS as one code byte per base; lane k reads byte (s - k) each step. The
-// FP32 prior-table kernel moves two values instead: the lane above mixes its
-// last row with the receiving lane's row-0 coefficients (sweep_step_xy).
+// LDS as one code byte per base (compare kernels) or as 2-bit codes
+// (prior-table kernels); lane k reads column (s - k) each step. The
+// prior-table kernels (FP32 and FP64) move two values instead: the lane above
+// mixes its last row with the receiving lane's row-0 coefficients
+// (sweep_step_xy).
 //
 // Rows past the end of a read are padding rows that carry the sum
 // M[R-1][j] + I[R-1][j] down unchanged (prior 0, I <- M + I), so the result
-// always comes out of the segment's last lane and last row register.
+// always comes out of the segment's last lane and last row register (the
+// 64-lane FP64 bin also takes reads that fill every row: there the last row's
+// M + I is summed directly).
 //
 // Recurrence and boundaries follow Rovaca's kernels:
 //   ph/roDV_inter/pairhmm_inter.cpp:16-102 (per-row parameters), :170-177
@@ -712,7 +716,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
 // the scaled form: the prior-table sweep in double (2 rows per ds_read_b128).
 // The segments of a wave hold different pairs, so each segment stages its own
 // haplotype stream and the wave sweeps to the longest of them; a segment adds
-// to its sum only while its own columns last.
+// to its sum only while its own columns last (the test is left out of the
+// loop while every segment is inside its own haplotype).
 template <int W, int RPL>
 __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
   constexpr int NSEG = 64 / W;
