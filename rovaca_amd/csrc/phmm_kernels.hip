@@ -135,18 +135,6 @@ __device__ __forceinline__ double lane_down0(double v, int k) {
   if constexpr (W == 32) return k == 0 ? 0.0 : r;
   return r;
 }
-template <int W>
-__device__ __forceinline__ double lane_down_keep(double v, double keep, double old, int k) {
-  if constexpr (W == 32) {
-    return lane_down<W>(v, old, k);
-  } else {
-    const long long vb = __double_as_longlong(v), kb = __double_as_longlong(keep);
-    const int lo = __builtin_amdgcn_update_dpp((int)(kb & 0xffffffffLL), (int)(vb & 0xffffffffLL), kDownCtrl<W>, 0xf,
-                                               0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(kb >> 32), (int)(vb >> 32), kDownCtrl<W>, 0xf, 0xf, false);
-    return __longlong_as_double(((long long)(unsigned)lo) | ((long long)hi << 32));
-  }
-}
 
 // v moved one lane down by the W-lane DPP control (row_shr for 8 and 16,
 // wave_shr:1 for 32 and 64, which crosses from lane 31 to lane 32); a lane
@@ -814,17 +802,10 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
     auto send = [&](const double (&nM)[RPL], const double (&nI)[RPL], const double (&nD)[RPL]) {
       const double x = fma(cN, nI[RPL - 1], nM[RPL - 1]);
       const double y = fma(dN, nD[RPL - 1], fma(rN, nI[RPL - 1], nM[RPL - 1]));
-      if constexpr (kPad) {
-        xu = dpp_down<W>(x, 0.0, true);
-        const double t = dpp_down<W>(y, yd, false);  // yd (two steps old) dies here
-        yd = yn;
-        yn = t;
-      } else {
-        xu = lane_down0<W>(x, k);
-        const double t = lane_down_keep<W>(y, yd, y0, k);
-        yd = yn;
-        yn = t;
-      }
+      xu = dpp_down<W>(x, 0.0, true);
+      const double t = dpp_down<W>(y, yd, false);  // yd (two steps old) dies here
+      yd = yn;
+      yn = t;
     };
     // the sum term of the last row: I~ of the padding row, or M + f * I
     auto term = [&](const double (&nM)[RPL], const double (&nI)[RPL]) -> double {
