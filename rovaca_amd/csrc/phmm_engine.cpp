@@ -416,6 +416,7 @@ struct PackScratch {
   std::vector<uint32_t> order, idx;
   std::vector<Tile> out, bin;
   std::vector<uint32_t> start, cost;
+  std::vector<uint32_t> len_at;  // counting sort of a region's reads by length
 };
 PackScratch& pack_scratch() {
   thread_local PackScratch s;
@@ -570,13 +571,29 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
         if (P.hap_nmask[w]) hap_has_n[h] = 1;
       hcur += align_up(len, kSeqAlign);
     }
-    // reads of a region in descending length, so that tiles group equal lengths
+    // reads of a region in descending length, so that tiles group equal
+    // lengths, in input order among equal lengths: as they come when they
+    // already descend, else by a counting sort over the lengths (at most
+    // kMaxReadLen in a validated region; a longer one falls back to a sort)
     std::vector<uint32_t>& order = W.order;
     order.resize(g.n_reads);
+    auto rlen = [&](uint32_t x) { return (uint32_t)(g.read_off[x + 1] - g.read_off[x]); };
+    bool descending = true;
+    uint32_t longest = 0;
+    for (uint32_t x = 0; x < g.n_reads; ++x) {
+      longest = std::max(longest, rlen(x));
+      if (x > 0 && rlen(x) > rlen(x - 1)) descending = false;
+    }
     for (uint32_t x = 0; x < g.n_reads; ++x) order[x] = x;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-      return (g.read_off[a + 1] - g.read_off[a]) > (g.read_off[b + 1] - g.read_off[b]);
-    });
+    if (!descending && longest <= (uint32_t)kMaxReadLen) {
+      std::vector<uint32_t>& at = W.len_at;
+      at.assign(kMaxReadLen + 2, 0);
+      for (uint32_t x = 0; x < g.n_reads; ++x) ++at[kMaxReadLen - rlen(x) + 1];
+      for (int l = 1; l <= kMaxReadLen + 1; ++l) at[l] += at[l - 1];
+      for (uint32_t x = 0; x < g.n_reads; ++x) order[at[kMaxReadLen - rlen(x)]++] = x;
+    } else if (!descending) {
+      std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return rlen(a) > rlen(b); });
+    }
     // range of the region's gap-open qualities (insertion and deletion) and
     // its smallest gap-continuation quality: they fix the region's form
     uint8_t gop_lo = 127, gop_hi = 0, gcp_lo = 127;
@@ -629,6 +646,15 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
         uint32_t nseg = 0;
         while (nseg < nseg_max && sx + nseg < g.n_reads && P.read_len[L.read0 + sx + nseg] > 0) ++nseg;
         const int v0 = fp32_variant_index(W, RPL);
+        // the read group's bases and its reads per FP64 bin: a tile's cells
+        // are (sum of read lengths) x (sum of haplotype lengths)
+        uint64_t rsum = 0;
+        uint32_t per_bin[kNumFp64Bins] = {};
+        for (uint32_t s = 0; s < nseg; ++s) {
+          const uint32_t rl = P.read_len[L.read0 + sx + s];
+          rsum += rl;
+          ++per_bin[fp64_bin(rl)];
+        }
         // runs of consecutive non-empty haplotypes of this kind, at most
         // run_max per tile
         uint32_t h = 0;
@@ -651,16 +677,12 @@ int build_host_pack(const rvc_region* regions, uint32_t n_regions, const FP32Pol
           bins[v].push_back(t);
           bin_cost[v].push_back(cost);
           P.bin_units[v] += nh;
-          for (uint32_t hh = h; hh < h + nh; ++hh) {
-            const uint32_t hl = P.hap_len[L.hap0 + hh];
-            for (uint32_t s = 0; s < nseg; ++s) {
-              const uint32_t rl = P.read_len[L.read0 + sx + s];
-              P.fp64_cap[cls * kNumFp64Bins + fp64_bin(rl)] += 1;
-              P.n_pairs += 1;
-              P.n_cells += (uint64_t)rl * hl;
-              P.variant_cells[v] += (uint64_t)rl * hl;
-            }
-          }
+          uint64_t hsum = 0;
+          for (uint32_t hh = h; hh < h + nh; ++hh) hsum += P.hap_len[L.hap0 + hh];
+          for (int b = 0; b < kNumFp64Bins; ++b) P.fp64_cap[cls * kNumFp64Bins + b] += per_bin[b] * nh;
+          P.n_pairs += (uint64_t)nseg * nh;
+          P.n_cells += rsum * hsum;
+          P.variant_cells[v] += rsum * hsum;
           h += nh;
         }
         sx += nseg;
