@@ -90,6 +90,41 @@ def accuracy(regions, batch, out, sample):
             "mean_abs_vs_reference_routing": float(dref.mean())}
 
 
+def pair_accuracy(reg, got, pairs=None, routing=True):
+    """|delta log10 L| of one region's result matrix (reads x haplotypes)
+    against the FP64 oracle and Rovaca's own intra kernel (built from its
+    sources into oracle/_ref; FP32 first, FP64 below 1e-28) on every pair or
+    the (read, haplotype) pairs given, and against the reference routing on
+    the whole region."""
+    import oracle
+    d64, dri = [], []
+    if pairs is None:
+        pairs = [(r, h) for r in range(reg["n_reads"]) for h in range(reg["n_haps"])]
+    have_ref = oracle.ref() is not None
+    for r, h in pairs:
+        o0, o1 = reg["read_off"][r], reg["read_off"][r + 1]
+        hb = reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]]
+        if o1 == o0 or len(hb) == 0:
+            continue
+        rd = reg["read_bases"][o0:o1]
+        args = [reg[k][o0:o1] for k in "qidc"]
+        d64.append(abs(got[r, h] - oracle.pair_log10_f64(hb, rd, *args)))
+        if have_ref:
+            dri.append(abs(got[r, h] - oracle.ref_intra_log10(hb, rd, *args)))
+    out = {"sample_pairs": len(d64), "max_abs_vs_fp64_oracle": float(np.max(d64)),
+           "mean_abs_vs_fp64_oracle": float(np.mean(d64))}
+    if dri:
+        out["max_abs_vs_reference_intra"] = float(np.max(dri))
+        out["mean_abs_vs_reference_intra"] = float(np.mean(dri))
+    if routing:
+        want, route, ok = oracle.region_reference(reg)
+        m = ~np.isnan(want)
+        d = np.abs(got[m] - want[m])
+        out["max_abs_vs_reference_routing"] = float(d.max())
+        out["mean_abs_vs_reference_routing"] = float(d.mean())
+    return out
+
+
 def cpu_baseline(regions, target_seconds):
     """Rovaca's region scheme (three passes, 16-lane inter kernel, Rovaca's own
     intra kernel for pass 3) on the host cores, over a bounded sample: the
@@ -152,12 +187,15 @@ def run_single(args):
             "config": {"workload": "single_region_8x32", "cells": rec["cells"]}, **rec}
 
 
-def served_stream(pool, deep, rates, duration, producers, warmup_s=0.3):
+def served_stream(pool, deep, rates, duration, producers, warmup_s=0.3, check=None):
     """Poisson arrivals of pool regions (and 1 % deep ones if given) from
     'producers' host threads at each rate, through the serving loop. Before
     each timed window a short untimed stream at the same rate (and with the
     same mix) sizes the slot arenas, staging and result buffers, so that the
-    timed window reallocates nothing (buffer_grows reports it)."""
+    timed window reallocates nothing (buffer_grows reports it). check = n:
+    after the timed windows, n arrivals at 8k/s with their outputs kept, each
+    compared bit for bit with the region computed alone (untimed); the
+    count goes into the returned stats as 'served_check'."""
     import rovaca_amd as rv
     regions = pool + deep
     rv.init(0, server=True, hbm_budget_bytes=64 << 30, max_batch_regions=4096)
@@ -203,6 +241,16 @@ def served_stream(pool, deep, rates, duration, producers, warmup_s=0.3):
         log(f"rate {rate:.0f}/s: {rec['gcups']} GCUPS, p50 {rec['p50_ms']} ms, p99 {rec['p99_ms']} ms, "
             f"{rec['launches']} launches, {rec['buffer_grows']} buffer grows")
     st = rv.server_stats()
+    if check:
+        arr, order = arrivals(8000.0, check)
+        order = order % len(pool)
+        _, _, outs = rv.loadgen_run(regions, arr, order, producers, outputs=True)
+        alone = {int(x): rv.region_sync(pool[int(x)])[0] for x in set(order.tolist())}
+        same = sum(1 for i in range(len(order))
+                   if np.array_equal(np.isnan(alone[int(order[i])]), np.isnan(outs[i]))
+                   and np.array_equal(alone[int(order[i])][~np.isnan(outs[i])], outs[i][~np.isnan(outs[i])]))
+        st["served_check"] = {"arrivals": int(len(order)), "rate_per_s": 8000.0,
+                              "bit_identical_to_region_alone": int(same)}
     rv.shutdown()
     return out, st
 
@@ -229,6 +277,8 @@ def single_region_config(no_cpu, reps=400):
     native_na = rv.region_latency(reg, reps, noalloc=True)
     for _ in range(20):
         rv.region_sync(reg)
+    got, _ = rv.region_sync(reg)
+    acc = pair_accuracy(reg, got)
     lat = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -256,7 +306,8 @@ def single_region_config(no_cpu, reps=400):
            "caller": "C++ (rvc_loadgen_region_latency)",
            "noalloc_median_ms": round(float(np.median(native_na)), 4),
            "python_median_ms": round(float(np.median(lat)), 4), "python_p99_ms": round(float(np.percentile(lat, 99)), 4),
-           "resident_launch_sequence_ms": {"eager": seq["eager"], "hipgraph_replay": seq["graph"]}}
+           "resident_launch_sequence_ms": {"eager": seq["eager"], "hipgraph_replay": seq["graph"]},
+           "accuracy": acc}
     if not no_cpu:
         import oracle
         oracle.baseline_run([reg], 1)
@@ -370,6 +421,12 @@ def deep_config(pool, args):
     import rovaca_amd as rv
     mix = pool[:99] + deep[:1]
     rv.init(0)
+    # accuracy on 256 seeded pairs of a deep region (the whole region is
+    # 6.4e10 cells, beyond the oracle's reach in a bench run)
+    got, _ = rv.region_sync(deep[0])
+    rng = np.random.default_rng(77)
+    sample = list(zip(rng.integers(0, deep[0]["n_reads"], 256).tolist(), rng.integers(0, deep[0]["n_haps"], 256).tolist()))
+    rec["accuracy"] = pair_accuracy(deep[0], got, sample, routing=False)
     rv.region_sync(mix[0])
     t0 = time.perf_counter()
     for r in mix:
@@ -729,11 +786,12 @@ def main():
         # the same regions arriving one by one through the serving loop
         log("served stream")
         rates = [float(x) for x in args.rates.split(",")]
-        srv, _ = served_stream(regions, [], rates, args.duration, args.producers)
+        srv, sst = served_stream(regions, [], rates, args.duration, args.producers, check=400)
         result["served"] = {"pool": wname, "producers": args.producers, "arrivals": "poisson",
                             "duration_s_per_rate": args.duration, "hbm_budget_gib": 64,
                             "warmup": "0.3 s of arrivals at each rate before its window", "rates": srv,
-                            "highest_rate_p99_under_10ms": highest_rate_under(srv)}
+                            "highest_rate_p99_under_10ms": highest_rate_under(srv),
+                            "accuracy": sst.get("served_check")}
         if "plain_gpu" in result:
             result["served"]["top_rate_over_plain"] = round(srv[-1]["gcups"] / result["plain_gpu"]["value"], 2)
     if not args.no_cpu:
