@@ -7,9 +7,11 @@
 // buffer and device arena, take work from one queue: a free slot admits every
 // waiting request, in arrival order, whose device footprint still fits the
 // HBM budget next to the requests already in flight, merges them straight into
-// its pinned staging buffer in device layout, and queues one host-to-device
-// copy, the FP32 pass, the FP64 rescue and the copy of the results on its
-// stream. Slots work concurrently, so requests that arrive while earlier
+// its pinned staging buffer in device layout, and queues the upload, the FP32
+// pass and one copy of results, flags and rescue counts (by a kernel, into a
+// pinned result buffer) on its stream; once those counts are back it queues
+// only the FP64 lists that hold pairs and the marginalisation of requests
+// with sites. Slots work concurrently, so requests that arrive while earlier
 // launches are on the device are staged and started while those finish.
 // Results land in the slot's pinned result buffer; when the launch finishes,
 // the slot thread copies each request's region out (in input read order) and
