@@ -382,12 +382,17 @@ __device__ __forceinline__ T forward(const DevSeqs& s, const DevTables& tb, cons
 // Result of one FP32 pair (output slot o): log10 likelihood, or a place on
 // the FP64 list.
 // x is the pair's scale exponent (S already brought back to the 2^120 scale).
-__device__ __forceinline__ void finish_fp32(const Fp32Args& a, uint32_t o, uint32_t hap, uint32_t read, uint32_t x,
-                                            float S) {
+__device__ __forceinline__ bool store_fp32(const Fp32Args& a, uint32_t o, float S) {
   if (S >= 1e-28f && S <= 0x1.fffffep127f) {
     a.out[o] = log10((double)S) - (double)36.1236000061f;
     a.flags[o] = 1;
-  } else {
+    return true;
+  }
+  return false;
+}
+__device__ __forceinline__ void finish_fp32(const Fp32Args& a, uint32_t o, uint32_t hap, uint32_t read, uint32_t x,
+                                            float S) {
+  if (!store_fp32(a, o, S)) {
     const int b = (int)a.rescue_list0 + fp64_bin(a.s.read_len[read]);
     const uint32_t idx = atomicAdd(a.rescue_count + b, 1u);
     RescueEntry e;
@@ -441,6 +446,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
 // in front); a funnel shift yields the codes of 4 consecutive columns.
 // ---------------------------------------------------------------------------
 constexpr int kHapWords = 112;  // 64 pad columns + 1536 bases + tail, 16 per word
+// Most consecutive rescue-list entries one segment of the FP64 prior-table
+// kernel takes per fetch (fewer on short lists).
+constexpr uint32_t kFp64Chunk = 8;
 
 template <typename T, int RPL, int F>
 __device__ __forceinline__ void sweep_step_pt(const T (&M)[RPL], const T (&I)[RPL], const T (&D)[RPL], T (&nM)[RPL],
@@ -564,6 +572,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     };
     uint32_t words[2];
     hap_words(tl.hap, words);
+    uint64_t resc = 0;  // the run's haplotypes whose pair goes to the FP64 rescue
     for (uint32_t h = 0; h < nh; ++h) {
       const uint32_t hap = tl.hap + h;
       const uint32_t H = a.s.hap_len[hap];
@@ -694,8 +703,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
           cw1 = cw2;
         }
       }
-      if (active && k == W - 1)
-        finish_fp32(a, tl.out0 + (uint32_t)seg * stride + h, hap, read, X, S * pow2f(120 - (int)X));
+      if (active && k == W - 1 && !store_fp32(a, tl.out0 + (uint32_t)seg * stride + h, S * pow2f(120 - (int)X)))
+        resc |= 1ull << h;
+    }
+    // the run's pairs below the FP32 range go onto the rescue list together,
+    // one read's haplotypes next to each other, so that the FP64 pass sets
+    // each read up once for several of its pairs
+    if (active && k == W - 1 && resc) {
+      const int b = (int)a.rescue_list0 + fp64_bin(a.s.read_len[read]);
+      uint32_t idx = a.rescue_base[b] + atomicAdd(a.rescue_count + b, (uint32_t)__popcll(resc));
+      for (uint64_t m = resc; m; m &= m - 1) {
+        const uint32_t h = (uint32_t)__ffsll((long long)m) - 1;
+        RescueEntry e;
+        e.hap = tl.hap + h; e.read = read; e.out = tl.out0 + (uint32_t)seg * stride + h; e.x = X;
+        a.rescue[idx++] = e;
+      }
     }
   }
 }
@@ -717,138 +739,159 @@ __global__ __launch_bounds__(64) void k_fp64pt(Fp64Args a) {
   const double* ph2pr = a.t.ph2pr_d;
   const double* mmt = a.t.mm_d;
   const uint32_t count = *a.count;
+  // Entries per segment and fetch. A read's pairs lie next to each other on
+  // the list (the FP32 prior-table kernel appends a tile's pairs of one read
+  // together), so a segment that takes several consecutive entries sets its
+  // read up (row parameters, prior table, next-lane coefficients) once for
+  // all of them; a short list keeps one entry per segment, so that every
+  // wave has work.
+  const uint32_t chunk = min(kFp64Chunk, max(1u, count / (gridDim.x * (uint32_t)NSEG * 4u)));
+  constexpr bool kPad = W < 64;
+  double tmm[RPL], tgm[RPL], tmi[RPL], tii[RPL], tmd[RPL], tdd[RPL];
+  double sfac = 1.0, cN = 0.0, rN = 0.0, dN = 0.0;
+  uint32_t cur_read = 0xffffffffu;
   for (;;) {
     uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(a.head, (uint32_t)NSEG);
+    if (lane == 0) t = atomicAdd(a.head, (uint32_t)NSEG * chunk);
     t = __builtin_amdgcn_readfirstlane(t);
     if (t >= count) break;
-    const uint32_t e = t + seg;
-    const bool active = e < count;
-    const RescueEntry en = a.list[active ? e : t];
-    const uint32_t H = a.s.hap_len[en.hap];
-    uint32_t Hmax = H, Hmin = H;
+    for (uint32_t j = 0; j < chunk; ++j) {
+      if (t + j >= count) break;  // segment 0 holds the lowest entries of the fetch
+      const uint32_t e = t + (uint32_t)seg * chunk + j;
+      const bool active = e < count;
+      const RescueEntry en = a.list[active ? e : t + j];
+      const uint32_t H = a.s.hap_len[en.hap];
+      uint32_t Hmax = H, Hmin = H;
 #pragma unroll
-    for (int g = 0; g < NSEG; ++g) {
-      const uint32_t h2 = (uint32_t)__shfl((int)H, g * W);
-      Hmax = h2 > Hmax ? h2 : Hmax;
-      Hmin = h2 < Hmin ? h2 : Hmin;
-    }
-    __syncthreads();
-    {
-      const uint32_t w0 = a.s.hap_off[en.hap] >> 4, nw = (H + 15) >> 4;
-      for (int x = k; x < kHapWords; x += W) {
-        const int hx = x - 4;
-        hpk[seg][x] = (hx >= 0 && hx < (int)nw) ? a.s.hap_pack[w0 + hx] : 0u;
+      for (int g = 0; g < NSEG; ++g) {
+        const uint32_t h2 = (uint32_t)__shfl((int)H, g * W);
+        Hmax = h2 > Hmax ? h2 : Hmax;
+        Hmin = h2 < Hmin ? h2 : Hmin;
       }
-    }
-    double tmm[RPL], tgm[RPL], tmi[RPL], tii[RPL], tmd[RPL], tdd[RPL];
-    double pm[NP * 2], px[NP * 2];
-    int rc[NP * 2];
-    row_params<double, RPL, NP * 2, kScaled>(a.s, ph2pr, mmt, en.read, k, tmm, tgm, tmi, tii, tmd, tdd, pm, px, rc);
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        double2 v;
-        v.x = (rc[2 * q + 0] < 0 || rc[2 * q + 0] == c) ? pm[2 * q + 0] : px[2 * q + 0];
-        v.y = (rc[2 * q + 1] < 0 || rc[2 * q + 1] == c) ? pm[2 * q + 1] : px[2 * q + 1];
-        ptab[(c * NP + q) * 64 + lane] = v;
+      __syncthreads();
+      {
+        const uint32_t w0 = a.s.hap_off[en.hap] >> 4, nw = (H + 15) >> 4;
+        for (int x = k; x < kHapWords; x += W) {
+          const int hx = x - 4;
+          hpk[seg][x] = (hx >= 0 && hx < (int)nw) ? a.s.hap_pack[w0 + hx] : 0u;
+        }
       }
-    __syncthreads();
+      // a new read in some segment: the whole wave sets its reads up again
+      // (a segment whose read is unchanged computes the same values)
+      if (__ballot(en.read != cur_read) != 0) {
+        cur_read = en.read;
+        double pm[NP * 2], px[NP * 2];
+        int rc[NP * 2];
+        row_params<double, RPL, NP * 2, kScaled>(a.s, ph2pr, mmt, en.read, k, tmm, tgm, tmi, tii, tmd, tdd, pm, px,
+                                                 rc);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int q = 0; q < NP; ++q) {
+            double2 v;
+            v.x = (rc[2 * q + 0] < 0 || rc[2 * q + 0] == c) ? pm[2 * q + 0] : px[2 * q + 0];
+            v.y = (rc[2 * q + 1] < 0 || rc[2 * q + 1] == c) ? pm[2 * q + 1] : px[2 * q + 1];
+            ptab[(c * NP + q) * 64 + lane] = v;
+          }
+        // The 16- and 32-lane bins hold reads of fewer than W * RPL bases, so
+        // a segment's last row is a padding row: M = 0, I~ carries M + I of
+        // the read's last row (its sum term), D~ holds what it starts with.
+        // The 64-lane bin takes reads of up to W * RPL bases and sums
+        // M + f * I.
+        sfac = kPad ? 1.0 : sum_factor<double, W, RPL, kScaled>(a.s, ph2pr, en.read);
+        // Row 0 takes X and Y premixed by the lane above with this lane's
+        // row-0 coefficients (as in the FP32 kernel): cN, rN, dN are the next
+        // lane's.
+        cN = lane_from(tii[0], 1);
+        rN = lane_from(tmi[0], 1);
+        dN = lane_from(tmd[0], 1);
+        if (kPad && k == W - 1) { cN = 0.0; rN = 0.0; }
+      }
+      __syncthreads();
 
-    const double initD = pow2d((int)en.x + 900) / (double)H;
-    // The 16- and 32-lane bins hold reads of fewer than W * RPL bases, so a
-    // segment's last row is a padding row: M = 0, I~ carries M + I of the
-    // read's last row (its sum term), D~ holds what it starts with. The
-    // 64-lane bin takes reads of up to W * RPL bases and sums M + f * I.
-    constexpr bool kPad = W < 64;
-    const double sfac = kPad ? 1.0 : sum_factor<double, W, RPL, kScaled>(a.s, ph2pr, en.read);
-    // Row 0 takes X and Y premixed by the lane above with this lane's row-0
-    // coefficients (as in the FP32 kernel): cN, rN, dN are the next lane's.
-    // A segment's first lane sees row -1 (X = 0, Y = delta * initD of its
-    // own pair). DPP bound control gives lane 0 of the move X = 0 and the
-    // kept register Y (16-lane rows, the 64-lane wave). In the 32-lane
-    // wave-wide move lane 32 receives lane 31's values instead: lane 31's
-    // padding row, with M = 0, next-lane c = rho = 0 and D~ held at the
-    // second pair's initD, sends exactly X = 0 and Y = delta * initD.
-    const double initDn = lane_from(initD, 1);
-    double cN = lane_from(tii[0], 1), rN = lane_from(tmi[0], 1);
-    const double dN = lane_from(tmd[0], 1);
-    if (kPad && k == W - 1) { cN = 0.0; rN = 0.0; }
-    double Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
+      const double initD = pow2d((int)en.x + 900) / (double)H;
+      // A segment's first lane sees row -1 (X = 0, Y = delta * initD of its
+      // own pair). DPP bound control gives lane 0 of the move X = 0 and the
+      // kept register Y (16-lane rows, the 64-lane wave). In the 32-lane
+      // wave-wide move lane 32 receives lane 31's values instead: lane 31's
+      // padding row, with M = 0, next-lane c = rho = 0 and D~ held at the
+      // second pair's initD, sends exactly X = 0 and Y = delta * initD.
+      const double initDn = lane_from(initD, 1);
+      double Ma[RPL], Ia[RPL], Da[RPL], Mb[RPL], Ib[RPL], Db[RPL];
 #pragma unroll
-    for (int r = 0; r < RPL; ++r) { Ma[r] = 0; Ia[r] = 0; Da[r] = 0; }
-    if (kPad && k == W - 1) Da[RPL - 1] = initDn;
-    const double y0 = k == 0 ? tmd[0] * initD : 0.0;
-    double yd = y0, yn = y0, xu = 0.0;
-    double S = 0;
-    const int nsteps = (int)Hmax + W - 1;
-    const int mysteps = (int)H + W - 1;
-    const char* tab = reinterpret_cast<const char*>(ptab) + lane * 16;
-    const uint32_t* hs = hpk[seg];
-    auto codes_at = [&](int st) -> uint32_t {
-      const int p = st - k + 64;
-      return __builtin_amdgcn_alignbit(hs[(p >> 4) + 1], hs[p >> 4], (uint32_t)(p & 15) * 2u);
-    };
-    auto load = [&](uint32_t codes, int i, double (&pr)[NP * 2]) {
-      const uint32_t c = (codes >> (2 * i)) & 3u;
-      const double2* b = reinterpret_cast<const double2*>(tab + c * (uint32_t)(NP * 1024));
+      for (int r = 0; r < RPL; ++r) { Ma[r] = 0; Ia[r] = 0; Da[r] = 0; }
+      if (kPad && k == W - 1) Da[RPL - 1] = initDn;
+      const double y0 = k == 0 ? tmd[0] * initD : 0.0;
+      double yd = y0, yn = y0, xu = 0.0;
+      double S = 0;
+      const int nsteps = (int)Hmax + W - 1;
+      const int mysteps = (int)H + W - 1;
+      const char* tab = reinterpret_cast<const char*>(ptab) + lane * 16;
+      const uint32_t* hs = hpk[seg];
+      auto codes_at = [&](int st) -> uint32_t {
+        const int p = st - k + 64;
+        return __builtin_amdgcn_alignbit(hs[(p >> 4) + 1], hs[p >> 4], (uint32_t)(p & 15) * 2u);
+      };
+      auto load = [&](uint32_t codes, int i, double (&pr)[NP * 2]) {
+        const uint32_t c = (codes >> (2 * i)) & 3u;
+        const double2* b = reinterpret_cast<const double2*>(tab + c * (uint32_t)(NP * 1024));
 #pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        const double2 v = b[q * 64];
-        pr[2 * q + 0] = v.x; pr[2 * q + 1] = v.y;
-      }
-    };
-    auto send = [&](const double (&nM)[RPL], const double (&nI)[RPL], const double (&nD)[RPL]) {
-      const double x = fma(cN, nI[RPL - 1], nM[RPL - 1]);
-      const double y = fma(dN, nD[RPL - 1], fma(rN, nI[RPL - 1], nM[RPL - 1]));
-      xu = dpp_down<W>(x, 0.0, true);
-      const double t = dpp_down<W>(y, yd, false);  // yd (two steps old) dies here
-      yd = yn;
-      yn = t;
-    };
-    // the sum term of the last row: I~ of the padding row, or M + f * I
-    auto term = [&](const double (&nM)[RPL], const double (&nI)[RPL]) -> double {
-      return kPad ? nI[RPL - 1] : fma(sfac, nI[RPL - 1], nM[RPL - 1]);
-    };
-    double pA[NP * 2], pB[NP * 2];
-    uint32_t cw = codes_at(0);
-    load(cw, 0, pA);
-    // four steps a -> b -> a -> b -> a per iteration, the table reads one
-    // step ahead, the next codes word read once per iteration; while every
-    // segment of the wave is inside its own columns the sums need no test
-    auto iter4 = [&](int st, auto guarded) {
-      constexpr bool G = decltype(guarded)::value;
-      const uint32_t cw_next = codes_at(st + 4);
-      load(cw, 1, pB);
-      __builtin_amdgcn_sched_barrier(0);
-      sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
-      if (!G || st < mysteps) S += term(Mb, Ib);
-      send(Mb, Ib, Db);
-      load(cw, 2, pA);
-      __builtin_amdgcn_sched_barrier(0);
-      sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
-      if (!G || st + 1 < mysteps) S += term(Ma, Ia);
-      send(Ma, Ia, Da);
-      load(cw, 3, pB);
-      __builtin_amdgcn_sched_barrier(0);
-      sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
-      if (!G || st + 2 < mysteps) S += term(Mb, Ib);
-      send(Mb, Ib, Db);
-      cw = cw_next;
+        for (int q = 0; q < NP; ++q) {
+          const double2 v = b[q * 64];
+          pr[2 * q + 0] = v.x; pr[2 * q + 1] = v.y;
+        }
+      };
+      auto send = [&](const double (&nM)[RPL], const double (&nI)[RPL], const double (&nD)[RPL]) {
+        const double x = fma(cN, nI[RPL - 1], nM[RPL - 1]);
+        const double y = fma(dN, nD[RPL - 1], fma(rN, nI[RPL - 1], nM[RPL - 1]));
+        xu = dpp_down<W>(x, 0.0, true);
+        const double t = dpp_down<W>(y, yd, false);  // yd (two steps old) dies here
+        yd = yn;
+        yn = t;
+      };
+      // the sum term of the last row: I~ of the padding row, or M + f * I
+      auto term = [&](const double (&nM)[RPL], const double (&nI)[RPL]) -> double {
+        return kPad ? nI[RPL - 1] : fma(sfac, nI[RPL - 1], nM[RPL - 1]);
+      };
+      double pA[NP * 2], pB[NP * 2];
+      uint32_t cw = codes_at(0);
       load(cw, 0, pA);
-      __builtin_amdgcn_sched_barrier(0);
-      sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
-      if (!G || st + 3 < mysteps) S += term(Ma, Ia);
-      send(Ma, Ia, Da);
-    };
-    const int minsteps = (int)Hmin + W - 1;
-    int st = 0;
-    for (; st + 4 <= minsteps; st += 4) iter4(st, std::false_type{});
-    for (; st < nsteps; st += 4) iter4(st, std::true_type{});
-    if (active && k == W - 1) {
-      a.out[en.out] = log10(S * pow2d(120 - (int)en.x)) - 307.050595577260822;
-      a.flags[en.out] = 3;
+      // four steps a -> b -> a -> b -> a per iteration, the table reads one
+      // step ahead, the next codes word read once per iteration; while every
+      // segment of the wave is inside its own columns the sums need no test
+      auto iter4 = [&](int st, auto guarded) {
+        constexpr bool G = decltype(guarded)::value;
+        const uint32_t cw_next = codes_at(st + 4);
+        load(cw, 1, pB);
+        __builtin_amdgcn_sched_barrier(0);
+        sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
+        if (!G || st < mysteps) S += term(Mb, Ib);
+        send(Mb, Ib, Db);
+        load(cw, 2, pA);
+        __builtin_amdgcn_sched_barrier(0);
+        sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
+        if (!G || st + 1 < mysteps) S += term(Ma, Ia);
+        send(Ma, Ia, Da);
+        load(cw, 3, pB);
+        __builtin_amdgcn_sched_barrier(0);
+        sweep_step_xy<double, RPL>(Ma, Ia, Da, Mb, Ib, Db, yd, xu, pA, tmm, tgm, tmi, tii, tmd, tdd);
+        if (!G || st + 2 < mysteps) S += term(Mb, Ib);
+        send(Mb, Ib, Db);
+        cw = cw_next;
+        load(cw, 0, pA);
+        __builtin_amdgcn_sched_barrier(0);
+        sweep_step_xy<double, RPL>(Mb, Ib, Db, Ma, Ia, Da, yd, xu, pB, tmm, tgm, tmi, tii, tmd, tdd);
+        if (!G || st + 3 < mysteps) S += term(Ma, Ia);
+        send(Ma, Ia, Da);
+      };
+      const int minsteps = (int)Hmin + W - 1;
+      int st = 0;
+      for (; st + 4 <= minsteps; st += 4) iter4(st, std::false_type{});
+      for (; st < nsteps; st += 4) iter4(st, std::true_type{});
+      if (active && k == W - 1) {
+        a.out[en.out] = log10(S * pow2d(120 - (int)en.x)) - 307.050595577260822;
+        a.flags[en.out] = 3;
+      }
     }
   }
 }
