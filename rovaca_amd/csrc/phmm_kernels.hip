@@ -572,6 +572,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     };
     uint32_t words[2];
     hap_words(tl.hap, words);
+    static_assert(kMaxHapRun <= 64, "a run's rescue mask has one bit per haplotype");
     uint64_t resc = 0;  // the run's haplotypes whose pair goes to the FP64 rescue
     for (uint32_t h = 0; h < nh; ++h) {
       const uint32_t hap = tl.hap + h;
