@@ -132,6 +132,39 @@ def test_rescue_both_fp64_kernels():
     b.close()
 
 
+def test_long_rescue_lists_equal_alone():
+    # enough rescued pairs in one FP64 list that each segment of the FP64
+    # prior-table kernel takes several consecutive entries (a read set up once
+    # for several of its pairs, entries appended a read at a time by the FP32
+    # pass); every result equals the region computed alone, whose short lists
+    # give one entry per segment
+    say("long FP64 rescue lists: batch equals each region alone bit for bit; a sample against the FP64 oracle")
+    regs = workload.stress_regions(64, seed=81)
+    b = rv.Batch(regs)
+    b.run()
+    out, flags = b.results()
+    assert b.info()["n_rescued"] > 40000
+    rng = np.random.default_rng(82)
+    worst, checked = 0.0, 0
+    for x, reg in enumerate(regs):
+        a, af = rv.region_sync(reg)
+        m = b.region_result(out, x)
+        mf = b.region_result(flags, x)
+        assert np.array_equal(np.isnan(a), np.isnan(m)), x
+        assert np.array_equal(a[~np.isnan(a)], m[~np.isnan(m)]), x
+        assert np.array_equal(af, mf), x
+        resc = np.argwhere((mf & 2) != 0)
+        for r, h in resc[rng.permutation(len(resc))[:3]]:
+            o0, o1 = reg["read_off"][r], reg["read_off"][r + 1]
+            hb = reg["hap_bases"][reg["hap_off"][h]:reg["hap_off"][h + 1]]
+            want = oracle.pair_log10_f64(hb, reg["read_bases"][o0:o1], *[reg[k][o0:o1] for k in "qidc"])
+            worst = max(worst, abs(m[r, h] - want))
+            checked += 1
+    b.close()
+    assert checked > 100 and worst < TOL
+    say(f"{checked} rescued pairs vs FP64 oracle, worst |d| {worst:.2e}")
+
+
 def _gap_region(seed, gop_lo, gop_hi, gcp, n_haps=5, n_reads=6):
     """Reads whose gap-open qualities alternate between gop_lo and gop_hi row by
     row (the largest step between rows), gap continuation gcp."""
