@@ -115,7 +115,10 @@ typedef struct rvc_config {
                              /* 1 = every shape launches on its own)               */
   uint32_t sw_bt_budget_mib; /* aligner: backtrack bytes per device pass, in MiB;   */
                              /* larger batches run in several passes (0 = 12 GiB)  */
-  uint32_t reserved[4];
+  uint32_t sw_wide_shapes;   /* aligner: sweep a small pass on 64 lanes per pair,  */
+                             /* 2-6 rows per lane (0 = when its pairs take at most */
+                             /* three waves per SIMD; 1 = never; 2 = always)       */
+  uint32_t reserved[3];
 } rvc_config;
 
 /* ---- lifecycle -------------------------------------------------------------- */

@@ -64,7 +64,7 @@ class RvcConfig(ctypes.Structure):
                 ("hbm_budget_bytes", ctypes.c_uint64), ("max_batch_regions", ctypes.c_uint32),
                 ("host_prep", ctypes.c_uint32), ("tune_table", ctypes.c_char_p),
                 ("sw_min_launch_pairs", ctypes.c_uint32), ("sw_bt_budget_mib", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 4)]
+                ("sw_wide_shapes", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
 class RvcBatchOpts(ctypes.Structure):
@@ -178,16 +178,19 @@ def make_c_region(region):
 
 
 def init(device=0, server=False, hbm_budget_bytes=0, max_batch_regions=0, tune_table=None, host_prep=0,
-         sw_min_launch_pairs=0, sw_bt_budget_mib=0):
+         sw_min_launch_pairs=0, sw_bt_budget_mib=0, sw_wide_shapes=0):
     """host_prep: where call_pairhmm preprocesses reads (0 = device when the
     PCR model is on, else host; 1 = host; 2 = device). sw_min_launch_pairs:
     the aligner runs a sweep shape with fewer pairs together with the next
     larger one (0 = enough pairs to fill the device once; 1 = never).
-    sw_bt_budget_mib: the aligner's backtrack bytes per device pass (0 = 12 GiB)."""
+    sw_bt_budget_mib: the aligner's backtrack bytes per device pass (0 = 12 GiB).
+    sw_wide_shapes: the aligner sweeps a pass on 64 lanes per pair (0 = when
+    the pass is too small to fill the device; 1 = never; 2 = always)."""
     cfg = RvcConfig()
     cfg.host_prep = int(host_prep)
     cfg.sw_min_launch_pairs = int(sw_min_launch_pairs)
     cfg.sw_bt_budget_mib = int(sw_bt_budget_mib)
+    cfg.sw_wide_shapes = int(sw_wide_shapes)
     cfg.tune_table = tune_table.encode() if tune_table else None
     cfg.device = device
     cfg.server_threads = 1 if server else 0

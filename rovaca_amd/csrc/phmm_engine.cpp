@@ -88,6 +88,7 @@ int engine_init(const rvc_config* cfg) {
   g_engine.host_prep = cfg ? std::min<uint32_t>(cfg->host_prep, 2) : 0;
   g_engine.sw_min_launch = cfg ? cfg->sw_min_launch_pairs : 0;
   g_engine.sw_bt_budget = (cfg && cfg->sw_bt_budget_mib) ? (uint64_t)cfg->sw_bt_budget_mib << 20 : 12ull << 30;
+  g_engine.sw_wide = cfg ? std::min<uint32_t>(cfg->sw_wide_shapes, 2) : 0;
   // kernel-shape table: the one named in the config, else tuned/<device>.txt beside the library
   std::string table;
   if (cfg && cfg->tune_table) {

@@ -46,6 +46,7 @@ struct Engine {
   uint32_t host_prep = 0;   // call_pairhmm preprocessing: 0 auto, 1 host, 2 device
   uint32_t sw_min_launch = 0;  // aligner: fewest pairs a sweep shape launches with (0 auto)
   uint64_t sw_bt_budget = 12ull << 30;  // aligner: backtrack bytes per device pass
+  uint32_t sw_wide = 0;  // aligner: wide sweep shapes (0 for small passes, 1 never, 2 always)
   std::string tune_source;  // where the kernel-shape table came from ("" = built-in defaults)
 };
 extern Engine g_engine;

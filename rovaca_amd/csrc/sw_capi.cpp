@@ -298,21 +298,30 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
       continue;
     int W = 0, RPL = 0;
     shape[r] = sw_shape(p.read_len, &W, &RPL);
-    narrow[r] = shape[r] < kSwPackedShapes && sw_narrow(p.hap_len, p.read_len, sc[0], sc[1], sc[2], sc[3]) ? 1 : 0;
+    narrow[r] = sw_packed_shape(shape[r]) && sw_narrow(p.hap_len, p.read_len, sc[0], sc[1], sc[2], sc[3]) ? 1 : 0;
     if (stats) stats->cells += (uint64_t)p.hap_len * p.read_len;
   }
   // a shape with too few pairs to fill the device once (or fewer than the
   // engine's sw_min_launch) joins the next larger shape launched anyway (of
   // the same sweep, at most 1.5 times its rows): a launch of its own would
   // cost about one work item's time for a handful of pairs
-  int promote[2][kSwShapes];
+  int promote[2][kSwBaseShapes];
+  // then a pass whose pairs take at most three waves per SIMD on the wide
+  // shapes runs on them (a wave holds one pair, or one duo of the packed
+  // sweep): such a pass waits for its longest chain of steps, which 64 lanes
+  // per pair shorten, and a SIMD that interleaves a few waves hides the
+  // latency of each one's dependent instructions. Passes of 1 / 16 / 32 / 48
+  // regions of ~126 reads sweep in 0.14 / 0.36 / 0.44 / 0.46 ms on the wide
+  // shapes against 0.21 / 0.60 / 0.61 / 0.61 on the base ones; at 64 regions
+  // (four waves per SIMD) the two are equal
+  bool wide = g_engine.sw_wide == 2;
   {
-    uint64_t cnt[2][kSwShapes] = {};
+    uint64_t cnt[2][kSwBaseShapes] = {};
     for (uint32_t r = 0; r < n_reads; ++r)
       if (shape[r] >= 0) ++cnt[narrow[r]][shape[r]];
     for (int nw = 0; nw < 2; ++nw) {
       int target = -1;  // the smallest larger shape that is launched
-      for (int s = kSwShapes - 1; s >= 0; --s) {
+      for (int s = kSwBaseShapes - 1; s >= 0; --s) {
         promote[nw][s] = s;
         if (cnt[nw][s] == 0) continue;
         const uint64_t fill =
@@ -325,11 +334,17 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
           target = s;
       }
     }
+    if (g_engine.sw_wide == 0) {
+      uint64_t waves = 0;
+      for (int s = 0; s < kSwBaseShapes; ++s) waves += cnt[0][s] + (cnt[1][s] + 1) / 2;
+      wide = waves > 0 && waves <= 12ull * (uint64_t)g_engine.n_cu;
+    }
   }
   for (uint32_t r = 0; r < n_reads; ++r) {
     if (shape[r] < 0) continue;
     SwPair& p = pairs[r];
-    const int s = promote[narrow[r]][shape[r]];
+    const int sb = promote[narrow[r]][shape[r]];
+    const int s = wide ? kSwWide[sb] : sb;
     shape[r] = s;
     p.width = (uint16_t)kSwW[s];
     p.rpl = (uint16_t)kSwR[s];

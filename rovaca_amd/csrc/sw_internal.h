@@ -11,8 +11,9 @@ namespace rvc {
 
 constexpr int kSwMaxLen = 2048;       // the reference aligner refuses lengths >= this (sw_avx.c:18, sw_core_template.h:358-361)
 constexpr int kSwMaxRead = 2048;      // rows of the largest sweep shape (reads up to 2047 bases)
-constexpr int kSwShapes = 13;         // (W, RPL): see sw_shape()
-constexpr int kSwPackedShapes = 12;   // the packed int16 sweep has the first 12 shapes
+constexpr int kSwBaseShapes = 13;     // the shapes sw_shape() picks from
+constexpr int kSwShapes = 18;         // (W, RPL): the base shapes, then the wide ones
+constexpr int kSwLongShape = 12;      // 64 x 32 rows: the one shape without a packed sweep
 constexpr int kSwLow = -(1 << 28);    // "no gap" at row and column 0
 constexpr int kSwBig = -(1 << 26);    // disables a move in the rows past the read
 constexpr uint32_t kSwOpM = 0, kSwOpI = 1, kSwOpD = 2, kSwOpS = 4;  // BAM CIGAR operations
@@ -37,8 +38,26 @@ inline bool sw_narrow(uint32_t n1, uint32_t n2, int match, int mismatch, int ope
 // per SIMD. The last shape (64 x 32 rows, int32 sweep only) holds the rare
 // reads of 1025-2047 bases that the reference still aligns: one wave per
 // SIMD, its registers spilled to the accumulation registers (no scratch).
-constexpr int kSwW[kSwShapes] = {16, 16, 16, 16, 16, 32, 32, 32, 32, 64, 64, 64, 64};
-constexpr int kSwR[kSwShapes] = {4, 7, 8, 10, 12, 7, 8, 10, 12, 8, 12, 16, 32};
+//
+// Wide shapes (64 lanes, 2-6 rows each) serve passes too small to fill the
+// device: a pair's sweep is a chain of n1 + W - 1 steps, each serial over its
+// lane's RPL rows, so the longest pair of such a pass sets its time. Moving a
+// pair from 16 or 32 lanes to 64 at the same or a little more capacity cuts
+// the rows per step 2-4 times for a few more steps. kSwWide maps each base
+// shape to its wide one (itself when it already has 64 lanes).
+constexpr int kSwW[kSwShapes] = {16, 16, 16, 16, 16, 32, 32, 32, 32, 64, 64, 64, 64, 64, 64, 64, 64, 64};
+constexpr int kSwR[kSwShapes] = {4, 7, 8, 10, 12, 7, 8, 10, 12, 8, 12, 16, 32, 2, 3, 4, 5, 6};
+constexpr int kSwWide[kSwBaseShapes] = {13, 13, 13, 14, 14, 15, 15, 16, 17, 9, 10, 11, 12};
+inline bool sw_packed_shape(int s) { return s >= 0 && s < kSwShapes && s != kSwLongShape; }
+constexpr bool sw_wide_map_ok() {
+  for (int s = 0; s < kSwBaseShapes; ++s) {
+    const int w = kSwWide[s];
+    if (kSwW[w] != 64 || kSwW[w] * kSwR[w] < kSwW[s] * kSwR[s] || (w != s && w < kSwBaseShapes)) return false;
+    if (w == s && kSwW[s] != 64) return false;
+  }
+  return true;
+}
+static_assert(sw_wide_map_ok(), "a wide shape has 64 lanes and at least its base shape's rows");
 
 // Backtrack groups per lane: 4 rows each, the last one possibly shorter
 // (its bits sit where a full group's would).
@@ -46,7 +65,7 @@ constexpr int sw_groups(int rpl) { return (rpl + 3) / 4; }
 
 // The shape for a read of n2 bases (the fewest rows with W * RPL >= n2), or -1.
 inline int sw_shape(uint32_t n2, int* W, int* RPL) {
-  for (int s = 0; s < kSwShapes; ++s)
+  for (int s = 0; s < kSwBaseShapes; ++s)
     if ((uint32_t)(kSwW[s] * kSwR[s]) >= n2) {
       *W = kSwW[s];
       *RPL = kSwR[s];

@@ -25,10 +25,12 @@
 // recurrence two pairs per lane.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <atomic>
 
 #include <cstdint>
 #include <type_traits>
+#include <utility>
 
 #include "sw_internal.h"
 
@@ -503,13 +505,21 @@ __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
 }
 
 using SwFn = void (*)(SwArgs);
-const SwFn kSwFill[kSwShapes] = {&k_sw_fill<kSwW[0], kSwR[0]>, &k_sw_fill<kSwW[1], kSwR[1]>, &k_sw_fill<kSwW[2], kSwR[2]>, &k_sw_fill<kSwW[3], kSwR[3]>, &k_sw_fill<kSwW[4], kSwR[4]>, &k_sw_fill<kSwW[5], kSwR[5]>, &k_sw_fill<kSwW[6], kSwR[6]>, &k_sw_fill<kSwW[7], kSwR[7]>, &k_sw_fill<kSwW[8], kSwR[8]>, &k_sw_fill<kSwW[9], kSwR[9]>, &k_sw_fill<kSwW[10], kSwR[10]>, &k_sw_fill<kSwW[11], kSwR[11]>, &k_sw_fill<kSwW[12], kSwR[12]>};
-const SwFn kSwFill16[kSwShapes] = {&k_sw_fill16<kSwW[0], kSwR[0]>, &k_sw_fill16<kSwW[1], kSwR[1]>, &k_sw_fill16<kSwW[2], kSwR[2]>, &k_sw_fill16<kSwW[3], kSwR[3]>, &k_sw_fill16<kSwW[4], kSwR[4]>, &k_sw_fill16<kSwW[5], kSwR[5]>, &k_sw_fill16<kSwW[6], kSwR[6]>, &k_sw_fill16<kSwW[7], kSwR[7]>, &k_sw_fill16<kSwW[8], kSwR[8]>, &k_sw_fill16<kSwW[9], kSwR[9]>, &k_sw_fill16<kSwW[10], kSwR[10]>, &k_sw_fill16<kSwW[11], kSwR[11]>,
-                                   nullptr};  // no packed sweep of the 2048-row shape
+template <int... S>
+constexpr std::array<SwFn, kSwShapes> fill_table(std::integer_sequence<int, S...>) {
+  return {&k_sw_fill<kSwW[S], kSwR[S]>...};
+}
+template <int... S>
+constexpr std::array<SwFn, kSwShapes> fill16_table(std::integer_sequence<int, S...>) {
+  // no packed sweep of the 2048-row shape
+  return {(S == kSwLongShape ? nullptr : &k_sw_fill16<kSwW[S], S == kSwLongShape ? 4 : kSwR[S]>)...};
+}
+constexpr std::array<SwFn, kSwShapes> kSwFill = fill_table(std::make_integer_sequence<int, kSwShapes>{});
+constexpr std::array<SwFn, kSwShapes> kSwFill16 = fill16_table(std::make_integer_sequence<int, kSwShapes>{});
 
 // Occupancy per shape, cached; asked from several threads at once (each
 // slot is atomic, and racing threads store the same value).
-int blocks_per_cu(const SwFn* table, std::atomic<int>* cache, int shape) {
+int blocks_per_cu(const std::array<SwFn, kSwShapes>& table, std::atomic<int>* cache, int shape) {
   if (shape < 0 || shape >= kSwShapes || !table[shape]) return 1;
   int n = cache[shape].load(std::memory_order_relaxed);
   if (n == 0) {
@@ -534,7 +544,7 @@ hipError_t launch_sw_trace(hipStream_t stream, const SwArgs& a) {
 }
 
 hipError_t launch_sw_fill16(int shape, int grid, hipStream_t stream, const SwArgs& a) {
-  if (shape < 0 || shape >= kSwPackedShapes) return hipErrorInvalidValue;
+  if (!sw_packed_shape(shape)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(kSwFill16[shape], dim3(grid), dim3(64), 0, stream, a);
   return hipGetLastError();
 }

@@ -18,15 +18,29 @@ def say(msg):
     sys.stdout.flush()
 
 
-@pytest.fixture(scope="module", autouse=True)
-def engine():
+# the module runs twice: on the base sweep shapes only (16, 32 or 64 lanes
+# by read length) and on the wide ones only (64 lanes, 2-6 rows per lane);
+# on its own the engine picks the wide ones for passes too small to fill the
+# device, which every batch here but one is
+WIDE = {"base": 1, "wide": 2}
+_mode = {"wide": 1}
+
+
+@pytest.fixture(scope="module", autouse=True, params=sorted(WIDE))
+def engine(request):
     # every sweep shape launches on its own, however few its pairs, so that
     # small batches reach every kernel
-    say("init device 0")
-    rv.init(0, sw_min_launch_pairs=1)
+    _mode["wide"] = WIDE[request.param]
+    say(f"init device 0, {request.param} shapes")
+    rv.init(0, sw_min_launch_pairs=1, sw_wide_shapes=_mode["wide"])
     yield
     say("shutdown")
     rv.shutdown()
+
+
+def reinit(**kw):
+    rv.shutdown()
+    rv.init(0, sw_min_launch_pairs=1, sw_wide_shapes=_mode["wide"], **kw)
 
 
 def check(haps, reads, read_hap, assemble=False):
@@ -178,8 +192,7 @@ def test_passes_under_a_small_backtrack_budget():
     # pair alone where its own area exceeds it): the same results as the
     # oracle, read by read
     say("many device passes under a 1 MiB backtrack budget")
-    rv.shutdown()
-    rv.init(0, sw_min_launch_pairs=1, sw_bt_budget_mib=1)
+    reinit(sw_bt_budget_mib=1)
     try:
         rng = np.random.default_rng(41)
         pairs = random_pairs(41, 300)
@@ -197,8 +210,7 @@ def test_passes_under_a_small_backtrack_budget():
         for r, read in enumerate(reads):
             assert got[r] == oracle.sw_align(haps[read_hap[r]], read), r
     finally:
-        rv.shutdown()
-        rv.init(0, sw_min_launch_pairs=1)
+        reinit()
 
 
 def test_cigar_longer_than_its_slot():
@@ -350,6 +362,27 @@ def test_engine_restart_and_folded_shapes():
     assert rv.sw_align_batch(haps, reads, list(range(len(pairs))), served=True) == first
     for assemble in (False, True):
         check(haps, reads, list(range(len(pairs))), assemble)
+    reinit()
+
+
+def test_shapes_chosen_by_pass_size():
+    # with the default setting a pass whose pairs take at most three waves
+    # per SIMD on the wide shapes (300 pairs) runs on them, a larger one (8000
+    # pairs, ~4000 packed duos: more than three waves for each of the
+    # device's 1024 SIMDs) on the base shapes; both equal the oracle read by
+    # read, served or not
+    say("wide shapes for a small pass, base shapes for a large one")
+    rv.shutdown()
+    rv.init(0)
+    try:
+        for seed, n in ((51, 300), (52, 8000)):
+            pairs = random_pairs(seed, n)
+            haps = [h for h, _ in pairs]
+            reads = [r for _, r in pairs]
+            got = check(haps, reads, list(range(n)))
+            assert rv.sw_align_batch(haps, reads, list(range(n)), served=True) == got
+    finally:
+        reinit()
 
 
 @pytest.mark.parametrize("assemble", [False, True])
