@@ -315,6 +315,7 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
   // shapes against 0.21 / 0.60 / 0.61 / 0.61 on the base ones; at 64 regions
   // (four waves per SIMD) the two are equal
   bool wide = g_engine.sw_wide == 2;
+  bool small = false;  // the pass size that picks the wide shapes by default
   {
     uint64_t cnt[2][kSwBaseShapes] = {};
     for (uint32_t r = 0; r < n_reads; ++r)
@@ -334,11 +335,10 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
           target = s;
       }
     }
-    if (g_engine.sw_wide == 0) {
-      uint64_t waves = 0;
-      for (int s = 0; s < kSwBaseShapes; ++s) waves += cnt[0][s] + (cnt[1][s] + 1) / 2;
-      wide = waves > 0 && waves <= 12ull * (uint64_t)g_engine.n_cu;
-    }
+    uint64_t waves = 0;
+    for (int s = 0; s < kSwBaseShapes; ++s) waves += cnt[0][s] + (cnt[1][s] + 1) / 2;
+    small = waves > 0 && waves <= 12ull * (uint64_t)g_engine.n_cu;
+    if (g_engine.sw_wide == 0) wide = small;
   }
   for (uint32_t r = 0; r < n_reads; ++r) {
     if (shape[r] < 0) continue;
@@ -483,6 +483,10 @@ int sw_align_batch(const uint8_t* hap_bases, const int32_t* hap_off, uint32_t n_
     a.offset = reinterpret_cast<int32_t*>(base + o_off);
     a.score = reinterpret_cast<int32_t*>(base + o_sc);
     a.best_ij = reinterpret_cast<int32_t*>(base + o_best);
+    // a small pass's walks are latency-bound: 8 diagonal cells per load (the
+    // traceback of 16-64 regions of ~126 reads: 0.13 against 0.20 ms); a
+    // large pass's are not (1024 regions: 0.57-0.65 against 0.29-0.44 ms)
+    a.trace_run = small ? 8 : 1;
     // each launch (a shape's sweep, then its pairs' best end and traceback)
     // on the next of the sub streams, forked from and joined back into st
     if (hipEventRecord(w.ev[0], st) != hipSuccess) return set_error(RVC_E_DEVICE, "aligner event");

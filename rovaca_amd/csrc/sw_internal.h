@@ -104,6 +104,7 @@ struct SwArgs {
   int32_t* offset;
   int32_t* score;
   int32_t* best_ij;      // per pair: the best end cell (i, j)
+  int32_t trace_run;     // traceback: cells of a diagonal loaded at once (1..8)
 };
 
 hipError_t launch_sw_fill(int shape, int grid, hipStream_t stream, const SwArgs& a);

@@ -452,12 +452,8 @@ __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
   int i = bi, j = bj;
   if (j < n2) push(kSwOpS, (uint32_t)(n2 - j));
   int state = 0;
-  while (i > 0 && j > 0) {
-    const int kk = (j - 1) / rpl;  // the lane of read row j
-    const int rr = (j - 1) - kk * rpl;
-    const size_t word = (size_t)(i - 1 + kk) * wg + (size_t)(kk * gpl + (rr >> 2));
-    const uint32_t w16 = bt[word * bt_lanes];
-    const int q = rr & 3;
+  // one move of the walk from the current cell, given its backtrack word
+  auto move = [&](uint32_t w16, int q) {
     // bit 0 insertion, 1 deletion, 2 E opened, 3 F opened
     const int v = bt_lanes == 1 ? (int)(w16 >> (4 * (3 - q))) & 15
                                 : (int)(((w16 >> (9 + 2 * q)) & 1) | ((w16 >> (2 * q)) & 2) |
@@ -486,6 +482,49 @@ __global__ __launch_bounds__(64) void k_sw_trace(SwArgs a) {
       --j;
       push(kSwOpM, 1);
       state = 0;
+    }
+  };
+  // The walk is a chain of dependent loads, one backtrack word per cell. With
+  // a.trace_run > 1 the words of the next a.trace_run cells along the
+  // diagonal (a run of matches, the common move) are loaded together, so such
+  // a run waits for one load per a.trace_run cells; the walk takes them while
+  // its moves stay on that diagonal and loads again from where it leaves it.
+  // Decisions are the same, cell by cell. That pays where few walks run at
+  // once (latency-bound); with many, the loads the walk leaves unused cost
+  // more than the waits they save, and the host passes 1 (a.trace_run is the
+  // same for the whole launch).
+  constexpr int kRun = 8;
+  const int run = min(kRun, max(1, a.trace_run));
+  if (run == 1) {
+    while (i > 0 && j > 0) {
+      const int kk = (j - 1) / rpl;  // the lane of read row j
+      const int rr = (j - 1) - kk * rpl;
+      const size_t word = (size_t)(i - 1 + kk) * wg + (size_t)(kk * gpl + (rr >> 2));
+      move(bt[word * bt_lanes], rr & 3);
+    }
+  }
+  while (i > 0 && j > 0) {
+    const int i0 = i, j0 = j;
+    const int nl = min(run, min(i, j));  // cells (i0 - t, j0 - t), t < nl, are inside the matrix
+    uint32_t wv[kRun];
+    int qv[kRun];
+#pragma unroll
+    for (int t = 0; t < kRun; ++t) {
+      wv[t] = 0;
+      qv[t] = 0;
+      if (t < nl) {
+        const int jj = j0 - t;
+        const int kk = (jj - 1) / rpl;  // the lane of read row jj
+        const int rr = (jj - 1) - kk * rpl;
+        const size_t word = (size_t)(i0 - t - 1 + kk) * wg + (size_t)(kk * gpl + (rr >> 2));
+        wv[t] = bt[word * bt_lanes];
+        qv[t] = rr & 3;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kRun; ++t) {
+      if (t >= nl || i != i0 - t || j != j0 - t) continue;  // off the loaded diagonal
+      move(wv[t], qv[t]);
     }
   }
   if (j > 0) push(kSwOpS, (uint32_t)j);
